@@ -1,0 +1,160 @@
+/*
+ * lsg.h - C ABI of the MI355X-native DAFMM+NCA Lippmann-Schwinger solver.
+ *
+ * This is the drop-in boundary for the reference's matvec/GMRES path
+ * (vaishna77/Lippmann_Schwinger_Solver, GMRES/ project):
+ *
+ *   reference                                              replaced by
+ *   ---------------------------------------------------    ------------------------
+ *   struct inputsToDFMM + argv parsing  GMRES/GMRES.cpp:129-138,322-346   lsg_inputs
+ *   new DFMM(inputs)                    GMRES/GMRES.cpp:152-157           lsg_create
+ *   DFMM::FMMTreeInitialisation()       GMRES/GMRES.cpp:161-199           lsg_create
+ *   DFMM::MatVecProduct(Vec&, Vec&)     GMRES/GMRES.cpp:201-254           lsg_matvec
+ *   RestartedGMRES<DFMM>(...)           GMRES/GMRES.cpp:307-319           lsg_gmres
+ *   classGMRES::gmres (tolerance stop)  GMRES_Hybrid/include/gmres.hpp:53-119  lsg_gmres
+ *   userkernel::RHSFunction             GMRES/GMRES.cpp:122-125           lsg_rhs
+ *   userkernel::IncidenceFunction       GMRES/GMRES.cpp:118-121           lsg_incident
+ *   DFMM::gridPoints                    GMRES/GMRES.cpp:145,171           lsg_grid_points
+ *   delete S                            GMRES/GMRES.cpp:648               lsg_destroy
+ *
+ * Conventions
+ *   - Complex vectors are interleaved complex128 (re, im) double pairs, the
+ *     layout of Eigen::VectorXcd; length N = (#leaves) * nChebNodes^2, unknown
+ *     index = leafIndex * nChebNodes^2 + (iy * nChebNodes + ix).
+ *   - The caller owns every host buffer; the context owns device memory.
+ *   - Every call is synchronous. One context must not be used from two host
+ *     threads at once (the reference tree is not re-entrant either).
+ *   - Return value 0 = success, negative = error; lsg_last_error() gives text.
+ *     There is no silent CPU fallback: without a usable GPU lsg_create fails.
+ */
+#ifndef LSG_H_
+#define LSG_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSG_OK 0
+#define LSG_ERR_ARG -1       /* invalid argument */
+#define LSG_ERR_DEVICE -2    /* HIP runtime / device error */
+#define LSG_ERR_SETUP -3     /* tree / table / NCA construction failed */
+#define LSG_ERR_NUMERIC -4   /* NaN/Inf input, zero right-hand side, breakdown */
+#define LSG_ERR_NOCONV -5    /* GMRES hit its iteration cap above tolerance */
+
+#define LSG_MAX_BUMPS 16
+
+/* Contrast q(x). kind 0..5 are the reference's Qchoice functions
+ * (gaussian, multipleGaussians, plasma, flatBump, cavity, lens;
+ * GMRES/GMRES.cpp:5-117). kind 6 is a sum of Gaussian bumps
+ * q(x) = sum_i amp_i * exp(-|x - c_i|^2 / width_i^2). */
+typedef struct lsg_contrast {
+  int kind;
+  int n_bumps;
+  double cx[LSG_MAX_BUMPS];
+  double cy[LSG_MAX_BUMPS];
+  double width[LSG_MAX_BUMPS];
+  double amp[LSG_MAX_BUMPS];
+} lsg_contrast;
+
+/* The reference's 12 run-time inputs (GMRES/GMRES.cpp:322-333) plus the
+ * choices the reference makes by editing source. */
+typedef struct lsg_inputs {
+  int nCones_LFR;      /* argv[1]  cones per box at level level_LFR-1 */
+  int nChebNodes;      /* argv[2]  Chebyshev nodes per leaf side */
+  int treeAdaptivity;  /* argv[3]  adaptive-tree tolerance exponent */
+  double L;            /* argv[4]  half side of the domain [-L,L]^2 */
+  double kappa;        /* argv[5]  wavenumber */
+  int yes2DFMM;        /* argv[6]  1: directional (HFR) levels enabled */
+  int degreeOfBases;   /* argv[7]  total-degree bound of the interpolation basis */
+  int TOL_POW;         /* argv[8]  NCA tolerance 10^-TOL_POW */
+  int m;               /* argv[9]  Krylov dimension (names the result dir) */
+  int restart;         /* argv[10] GMRES restarts */
+  int nLevelsUniform;  /* argv[11] depth of the uniform tree */
+  int Qchoice;         /* argv[12] contrast selector (== contrast.kind) */
+  int tree_mode;       /* 0: uniform tree (createUniformTree); 1: adaptive */
+  int find_phi;        /* 1: A = I + diag(k^2 q) V (solve); 0: V (field) */
+  double inc_angle;    /* plane-wave direction; 0 = +x as in the reference */
+  lsg_contrast contrast;
+} lsg_inputs;
+
+/* Work and size figures of a built operator (for roofline accounting). */
+typedef struct lsg_info {
+  int64_t N;              /* unknowns */
+  int nLevels;            /* leaf level of the tree */
+  int level_LFR;          /* first low-frequency level */
+  int n_leaves;
+  int64_t E_far;          /* complex entries of the fused far-field operators */
+  int64_t E_near;         /* near-field complex MACs per matvec */
+  int64_t bytes_matvec;   /* algorithmic HBM bytes of one FP64 matvec */
+  int64_t flops_matvec;   /* algorithmic flops of one matvec */
+  int64_t n_state;        /* complex slots in the device state vector */
+  int max_rank;           /* largest NCA rank */
+  double setup_seconds;   /* host operator build time */
+} lsg_info;
+
+/* GMRES statistics. */
+typedef struct lsg_stats {
+  int iterations;         /* Arnoldi steps (= matvecs excluding residual checks) */
+  int restarts;           /* completed restart cycles */
+  int converged;          /* 1 if the true FP64 residual met tol */
+  double rel_residual;    /* true ||b - A x|| / ||b|| in FP64 */
+  double est_residual;    /* Givens residual estimate at exit */
+  double seconds;         /* wall time */
+} lsg_stats;
+
+typedef struct lsg_ctx lsg_ctx;
+
+/* Fill a kind-6 contrast with n seeded bumps (SplitMix64 stream, draws per
+ * bump in the order cx, cy, width, amp, each uniform in its range). */
+int lsg_bumps_from_seed(uint64_t seed, int n, double cmin, double cmax,
+                        double wmin, double wmax, double amin, double amax,
+                        lsg_contrast* out);
+
+/* Build tree, tables, NCA skeletons and fused operators on the host, upload
+ * them to the current HIP device. Replaces new DFMM + FMMTreeInitialisation. */
+int lsg_create(const lsg_inputs* in, lsg_ctx** out);
+void lsg_destroy(lsg_ctx* ctx);
+const char* lsg_last_error(const lsg_ctx* ctx);
+/* Error text of the last failed lsg_create (no context exists then). */
+const char* lsg_create_error(void);
+
+int lsg_get_info(const lsg_ctx* ctx, lsg_info* info);
+/* Leaf-major node coordinates, x[N], y[N]. */
+int lsg_grid_points(const lsg_ctx* ctx, double* x, double* y);
+/* k^2 q(x_i) at every node (the diagonal D of A = I + D V). */
+int lsg_contrast_at_nodes(const lsg_ctx* ctx, double* qk2);
+/* u_inc(x_i) = exp(i k (x cos a + y sin a)), interleaved complex. */
+int lsg_incident(const lsg_ctx* ctx, double* u);
+/* b_i = -k^2 q(x_i) u_inc(x_i) (RHSFunction). */
+int lsg_rhs(const lsg_ctx* ctx, double* b);
+
+/* y = A x for nrhs right-hand sides (column j at x + 2*ld*j doubles),
+ * FP64 on the device. Replaces DFMM::MatVecProduct. */
+int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
+
+/* Device-pointer variant: x, y are device addresses of N complex128 each;
+ * the call is enqueued on the context's stream and returns without waiting. */
+int lsg_matvec_device(lsg_ctx* ctx, const void* x_dev, void* y_dev);
+/* Wait for all work queued on the context's stream. */
+int lsg_sync(lsg_ctx* ctx);
+/* Average device time (ms) of each matvec pass recorded with HIP events over
+ * `reps` matvecs of a device-resident vector; writes up to max_pass values
+ * into ms[] and the pass algorithmic bytes into bytes[]; returns #passes. */
+int lsg_profile_passes(lsg_ctx* ctx, int reps, double* ms, int64_t* bytes,
+                       int max_pass);
+
+/* Restarted GMRES on the device: x <- solution of A x = b (x holds the
+ * initial guess). restart_len = Krylov dimension per cycle, max_restarts =
+ * cycle cap. tol > 0: stop when the true FP64 relative residual <= tol;
+ * tol <= 0: reference mode, run exactly max_restarts cycles of restart_len
+ * Arnoldi steps (GMRES/GMRES.cpp:257-319, with its 1e-8 breakdown exit). */
+int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len,
+              int max_restarts, double tol, lsg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSG_H_ */

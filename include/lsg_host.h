@@ -1,0 +1,59 @@
+/*
+ * lsg_host.h - host-side inspection of the operator build (no device needed).
+ *
+ * These entry points run only the host part of lsg_create: tree, level
+ * tables, NCA skeletons and fused-operator sizes. They exist so that the build
+ * can be checked against a CPU restatement of the reference on machines
+ * without a GPU; they perform no matrix-vector products.
+ *
+ *   reference                                         inspected by
+ *   createUniformTree / assign_Tree_Interactions       lsg_host_info
+ *     GMRES/G_FMM2DTree.hpp:833-941, 1356-1498
+ *   evaluateM / writeMToFile (6-digit tables)          lsg_host_far_table
+ *     GMRES/FarFieldInteraction.hpp:152-160, G_FMM2DTree.hpp:1252-1271
+ *   getOperator / colleagueNeighborInteraction         lsg_host_near_tables
+ *     GMRES/G_FMM2DTree.hpp:1773-1911
+ *   getNodes_LFR / getNodes_HFR / getUserCheckPoints   lsg_host_skeleton
+ *     GMRES/G_FMM2DTree.hpp:2062-2385, 3146-3571
+ */
+#ifndef LSG_HOST_H_
+#define LSG_HOST_H_
+
+#include <stdint.h>
+#include "lsg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lsg_host lsg_host;
+
+int lsg_host_build(const lsg_inputs* in, lsg_host** out);
+/* Tree only (levels, interaction lists, cones); the table, skeleton and
+ * operator queries then return empty results. */
+int lsg_host_build_tree(const lsg_inputs* in, lsg_host** out);
+void lsg_host_destroy(lsg_host* h);
+const char* lsg_host_error(void);
+int lsg_host_info(const lsg_host* h, lsg_info* info);
+/* Far-field table of the leaf level: (2*15+1) x nChebNodes^2, row-major,
+ * interleaved complex; returns the number of complex values (call with
+ * out == NULL to query). */
+int64_t lsg_host_far_table(const lsg_host* h, double* out);
+/* The 9 colleague near-field tables of the leaf level, slot-major,
+ * each nChebNodes^2 x nChebNodes^2 row-major, interleaved complex. */
+int64_t lsg_host_near_tables(const lsg_host* h, double* out);
+/* Skeleton dump (see the layout comment in nca.hpp); returns its length. */
+int64_t lsg_host_skeleton(const lsg_host* h, int* out);
+/* Interaction-list pairs per level (sum over boxes of |IL|, cones summed in
+ * the high-frequency levels), levels 0..nLevels. */
+int lsg_host_il_pairs(const lsg_host* h, int64_t* out, int max_levels);
+/* Q_pinv of the basis (NumberOfBases x nChebNodes^2, row-major). */
+int64_t lsg_host_qpinv(const lsg_host* h, double* out);
+/* Per-level largest NCA rank, levels 0..nLevels. */
+int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSG_HOST_H_ */

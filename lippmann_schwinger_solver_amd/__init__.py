@@ -1,0 +1,342 @@
+"""MI355X-native DAFMM+NCA solver for the 2-D Lippmann-Schwinger equation.
+
+Python mirror of the reference's matvec/GMRES interface (GMRES/GMRES.cpp):
+
+    inputs = InputsToDFMM(nCones_LFR=16, nChebNodes=6, L=0.5, kappa=20.0, ...)
+    S = DFMM(inputs)                 # GMRES/GMRES.cpp:152-157
+    S.FMMTreeInitialisation()        # GMRES/GMRES.cpp:161-199
+    out = S.MatVecProduct(b)         # GMRES/GMRES.cpp:201-254
+    phi = RestartedGMRES(restart, S, m, inputs, b)   # GMRES/GMRES.cpp:307-319
+
+Everything runs through the C ABI of liblsg.so (include/lsg.h); there is no
+CPU execution path - DFMM.FMMTreeInitialisation raises when no GPU is usable.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblsg.so")
+MAX_BUMPS = 16
+
+LSG_OK = 0
+LSG_ERR_ARG = -1
+LSG_ERR_DEVICE = -2
+LSG_ERR_SETUP = -3
+LSG_ERR_NUMERIC = -4
+LSG_ERR_NOCONV = -5
+
+
+class LsgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lsg error {code}: {msg}")
+        self.code = code
+
+
+class lsg_contrast(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n_bumps", C.c_int), ("cx", C.c_double * MAX_BUMPS),
+                ("cy", C.c_double * MAX_BUMPS), ("width", C.c_double * MAX_BUMPS),
+                ("amp", C.c_double * MAX_BUMPS)]
+
+
+class lsg_inputs(C.Structure):
+    _fields_ = [("nCones_LFR", C.c_int), ("nChebNodes", C.c_int), ("treeAdaptivity", C.c_int),
+                ("L", C.c_double), ("kappa", C.c_double), ("yes2DFMM", C.c_int),
+                ("degreeOfBases", C.c_int), ("TOL_POW", C.c_int), ("m", C.c_int), ("restart", C.c_int),
+                ("nLevelsUniform", C.c_int), ("Qchoice", C.c_int), ("tree_mode", C.c_int),
+                ("find_phi", C.c_int), ("inc_angle", C.c_double), ("contrast", lsg_contrast)]
+
+
+class lsg_info(C.Structure):
+    _fields_ = [("N", C.c_int64), ("nLevels", C.c_int), ("level_LFR", C.c_int), ("n_leaves", C.c_int),
+                ("E_far", C.c_int64), ("E_near", C.c_int64), ("bytes_matvec", C.c_int64),
+                ("flops_matvec", C.c_int64), ("n_state", C.c_int64), ("max_rank", C.c_int),
+                ("setup_seconds", C.c_double)]
+
+
+class lsg_stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
+                ("rel_residual", C.c_double), ("est_residual", C.c_double), ("seconds", C.c_double)]
+
+
+# (name, restype, argtypes) of every symbol in include/lsg.h and include/lsg_host.h
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+SYMBOLS = [
+    ("lsg_bumps_from_seed", C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.POINTER(lsg_contrast)]),
+    ("lsg_create", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
+    ("lsg_destroy", None, [_P]),
+    ("lsg_last_error", C.c_char_p, [_P]),
+    ("lsg_create_error", C.c_char_p, []),
+    ("lsg_get_info", C.c_int, [_P, C.POINTER(lsg_info)]),
+    ("lsg_grid_points", C.c_int, [_P, _D, _D]),
+    ("lsg_contrast_at_nodes", C.c_int, [_P, _D]),
+    ("lsg_incident", C.c_int, [_P, _D]),
+    ("lsg_rhs", C.c_int, [_P, _D]),
+    ("lsg_matvec", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
+    ("lsg_matvec_device", C.c_int, [_P, _P, _P]),
+    ("lsg_sync", C.c_int, [_P]),
+    ("lsg_profile_passes", C.c_int, [_P, C.c_int, _D, C.POINTER(C.c_int64), C.c_int]),
+    ("lsg_gmres", C.c_int, [_P, _D, _D, C.c_int, C.c_int, C.c_double, C.POINTER(lsg_stats)]),
+    ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
+    ("lsg_host_build_tree", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
+    ("lsg_host_destroy", None, [_P]),
+    ("lsg_host_error", C.c_char_p, []),
+    ("lsg_host_info", C.c_int, [_P, C.POINTER(lsg_info)]),
+    ("lsg_host_far_table", C.c_int64, [_P, _D]),
+    ("lsg_host_near_tables", C.c_int64, [_P, _D]),
+    ("lsg_host_skeleton", C.c_int64, [_P, C.POINTER(C.c_int)]),
+    ("lsg_host_level_ranks", C.c_int, [_P, C.POINTER(C.c_int), C.c_int]),
+    ("lsg_host_il_pairs", C.c_int, [_P, C.POINTER(C.c_int64), C.c_int]),
+    ("lsg_host_qpinv", C.c_int64, [_P, _D]),
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liblsg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing; run __graft_entry__.build() or "
+                      "python lippmann_schwinger_solver_amd/build.py")
+    lib = C.CDLL(path)
+    for name, res, args in SYMBOLS:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def bumps_from_seed(seed, n, cmin=-0.35, cmax=0.35, wmin=0.03, wmax=0.10, amin=0.5, amax=1.5):
+    """Seeded Gaussian-bump contrast (SplitMix64 stream)."""
+    lib = load_library()
+    c = lsg_contrast()
+    rc = lib.lsg_bumps_from_seed(C.c_uint64(seed), n, cmin, cmax, wmin, wmax, amin, amax, C.byref(c))
+    if rc != 0:
+        raise LsgError(rc, "bad bump arguments")
+    return c
+
+
+class InputsToDFMM:
+    """The reference's run-time inputs (GMRES/GMRES.cpp:129-138, 322-346)."""
+
+    def __init__(self, nCones_LFR=16, nChebNodes=6, treeAdaptivity=6, L=0.5, kappa=20.0, yes2DFMM=1,
+                 degreeOfBases=6, TOL_POW=10, m=50, restart=1, nLevelsUniform=4, Qchoice=0,
+                 tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None):
+        self.nCones_LFR = nCones_LFR
+        self.nChebNodes = nChebNodes
+        self.treeAdaptivity = treeAdaptivity
+        self.L = L
+        self.kappa = kappa
+        self.yes2DFMM = yes2DFMM
+        self.degreeOfBases = degreeOfBases
+        self.TOL_POW = TOL_POW
+        self.m = m
+        self.restart = restart
+        self.nLevelsUniform = nLevelsUniform
+        self.Qchoice = Qchoice
+        self.tree_mode = tree_mode
+        self.find_phi = find_phi
+        self.inc_angle = inc_angle
+        self.contrast = contrast
+
+    def to_c(self):
+        s = lsg_inputs()
+        for k in ("nCones_LFR", "nChebNodes", "treeAdaptivity", "L", "kappa", "yes2DFMM", "degreeOfBases",
+                  "TOL_POW", "m", "restart", "nLevelsUniform", "Qchoice", "tree_mode", "find_phi", "inc_angle"):
+            setattr(s, k, getattr(self, k))
+        if self.contrast is not None:
+            s.contrast = self.contrast
+            s.Qchoice = self.contrast.kind
+        else:
+            s.contrast.kind = self.Qchoice
+        return s
+
+
+class HostBuild:
+    """Host-only operator build (tree, tables, NCA) for inspection without a GPU."""
+
+    def __init__(self, inputs, tree_only=False):
+        self._lib = load_library()
+        self._c = inputs.to_c()
+        self._h = C.c_void_p()
+        fn = self._lib.lsg_host_build_tree if tree_only else self._lib.lsg_host_build
+        rc = fn(C.byref(self._c), C.byref(self._h))
+        if rc != 0:
+            raise LsgError(rc, self._lib.lsg_host_error().decode())
+
+    def info(self):
+        inf = lsg_info()
+        self._lib.lsg_host_info(self._h, C.byref(inf))
+        return {k: getattr(inf, k) for k, _ in lsg_info._fields_}
+
+    def far_table(self):
+        n = self._lib.lsg_host_far_table(self._h, None)
+        out = np.zeros(2 * n)
+        self._lib.lsg_host_far_table(self._h, _dptr(out))
+        return out.view(np.complex128)
+
+    def near_tables(self):
+        n = self._lib.lsg_host_near_tables(self._h, None)
+        out = np.zeros(2 * n)
+        self._lib.lsg_host_near_tables(self._h, _dptr(out))
+        return out.view(np.complex128)
+
+    def skeleton(self):
+        n = self._lib.lsg_host_skeleton(self._h, None)
+        out = np.zeros(n, dtype=np.int32)
+        self._lib.lsg_host_skeleton(self._h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def level_ranks(self):
+        out = np.zeros(16, dtype=np.int32)
+        n = self._lib.lsg_host_level_ranks(self._h, out.ctypes.data_as(C.POINTER(C.c_int)), 16)
+        return out[:n]
+
+    def il_pairs(self):
+        out = np.zeros(16, dtype=np.int64)
+        n = self._lib.lsg_host_il_pairs(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), 16)
+        return out[:n]
+
+    def qpinv(self):
+        n = self._lib.lsg_host_qpinv(self._h, None)
+        out = np.zeros(n)
+        self._lib.lsg_host_qpinv(self._h, _dptr(out))
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.lsg_host_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DFMM:
+    """Drop-in for the reference's DFMM facade (GMRES/GMRES.cpp:140-255)."""
+
+    def __init__(self, inputs):
+        self.inputs = inputs
+        self._lib = load_library()
+        self._ctx = None
+        self.N = 0
+        self.gridPoints = None
+
+    def FMMTreeInitialisation(self):
+        c = self.inputs.to_c()
+        ctx = C.c_void_p()
+        rc = self._lib.lsg_create(C.byref(c), C.byref(ctx))
+        if rc != 0:
+            raise LsgError(rc, self._lib.lsg_create_error().decode())
+        self._ctx = ctx
+        inf = self.info()
+        self.N = int(inf["N"])
+        gx = np.zeros(self.N)
+        gy = np.zeros(self.N)
+        self._lib.lsg_grid_points(self._ctx, _dptr(gx), _dptr(gy))
+        self.gridPoints = np.stack([gx, gy], axis=1)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LsgError(rc, self._lib.lsg_last_error(self._ctx).decode())
+
+    def info(self):
+        inf = lsg_info()
+        self._check(self._lib.lsg_get_info(self._ctx, C.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in lsg_info._fields_}
+
+    def MatVecProduct(self, b, output=None):
+        """output = A b (complex128, length N)."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        if b.shape != (self.N,):
+            raise ValueError("b must have shape (N,)")
+        out = np.empty(self.N, dtype=np.complex128)
+        bd = b.view(np.float64)
+        od = out.view(np.float64)
+        self._check(self._lib.lsg_matvec(self._ctx, _dptr(bd), _dptr(od), 1, self.N))
+        if output is not None:
+            output[...] = out
+        return out
+
+    def matvec_block(self, X):
+        """A X for X of shape (N, nrhs)."""
+        X = np.asfortranarray(X, dtype=np.complex128)
+        Y = np.empty_like(X, order="F")
+        self._check(self._lib.lsg_matvec(self._ctx, _dptr(X.view(np.float64)), _dptr(Y.view(np.float64)),
+                                         X.shape[1], self.N))
+        return Y
+
+    def rhs(self):
+        out = np.empty(self.N, dtype=np.complex128)
+        self._check(self._lib.lsg_rhs(self._ctx, _dptr(out.view(np.float64))))
+        return out
+
+    def incident(self):
+        out = np.empty(self.N, dtype=np.complex128)
+        self._check(self._lib.lsg_incident(self._ctx, _dptr(out.view(np.float64))))
+        return out
+
+    def contrast_k2(self):
+        out = np.empty(self.N)
+        self._check(self._lib.lsg_contrast_at_nodes(self._ctx, _dptr(out)))
+        return out
+
+    def gmres(self, b, x0=None, restart_len=50, max_restarts=20, tol=1e-10):
+        """Device GMRES; returns (x, stats dict). tol <= 0 runs the reference's fixed cycles."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        st = lsg_stats()
+        rc = self._lib.lsg_gmres(self._ctx, _dptr(b.view(np.float64)), _dptr(x.view(np.float64)), restart_len,
+                                 max_restarts, tol, C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats["status"] = rc
+        return x, stats
+
+    def profile_passes(self, reps=5):
+        ms = np.zeros(64)
+        by = np.zeros(64, dtype=np.int64)
+        n = self._lib.lsg_profile_passes(self._ctx, reps, _dptr(ms), by.ctypes.data_as(C.POINTER(C.c_int64)), 64)
+        if n < 0:
+            self._check(n)
+        names = self._lib.lsg_last_error(self._ctx).decode().strip(",").split(",")
+        return [(names[i], float(ms[i]), int(by[i])) for i in range(n)]
+
+    def close(self):
+        if self._ctx:
+            self._lib.lsg_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def GMRES(x0, S, m, inputs, b):
+    """One GMRES(m) cycle from x0 (GMRES/GMRES.cpp:257-305); returns the solution."""
+    x, _ = S.gmres(b, x0=x0, restart_len=m, max_restarts=1, tol=0.0)
+    return x
+
+
+def RestartedGMRES(restart, S, m, inputs, b):
+    """`restart` cycles of GMRES(m) from x0 = 0 (GMRES/GMRES.cpp:307-319)."""
+    x, _ = S.gmres(b, x0=None, restart_len=m, max_restarts=restart, tol=0.0)
+    return x

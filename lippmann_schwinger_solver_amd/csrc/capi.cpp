@@ -1,0 +1,403 @@
+// C ABI of the solver (include/lsg.h, include/lsg_host.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/lsg.h"
+#include "../../include/lsg_host.h"
+#include "device.hpp"
+#include "nca.hpp"
+#include "problem.hpp"
+#include "tables.hpp"
+#include "tree.hpp"
+
+using namespace lsg;
+
+namespace {
+
+thread_local std::string g_create_error;
+thread_local std::string g_host_error;
+
+struct HostBuild {
+  lsg_inputs in{};
+  BasisSet bs;
+  Tree tree;
+  Problem prob;
+  Plan plan;
+  double setup_seconds = 0;
+  std::vector<cplx> far;  // leaf-level M table
+};
+
+std::string validate(const lsg_inputs* in) {
+  if (!in) return "inputs is NULL";
+  if (in->nChebNodes < 2 || in->nChebNodes > 8) return "nChebNodes must be in [2, 8]";
+  if (in->degreeOfBases < 1 || in->degreeOfBases > in->nChebNodes) return "degreeOfBases must be in [1, nChebNodes]";
+  if (!(in->L > 0) || !std::isfinite(in->L)) return "L must be positive";
+  if (!(in->kappa > 0) || !std::isfinite(in->kappa)) return "kappa must be positive";
+  if (in->nCones_LFR < 1) return "nCones_LFR must be >= 1";
+  if (in->TOL_POW < 1 || in->TOL_POW > 16) return "TOL_POW must be in [1, 16]";
+  if (in->tree_mode != 0) return "tree_mode must be 0: only the uniform tree is supported";
+  if (in->nLevelsUniform < 2 || in->nLevelsUniform > 9) return "nLevelsUniform must be in [2, 9]";
+  if (in->Qchoice != in->contrast.kind) return "Qchoice must equal contrast.kind";
+  if (in->contrast.kind < 0 || in->contrast.kind > 6) return "contrast.kind must be in [0, 6]";
+  if (in->contrast.kind == 6 && (in->contrast.n_bumps < 0 || in->contrast.n_bumps > LSG_MAX_BUMPS))
+    return "contrast.n_bumps out of range";
+  return "";
+}
+
+// Returns LSG_OK, LSG_ERR_ARG (invalid inputs) or LSG_ERR_SETUP; message in err.
+int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_only = false) {
+  err = validate(in);
+  if (!err.empty()) return LSG_ERR_ARG;
+  std::string e;
+  auto t0 = std::chrono::steady_clock::now();
+  hb.in = *in;
+  hb.bs.init(in->nChebNodes, in->degreeOfBases);
+  TreeParams tp;
+  tp.nCones_LFR = in->nCones_LFR;
+  tp.nChebNodes = in->nChebNodes;
+  tp.L = in->L;
+  tp.kappa = in->kappa;
+  tp.yes2DFMM = in->yes2DFMM;
+  tp.nLevelsUniform = in->nLevelsUniform;
+  tp.tree_mode = in->tree_mode;
+  e = hb.tree.build(tp, hb.bs);
+  if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
+  hb.prob.kappa = in->kappa;
+  hb.prob.inc_angle = in->inc_angle;
+  hb.prob.contrast = in->contrast;
+  if (tree_only) return LSG_OK;
+  BuildOptions bo;
+  bo.TOL_POW = in->TOL_POW;
+  bo.find_phi = in->find_phi != 0;
+  e = build_plan(hb.tree, hb.bs, hb.prob, bo, hb.plan);
+  if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
+  hb.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return LSG_OK;
+}
+
+void fill_info(const HostBuild& hb, lsg_info* info) {
+  std::memset(info, 0, sizeof(*info));
+  info->N = hb.plan.N;
+  info->nLevels = hb.tree.nLevels;
+  info->level_LFR = hb.tree.level_LFR;
+  info->n_leaves = hb.plan.n_leaves;
+  info->E_far = hb.plan.E_far;
+  info->E_near = hb.plan.E_near;
+  int64_t bytes = 0, nsrc = 0;
+  for (const PassHost& p : hb.plan.passes) {
+    bytes += 16 * p.entries + 4 * p.n_src + 16 * p.n_dst;
+    nsrc += p.n_src;
+  }
+  // operators + index lists + destinations + x in + y out + qk2 + near tables
+  info->bytes_matvec = bytes + 16 * int64_t(hb.plan.N) * 2 + 8 * int64_t(hb.plan.N) +
+                       16 * int64_t(hb.plan.T.size());
+  info->flops_matvec = 8 * (hb.plan.E_far + hb.plan.E_near) + 4 * int64_t(hb.plan.N);
+  info->n_state = hb.plan.n_state;
+  info->max_rank = hb.plan.max_rank;
+  info->setup_seconds = hb.setup_seconds;
+}
+
+}  // namespace
+
+struct lsg_ctx {
+  HostBuild hb;
+  DeviceOperator dev;
+  std::string err;
+  double2* dx = nullptr;
+  double2* dy = nullptr;
+  int set(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+struct lsg_host {
+  HostBuild hb;
+};
+
+extern "C" {
+
+int lsg_bumps_from_seed(uint64_t seed, int n, double cmin, double cmax, double wmin, double wmax, double amin,
+                        double amax, lsg_contrast* out) {
+  if (!out || n < 0 || n > LSG_MAX_BUMPS) return LSG_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  out->kind = 6;
+  out->n_bumps = n;
+  SplitMix64 rng(seed);
+  for (int i = 0; i < n; ++i) {
+    out->cx[i] = cmin + (cmax - cmin) * rng.uniform();
+    out->cy[i] = cmin + (cmax - cmin) * rng.uniform();
+    out->width[i] = wmin + (wmax - wmin) * rng.uniform();
+    out->amp[i] = amin + (amax - amin) * rng.uniform();
+  }
+  return LSG_OK;
+}
+
+int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev == 0) {
+    g_create_error = std::string("no HIP device available (") + (he != hipSuccess ? hipGetErrorString(he) : "0 devices") +
+                     "); this library has no CPU execution path";
+    return LSG_ERR_DEVICE;
+  }
+  auto ctx = std::make_unique<lsg_ctx>();
+  std::string e;
+  int rc = host_build(in, ctx->hb, e);
+  if (rc != LSG_OK) {
+    g_create_error = e;
+    return rc;
+  }
+  e = ctx->dev.upload(ctx->hb.plan);
+  if (!e.empty()) {
+    ctx->dev.release();
+    g_create_error = e;
+    return LSG_ERR_DEVICE;
+  }
+  // the host copy of the operators is no longer needed
+  std::vector<cplx>().swap(ctx->hb.plan.ops);
+  std::vector<int>().swap(ctx->hb.plan.src);
+  const size_t nb = sizeof(double2) * size_t(ctx->hb.plan.N);
+  if (hipMalloc((void**)&ctx->dx, nb) != hipSuccess || hipMalloc((void**)&ctx->dy, nb) != hipSuccess) {
+    ctx->dev.release();
+    g_create_error = "hipMalloc of matvec buffers failed";
+    return LSG_ERR_DEVICE;
+  }
+  *out = ctx.release();
+  return LSG_OK;
+}
+
+void lsg_destroy(lsg_ctx* ctx) {
+  if (!ctx) return;
+  ctx->dev.release();
+  if (ctx->dx) hipFree(ctx->dx);
+  if (ctx->dy) hipFree(ctx->dy);
+  delete ctx;
+}
+
+const char* lsg_last_error(const lsg_ctx* ctx) { return ctx ? ctx->err.c_str() : "context is NULL"; }
+const char* lsg_create_error(void) { return g_create_error.c_str(); }
+
+int lsg_get_info(const lsg_ctx* ctx, lsg_info* info) {
+  if (!ctx || !info) return LSG_ERR_ARG;
+  fill_info(ctx->hb, info);
+  return LSG_OK;
+}
+
+int lsg_grid_points(const lsg_ctx* ctx, double* x, double* y) {
+  if (!ctx || !x || !y) return LSG_ERR_ARG;
+  std::memcpy(x, ctx->hb.tree.gx.data(), sizeof(double) * ctx->hb.tree.N);
+  std::memcpy(y, ctx->hb.tree.gy.data(), sizeof(double) * ctx->hb.tree.N);
+  return LSG_OK;
+}
+
+int lsg_contrast_at_nodes(const lsg_ctx* ctx, double* qk2) {
+  if (!ctx || !qk2) return LSG_ERR_ARG;
+  const Tree& t = ctx->hb.tree;
+  const double k = ctx->hb.prob.kappa;
+  for (int i = 0; i < t.N; ++i) qk2[i] = k * k * ctx->hb.prob.q(t.gx[i], t.gy[i]);
+  return LSG_OK;
+}
+
+int lsg_incident(const lsg_ctx* ctx, double* u) {
+  if (!ctx || !u) return LSG_ERR_ARG;
+  const Tree& t = ctx->hb.tree;
+  for (int i = 0; i < t.N; ++i) {
+    cplx v = ctx->hb.prob.u_inc(t.gx[i], t.gy[i]);
+    u[2 * i] = v.real();
+    u[2 * i + 1] = v.imag();
+  }
+  return LSG_OK;
+}
+
+int lsg_rhs(const lsg_ctx* ctx, double* b) {
+  if (!ctx || !b) return LSG_ERR_ARG;
+  const Tree& t = ctx->hb.tree;
+  for (int i = 0; i < t.N; ++i) {
+    cplx v = ctx->hb.prob.rhs(t.gx[i], t.gy[i]);
+    b[2 * i] = v.real();
+    b[2 * i + 1] = v.imag();
+  }
+  return LSG_OK;
+}
+
+int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld) {
+  if (!ctx) return LSG_ERR_ARG;
+  const int64_t N = ctx->hb.plan.N;
+  if (!x || !y || nrhs < 1 || ld < N) return ctx->set(LSG_ERR_ARG, "bad matvec arguments");
+  const size_t nb = sizeof(double2) * size_t(N);
+  for (int r = 0; r < nrhs; ++r) {
+    const double* xr = x + 2 * ld * r;
+    for (int64_t i = 0; i < 2 * N; ++i)
+      if (!std::isfinite(xr[i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    hipStream_t s = ctx->dev.stream;
+    if (hipMemcpyAsync(ctx->dx, xr, nb, hipMemcpyHostToDevice, s) != hipSuccess)
+      return ctx->set(LSG_ERR_DEVICE, "copy in failed");
+    ctx->dev.matvec(ctx->dx, ctx->dy);
+    if (hipMemcpyAsync(y + 2 * ld * r, ctx->dy, nb, hipMemcpyDeviceToHost, s) != hipSuccess)
+      return ctx->set(LSG_ERR_DEVICE, "copy out failed");
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("matvec: ") + hipGetErrorString(e));
+  }
+  return LSG_OK;
+}
+
+int lsg_matvec_device(lsg_ctx* ctx, const void* x_dev, void* y_dev) {
+  if (!ctx || !x_dev || !y_dev) return LSG_ERR_ARG;
+  ctx->dev.matvec(static_cast<const double2*>(x_dev), static_cast<double2*>(y_dev));
+  return LSG_OK;
+}
+
+int lsg_sync(lsg_ctx* ctx) {
+  if (!ctx) return LSG_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(ctx->dev.stream);
+  if (e != hipSuccess) return ctx->set(LSG_ERR_DEVICE, hipGetErrorString(e));
+  return LSG_OK;
+}
+
+int lsg_profile_passes(lsg_ctx* ctx, int reps, double* ms, int64_t* bytes, int max_pass) {
+  if (!ctx || reps < 1 || !ms || !bytes) return LSG_ERR_ARG;
+  std::vector<double> m;
+  std::vector<int64_t> b;
+  std::vector<std::string> names;
+  std::string e = ctx->dev.profile(reps, m, b, names);
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  const int n = std::min<int>(max_pass, int(m.size()));
+  for (int i = 0; i < n; ++i) { ms[i] = m[i]; bytes[i] = b[i]; }
+  std::string all;
+  for (auto& s : names) all += s + ",";
+  ctx->err = all;  // pass names are reported through lsg_last_error
+  return int(m.size());
+}
+
+int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max_restarts, double tol,
+              lsg_stats* stats) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (!b || !x || restart_len < 1 || restart_len > 500 || max_restarts < 1)
+    return ctx->set(LSG_ERR_ARG, "bad gmres arguments (restart_len in [1,500], max_restarts >= 1)");
+  const int64_t N = ctx->hb.plan.N;
+  for (int64_t i = 0; i < 2 * N; ++i)
+    if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+  const size_t nb = sizeof(double2) * size_t(N);
+  double2 *db = nullptr, *dxs = nullptr;
+  if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
+    if (db) hipFree(db);
+    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+  }
+  hipStream_t s = ctx->dev.stream;
+  hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
+  GmresResult r = device_gmres(ctx->dev, db, dxs, restart_len, max_restarts, tol);
+  hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  hipFree(db);
+  hipFree(dxs);
+  if (stats) {
+    stats->iterations = r.iterations;
+    stats->restarts = r.restarts;
+    stats->converged = r.converged;
+    stats->rel_residual = r.rel_residual;
+    stats->est_residual = r.est_residual;
+    stats->seconds = r.seconds;
+  }
+  if (r.status != 0) return ctx->set(r.status, r.error);
+  return LSG_OK;
+}
+
+// ----------------------------------------------------------- host-only API
+int lsg_host_build(const lsg_inputs* in, lsg_host** out) {
+  g_host_error.clear();
+  if (!out) { g_host_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  auto h = std::make_unique<lsg_host>();
+  std::string e;
+  int rc = host_build(in, h->hb, e);
+  if (rc != LSG_OK) {
+    g_host_error = e;
+    return rc;
+  }
+  h->hb.far = h->hb.plan.far_leaf;
+  *out = h.release();
+  return LSG_OK;
+}
+
+int lsg_host_build_tree(const lsg_inputs* in, lsg_host** out) {
+  g_host_error.clear();
+  if (!out) { g_host_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  auto h = std::make_unique<lsg_host>();
+  std::string e;
+  int rc = host_build(in, h->hb, e, true);
+  if (rc != LSG_OK) {
+    g_host_error = e;
+    return rc;
+  }
+  h->hb.plan.N = h->hb.tree.N;
+  *out = h.release();
+  return LSG_OK;
+}
+
+void lsg_host_destroy(lsg_host* h) { delete h; }
+const char* lsg_host_error(void) { return g_host_error.c_str(); }
+
+int lsg_host_info(const lsg_host* h, lsg_info* info) {
+  if (!h || !info) return LSG_ERR_ARG;
+  fill_info(h->hb, info);
+  return LSG_OK;
+}
+
+int64_t lsg_host_far_table(const lsg_host* h, double* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.far.data(), sizeof(cplx) * h->hb.far.size());
+  return int64_t(h->hb.far.size());
+}
+
+int64_t lsg_host_near_tables(const lsg_host* h, double* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.T.data(), sizeof(cplx) * h->hb.plan.T.size());
+  return int64_t(h->hb.plan.T.size());
+}
+
+int64_t lsg_host_skeleton(const lsg_host* h, int* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.skeleton.data(), sizeof(int) * h->hb.plan.skeleton.size());
+  return int64_t(h->hb.plan.skeleton.size());
+}
+
+int lsg_host_il_pairs(const lsg_host* h, int64_t* out, int max_levels) {
+  if (!h || !out) return LSG_ERR_ARG;
+  const Tree& t = h->hb.tree;
+  const int n = std::min<int>(max_levels, t.nLevels + 1);
+  for (int l = 0; l < n; ++l) {
+    int64_t s = 0;
+    for (const Box& b : t.lv[l]) {
+      s += int64_t(b.il.size());
+      for (const auto& c : b.cone_il) s += int64_t(c.size());
+    }
+    out[l] = s;
+  }
+  return n;
+}
+
+int64_t lsg_host_qpinv(const lsg_host* h, double* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.bs.qpinv.data(), sizeof(double) * h->hb.bs.qpinv.size());
+  return int64_t(h->hb.bs.qpinv.size());
+}
+
+int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels) {
+  if (!h || !out) return LSG_ERR_ARG;
+  const auto& v = h->hb.plan.level_max_rank;
+  const int n = std::min<int>(max_levels, int(v.size()));
+  for (int i = 0; i < n; ++i) out[i] = v[i];
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Device-resident operator and solver state (HIP, gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "nca.hpp"
+
+namespace lsg {
+
+struct DevPass {
+  std::string name;
+  int kind = 0;        // 0 store, 1 accumulate
+  int ndest = 0;
+  int64_t dest_base = 0;  // index into the concatenated destination arrays
+  int64_t entries = 0, n_src = 0, n_dst = 0;
+  int64_t bytes() const { return 16 * entries + 4 * n_src + 16 * n_src + 16 * n_dst; }
+};
+
+struct DeviceOperator {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int N = 0, rank = 0, n_leaves = 0;
+  bool find_phi = true;
+  int64_t n_state = 0;
+  double2* state = nullptr;    // [x | y | charges | potentials]
+  double2* ops = nullptr;
+  int* src = nullptr;
+  int* d_rows = nullptr;
+  int* d_ncol = nullptr;
+  int* d_dst = nullptr;
+  long long* d_op = nullptr;
+  long long* d_srcoff = nullptr;
+  double2* T = nullptr;        // 9 x rank x rank, column-major per slot
+  int* nbr = nullptr;          // n_leaves x 9
+  double* qk2 = nullptr;       // N
+  std::vector<DevPass> passes;
+  int64_t near_bytes = 0;
+
+  std::string upload(const Plan& plan);
+  void release();
+  // y = A x on the stream; x, y device pointers of N complex128
+  void matvec(const double2* x, double2* y);
+  // event-timed per-pass averages over reps matvecs
+  std::string profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes, std::vector<std::string>& names);
+};
+
+// Restarted GMRES (CGS2 Arnoldi, complex Givens) with device-resident Krylov
+// basis, Hessenberg and rotations.
+struct GmresResult {
+  int iterations = 0, restarts = 0, converged = 0;
+  double rel_residual = 0, est_residual = 0, seconds = 0;
+  int status = 0;  // 0 ok, else LSG_ERR_*
+  std::string error;
+};
+GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts, double tol);
+
+}  // namespace lsg

@@ -1,0 +1,800 @@
+#include "nca.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <set>
+
+namespace lsg {
+
+// ---------------------------------------------------------------------------
+// Entry kernel: G_FMM2DTree.hpp:630-657
+// ---------------------------------------------------------------------------
+void EntryKernel::hvec(int i, int leaf, cplx* h) const {
+  const double xi = tree->gx[i], yi = tree->gy[i];
+  const double cx = leaf_cx[leaf], cy = leaf_cy[leaf];
+  const double th = std::atan2(yi - cy, xi - cx);
+  const double R2 = (yi - cy) * (yi - cy) + (xi - cx) * (xi - cx);
+  const double R = kappa * std::sqrt(R2);
+  double J[kSeries + 1], Y[kSeries + 1];
+  bessel_jy(R, kSeries, J, Y);
+  const cplx I(0.0, 1.0);
+  for (int n = -kSeries; n <= kSeries; ++n) {
+    int an = n < 0 ? -n : n;
+    double jn = J[an], yn = Y[an];
+    if (n < 0 && (an & 1)) { jn = -jn; yn = -yn; }
+    h[n + kSeries] = (jn + I * yn) * std::exp(I * double(n) * th);
+  }
+}
+
+cplx EntryKernel::entry(const cplx* h, int i, int j) const {
+  const int leaf = j / rank, idx = j % rank;
+  const cplx* Mr = M[leaf_level[leaf]].data();
+  cplx s(0.0, 0.0);
+  for (int t = 0; t < 2 * kSeries + 1; ++t) s += Mr[size_t(t) * rank + idx] * h[t];
+  s *= cplx(0.0, 1.0) / 4.0;
+  if (find_phi) {
+    s *= qk2[i];
+    if (i == j) s += 1.0;
+  }
+  return s;
+}
+
+void EntryKernel::block(const int* rows, int nr, const int* cols, int nc, cplx* out) const {
+  cplx h[2 * kSeries + 1];
+  for (int r = 0; r < nr; ++r) {
+    int prev = -1;
+    for (int c = 0; c < nc; ++c) {
+      int leaf = cols[c] / rank;
+      if (leaf != prev) { hvec(rows[r], leaf, h); prev = leaf; }
+      out[size_t(c) * nr + r] = entry(h, rows[r], cols[c]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ACA_only_nodes, GMRES/ACA.hpp:253-481
+// ---------------------------------------------------------------------------
+namespace {
+// maxAbsVector (ACA.hpp:153-166): first index of the largest |v| over `allowed`
+// (ascending); -1 if `allowed` is empty.
+int max_abs(const std::vector<cplx>& v, const std::set<int>& allowed) {
+  if (allowed.empty()) return -1;
+  auto it = allowed.begin();
+  int index = *it;
+  double mx = std::abs(v[index]);
+  for (; it != allowed.end(); ++it)
+    if (std::abs(v[*it]) > mx) { index = *it; mx = std::abs(v[index]); }
+  return index;
+}
+double vnorm(const std::vector<cplx>& v) {
+  double s = 0;
+  for (const cplx& z : v) s += std::norm(z);
+  return std::sqrt(s);
+}
+cplx vdot(const std::vector<cplx>& a, const std::vector<cplx>& b) {  // Eigen a.dot(b) = a^H b
+  cplx s(0.0, 0.0);
+  for (size_t i = 0; i < a.size(); ++i) s += std::conj(a[i]) * b[i];
+  return s;
+}
+}  // namespace
+
+void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const std::vector<int>& cols, double eps,
+                    AcaResult& out) {
+  const int N1 = int(rows.size()), N2 = int(cols.size());
+  out.row_bases.clear(); out.col_bases.clear(); out.Ar.clear(); out.rank = 0;
+  const int mn = std::min(N1, N2);
+  std::vector<cplx> row(N2), col(N1), u, v, row_temp, col_temp;
+  std::vector<std::vector<cplx>> U, V;
+  std::vector<std::vector<cplx>> Arows;
+  std::set<int> rem_row, rem_col;
+  for (int l = 0; l < N1; ++l) rem_row.insert(l);
+  for (int l = 0; l < N2; ++l) rem_col.insert(l);
+  auto getRow = [&](int i, std::vector<cplx>& dst) {
+    dst.resize(N2);
+    K.block(&rows[i], 1, cols.data(), N2, dst.data());
+  };
+  auto getCol = [&](int j, std::vector<cplx>& dst) {
+    dst.resize(N1);
+    K.block(rows.data(), N1, &cols[j], 1, dst.data());
+  };
+  int rank = 0;
+  int row_index = 0, col_index = 0;
+  if (N1 < N2) {
+    int l_local;
+    for (l_local = 0; l_local < N1; ++l_local) {
+      row_index = l_local;
+      getRow(row_index, row);
+      col_index = max_abs(row, rem_col);
+      if (std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0)) break;
+    }
+    if (l_local == N1) return;
+    v.resize(N2);
+    for (int l = 0; l < N2; ++l) v[l] = row[l] / row[col_index];
+    out.row_bases.push_back(row_index);
+    out.col_bases.push_back(col_index);
+    getCol(col_index, col);
+    u = col;
+    U.push_back(u); V.push_back(v);
+    Arows.push_back(row);
+    rem_col.erase(col_index); rem_row.erase(row_index);
+    rank = 1;
+    double normS = 0.0, prev_normS = 1000.0;
+    row_index = max_abs(col, rem_row);
+    while (row_index >= 0 && std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > eps &&
+           std::fabs(prev_normS - normS) >= eps * prev_normS && rank < mn) {
+      out.row_bases.push_back(row_index);
+      getRow(row_index, row);
+      row_temp = row;
+      for (int l = 0; l <= rank - 1; ++l)
+        for (size_t s = 0; s < V[l].size(); ++s) row[s] -= U[l][row_index] * V[l][s];
+      col_index = max_abs(row, rem_col);
+      out.col_bases.push_back(col_index);
+      for (size_t l = 0; l < row.size(); ++l) v[l] = row[l] / row[col_index];
+      getCol(col_index, col);
+      col_temp = col;
+      for (int l = 0; l <= rank - 1; ++l)
+        for (size_t s = 0; s < U[l].size(); ++s) col[s] -= V[l][col_index] * U[l][s];
+      u = col;
+      if (vnorm(u) < 1e-8 || vnorm(v) < 1e-8) {
+        out.row_bases.pop_back(); out.col_bases.pop_back();
+        break;
+      }
+      U.push_back(u); V.push_back(v);
+      Arows.push_back(row_temp);
+      ++rank;
+      rem_col.erase(col_index); rem_row.erase(row_index);
+      if (rank != 2) prev_normS = normS;
+      normS = std::pow(normS, 2) + std::pow(vnorm(u) * vnorm(v), 2);
+      for (int l = 0; l <= rank - 1; ++l) normS += 2 * std::abs(vdot(U[l], u) * vdot(V[l], v));
+      normS = std::sqrt(normS);
+      row_index = max_abs(col, rem_row);
+    }
+  } else {
+    int l_local;
+    for (l_local = 0; l_local < N2; ++l_local) {
+      col_index = l_local;
+      getCol(col_index, col);
+      row_index = max_abs(col, rem_row);
+      if (std::abs(col[row_index]) > eps * std::pow(10, -1.0 * 0)) break;
+    }
+    if (l_local == N2) return;
+    out.col_bases.push_back(col_index);
+    v.resize(N1);
+    for (int l = 0; l < N1; ++l) v[l] = col[l] / col[row_index];
+    out.row_bases.push_back(row_index);
+    getRow(row_index, row);
+    u = row;
+    U.push_back(u); V.push_back(v);
+    Arows.push_back(row);
+    rank = 1;
+    rem_row.erase(row_index); rem_col.erase(col_index);
+    double normS = 0.0, prev_normS = 1000.0;
+    col_index = max_abs(row, rem_col);
+    while (col_index >= 0 && std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > eps &&
+           std::fabs(prev_normS - normS) >= eps * prev_normS && rank < mn) {
+      out.col_bases.push_back(col_index);
+      getCol(col_index, col);
+      col_temp = col;
+      for (int l = 0; l <= rank - 1; ++l)
+        for (size_t s = 0; s < V[l].size(); ++s) col[s] -= U[l][col_index] * V[l][s];
+      row_index = max_abs(col, rem_row);
+      out.row_bases.push_back(row_index);
+      for (size_t l = 0; l < col.size(); ++l) v[l] = col[l] / col[row_index];
+      getRow(row_index, row);
+      row_temp = row;
+      for (int l = 0; l <= rank - 1; ++l)
+        for (size_t s = 0; s < U[l].size(); ++s) row[s] -= V[l][row_index] * U[l][s];
+      u = row;
+      if (vnorm(u) < 1e-8 || vnorm(v) < 1e-8) {
+        out.col_bases.pop_back(); out.row_bases.pop_back();
+        break;
+      }
+      U.push_back(u); V.push_back(v);
+      Arows.push_back(row_temp);
+      ++rank;
+      rem_row.erase(row_index); rem_col.erase(col_index);
+      if (rank != 2) prev_normS = normS;
+      normS = std::pow(normS, 2) + std::pow(vnorm(u) * vnorm(v), 2);
+      for (int l = 0; l <= rank - 1; ++l) normS += 2 * std::abs(vdot(U[l], u) * vdot(V[l], v));
+      normS = std::sqrt(normS);
+      col_index = max_abs(row, rem_col);
+    }
+  }
+  out.rank = rank;
+  out.Ar.resize(size_t(rank) * N2);
+  for (int r = 0; r < rank; ++r) std::copy(Arows[r].begin(), Arows[r].end(), out.Ar.begin() + size_t(r) * N2);
+}
+
+// ---------------------------------------------------------------------------
+// NCA skeletons and plan
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Contrib {
+  std::vector<int> rows;                 // target nodes
+  const std::vector<int>* cols = nullptr;  // source charge nodes
+  const std::vector<cplx>* Z = nullptr;    // right factor (|cols| x |cols|, col-major), L2L only
+  const std::vector<int>* src_list = nullptr;  // source state indices (M2L)
+  int64_t src_base = -1;                       // contiguous source (L2L)
+};
+
+struct Owner {
+  bool out_active = false, in_active = false;
+  std::vector<int> out_check, out_charge, in_check, in_charge, ucp;
+  std::vector<int> src_nodes;  // outgoing ACA columns
+  std::vector<cplx> Ar;        // rank x src, row-major
+  std::vector<cplx> P;         // rank x src, col-major (Atilde^-1 Ar)
+  std::vector<cplx> Zin;       // rank x rank, col-major (Atilde_in^-1)
+  std::vector<int> fout;       // state indices aligned with out_charge
+  int64_t uin = -1;            // state offset of the incoming potential (size ucp)
+  std::vector<Contrib> contribs;
+};
+
+struct Builder {
+  const Tree& t;
+  const BasisSet& bs;
+  const EntryKernel& K;
+  double eps;
+  std::vector<std::vector<Owner>> L;  // L[l][k]: LFR owners (and leaves)
+  std::vector<std::vector<Owner>> H;  // H[l][k*nc+c]: HFR cones
+  int rank;
+
+  Builder(const Tree& tr, const BasisSet& b, const EntryKernel& k, double e) : t(tr), bs(b), K(k), eps(e) {
+    rank = bs.rank;
+    L.resize(t.nLevels + 1);
+    H.resize(t.nLevels + 1);
+    for (int l = 0; l <= t.nLevels; ++l) {
+      L[l].resize(t.lv[l].size());
+      if (l < t.level_LFR) H[l].resize(t.lv[l].size() * size_t(t.nCones[l]));
+    }
+  }
+  bool is_lfr(int l, int k) const { return l >= t.level_LFR || t.lv[l][k].leaf; }
+  Owner& hc(int l, int k, int c) { return H[l][size_t(k) * t.nCones[l] + c]; }
+  void leaf_nodes(int l, int k, std::vector<int>& out) const {
+    int li = t.lv[l][k].leaf_index;
+    for (int i = 0; i < rank; ++i) out.push_back(li * rank + i);
+  }
+  static void app(std::vector<int>& dst, const std::vector<int>& src) { dst.insert(dst.end(), src.begin(), src.end()); }
+
+  // getParticlesFromChildrenLFR_* (G_FMM2DTree.hpp:2013-2060, 2242-2292)
+  // which: 0 = outgoing charges, 1 = incoming checks
+  void lfr_children(int j, int k, int which, std::vector<int>& out) {
+    if (t.lv[j][k].leaf) { leaf_nodes(j, k, out); return; }
+    const int J = j + 1;
+    for (int c = 0; c < 4; ++c) {
+      int Kc = t.lv[j][k].child[c];
+      if (is_lfr(J, Kc)) app(out, which == 0 ? L[J][Kc].out_charge : L[J][Kc].in_check);
+      else
+        for (int cone = 0; cone < t.nCones[J]; ++cone) app(out, which == 0 ? hc(J, Kc, cone).out_charge : hc(J, Kc, cone).in_check);
+    }
+  }
+  // getParticlesFromChildrenHFR_* (G_FMM2DTree.hpp:3258-3320, 3421-3483)
+  void hfr_children(int j, int k, int cone, int which, std::vector<int>& out) {
+    if (t.lv[j][k].leaf) { leaf_nodes(j, k, out); return; }
+    const int J = j + 1;
+    for (int c = 0; c < 4; ++c) {
+      int Kc = t.lv[j][k].child[c];
+      if (j == t.level_LFR - 1 || t.lv[J][Kc].leaf) app(out, which == 0 ? L[J][Kc].out_charge : L[J][Kc].in_check);
+      else app(out, which == 0 ? hc(J, Kc, cone / 2).out_charge : hc(J, Kc, cone / 2).in_check);
+    }
+  }
+
+  void qr_of(const std::vector<int>& r, const std::vector<int>& c, ColPivQR& qr) {
+    std::vector<cplx> A(r.size() * c.size());
+    K.block(r.data(), int(r.size()), c.data(), int(c.size()), A.data());
+    qr.factor(A.data(), int(r.size()), int(c.size()));
+  }
+  void finish_out(Owner& o, const std::vector<int>& rows, const std::vector<int>& cols, const AcaResult& res) {
+    for (int r : res.row_bases) o.out_check.push_back(rows[r]);
+    for (int c : res.col_bases) o.out_charge.push_back(cols[c]);
+    ColPivQR qr;
+    qr_of(o.out_check, o.out_charge, qr);
+    const int r = res.rank, ns = int(cols.size());
+    std::vector<cplx> B(size_t(r) * ns);
+    for (int i = 0; i < r; ++i)
+      for (int j = 0; j < ns; ++j) B[size_t(j) * r + i] = res.Ar[size_t(i) * ns + j];
+    o.P.resize(size_t(qr.cols) * ns);
+    qr.solve(B.data(), ns, o.P.data());
+    o.src_nodes = cols;
+  }
+  void finish_in(Owner& o, const std::vector<int>& rows, const std::vector<int>& cols, const AcaResult& res) {
+    for (int r : res.row_bases) o.in_check.push_back(rows[r]);
+    for (int c : res.col_bases) o.in_charge.push_back(cols[c]);
+    ColPivQR qr;
+    qr_of(o.in_check, o.in_charge, qr);
+    const int r = res.rank;
+    std::vector<cplx> I(size_t(r) * r, cplx(0));
+    for (int i = 0; i < r; ++i) I[size_t(i) * r + i] = 1.0;
+    o.Zin.resize(size_t(r) * r);
+    qr.solve(I.data(), r, o.Zin.data());
+  }
+
+  // getNodes_LFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:2083-2212, 2294-2358)
+  void lfr_out(int j, int k) {
+    Owner& o = L[j][k];
+    std::vector<int> cols, rows;
+    lfr_children(j, k, 0, cols);
+    for (const BoxRef& b : t.lv[j][k].il) lfr_children(b.l, b.k, 1, rows);
+    if (!rows.empty() && !cols.empty()) {
+      o.out_active = true;
+      AcaResult res;
+      aca_only_nodes(K, rows, cols, eps, res);
+      if (res.rank > 0) finish_out(o, rows, cols, res);
+    } else {
+      o.out_active = false;
+      o.out_charge = cols;
+    }
+  }
+  void lfr_in(int j, int k) {
+    Owner& o = L[j][k];
+    std::vector<int> rows, cols;
+    lfr_children(j, k, 1, rows);
+    for (const BoxRef& b : t.lv[j][k].il) lfr_children(b.l, b.k, 0, cols);
+    if (!rows.empty() && !cols.empty()) {
+      o.in_active = true;
+      AcaResult res;
+      aca_only_nodes(K, rows, cols, eps, res);
+      if (res.rank > 0) finish_in(o, rows, cols, res);
+    } else {
+      o.in_active = false;
+      o.in_check = rows;
+    }
+  }
+  // getNodes_HFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:3322-3392, 3485-3546)
+  void hfr_out(int j, int k, int cone) {
+    Owner& o = hc(j, k, cone);
+    std::vector<int> cols, rows;
+    hfr_children(j, k, cone, 0, cols);
+    for (const BoxRef& b : t.lv[j][k].cone_il[cone]) {
+      int coneB = t.cone_toward(t.lv[b.l][b.k], t.lv[j][k], b.l);
+      hfr_children(b.l, b.k, coneB, 1, rows);
+    }
+    if (!rows.empty() && !cols.empty()) {
+      o.out_active = true;
+      AcaResult res;
+      aca_only_nodes(K, rows, cols, eps, res);
+      if (res.rank > 0) finish_out(o, rows, cols, res);
+    } else {
+      o.out_active = false;
+      o.out_charge = cols;
+    }
+  }
+  void hfr_in(int j, int k, int cone) {
+    Owner& o = hc(j, k, cone);
+    std::vector<int> rows, cols;
+    hfr_children(j, k, cone, 1, rows);
+    for (const BoxRef& b : t.lv[j][k].cone_il[cone]) {
+      int coneB = t.cone_toward(t.lv[b.l][b.k], t.lv[j][k], b.l);
+      hfr_children(b.l, b.k, coneB, 0, cols);
+    }
+    if (!rows.empty() && !cols.empty()) {
+      o.in_active = true;
+      AcaResult res;
+      aca_only_nodes(K, rows, cols, eps, res);
+      if (res.rank > 0) finish_in(o, rows, cols, res);
+    } else {
+      o.in_active = false;
+      o.in_check = rows;
+    }
+  }
+
+  void run_nca() {
+    // getNodes_LFR (G_FMM2DTree.hpp:2062-2081)
+    for (int j = t.nLevels; j >= t.level_LFR; --j) {
+      const int nb = int(t.lv[j].size());
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nb; ++k) lfr_out(j, k);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nb; ++k) lfr_in(j, k);
+    }
+    // getNodes_HFR (G_FMM2DTree.hpp:3232-3255)
+    for (int j = t.level_LFR - 1; j >= 2; --j) {
+      const int nb = int(t.lv[j].size()), nc = t.nCones[j];
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int w = 0; w < nb * nc; ++w) {
+        int k = w / nc, c = w % nc;
+        if (t.lv[j][k].leaf) { if (c == 0) lfr_out(j, k); }
+        else hfr_out(j, k, c);
+      }
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int w = 0; w < nb * nc; ++w) {
+        int k = w / nc, c = w % nc;
+        if (t.lv[j][k].leaf) { if (c == 0) lfr_in(j, k); }
+        else hfr_in(j, k, c);
+      }
+    }
+    // getUserCheckPoints (G_FMM2DTree.hpp:3146-3230)
+    for (int j = t.nLevels; j >= t.level_LFR; --j)
+      for (int k = 0; k < int(t.lv[j].size()); ++k) {
+        Owner& o = L[j][k];
+        if (t.lv[j][k].leaf) leaf_nodes(j, k, o.ucp);
+        else if (o.in_active) o.ucp = o.in_check;
+        else
+          for (int c = 0; c < 4; ++c) app(o.ucp, L[j + 1][t.lv[j][k].child[c]].ucp);
+      }
+    for (int j = t.level_LFR - 1; j >= 2; --j)
+      for (int k = 0; k < int(t.lv[j].size()); ++k) {
+        if (t.lv[j][k].leaf) { leaf_nodes(j, k, L[j][k].ucp); continue; }
+        for (int cone = 0; cone < t.nCones[j]; ++cone) {
+          Owner& o = hc(j, k, cone);
+          if (o.in_active) { o.ucp = o.in_check; continue; }
+          for (int c = 0; c < 4; ++c) {
+            int J = j + 1, Kc = t.lv[j][k].child[c];
+            if (t.lv[J][Kc].leaf || J == t.level_LFR) app(o.ucp, L[J][Kc].ucp);
+            else app(o.ucp, hc(J, Kc, cone / 2).ucp);
+          }
+        }
+      }
+  }
+
+  // ------------------------------------------------------------------ plan
+  int64_t next_slot = 0;
+
+  // outgoing charge state indices, aligned with out_charge (reference:
+  // LFR_M2M_ILActive_False / HFR_M2M_ILActiveFalse concatenations)
+  void assign_fout() {
+    for (int j = t.nLevels; j >= 2; --j) {
+      for (int k = 0; k < int(t.lv[j].size()); ++k) {
+        if (is_lfr(j, k)) {
+          Owner& o = L[j][k];
+          if (o.out_active) {
+            for (size_t i = 0; i < o.out_charge.size(); ++i) o.fout.push_back(int(next_slot++));
+          } else if (t.lv[j][k].leaf) {
+            leaf_nodes(j, k, o.fout);
+          } else {
+            for (int c = 0; c < 4; ++c) {
+              int J = j + 1, Kc = t.lv[j][k].child[c];
+              if (is_lfr(J, Kc)) app(o.fout, L[J][Kc].fout);
+              else
+                for (int cone = 0; cone < t.nCones[J]; ++cone) app(o.fout, hc(J, Kc, cone).fout);
+            }
+          }
+        } else {
+          for (int cone = 0; cone < t.nCones[j]; ++cone) {
+            Owner& o = hc(j, k, cone);
+            if (o.out_active) {
+              for (size_t i = 0; i < o.out_charge.size(); ++i) o.fout.push_back(int(next_slot++));
+            } else {
+              for (int c = 0; c < 4; ++c) {
+                int J = j + 1, Kc = t.lv[j][k].child[c];
+                if (j == t.level_LFR - 1 || t.lv[J][Kc].leaf) app(o.fout, L[J][Kc].fout);
+                else app(o.fout, hc(J, Kc, cone / 2).fout);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  // source state indices of an outgoing operator, aligned with src_nodes
+  void out_src(int j, int k, int cone, std::vector<int>& s) {
+    if (t.lv[j][k].leaf) { leaf_nodes(j, k, s); return; }
+    const int J = j + 1;
+    for (int c = 0; c < 4; ++c) {
+      int Kc = t.lv[j][k].child[c];
+      if (cone < 0) {
+        if (is_lfr(J, Kc)) app(s, L[J][Kc].fout);
+        else
+          for (int cc = 0; cc < t.nCones[J]; ++cc) app(s, hc(J, Kc, cc).fout);
+      } else {
+        if (j == t.level_LFR - 1 || t.lv[J][Kc].leaf) app(s, L[J][Kc].fout);
+        else app(s, hc(J, Kc, cone / 2).fout);
+      }
+    }
+  }
+
+  struct Dest {
+    int64_t dst;
+    int rows;
+    const std::vector<cplx>* P = nullptr;      // upward: single operator
+    const std::vector<int>* P_src = nullptr;
+    std::vector<int> up_src;
+    std::vector<Contrib>* contribs = nullptr;  // downward
+  };
+
+  // Children that receive a share of an owner's incoming potential, with the
+  // row segment of the owner's ucp each one holds (getUserCheckPoints order).
+  struct ChildTarget { Owner* o; int level; };
+  void child_targets(int j, int k, int cone, std::vector<ChildTarget>& out) {
+    const int J = j + 1;
+    for (int c = 0; c < 4; ++c) {
+      int Kc = t.lv[j][k].child[c];
+      if (cone < 0) out.push_back({&L[J][Kc], J});
+      else if (t.lv[J][Kc].leaf || J == t.level_LFR) out.push_back({&L[J][Kc], J});
+      else out.push_back({&hc(J, Kc, cone / 2), J});
+    }
+  }
+
+  void add_m2l(int j, int k, int cone, Owner& o) {
+    const Box& A = t.lv[j][k];
+    const std::vector<BoxRef>& il = cone < 0 ? A.il : A.cone_il[cone];
+    if (!o.in_active || o.ucp.empty()) return;
+    for (const BoxRef& b : il) {
+      Owner* s;
+      if (b.l < t.level_LFR && !t.lv[b.l][b.k].leaf) s = &hc(b.l, b.k, t.cone_toward(t.lv[b.l][b.k], A, b.l));
+      else s = &L[b.l][b.k];
+      if (s->out_charge.empty()) continue;
+      Contrib c;
+      c.rows = o.ucp;
+      c.cols = &s->out_charge;
+      c.src_list = &s->fout;
+      o.contribs.push_back(std::move(c));
+    }
+  }
+
+  // resolve an owner at level j after all its contributions are known
+  void resolve(int j, int k, int cone, Owner& o, std::vector<Dest>& dests, std::vector<Dest>& leaf_dests) {
+    const bool leaf = t.lv[j][k].leaf;
+    if (leaf) {
+      if (!o.contribs.empty()) {
+        Dest d;
+        d.dst = t.N + int64_t(t.lv[j][k].leaf_index) * rank;
+        d.rows = rank;
+        d.contribs = &o.contribs;
+        leaf_dests.push_back(std::move(d));
+      }
+      return;
+    }
+    if (!o.in_active) {
+      // pass-through: hand each contribution's row segments to the children
+      std::vector<ChildTarget> ch;
+      child_targets(j, k, cone, ch);
+      for (Contrib& c : o.contribs) {
+        size_t off = 0;
+        for (ChildTarget& tg : ch) {
+          size_t n = tg.o->ucp.size();
+          if (n > 0) {
+            Contrib s;
+            s.rows.assign(c.rows.begin() + off, c.rows.begin() + off + n);
+            s.cols = c.cols; s.Z = c.Z; s.src_list = c.src_list; s.src_base = c.src_base;
+            tg.o->contribs.push_back(std::move(s));
+          }
+          off += n;
+        }
+      }
+      o.contribs.clear();
+      return;
+    }
+    if (o.ucp.empty() || o.contribs.empty()) return;
+    o.uin = next_slot;
+    next_slot += int64_t(o.ucp.size());
+    Dest d;
+    d.dst = o.uin;
+    d.rows = int(o.ucp.size());
+    d.contribs = &o.contribs;
+    dests.push_back(std::move(d));
+    if (o.in_charge.empty()) return;
+    // L2L (fused with Atilde_in^-1) into the children
+    std::vector<ChildTarget> ch;
+    child_targets(j, k, cone, ch);
+    for (ChildTarget& tg : ch) {
+      if (tg.o->ucp.empty()) continue;
+      Contrib c;
+      c.rows = tg.o->ucp;
+      c.cols = &o.in_charge;
+      c.Z = &o.Zin;
+      c.src_base = o.uin;
+      tg.o->contribs.push_back(std::move(c));
+    }
+  }
+};
+
+int64_t contrib_cols(const Contrib& c) { return c.Z ? int64_t(c.cols->size()) : int64_t(c.cols->size()); }
+
+}  // namespace
+
+std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
+  EntryKernel K;
+  K.tree = &t;
+  K.rank = bs.rank;
+  K.kappa = prob.kappa;
+  K.find_phi = opt.find_phi;
+  K.qk2.resize(t.N);
+  for (int i = 0; i < t.N; ++i) K.qk2[i] = prob.kappa * prob.kappa * prob.q(t.gx[i], t.gy[i]);
+  K.leaf_level.resize(t.leaves.size());
+  K.leaf_cx.resize(t.leaves.size());
+  K.leaf_cy.resize(t.leaves.size());
+  K.M.resize(t.nLevels + 1);
+  for (size_t i = 0; i < t.leaves.size(); ++i) {
+    const Box& b = t.box(t.leaves[i]);
+    K.leaf_level[i] = t.leaves[i].l;
+    K.leaf_cx[i] = b.cx;
+    K.leaf_cy[i] = b.cy;
+    if (K.M[t.leaves[i].l].empty()) K.M[t.leaves[i].l] = far_table(bs, prob.kappa, t.boxRadius[t.leaves[i].l]);
+  }
+
+  Builder B(t, bs, K, std::pow(10, -1.0 * opt.TOL_POW));
+  B.run_nca();
+
+  plan = Plan();
+  plan.far_leaf = K.M[t.nLevels];
+  plan.N = t.N;
+  plan.rank = bs.rank;
+  plan.n_leaves = int(t.leaves.size());
+  plan.find_phi = opt.find_phi;
+  plan.qk2 = opt.find_phi ? K.qk2 : std::vector<double>(t.N, 0.0);
+  B.next_slot = 2 * int64_t(t.N);
+  B.assign_fout();
+  auto dump = [&](int l, int k, int cone, const Owner& o) {
+    std::vector<int>& d = plan.skeleton;
+    d.insert(d.end(), {l, k, cone, o.out_active ? 1 : 0, o.in_active ? 1 : 0, int(o.out_check.size()),
+                       int(o.out_charge.size()), int(o.in_check.size()), int(o.in_charge.size()), int(o.ucp.size())});
+    for (const std::vector<int>* v : {&o.out_check, &o.out_charge, &o.in_check, &o.in_charge, &o.ucp})
+      d.insert(d.end(), v->begin(), v->end());
+  };
+  for (int l = t.nLevels; l >= 2; --l) {
+    for (int k = 0; k < int(t.lv[l].size()); ++k)
+      if (B.is_lfr(l, k)) dump(l, k, -1, B.L[l][k]);
+    for (int k = 0; k < int(t.lv[l].size()); ++k)
+      if (!B.is_lfr(l, k))
+        for (int c = 0; c < t.nCones[l]; ++c) dump(l, k, c, B.hc(l, k, c));
+  }
+
+  plan.level_max_rank.assign(t.nLevels + 1, 0);
+  auto note_rank = [&](int l, int r) {
+    plan.level_max_rank[l] = std::max(plan.level_max_rank[l], r);
+    plan.max_rank = std::max(plan.max_rank, r);
+  };
+
+  using Dest = Builder::Dest;
+  std::vector<std::pair<PassHost, std::vector<Dest>>> passes;
+
+  // upward passes: leaf level to level 2 (LFR_M2M then HFR_M2M)
+  for (int j = t.nLevels; j >= 2; --j) {
+    std::vector<Dest> ds;
+    for (int k = 0; k < int(t.lv[j].size()); ++k) {
+      if (B.is_lfr(j, k)) {
+        Owner& o = B.L[j][k];
+        note_rank(j, int(o.out_charge.size()) * (o.out_active ? 1 : 0));
+        note_rank(j, int(o.in_charge.size()));
+        if (o.out_active && !o.out_charge.empty()) {
+          Dest d;
+          d.dst = o.fout[0];
+          d.rows = int(o.out_charge.size());
+          d.P = &o.P;
+          B.out_src(j, k, -1, d.up_src);
+          ds.push_back(std::move(d));
+        }
+      } else {
+        for (int c = 0; c < t.nCones[j]; ++c) {
+          Owner& o = B.hc(j, k, c);
+          note_rank(j, int(o.out_charge.size()) * (o.out_active ? 1 : 0));
+          note_rank(j, int(o.in_charge.size()));
+          if (o.out_active && !o.out_charge.empty()) {
+            Dest d;
+            d.dst = o.fout[0];
+            d.rows = int(o.out_charge.size());
+            d.P = &o.P;
+            B.out_src(j, k, c, d.up_src);
+            ds.push_back(std::move(d));
+          }
+        }
+      }
+    }
+    PassHost ph;
+    ph.name = "up_L" + std::to_string(j);
+    ph.level = j;
+    ph.kind = 0;
+    if (!ds.empty()) passes.push_back({ph, std::move(ds)});
+  }
+
+  // downward passes: level 2 to leaves (M2L + L2L grouped by destination)
+  std::vector<Dest> leaf_dests;
+  for (int j = 2; j <= t.nLevels; ++j) {
+    std::vector<Dest> ds;
+    for (int k = 0; k < int(t.lv[j].size()); ++k) {
+      if (B.is_lfr(j, k)) B.add_m2l(j, k, -1, B.L[j][k]);
+      else
+        for (int c = 0; c < t.nCones[j]; ++c) B.add_m2l(j, k, c, B.hc(j, k, c));
+    }
+    for (int k = 0; k < int(t.lv[j].size()); ++k) {
+      if (B.is_lfr(j, k)) B.resolve(j, k, -1, B.L[j][k], ds, leaf_dests);
+      else
+        for (int c = 0; c < t.nCones[j]; ++c) B.resolve(j, k, c, B.hc(j, k, c), ds, leaf_dests);
+    }
+    PassHost ph;
+    ph.name = "down_L" + std::to_string(j);
+    ph.level = j;
+    ph.kind = 0;
+    if (!ds.empty()) passes.push_back({ph, std::move(ds)});
+  }
+  {
+    PassHost ph;
+    ph.name = "leaf_far";
+    ph.level = t.nLevels;
+    ph.kind = 1;
+    if (!leaf_dests.empty()) passes.push_back({ph, std::move(leaf_dests)});
+  }
+  plan.n_state = B.next_slot;
+
+  // sizes and offsets
+  int64_t op_total = 0, src_total = 0;
+  for (auto& pr : passes) {
+    PassHost& ph = pr.first;
+    for (Dest& d : pr.second) {
+      int64_t nc = 0;
+      if (d.P) nc = int64_t(d.up_src.size());
+      else
+        for (const Contrib& c : *d.contribs) nc += contrib_cols(c);
+      ph.rows.push_back(d.rows);
+      ph.ncol.push_back(int(nc));
+      ph.dst.push_back(int(d.dst));
+      ph.op_off.push_back(op_total);
+      ph.src_off.push_back(src_total);
+      op_total += nc * d.rows;
+      src_total += nc;
+      ph.entries += nc * d.rows;
+      ph.n_src += nc;
+      ph.n_dst += d.rows;
+    }
+    plan.E_far += ph.entries;
+  }
+  plan.ops.resize(op_total);
+  plan.src.resize(src_total);
+
+  // fill operators
+  for (auto& pr : passes) {
+    PassHost& ph = pr.first;
+    std::vector<Dest>& ds = pr.second;
+    const int nd = int(ds.size());
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int di = 0; di < nd; ++di) {
+      Dest& d = ds[di];
+      cplx* op = plan.ops.data() + ph.op_off[di];
+      int* sp = plan.src.data() + ph.src_off[di];
+      const int nr = d.rows;
+      if (d.P) {
+        std::copy(d.P->begin(), d.P->end(), op);
+        std::copy(d.up_src.begin(), d.up_src.end(), sp);
+        continue;
+      }
+      std::vector<cplx> kb, zb;
+      for (const Contrib& c : *d.contribs) {
+        const int nc = int(c.cols->size());
+        if (int(c.rows.size()) != nr) continue;  // cannot happen: rows are the destination's ucp
+        kb.resize(size_t(nr) * nc);
+        K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());
+        if (c.Z) {
+          // op = K * Z  (nr x nc) * (nc x nc)
+          for (int q = 0; q < nc; ++q) {
+            cplx* o = op + size_t(q) * nr;
+            for (int r = 0; r < nr; ++r) o[r] = 0;
+            for (int p = 0; p < nc; ++p) {
+              const cplx z = (*c.Z)[size_t(q) * nc + p];
+              const cplx* kc = kb.data() + size_t(p) * nr;
+              for (int r = 0; r < nr; ++r) o[r] += kc[r] * z;
+            }
+          }
+          for (int q = 0; q < nc; ++q) sp[q] = int(c.src_base + q);
+        } else {
+          std::copy(kb.begin(), kb.end(), op);
+          std::copy(c.src_list->begin(), c.src_list->end(), sp);
+        }
+        op += size_t(nc) * nr;
+        sp += nc;
+      }
+    }
+    plan.passes.push_back(std::move(ph));
+  }
+
+  // near field: colleague tables of the leaf level (uniform tree)
+  const int lvl = t.nLevels;
+  const double rA = t.boxRadius[lvl];
+  plan.T.resize(size_t(9) * bs.rank * bs.rank);
+  for (int s = 0; s < 9; ++s) {
+    std::vector<cplx> Ts = near_table(bs, prob.kappa, rA, rA, kColleagueCenters[s][0] * rA, kColleagueCenters[s][1] * rA);
+    std::copy(Ts.begin(), Ts.end(), plan.T.begin() + size_t(s) * bs.rank * bs.rank);
+  }
+  plan.nbr.assign(size_t(plan.n_leaves) * 9, -1);
+  for (int li = 0; li < plan.n_leaves; ++li) {
+    const Box& b = t.box(t.leaves[li]);
+    for (int s = 0; s < 9; ++s)
+      if (b.colleague[s] >= 0) {
+        plan.nbr[size_t(li) * 9 + s] = t.lv[t.leaves[li].l][b.colleague[s]].leaf_index;
+        plan.E_near += int64_t(bs.rank) * bs.rank;
+      }
+  }
+  return "";
+}
+
+}  // namespace lsg

@@ -1,0 +1,93 @@
+// Nested cross approximation (NCA) skeletons and the fused far-field operators
+// of the DAFMM, flattened into destination-major passes for the device.
+//
+// Reference: GMRES/ACA.hpp:253-481 (ACA_only_nodes),
+//            GMRES/G_FMM2DTree.hpp:603-658 (getMatrixEntry),
+//            :2013-2385 (getNodes_LFR), :3232-3571 (getNodes_HFR),
+//            :3146-3230 (getUserCheckPoints),
+//            :2387-2500 (LFR_M2M), :3573-3827 (HFR_M2M),
+//            :2502-2651 (LFR_M2L), :3829-3942 (HFR_M2L),
+//            :2653-2841 (LFR_L2L), :3944-4236 (HFR_L2L),
+//            :2884-2960 (evaluate_NearField_Using_Precomputations_LS).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "numerics.hpp"
+#include "problem.hpp"
+#include "tables.hpp"
+#include "tree.hpp"
+
+namespace lsg {
+
+// Far-field matrix entry of the (optionally contrast-scaled) volume operator
+// between a target node i and a source node j in a different, well separated
+// leaf: the Graf series of G = (i/4) H0 against the projected source basis.
+struct EntryKernel {
+  const Tree* tree = nullptr;
+  int rank = 0;
+  double kappa = 0;
+  bool find_phi = true;
+  std::vector<double> qk2;                // kappa*kappa*q(x_i)
+  std::vector<std::vector<cplx>> M;       // per level: (2*kSeries+1) x rank row-major
+  std::vector<int> leaf_level;            // per leaf
+  std::vector<double> leaf_cx, leaf_cy;   // per leaf
+
+  // h[n+kSeries] = (J_n(R) + i Y_n(R)) e^{i n theta}, R = kappa |x_i - c_leaf|
+  void hvec(int i, int leaf, cplx* h) const;
+  cplx entry(const cplx* h, int i, int j) const;
+  // out (col-major, nr x nc) = K(rows, cols)
+  void block(const int* rows, int nr, const int* cols, int nc, cplx* out) const;
+};
+
+struct AcaResult {
+  std::vector<int> row_bases, col_bases;
+  int rank = 0;
+  std::vector<cplx> Ar;  // rank x ncols, row-major: K(row pivots, all cols)
+};
+// Partially pivoted ACA of K(rows, cols) with the reference's stopping rules.
+void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const std::vector<int>& cols, double eps,
+                    AcaResult& out);
+
+// One destination of a device pass: rows x ncol column-major operator,
+// ncol source state indices, contiguous destination [dst, dst+rows).
+struct PassHost {
+  std::string name;
+  int level = 0;
+  int kind = 0;  // 0 = store (y = A x), 1 = accumulate (y += A x)
+  std::vector<int> rows, ncol, dst;
+  std::vector<int64_t> op_off, src_off;
+  int64_t entries = 0, n_src = 0, n_dst = 0;
+};
+
+struct Plan {
+  int N = 0, rank = 0, n_leaves = 0;
+  int64_t n_state = 0;      // complex slots: [x | y | outgoing charges | incoming potentials]
+  std::vector<cplx> ops;    // operator entries of every pass
+  std::vector<int> src;     // source state indices of every pass
+  std::vector<PassHost> passes;   // execution order: upward, downward, leaf (after near field)
+  // near field (leaf level of a uniform tree): y = x + qk2 .* sum_s T_s x_nbr(s)
+  std::vector<cplx> T;      // 9 x rank x rank, row-major per slot
+  std::vector<int> nbr;     // n_leaves x 9 leaf indices, -1 where absent
+  std::vector<double> qk2;  // N
+  bool find_phi = true;
+  int64_t E_far = 0, E_near = 0;
+  int max_rank = 0;
+  std::vector<int> level_max_rank;
+  // Skeleton dump, owners in the order: level nLevels..2, LFR boxes by index
+  // then HFR (box, cone); per owner: level, box, cone (-1 = LFR), out_active,
+  // in_active, |out_check|, |out_charge|, |in_check|, |in_charge|, |ucp|, then
+  // the five lists (node ids).
+  std::vector<int> skeleton;
+  std::vector<cplx> far_leaf;  // far-field table of the leaf level (after rounding)
+};
+
+struct BuildOptions {
+  int TOL_POW = 10;
+  bool find_phi = true;
+};
+
+// Build tables, NCA skeletons and the plan. Returns "" on success.
+std::string build_plan(const Tree& tree, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan);
+
+}  // namespace lsg

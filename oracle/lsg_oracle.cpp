@@ -1,0 +1,1616 @@
+// ORACLE - test infrastructure only. CPU restatement of the reference's
+// DAFMM+NCA matvec and restarted GMRES (vaishna77/Lippmann_Schwinger_Solver,
+// GMRES/ project). Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load this library, and only as the checker or the
+// timed CPU baseline; the product library never links it.
+//
+// It keeps the reference's structure - per-box and per-cone objects holding
+// their own Ar, Atilde QR factors, M2M/M2L/L2L matrices, the QR solve inside
+// every matvec, the pass order of DFMM::MatVecProduct and modified
+// Gram-Schmidt GMRES with a column-pivoted QR least-squares solve.
+//
+// Restated files (file:line cited at each function):
+//   GMRES/GMRES.cpp, GMRES/G_FMM2DTree.hpp, GMRES/ACA.hpp,
+//   GMRES/FarFieldInteraction.hpp, GMRES/NearFieldInteraction1.hpp,
+//   GMRES/domain2D.hpp, the 10-point case of
+//   GMRES/Gauss_Legendre_Nodes_and_Weights.hpp.
+// Third-party arithmetic the reference takes from Boost.Math (cyl_bessel_j,
+// cyl_neumann; version unpinned, headers absent here) and Eigen
+// (ColPivHouseholderQR, completeOrthogonalDecomposition; version unpinned,
+// absent) is restated from published methods: Bessel functions by Zhang & Jin
+// (Computation of Special Functions, 1996, JYNB) with a power series below 1
+// and the Hankel expansion above 25; the QR by Eigen 3.3's documented
+// column-pivoting Householder algorithm; the Vandermonde pseudo-inverse by
+// its closed form (the columns are orthogonal at Chebyshev nodes).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../include/lsg.h"
+
+namespace oracle {
+
+typedef std::complex<double> cd;
+const double PI = 3.1415926535897932384;  // GMRES/ACA.hpp:14
+const cd I(0.0, 1.0);                      // GMRES/ACA.hpp:10
+const double EULER = 0.57721566490153286061;
+
+// ------------------------------------------------------------------ Bessel
+// J_0..J_nmax, Y_0..Y_nmax at x > 0.
+static void jy_all(double x, int nmax, double* J, double* Y) {
+  const int nm = nmax < 1 ? 1 : nmax;
+  std::vector<double> j(nm + 1);
+  double y0, y1;
+  if (x < 1.0) {
+    const double h = x / 2.0, hh = h * h;
+    double hp = 1.0, nf = 1.0;
+    for (int n = 0; n <= nm; ++n) {
+      if (n) { hp *= h; nf *= n; }
+      double term = hp / nf, s = term;
+      for (int k = 1; k < 40; ++k) {
+        term *= -hh / (double(k) * double(n + k));
+        s += term;
+        if (std::fabs(term) < 1e-18 * std::fabs(s)) break;
+      }
+      j[n] = s;
+    }
+    double t = 1.0, H = 0.0, s0 = 0.0;
+    for (int k = 1; k < 40; ++k) {
+      t *= -hh / (double(k) * double(k));
+      H += 1.0 / k;
+      double c = -t * H;
+      s0 += c;
+      if (std::fabs(c) < 1e-18 * std::fabs(s0)) break;
+    }
+    y0 = (2.0 / PI) * ((std::log(h) + EULER) * j[0] + s0);
+    double tk = 1.0, p1 = -EULER, p2 = 1.0 - EULER, s1 = 0.0;
+    for (int k = 0; k < 40; ++k) {
+      if (k > 0) { tk *= -hh / (double(k) * double(k + 1)); p1 += 1.0 / k; p2 += 1.0 / (k + 1); }
+      double c = (p1 + p2) * tk;
+      s1 += c;
+      if (k > 2 && std::fabs(c) < 1e-18 * std::fabs(s1)) break;
+    }
+    y1 = -2.0 / (PI * x) + (2.0 / PI) * std::log(h) * j[1] - (1.0 / PI) * h * s1;
+  } else if (x < 25.0) {
+    const double xm = std::max(x, double(nm));
+    const int M = int(xm + 30.0 + 10.0 * std::cbrt(xm));
+    double f2 = 0.0, f1 = 1e-100, f = 0.0, bs = 0.0, su = 0.0, sv = 0.0;
+    for (int k = M; k >= 0; --k) {
+      f = 2.0 * (k + 1.0) / x * f1 - f2;
+      if (k <= nm) j[k] = f;
+      if ((k & 1) == 0 && k != 0) {
+        bs += 2.0 * f;
+        su += (((k / 2) & 1) ? -1.0 : 1.0) * f / k;
+      } else if (k > 1) {
+        sv += (((k / 2) & 1) ? -1.0 : 1.0) * k / (double(k) * k - 1.0) * f;
+      }
+      f2 = f1;
+      f1 = f;
+    }
+    const double s0 = bs + f;
+    for (int k = 0; k <= nm; ++k) j[k] /= s0;
+    const double ec = std::log(x / 2.0) + EULER;
+    y0 = (2.0 / PI) * (ec * j[0] - 4.0 * su / s0);
+    y1 = (2.0 / PI) * ((ec - 1.0) * j[1] - j[0] / x - 4.0 * sv / s0);
+  } else {
+    const double sx = std::sin(x), cx = std::cos(x), pre = std::sqrt(2.0 / (PI * x));
+    const double r2 = 0.70710678118654752440;
+    double jj[2], yy[2];
+    for (int nu = 0; nu < 2; ++nu) {
+      const double mu = 4.0 * nu * nu;
+      double P = 0, Q = 0, term = 1.0;
+      for (int k = 0; k < 60; ++k) {
+        switch (k & 3) {
+          case 0: P += term; break;
+          case 1: Q += term; break;
+          case 2: P -= term; break;
+          default: Q -= term;
+        }
+        const double odd = 2.0 * k + 1.0;
+        const double nx = term * (mu - odd * odd) / ((k + 1.0) * 8.0 * x);
+        if (std::fabs(nx) < 1e-18 * std::fabs(P) || std::fabs(nx) > std::fabs(term)) break;
+        term = nx;
+      }
+      double cc, ss;
+      if (nu == 0) { cc = r2 * (cx + sx); ss = r2 * (sx - cx); }
+      else { cc = r2 * (sx - cx); ss = -r2 * (cx + sx); }
+      jj[nu] = pre * (P * cc - Q * ss);
+      yy[nu] = pre * (P * ss + Q * cc);
+    }
+    j[0] = jj[0]; j[1] = jj[1]; y0 = yy[0]; y1 = yy[1];
+    for (int n = 1; n < nm; ++n) j[n + 1] = 2.0 * n / x * j[n] - j[n - 1];
+  }
+  for (int n = 0; n <= nmax; ++n) J[n] = j[n];
+  Y[0] = y0;
+  if (nmax >= 1) Y[1] = y1;
+  for (int n = 1; n < nmax; ++n) Y[n + 1] = 2.0 * n / x * Y[n] - Y[n - 1];
+}
+
+// besselJ / besselY wrappers with the negative-order symmetry
+// (GMRES/G_FMM2DTree.hpp:73-100)
+static double besselJ(int n, double x) {
+  const int a = n < 0 ? -n : n;
+  std::vector<double> J(a + 2), Y(a + 2);
+  jy_all(x, a, J.data(), Y.data());
+  return (n < 0 && (a % 2)) ? -J[a] : J[a];
+}
+static double besselY(int n, double x) {
+  const int a = n < 0 ? -n : n;
+  std::vector<double> J(a + 2), Y(a + 2);
+  jy_all(x, a, J.data(), Y.data());
+  return (n < 0 && (a % 2)) ? -Y[a] : Y[a];
+}
+
+// ------------------------------------------------------------ dense types
+struct Mat {
+  int r = 0, c = 0;
+  std::vector<cd> a;  // column-major
+  Mat() {}
+  Mat(int r_, int c_) : r(r_), c(c_), a(size_t(r_) * c_, cd(0)) {}
+  cd& operator()(int i, int j) { return a[size_t(j) * r + i]; }
+  const cd& operator()(int i, int j) const { return a[size_t(j) * r + i]; }
+};
+typedef std::vector<cd> Vec;
+
+static Vec matvec(const Mat& A, const Vec& x) {
+  Vec y(A.r, cd(0));
+  for (int j = 0; j < A.c; ++j)
+    for (int i = 0; i < A.r; ++i) y[i] += A(i, j) * x[j];
+  return y;
+}
+
+// Eigen 3.3 ColPivHouseholderQR (factor + basic solve) restated.
+struct ColPivHouseholderQR {
+  Mat qr;
+  std::vector<cd> hcoeffs;
+  std::vector<int> perm;
+  int nonzero = 0;
+  bool valid = false;
+  void compute(const Mat& A) {
+    qr = A;
+    const int rows = A.r, cols = A.c, size = std::min(rows, cols);
+    hcoeffs.assign(size, cd(0));
+    std::vector<int> transp(cols);
+    std::vector<double> upd(cols), dir(cols);
+    for (int k = 0; k < cols; ++k) {
+      double s = 0;
+      for (int i = 0; i < rows; ++i) s += std::norm(qr(i, k));
+      dir[k] = std::sqrt(s);
+      upd[k] = dir[k];
+    }
+    double mx = 0;
+    for (int k = 0; k < cols; ++k) mx = std::max(mx, upd[k]);
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double thr = (mx * eps) * (mx * eps) / double(rows);
+    const double dthr = std::sqrt(eps);
+    nonzero = size;
+    for (int k = 0; k < size; ++k) {
+      int big = k;
+      for (int j = k + 1; j < cols; ++j)
+        if (upd[j] > upd[big]) big = j;
+      if (nonzero == size && upd[big] * upd[big] < thr * double(rows - k)) nonzero = k;
+      transp[k] = big;
+      if (big != k) {
+        for (int i = 0; i < rows; ++i) std::swap(qr(i, k), qr(i, big));
+        std::swap(upd[k], upd[big]);
+        std::swap(dir[k], dir[big]);
+      }
+      // makeHouseholderInPlace
+      const cd c0 = qr(k, k);
+      double tail = 0;
+      for (int i = k + 1; i < rows; ++i) tail += std::norm(qr(i, k));
+      const double tol = std::numeric_limits<double>::min();
+      double beta;
+      cd tau;
+      if (tail <= tol && std::imag(c0) * std::imag(c0) <= tol) {
+        tau = 0;
+        beta = std::real(c0);
+        for (int i = k + 1; i < rows; ++i) qr(i, k) = 0;
+      } else {
+        beta = std::sqrt(std::norm(c0) + tail);
+        if (std::real(c0) >= 0) beta = -beta;
+        for (int i = k + 1; i < rows; ++i) qr(i, k) /= (c0 - beta);
+        tau = std::conj((beta - c0) / beta);
+      }
+      hcoeffs[k] = tau;
+      qr(k, k) = beta;
+      // applyHouseholderOnTheLeft on the trailing block
+      if (k + 1 < cols) {
+        if (rows - k == 1) {
+          for (int j = k + 1; j < cols; ++j) qr(k, j) *= (1.0 - tau);
+        } else if (tau != cd(0)) {
+          for (int j = k + 1; j < cols; ++j) {
+            cd t = qr(k, j);
+            for (int i = k + 1; i < rows; ++i) t += std::conj(qr(i, k)) * qr(i, j);
+            qr(k, j) -= tau * t;
+            for (int i = k + 1; i < rows; ++i) qr(i, j) -= tau * qr(i, k) * t;
+          }
+        }
+      }
+      for (int j = k + 1; j < cols; ++j) {
+        if (upd[j] != 0.0) {
+          double t = std::abs(qr(k, j)) / upd[j];
+          t = (1.0 + t) * (1.0 - t);
+          if (t < 0) t = 0;
+          const double t2 = t * (upd[j] / dir[j]) * (upd[j] / dir[j]);
+          if (t2 <= dthr) {
+            double s = 0;
+            for (int i = k + 1; i < rows; ++i) s += std::norm(qr(i, j));
+            dir[j] = std::sqrt(s);
+            upd[j] = dir[j];
+          } else {
+            upd[j] *= std::sqrt(t);
+          }
+        }
+      }
+    }
+    perm.resize(cols);
+    for (int i = 0; i < cols; ++i) perm[i] = i;
+    for (int k = 0; k < size; ++k) std::swap(perm[k], perm[transp[k]]);
+    valid = true;
+  }
+  Vec solve(const Vec& b) const {
+    const int rows = qr.r, cols = qr.c;
+    Vec x(cols, cd(0));
+    if (nonzero == 0) return x;
+    Vec c = b;
+    for (int k = 0; k < nonzero; ++k) {
+      // Eigen's householderQ() holds the conjugated coefficients, so its
+      // adjoint applies H_k = I - hcoeffs[k] v v^H (the reflector that made R).
+      const cd t = hcoeffs[k];
+      if (rows - k == 1) { c[k] *= (1.0 - t); continue; }
+      if (t == cd(0)) continue;
+      cd s = c[k];
+      for (int i = k + 1; i < rows; ++i) s += std::conj(qr(i, k)) * c[i];
+      c[k] -= t * s;
+      for (int i = k + 1; i < rows; ++i) c[i] -= t * qr(i, k) * s;
+    }
+    for (int i = nonzero - 1; i >= 0; --i) {
+      cd s = c[i];
+      for (int j = i + 1; j < nonzero; ++j) s -= qr(i, j) * c[j];
+      c[i] = s / qr(i, i);
+    }
+    for (int i = 0; i < nonzero; ++i) x[perm[i]] = c[i];
+    return x;
+  }
+};
+
+// --------------------------------------------------------- quadrature
+// Gauss-Legendre nodes/weights by Newton iteration on P_n (long double).
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    long double z = std::cos(PI * (i + 0.75L) / (n + 0.5L)), dp = 0;
+    for (int it = 0; it < 100; ++it) {
+      long double p0 = 1, p1 = z;
+      for (int k = 2; k <= n; ++k) { long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+      dp = n * (z * p1 - p0) / (z * z - 1);
+      long double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs((double)dz) < 1e-19) break;
+    }
+    long double p0 = 1, p1 = z;
+    for (int k = 2; k <= n; ++k) { long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+    dp = n * (z * p1 - p0) / (z * z - 1);
+    x[n - 1 - i] = (double)z;
+    w[n - 1 - i] = (double)(2.0L / ((1 - z * z) * dp * dp));
+  }
+}
+
+// Adaptive quadtree Gauss-Legendre (GMRES/domain2D.hpp:38-138)
+template <class F>
+struct Quadrature {
+  const F& f;
+  std::vector<double> nodes, weights;
+  double epsilon;
+  explicit Quadrature(const F& fn, double eps) : f(fn), epsilon(eps) { gauss_legendre(10, nodes, weights); }
+  cd panel(double xc, double yc, double xr, double yr) const {  // panel2DIntegral
+    cd integral(0.0, 0.0);
+    const int n = int(nodes.size());
+    for (int m = 0; m < n; ++m)
+      for (int q = 0; q < n; ++q) integral += weights[m] * weights[q] * f(xc + xr * nodes[m], yc + yr * nodes[q]);
+    integral *= (xr * yr);
+    return integral;
+  }
+  void create_tree(double xc, double yc, double xr, double yr, cd self, std::vector<cd>& leaves) const {
+    const double hx = 0.5 * xr, hy = 0.5 * yr;
+    cd ch[4] = {panel(xc - hx, yc - hy, hx, hy), panel(xc + hx, yc - hy, hx, hy), panel(xc + hx, yc + hy, hx, hy),
+                panel(xc - hx, yc + hy, hx, hy)};
+    cd err = self;
+    for (int l = 0; l < 4; ++l) err -= ch[l];
+    if (std::abs(err) < epsilon) {
+      leaves.push_back(self);
+    } else {
+      create_tree(xc - hx, yc - hy, hx, hy, ch[0], leaves);
+      create_tree(xc + hx, yc - hy, hx, hy, ch[1], leaves);
+      create_tree(xc + hx, yc + hy, hx, hy, ch[2], leaves);
+      create_tree(xc - hx, yc + hy, hx, hy, ch[3], leaves);
+    }
+  }
+  cd obtain() const {  // obtain_Quadrature over [-1,1]^2
+    cd root = panel(0.0, 0.0, 1.0, 1.0);
+    std::vector<cd> leaves;
+    create_tree(0.0, 0.0, 1.0, 1.0, root, leaves);
+    cd integral(0.0, 0.0);
+    for (const cd& v : leaves) integral += v;
+    return integral;
+  }
+};
+
+static double get_ChebPoly(double x, int n) { return std::cos(n * std::acos(x)); }
+
+static double round6(double v) {  // Eigen/iostream text round trip, precision 6
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.6g", v);
+  return std::strtod(buf, nullptr);
+}
+
+// --------------------------------------------------- problem definition
+struct Problem {
+  double kappa;
+  double angle;
+  lsg_contrast con;
+  double ContrastFunction(double x, double y) const {  // GMRES/GMRES.cpp:5-117
+    switch (con.kind) {
+      case 0: { double R2 = x * x + y * y; return 1.5 * std::exp(-160.0 * R2); }
+      case 1: {
+        static const double cx[20] = {-1.17182014706864, -0.861066625441127, -1.14533939124057, -0.856639124568525,
+                                      -0.620066350133170, -0.457064031294315, -0.482782006538245, -0.635070123186999,
+                                      0.108594058866164, -0.0315722590552472, -0.177274727610547, 0.0342987980936911,
+                                      0.369662097773613, 0.591444264226524, 0.401225715100565, 0.666822699314234,
+                                      1.10327781188617, 0.827519268441487, 0.873411278211136, 1.09482906183464};
+        static const double cy[20] = {-0.361066625441127, 0.0719520938231629, 0.354660608759432, 1.14336087543147,
+                                      -1.12006635013317, -0.457064031294315, 0.364929876813001, 0.802261227250738,
+                                      -0.394084977497864, -0.0315722590552472, 0.322725272389453, 1.03429879809369,
+                                      -1.13033790222639, 0.0137164199764117, 0.401225715100565, 1.16682269931423,
+                                      -0.345187707108795, -0.172480731558513, 0.373411278211136, 1.09482906183464};
+        double q = 0.0, a = 0.0013;
+        for (int i = 0; i < 20; ++i) {
+          double R2 = (x - cx[i]) * (x - cx[i]) + (y - cy[i]) * (y - cy[i]);
+          q += 1.5 * std::exp(-R2 / a);
+        }
+        return q;
+      }
+      case 2: {
+        double C = 0.4987;
+        double psi = 1.0 - (x - 0.15 * (1 - x * x)) * (x - 0.15 * (1 - x * x)) - C * (1.0 + 0.3 * x) * (1.0 + 0.3 * x) * y * y;
+        double a[5] = {0.45, 0.195, 0.51, 0.195, 0.63}, xx[5] = {0.8, 0.54, -0.14, -0.5, 0.18},
+               yy[5] = {0.0, -0.28, 0.7, -0.01, 0.8};
+        if (psi <= 0.05) return 0.0;
+        double g = 0.0;
+        for (int i = 0; i < 5; ++i) {
+          double t = (x - xx[i]) * (x - xx[i]) + (y - yy[i]) * (y - yy[i]);
+          g += a[i] * std::exp(-t / 0.01);
+        }
+        return -1.5 * (psi - 0.05) - g * std::cos(0.9 * y);
+      }
+      case 3: { double R2 = x * x + y * y; return 0.5 * std::erfc(5.0 * (R2 - 1.0)); }
+      case 4: {
+        double R = std::sqrt(x * x + y * y), th = std::atan2(y, x);
+        return (1.0 - std::pow(std::sin(0.5 * th), 500)) * std::exp(-2000 * (0.1 - R * R) * (0.1 - R * R));
+      }
+      case 5: { double R = std::sqrt(x * x + y * y); return 4 * (y - 0.1) * (1.0 - std::erf(25.0 * (R - 0.3))); }
+      default: {
+        double s = 0;
+        for (int i = 0; i < con.n_bumps; ++i) {
+          double dx = x - con.cx[i], dy = y - con.cy[i], w = con.width[i];
+          s += con.amp[i] * std::exp(-(dx * dx + dy * dy) / (w * w));
+        }
+        return s;
+      }
+    }
+  }
+  cd IncidenceFunction(double x, double y) const {
+    double ph = (angle == 0.0) ? kappa * x : kappa * (x * std::cos(angle) + y * std::sin(angle));
+    return std::exp(I * ph);
+  }
+  cd RHSFunction(double x, double y) const { return -1.0 * kappa * kappa * ContrastFunction(x, y) * IncidenceFunction(x, y); }
+};
+
+// ------------------------------------------------------------------ tree
+struct orderedPair { int x, y; };
+
+struct FMM2DCone {  // GMRES/G_FMM2DTree.hpp:119-152
+  bool outgoing_ILActive = false, incoming_ILActive = false;
+  Vec outgoing_charges, outgoing_potential, incoming_charges, incoming_potential;
+  std::vector<int> incoming_chargePoints, incoming_checkPoints, outgoing_chargePoints, outgoing_checkPoints,
+      user_checkPoints;
+  ColPivHouseholderQR outgoing_Atilde_dec, incoming_Atilde_dec;
+  std::vector<orderedPair> InteractionList;
+  Mat outgoing_Ar, L2L[4], M2M;
+  std::vector<Mat> M2L;
+};
+
+struct FMM2DBox {  // GMRES/G_FMM2DTree.hpp:154-227
+  double radius = 0;
+  bool outgoing_ILActive = false, incoming_ILActive = false;
+  int level = 0, boxNumber = -1, parentNumber = -1, childrenNumbers[4] = {-1, -1, -1, -1};
+  int child[4] = {-1, -1, -1, -1};  // index of each child in tree[level+1]
+  bool isLeaf = false;
+  std::vector<orderedPair> neighborNumbers, InteractionList;
+  int colleagueNeighbors[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+  Vec outgoing_charges, outgoing_potential, incoming_charges, incoming_potential, multipoles;
+  std::vector<int> incoming_chargePoints, incoming_checkPoints, user_checkPoints, outgoing_chargePoints,
+      outgoing_checkPoints, chargeLocations;
+  ColPivHouseholderQR outgoing_Atilde_dec, incoming_Atilde_dec;
+  Mat outgoing_Ar, L2L;
+  std::vector<Mat> M2L;
+  double cx = 0, cy = 0;
+  std::vector<FMM2DCone> ConeTree;
+};
+
+const double N_centers[9][2] = {{-2, -2}, {0, -2}, {2, -2}, {2, 0}, {2, 2}, {0, 2}, {-2, 2}, {-2, 0}, {0, 0}};
+const double epsilonRoundOff = 1e-8;
+
+struct FMM2DTree {
+  Problem Q;
+  bool findPhi = true;
+  int nLevels = 0, nChebNodes = 0, rank = 0, N = 0, level_LFR = 2, nCones_LFR = 16, TOL_POW = 10, degreeOfBases = 6,
+      nLevelsUniform = 4, yes2DFMM = 1;
+  double L = 0.5, kappa = 1.0;
+  std::vector<double> boxRadius, ConeAperture;
+  std::vector<int> nCones;
+  std::vector<std::vector<FMM2DBox>> tree;
+  std::vector<orderedPair> leafNodes;
+  std::vector<double> standardChebNodes1D;
+  std::vector<double> scx, scy;  // standardChebNodes
+  std::vector<double> gpx, gpy;  // gridPoints / particles_X
+  std::vector<std::pair<int, int>> degreeOfBasesVec;
+  std::vector<double> Q_pinv;        // NumberOfBases x rank, row-major
+  std::vector<Mat> M;                // per level (only leaf levels filled)
+  Mat colleagueNeighborInteraction[9];  // leaf level (uniform tree)
+  std::vector<double> contrastK2;    // kappa*kappa*q at nodes
+
+  // ---------------------------------------------------------- setup
+  std::string setup(const lsg_inputs& in) {
+    nCones_LFR = in.nCones_LFR; nChebNodes = in.nChebNodes; L = in.L; kappa = in.kappa; yes2DFMM = in.yes2DFMM;
+    degreeOfBases = in.degreeOfBases; TOL_POW = in.TOL_POW; nLevelsUniform = in.nLevelsUniform;
+    findPhi = in.find_phi != 0;
+    Q.kappa = kappa; Q.angle = in.inc_angle; Q.con = in.contrast;
+    rank = nChebNodes * nChebNodes;
+    boxRadius.assign(1, L);
+    // FMM2DTree ctor (G_FMM2DTree.hpp:316-326)
+    if (yes2DFMM == 1) {
+      level_LFR = (kappa == 0) ? 2 : int(std::floor(std::log(kappa * L / 0.35) / std::log(2.0)));
+      if (level_LFR < 2) level_LFR = 2;
+    } else {
+      level_LFR = 2;
+    }
+    set_Standard_Cheb_Nodes();
+    createUniformTree();
+    if (nLevels < 2) return "tree too shallow";
+    createCones();
+    assign_Tree_Interactions();
+    assignLeafChargeLocations();
+    evaluateQ();
+    getNeighbors();
+    for (int i = 0; i < N; ++i) contrastK2.push_back(kappa * kappa * Q.ContrastFunction(gpx[i], gpy[i]));
+    // tables of the leaf level, with the 6-digit text round trip
+    M.assign(nLevels + 1, Mat());
+    M[nLevels] = evaluateM(nLevels);
+    for (int n = 0; n < 9; ++n)
+      colleagueNeighborInteraction[n] = getOperator(nLevels, nLevels, N_centers[n][0] * boxRadius[nLevels],
+                                                    N_centers[n][1] * boxRadius[nLevels]);
+    getNodes_LFR();
+    getNodes_HFR();
+    getUserCheckPoints();
+    Assemble_HFR_M2M();
+    Assemble_HFR_M2L();
+    Assemble_LFR_M2L();
+    Assemble_LFR_L2L();
+    Assemble_HFR_L2L();
+    return "";
+  }
+
+  void set_Standard_Cheb_Nodes() {  // :683-694
+    for (int k = 0; k < nChebNodes; ++k) standardChebNodes1D.push_back(-std::cos((k + 0.5) / nChebNodes * PI));
+    for (int j = 0; j < nChebNodes; ++j)
+      for (int k = 0; k < nChebNodes; ++k) { scx.push_back(standardChebNodes1D[k]); scy.push_back(standardChebNodes1D[j]); }
+    for (int d = 0; d < degreeOfBases; ++d)
+      for (int a = 0; a < d + 1; ++a) degreeOfBasesVec.push_back({a, d - a});
+  }
+
+  void createTreeU(int j, int k) {  // :833-889
+    bool condition3Leaf = kappa * tree[j][k].radius <= 2 * PI * 3.0;
+    if (condition3Leaf && j == nLevelsUniform) {
+      tree[j][k].isLeaf = true;
+      leafNodes.push_back({j, k});
+      return;
+    }
+    if (j > nLevelsUniform) return;  // the reference recurses without end here
+    if (int(tree.size()) == j + 1) tree.emplace_back();
+    int n = int(tree[j + 1].size());
+    int b = tree[j][k].boxNumber;
+    for (int c = 0; c < 4; ++c) {
+      FMM2DBox box;
+      box.level = j + 1;
+      box.boxNumber = b * 4 + c;
+      box.parentNumber = b;
+      box.radius = 0.5 * tree[j][k].radius;
+      const double r = tree[j][k].radius;
+      if (c == 0) { box.cx = tree[j][k].cx - 0.5 * r; box.cy = tree[j][k].cy - 0.5 * r; }
+      else if (c == 1) { box.cx = tree[j][k].cx + 0.5 * r; box.cy = tree[j][k].cy - 0.5 * r; }
+      else if (c == 2) { box.cx = tree[j][k].cx + 0.5 * r; box.cy = tree[j][k].cy + 0.5 * r; }
+      else { box.cx = tree[j][k].cx - 0.5 * r; box.cy = tree[j][k].cy + 0.5 * r; }
+      tree[j][k].child[c] = n + c;
+      tree[j + 1].push_back(box);
+    }
+    for (int c = 0; c < 4; ++c) createTreeU(j + 1, n + c);
+  }
+  void createUniformTree() {  // :917-941
+    FMM2DBox root;
+    root.level = 0; root.boxNumber = 0; root.radius = L;
+    tree.assign(1, std::vector<FMM2DBox>(1, root));
+    createTreeU(0, 0);
+    nLevels = int(tree.size()) - 1;
+    for (int j = 1; j <= std::max(10, nLevels + 1); ++j) boxRadius.push_back(boxRadius[j - 1] / 2.0);
+    if (level_LFR >= nLevels) level_LFR = nLevels - 1;
+  }
+  void createCones() {  // :1952-1974
+    N = int(leafNodes.size()) * rank;
+    nCones.push_back(int(nCones_LFR * std::pow(2.0, level_LFR - 1)));
+    ConeAperture.push_back(2 * PI / nCones[0]);
+    for (int j = 1; j <= level_LFR - 1; ++j) {
+      nCones.push_back(int(nCones[j - 1] / 2.0));
+      ConeAperture.push_back(ConeAperture[j - 1] * 2.0);
+    }
+    for (int j = 0; j <= level_LFR - 1; ++j)
+      for (auto& b : tree[j])
+        if (!b.isLeaf) b.ConeTree.assign(nCones[j], FMM2DCone());
+  }
+  bool farLFR(const FMM2DBox& A, int ja, const FMM2DBox& B, int jb) const {
+    return std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[ja] + boxRadius[jb] - epsilonRoundOff ||
+           std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[ja] + boxRadius[jb] - epsilonRoundOff;
+  }
+  void getFutureGeneration_LFR(int bj, int bk, int pnj, int pni, std::vector<orderedPair>& fg) {  // :943-964
+    if (tree[pnj][pni].isLeaf) {
+      if (!farLFR(tree[bj][bk], bj, tree[pnj][pni], pnj)) fg.push_back({pnj, pni});
+    } else {
+      for (int nc = 0; nc < 4; ++nc) getFutureGeneration_LFR(bj, bk, pnj + 1, tree[pnj][pni].child[nc], fg);
+    }
+  }
+  void assign_Child_Interaction(int c, int j, int k) {  // :1356-1461
+    const int boxA_index = tree[j][k].child[c];
+    FMM2DBox& A = tree[j + 1][boxA_index];
+    for (size_t n = 0; n < tree[j][k].neighborNumbers.size(); ++n) {
+      const int pnj = tree[j][k].neighborNumbers[n].x, pni = tree[j][k].neighborNumbers[n].y;
+      if (j + 1 >= level_LFR || (j + 1 < level_LFR && A.isLeaf)) {
+        if (tree[pnj][pni].isLeaf) {
+          if (farLFR(A, j + 1, tree[pnj][pni], pnj)) A.InteractionList.push_back({pnj, pni});
+          else A.neighborNumbers.push_back({pnj, pni});
+        } else {
+          for (int nc = 0; nc < 4; ++nc) {
+            const int boxB_index = tree[pnj][pni].child[nc];
+            if (farLFR(A, j + 1, tree[pnj + 1][boxB_index], pnj + 1)) {
+              A.InteractionList.push_back({pnj + 1, boxB_index});
+            } else if (!A.isLeaf) {
+              A.neighborNumbers.push_back({pnj + 1, boxB_index});
+            } else {
+              std::vector<orderedPair> fg;
+              getFutureGeneration_LFR(j + 1, boxA_index, pnj + 1, boxB_index, fg);
+              for (auto& f : fg) A.neighborNumbers.push_back(f);
+            }
+          }
+        }
+      } else {
+        const double r = boxRadius[j + 1];
+        if (tree[pnj][pni].isLeaf) {
+          const FMM2DBox& P = tree[pnj][pni];
+          if (std::fabs(A.cx - P.cx) >= kappa * r * r + r + boxRadius[pnj] - epsilonRoundOff ||
+              std::fabs(A.cy - P.cy) >= kappa * r * r + r + boxRadius[pnj] - epsilonRoundOff) {
+            double arg = std::atan2(A.cy - P.cy, A.cx - P.cx);
+            arg = std::fmod(arg + 2 * PI + PI, 2 * PI);
+            A.ConeTree[int(arg / ConeAperture[j + 1])].InteractionList.push_back({pnj, pni});
+          } else {
+            A.neighborNumbers.push_back({pnj, pni});
+          }
+        } else {
+          for (int nc = 0; nc < 4; ++nc) {
+            const int boxB_index = tree[pnj][pni].child[nc];
+            const FMM2DBox& B = tree[pnj + 1][boxB_index];
+            if (std::fabs(A.cx - B.cx) >= kappa * r * r + r + boxRadius[pnj + 1] - epsilonRoundOff ||
+                std::fabs(A.cy - B.cy) >= kappa * r * r + boxRadius[pnj + 1] - epsilonRoundOff) {
+              double arg = std::atan2(A.cy - B.cy, A.cx - B.cx);
+              arg = std::fmod(arg + 2 * PI + PI, 2 * PI);
+              A.ConeTree[int(arg / ConeAperture[j + 1])].InteractionList.push_back({pnj + 1, boxB_index});
+            } else {
+              A.neighborNumbers.push_back({pnj + 1, boxB_index});
+            }
+          }
+        }
+      }
+    }
+  }
+  void assign_Tree_Interactions() {  // :1482-1498
+    for (int c = 0; c < 4; ++c)
+      for (int n = 0; n < 4; ++n) tree[1][c].neighborNumbers.push_back({1, n});
+    for (int j = 1; j <= nLevels - 1; ++j)
+      for (int k = 0; k < int(tree[j].size()); ++k)
+        if (!tree[j][k].isLeaf)
+          for (int c = 0; c < 4; ++c) assign_Child_Interaction(c, j, k);
+  }
+  void assignLeafChargeLocations() {  // :1150-1163
+    for (auto& lf : leafNodes) {
+      FMM2DBox& b = tree[lf.x][lf.y];
+      int start = int(gpx.size());
+      for (int i = 0; i < rank; ++i) b.chargeLocations.push_back(start + i);
+      for (int i = 0; i < rank; ++i) {
+        gpx.push_back(boxRadius[lf.x] * scx[i] + b.cx);
+        gpy.push_back(boxRadius[lf.x] * scy[i] + b.cy);
+      }
+    }
+  }
+  void getNeighbors() {  // colleague slots, :1567-1601
+    double N_angles[9];
+    for (int i = 0; i < 9; ++i) N_angles[i] = std::atan2(N_centers[i][1], N_centers[i][0]);
+    for (int j = 1; j <= nLevels; ++j)
+      for (auto& A : tree[j])
+        for (auto& nb : A.neighborNumbers) {
+          const FMM2DBox& B = tree[nb.x][nb.y];
+          double angle = std::atan2(B.cy - A.cy, B.cx - A.cx);
+          if (A.radius == B.radius) {
+            if (std::fabs(B.cy - A.cy) < epsilonRoundOff && std::fabs(B.cx - A.cx) < epsilonRoundOff)
+              A.colleagueNeighbors[8] = nb.y;
+            else
+              for (int i = 0; i < 8; ++i)
+                if (std::fabs(N_angles[i] - angle) < epsilonRoundOff) { A.colleagueNeighbors[i] = nb.y; break; }
+          }
+        }
+  }
+
+  // ------------------------------------------------------- tables
+  // FarFieldInteraction.hpp:138-150. The columns T_a(x)T_b(y), a,b < n, are
+  // orthogonal at the tensor Chebyshev nodes, so pinv(Q) = diag(1/d) Q^T with
+  // d_l = sum_i Q(i,l)^2 = n^2 (a>0 ? 1/2 : 1)(b>0 ? 1/2 : 1).
+  void evaluateQ() {
+    const int nb = int(degreeOfBasesVec.size());
+    Q_pinv.assign(size_t(nb) * rank, 0.0);
+    for (int l = 0; l < nb; ++l) {
+      const int a = degreeOfBasesVec[l].first, b = degreeOfBasesVec[l].second;
+      const double d = double(nChebNodes) * nChebNodes * (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0);
+      for (int i = 0; i < rank; ++i) Q_pinv[size_t(l) * rank + i] = get_ChebPoly(scx[i], a) * get_ChebPoly(scy[i], b) / d;
+    }
+  }
+  Mat evaluateM(int level) {  // FarFieldInteraction.hpp:62-74, 115-160, then 6-digit round trip
+    const int nb = int(degreeOfBasesVec.size()), sl = 15;
+    Mat C(2 * sl + 1, nb);
+    const double r = boxRadius[level];
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < (2 * sl + 1) * nb; ++t) {
+      const int n = t / nb - sl, l = t % nb;
+      const int a = degreeOfBasesVec[l].first, b = degreeOfBasesVec[l].second;
+      auto f = [&](double px, double py) -> cd {
+        double normX = std::sqrt(px * px + py * py);
+        double argumentJ = kappa * r * normX;
+        double theta_y = std::atan2(py, px);
+        return besselJ(n, argumentJ) * std::exp(-I * double(n) * theta_y) * get_ChebPoly(px, a) * get_ChebPoly(py, b);
+      };
+      Quadrature<decltype(f)> q(f, 1e-7);
+      C(n + sl, l) = r * r * q.obtain();
+    }
+    Mat Mm(2 * sl + 1, rank);
+    for (int i = 0; i < 2 * sl + 1; ++i)
+      for (int j = 0; j < rank; ++j) {
+        cd s(0);
+        for (int l = 0; l < nb; ++l) s += C(i, l) * Q_pinv[size_t(l) * rank + j];
+        Mm(i, j) = cd(round6(s.real()), round6(s.imag()));
+      }
+    return Mm;
+  }
+  // getOperator (G_FMM2DTree.hpp:1773-1805) for a source box of the same level
+  Mat getOperator(int level_A, int level_B, double bx, double by) {
+    const int nb = int(degreeOfBasesVec.size());
+    const double rA = boxRadius[level_A], rB = boxRadius[level_B];
+    Mat V1(rank, nb);
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < rank * nb; ++t) {
+      const int i = t / nb, l = t % nb;
+      const double Xx = (rA * scx[i] + 0.0 - bx) / rB, Xy = (rA * scy[i] + 0.0 - by) / rB;
+      const int a = degreeOfBasesVec[l].first, b = degreeOfBasesVec[l].second;
+      auto f = [&](double px, double py) -> cd {
+        double R = std::sqrt((Xx - px) * (Xx - px) + (Xy - py) * (Xy - py));
+        double arg = kappa * rB * R;
+        return (besselJ(0, arg) + I * besselY(0, arg)) * get_ChebPoly(px, a) * get_ChebPoly(py, b);
+      };
+      Quadrature<decltype(f)> q(f, 1e-7);
+      V1(i, l) = q.obtain();
+    }
+    Mat T(rank, rank);
+    for (int i = 0; i < rank; ++i)
+      for (int j = 0; j < rank; ++j) {
+        cd s(0);
+        for (int l = 0; l < nb; ++l) s += (V1(i, l) * I * rB * rB / 4.0) * Q_pinv[size_t(l) * rank + j];
+        T(i, j) = cd(round6(s.real()), round6(s.imag()));
+      }
+    return T;
+  }
+
+  // ------------------------------------------------- matrix entries
+  int leafOf(int j) const { return j / rank; }
+  // getMatrixEntry (G_FMM2DTree.hpp:603-658)
+  cd getMatrixEntry(int i, int j) const {
+    const int l = j / rank, chargeIndex = j % rank;
+    const int tj = leafNodes[l].x, tk = leafNodes[l].y;
+    const FMM2DBox& B = tree[tj][tk];
+    const double polar_angle_i = std::atan2(gpy[i] - B.cy, gpx[i] - B.cx);
+    const double R2 = (gpy[i] - B.cy) * (gpy[i] - B.cy) + (gpx[i] - B.cx) * (gpx[i] - B.cx);
+    const double R = kappa * std::sqrt(R2);
+    // J_n, Y_n of one argument computed once for all orders
+    double J[16], Y[16];
+    jy_all(R, 15, J, Y);
+    cd tempSum(0.0, 0.0);
+    for (int n = -15; n <= 15; ++n) {
+      const int a = n < 0 ? -n : n;
+      double jn = J[a], yn = Y[a];
+      if (n < 0 && (a % 2)) { jn = -jn; yn = -yn; }
+      cd temp = (jn + I * yn) * std::exp(I * double(n) * polar_angle_i);
+      tempSum += M[tj](n + 15, chargeIndex) * temp;
+    }
+    tempSum *= I / 4.0;
+    if (findPhi) {
+      tempSum *= contrastK2[i];
+      if (i == j) tempSum += 1.0;
+    }
+    return tempSum;
+  }
+  // getMatrixEntry2 (G_FMM2DTree.hpp:518-601): near-field table if the leaves
+  // touch (colleagues in a uniform tree), else the far-field entry.
+  cd getMatrixEntry2(int i, int j) const {
+    const int li = i / rank, lj = j / rank;
+    const FMM2DBox& A = tree[leafNodes[li].x][leafNodes[li].y];
+    const FMM2DBox& B = tree[leafNodes[lj].x][leafNodes[lj].y];
+    const int lev = leafNodes[li].x;
+    const bool touch = !(std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff ||
+                         std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff);
+    if (touch) {
+      int slot = -1;
+      for (int s = 0; s < 9; ++s)
+        if (A.colleagueNeighbors[s] == leafNodes[lj].y) slot = s;
+      cd entry = colleagueNeighborInteraction[slot](i % rank, j % rank);
+      if (findPhi) {
+        entry = entry * contrastK2[i];
+        if (i == j) entry = 1.0 + entry;
+      }
+      return entry;
+    }
+    return getMatrixEntry(i, j);
+  }
+  // getMatrixEntry2(i, lj*rank + q) for q = 0..rank-1
+  void denseRowLeaf(int i, int lj, cd* out) const {
+    const int li = i / rank;
+    const FMM2DBox& A = tree[leafNodes[li].x][leafNodes[li].y];
+    const FMM2DBox& B = tree[leafNodes[lj].x][leafNodes[lj].y];
+    const int lev = leafNodes[li].x;
+    const bool touch = !(std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff ||
+                         std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff);
+    if (touch) {
+      for (int q = 0; q < rank; ++q) out[q] = getMatrixEntry2(i, lj * rank + q);
+      return;
+    }
+    const double th = std::atan2(gpy[i] - B.cy, gpx[i] - B.cx);
+    const double R = kappa * std::sqrt((gpy[i] - B.cy) * (gpy[i] - B.cy) + (gpx[i] - B.cx) * (gpx[i] - B.cx));
+    double J[16], Y[16];
+    jy_all(R, 15, J, Y);
+    cd h[31];
+    for (int n = -15; n <= 15; ++n) {
+      const int a = n < 0 ? -n : n;
+      double jn = J[a], yn = Y[a];
+      if (n < 0 && (a % 2)) { jn = -jn; yn = -yn; }
+      h[n + 15] = (jn + I * yn) * std::exp(I * double(n) * th);
+    }
+    const Mat& Mt = M[leafNodes[lj].x];
+    for (int q = 0; q < rank; ++q) {
+      cd t(0.0, 0.0);
+      for (int n = 0; n < 31; ++n) t += Mt(n, q) * h[n];
+      t *= I / 4.0;
+      if (findPhi) {
+        t *= contrastK2[i];
+        if (i == lj * rank + q) t += 1.0;
+      }
+      out[q] = t;
+    }
+  }
+  Vec getRow(int j, const std::vector<int>& col_indices) const {
+    Vec row(col_indices.size());
+    for (size_t k = 0; k < col_indices.size(); ++k) row[k] = getMatrixEntry(j, col_indices[k]);
+    return row;
+  }
+  Vec getCol(int k, const std::vector<int>& row_indices) const {
+    Vec col(row_indices.size());
+    for (size_t j = 0; j < row_indices.size(); ++j) col[j] = getMatrixEntry(row_indices[j], k);
+    return col;
+  }
+  Mat getMatrix(const std::vector<int>& r, const std::vector<int>& c) const {
+    Mat m(int(r.size()), int(c.size()));
+    for (size_t j = 0; j < r.size(); ++j)
+      for (size_t k = 0; k < c.size(); ++k) m(int(j), int(k)) = getMatrixEntry(r[j], c[k]);
+    return m;
+  }
+
+  // -------------------------------------------------------------- ACA
+  static int maxAbsVector(const Vec& v, const std::set<int>& allowed) {  // ACA.hpp:153-166
+    if (allowed.empty()) return -1;
+    int index = *allowed.begin();
+    double max = std::abs(v[index]);
+    for (int it : allowed)
+      if (std::abs(v[it]) > max) { index = it; max = std::abs(v[index]); }
+    return index;
+  }
+  static double norm(const Vec& v) { double s = 0; for (auto& z : v) s += std::norm(z); return std::sqrt(s); }
+  static cd dot(const Vec& a, const Vec& b) { cd s(0); for (size_t i = 0; i < a.size(); ++i) s += std::conj(a[i]) * b[i]; return s; }
+
+  // ACA_only_nodes (ACA.hpp:253-481); Ar = K(row pivots, all columns)
+  void ACA_only_nodes(const std::vector<int>& row_indices, const std::vector<int>& col_indices,
+                      std::vector<int>& row_bases, std::vector<int>& col_bases, int& computed_rank, double epsilon,
+                      Mat* Ar) const {
+    int col_index = 0, row_index = 0;
+    const int N1 = int(row_indices.size()), N2 = int(col_indices.size());
+    Vec row(N2), col(N1), v, u, row_temp, col_temp;
+    std::vector<Vec> Uvec, Vvec, Arows;
+    computed_rank = 0;
+    const int mn = std::min(N1, N2);
+    std::set<int> rr, rc;
+    for (int l = 0; l < N1; ++l) rr.insert(l);
+    for (int l = 0; l < N2; ++l) rc.insert(l);
+    if (N1 < N2) {
+      int l_local;
+      for (l_local = 0; l_local < N1; ++l_local) {
+        row_index = l_local;
+        row = getRow(row_indices[row_index], col_indices);
+        col_index = maxAbsVector(row, rc);
+        if (std::abs(row[col_index]) > epsilon * std::pow(10, -1.0 * 0)) break;
+      }
+      if (l_local == N1) { if (Ar) *Ar = Mat(0, N2); return; }
+      v.assign(N2, 0);
+      for (int l = 0; l < N2; ++l) v[l] = row[l] / row[col_index];
+      row_bases.push_back(row_index);
+      col_bases.push_back(col_index);
+      col = getCol(col_indices[col_index], row_indices);
+      u = col;
+      Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row);
+      rc.erase(col_index); rr.erase(row_index);
+      computed_rank = 1;
+      double normS = 0.0, prev_normS = 1000.0;
+      row_index = maxAbsVector(col, rr);
+      while (row_index >= 0 && std::abs(row[col_index]) > epsilon * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > epsilon &&
+             std::abs(prev_normS - normS) >= epsilon * prev_normS && computed_rank < mn) {
+        row_bases.push_back(row_index);
+        row = getRow(row_indices[row_index], col_indices);
+        row_temp = row;
+        for (int l = 0; l <= computed_rank - 1; ++l)
+          for (size_t s = 0; s < Vvec[l].size(); ++s) row[s] -= Uvec[l][row_index] * Vvec[l][s];
+        col_index = maxAbsVector(row, rc);
+        col_bases.push_back(col_index);
+        for (size_t l = 0; l < row.size(); ++l) v[l] = row[l] / row[col_index];
+        col = getCol(col_indices[col_index], row_indices);
+        col_temp = col;
+        for (int l = 0; l <= computed_rank - 1; ++l)
+          for (size_t s = 0; s < Uvec[l].size(); ++s) col[s] -= Vvec[l][col_index] * Uvec[l][s];
+        u = col;
+        if (norm(u) < 1e-8 || norm(v) < 1e-8) { row_bases.pop_back(); col_bases.pop_back(); break; }
+        Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row_temp);
+        ++computed_rank;
+        rc.erase(col_index); rr.erase(row_index);
+        if (computed_rank != 2) prev_normS = normS;
+        normS = std::pow(normS, 2) + std::pow(norm(u) * norm(v), 2);
+        for (int l = 0; l <= computed_rank - 1; ++l) normS += 2 * std::abs(dot(Uvec[l], u) * dot(Vvec[l], v));
+        normS = std::sqrt(normS);
+        row_index = maxAbsVector(col, rr);
+      }
+    } else {
+      int l_local;
+      for (l_local = 0; l_local < N2; ++l_local) {
+        col_index = l_local;
+        col = getCol(col_indices[col_index], row_indices);
+        row_index = maxAbsVector(col, rr);
+        if (std::abs(col[row_index]) > epsilon * std::pow(10, -1.0 * 0)) break;
+      }
+      if (l_local == N2) { if (Ar) *Ar = Mat(0, N2); return; }
+      col_bases.push_back(col_index);
+      v.assign(N1, 0);
+      for (int l = 0; l < N1; ++l) v[l] = col[l] / col[row_index];
+      row_bases.push_back(row_index);
+      row = getRow(row_indices[row_index], col_indices);
+      u = row;
+      Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row);
+      computed_rank = 1;
+      rr.erase(row_index); rc.erase(col_index);
+      double normS = 0.0, prev_normS = 1000.0;
+      col_index = maxAbsVector(row, rc);
+      while (col_index >= 0 && std::abs(row[col_index]) > epsilon * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > epsilon &&
+             std::abs(prev_normS - normS) >= epsilon * prev_normS && computed_rank < mn) {
+        col_bases.push_back(col_index);
+        col = getCol(col_indices[col_index], row_indices);
+        col_temp = col;
+        for (int l = 0; l <= computed_rank - 1; ++l)
+          for (size_t s = 0; s < Vvec[l].size(); ++s) col[s] -= Uvec[l][col_index] * Vvec[l][s];
+        row_index = maxAbsVector(col, rr);
+        row_bases.push_back(row_index);
+        for (size_t l = 0; l < col.size(); ++l) v[l] = col[l] / col[row_index];
+        row = getRow(row_indices[row_index], col_indices);
+        row_temp = row;
+        for (int l = 0; l <= computed_rank - 1; ++l)
+          for (size_t s = 0; s < Uvec[l].size(); ++s) row[s] -= Vvec[l][row_index] * Uvec[l][s];
+        u = row;
+        if (norm(u) < 1e-8 || norm(v) < 1e-8) { col_bases.pop_back(); row_bases.pop_back(); break; }
+        Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row_temp);
+        ++computed_rank;
+        rr.erase(row_index); rc.erase(col_index);
+        if (computed_rank != 2) prev_normS = normS;
+        normS = std::pow(normS, 2) + std::pow(norm(u) * norm(v), 2);
+        for (int l = 0; l <= computed_rank - 1; ++l) normS += 2 * std::abs(dot(Uvec[l], u) * dot(Vvec[l], v));
+        normS = std::sqrt(normS);
+        col_index = maxAbsVector(row, rc);
+      }
+    }
+    if (Ar) {
+      *Ar = Mat(computed_rank, N2);
+      for (int r = 0; r < computed_rank; ++r)
+        for (int c = 0; c < N2; ++c) (*Ar)(r, c) = Arows[r][c];
+    }
+  }
+
+  // ------------------------------------------------------------ NCA
+  bool isLFR(int J, int K) const { return J >= level_LFR || tree[J][K].isLeaf; }
+  static void app(std::vector<int>& d, const std::vector<int>& s) { d.insert(d.end(), s.begin(), s.end()); }
+  // getParticlesFromChildrenLFR_outgoing_col / incoming_col (:2013-2037, :2268-2292)
+  void lfr_charges(int j, int k, std::vector<int>& s) const {
+    if (tree[j][k].isLeaf) { app(s, tree[j][k].chargeLocations); return; }
+    for (int c = 0; c < 4; ++c) {
+      int K = tree[j][k].child[c];
+      if (isLFR(j + 1, K)) app(s, tree[j + 1][K].outgoing_chargePoints);
+      else for (int cone = 0; cone < nCones[j + 1]; ++cone) app(s, tree[j + 1][K].ConeTree[cone].outgoing_chargePoints);
+    }
+  }
+  // getParticlesFromChildrenLFR_outgoing_row / incoming_row (:2039-2060, :2242-2266)
+  void lfr_checks(int j, int k, std::vector<int>& s) const {
+    if (tree[j][k].isLeaf) { app(s, tree[j][k].chargeLocations); return; }
+    for (int c = 0; c < 4; ++c) {
+      int K = tree[j][k].child[c];
+      if (isLFR(j + 1, K)) app(s, tree[j + 1][K].incoming_checkPoints);
+      else for (int cone = 0; cone < nCones[j + 1]; ++cone) app(s, tree[j + 1][K].ConeTree[cone].incoming_checkPoints);
+    }
+  }
+  // getParticlesFromChildrenHFR_* (:3258-3320, :3421-3483)
+  void hfr_charges(int j, int k, int cone, std::vector<int>& s) const {
+    if (tree[j][k].isLeaf) { app(s, tree[j][k].chargeLocations); return; }
+    for (int c = 0; c < 4; ++c) {
+      int K = tree[j][k].child[c];
+      if (j == level_LFR - 1 || tree[j + 1][K].isLeaf) app(s, tree[j + 1][K].outgoing_chargePoints);
+      else app(s, tree[j + 1][K].ConeTree[cone / 2].outgoing_chargePoints);
+    }
+  }
+  void hfr_checks(int j, int k, int cone, std::vector<int>& s) const {
+    if (tree[j][k].isLeaf) { app(s, tree[j][k].chargeLocations); return; }
+    for (int c = 0; c < 4; ++c) {
+      int K = tree[j][k].child[c];
+      if (j == level_LFR - 1 || tree[j + 1][K].isLeaf) app(s, tree[j + 1][K].incoming_checkPoints);
+      else app(s, tree[j + 1][K].ConeTree[cone / 2].incoming_checkPoints);
+    }
+  }
+  int coneOf(int jB, int kB, int j, int k) const {  // direction of box (j,k) seen from B
+    double arg = std::atan2(tree[j][k].cy - tree[jB][kB].cy, tree[j][k].cx - tree[jB][kB].cx);
+    double argB = std::fmod(arg + 2 * PI, 2 * PI);
+    return int(argB / ConeAperture[jB]);
+  }
+  template <class Owner>
+  void outgoing(Owner& o, const std::vector<int>& IL_Nodes, const std::vector<int>& boxA_Nodes) {
+    const double tol = std::pow(10, -1.0 * TOL_POW);
+    if (!IL_Nodes.empty() && !boxA_Nodes.empty()) {
+      o.outgoing_ILActive = true;
+      std::vector<int> rb, cb;
+      int r;
+      ACA_only_nodes(IL_Nodes, boxA_Nodes, rb, cb, r, tol, &o.outgoing_Ar);
+      if (r > 0) {
+        for (int x : rb) o.outgoing_checkPoints.push_back(IL_Nodes[x]);
+        for (int x : cb) o.outgoing_chargePoints.push_back(boxA_Nodes[x]);
+        o.outgoing_Atilde_dec.compute(getMatrix(o.outgoing_checkPoints, o.outgoing_chargePoints));
+      }
+    } else {
+      o.outgoing_ILActive = false;
+      o.outgoing_chargePoints = boxA_Nodes;
+    }
+  }
+  template <class Owner>
+  void incoming(Owner& o, const std::vector<int>& boxA_Nodes, const std::vector<int>& IL_Nodes) {
+    const double tol = std::pow(10, -1.0 * TOL_POW);
+    if (!boxA_Nodes.empty() && !IL_Nodes.empty()) {
+      o.incoming_ILActive = true;
+      std::vector<int> rb, cb;
+      int r;
+      ACA_only_nodes(boxA_Nodes, IL_Nodes, rb, cb, r, tol, nullptr);
+      if (r > 0) {
+        for (int x : rb) o.incoming_checkPoints.push_back(boxA_Nodes[x]);
+        for (int x : cb) o.incoming_chargePoints.push_back(IL_Nodes[x]);
+        o.incoming_Atilde_dec.compute(getMatrix(o.incoming_checkPoints, o.incoming_chargePoints));
+      }
+    } else {
+      o.incoming_ILActive = false;
+      o.incoming_checkPoints = boxA_Nodes;
+    }
+  }
+  void lfr_out_box(int j, int k) {  // getNodes_LFR_outgoing_box (:2083-2212)
+    std::vector<int> boxA, IL;
+    lfr_charges(j, k, boxA);
+    for (auto& b : tree[j][k].InteractionList) lfr_checks(b.x, b.y, IL);
+    outgoing(tree[j][k], IL, boxA);
+  }
+  void lfr_in_box(int j, int k) {  // getNodes_LFR_incoming_box (:2294-2358)
+    std::vector<int> boxA, IL;
+    lfr_checks(j, k, boxA);
+    for (auto& b : tree[j][k].InteractionList) lfr_charges(b.x, b.y, IL);
+    incoming(tree[j][k], boxA, IL);
+  }
+  void hfr_out_box(int j, int k, int cone) {  // getNodes_HFR_outgoing_box (:3322-3392)
+    std::vector<int> boxA, IL;
+    hfr_charges(j, k, cone, boxA);
+    for (auto& b : tree[j][k].ConeTree[cone].InteractionList) hfr_checks(b.x, b.y, coneOf(b.x, b.y, j, k), IL);
+    outgoing(tree[j][k].ConeTree[cone], IL, boxA);
+  }
+  void hfr_in_box(int j, int k, int cone) {  // getNodes_HFR_incoming_box (:3485-3546)
+    std::vector<int> boxA, IL;
+    hfr_checks(j, k, cone, boxA);
+    for (auto& b : tree[j][k].ConeTree[cone].InteractionList) hfr_charges(b.x, b.y, coneOf(b.x, b.y, j, k), IL);
+    incoming(tree[j][k].ConeTree[cone], boxA, IL);
+  }
+  void getNodes_LFR() {  // :2062-2081 (boxes are independent within a level)
+    for (int j = nLevels; j >= level_LFR; --j) {
+      const int nb = int(tree[j].size());
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nb; ++k) lfr_out_box(j, k);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < nb; ++k) lfr_in_box(j, k);
+    }
+  }
+  void getNodes_HFR() {  // :3232-3255
+    for (int j = level_LFR - 1; j >= 2; --j) {
+      const int nb = int(tree[j].size()), nc = nCones[j];
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int w = 0; w < nb * nc; ++w) {
+        int k = w / nc, c = w % nc;
+        if (tree[j][k].isLeaf) { if (c == 0) lfr_out_box(j, k); }
+        else hfr_out_box(j, k, c);
+      }
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int w = 0; w < nb * nc; ++w) {
+        int k = w / nc, c = w % nc;
+        if (tree[j][k].isLeaf) { if (c == 0) lfr_in_box(j, k); }
+        else hfr_in_box(j, k, c);
+      }
+    }
+  }
+  void getUserCheckPoints() {  // :3146-3230
+    for (int j = nLevels; j >= level_LFR; --j)
+      for (auto& b : tree[j]) {
+        if (b.isLeaf) app(b.user_checkPoints, b.chargeLocations);
+        else if (b.incoming_ILActive) app(b.user_checkPoints, b.incoming_checkPoints);
+        else for (int c = 0; c < 4; ++c) app(b.user_checkPoints, tree[j + 1][b.child[c]].user_checkPoints);
+      }
+    for (int j = level_LFR - 1; j >= 2; --j)
+      for (auto& b : tree[j]) {
+        if (b.isLeaf) { app(b.user_checkPoints, b.chargeLocations); continue; }
+        for (int cone = 0; cone < nCones[j]; ++cone) {
+          FMM2DCone& cn = b.ConeTree[cone];
+          if (cn.incoming_ILActive) { app(cn.user_checkPoints, cn.incoming_checkPoints); continue; }
+          for (int c = 0; c < 4; ++c) {
+            const FMM2DBox& ch = tree[j + 1][b.child[c]];
+            if (ch.isLeaf || j + 1 == level_LFR) app(cn.user_checkPoints, ch.user_checkPoints);
+            else app(cn.user_checkPoints, ch.ConeTree[cone / 2].user_checkPoints);
+          }
+        }
+      }
+  }
+
+  // ------------------------------------------------------ assembly
+  void Assemble_HFR_M2M() {  // :3573-3622, 3804-3827
+    for (int j = level_LFR - 1; j >= 2; --j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        if (tree[j][k].isLeaf) continue;
+        for (int cone = 0; cone < nCones[j]; ++cone) {
+          FMM2DCone& cn = tree[j][k].ConeTree[cone];
+          if (!cn.outgoing_ILActive) continue;
+          std::vector<int> src;
+          hfr_charges(j, k, cone, src);
+          if (!cn.outgoing_checkPoints.empty() && !src.empty()) cn.M2M = getMatrix(cn.outgoing_checkPoints, src);
+        }
+      }
+  }
+  void Assemble_HFR_M2L() {  // :3829-3880
+    for (int j = 2; j < level_LFR; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        if (tree[j][k].isLeaf) continue;
+        for (int coneB = 0; coneB < nCones[j]; ++coneB) {
+          FMM2DCone& cn = tree[j][k].ConeTree[coneB];
+          if (!cn.incoming_ILActive) continue;
+          for (auto& b : cn.InteractionList) {
+            if (tree[b.x][b.y].isLeaf) cn.M2L.push_back(getMatrix(cn.user_checkPoints, tree[b.x][b.y].outgoing_chargePoints));
+            else cn.M2L.push_back(getMatrix(cn.user_checkPoints,
+                                            tree[b.x][b.y].ConeTree[coneOf(b.x, b.y, j, k)].outgoing_chargePoints));
+          }
+        }
+      }
+  }
+  void Assemble_LFR_M2L() {  // :2502-2575
+    for (int j = 2; j <= nLevels; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& A = tree[j][k];
+        if (j < level_LFR && !A.isLeaf) continue;
+        if (!A.incoming_ILActive) continue;
+        const std::vector<int>& rows = A.isLeaf ? A.chargeLocations : A.user_checkPoints;
+        for (auto& b : A.InteractionList) {
+          if (b.x < level_LFR && !tree[b.x][b.y].isLeaf)
+            A.M2L.push_back(getMatrix(rows, tree[b.x][b.y].ConeTree[coneOf(b.x, b.y, j, k)].outgoing_chargePoints));
+          else
+            A.M2L.push_back(getMatrix(rows, tree[b.x][b.y].outgoing_chargePoints));
+        }
+      }
+  }
+  void Assemble_LFR_L2L() {  // :2653-2700, 2824-2841
+    for (int j = 2; j < nLevels; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& P = tree[j][k];
+        if (j < level_LFR && !P.isLeaf) continue;
+        if (P.isLeaf || !P.incoming_ILActive) continue;
+        for (int c = 0; c < 4; ++c) {
+          FMM2DBox& ch = tree[j + 1][P.child[c]];
+          ch.L2L = getMatrix(ch.isLeaf ? ch.chargeLocations : ch.user_checkPoints, P.incoming_chargePoints);
+        }
+      }
+  }
+  void Assemble_HFR_L2L() {  // :3944-4019, 4183-4206
+    for (int j = 2; j < level_LFR; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& P = tree[j][k];
+        if (P.isLeaf) continue;
+        for (int cone = 0; cone < nCones[j]; ++cone) {
+          FMM2DCone& cn = P.ConeTree[cone];
+          if (!cn.incoming_ILActive) continue;
+          for (int c = 0; c < 4; ++c) {
+            FMM2DBox& ch = tree[j + 1][P.child[c]];
+            const std::vector<int>* rows;
+            if (j == level_LFR - 1) rows = ch.isLeaf ? &ch.chargeLocations : &ch.user_checkPoints;
+            else rows = ch.isLeaf ? &ch.user_checkPoints : &ch.ConeTree[cone / 2].user_checkPoints;
+            if (!rows->empty() && !cn.incoming_chargePoints.empty()) cn.L2L[c] = getMatrix(*rows, cn.incoming_chargePoints);
+          }
+        }
+      }
+  }
+
+  // --------------------------------------------------------- matvec
+  // DFMM::MatVecProduct (GMRES/GMRES.cpp:201-254)
+  void MatVecProduct(const Vec& b, Vec& output) {
+    // assignLeafCharges (:1139-1148)
+    for (size_t t = 0; t < leafNodes.size(); ++t) {
+      FMM2DBox& B = tree[leafNodes[t].x][leafNodes[t].y];
+      B.multipoles.assign(b.begin() + t * rank, b.begin() + (t + 1) * rank);
+    }
+    LFR_M2M();
+    HFR_M2M();
+    LFR_M2L();
+    HFR_M2L();
+    HFR_L2L();
+    LFR_L2L();
+    evaluate_NearField_Using_Precomputations_LS();
+    collectPotential(output);
+  }
+  static void concat(Vec& d, const Vec& s) { d.insert(d.end(), s.begin(), s.end()); }
+  void LFR_M2M() {  // :2387-2500
+    for (int j = nLevels; j >= 2; --j)
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& B = tree[j][k];
+        if (j < level_LFR && !B.isLeaf) continue;
+        if (B.outgoing_ILActive) {
+          if (B.outgoing_chargePoints.empty()) continue;
+          Vec src;
+          if (B.isLeaf) src = B.multipoles;
+          else for (int c = 0; c < 4; ++c) concat(src, tree[j + 1][B.child[c]].outgoing_charges);
+          B.outgoing_potential = matvec(B.outgoing_Ar, src);
+          B.outgoing_charges = B.outgoing_Atilde_dec.solve(B.outgoing_potential);
+        } else {
+          if (B.isLeaf) B.outgoing_charges = B.multipoles;
+          else {
+            B.outgoing_charges.clear();
+            for (int c = 0; c < 4; ++c) concat(B.outgoing_charges, tree[j + 1][B.child[c]].outgoing_charges);
+          }
+        }
+      }
+  }
+  void HFR_M2M() {  // :3624-3802
+    for (int j = level_LFR - 1; j >= 2; --j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        if (tree[j][k].isLeaf) continue;
+        for (int cone = 0; cone < nCones[j]; ++cone) {
+          FMM2DCone& cn = tree[j][k].ConeTree[cone];
+          Vec src;
+          for (int c = 0; c < 4; ++c) {
+            const FMM2DBox& ch = tree[j + 1][tree[j][k].child[c]];
+            if (j == level_LFR - 1 || ch.isLeaf) concat(src, ch.outgoing_charges);
+            else concat(src, ch.ConeTree[cone / 2].outgoing_charges);
+          }
+          if (cn.outgoing_ILActive) {
+            if (cn.outgoing_chargePoints.empty()) continue;
+            if (!cn.outgoing_checkPoints.empty() && !src.empty()) {
+              cn.outgoing_potential = matvec(cn.M2M, src);
+              cn.outgoing_charges = cn.outgoing_Atilde_dec.solve(cn.outgoing_potential);
+            }
+          } else {
+            cn.outgoing_charges = src;
+          }
+        }
+      }
+  }
+  static void addto(Vec& y, const Vec& x) { for (size_t i = 0; i < y.size(); ++i) y[i] += x[i]; }
+  void LFR_M2L() {  // :2577-2651
+    for (int j = 2; j <= nLevels; ++j)
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& A = tree[j][k];
+        if (j < level_LFR && !A.isLeaf) continue;
+        A.incoming_potential.assign(A.isLeaf ? A.chargeLocations.size() : A.user_checkPoints.size(), cd(0));
+        if (!A.incoming_ILActive) continue;
+        for (size_t l = 0; l < A.InteractionList.size(); ++l) {
+          const orderedPair& b = A.InteractionList[l];
+          const Vec* f;
+          if (b.x < level_LFR && !tree[b.x][b.y].isLeaf) f = &tree[b.x][b.y].ConeTree[coneOf(b.x, b.y, j, k)].outgoing_charges;
+          else f = &tree[b.x][b.y].outgoing_charges;
+          if (A.M2L[l].r == 0 || A.M2L[l].c == 0 || f->empty()) continue;
+          addto(A.incoming_potential, matvec(A.M2L[l], *f));
+        }
+      }
+  }
+  void HFR_M2L() {  // :3882-3942
+    for (int j = 2; j < level_LFR; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        if (tree[j][k].isLeaf) continue;
+        for (int coneB = 0; coneB < nCones[j]; ++coneB) {
+          FMM2DCone& cn = tree[j][k].ConeTree[coneB];
+          cn.incoming_potential.assign(cn.user_checkPoints.size(), cd(0));
+          if (!cn.incoming_ILActive) continue;
+          for (size_t l = 0; l < cn.InteractionList.size(); ++l) {
+            const orderedPair& b = cn.InteractionList[l];
+            const Vec* f;
+            if (tree[b.x][b.y].isLeaf) f = &tree[b.x][b.y].outgoing_charges;
+            else f = &tree[b.x][b.y].ConeTree[coneOf(b.x, b.y, j, k)].outgoing_charges;
+            if (cn.M2L[l].r == 0 || cn.M2L[l].c == 0 || f->empty()) continue;
+            addto(cn.incoming_potential, matvec(cn.M2L[l], *f));
+          }
+        }
+      }
+  }
+  void HFR_L2L() {  // :4021-4236
+    for (int j = 2; j < level_LFR; ++j)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& P = tree[j][k];
+        if (P.isLeaf) continue;
+        for (int cone = 0; cone < nCones[j]; ++cone) {
+          FMM2DCone& cn = P.ConeTree[cone];
+          if (cn.incoming_ILActive) {
+            if (cn.incoming_chargePoints.empty()) continue;
+            cn.incoming_charges = cn.incoming_Atilde_dec.solve(cn.incoming_potential);
+            for (int c = 0; c < 4; ++c) {
+              FMM2DBox& ch = tree[j + 1][P.child[c]];
+              Vec* tgt;
+              if (j == level_LFR - 1 || ch.isLeaf) tgt = &ch.incoming_potential;
+              else tgt = &ch.ConeTree[cone / 2].incoming_potential;
+              if (tgt->empty() || cn.L2L[c].r == 0) continue;
+              addto(*tgt, matvec(cn.L2L[c], cn.incoming_charges));
+            }
+          } else {
+            size_t offset = 0;
+            for (int c = 0; c < 4; ++c) {
+              FMM2DBox& ch = tree[j + 1][P.child[c]];
+              Vec* tgt;
+              if (j == level_LFR - 1 || ch.isLeaf) tgt = &ch.incoming_potential;
+              else tgt = &ch.ConeTree[cone / 2].incoming_potential;
+              for (size_t i = 0; i < tgt->size(); ++i) (*tgt)[i] += cn.incoming_potential[offset + i];
+              offset += tgt->size();
+            }
+          }
+        }
+      }
+  }
+  void LFR_L2L() {  // :2702-2821
+    for (int j = 2; j < nLevels; ++j)
+#pragma omp parallel for schedule(dynamic, 4)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& P = tree[j][k];
+        if (j < level_LFR && !P.isLeaf) continue;
+        if (P.isLeaf) continue;
+        if (P.incoming_ILActive) {
+          if (P.incoming_chargePoints.empty()) continue;
+          P.incoming_charges = P.incoming_Atilde_dec.solve(P.incoming_potential);
+          for (int c = 0; c < 4; ++c) {
+            FMM2DBox& ch = tree[j + 1][P.child[c]];
+            addto(ch.incoming_potential, matvec(ch.L2L, P.incoming_charges));
+          }
+        } else {
+          size_t offset = 0;
+          for (int c = 0; c < 4; ++c) {
+            FMM2DBox& ch = tree[j + 1][P.child[c]];
+            for (size_t i = 0; i < ch.incoming_potential.size(); ++i) ch.incoming_potential[i] += P.incoming_potential[offset + i];
+            offset += ch.incoming_potential.size();
+          }
+        }
+      }
+  }
+  void evaluate_NearField_Using_Precomputations_LS() {  // :2884-2960
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int t = 0; t < int(leafNodes.size()); ++t) {
+      FMM2DBox& A = tree[leafNodes[t].x][leafNodes[t].y];
+      for (int n = 0; n < 9; ++n) {
+        const int nk = A.colleagueNeighbors[n];
+        if (nk == -1) continue;
+        const FMM2DBox& B = tree[leafNodes[t].x][nk];
+        Mat op = colleagueNeighborInteraction[n];
+        if (findPhi) {
+          for (int i = 0; i < rank; ++i)
+            for (int c = 0; c < rank; ++c) op(i, c) = contrastK2[A.chargeLocations[i]] * op(i, c);
+          if (n == 8)
+            for (int i = 0; i < rank; ++i) op(i, i) = op(i, i) + 1.0;
+        }
+        addto(A.incoming_potential, matvec(op, B.multipoles));
+      }
+    }
+  }
+  void collectPotential(Vec& potential) {  // :3045-3054
+    potential.assign(N, cd(0));
+    for (size_t t = 0; t < leafNodes.size(); ++t) {
+      const FMM2DBox& A = tree[leafNodes[t].x][leafNodes[t].y];
+      for (int i = 0; i < rank; ++i) potential[t * rank + i] = A.incoming_potential[i];
+    }
+  }
+
+  // ---------------------------------------------------------- GMRES
+  // GMRES / RestartedGMRES (GMRES/GMRES.cpp:257-319): MGS Arnoldi, breakdown
+  // exit at |H(j+1,j)| < 1e-8, least squares by column-pivoted QR.
+  void GMRES(const Vec& x0, int m, const Vec& b, Vec& solution, int* iters) {
+    const int Nn = int(b.size());
+    Vec output;
+    MatVecProduct(x0, output);
+    Vec r0(Nn);
+    for (int i = 0; i < Nn; ++i) r0[i] = b[i] - output[i];
+    const double beta = norm(r0);
+    std::vector<Vec> V(m + 1, Vec(Nn, cd(0)));
+    Mat H(m + 1, m);
+    for (int i = 0; i < Nn; ++i) V[0][i] = r0[i] / beta;
+    for (int j = 0; j < m; ++j) {
+      Vec w;
+      MatVecProduct(V[j], w);
+      if (iters) ++*iters;
+      for (int i = 0; i <= j; ++i) {
+        H(i, j) = dot(V[i], w);
+        for (int q = 0; q < Nn; ++q) w[q] = w[q] - H(i, j) * V[i][q];
+      }
+      H(j + 1, j) = norm(w);
+      if (std::fabs(H(j + 1, j).real()) < 1e-8) { m = j + 1; break; }
+      for (int q = 0; q < Nn; ++q) V[j + 1][q] = w[q] / H(j + 1, j);
+    }
+    Mat Hm(m + 1, m);
+    for (int i = 0; i <= m; ++i)
+      for (int j = 0; j < m; ++j) Hm(i, j) = H(i, j);
+    Vec be1(m + 1, cd(0));
+    be1[0] = beta;
+    ColPivHouseholderQR qr;
+    qr.compute(Hm);
+    Vec ym = qr.solve(be1);
+    solution = x0;
+    for (int j = 0; j < m; ++j)
+      for (int q = 0; q < Nn; ++q) solution[q] += V[j][q] * ym[j];
+  }
+  void RestartedGMRES(int restart, int m, const Vec& b, Vec& solution, int* iters) {
+    Vec x0(b.size(), cd(0));
+    for (int r = 0; r < restart; ++r) {
+      GMRES(x0, m, b, solution, iters);
+      x0 = solution;
+    }
+  }
+};
+
+}  // namespace oracle
+
+// ----------------------------------------------------------------- C ABI
+using oracle::cd;
+struct or_ctx {
+  oracle::FMM2DTree t;
+  std::string err;
+};
+
+static thread_local std::string g_or_err;
+
+extern "C" {
+
+int or_create(const lsg_inputs* in, or_ctx** out) {
+  g_or_err.clear();
+  if (!in || !out) return -1;
+  or_ctx* c = new or_ctx();
+  std::string e = c->t.setup(*in);
+  if (!e.empty()) { g_or_err = e; delete c; return -3; }
+  *out = c;
+  return 0;
+}
+const char* or_error(void) { return g_or_err.c_str(); }
+void or_destroy(or_ctx* c) { delete c; }
+int64_t or_n(const or_ctx* c) { return c->t.N; }
+
+int or_matvec(or_ctx* c, const double* x, double* y) {
+  const int N = c->t.N;
+  oracle::Vec b(N), out;
+  for (int i = 0; i < N; ++i) b[i] = cd(x[2 * i], x[2 * i + 1]);
+  c->t.MatVecProduct(b, out);
+  for (int i = 0; i < N; ++i) { y[2 * i] = out[i].real(); y[2 * i + 1] = out[i].imag(); }
+  return 0;
+}
+
+// y = S x with S(i,j) = getMatrixEntry2(i,j) (dense oracle, O(N^2) entries).
+// Row i is formed one source leaf at a time: getMatrixEntry2 for the first
+// node of the leaf decides near/far, and a far leaf's 36 entries share one set
+// of Bessel values (the same values getMatrixEntry computes per entry).
+int or_dense_matvec(or_ctx* c, const double* x, double* y) {
+  const int N = c->t.N, rank = c->t.rank;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < N; ++i) {
+    cd s(0);
+    std::vector<cd> row(rank);
+    for (int lj = 0; lj < N / rank; ++lj) {
+      c->t.denseRowLeaf(i, lj, row.data());
+      for (int q = 0; q < rank; ++q) s += row[q] * cd(x[2 * (lj * rank + q)], x[2 * (lj * rank + q) + 1]);
+    }
+    y[2 * i] = s.real();
+    y[2 * i + 1] = s.imag();
+  }
+  return 0;
+}
+
+int or_dense_entry(or_ctx* c, int i, int j, double* v) {
+  cd e = c->t.getMatrixEntry2(i, j);
+  v[0] = e.real();
+  v[1] = e.imag();
+  return 0;
+}
+
+int or_far_entry(or_ctx* c, int i, int j, double* v) {
+  cd e = c->t.getMatrixEntry(i, j);
+  v[0] = e.real();
+  v[1] = e.imag();
+  return 0;
+}
+
+int or_rhs(or_ctx* c, double* b) {
+  for (int i = 0; i < c->t.N; ++i) {
+    cd v = c->t.Q.RHSFunction(c->t.gpx[i], c->t.gpy[i]);
+    b[2 * i] = v.real();
+    b[2 * i + 1] = v.imag();
+  }
+  return 0;
+}
+
+int or_grid(or_ctx* c, double* x, double* y) {
+  for (int i = 0; i < c->t.N; ++i) { x[i] = c->t.gpx[i]; y[i] = c->t.gpy[i]; }
+  return 0;
+}
+
+int or_gmres(or_ctx* c, const double* b, double* x, int restart, int m, int* iters) {
+  const int N = c->t.N;
+  oracle::Vec bb(N), sol;
+  for (int i = 0; i < N; ++i) bb[i] = cd(b[2 * i], b[2 * i + 1]);
+  int it = 0;
+  c->t.RestartedGMRES(restart, m, bb, sol, &it);
+  for (int i = 0; i < N; ++i) { x[2 * i] = sol[i].real(); x[2 * i + 1] = sol[i].imag(); }
+  if (iters) *iters = it;
+  return 0;
+}
+
+int64_t or_far_table(const or_ctx* c, double* out) {
+  const oracle::Mat& M = c->t.M[c->t.nLevels];
+  if (out)
+    for (int i = 0; i < M.r; ++i)
+      for (int j = 0; j < M.c; ++j) { out[2 * (size_t(i) * M.c + j)] = M(i, j).real(); out[2 * (size_t(i) * M.c + j) + 1] = M(i, j).imag(); }
+  return int64_t(M.r) * M.c;
+}
+
+int64_t or_near_tables(const or_ctx* c, double* out) {
+  const int r = c->t.rank;
+  if (out)
+    for (int s = 0; s < 9; ++s)
+      for (int i = 0; i < r; ++i)
+        for (int j = 0; j < r; ++j) {
+          size_t o = (size_t(s) * r + i) * r + j;
+          out[2 * o] = c->t.colleagueNeighborInteraction[s](i, j).real();
+          out[2 * o + 1] = c->t.colleagueNeighborInteraction[s](i, j).imag();
+        }
+  return int64_t(9) * r * r;
+}
+
+// same layout as the product's skeleton dump
+int64_t or_skeleton(const or_ctx* c, int* out) {
+  std::vector<int> d;
+  const oracle::FMM2DTree& t = c->t;
+  auto dumpb = [&](int l, int k, int cone, bool oa, bool ia, const std::vector<int>& och, const std::vector<int>& ochg,
+                   const std::vector<int>& ich, const std::vector<int>& ichg, const std::vector<int>& ucp) {
+    d.insert(d.end(), {l, k, cone, oa ? 1 : 0, ia ? 1 : 0, int(och.size()), int(ochg.size()), int(ich.size()),
+                       int(ichg.size()), int(ucp.size())});
+    for (const std::vector<int>* v : {&och, &ochg, &ich, &ichg, &ucp}) d.insert(d.end(), v->begin(), v->end());
+  };
+  for (int l = t.nLevels; l >= 2; --l) {
+    for (int k = 0; k < int(t.tree[l].size()); ++k)
+      if (t.isLFR(l, k)) {
+        const auto& b = t.tree[l][k];
+        dumpb(l, k, -1, b.outgoing_ILActive, b.incoming_ILActive, b.outgoing_checkPoints, b.outgoing_chargePoints,
+              b.incoming_checkPoints, b.incoming_chargePoints, b.user_checkPoints);
+      }
+    for (int k = 0; k < int(t.tree[l].size()); ++k)
+      if (!t.isLFR(l, k))
+        for (int cone = 0; cone < t.nCones[l]; ++cone) {
+          const auto& b = t.tree[l][k].ConeTree[cone];
+          dumpb(l, k, cone, b.outgoing_ILActive, b.incoming_ILActive, b.outgoing_checkPoints, b.outgoing_chargePoints,
+                b.incoming_checkPoints, b.incoming_chargePoints, b.user_checkPoints);
+        }
+  }
+  if (out) std::memcpy(out, d.data(), sizeof(int) * d.size());
+  return int64_t(d.size());
+}
+
+// interaction list of box (l,k) (cone < 0) or of one of its cones, as (level, index) pairs
+int64_t or_interaction_list(const or_ctx* c, int l, int k, int cone, int* out) {
+  const auto& b = c->t.tree[l][k];
+  const auto& il = cone < 0 ? b.InteractionList : b.ConeTree[cone].InteractionList;
+  if (out)
+    for (size_t i = 0; i < il.size(); ++i) { out[2 * i] = il[i].x; out[2 * i + 1] = il[i].y; }
+  return int64_t(il.size());
+}
+
+// centre of box (l,k) and the tree's cone counts / apertures per level
+int or_box_center(const or_ctx* c, int l, int k, double* xy) {
+  xy[0] = c->t.tree[l][k].cx;
+  xy[1] = c->t.tree[l][k].cy;
+  return c->t.tree[l][k].isLeaf ? 1 : 0;
+}
+int or_contrast_k2(const or_ctx* c, double* out) {
+  for (int i = 0; i < c->t.N; ++i) out[i] = c->t.contrastK2[i];
+  return 0;
+}
+int or_levels(const or_ctx* c, int* out) {  // nLevels, level_LFR, nCones[0..level_LFR-1]
+  out[0] = c->t.nLevels;
+  out[1] = c->t.level_LFR;
+  for (size_t j = 0; j < c->t.nCones.size(); ++j) out[2 + j] = c->t.nCones[j];
+  return 2 + int(c->t.nCones.size());
+}
+
+// Q_pinv (NumberOfBases x rank, row-major)
+int64_t or_qpinv(const or_ctx* c, double* out) {
+  if (out) std::memcpy(out, c->t.Q_pinv.data(), sizeof(double) * c->t.Q_pinv.size());
+  return int64_t(c->t.Q_pinv.size());
+}
+
+// scalar Bessel values for fixture checks
+int or_bessel(int n, double x, double* j, double* y) {
+  *j = oracle::besselJ(n, x);
+  *y = oracle::besselY(n, x);
+  return 0;
+}
+
+int or_gauss_legendre(int n, double* x, double* w) {
+  std::vector<double> xx, ww;
+  oracle::gauss_legendre(n, xx, ww);
+  for (int i = 0; i < n; ++i) { x[i] = xx[i]; w[i] = ww[i]; }
+  return 0;
+}
+
+}  // extern "C"
