@@ -49,6 +49,19 @@ int64_t lsg_host_skeleton(const lsg_host* h, int* out);
 int lsg_host_il_pairs(const lsg_host* h, int64_t* out, int max_levels);
 /* Q_pinv of the basis (NumberOfBases x nChebNodes^2, row-major). */
 int64_t lsg_host_qpinv(const lsg_host* h, double* out);
+/* The flattened device plan, for checking the fused operators on the CPU:
+ * operator entries (interleaved complex, returns the complex count), source
+ * state indices, and per destination 7 values {pass, kind (0 store,
+ * 1 accumulate), rows, ncol, dst, op_off, src_off}; returns #destinations.
+ * Passes run in index order; the near field runs before the first kind-1
+ * pass. */
+int64_t lsg_host_plan_ops(const lsg_host* h, double* out);
+int64_t lsg_host_plan_src(const lsg_host* h, int* out);
+int64_t lsg_host_plan_dests(const lsg_host* h, int64_t* out);
+/* Leaf colleague indices (n_leaves x 9, -1 where absent) and k^2 q at nodes
+ * as used by the near-field kernel. */
+int64_t lsg_host_plan_nbr(const lsg_host* h, int* out);
+int64_t lsg_host_plan_qk2(const lsg_host* h, double* out);
 /* Per-level largest NCA rank, levels 0..nLevels. */
 int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels);
 

@@ -92,6 +92,11 @@ SYMBOLS = [
     ("lsg_host_level_ranks", C.c_int, [_P, C.POINTER(C.c_int), C.c_int]),
     ("lsg_host_il_pairs", C.c_int, [_P, C.POINTER(C.c_int64), C.c_int]),
     ("lsg_host_qpinv", C.c_int64, [_P, _D]),
+    ("lsg_host_plan_ops", C.c_int64, [_P, _D]),
+    ("lsg_host_plan_src", C.c_int64, [_P, C.POINTER(C.c_int)]),
+    ("lsg_host_plan_dests", C.c_int64, [_P, C.POINTER(C.c_int64)]),
+    ("lsg_host_plan_nbr", C.c_int64, [_P, C.POINTER(C.c_int)]),
+    ("lsg_host_plan_qk2", C.c_int64, [_P, _D]),
 ]
 
 _lib = None
@@ -209,6 +214,27 @@ class HostBuild:
         n = self._lib.lsg_host_il_pairs(self._h, out.ctypes.data_as(C.POINTER(C.c_int64)), 16)
         return out[:n]
 
+    def plan(self):
+        """Flattened device plan: dict(ops, src, dests (n x 7), nbr, qk2, T)."""
+        lib, h = self._lib, self._h
+        n = lib.lsg_host_plan_ops(h, None)
+        ops = np.zeros(2 * n)
+        lib.lsg_host_plan_ops(h, _dptr(ops))
+        n = lib.lsg_host_plan_src(h, None)
+        src = np.zeros(n, dtype=np.int32)
+        lib.lsg_host_plan_src(h, src.ctypes.data_as(C.POINTER(C.c_int)))
+        n = lib.lsg_host_plan_dests(h, None)
+        dests = np.zeros((n, 7), dtype=np.int64)
+        lib.lsg_host_plan_dests(h, dests.ctypes.data_as(C.POINTER(C.c_int64)))
+        n = lib.lsg_host_plan_nbr(h, None)
+        nbr = np.zeros(n, dtype=np.int32)
+        lib.lsg_host_plan_nbr(h, nbr.ctypes.data_as(C.POINTER(C.c_int)))
+        n = lib.lsg_host_plan_qk2(h, None)
+        qk2 = np.zeros(n)
+        lib.lsg_host_plan_qk2(h, _dptr(qk2))
+        return dict(ops=ops.view(np.complex128), src=src, dests=dests, nbr=nbr.reshape(-1, 9), qk2=qk2,
+                    T=self.near_tables())
+
     def qpinv(self):
         n = self._lib.lsg_host_qpinv(self._h, None)
         out = np.zeros(n)
@@ -275,11 +301,11 @@ class DFMM:
 
     def matvec_block(self, X):
         """A X for X of shape (N, nrhs)."""
-        X = np.asfortranarray(X, dtype=np.complex128)
-        Y = np.empty_like(X, order="F")
-        self._check(self._lib.lsg_matvec(self._ctx, _dptr(X.view(np.float64)), _dptr(Y.view(np.float64)),
-                                         X.shape[1], self.N))
-        return Y
+        Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)   # nrhs x N, column j at j*N
+        Yt = np.empty_like(Xt)
+        self._check(self._lib.lsg_matvec(self._ctx, _dptr(Xt.view(np.float64)), _dptr(Yt.view(np.float64)),
+                                         Xt.shape[0], self.N))
+        return Yt.T
 
     def rhs(self):
         out = np.empty(self.N, dtype=np.complex128)

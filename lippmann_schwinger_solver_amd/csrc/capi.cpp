@@ -392,6 +392,44 @@ int64_t lsg_host_qpinv(const lsg_host* h, double* out) {
   return int64_t(h->hb.bs.qpinv.size());
 }
 
+int64_t lsg_host_plan_ops(const lsg_host* h, double* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.ops.data(), sizeof(cplx) * h->hb.plan.ops.size());
+  return int64_t(h->hb.plan.ops.size());
+}
+
+int64_t lsg_host_plan_src(const lsg_host* h, int* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.src.data(), sizeof(int) * h->hb.plan.src.size());
+  return int64_t(h->hb.plan.src.size());
+}
+
+int64_t lsg_host_plan_dests(const lsg_host* h, int64_t* out) {
+  if (!h) return LSG_ERR_ARG;
+  int64_t n = 0;
+  const auto& ps = h->hb.plan.passes;
+  for (size_t p = 0; p < ps.size(); ++p)
+    for (size_t d = 0; d < ps[p].rows.size(); ++d, ++n)
+      if (out) {
+        int64_t* o = out + 7 * n;
+        o[0] = int64_t(p); o[1] = ps[p].kind; o[2] = ps[p].rows[d]; o[3] = ps[p].ncol[d];
+        o[4] = ps[p].dst[d]; o[5] = ps[p].op_off[d]; o[6] = ps[p].src_off[d];
+      }
+  return n;
+}
+
+int64_t lsg_host_plan_nbr(const lsg_host* h, int* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.nbr.data(), sizeof(int) * h->hb.plan.nbr.size());
+  return int64_t(h->hb.plan.nbr.size());
+}
+
+int64_t lsg_host_plan_qk2(const lsg_host* h, double* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.qk2.data(), sizeof(double) * h->hb.plan.qk2.size());
+  return int64_t(h->hb.plan.qk2.size());
+}
+
 int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels) {
   if (!h || !out) return LSG_ERR_ARG;
   const auto& v = h->hb.plan.level_max_rank;

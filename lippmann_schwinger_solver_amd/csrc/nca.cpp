@@ -213,7 +213,7 @@ namespace {
 struct Contrib {
   std::vector<int> rows;                 // target nodes
   const std::vector<int>* cols = nullptr;  // source charge nodes
-  const std::vector<cplx>* Z = nullptr;    // right factor (|cols| x |cols|, col-major), L2L only
+  const ColPivQR* Zt = nullptr;            // L2L only: QR of Atilde_in^T; operator = K Atilde_in^-1
   const std::vector<int>* src_list = nullptr;  // source state indices (M2L)
   int64_t src_base = -1;                       // contiguous source (L2L)
 };
@@ -224,7 +224,7 @@ struct Owner {
   std::vector<int> src_nodes;  // outgoing ACA columns
   std::vector<cplx> Ar;        // rank x src, row-major
   std::vector<cplx> P;         // rank x src, col-major (Atilde^-1 Ar)
-  std::vector<cplx> Zin;       // rank x rank, col-major (Atilde_in^-1)
+  ColPivQR in_qrT;             // QR of Atilde_in^T (incoming check x charge points)
   std::vector<int> fout;       // state indices aligned with out_charge
   int64_t uin = -1;            // state offset of the incoming potential (size ucp)
   std::vector<Contrib> contribs;
@@ -300,13 +300,16 @@ struct Builder {
   void finish_in(Owner& o, const std::vector<int>& rows, const std::vector<int>& cols, const AcaResult& res) {
     for (int r : res.row_bases) o.in_check.push_back(rows[r]);
     for (int c : res.col_bases) o.in_charge.push_back(cols[c]);
-    ColPivQR qr;
-    qr_of(o.in_check, o.in_charge, qr);
+    // The downward operator K(child, in_charge) Atilde_in^-1 is formed row by
+    // row from solves with Atilde_in^T (backward stable), never through an
+    // explicit inverse, whose entries grow like cond(Atilde_in).
+    std::vector<cplx> A(o.in_check.size() * o.in_charge.size());
+    K.block(o.in_check.data(), int(o.in_check.size()), o.in_charge.data(), int(o.in_charge.size()), A.data());
     const int r = res.rank;
-    std::vector<cplx> I(size_t(r) * r, cplx(0));
-    for (int i = 0; i < r; ++i) I[size_t(i) * r + i] = 1.0;
-    o.Zin.resize(size_t(r) * r);
-    qr.solve(I.data(), r, o.Zin.data());
+    std::vector<cplx> At(size_t(r) * r);
+    for (int i = 0; i < r; ++i)
+      for (int j = 0; j < r; ++j) At[size_t(i) * r + j] = A[size_t(j) * r + i];
+    o.in_qrT.factor(At.data(), r, r);
   }
 
   // getNodes_LFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:2083-2212, 2294-2358)
@@ -546,7 +549,7 @@ struct Builder {
           if (n > 0) {
             Contrib s;
             s.rows.assign(c.rows.begin() + off, c.rows.begin() + off + n);
-            s.cols = c.cols; s.Z = c.Z; s.src_list = c.src_list; s.src_base = c.src_base;
+            s.cols = c.cols; s.Zt = c.Zt; s.src_list = c.src_list; s.src_base = c.src_base;
             tg.o->contribs.push_back(std::move(s));
           }
           off += n;
@@ -572,14 +575,14 @@ struct Builder {
       Contrib c;
       c.rows = tg.o->ucp;
       c.cols = &o.in_charge;
-      c.Z = &o.Zin;
+      c.Zt = &o.in_qrT;
       c.src_base = o.uin;
       tg.o->contribs.push_back(std::move(c));
     }
   }
 };
 
-int64_t contrib_cols(const Contrib& c) { return c.Z ? int64_t(c.cols->size()) : int64_t(c.cols->size()); }
+int64_t contrib_cols(const Contrib& c) { return int64_t(c.cols->size()); }
 
 }  // namespace
 
@@ -754,17 +757,14 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         if (int(c.rows.size()) != nr) continue;  // cannot happen: rows are the destination's ucp
         kb.resize(size_t(nr) * nc);
         K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());
-        if (c.Z) {
-          // op = K * Z  (nr x nc) * (nc x nc)
-          for (int q = 0; q < nc; ++q) {
-            cplx* o = op + size_t(q) * nr;
-            for (int r = 0; r < nr; ++r) o[r] = 0;
-            for (int p = 0; p < nc; ++p) {
-              const cplx z = (*c.Z)[size_t(q) * nc + p];
-              const cplx* kc = kb.data() + size_t(p) * nr;
-              for (int r = 0; r < nr; ++r) o[r] += kc[r] * z;
-            }
-          }
+        if (c.Zt) {
+          // op = K Atilde^-1: rows of op solve Atilde^T op_row^T = K_row^T
+          std::vector<cplx> B(size_t(nc) * nr), X(size_t(nc) * nr);
+          for (int r = 0; r < nr; ++r)
+            for (int p = 0; p < nc; ++p) B[size_t(r) * nc + p] = kb[size_t(p) * nr + r];
+          c.Zt->solve(B.data(), nr, X.data());
+          for (int q = 0; q < nc; ++q)
+            for (int r = 0; r < nr; ++r) op[size_t(q) * nr + r] = X[size_t(r) * nc + q];
           for (int q = 0; q < nc; ++q) sp[q] = int(c.src_base + q);
         } else {
           std::copy(kb.begin(), kb.end(), op);

@@ -1,0 +1,81 @@
+"""Device GMRES against the oracle's restarted GMRES and the FP64 residual."""
+import numpy as np
+import pytest
+
+import lippmann_schwinger_solver_amd as L
+import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    inp = H.inputs(depth=4, kappa=20.0)     # config C1: N = 9216, kappa 20, Gaussian q
+    S = L.DFMM(inp)
+    S.FMMTreeInitialisation()
+    o = O.Oracle(inp.to_c())
+    return S, o
+
+
+def test_reference_mode_matches_oracle_gmres(c1):
+    """RestartedGMRES(restart=1, m=10 and 50) as GMRES/GMRES.cpp:384,403."""
+    S, o = c1
+    b = S.rhs()
+    for m in (10, 50):
+        xg = L.RestartedGMRES(1, S, m, S.inputs, b)
+        xo, it = o.gmres(b, 1, m)
+        rg = H.relerr(S.MatVecProduct(xg), b)       # each solver against its own operator
+        ro = H.relerr(o.matvec(xo), b)
+        rx = H.relerr(o.matvec(xg), b)              # GPU solution through the oracle operator
+        print(f"m={m}: residual gpu {rg:.3e} oracle {ro:.3e} gpu-x-through-oracle {rx:.3e} "
+              f"x diff {H.relerr(xg, xo):.3e} |x|/|b| {np.linalg.norm(xg) / np.linalg.norm(b):.3e}", flush=True)
+        assert rg <= 2.0 * ro + 1e-12
+        assert rx <= 2.0 * ro + 1e-12
+
+
+@pytest.mark.parametrize("tol", [1e-8, 1e-10])
+def test_tolerance_mode_true_residual(c1, tol):
+    S, o = c1
+    b = S.rhs()
+    x, st = S.gmres(b, restart_len=50, max_restarts=10, tol=tol)
+    assert st["status"] == 0 and st["converged"] == 1
+    assert st["rel_residual"] <= tol
+    # independent FP64 check with the oracle's matvec
+    assert H.relerr(o.matvec(x), b) <= 2 * tol
+
+
+def test_gmres_failure_modes(c1):
+    S, _ = c1
+    with pytest.raises(L.LsgError) as e:
+        S.gmres(np.zeros(S.N, complex), tol=1e-8)
+    assert e.value.code == L.LSG_ERR_NUMERIC
+    b = S.rhs()
+    b[3] = np.inf
+    with pytest.raises(L.LsgError) as e:
+        S.gmres(b, tol=1e-8)
+    assert e.value.code == L.LSG_ERR_NUMERIC
+    x, st = S.gmres(S.rhs(), restart_len=2, max_restarts=1, tol=1e-14)
+    assert st["status"] == L.LSG_ERR_NOCONV and st["converged"] == 0
+
+
+def test_identity_operator_breakdown_exit():
+    """q = 0 gives A = I: the first Arnoldi step breaks down (|H(1,0)| < 1e-8)
+    and the solution is b itself."""
+    S = L.DFMM(H.inputs(depth=3, kappa=10.0, q=6, amp_scale=0.0))
+    S.FMMTreeInitialisation()
+    b = H.rand_vec(S.N, 9)
+    x, st = S.gmres(b, restart_len=10, max_restarts=2, tol=1e-12)
+    assert st["converged"] == 1 and st["iterations"] == 1
+    assert H.relerr(x, b) < 1e-14
+
+
+def test_c2_solve_fp64_verified():
+    """Config C2 at full size: tolerance GMRES reaches 1e-8 in FP64."""
+    S = L.DFMM(H.inputs(depth=6, kappa=80.0, q=6, seed=0))
+    S.FMMTreeInitialisation()
+    b = S.rhs()
+    x, st = S.gmres(b, restart_len=50, max_restarts=20, tol=1e-8)
+    print(st, flush=True)
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-8
+    assert H.relerr(S.MatVecProduct(x), b) <= 1e-8
