@@ -123,6 +123,9 @@ const char* lsg_create_error(void);
 int lsg_get_info(const lsg_ctx* ctx, lsg_info* info);
 /* Leaf-major node coordinates, x[N], y[N]. */
 int lsg_grid_points(const lsg_ctx* ctx, double* x, double* y);
+/* Leaf boxes in leaf order: centres and half side (collectBoundary input,
+ * GMRES/G_FMM2DTree.hpp:1165-1177). Arrays of n_leaves. */
+int lsg_leaf_boxes(const lsg_ctx* ctx, double* cx, double* cy, double* radius);
 /* k^2 q(x_i) at every node (the diagonal D of A = I + D V). */
 int lsg_contrast_at_nodes(const lsg_ctx* ctx, double* qk2);
 /* u_inc(x_i) = exp(i k (x cos a + y sin a)), interleaved complex. */

@@ -73,6 +73,7 @@ SYMBOLS = [
     ("lsg_create_error", C.c_char_p, []),
     ("lsg_get_info", C.c_int, [_P, C.POINTER(lsg_info)]),
     ("lsg_grid_points", C.c_int, [_P, _D, _D]),
+    ("lsg_leaf_boxes", C.c_int, [_P, _D, _D, _D]),
     ("lsg_contrast_at_nodes", C.c_int, [_P, _D]),
     ("lsg_incident", C.c_int, [_P, _D]),
     ("lsg_rhs", C.c_int, [_P, _D]),

@@ -69,6 +69,11 @@ def build(verbose=False):
     _run(["g++", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-fopenmp", "-L" + libdir, "-lamdhip64", "-Wl,-rpath," + libdir])
     os.replace(tmp, LIB)
+    # reference-style driver (12 positional inputs, result/ files)
+    exe = os.path.join(HERE, "lsg_solve")
+    _run(["g++", "-O2", "-std=c++17", "-o", exe + ".tmp", os.path.join(HERE, "cpp", "lsg_main.cpp"),
+          "-L" + HERE, "-llsg", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -1,0 +1,151 @@
+// C++ host mirror of the reference's matvec/GMRES interface, on top of the
+// C ABI in include/lsg.h. A reference driver written against
+//   inputsToDFMM, DFMM(inputs), FMMTreeInitialisation(), MatVecProduct(b, out),
+//   RestartedGMRES<DFMM>(restart, S, m, inputs, b, solution)
+// (GMRES/GMRES.cpp:129-319) compiles against this header with Vec = a vector
+// of std::complex<double> in place of Eigen::VectorXcd. Every product runs on
+// the GPU; errors throw lsg::Error.
+#pragma once
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lsg.h"
+
+namespace lsg {
+
+using Vec = std::vector<std::complex<double>>;
+
+struct pts2D {
+  double x, y;
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// GMRES/GMRES.cpp:129-138 (+ the choices the reference makes by editing code)
+struct inputsToDFMM {
+  int nCones_LFR = 16;
+  int nChebNodes = 6;
+  double L = 0.5;
+  int yes2DFMM = 1;
+  int TOL_POW = 10;
+  int degreeOfBases = 6;
+  int treeAdaptivity = 6;
+  int nLevelsUniform = 4;
+  // not in the reference struct: kappa and Qchoice are globals there
+  double kappa = 20.0;
+  int Qchoice = 0;
+  double inc_angle = 0.0;
+  lsg_contrast contrast{};
+};
+
+class DFMM {
+ public:
+  inputsToDFMM inputs;
+  std::vector<pts2D> gridPoints;
+  bool findPhi = true;  // K->findPhi of the reference (GMRES/GMRES.cpp:583)
+  int N = 0;
+
+  explicit DFMM(const inputsToDFMM& in) : inputs(in) {}
+  ~DFMM() { lsg_destroy(ctx_); }
+  DFMM(const DFMM&) = delete;
+  DFMM& operator=(const DFMM&) = delete;
+
+  // GMRES/GMRES.cpp:161-199
+  void FMMTreeInitialisation() {
+    lsg_inputs c{};
+    c.nCones_LFR = inputs.nCones_LFR;
+    c.nChebNodes = inputs.nChebNodes;
+    c.treeAdaptivity = inputs.treeAdaptivity;
+    c.L = inputs.L;
+    c.kappa = inputs.kappa;
+    c.yes2DFMM = inputs.yes2DFMM;
+    c.degreeOfBases = inputs.degreeOfBases;
+    c.TOL_POW = inputs.TOL_POW;
+    c.nLevelsUniform = inputs.nLevelsUniform;
+    c.Qchoice = inputs.Qchoice;
+    c.tree_mode = 0;
+    c.find_phi = findPhi ? 1 : 0;
+    c.inc_angle = inputs.inc_angle;
+    c.contrast = inputs.contrast;
+    c.contrast.kind = inputs.Qchoice;
+    int rc = lsg_create(&c, &ctx_);
+    if (rc != LSG_OK) throw Error(rc, lsg_create_error());
+    lsg_info info;
+    lsg_get_info(ctx_, &info);
+    N = int(info.N);
+    std::vector<double> x(N), y(N);
+    lsg_grid_points(ctx_, x.data(), y.data());
+    gridPoints.resize(N);
+    for (int i = 0; i < N; ++i) gridPoints[i] = {x[i], y[i]};
+  }
+
+  // GMRES/GMRES.cpp:201-254; output is resized to N
+  void MatVecProduct(const Vec& b, Vec& output) {
+    if (int(b.size()) != N) throw Error(LSG_ERR_ARG, "MatVecProduct: b has the wrong length");
+    output.resize(N);
+    check(lsg_matvec(ctx_, reinterpret_cast<const double*>(b.data()), reinterpret_cast<double*>(output.data()), 1, N));
+  }
+
+  // userkernel::RHSFunction / IncidenceFunction at the grid points
+  Vec rhs() const {
+    Vec b(N);
+    check(lsg_rhs(ctx_, reinterpret_cast<double*>(b.data())));
+    return b;
+  }
+  Vec incident() const {
+    Vec u(N);
+    check(lsg_incident(ctx_, reinterpret_cast<double*>(u.data())));
+    return u;
+  }
+  // collectBoundary (GMRES/G_FMM2DTree.hpp:1165-1177)
+  void boundaries(std::vector<pts2D>& leftRight, std::vector<pts2D>& bottomTop) const {
+    lsg_info info;
+    lsg_get_info(ctx_, &info);
+    std::vector<double> cx(info.n_leaves), cy(info.n_leaves), r(info.n_leaves);
+    check(lsg_leaf_boxes(ctx_, cx.data(), cy.data(), r.data()));
+    leftRight.clear();
+    bottomTop.clear();
+    for (int i = 0; i < info.n_leaves; ++i) {
+      leftRight.push_back({cx[i] - r[i], cx[i] + r[i]});
+      bottomTop.push_back({cy[i] - r[i], cy[i] + r[i]});
+    }
+  }
+  // device GMRES: tol > 0 stops on the true FP64 residual, tol <= 0 runs
+  // exactly max_restarts cycles (reference semantics)
+  lsg_stats gmres(const Vec& b, Vec& x, int m, int max_restarts, double tol) {
+    if (int(x.size()) != N) x.assign(N, 0.0);
+    lsg_stats st{};
+    int rc = lsg_gmres(ctx_, reinterpret_cast<const double*>(b.data()), reinterpret_cast<double*>(x.data()), m,
+                       max_restarts, tol, &st);
+    if (rc != LSG_OK && !(rc == LSG_ERR_NOCONV && tol > 0)) check(rc);
+    return st;
+  }
+  lsg_ctx* handle() const { return ctx_; }
+
+ private:
+  lsg_ctx* ctx_ = nullptr;
+  void check(int rc) const {
+    if (rc != LSG_OK) throw Error(rc, lsg_last_error(ctx_));
+  }
+};
+
+// GMRES/GMRES.cpp:257-305: one GMRES(m) cycle from x0
+template <typename SolverType>
+void GMRES(Vec& x0, SolverType* S, int m, inputsToDFMM, Vec& b, Vec& solution) {
+  solution = x0;
+  S->gmres(b, solution, m, 1, 0.0);
+}
+
+// GMRES/GMRES.cpp:307-319: `restart` cycles of GMRES(m) from x0 = 0
+template <typename SolverType>
+void RestartedGMRES(int restart, SolverType* S, int m, inputsToDFMM, Vec& b, Vec& solution) {
+  solution.assign(b.size(), 0.0);
+  S->gmres(b, solution, m, restart, 0.0);
+}
+
+}  // namespace lsg

@@ -198,6 +198,18 @@ int lsg_grid_points(const lsg_ctx* ctx, double* x, double* y) {
   return LSG_OK;
 }
 
+int lsg_leaf_boxes(const lsg_ctx* ctx, double* cx, double* cy, double* radius) {
+  if (!ctx || !cx || !cy || !radius) return LSG_ERR_ARG;
+  const Tree& t = ctx->hb.tree;
+  for (size_t i = 0; i < t.leaves.size(); ++i) {
+    const Box& b = t.box(t.leaves[i]);
+    cx[i] = b.cx;
+    cy[i] = b.cy;
+    radius[i] = b.radius;
+  }
+  return LSG_OK;
+}
+
 int lsg_contrast_at_nodes(const lsg_ctx* ctx, double* qk2) {
   if (!ctx || !qk2) return LSG_ERR_ARG;
   const Tree& t = ctx->hb.tree;
