@@ -148,6 +148,10 @@ int lsg_sync(lsg_ctx* ctx);
 int lsg_profile_passes(lsg_ctx* ctx, int reps, double* ms, int64_t* bytes,
                        int max_pass);
 
+/* Average device time (ms) of one full matvec (all passes, both streams) over
+ * `reps` matvecs, measured with HIP events on the context's stream. */
+int lsg_time_matvec(lsg_ctx* ctx, int reps, double* ms);
+
 /* Restarted GMRES on the device: x <- solution of A x = b (x holds the
  * initial guess). restart_len = Krylov dimension per cycle, max_restarts =
  * cycle cap. tol > 0: stop when the true FP64 relative residual <= tol;

@@ -81,6 +81,7 @@ SYMBOLS = [
     ("lsg_matvec_device", C.c_int, [_P, _P, _P]),
     ("lsg_sync", C.c_int, [_P]),
     ("lsg_profile_passes", C.c_int, [_P, C.c_int, _D, C.POINTER(C.c_int64), C.c_int]),
+    ("lsg_time_matvec", C.c_int, [_P, C.c_int, _D]),
     ("lsg_gmres", C.c_int, [_P, _D, _D, C.c_int, C.c_int, C.c_double, C.POINTER(lsg_stats)]),
     ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_build_tree", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
@@ -344,6 +345,12 @@ class DFMM:
             self._check(n)
         names = self._lib.lsg_last_error(self._ctx).decode().strip(",").split(",")
         return [(names[i], float(ms[i]), int(by[i])) for i in range(n)]
+
+    def time_matvec(self, reps=20):
+        """Average device time (ms) of one full matvec."""
+        ms = C.c_double()
+        self._check(self._lib.lsg_time_matvec(self._ctx, reps, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if self._ctx:

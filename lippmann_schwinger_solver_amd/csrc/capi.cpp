@@ -289,6 +289,13 @@ int lsg_profile_passes(lsg_ctx* ctx, int reps, double* ms, int64_t* bytes, int m
   return int(m.size());
 }
 
+int lsg_time_matvec(lsg_ctx* ctx, int reps, double* ms) {
+  if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
+  *ms = ctx->dev.time_matvec(reps);
+  if (*ms < 0) return ctx->set(LSG_ERR_DEVICE, "time_matvec allocation failed");
+  return LSG_OK;
+}
+
 int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max_restarts, double tol,
               lsg_stats* stats) {
   if (!ctx) return LSG_ERR_ARG;

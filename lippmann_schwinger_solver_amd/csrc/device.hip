@@ -2,10 +2,11 @@
 //
 // Matvec = upward passes (LFR_M2M / HFR_M2M, GMRES/G_FMM2DTree.hpp:2472, 3772),
 // near field (:2884), downward passes (LFR_M2L / HFR_M2L / HFR_L2L / LFR_L2L,
-// :2577, 3882, 4208, 2800) and the leaf far field, all as destination-major
-// gather-GEMV passes over fused operators (see nca.cpp).
+// :2577, 3882, 4208, 2800) and the two leaf passes, all as destination-major
+// gather-GEMV passes over fused operators (nca.cpp builds them).
 #include "device.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -15,18 +16,26 @@ namespace lsg {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 1024;  // source columns staged in LDS per step (16 KB)
+constexpr int kChunk = 1024;   // source columns staged in LDS per step (16 KB)
+constexpr int kNearLeaves = 16;  // leaves per workgroup in the near-field kernel (rank <= 48)
 
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void cmac(double2& acc, const dvec2 a, const double2 x) {
+  acc.x += a.x * x.x - a.y * x.y;
+  acc.y += a.x * x.y + a.y * x.x;
+}
 __device__ __forceinline__ void cmac(double2& acc, const double2 a, const double2 x) {
   acc.x += a.x * x.x - a.y * x.y;
   acc.y += a.x * x.y + a.y * x.x;
 }
 
-// One workgroup per destination: y[dst : dst+R] (=|+=) A (R x C, col-major) * state[src[0..C)].
-// Threads are split into (row, phase); phase p walks columns p, p+nph, ... so a
-// workgroup step reads nph*R consecutive complex entries (coalesced stream).
+// One workgroup per destination: Y[dst : dst+R] (=|+=) A (R x C, col-major) * state[src[0..C)].
+// Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
+// step of the workgroup reads nph*R consecutive complex entries. The operator is
+// streamed once per matvec, so it is read with non-temporal loads.
 template <bool ACC>
-__global__ __launch_bounds__(kThreads) void pass_kernel(const double2* __restrict__ ops, const int* __restrict__ src,
+__global__ __launch_bounds__(kThreads) void pass_kernel(const dvec2* __restrict__ ops, const int* __restrict__ src,
                                                         double2* __restrict__ state, const int* __restrict__ rows,
                                                         const int* __restrict__ ncol, const int* __restrict__ dst,
                                                         const long long* __restrict__ op_off,
@@ -35,7 +44,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const double2* __restric
   __shared__ double2 red[kThreads];
   const int d = blockIdx.x;
   const int R = rows[d], C = ncol[d];
-  const double2* A = ops + op_off[d];
+  const dvec2* A = ops + op_off[d];
   const int* S = src + src_off[d];
   double2* Y = state + dst[d];
   const int t = threadIdx.x;
@@ -51,20 +60,16 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const double2* __restric
       for (int i = t; i < Cc; i += kThreads) xs[i] = state[S[c0 + i]];
       __syncthreads();
       if (active) {
-        const double2* Ap = A + (size_t)c0 * R + r0 + row;
+        const dvec2* Ap = A + (size_t)c0 * R + r0 + row;
         int c = ph;
-#pragma unroll 1
-        for (; c + 3 * nph < Cc; c += 4 * nph) {
-          const double2 a0 = Ap[(size_t)c * R];
-          const double2 a1 = Ap[(size_t)(c + nph) * R];
-          const double2 a2 = Ap[(size_t)(c + 2 * nph) * R];
-          const double2 a3 = Ap[(size_t)(c + 3 * nph) * R];
-          cmac(acc, a0, xs[c]);
-          cmac(acc, a1, xs[c + nph]);
-          cmac(acc, a2, xs[c + 2 * nph]);
-          cmac(acc, a3, xs[c + 3 * nph]);
+        for (; c + 7 * nph < Cc; c += 8 * nph) {
+          dvec2 a[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(Ap + (size_t)(c + u * nph) * R);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) cmac(acc, a[u], xs[c + u * nph]);
         }
-        for (; c < Cc; c += nph) cmac(acc, Ap[(size_t)c * R], xs[c]);
+        for (; c < Cc; c += nph) cmac(acc, __builtin_nontemporal_load(Ap + (size_t)c * R), xs[c]);
       }
     }
     red[t] = acc;
@@ -79,50 +84,61 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(const double2* __restric
   }
 }
 
-// Near field of one leaf per workgroup: y_t = [x_t +] qk2_t * sum_s T_s x_nbr(s).
-// T column-major per slot so that row-threads read consecutive addresses.
+// Near field for `lpw` leaves per workgroup: each colleague table T_s
+// (rank x rank, column-major) is staged once in LDS and applied to the slot-s
+// neighbours of all leaves of the group; y_t = [x_t +] qk2_t * sum_s T_s x_nbr(s).
+// Dynamic LDS: rank*rank + lpw*rank complex values; lpw*rank <= 3*kThreads.
 __global__ __launch_bounds__(kThreads) void near_kernel(const double2* __restrict__ T, const int* __restrict__ nbr,
                                                         const double* __restrict__ qk2, double2* __restrict__ state,
-                                                        int N, int rank, int find_phi) {
-  __shared__ double2 xs[9 * 64];
-  __shared__ double2 red[kThreads];
-  const int leaf = blockIdx.x, t = threadIdx.x;
+                                                        int N, int n_leaves, int rank, int lpw, int find_phi) {
+  extern __shared__ double2 smem[];
+  double2* Ts = smem;
+  double2* Xs = smem + rank * rank;
+  const int t = threadIdx.x;
+  const int leaf0 = blockIdx.x * lpw;
+  const int nl = min(lpw, n_leaves - leaf0);
+  const int nout = nl * rank;
   const double2* x = state;
   double2* y = state + N;
-  for (int i = t; i < 9 * rank; i += kThreads) {
-    const int s = i / rank, c = i % rank;
-    const int nb = nbr[leaf * 9 + s];
-    xs[i] = nb >= 0 ? x[(size_t)nb * rank + c] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  const int nph = kThreads / rank;
-  const int row = t % rank, ph = t / rank;
-  double2 acc = make_double2(0.0, 0.0);
-  if (ph < nph) {
-    for (int s = ph; s < 9; s += nph) {
-      if (nbr[leaf * 9 + s] < 0) continue;
-      const double2* Ts = T + (size_t)s * rank * rank + row;
-      const double2* xv = xs + s * rank;
-      for (int c = 0; c < rank; ++c) cmac(acc, Ts[(size_t)c * rank], xv[c]);
+  double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+  for (int s = 0; s < 9; ++s) {
+    __syncthreads();
+    for (int i = t; i < rank * rank; i += kThreads) Ts[i] = T[(size_t)s * rank * rank + i];
+    for (int i = t; i < nout; i += kThreads) {
+      const int l = i / rank, c = i % rank;
+      const int nb = nbr[(leaf0 + l) * 9 + s];
+      Xs[i] = nb >= 0 ? x[(size_t)nb * rank + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int o = t + q * kThreads;
+      if (o < nout) {
+        const int l = o / rank, i = o % rank;
+        const double2* xv = Xs + l * rank;
+        for (int c = 0; c < rank; ++c) cmac(acc[q], Ts[c * rank + i], xv[c]);
+      }
     }
   }
-  red[t] = acc;
-  __syncthreads();
-  if (t < rank) {
-    double2 s = red[t];
-    for (int p = 1; p < nph; ++p) { s.x += red[p * rank + t].x; s.y += red[p * rank + t].y; }
-    const size_t g = (size_t)leaf * rank + t;
-    if (find_phi) {
-      const double q = qk2[g];
-      s.x = x[g].x + q * s.x;
-      s.y = x[g].y + q * s.y;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int o = t + q * kThreads;
+    if (o < nout) {
+      const size_t g = (size_t)leaf0 * rank + o;
+      double2 s = acc[q];
+      if (find_phi) {
+        const double qq = qk2[g];
+        s.x = x[g].x + qq * s.x;
+        s.y = x[g].y + qq * s.y;
+      }
+      y[g] = s;
     }
-    y[g] = s;
   }
 }
 
 // ---------------------------------------------------------------- GMRES ----
-constexpr int kDotRows = 2;  // rows per thread in the block dot
+constexpr int kDotRows = 2;   // rows per thread in the block dot
+constexpr int kDotCols = 8;   // columns reduced together
 
 __device__ __forceinline__ double2 wave_sum(double2 v) {
   for (int o = 32; o > 0; o >>= 1) {
@@ -132,48 +148,57 @@ __device__ __forceinline__ double2 wave_sum(double2 v) {
   return v;
 }
 
-// partial[b*ncols + i] = sum over the block's rows of conj(V[n,i]) * w[n]
+// partial[wave*(ncols+1) + i] = sum over the wave's rows of conj(V[n,i]) * w[n]
+// for i < ncols, and of |w[n]|^2 for i = ncols. Skipped when *flag == 0.
 __global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double2* __restrict__ V, long long ldv, int ncols,
                                                                const double2* __restrict__ w, int N,
-                                                               double2* __restrict__ partial) {
-  __shared__ double2 wsum[kThreads / 64];
-  const int t = threadIdx.x;
+                                                               double2* __restrict__ partial,
+                                                               const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
+  const int t = threadIdx.x, lane = t & 63;
+  const long long wave = (long long)blockIdx.x * (kThreads / 64) + (t >> 6);
   const long long base = (long long)blockIdx.x * kThreads * kDotRows;
+  const int ncx = ncols + 1;
   double2 wv[kDotRows];
   long long n[kDotRows];
   for (int r = 0; r < kDotRows; ++r) {
     n[r] = base + (long long)r * kThreads + t;
     wv[r] = n[r] < N ? w[n[r]] : make_double2(0.0, 0.0);
   }
-  for (int i = 0; i < ncols; ++i) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int r = 0; r < kDotRows; ++r) {
-      if (n[r] < N) {
-        const double2 v = V[(long long)i * ldv + n[r]];
-        acc.x += v.x * wv[r].x + v.y * wv[r].y;
-        acc.y += v.x * wv[r].y - v.y * wv[r].x;
+  for (int g = 0; g < ncx; g += kDotCols) {
+    double2 acc[kDotCols];
+#pragma unroll
+    for (int u = 0; u < kDotCols; ++u) {
+      acc[u] = make_double2(0.0, 0.0);
+      const int i = g + u;
+      if (i < ncx) {
+#pragma unroll
+        for (int r = 0; r < kDotRows; ++r) {
+          if (n[r] < N) {
+            const double2 v = i < ncols ? V[(long long)i * ldv + n[r]] : wv[r];
+            acc[u].x += v.x * wv[r].x + v.y * wv[r].y;
+            acc[u].y += v.x * wv[r].y - v.y * wv[r].x;
+          }
+        }
       }
     }
-    acc = wave_sum(acc);
-    if ((t & 63) == 0) wsum[t >> 6] = acc;
-    __syncthreads();
-    if (t == 0) {
-      double2 s = wsum[0];
-      for (int q = 1; q < kThreads / 64; ++q) { s.x += wsum[q].x; s.y += wsum[q].y; }
-      partial[(long long)blockIdx.x * ncols + i] = s;
+#pragma unroll
+    for (int u = 0; u < kDotCols; ++u) {
+      const double2 s = wave_sum(acc[u]);
+      if (lane == 0 && g + u < ncx) partial[wave * ncx + g + u] = s;
     }
-    __syncthreads();
   }
 }
 
-// out[i] (=|+=) sum_b partial[b*ncols + i]
-__global__ __launch_bounds__(kThreads) void dot_reduce_kernel(const double2* __restrict__ partial, int nblocks,
-                                                              int ncols, double2* __restrict__ out, int accumulate) {
+// out[i] = sum over parts of partial[p*ncx + i]; one block per column.
+__global__ __launch_bounds__(kThreads) void dot_reduce_kernel(const double2* __restrict__ partial, int nparts, int ncx,
+                                                              double2* __restrict__ out, const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
   __shared__ double2 wsum[kThreads / 64];
   const int i = blockIdx.x, t = threadIdx.x;
   double2 acc = make_double2(0.0, 0.0);
-  for (int b = t; b < nblocks; b += kThreads) {
-    const double2 p = partial[(long long)b * ncols + i];
+  for (int b = t; b < nparts; b += kThreads) {
+    const double2 p = partial[(long long)b * ncx + i];
     acc.x += p.x; acc.y += p.y;
   }
   acc = wave_sum(acc);
@@ -182,18 +207,19 @@ __global__ __launch_bounds__(kThreads) void dot_reduce_kernel(const double2* __r
   if (t == 0) {
     double2 s = wsum[0];
     for (int q = 1; q < kThreads / 64; ++q) { s.x += wsum[q].x; s.y += wsum[q].y; }
-    if (accumulate) { s.x += out[i].x; s.y += out[i].y; }
     out[i] = s;
   }
 }
 
-// w[n] += sgn * sum_i V[n,i] coef[i]
+// w[n] += sgn * sum_i V[n,i] coef[i]; skipped when *flag == 0
 __global__ __launch_bounds__(kThreads) void axpy_cols_kernel(const double2* __restrict__ V, long long ldv, int ncols,
                                                              const double2* __restrict__ coef, double sgn,
-                                                             double2* __restrict__ w, int N) {
+                                                             double2* __restrict__ w, int N,
+                                                             const int* __restrict__ flag) {
+  if (flag && *flag == 0) return;
   __shared__ double2 cs[512];
   const int t = threadIdx.x;
-  for (int i = t; i < ncols && i < 512; i += kThreads) cs[i] = coef[i];
+  for (int i = t; i < ncols; i += kThreads) cs[i] = coef[i];
   __syncthreads();
   const long long n = (long long)blockIdx.x * kThreads + t;
   if (n >= N) return;
@@ -205,82 +231,121 @@ __global__ __launch_bounds__(kThreads) void axpy_cols_kernel(const double2* __re
   w[n] = o;
 }
 
-// out = b - a (residual), out may alias a
+// out = b - a (residual); out may alias a
 __global__ void sub_kernel(const double2* __restrict__ b, const double2* a, double2* out, int N) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) { double2 v = a[n]; out[n] = make_double2(b[n].x - v.x, b[n].y - v.y); }
 }
 
-// v = w / s where s = sqrt(real(nrm2[0])) (or scal[0] if given)
+// v = w / s[0] (zero when s[0] vanishes: breakdown, the column is never used)
 __global__ void scale_kernel(const double2* __restrict__ w, const double* __restrict__ s, double2* __restrict__ v,
                              int N) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double inv = 1.0 / s[0];
+  const double d = s[0];
+  const double inv = d > 1e-300 ? 1.0 / d : 0.0;
   if (n < N) v[n] = make_double2(w[n].x * inv, w[n].y * inv);
 }
 
 struct GivensState {
-  // H column-major (m+1) x m, rotations, rhs g, misc scalars
-  double2* H;
-  double* cs;
-  double2* sn;
-  double2* g;
-  double* hn;     // ||w|| of the current step (device)
-  double* beta;   // ||r0|| (device)
+  double2* H;      // (m+1) x m, column-major, rotated in place (R factor)
+  double* cs;      // rotation cosines
+  double2* sn;     // rotation sines
+  double2* g;      // rotated beta e1
+  double* hn;      // ||w|| of the current Arnoldi step
+  double* beta;    // ||r0||
+  int* flag;       // 1: the current step needs a second Gram-Schmidt pass
 };
 
-// Cycle start: beta = sqrt(re(nrm2)), g = beta e1, out[0] = beta
+// Cycle start: beta = ||r|| from the fused dot (|w|^2 in column 0), g = beta e1.
 __global__ void cycle_init_kernel(const double2* nrm2, GivensState st, int m, double* host_out) {
-  if (threadIdx.x != 0) return;
-  const double b = sqrt(nrm2[0].x);
-  st.beta[0] = b;
-  for (int i = 0; i <= m; ++i) st.g[i] = make_double2(0.0, 0.0);
-  st.g[0] = make_double2(b, 0.0);
-  host_out[0] = b;
+  const int t = threadIdx.x;
+  const double b = sqrt(fmax(nrm2[0].x, 0.0));
+  for (int i = t; i <= m; i += blockDim.x) st.g[i] = make_double2(i == 0 ? b : 0.0, 0.0);
+  if (t == 0) {
+    st.beta[0] = b;
+    host_out[0] = b;
+  }
 }
 
-// Column j: H[0..j, j] = h (already summed), H[j+1, j] = sqrt(re(nrm2)).
-// Apply previous rotations, build rotation j, update g, write |g[j+1]| and hn.
-__global__ void givens_kernel(const double2* h, const double2* nrm2, GivensState st, int m, int j, double* host_out) {
-  if (threadIdx.x != 0) return;
-  double2* col = st.H + (long long)j * (m + 1);
-  for (int i = 0; i <= j; ++i) col[i] = h[i];
-  const double hn = sqrt(fmax(nrm2[0].x, 0.0));
-  st.hn[0] = hn;
+// Arnoldi step j, after the first (phase 1) or second (phase 2) Gram-Schmidt
+// pass. Phase 1: h1 = V^H w and |w|^2 (before projection) are in d1[0..j+1].
+// If |w - V h1|^2 = |w|^2 - |h1|^2 > |w|^2 / 2 (DGKS test, eta = 1/sqrt 2),
+// the step is complete; otherwise flag = 1 and phase 2 adds h2 from d2 with
+// |w'|^2 measured directly. The completed column is rotated with the previous
+// Givens rotations, a new rotation zeroes H(j+1,j), and |g(j+1)| (the residual
+// estimate) and ||w|| are written to host_out[2j], host_out[2j+1].
+__global__ void arnoldi_tail_kernel(const double2* d1, const double2* d2, double2* hsum, GivensState st, int m, int j,
+                                    int phase, double* host_out) {
+  __shared__ double2 col[512 + 2];
+  __shared__ double csv[512];
+  __shared__ double2 snv[512];
+  __shared__ int done;
+  const int t = threadIdx.x;
+  if (phase == 2 && st.flag[0] == 0) return;
+  if (t == 0) {
+    double hn2;
+    if (phase == 1) {
+      double s = 0.0;
+      for (int i = 0; i <= j; ++i) s += d1[i].x * d1[i].x + d1[i].y * d1[i].y;
+      const double n1 = d1[j + 1].x;
+      hn2 = n1 - s;
+      done = (hn2 > 0.5 * n1) ? 1 : 0;
+      st.flag[0] = done ? 0 : 1;
+    } else {
+      double s = 0.0;
+      for (int i = 0; i <= j; ++i) s += d2[i].x * d2[i].x + d2[i].y * d2[i].y;
+      hn2 = d2[j + 1].x - s;
+      done = 1;
+    }
+    st.hn[0] = sqrt(fmax(hn2, 0.0));
+  }
+  __syncthreads();
+  if (!done) return;
+  // H column j = h1 (+ h2), loaded in parallel with the rotations
+  for (int i = t; i <= j; i += blockDim.x) {
+    double2 v = d1[i];
+    if (phase == 2) { v.x += d2[i].x; v.y += d2[i].y; }
+    col[i] = v;
+    hsum[i] = v;
+    if (i < j) { csv[i] = st.cs[i]; snv[i] = st.sn[i]; }
+  }
+  __syncthreads();
+  if (t != 0) return;
+  const double hn = st.hn[0];
   col[j + 1] = make_double2(hn, 0.0);
   for (int i = 0; i < j; ++i) {
-    const double c = st.cs[i];
-    const double2 s = st.sn[i];
+    const double c = csv[i];
+    const double2 s = snv[i];
     const double2 a = col[i], b = col[i + 1];
     // [a; b] <- [c a + s b; -conj(s) a + c b]
     col[i] = make_double2(c * a.x + (s.x * b.x - s.y * b.y), c * a.y + (s.x * b.y + s.y * b.x));
     col[i + 1] = make_double2(c * b.x - (s.x * a.x + s.y * a.y), c * b.y - (s.x * a.y - s.y * a.x));
   }
   const double2 a = col[j];
-  const double bb = hn;
   const double an = hypot(a.x, a.y);
   double c;
   double2 s;
   if (an == 0.0) {
     c = 0.0;
     s = make_double2(1.0, 0.0);
-    col[j] = make_double2(bb, 0.0);
+    col[j] = make_double2(hn, 0.0);
   } else {
-    const double r = hypot(an, bb);
+    const double r = hypot(an, hn);
     c = an / r;
     const double2 ph = make_double2(a.x / an, a.y / an);
-    s = make_double2(ph.x * bb / r, ph.y * bb / r);
+    s = make_double2(ph.x * hn / r, ph.y * hn / r);
     col[j] = make_double2(ph.x * r, ph.y * r);
   }
   col[j + 1] = make_double2(0.0, 0.0);
+  double2* Hc = st.H + (long long)j * (m + 1);
+  for (int i = 0; i <= j + 1; ++i) Hc[i] = col[i];
   st.cs[j] = c;
   st.sn[j] = s;
   const double2 gj = st.g[j];
   st.g[j] = make_double2(c * gj.x, c * gj.y);
-  // g[j+1] = -conj(s) g_j
   st.g[j + 1] = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
-  host_out[0] = hypot(st.g[j + 1].x, st.g[j + 1].y);
-  host_out[1] = hn;
+  host_out[2 * j] = hypot(st.g[j + 1].x, st.g[j + 1].y);
+  host_out[2 * j + 1] = hn;
 }
 
 // y = R^{-1} g (upper triangular k x k of the rotated H)
@@ -324,7 +389,16 @@ std::string DeviceOperator::upload(const Plan& plan) {
   n_state = plan.n_state;
   if (rank > 64) return "nChebNodes^2 > 64 is not supported by the near-field kernel";
   HIP_OK(hipGetDevice(&device));
+  HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  // the near kernel stages one rank x rank table plus at least one leaf
+  if (int64_t(sizeof(double2)) * (rank * rank + rank) > max_lds)
+    return "near-field table does not fit in LDS: nChebNodes too large for this device";
+  if (int64_t(sizeof(double2)) * (rank * rank + std::min(kNearLeaves, 3 * kThreads / rank) * rank) > 65536)
+    HIP_OK(hipFuncSetAttribute((const void*)near_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_OK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+  HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   HIP_OK(hipMalloc((void**)&state, size_t(n_state) * sizeof(double2)));
   HIP_OK(hipMemset(state, 0, size_t(n_state) * sizeof(double2)));
   static_assert(sizeof(cplx) == sizeof(double2), "layout");
@@ -334,6 +408,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   std::vector<int> rows, ncol, dst;
   std::vector<long long> opo, srco;
   passes.clear();
+  up.clear(); down.clear(); leaf_m2l.clear(); leaf_l2l.clear();
   for (const PassHost& ph : plan.passes) {
     DevPass dp;
     dp.name = ph.name;
@@ -348,6 +423,11 @@ std::string DeviceOperator::upload(const Plan& plan) {
     dst.insert(dst.end(), ph.dst.begin(), ph.dst.end());
     opo.insert(opo.end(), ph.op_off.begin(), ph.op_off.end());
     srco.insert(srco.end(), ph.src_off.begin(), ph.src_off.end());
+    const int idx = int(passes.size());
+    if (ph.name.rfind("up_", 0) == 0) up.push_back(idx);
+    else if (ph.name == "leaf_m2l") leaf_m2l.push_back(idx);
+    else if (ph.name == "leaf_l2l") leaf_l2l.push_back(idx);
+    else down.push_back(idx);
     passes.push_back(dp);
   }
   if (!(e = upload_vec(rows, &d_rows)).empty()) return e;
@@ -366,13 +446,14 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (!(e = upload_vec(Tc, &T)).empty()) return e;
   if (!(e = upload_vec(plan.nbr, &nbr)).empty()) return e;
   if (!(e = upload_vec(plan.qk2, &qk2)).empty()) return e;
-  near_bytes = 16 * int64_t(9) * rank * rank + 16 * 3 * int64_t(N) + 8 * int64_t(N);
+  near_bytes = 16 * int64_t(9) * rank * rank + 16 * 10 * int64_t(N) + 8 * int64_t(N);
   HIP_OK(hipDeviceSynchronize());
   return "";
 }
 
 void DeviceOperator::release() {
   if (stream) hipStreamSynchronize(stream);
+  if (aux) hipStreamSynchronize(aux);
   void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -381,40 +462,80 @@ void DeviceOperator::release() {
   d_op = d_srcoff = nullptr;
   T = nullptr;
   qk2 = nullptr;
+  if (ev_fork) hipEventDestroy(ev_fork);
+  if (ev_join) hipEventDestroy(ev_join);
+  ev_fork = ev_join = nullptr;
+  if (aux) hipStreamDestroy(aux);
   if (stream) hipStreamDestroy(stream);
-  stream = nullptr;
+  aux = stream = nullptr;
 }
 
-static void launch_pass(const DeviceOperator& o, const DevPass& p) {
+static void launch_pass(const DeviceOperator& o, const DevPass& p, hipStream_t s) {
   const int64_t b = p.dest_base;
+  const dvec2* ops = reinterpret_cast<const dvec2*>(o.ops);
   if (p.kind == 0)
-    hipLaunchKernelGGL(pass_kernel<false>, dim3(p.ndest), dim3(kThreads), 0, o.stream, o.ops, o.src, o.state,
+    hipLaunchKernelGGL(pass_kernel<false>, dim3(p.ndest), dim3(kThreads), 0, s, ops, o.src, o.state,
                        o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
   else
-    hipLaunchKernelGGL(pass_kernel<true>, dim3(p.ndest), dim3(kThreads), 0, o.stream, o.ops, o.src, o.state,
+    hipLaunchKernelGGL(pass_kernel<true>, dim3(p.ndest), dim3(kThreads), 0, s, ops, o.src, o.state,
                        o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+}
+
+static void launch_near(const DeviceOperator& o, hipStream_t s) {
+  const int fit = int(o.max_lds / int(sizeof(double2))) - o.rank * o.rank;
+  const int lpw = std::max(1, std::min({kNearLeaves, 3 * kThreads / o.rank, fit / o.rank}));
+  const size_t lds = sizeof(double2) * size_t(o.rank * o.rank + lpw * o.rank);
+  hipLaunchKernelGGL(near_kernel, dim3((o.n_leaves + lpw - 1) / lpw), dim3(kThreads), lds, s, o.T, o.nbr, o.qk2,
+                     o.state, o.N, o.n_leaves, o.rank, lpw, o.find_phi ? 1 : 0);
 }
 
 void DeviceOperator::matvec(const double2* x, double2* y) {
   hipMemcpyAsync(state, x, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
-  for (const DevPass& p : passes)
-    if (p.kind == 0 && p.name.rfind("up_", 0) == 0) launch_pass(*this, p);
-  hipLaunchKernelGGL(near_kernel, dim3(n_leaves), dim3(kThreads), 0, stream, T, nbr, qk2, state, N, rank,
-                     find_phi ? 1 : 0);
-  for (const DevPass& p : passes)
-    if (!(p.kind == 0 && p.name.rfind("up_", 0) == 0)) launch_pass(*this, p);
+  // the first upward pass is the leaf level (the only one the leaf M2L reads)
+  size_t u = 0;
+  if (!up.empty()) launch_pass(*this, passes[up[0]], stream), u = 1;
+  hipEventRecord(ev_fork, stream);
+  hipStreamWaitEvent(aux, ev_fork, 0);
+  launch_near(*this, aux);
+  for (int p : leaf_m2l) launch_pass(*this, passes[p], aux);
+  hipEventRecord(ev_join, aux);
+  for (; u < up.size(); ++u) launch_pass(*this, passes[up[u]], stream);
+  for (int p : down) launch_pass(*this, passes[p], stream);
+  hipStreamWaitEvent(stream, ev_join, 0);
+  for (int p : leaf_l2l) launch_pass(*this, passes[p], stream);
   hipMemcpyAsync(y, state + N, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
+}
+
+double DeviceOperator::time_matvec(int reps) {
+  double2 *x = nullptr, *y = nullptr;
+  if (hipMalloc((void**)&x, sizeof(double2) * size_t(N)) != hipSuccess) return -1;
+  if (hipMalloc((void**)&y, sizeof(double2) * size_t(N)) != hipSuccess) { hipFree(x); return -1; }
+  hipMemcpyAsync(x, state, sizeof(double2) * size_t(N), hipMemcpyDeviceToDevice, stream);
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  matvec(x, y);
+  hipEventRecord(a, stream);
+  for (int r = 0; r < reps; ++r) matvec(x, y);
+  hipEventRecord(b, stream);
+  hipEventSynchronize(b);
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipFree(x);
+  hipFree(y);
+  return double(ms) / reps;
 }
 
 std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes,
                                     std::vector<std::string>& names) {
-  // order of execution: upward passes, near field, the rest
-  std::vector<int> order;
-  for (size_t i = 0; i < passes.size(); ++i)
-    if (passes[i].kind == 0 && passes[i].name.rfind("up_", 0) == 0) order.push_back(int(i));
+  // serialised on the main stream: upward, near field, downward, leaf M2L, leaf L2L
+  std::vector<int> order(up.begin(), up.end());
   order.push_back(-1);
-  for (size_t i = 0; i < passes.size(); ++i)
-    if (!(passes[i].kind == 0 && passes[i].name.rfind("up_", 0) == 0)) order.push_back(int(i));
+  order.insert(order.end(), down.begin(), down.end());
+  order.insert(order.end(), leaf_m2l.begin(), leaf_m2l.end());
+  order.insert(order.end(), leaf_l2l.begin(), leaf_l2l.end());
   const int np = int(order.size());
   std::vector<hipEvent_t> ev(np + 1);
   for (auto& e : ev) HIP_OK(hipEventCreate(&e));
@@ -424,11 +545,8 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
   for (int r = 0; r < reps; ++r) {
     HIP_OK(hipEventRecord(ev[0], stream));
     for (int q = 0; q < np; ++q) {
-      if (order[q] < 0)
-        hipLaunchKernelGGL(near_kernel, dim3(n_leaves), dim3(kThreads), 0, stream, T, nbr, qk2, state, N, rank,
-                           find_phi ? 1 : 0);
-      else
-        launch_pass(*this, passes[order[q]]);
+      if (order[q] < 0) launch_near(*this, stream);
+      else launch_pass(*this, passes[order[q]], stream);
       HIP_OK(hipEventRecord(ev[q + 1], stream));
     }
     HIP_OK(hipEventSynchronize(ev[np]));
@@ -454,44 +572,47 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   const int N = op.N;
   hipStream_t s = op.stream;
   const int nb_dot = (N + kThreads * kDotRows - 1) / (kThreads * kDotRows);
+  const int nparts = nb_dot * (kThreads / 64);
   const int nb_vec = (N + kThreads - 1) / kThreads;
-  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *h = nullptr, *h2 = nullptr, *nrm = nullptr, *y = nullptr;
+  const long long ldv = N;
+  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *hsum = nullptr,
+          *y = nullptr;
   GivensState st{};
   double* hostbuf = nullptr;
-  auto fail = [&](const char* what, hipError_t e) {
-    res.status = LSG_ERR_DEVICE;
-    res.error = std::string(what) + ": " + hipGetErrorString(e);
+  std::vector<void*> owned;
+  auto alloc = [&](void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) owned.push_back(*p);
+    return e == hipSuccess;
   };
-  hipError_t e;
-  const long long ldv = N;
-  if ((e = hipMalloc((void**)&V, sizeof(double2) * size_t(N) * (m + 1))) != hipSuccess) { fail("V", e); return res; }
-  hipMalloc((void**)&w, sizeof(double2) * size_t(N));
-  hipMalloc((void**)&partial, sizeof(double2) * size_t(nb_dot) * (m + 1));
-  hipMalloc((void**)&h, sizeof(double2) * (m + 2));
-  hipMalloc((void**)&h2, sizeof(double2) * (m + 2));
-  hipMalloc((void**)&nrm, sizeof(double2) * 2);
-  hipMalloc((void**)&y, sizeof(double2) * (m + 1));
-  hipMalloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m);
-  hipMalloc((void**)&st.cs, sizeof(double) * (m + 1));
-  hipMalloc((void**)&st.sn, sizeof(double2) * (m + 1));
-  hipMalloc((void**)&st.g, sizeof(double2) * (m + 2));
-  hipMalloc((void**)&st.hn, sizeof(double));
-  hipMalloc((void**)&st.beta, sizeof(double));
-  if ((e = hipHostMalloc((void**)&hostbuf, sizeof(double) * 4, hipHostMallocDefault)) != hipSuccess) {
-    fail("hostbuf", e);
+  bool ok = alloc((void**)&V, sizeof(double2) * size_t(N) * (m + 1)) && alloc((void**)&w, sizeof(double2) * size_t(N)) &&
+            alloc((void**)&partial, sizeof(double2) * size_t(nparts) * (m + 2)) &&
+            alloc((void**)&d1, sizeof(double2) * (m + 2)) && alloc((void**)&d2, sizeof(double2) * (m + 2)) &&
+            alloc((void**)&hsum, sizeof(double2) * (m + 2)) && alloc((void**)&y, sizeof(double2) * (m + 1)) &&
+            alloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m) && alloc((void**)&st.cs, sizeof(double) * (m + 1)) &&
+            alloc((void**)&st.sn, sizeof(double2) * (m + 1)) && alloc((void**)&st.g, sizeof(double2) * (m + 2)) &&
+            alloc((void**)&st.hn, sizeof(double)) && alloc((void**)&st.beta, sizeof(double)) &&
+            alloc((void**)&st.flag, sizeof(int));
+  if (!ok || hipHostMalloc((void**)&hostbuf, sizeof(double) * 2 * (m + 2), hipHostMallocDefault) != hipSuccess) {
+    for (void* p : owned) hipFree(p);
+    res.status = LSG_ERR_DEVICE;
+    res.error = "GMRES workspace allocation failed";
     return res;
   }
-  auto norm2 = [&](const double2* v, double2* out) {
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb_dot), dim3(kThreads), 0, s, v, ldv, 1, v, N, partial);
-    hipLaunchKernelGGL(dot_reduce_kernel, dim3(1), dim3(kThreads), 0, s, partial, nb_dot, 1, out, 0);
+  auto dotw = [&](const double2* Vb, int ncols, const double2* vec, double2* out, const int* flag) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb_dot), dim3(kThreads), 0, s, Vb, ldv, ncols, vec, N, partial, flag);
+    hipLaunchKernelGGL(dot_reduce_kernel, dim3(ncols + 1), dim3(kThreads), 0, s, partial, nparts, ncols + 1, out, flag);
   };
   // ||b||
-  norm2(b_dev, nrm);
+  dotw(V, 0, b_dev, d1, nullptr);
   double2 bn2;
-  hipMemcpyAsync(&bn2, nrm, sizeof(double2), hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
+  hipMemcpyAsync(&bn2, d1, sizeof(double2), hipMemcpyDeviceToHost, s);
+  hipError_t e = hipStreamSynchronize(s);
   const double bnorm = std::sqrt(bn2.x);
-  if (!(bnorm > 0) || !std::isfinite(bnorm)) {
+  if (e != hipSuccess) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = hipGetErrorString(e);
+  } else if (!(bnorm > 0) || !std::isfinite(bnorm)) {
     res.status = LSG_ERR_NUMERIC;
     res.error = !std::isfinite(bnorm) ? "right-hand side holds NaN/Inf" : "right-hand side is zero";
   }
@@ -504,9 +625,9 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   for (int cyc = 0; res.status == 0 && cyc <= max_restarts; ++cyc) {
     op.matvec(x_dev, w);
     hipLaunchKernelGGL(sub_kernel, dim3(nb_vec), dim3(kThreads), 0, s, b_dev, w, w, N);
-    norm2(w, nrm);
-    hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(64), 0, s, nrm, st, m, hostbuf);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) { fail("cycle", e); break; }
+    dotw(V, 0, w, d1, nullptr);
+    hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(64), 0, s, d1, st, m, hostbuf);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) { res.status = LSG_ERR_DEVICE; res.error = hipGetErrorString(e); break; }
     const double beta = hostbuf[0];
     res.rel_residual = beta / bnorm;
     if (!std::isfinite(beta)) { res.status = LSG_ERR_NUMERIC; res.error = "residual is NaN/Inf"; break; }
@@ -514,35 +635,43 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     if (cyc == max_restarts || beta == 0.0) break;
     hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.beta, V, N);
     int k = 0;
-    for (int j = 0; j < m; ++j) {
-      double2* vj = V + (size_t)j * N;
-      op.matvec(vj, w);
-      const int nc = j + 1;
-      // classical Gram-Schmidt, twice
-      hipLaunchKernelGGL(dot_partial_kernel, dim3(nb_dot), dim3(kThreads), 0, s, V, ldv, nc, w, N, partial);
-      hipLaunchKernelGGL(dot_reduce_kernel, dim3(nc), dim3(kThreads), 0, s, partial, nb_dot, nc, h, 0);
-      hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, h, -1.0, w, N);
-      hipLaunchKernelGGL(dot_partial_kernel, dim3(nb_dot), dim3(kThreads), 0, s, V, ldv, nc, w, N, partial);
-      hipLaunchKernelGGL(dot_reduce_kernel, dim3(nc), dim3(kThreads), 0, s, partial, nb_dot, nc, h2, 0);
-      hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, h2, -1.0, w, N);
-      hipLaunchKernelGGL(dot_reduce_kernel, dim3(nc), dim3(kThreads), 0, s, partial, nb_dot, nc, h, 1);
-      norm2(w, nrm);
-      hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(64), 0, s, h, nrm, st, m, j, hostbuf);
-      if ((e = hipStreamSynchronize(s)) != hipSuccess) { fail("arnoldi", e); break; }
-      ++total_it;
-      k = j + 1;
-      const double est = hostbuf[0], hn = hostbuf[1];
-      res.est_residual = est / bnorm;
-      if (!std::isfinite(est)) { res.status = LSG_ERR_NUMERIC; res.error = "Arnoldi produced NaN/Inf"; break; }
-      // breakdown exit as GMRES/GMRES.cpp:284-288
-      if (std::fabs(hn) < 1e-8) break;
-      hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.hn, V + (size_t)(j + 1) * N, N);
-      if (!ref_mode && est / bnorm <= 0.5 * tol) break;
+    bool stop = false;
+    double last_est = 1.0;
+    for (int j = 0; j < m && !stop;) {
+      // enqueue a batch of Arnoldi steps before reading their estimates back;
+      // steps past an exit found inside the batch are discarded
+      const int batch = std::min(m - j, (!ref_mode && last_est > 100.0 * tol) || ref_mode ? 4 : 1);
+      for (int jj = j; jj < j + batch; ++jj) {
+        op.matvec(V + (size_t)jj * N, w);
+        const int nc = jj + 1;
+        dotw(V, nc, w, d1, nullptr);
+        hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d1, -1.0, w, N,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(arnoldi_tail_kernel, dim3(1), dim3(kThreads), 0, s, d1, d2, hsum, st, m, jj, 1, hostbuf);
+        dotw(V, nc, w, d2, st.flag);
+        hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d2, -1.0, w, N,
+                           (const int*)st.flag);
+        hipLaunchKernelGGL(arnoldi_tail_kernel, dim3(1), dim3(kThreads), 0, s, d1, d2, hsum, st, m, jj, 2, hostbuf);
+        hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.hn, V + (size_t)(jj + 1) * N, N);
+      }
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) { res.status = LSG_ERR_DEVICE; res.error = hipGetErrorString(e); break; }
+      for (int jj = j; jj < j + batch; ++jj) {
+        const double est = hostbuf[2 * jj], hn = hostbuf[2 * jj + 1];
+        ++total_it;
+        k = jj + 1;
+        res.est_residual = est / bnorm;
+        last_est = est / bnorm;
+        if (!std::isfinite(est)) { res.status = LSG_ERR_NUMERIC; res.error = "Arnoldi produced NaN/Inf"; stop = true; break; }
+        if (std::fabs(hn) < 1e-8) { stop = true; break; }   // breakdown exit, GMRES/GMRES.cpp:284-288
+        if (!ref_mode && est / bnorm <= 0.5 * tol) { stop = true; break; }
+      }
+      j += batch;
     }
     if (res.status != 0) break;
     // x += V y, y = R^-1 g
     hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(64), 0, s, st, m, k, y);
-    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, k, y, 1.0, x_dev, N);
+    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, k, y, 1.0, x_dev, N,
+                       (const int*)nullptr);
     res.restarts = cyc + 1;
   }
   if (res.status == 0) {
@@ -551,8 +680,8 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   }
   res.iterations = total_it;
   res.converged = converged ? 1 : 0;
-  hipFree(V); hipFree(w); hipFree(partial); hipFree(h); hipFree(h2); hipFree(nrm); hipFree(y);
-  hipFree(st.H); hipFree(st.cs); hipFree(st.sn); hipFree(st.g); hipFree(st.hn); hipFree(st.beta);
+  hipStreamSynchronize(s);
+  for (void* p : owned) hipFree(p);
   hipHostFree(hostbuf);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return res;

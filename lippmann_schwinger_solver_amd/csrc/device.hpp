@@ -17,9 +17,16 @@ struct DevPass {
   int64_t bytes() const { return 16 * entries + 4 * n_src + 16 * n_src + 16 * n_dst; }
 };
 
+// One matvec = upward passes -> downward passes -> leaf L2L on the main
+// stream, while the near field and the leaf M2L (which need only x and the
+// leaf-level outgoing charges) run on a second stream after the first upward
+// pass and join before the leaf L2L.
 struct DeviceOperator {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // main stream (all public calls are ordered on it)
+  hipStream_t aux = nullptr;      // second stream, same default priority
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
+  int max_lds = 65536;            // LDS bytes per workgroup on this device
   int N = 0, rank = 0, n_leaves = 0;
   bool find_phi = true;
   int64_t n_state = 0;
@@ -35,18 +42,21 @@ struct DeviceOperator {
   int* nbr = nullptr;          // n_leaves x 9
   double* qk2 = nullptr;       // N
   std::vector<DevPass> passes;
+  std::vector<int> up, down, leaf_m2l, leaf_l2l;  // pass indices by role
   int64_t near_bytes = 0;
 
   std::string upload(const Plan& plan);
   void release();
-  // y = A x on the stream; x, y device pointers of N complex128
+  // y = A x on the main stream; x, y device pointers of N complex128
   void matvec(const double2* x, double2* y);
-  // event-timed per-pass averages over reps matvecs
+  // event-timed per-pass averages over reps matvecs (passes run serialised)
   std::string profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes, std::vector<std::string>& names);
+  // average time of the full two-stream matvec (ms)
+  double time_matvec(int reps);
 };
 
-// Restarted GMRES (CGS2 Arnoldi, complex Givens) with device-resident Krylov
-// basis, Hessenberg and rotations.
+// Restarted GMRES (CGS with DGKS-triggered reorthogonalisation, complex
+// Givens) with device-resident Krylov basis, Hessenberg and rotations.
 struct GmresResult {
   int iterations = 0, restarts = 0, converged = 0;
   double rel_residual = 0, est_residual = 0, seconds = 0;

@@ -228,6 +228,7 @@ struct Owner {
   std::vector<int> fout;       // state indices aligned with out_charge
   int64_t uin = -1;            // state offset of the incoming potential (size ucp)
   std::vector<Contrib> contribs;
+  std::vector<Contrib> leaf_m2l, leaf_l2l;  // leaves only
 };
 
 struct Builder {
@@ -526,15 +527,23 @@ struct Builder {
   }
 
   // resolve an owner at level j after all its contributions are known
-  void resolve(int j, int k, int cone, Owner& o, std::vector<Dest>& dests, std::vector<Dest>& leaf_dests) {
+  // Leaves get two destinations: the M2L blocks (sources are outgoing charges
+  // of same-level boxes, ready right after the leaf-level upward pass) and the
+  // L2L blocks (sources are incoming potentials of ancestors).
+  void resolve(int j, int k, int cone, Owner& o, std::vector<Dest>& dests, std::vector<Dest>& leaf_m2l,
+               std::vector<Dest>& leaf_l2l) {
     const bool leaf = t.lv[j][k].leaf;
     if (leaf) {
-      if (!o.contribs.empty()) {
+      for (Contrib& c : o.contribs) (c.src_list ? o.leaf_m2l : o.leaf_l2l).push_back(std::move(c));
+      o.contribs.clear();
+      for (int part = 0; part < 2; ++part) {
+        std::vector<Contrib>& cs = part == 0 ? o.leaf_m2l : o.leaf_l2l;
+        if (cs.empty()) continue;
         Dest d;
         d.dst = t.N + int64_t(t.lv[j][k].leaf_index) * rank;
         d.rows = rank;
-        d.contribs = &o.contribs;
-        leaf_dests.push_back(std::move(d));
+        d.contribs = &cs;
+        (part == 0 ? leaf_m2l : leaf_l2l).push_back(std::move(d));
       }
       return;
     }
@@ -682,7 +691,7 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   }
 
   // downward passes: level 2 to leaves (M2L + L2L grouped by destination)
-  std::vector<Dest> leaf_dests;
+  std::vector<Dest> leaf_m2l, leaf_l2l;
   for (int j = 2; j <= t.nLevels; ++j) {
     std::vector<Dest> ds;
     for (int k = 0; k < int(t.lv[j].size()); ++k) {
@@ -691,9 +700,9 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         for (int c = 0; c < t.nCones[j]; ++c) B.add_m2l(j, k, c, B.hc(j, k, c));
     }
     for (int k = 0; k < int(t.lv[j].size()); ++k) {
-      if (B.is_lfr(j, k)) B.resolve(j, k, -1, B.L[j][k], ds, leaf_dests);
+      if (B.is_lfr(j, k)) B.resolve(j, k, -1, B.L[j][k], ds, leaf_m2l, leaf_l2l);
       else
-        for (int c = 0; c < t.nCones[j]; ++c) B.resolve(j, k, c, B.hc(j, k, c), ds, leaf_dests);
+        for (int c = 0; c < t.nCones[j]; ++c) B.resolve(j, k, c, B.hc(j, k, c), ds, leaf_m2l, leaf_l2l);
     }
     PassHost ph;
     ph.name = "down_L" + std::to_string(j);
@@ -702,11 +711,14 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     if (!ds.empty()) passes.push_back({ph, std::move(ds)});
   }
   {
+    // uniform tree: every leaf M2L source is a leaf-level box
     PassHost ph;
-    ph.name = "leaf_far";
+    ph.name = "leaf_m2l";
     ph.level = t.nLevels;
     ph.kind = 1;
-    if (!leaf_dests.empty()) passes.push_back({ph, std::move(leaf_dests)});
+    if (!leaf_m2l.empty()) passes.push_back({ph, std::move(leaf_m2l)});
+    ph.name = "leaf_l2l";
+    if (!leaf_l2l.empty()) passes.push_back({ph, std::move(leaf_l2l)});
   }
   plan.n_state = B.next_slot;
 
