@@ -64,6 +64,35 @@ def cpu_baseline(inp, iters_per_solve, restarts_per_solve):
                       f"oracle setup {setup:.1f} s not timed"}
 
 
+def measured_traffic(pass_name):
+    """HBM bytes per launch of the dominant pass from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (scaled by a plain copy's counters); None if absent."""
+    f = os.path.join(ROOT, "profiles", "c2_leaf_m2l_pmc_traffic.json")
+    try:
+        with open(f) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return d.get("traffic_bytes") if d.get("pass") == pass_name else None
+
+
+def pmc_probe(S):
+    """A plain device-to-device copy of 256 MiB (its FETCH_SIZE / WRITE_SIZE
+    calibrate the counters: 262144 KiB each) followed by 3 matvecs."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nb = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nb)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nb)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nb)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nb), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    ms = S.time_matvec(reps=3)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"pmc probe: copy {nb >> 20} MiB, matvec {ms * 1e3:.1f} us")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +101,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver is single-device (multi-GPU sharding is a design "
@@ -85,6 +116,10 @@ def main():
     info = S.info()
     log(f"setup {time.time() - t0:.1f} s: N={info['N']} E_far={info['E_far']} bytes/matvec={info['bytes_matvec']}")
     b = S.rhs()
+    if args.pmc_probe:
+        pmc_probe(S)
+        S.close()
+        return
 
     def solve():
         x, st = S.gmres(b, restart_len=50, max_restarts=20, tol=args.tol)
@@ -138,7 +173,7 @@ def main():
         "matvec": {"ms": total_ms, "algorithmic_bytes": int(info["bytes_matvec"]), "GBps": mv_gbs,
                    "frac_hbm": mv_gbs / HBM_PEAK_GBS, "E_far": int(info["E_far"])},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom[0])},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

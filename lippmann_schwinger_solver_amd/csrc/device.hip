@@ -16,129 +16,285 @@ namespace lsg {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 1024;   // source columns staged in LDS per step (16 KB)
-constexpr int kNearLeaves = 16;  // leaves per workgroup in the near-field kernel (rank <= 48)
+constexpr int kChunk = 1024;    // source columns staged in LDS per step (16 KB)
+constexpr int kMaxRestart = 500;
+constexpr int kWaveEntries = 4096;  // passes with fewer entries per destination use one wave per destination
+constexpr int kWaveCols = 16;       // columns per lane per step in the one-wave pass kernel
 
-typedef double dvec2 __attribute__((ext_vector_type(2)));
+template <class R>
+struct cx;
+template <>
+struct cx<double> {
+  typedef double2 t;
+  typedef double v __attribute__((ext_vector_type(2)));
+};
+template <>
+struct cx<float> {
+  typedef float2 t;
+  typedef float v __attribute__((ext_vector_type(2)));
+};
 
-__device__ __forceinline__ void cmac(double2& acc, const dvec2 a, const double2 x) {
-  acc.x += a.x * x.x - a.y * x.y;
-  acc.y += a.x * x.y + a.y * x.x;
+// acc += a * x as four fused multiply-adds
+template <class Acc, class A, class X>
+__device__ __forceinline__ void cmac(Acc& acc, const A a, const X x) {
+  acc.x = fma(a.x, x.x, acc.x);
+  acc.x = fma(-a.y, x.y, acc.x);
+  acc.y = fma(a.x, x.y, acc.y);
+  acc.y = fma(a.y, x.x, acc.y);
 }
-__device__ __forceinline__ void cmac(double2& acc, const double2 a, const double2 x) {
-  acc.x += a.x * x.x - a.y * x.y;
-  acc.y += a.x * x.y + a.y * x.x;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One workgroup per destination: Y[dst : dst+R] (=|+=) A (R x C, col-major) * state[src[0..C)].
+// Source value of a pass: state indices below N are the input vector x,
+// which is read in place (the state's x region is not filled).
+template <class C>
+__device__ __forceinline__ C load_src(const C* __restrict__ x, const C* __restrict__ state, int i, int N) {
+  return i < N ? x[i] : state[i];
+}
+// Destination of a pass: [N, 2N) is the output vector y, written in place.
+template <class C>
+__device__ __forceinline__ C* dest_ptr(C* state, C* y, int d, int N) {
+  return (d >= N && d < 2 * N) ? y + (d - N) : state + d;
+}
+
+// One workgroup per destination: Y[0:R] (=|+=) A (R x C, col-major) * src[0..C).
 // Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
 // step of the workgroup reads nph*R consecutive complex entries. The operator is
-// streamed once per matvec, so it is read with non-temporal loads.
-template <bool ACC>
-__global__ __launch_bounds__(kThreads) void pass_kernel(const dvec2* __restrict__ ops, const int* __restrict__ src,
-                                                        double2* __restrict__ state, const int* __restrict__ rows,
-                                                        const int* __restrict__ ncol, const int* __restrict__ dst,
-                                                        const long long* __restrict__ op_off,
-                                                        const long long* __restrict__ src_off) {
-  __shared__ double2 xs[kChunk];
-  __shared__ double2 red[kThreads];
+// streamed once per matvec, so it is read with non-temporal loads. ROLE only
+// names the leaf M2L instantiation apart in kernel traces.
+template <bool ACC, class R, int ROLE>
+__global__ __launch_bounds__(kThreads) void pass_block_kernel(
+    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
+    const typename cx<R>::t* __restrict__ x, typename cx<R>::t* y, int N, const int* __restrict__ rows,
+    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
+    const long long* __restrict__ src_off) {
+  typedef typename cx<R>::t C;
+  typedef typename cx<R>::v V;
+  __shared__ C xs[kChunk];
+  __shared__ C red[kThreads];
   const int d = blockIdx.x;
-  const int R = rows[d], C = ncol[d];
-  const dvec2* A = ops + op_off[d];
+  const int Rw = rows[d], Cn = ncol[d];
+  const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
-  double2* Y = state + dst[d];
+  C* Y = dest_ptr(state, y, dst[d], N);
   const int t = threadIdx.x;
-  for (int r0 = 0; r0 < R; r0 += kThreads) {
-    const int Rt = min(kThreads, R - r0);
+  for (int r0 = 0; r0 < Rw; r0 += kThreads) {
+    const int Rt = min(kThreads, Rw - r0);
     const int nph = kThreads / Rt;
     const int row = t % Rt, ph = t / Rt;
     const bool active = ph < nph;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int c0 = 0; c0 < C; c0 += kChunk) {
-      const int Cc = min(kChunk, C - c0);
+    C acc;
+    acc.x = 0;
+    acc.y = 0;
+    for (int c0 = 0; c0 < Cn; c0 += kChunk) {
+      const int Cc = min(kChunk, Cn - c0);
       __syncthreads();
-      for (int i = t; i < Cc; i += kThreads) xs[i] = state[S[c0 + i]];
+      for (int i = t; i < Cc; i += kThreads) xs[i] = load_src(x, state, S[c0 + i], N);
       __syncthreads();
       if (active) {
-        const dvec2* Ap = A + (size_t)c0 * R + r0 + row;
+        const V* Ap = A + (size_t)c0 * Rw + r0 + row;
         int c = ph;
         for (; c + 7 * nph < Cc; c += 8 * nph) {
-          dvec2 a[8];
+          V a[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(Ap + (size_t)(c + u * nph) * R);
+          for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(Ap + (size_t)(c + u * nph) * Rw);
 #pragma unroll
           for (int u = 0; u < 8; ++u) cmac(acc, a[u], xs[c + u * nph]);
         }
-        for (; c < Cc; c += nph) cmac(acc, __builtin_nontemporal_load(Ap + (size_t)c * R), xs[c]);
+        for (; c < Cc; c += nph) cmac(acc, __builtin_nontemporal_load(Ap + (size_t)c * Rw), xs[c]);
       }
     }
     red[t] = acc;
     __syncthreads();
     if (t < Rt) {
-      double2 s = red[t];
-      for (int p = 1; p < nph; ++p) { s.x += red[p * Rt + t].x; s.y += red[p * Rt + t].y; }
-      if (ACC) { s.x += Y[r0 + t].x; s.y += Y[r0 + t].y; }
+      C s = red[t];
+      for (int p = 1; p < nph; ++p) {
+        s.x += red[p * Rt + t].x;
+        s.y += red[p * Rt + t].y;
+      }
+      if (ACC) {
+        s.x += Y[r0 + t].x;
+        s.y += Y[r0 + t].y;
+      }
       Y[r0 + t] = s;
     }
     __syncthreads();
   }
 }
 
-// Near field for `lpw` leaves per workgroup: each colleague table T_s
-// (rank x rank, column-major) is staged once in LDS and applied to the slot-s
-// neighbours of all leaves of the group; y_t = [x_t +] qk2_t * sum_s T_s x_nbr(s).
-// Dynamic LDS: rank*rank + lpw*rank complex values; lpw*rank <= 3*kThreads.
-__global__ __launch_bounds__(kThreads) void near_kernel(const double2* __restrict__ T, const int* __restrict__ nbr,
-                                                        const double* __restrict__ qk2, double2* __restrict__ state,
-                                                        int N, int n_leaves, int rank, int lpw, int find_phi) {
-  extern __shared__ double2 smem[];
-  double2* Ts = smem;
-  double2* Xs = smem + rank * rank;
+// One wavefront per destination (four per workgroup) for passes of small
+// operators: lanes split into (row, phase) as above and gather their source
+// values straight from memory; the phases are summed through the wave's own
+// LDS slice, so no workgroup barrier is needed and waves retire independently.
+template <bool ACC, class R>
+__global__ __launch_bounds__(kThreads) void pass_wave_kernel(
+    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
+    const typename cx<R>::t* __restrict__ x, typename cx<R>::t* y, int N, const int* __restrict__ rows,
+    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
+    const long long* __restrict__ src_off, int ndest) {
+  typedef typename cx<R>::t C;
+  typedef typename cx<R>::v V;
+  __shared__ C red[kThreads];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int d = blockIdx.x * (kThreads / 64) + wv;
+  if (d >= ndest) return;
+  const int Rw = rows[d], Cn = ncol[d];
+  const V* A = ops + op_off[d];
+  const int* S = src + src_off[d];
+  C* Y = dest_ptr(state, y, dst[d], N);
+  C* rw = red + wv * 64;
+  for (int r0 = 0; r0 < Rw; r0 += 64) {
+    const int Rt = min(64, Rw - r0);
+    const int nph = 64 / Rt;
+    const int row = lane % Rt, ph = lane / Rt;
+    C acc;
+    acc.x = 0;
+    acc.y = 0;
+    if (ph < nph) {
+      const V* Ap = A + r0 + row;
+      // up to 16 columns per lane per step: all index loads, then all operand
+      // and source loads, then the multiply-adds (two memory round trips)
+      for (int c = ph; c < Cn; c += kWaveCols * nph) {
+        int si[kWaveCols];
+        V a[kWaveCols];
+#pragma unroll
+        for (int u = 0; u < kWaveCols; ++u) {
+          const int cc = c + u * nph;
+          si[u] = cc < Cn ? S[cc] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kWaveCols; ++u) {
+          const int cc = c + u * nph;
+          if (cc < Cn) a[u] = __builtin_nontemporal_load(Ap + (size_t)cc * Rw);
+          else a[u] = V{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kWaveCols; ++u) {
+          C xv;
+          if (si[u] >= 0) xv = load_src(x, state, si[u], N);
+          else xv.x = xv.y = 0;
+          cmac(acc, a[u], xv);
+        }
+      }
+    }
+    rw[lane] = acc;
+    wave_sync();
+    if (lane < Rt) {
+      C s = rw[lane];
+      for (int p = 1; p < nph; ++p) {
+        s.x += rw[p * Rt + lane].x;
+        s.y += rw[p * Rt + lane].y;
+      }
+      if (ACC) {
+        s.x += Y[r0 + lane].x;
+        s.y += Y[r0 + lane].y;
+      }
+      Y[r0 + lane] = s;
+    }
+    wave_sync();
+  }
+}
+
+// Near field of a uniform tree as one GEMM over the 9 colleague slots:
+// y_l = [x_l +] qk2_l * sum_s T_s x_nbr(l,s). A workgroup owns LB = 2*(256/RK)
+// leaves; thread (row i, pair p) accumulates row i of leaves 2p and 2p+1, so
+// each table entry read from LDS feeds two complex multiply-adds. The next
+// slot's table and neighbour vectors are fetched into registers while the
+// current slot is being applied.
+template <class R, int RK>
+__global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t* __restrict__ T,
+                                                        const int* __restrict__ nbr, const R* __restrict__ qk2,
+                                                        const typename cx<R>::t* __restrict__ x,
+                                                        typename cx<R>::t* __restrict__ y, int n_leaves,
+                                                        int find_phi) {
+  typedef typename cx<R>::t C;
+  constexpr int LP = kThreads / RK;
+  constexpr int LB = 2 * LP;
+  constexpr int NT = (RK * RK + kThreads - 1) / kThreads;
+  constexpr int NX = (LB * RK + kThreads - 1) / kThreads;
+  __shared__ C Ts[RK * RK];
+  __shared__ C Xs[LB * RK];
   const int t = threadIdx.x;
-  const int leaf0 = blockIdx.x * lpw;
-  const int nl = min(lpw, n_leaves - leaf0);
-  const int nout = nl * rank;
-  const double2* x = state;
-  double2* y = state + N;
-  double2 acc[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+  const int i = t % RK, p = t / RK;
+  const int leaf0 = blockIdx.x * LB;
+  // registers holding the next slot's table entries and neighbour values
+  C tr[NT], xr[NX];
+#define LSG_NEAR_FETCH(slot)                                                  \
+  {                                                                           \
+    _Pragma("unroll") for (int k = 0; k < NT; ++k) {                          \
+      const int idx = t + k * kThreads;                                       \
+      tr[k] = idx < RK * RK ? T[(slot) * RK * RK + idx] : C{};                \
+    }                                                                         \
+    _Pragma("unroll") for (int k = 0; k < NX; ++k) {                          \
+      const int idx = t + k * kThreads;                                       \
+      const int l = idx / RK, c = idx - l * RK;                               \
+      const int leaf = leaf0 + l;                                             \
+      const int nb = (idx < LB * RK && leaf < n_leaves) ? nbr[leaf * 9 + (slot)] : -1; \
+      xr[k] = nb >= 0 ? x[(size_t)nb * RK + c] : C{};                         \
+    }                                                                         \
+  }
+  LSG_NEAR_FETCH(0)
+  C a0, a1;
+  a0.x = a0.y = a1.x = a1.y = 0;
   for (int s = 0; s < 9; ++s) {
     __syncthreads();
-    for (int i = t; i < rank * rank; i += kThreads) Ts[i] = T[(size_t)s * rank * rank + i];
-    for (int i = t; i < nout; i += kThreads) {
-      const int l = i / rank, c = i % rank;
-      const int nb = nbr[(leaf0 + l) * 9 + s];
-      Xs[i] = nb >= 0 ? x[(size_t)nb * rank + c] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      const int idx = t + k * kThreads;
+      if (idx < RK * RK) Ts[idx] = tr[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const int idx = t + k * kThreads;
+      if (idx < LB * RK) Xs[idx] = xr[k];
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int o = t + q * kThreads;
-      if (o < nout) {
-        const int l = o / rank, i = o % rank;
-        const double2* xv = Xs + l * rank;
-        for (int c = 0; c < rank; ++c) cmac(acc[q], Ts[c * rank + i], xv[c]);
+    if (s + 1 < 9) LSG_NEAR_FETCH(s + 1)
+    if (p < LP) {
+      const C* xa = Xs + 2 * p * RK;
+      const C* xb = xa + RK;
+#pragma unroll 6
+      for (int c = 0; c < RK; ++c) {
+        const C tv = Ts[c * RK + i];
+        cmac(a0, tv, xa[c]);
+        cmac(a1, tv, xb[c]);
       }
     }
   }
+#undef LSG_NEAR_FETCH
+  if (p < LP) {
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int o = t + q * kThreads;
-    if (o < nout) {
-      const size_t g = (size_t)leaf0 * rank + o;
-      double2 s = acc[q];
-      if (find_phi) {
-        const double qq = qk2[g];
-        s.x = x[g].x + qq * s.x;
-        s.y = x[g].y + qq * s.y;
+    for (int q = 0; q < 2; ++q) {
+      const int leaf = leaf0 + 2 * p + q;
+      if (leaf < n_leaves) {
+        const size_t g = (size_t)leaf * RK + i;
+        C s = q == 0 ? a0 : a1;
+        if (find_phi) {
+          const R qq = qk2[g];
+          const C xv = x[g];
+          s.x = xv.x + qq * s.x;
+          s.y = xv.y + qq * s.y;
+        }
+        y[g] = s;
       }
-      y[g] = s;
     }
   }
 }
 
 // ---------------------------------------------------------------- GMRES ----
-constexpr int kDotRows = 2;   // rows per thread in the block dot
-constexpr int kDotCols = 8;   // columns reduced together
+// Classical Gram-Schmidt with one full reorthogonalisation (CGS2) per Arnoldi
+// step, in five launches:
+//   gs_dot      partials of V^H w                  (reads V once)
+//   gs_reduce   h1 = sum of the partials
+//   gs_proj_dot w' = w - V h1, partials of V^H w' and |w'|^2
+//   gs_tail     h2, h = h1 + h2, ||w''||^2 = |w'|^2 - |h2|^2, Givens update
+//   gs_update   v_{j+1} = (w' - V h2) / ||w''||
+constexpr int kGsRows = 2 * kThreads;  // rows per workgroup in the dot kernels
+constexpr int kGsCols = 8;             // columns per workgroup in gs_dot
 
 __device__ __forceinline__ double2 wave_sum(double2 v) {
   for (int o = 32; o > 0; o >>= 1) {
@@ -148,102 +304,164 @@ __device__ __forceinline__ double2 wave_sum(double2 v) {
   return v;
 }
 
-// partial[wave*(ncols+1) + i] = sum over the wave's rows of conj(V[n,i]) * w[n]
-// for i < ncols, and of |w[n]|^2 for i = ncols. Skipped when *flag == 0.
-__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double2* __restrict__ V, long long ldv, int ncols,
-                                                               const double2* __restrict__ w, int N,
-                                                               double2* __restrict__ partial,
-                                                               const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  const int t = threadIdx.x, lane = t & 63;
-  const long long wave = (long long)blockIdx.x * (kThreads / 64) + (t >> 6);
-  const long long base = (long long)blockIdx.x * kThreads * kDotRows;
-  const int ncx = ncols + 1;
-  double2 wv[kDotRows];
-  long long n[kDotRows];
-  for (int r = 0; r < kDotRows; ++r) {
-    n[r] = base + (long long)r * kThreads + t;
-    wv[r] = n[r] < N ? w[n[r]] : make_double2(0.0, 0.0);
+// acc += conj(v) * w
+__device__ __forceinline__ void cdot(double2& acc, const double2 v, const double2 w) {
+  acc.x = fma(v.x, w.x, acc.x);
+  acc.x = fma(v.y, w.y, acc.x);
+  acc.y = fma(v.x, w.y, acc.y);
+  acc.y = fma(-v.y, w.x, acc.y);
+}
+
+// Sum of four per-wave values of column u (ws[wave][u]) into partial[blk*ncx + g0 + u].
+__device__ __forceinline__ void store_block_partials(double2 (*ws)[kGsCols], const double2 (&acc)[kGsCols], int g0,
+                                                     int ncx, double2* partial, long long blk) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int u = 0; u < kGsCols; ++u) {
+    const double2 s = wave_sum(acc[u]);
+    if (lane == 0) ws[wv][u] = s;
   }
-  for (int g = 0; g < ncx; g += kDotCols) {
-    double2 acc[kDotCols];
+  __syncthreads();
+  if (t < kGsCols && g0 + t < ncx) {
+    double2 s = ws[0][t];
+    for (int q = 1; q < kThreads / 64; ++q) {
+      s.x += ws[q][t].x;
+      s.y += ws[q][t].y;
+    }
+    partial[blk * ncx + g0 + t] = s;
+  }
+  __syncthreads();
+}
+
+// partial[blockIdx.x*ncx + col] for col in [blockIdx.y*8, +8): conj(V[:,col]).w over
+// the workgroup's 512 rows for col < nc, |w|^2 for col == nc (when ncx = nc+1).
+__global__ __launch_bounds__(kThreads) void gs_dot_kernel(const double2* __restrict__ V, long long ldv, int nc,
+                                                          int ncx, const double2* __restrict__ w, int N,
+                                                          double2* __restrict__ partial) {
+  __shared__ double2 ws[kThreads / 64][kGsCols];
+  const int t = threadIdx.x;
+  const long long n0 = (long long)blockIdx.x * kGsRows + t, n1 = n0 + kThreads;
+  const int g0 = blockIdx.y * kGsCols;
+  const double2 z = make_double2(0.0, 0.0);
+  const double2 w0 = n0 < N ? w[n0] : z, w1 = n1 < N ? w[n1] : z;
+  double2 acc[kGsCols];
 #pragma unroll
-    for (int u = 0; u < kDotCols; ++u) {
-      acc[u] = make_double2(0.0, 0.0);
-      const int i = g + u;
-      if (i < ncx) {
-#pragma unroll
-        for (int r = 0; r < kDotRows; ++r) {
-          if (n[r] < N) {
-            const double2 v = i < ncols ? V[(long long)i * ldv + n[r]] : wv[r];
-            acc[u].x += v.x * wv[r].x + v.y * wv[r].y;
-            acc[u].y += v.x * wv[r].y - v.y * wv[r].x;
-          }
-        }
+  for (int u = 0; u < kGsCols; ++u) {
+    const int col = g0 + u;
+    double2 v0 = z, v1 = z;
+    if (col < nc) {
+      if (n0 < N) v0 = V[col * ldv + n0];
+      if (n1 < N) v1 = V[col * ldv + n1];
+    } else if (col == nc) {
+      v0 = w0;
+      v1 = w1;
+    }
+    acc[u] = z;
+    cdot(acc[u], v0, w0);
+    cdot(acc[u], v1, w1);
+  }
+  store_block_partials(ws, acc, g0, ncx, partial, blockIdx.x);
+}
+
+// out[c] = sum_p partial[p*ncx + c] for c < ncx, by one workgroup; also kept in lds_out.
+__device__ void block_reduce_cols(const double2* __restrict__ partial, int nparts, int ncx, double2* out,
+                                  double2* lds_out, double2* red) {
+  const int t = threadIdx.x;
+  for (int c0 = 0; c0 < ncx; c0 += kThreads) {
+    const int ncc = min(kThreads, ncx - c0);
+    const int S = kThreads / ncc;
+    const int c = t % ncc, sg = t / ncc;
+    double2 a = make_double2(0.0, 0.0);
+    if (sg < S) {
+#pragma unroll 4
+      for (int q = sg; q < nparts; q += S) {
+        const double2 v = partial[(long long)q * ncx + c0 + c];
+        a.x += v.x;
+        a.y += v.y;
       }
     }
-#pragma unroll
-    for (int u = 0; u < kDotCols; ++u) {
-      const double2 s = wave_sum(acc[u]);
-      if (lane == 0 && g + u < ncx) partial[wave * ncx + g + u] = s;
+    red[t] = a;
+    __syncthreads();
+    if (t < ncc) {
+      double2 s = red[t];
+      for (int q = 1; q < S; ++q) {
+        s.x += red[q * ncc + t].x;
+        s.y += red[q * ncc + t].y;
+      }
+      if (out) out[c0 + t] = s;
+      if (lds_out) lds_out[c0 + t] = s;
     }
+    __syncthreads();
   }
 }
 
-// out[i] = sum over parts of partial[p*ncx + i]; one block per column.
-__global__ __launch_bounds__(kThreads) void dot_reduce_kernel(const double2* __restrict__ partial, int nparts, int ncx,
-                                                              double2* __restrict__ out, const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  __shared__ double2 wsum[kThreads / 64];
-  const int i = blockIdx.x, t = threadIdx.x;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int b = t; b < nparts; b += kThreads) {
-    const double2 p = partial[(long long)b * ncx + i];
-    acc.x += p.x; acc.y += p.y;
-  }
-  acc = wave_sum(acc);
-  if ((t & 63) == 0) wsum[t >> 6] = acc;
-  __syncthreads();
-  if (t == 0) {
-    double2 s = wsum[0];
-    for (int q = 1; q < kThreads / 64; ++q) { s.x += wsum[q].x; s.y += wsum[q].y; }
-    out[i] = s;
-  }
+__global__ __launch_bounds__(kThreads) void gs_reduce_kernel(const double2* __restrict__ partial, int nparts,
+                                                             int ncx, double2* __restrict__ out) {
+  __shared__ double2 red[kThreads];
+  block_reduce_cols(partial, nparts, ncx, out, nullptr, red);
 }
 
-// w[n] += sgn * sum_i V[n,i] coef[i]; skipped when *flag == 0
-__global__ __launch_bounds__(kThreads) void axpy_cols_kernel(const double2* __restrict__ V, long long ldv, int ncols,
-                                                             const double2* __restrict__ coef, double sgn,
-                                                             double2* __restrict__ w, int N,
-                                                             const int* __restrict__ flag) {
-  if (flag && *flag == 0) return;
-  __shared__ double2 cs[512];
+// w <- w - V h (h = first nc entries of hg); then partials of conj(V[:,c]).w for c < nc and
+// |w|^2 (column nc) per workgroup of 512 rows.
+__global__ __launch_bounds__(kThreads) void gs_proj_dot_kernel(const double2* __restrict__ V, long long ldv, int nc,
+                                                               const double2* __restrict__ hg, double2* __restrict__ w,
+                                                               int N, double2* __restrict__ partial) {
+  __shared__ double2 hs[kMaxRestart + 1];
+  __shared__ double2 ws[kThreads / 64][kGsCols];
   const int t = threadIdx.x;
-  for (int i = t; i < ncols; i += kThreads) cs[i] = coef[i];
+  for (int i = t; i < nc; i += kThreads) hs[i] = hg[i];
   __syncthreads();
-  const long long n = (long long)blockIdx.x * kThreads + t;
-  if (n >= N) return;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int i = 0; i < ncols; ++i) cmac(acc, V[(long long)i * ldv + n], cs[i]);
-  double2 o = w[n];
-  o.x += sgn * acc.x;
-  o.y += sgn * acc.y;
-  w[n] = o;
-}
-
-// out = b - a (residual); out may alias a
-__global__ void sub_kernel(const double2* __restrict__ b, const double2* a, double2* out, int N) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < N) { double2 v = a[n]; out[n] = make_double2(b[n].x - v.x, b[n].y - v.y); }
-}
-
-// v = w / s[0] (zero when s[0] vanishes: breakdown, the column is never used)
-__global__ void scale_kernel(const double2* __restrict__ w, const double* __restrict__ s, double2* __restrict__ v,
-                             int N) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double d = s[0];
-  const double inv = d > 1e-300 ? 1.0 / d : 0.0;
-  if (n < N) v[n] = make_double2(w[n].x * inv, w[n].y * inv);
+  const long long n0 = (long long)blockIdx.x * kGsRows + t, n1 = n0 + kThreads;
+  const bool ok0 = n0 < N, ok1 = n1 < N;
+  const double2 z = make_double2(0.0, 0.0);
+  double2 w0 = ok0 ? w[n0] : z, w1 = ok1 ? w[n1] : z;
+  {
+    double2 a0 = z, a1 = z;
+    int i = 0;
+    for (; i + 4 <= nc; i += 4) {
+      double2 v0[4], v1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v0[u] = ok0 ? V[(i + u) * ldv + n0] : z;
+        v1[u] = ok1 ? V[(i + u) * ldv + n1] : z;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cmac(a0, v0[u], hs[i + u]);
+        cmac(a1, v1[u], hs[i + u]);
+      }
+    }
+    for (; i < nc; ++i) {
+      if (ok0) cmac(a0, V[i * ldv + n0], hs[i]);
+      if (ok1) cmac(a1, V[i * ldv + n1], hs[i]);
+    }
+    w0.x -= a0.x;
+    w0.y -= a0.y;
+    w1.x -= a1.x;
+    w1.y -= a1.y;
+    if (ok0) w[n0] = w0;
+    if (ok1) w[n1] = w1;
+  }
+  const int ncx = nc + 1;
+  for (int g0 = 0; g0 < ncx; g0 += kGsCols) {
+    double2 acc[kGsCols];
+#pragma unroll
+    for (int u = 0; u < kGsCols; ++u) {
+      const int col = g0 + u;
+      double2 v0 = z, v1 = z;
+      if (col < nc) {
+        if (ok0) v0 = V[col * ldv + n0];
+        if (ok1) v1 = V[col * ldv + n1];
+      } else if (col == nc) {
+        v0 = w0;
+        v1 = w1;
+      }
+      acc[u] = z;
+      cdot(acc[u], v0, w0);
+      cdot(acc[u], v1, w1);
+    }
+    store_block_partials(ws, acc, g0, ncx, partial, blockIdx.x);
+  }
 }
 
 struct GivensState {
@@ -253,11 +471,14 @@ struct GivensState {
   double2* g;      // rotated beta e1
   double* hn;      // ||w|| of the current Arnoldi step
   double* beta;    // ||r0||
-  int* flag;       // 1: the current step needs a second Gram-Schmidt pass
 };
 
-// Cycle start: beta = ||r|| from the fused dot (|w|^2 in column 0), g = beta e1.
-__global__ void cycle_init_kernel(const double2* nrm2, GivensState st, int m, double* host_out) {
+// Cycle start: beta = ||r|| from the partials of |r|^2 (one column), g = beta e1.
+__global__ __launch_bounds__(kThreads) void gs_cycle_init_kernel(const double2* __restrict__ partial, int nparts,
+                                                                 GivensState st, int m, double* host_out) {
+  __shared__ double2 red[kThreads];
+  __shared__ double2 nrm2[1];
+  block_reduce_cols(partial, nparts, 1, nullptr, nrm2, red);
   const int t = threadIdx.x;
   const double b = sqrt(fmax(nrm2[0].x, 0.0));
   for (int i = t; i <= m; i += blockDim.x) st.g[i] = make_double2(i == 0 ? b : 0.0, 0.0);
@@ -267,51 +488,35 @@ __global__ void cycle_init_kernel(const double2* nrm2, GivensState st, int m, do
   }
 }
 
-// Arnoldi step j, after the first (phase 1) or second (phase 2) Gram-Schmidt
-// pass. Phase 1: h1 = V^H w and |w|^2 (before projection) are in d1[0..j+1].
-// If |w - V h1|^2 = |w|^2 - |h1|^2 > |w|^2 / 2 (DGKS test, eta = 1/sqrt 2),
-// the step is complete; otherwise flag = 1 and phase 2 adds h2 from d2 with
-// |w'|^2 measured directly. The completed column is rotated with the previous
-// Givens rotations, a new rotation zeroes H(j+1,j), and |g(j+1)| (the residual
-// estimate) and ||w|| are written to host_out[2j], host_out[2j+1].
-__global__ void arnoldi_tail_kernel(const double2* d1, const double2* d2, double2* hsum, GivensState st, int m, int j,
-                                    int phase, double* host_out) {
-  __shared__ double2 col[512 + 2];
-  __shared__ double csv[512];
-  __shared__ double2 snv[512];
-  __shared__ int done;
+// Arnoldi step j after both Gram-Schmidt passes: d2 = reduced second-pass
+// partials (h2 in [0, j], |w'|^2 at j+1), h1 in d1. Writes H column j = h1 + h2,
+// ||w''|| = sqrt(|w'|^2 - |h2|^2), rotates the column by the previous Givens
+// rotations, forms the rotation that zeroes H(j+1, j), and writes |g(j+1)|
+// (the residual estimate) and ||w''|| to host_out[2j], host_out[2j+1].
+__global__ __launch_bounds__(kThreads) void gs_tail_kernel(const double2* __restrict__ partial, int nparts,
+                                                           const double2* __restrict__ d1, double2* __restrict__ d2,
+                                                           GivensState st, int m, int j, double* host_out) {
+  __shared__ double2 red[kThreads];
+  __shared__ double2 h2[kMaxRestart + 2];
+  __shared__ double2 col[kMaxRestart + 2];
+  __shared__ double csv[kMaxRestart];
+  __shared__ double2 snv[kMaxRestart];
+  block_reduce_cols(partial, nparts, j + 2, d2, h2, red);
   const int t = threadIdx.x;
-  if (phase == 2 && st.flag[0] == 0) return;
-  if (t == 0) {
-    double hn2;
-    if (phase == 1) {
-      double s = 0.0;
-      for (int i = 0; i <= j; ++i) s += d1[i].x * d1[i].x + d1[i].y * d1[i].y;
-      const double n1 = d1[j + 1].x;
-      hn2 = n1 - s;
-      done = (hn2 > 0.5 * n1) ? 1 : 0;
-      st.flag[0] = done ? 0 : 1;
-    } else {
-      double s = 0.0;
-      for (int i = 0; i <= j; ++i) s += d2[i].x * d2[i].x + d2[i].y * d2[i].y;
-      hn2 = d2[j + 1].x - s;
-      done = 1;
-    }
-    st.hn[0] = sqrt(fmax(hn2, 0.0));
-  }
-  __syncthreads();
-  if (!done) return;
-  // H column j = h1 (+ h2), loaded in parallel with the rotations
   for (int i = t; i <= j; i += blockDim.x) {
-    double2 v = d1[i];
-    if (phase == 2) { v.x += d2[i].x; v.y += d2[i].y; }
-    col[i] = v;
-    hsum[i] = v;
-    if (i < j) { csv[i] = st.cs[i]; snv[i] = st.sn[i]; }
+    const double2 a = d1[i];
+    col[i] = make_double2(a.x + h2[i].x, a.y + h2[i].y);
+    if (i < j) {
+      csv[i] = st.cs[i];
+      snv[i] = st.sn[i];
+    }
   }
   __syncthreads();
   if (t != 0) return;
-  const double hn = st.hn[0];
+  double s2 = 0.0;
+  for (int i = 0; i <= j; ++i) s2 += h2[i].x * h2[i].x + h2[i].y * h2[i].y;
+  const double hn = sqrt(fmax(h2[j + 1].x - s2, 0.0));
+  st.hn[0] = hn;
   col[j + 1] = make_double2(hn, 0.0);
   for (int i = 0; i < j; ++i) {
     const double c = csv[i];
@@ -348,20 +553,105 @@ __global__ void arnoldi_tail_kernel(const double2* d1, const double2* d2, double
   host_out[2 * j + 1] = hn;
 }
 
-// y = R^{-1} g (upper triangular k x k of the rotated H)
-__global__ void trisolve_kernel(GivensState st, int m, int k, double2* y) {
-  if (threadIdx.x != 0) return;
-  for (int i = k - 1; i >= 0; --i) {
-    double2 s = st.g[i];
-    for (int q = i + 1; q < k; ++q) {
-      const double2 r = st.H[(long long)q * (m + 1) + i];
-      s.x -= r.x * y[q].x - r.y * y[q].y;
-      s.y -= r.x * y[q].y + r.y * y[q].x;
-    }
-    const double2 d = st.H[(long long)i * (m + 1) + i];
-    const double den = d.x * d.x + d.y * d.y;
-    y[i] = make_double2((s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den);
+// vout = (w - V h) * (1/s[0]) with h the first nc entries of hg; vout = 0 when
+// s[0] vanishes (breakdown: the column is never used).
+__global__ __launch_bounds__(kThreads) void gs_update_kernel(const double2* __restrict__ V, long long ldv, int nc,
+                                                             const double2* __restrict__ hg,
+                                                             const double2* __restrict__ w,
+                                                             const double* __restrict__ s, double2* __restrict__ vout,
+                                                             int N) {
+  __shared__ double2 hs[kMaxRestart + 1];
+  const int t = threadIdx.x;
+  for (int i = t; i < nc; i += kThreads) hs[i] = hg[i];
+  __syncthreads();
+  const long long n = (long long)blockIdx.x * kThreads + t;
+  if (n >= N) return;
+  double2 a = make_double2(0.0, 0.0);
+  int i = 0;
+  for (; i + 8 <= nc; i += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = V[(i + u) * ldv + n];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cmac(a, v[u], hs[i + u]);
   }
+  for (; i < nc; ++i) cmac(a, V[i * ldv + n], hs[i]);
+  const double d = s ? s[0] : 1.0;
+  const double inv = d > 1e-300 ? 1.0 / d : 0.0;
+  const double2 wv = w[n];
+  vout[n] = make_double2((wv.x - a.x) * inv, (wv.y - a.y) * inv);
+}
+
+// x += V y (k columns)
+__global__ __launch_bounds__(kThreads) void axpy_cols_kernel(const double2* __restrict__ V, long long ldv, int nc,
+                                                             const double2* __restrict__ coef,
+                                                             double2* __restrict__ x, int N) {
+  __shared__ double2 cs[kMaxRestart + 1];
+  const int t = threadIdx.x;
+  for (int i = t; i < nc; i += kThreads) cs[i] = coef[i];
+  __syncthreads();
+  const long long n = (long long)blockIdx.x * kThreads + t;
+  if (n >= N) return;
+  double2 a = make_double2(0.0, 0.0);
+  int i = 0;
+  for (; i + 8 <= nc; i += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = V[(i + u) * ldv + n];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cmac(a, v[u], cs[i + u]);
+  }
+  for (; i < nc; ++i) cmac(a, V[i * ldv + n], cs[i]);
+  double2 o = x[n];
+  o.x += a.x;
+  o.y += a.y;
+  x[n] = o;
+}
+
+// out = b - a (residual); out may alias a
+__global__ void sub_kernel(const double2* __restrict__ b, const double2* a, double2* out, int N) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) {
+    double2 v = a[n];
+    out[n] = make_double2(b[n].x - v.x, b[n].y - v.y);
+  }
+}
+
+// y = R^{-1} g (upper triangular k x k of the rotated H), column-oriented back
+// substitution by one workgroup.
+__global__ __launch_bounds__(kThreads) void trisolve_kernel(GivensState st, int m, int k, double2* y) {
+  __shared__ double2 s[kMaxRestart];
+  __shared__ double2 yi;
+  const int t = threadIdx.x;
+  for (int i = t; i < k; i += blockDim.x) s[i] = st.g[i];
+  __syncthreads();
+  for (int i = k - 1; i >= 0; --i) {
+    const double2* Rc = st.H + (long long)i * (m + 1);
+    if (t == 0) {
+      const double2 d = Rc[i];
+      const double den = d.x * d.x + d.y * d.y;
+      const double2 v = s[i];
+      const double2 r = make_double2((v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den);
+      y[i] = r;
+      yi = r;
+    }
+    __syncthreads();
+    const double2 r = yi;
+    for (int q = t; q < i; q += blockDim.x) {
+      const double2 a = Rc[q];
+      s[q].x -= a.x * r.x - a.y * r.y;
+      s[q].y -= a.x * r.y + a.y * r.x;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void scale_kernel(const double2* __restrict__ w, const double* __restrict__ s, double2* __restrict__ v,
+                             int N) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double d = s[0];
+  const double inv = d > 1e-300 ? 1.0 / d : 0.0;
+  if (n < N) v[n] = make_double2(w[n].x * inv, w[n].y * inv);
 }
 
 #define HIP_OK(expr)                                                                   \
@@ -390,11 +680,13 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (rank > 64) return "nChebNodes^2 > 64 is not supported by the near-field kernel";
   HIP_OK(hipGetDevice(&device));
   HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-  // the near kernel stages one rank x rank table plus at least one leaf
-  if (int64_t(sizeof(double2)) * (rank * rank + rank) > max_lds)
-    return "near-field table does not fit in LDS: nChebNodes too large for this device";
-  if (int64_t(sizeof(double2)) * (rank * rank + std::min(kNearLeaves, 3 * kThreads / rank) * rank) > 65536)
-    HIP_OK(hipFuncSetAttribute((const void*)near_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  {
+    const int nc = int(std::lround(std::sqrt(double(rank))));
+    if (nc * nc != rank || nc < 2 || nc > 8) return "rank must be nChebNodes^2 with nChebNodes in [2, 8]";
+    // the near kernel holds one rank x rank table and 2*(256/rank) leaf vectors in LDS
+    if (int64_t(sizeof(double2)) * (rank * rank + 2 * (kThreads / rank) * rank) > max_lds)
+      return "near-field table does not fit in LDS: nChebNodes too large for this device";
+  }
   HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   HIP_OK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -425,9 +717,21 @@ std::string DeviceOperator::upload(const Plan& plan) {
     srco.insert(srco.end(), ph.src_off.begin(), ph.src_off.end());
     const int idx = int(passes.size());
     if (ph.name.rfind("up_", 0) == 0) up.push_back(idx);
-    else if (ph.name == "leaf_m2l") leaf_m2l.push_back(idx);
+    else if (ph.name == "leaf_m2l") leaf_m2l.push_back(idx), dp.role = 1;
     else if (ph.name == "leaf_l2l") leaf_l2l.push_back(idx);
     else down.push_back(idx);
+    dp.wave = dp.ndest > 0 && ph.entries <= int64_t(kWaveEntries) * dp.ndest;
+    for (size_t q = 0; q < ph.rows.size(); ++q) {
+      // a destination lies wholly in the output vector or wholly in the charge/potential slots
+      const int64_t d0 = ph.dst[q], d1 = d0 + ph.rows[q];
+      if (!((d0 >= N && d1 <= 2 * int64_t(N)) || d0 >= 2 * int64_t(N)) || d1 > n_state)
+        return "plan destination straddles the state regions";
+    }
+    for (int64_t q = 0; q < int64_t(ph.src_off.size()); ++q)
+      for (int c = 0; c < ph.ncol[q]; ++c) {
+        const int si = plan.src[ph.src_off[q] + c];
+        if (si < 0 || si >= n_state || (si >= N && si < 2 * N)) return "plan source index out of range";
+      }
     passes.push_back(dp);
   }
   if (!(e = upload_vec(rows, &d_rows)).empty()) return e;
@@ -470,47 +774,80 @@ void DeviceOperator::release() {
   aux = stream = nullptr;
 }
 
-static void launch_pass(const DeviceOperator& o, const DevPass& p, hipStream_t s) {
+static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2* x, double2* y, hipStream_t s) {
   const int64_t b = p.dest_base;
-  const dvec2* ops = reinterpret_cast<const dvec2*>(o.ops);
+  typedef cx<double>::v V;
+  const V* ops = reinterpret_cast<const V*>(o.ops);
+  if (p.wave) {
+    const dim3 grid((p.ndest + 3) / 4);
+    if (p.kind == 0)
+      hipLaunchKernelGGL((pass_wave_kernel<false, double>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
+                         o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+    else
+      hipLaunchKernelGGL((pass_wave_kernel<true, double>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
+                         o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+    return;
+  }
+  const dim3 grid(p.ndest);
   if (p.kind == 0)
-    hipLaunchKernelGGL(pass_kernel<false>, dim3(p.ndest), dim3(kThreads), 0, s, ops, o.src, o.state,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+    hipLaunchKernelGGL((pass_block_kernel<false, double, 0>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
+                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+  else if (p.role == 1)
+    hipLaunchKernelGGL((pass_block_kernel<true, double, 1>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
+                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
   else
-    hipLaunchKernelGGL(pass_kernel<true>, dim3(p.ndest), dim3(kThreads), 0, s, ops, o.src, o.state,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+    hipLaunchKernelGGL((pass_block_kernel<true, double, 0>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
+                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
 }
 
-static void launch_near(const DeviceOperator& o, hipStream_t s) {
-  const int fit = int(o.max_lds / int(sizeof(double2))) - o.rank * o.rank;
-  const int lpw = std::max(1, std::min({kNearLeaves, 3 * kThreads / o.rank, fit / o.rank}));
-  const size_t lds = sizeof(double2) * size_t(o.rank * o.rank + lpw * o.rank);
-  hipLaunchKernelGGL(near_kernel, dim3((o.n_leaves + lpw - 1) / lpw), dim3(kThreads), lds, s, o.T, o.nbr, o.qk2,
-                     o.state, o.N, o.n_leaves, o.rank, lpw, o.find_phi ? 1 : 0);
+template <int RK>
+static void launch_near_rk(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
+  constexpr int LB = 2 * (kThreads / RK);
+  hipLaunchKernelGGL((near_kernel<double, RK>), dim3((o.n_leaves + LB - 1) / LB), dim3(kThreads), 0, s, o.T, o.nbr,
+                     o.qk2, x, y, o.n_leaves, o.find_phi ? 1 : 0);
+}
+
+static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
+  switch (o.rank) {
+    case 4: launch_near_rk<4>(o, x, y, s); break;
+    case 9: launch_near_rk<9>(o, x, y, s); break;
+    case 16: launch_near_rk<16>(o, x, y, s); break;
+    case 25: launch_near_rk<25>(o, x, y, s); break;
+    case 36: launch_near_rk<36>(o, x, y, s); break;
+    case 49: launch_near_rk<49>(o, x, y, s); break;
+    case 64: launch_near_rk<64>(o, x, y, s); break;
+    default: break;  // rejected at upload
+  }
 }
 
 void DeviceOperator::matvec(const double2* x, double2* y) {
-  hipMemcpyAsync(state, x, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
+  if (x == y) {
+    // in-place call: the passes read x while the output is written
+    hipMemcpyAsync(state, x, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
+    x = state;
+  }
   // the first upward pass is the leaf level (the only one the leaf M2L reads)
   size_t u = 0;
-  if (!up.empty()) launch_pass(*this, passes[up[0]], stream), u = 1;
+  if (!up.empty()) launch_pass(*this, passes[up[0]], x, y, stream), u = 1;
   hipEventRecord(ev_fork, stream);
   hipStreamWaitEvent(aux, ev_fork, 0);
-  launch_near(*this, aux);
-  for (int p : leaf_m2l) launch_pass(*this, passes[p], aux);
+  launch_near(*this, x, y, aux);
+  for (int p : leaf_m2l) launch_pass(*this, passes[p], x, y, aux);
   hipEventRecord(ev_join, aux);
-  for (; u < up.size(); ++u) launch_pass(*this, passes[up[u]], stream);
-  for (int p : down) launch_pass(*this, passes[p], stream);
+  for (; u < up.size(); ++u) launch_pass(*this, passes[up[u]], x, y, stream);
+  for (int p : down) launch_pass(*this, passes[p], x, y, stream);
   hipStreamWaitEvent(stream, ev_join, 0);
-  for (int p : leaf_l2l) launch_pass(*this, passes[p], stream);
-  hipMemcpyAsync(y, state + N, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
+  for (int p : leaf_l2l) launch_pass(*this, passes[p], x, y, stream);
 }
 
 double DeviceOperator::time_matvec(int reps) {
   double2 *x = nullptr, *y = nullptr;
   if (hipMalloc((void**)&x, sizeof(double2) * size_t(N)) != hipSuccess) return -1;
-  if (hipMalloc((void**)&y, sizeof(double2) * size_t(N)) != hipSuccess) { hipFree(x); return -1; }
-  hipMemcpyAsync(x, state, sizeof(double2) * size_t(N), hipMemcpyDeviceToDevice, stream);
+  if (hipMalloc((void**)&y, sizeof(double2) * size_t(N)) != hipSuccess) {
+    hipFree(x);
+    return -1;
+  }
+  hipMemsetAsync(x, 0, sizeof(double2) * size_t(N), stream);
   hipEvent_t a, b;
   hipEventCreate(&a);
   hipEventCreate(&b);
@@ -530,7 +867,10 @@ double DeviceOperator::time_matvec(int reps) {
 
 std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes,
                                     std::vector<std::string>& names) {
-  // serialised on the main stream: upward, near field, downward, leaf M2L, leaf L2L
+  // serialised on the main stream: upward, near field, downward, leaf M2L, leaf L2L;
+  // input and output are the state's own x and y regions
+  const double2* x = state;
+  double2* y = state + N;
   std::vector<int> order(up.begin(), up.end());
   order.push_back(-1);
   order.insert(order.end(), down.begin(), down.end());
@@ -545,8 +885,8 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
   for (int r = 0; r < reps; ++r) {
     HIP_OK(hipEventRecord(ev[0], stream));
     for (int q = 0; q < np; ++q) {
-      if (order[q] < 0) launch_near(*this, stream);
-      else launch_pass(*this, passes[order[q]], stream);
+      if (order[q] < 0) launch_near(*this, x, y, stream);
+      else launch_pass(*this, passes[order[q]], x, y, stream);
       HIP_OK(hipEventRecord(ev[q + 1], stream));
     }
     HIP_OK(hipEventSynchronize(ev[np]));
@@ -557,8 +897,13 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
     }
   }
   for (int q = 0; q < np; ++q) {
-    if (order[q] < 0) { names[q] = "near_field"; bytes[q] = near_bytes; }
-    else { names[q] = passes[order[q]].name; bytes[q] = passes[order[q]].bytes(); }
+    if (order[q] < 0) {
+      names[q] = "near_field";
+      bytes[q] = near_bytes;
+    } else {
+      names[q] = passes[order[q]].name;
+      bytes[q] = passes[order[q]].bytes();
+    }
   }
   for (auto& e : ev) hipEventDestroy(e);
   return "";
@@ -571,12 +916,15 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   auto t0 = std::chrono::steady_clock::now();
   const int N = op.N;
   hipStream_t s = op.stream;
-  const int nb_dot = (N + kThreads * kDotRows - 1) / (kThreads * kDotRows);
-  const int nparts = nb_dot * (kThreads / 64);
+  if (m < 1 || m > kMaxRestart) {
+    res.status = LSG_ERR_ARG;
+    res.error = "restart length must be in [1, 500]";
+    return res;
+  }
+  const int nrb = (N + kGsRows - 1) / kGsRows;   // row blocks of the dot kernels
   const int nb_vec = (N + kThreads - 1) / kThreads;
   const long long ldv = N;
-  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *hsum = nullptr,
-          *y = nullptr;
+  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *y = nullptr;
   GivensState st{};
   double* hostbuf = nullptr;
   std::vector<void*> owned;
@@ -586,25 +934,27 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     return e == hipSuccess;
   };
   bool ok = alloc((void**)&V, sizeof(double2) * size_t(N) * (m + 1)) && alloc((void**)&w, sizeof(double2) * size_t(N)) &&
-            alloc((void**)&partial, sizeof(double2) * size_t(nparts) * (m + 2)) &&
+            alloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m + 2)) &&
             alloc((void**)&d1, sizeof(double2) * (m + 2)) && alloc((void**)&d2, sizeof(double2) * (m + 2)) &&
-            alloc((void**)&hsum, sizeof(double2) * (m + 2)) && alloc((void**)&y, sizeof(double2) * (m + 1)) &&
+            alloc((void**)&y, sizeof(double2) * (m + 1)) &&
             alloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m) && alloc((void**)&st.cs, sizeof(double) * (m + 1)) &&
             alloc((void**)&st.sn, sizeof(double2) * (m + 1)) && alloc((void**)&st.g, sizeof(double2) * (m + 2)) &&
-            alloc((void**)&st.hn, sizeof(double)) && alloc((void**)&st.beta, sizeof(double)) &&
-            alloc((void**)&st.flag, sizeof(int));
+            alloc((void**)&st.hn, sizeof(double)) && alloc((void**)&st.beta, sizeof(double));
   if (!ok || hipHostMalloc((void**)&hostbuf, sizeof(double) * 2 * (m + 2), hipHostMallocDefault) != hipSuccess) {
     for (void* p : owned) hipFree(p);
     res.status = LSG_ERR_DEVICE;
     res.error = "GMRES workspace allocation failed";
     return res;
   }
-  auto dotw = [&](const double2* Vb, int ncols, const double2* vec, double2* out, const int* flag) {
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nb_dot), dim3(kThreads), 0, s, Vb, ldv, ncols, vec, N, partial, flag);
-    hipLaunchKernelGGL(dot_reduce_kernel, dim3(ncols + 1), dim3(kThreads), 0, s, partial, nparts, ncols + 1, out, flag);
+  // partials of V[:, 0:nc]^H vec (+ |vec|^2 as column nc when norm)
+  auto dot = [&](int nc, const double2* vec, bool norm) {
+    const int ncx = nc + (norm ? 1 : 0);
+    hipLaunchKernelGGL(gs_dot_kernel, dim3(nrb, (ncx + kGsCols - 1) / kGsCols), dim3(kThreads), 0, s, V, ldv, nc,
+                       ncx, vec, N, partial);
   };
   // ||b||
-  dotw(V, 0, b_dev, d1, nullptr);
+  dot(0, b_dev, true);
+  hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, 1, d1);
   double2 bn2;
   hipMemcpyAsync(&bn2, d1, sizeof(double2), hipMemcpyDeviceToHost, s);
   hipError_t e = hipStreamSynchronize(s);
@@ -625,13 +975,24 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   for (int cyc = 0; res.status == 0 && cyc <= max_restarts; ++cyc) {
     op.matvec(x_dev, w);
     hipLaunchKernelGGL(sub_kernel, dim3(nb_vec), dim3(kThreads), 0, s, b_dev, w, w, N);
-    dotw(V, 0, w, d1, nullptr);
-    hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(64), 0, s, d1, st, m, hostbuf);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) { res.status = LSG_ERR_DEVICE; res.error = hipGetErrorString(e); break; }
+    dot(0, w, true);
+    hipLaunchKernelGGL(gs_cycle_init_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, st, m, hostbuf);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+      res.status = LSG_ERR_DEVICE;
+      res.error = hipGetErrorString(e);
+      break;
+    }
     const double beta = hostbuf[0];
     res.rel_residual = beta / bnorm;
-    if (!std::isfinite(beta)) { res.status = LSG_ERR_NUMERIC; res.error = "residual is NaN/Inf"; break; }
-    if (!ref_mode && beta / bnorm <= tol) { converged = true; break; }
+    if (!std::isfinite(beta)) {
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "residual is NaN/Inf";
+      break;
+    }
+    if (!ref_mode && beta / bnorm <= tol) {
+      converged = true;
+      break;
+    }
     if (cyc == max_restarts || beta == 0.0) break;
     hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.beta, V, N);
     int k = 0;
@@ -644,39 +1005,53 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
       for (int jj = j; jj < j + batch; ++jj) {
         op.matvec(V + (size_t)jj * N, w);
         const int nc = jj + 1;
-        dotw(V, nc, w, d1, nullptr);
-        hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d1, -1.0, w, N,
-                           (const int*)nullptr);
-        hipLaunchKernelGGL(arnoldi_tail_kernel, dim3(1), dim3(kThreads), 0, s, d1, d2, hsum, st, m, jj, 1, hostbuf);
-        dotw(V, nc, w, d2, st.flag);
-        hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d2, -1.0, w, N,
-                           (const int*)st.flag);
-        hipLaunchKernelGGL(arnoldi_tail_kernel, dim3(1), dim3(kThreads), 0, s, d1, d2, hsum, st, m, jj, 2, hostbuf);
-        hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.hn, V + (size_t)(jj + 1) * N, N);
+        dot(nc, w, false);
+        hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, nc, d1);
+        hipLaunchKernelGGL(gs_proj_dot_kernel, dim3(nrb), dim3(kThreads), 0, s, V, ldv, nc, d1, w, N, partial);
+        hipLaunchKernelGGL(gs_tail_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, d1, d2, st, m, jj, hostbuf);
+        hipLaunchKernelGGL(gs_update_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d2, w, st.hn,
+                           V + (size_t)(jj + 1) * N, N);
       }
-      if ((e = hipStreamSynchronize(s)) != hipSuccess) { res.status = LSG_ERR_DEVICE; res.error = hipGetErrorString(e); break; }
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+        res.status = LSG_ERR_DEVICE;
+        res.error = hipGetErrorString(e);
+        break;
+      }
       for (int jj = j; jj < j + batch; ++jj) {
         const double est = hostbuf[2 * jj], hn = hostbuf[2 * jj + 1];
         ++total_it;
         k = jj + 1;
         res.est_residual = est / bnorm;
         last_est = est / bnorm;
-        if (!std::isfinite(est)) { res.status = LSG_ERR_NUMERIC; res.error = "Arnoldi produced NaN/Inf"; stop = true; break; }
-        if (std::fabs(hn) < 1e-8) { stop = true; break; }   // breakdown exit, GMRES/GMRES.cpp:284-288
-        if (!ref_mode && est / bnorm <= 0.5 * tol) { stop = true; break; }
+        if (!std::isfinite(est)) {
+          res.status = LSG_ERR_NUMERIC;
+          res.error = "Arnoldi produced NaN/Inf";
+          stop = true;
+          break;
+        }
+        if (std::fabs(hn) < 1e-8) {  // breakdown exit, GMRES/GMRES.cpp:284-288
+          stop = true;
+          break;
+        }
+        if (!ref_mode && est / bnorm <= 0.5 * tol) {
+          stop = true;
+          break;
+        }
       }
       j += batch;
     }
     if (res.status != 0) break;
     // x += V y, y = R^-1 g
-    hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(64), 0, s, st, m, k, y);
-    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, k, y, 1.0, x_dev, N,
-                       (const int*)nullptr);
+    hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(kThreads), 0, s, st, m, k, y);
+    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, k, y, x_dev, N);
     res.restarts = cyc + 1;
   }
   if (res.status == 0) {
     converged = !ref_mode ? (res.rel_residual <= tol) : true;
-    if (!ref_mode && !converged) { res.status = LSG_ERR_NOCONV; res.error = "GMRES did not reach tol"; }
+    if (!ref_mode && !converged) {
+      res.status = LSG_ERR_NOCONV;
+      res.error = "GMRES did not reach tol";
+    }
   }
   res.iterations = total_it;
   res.converged = converged ? 1 : 0;

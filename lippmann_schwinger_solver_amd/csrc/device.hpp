@@ -11,6 +11,8 @@ namespace lsg {
 struct DevPass {
   std::string name;
   int kind = 0;        // 0 store, 1 accumulate
+  int role = 0;        // 1: leaf M2L
+  bool wave = false;   // one wavefront (not one workgroup) per destination
   int ndest = 0;
   int64_t dest_base = 0;  // index into the concatenated destination arrays
   int64_t entries = 0, n_src = 0, n_dst = 0;
@@ -20,7 +22,8 @@ struct DevPass {
 // One matvec = upward passes -> downward passes -> leaf L2L on the main
 // stream, while the near field and the leaf M2L (which need only x and the
 // leaf-level outgoing charges) run on a second stream after the first upward
-// pass and join before the leaf L2L.
+// pass and join before the leaf L2L. x is read and y written in place; the
+// state's own x and y regions are used only by profile() and in-place calls.
 struct DeviceOperator {
   hipStream_t stream = nullptr;   // main stream (all public calls are ordered on it)
   hipStream_t aux = nullptr;      // second stream, same default priority
@@ -47,7 +50,7 @@ struct DeviceOperator {
 
   std::string upload(const Plan& plan);
   void release();
-  // y = A x on the main stream; x, y device pointers of N complex128
+  // y = A x on the main stream; x, y device pointers of N complex128 (may alias)
   void matvec(const double2* x, double2* y);
   // event-timed per-pass averages over reps matvecs (passes run serialised)
   std::string profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes, std::vector<std::string>& names);
@@ -55,8 +58,8 @@ struct DeviceOperator {
   double time_matvec(int reps);
 };
 
-// Restarted GMRES (CGS with DGKS-triggered reorthogonalisation, complex
-// Givens) with device-resident Krylov basis, Hessenberg and rotations.
+// Restarted GMRES (classical Gram-Schmidt with one reorthogonalisation,
+// complex Givens) with device-resident Krylov basis, Hessenberg and rotations.
 struct GmresResult {
   int iterations = 0, restarts = 0, converged = 0;
   double rel_residual = 0, est_residual = 0, seconds = 0;
