@@ -102,7 +102,22 @@ typedef struct lsg_stats {
   double rel_residual;    /* true ||b - A x|| / ||b|| in FP64 */
   double est_residual;    /* Givens residual estimate at exit */
   double seconds;         /* wall time */
+  int refinements;        /* FP64 refinement steps (mode LSG_SOLVE_FP32_IR), else 0 */
 } lsg_stats;
+
+/* Solver selection for lsg_gmres_ex. */
+#define LSG_SOLVE_FP64 0     /* GMRES with the FP64 operator (lsg_gmres) */
+#define LSG_SOLVE_FP32_IR 1  /* FP64 iterative refinement around GMRES with the FP32 operator */
+
+typedef struct lsg_solve_opts {
+  int mode;               /* LSG_SOLVE_FP64 or LSG_SOLVE_FP32_IR */
+  int restart_len;        /* Krylov dimension per cycle, 1..500 */
+  int max_restarts;       /* cycle cap (per inner solve in FP32_IR mode) */
+  double tol;             /* target true FP64 relative residual (> 0 for FP32_IR) */
+  double inner_tol;       /* FP32_IR: floor of the inner relative residual (e.g. 1e-6) */
+  int max_outer;          /* FP32_IR: refinement cap */
+  int device_ptrs;        /* 1: b and x are device addresses of N complex128 */
+} lsg_solve_opts;
 
 typedef struct lsg_ctx lsg_ctx;
 
@@ -159,6 +174,23 @@ int lsg_time_matvec(lsg_ctx* ctx, int reps, double* ms);
  * Arnoldi steps (GMRES/GMRES.cpp:257-319, with its 1e-8 breakdown exit). */
 int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len,
               int max_restarts, double tol, lsg_stats* stats);
+
+/* Mixed-precision variant of lsg_matvec (no reference counterpart): the
+ * operators and the accumulation inside every pass are FP32 (interleaved
+ * re/im), x and y stay FP64; the FP32 copy of the operators is made on the
+ * device on first use. */
+int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
+/* lsg_time_matvec for the FP32 operator. */
+int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms);
+
+/* GMRES with solver options: lsg_gmres (mode LSG_SOLVE_FP64), or FP64
+ * iterative refinement (mode LSG_SOLVE_FP32_IR): r = b - A x with the FP64
+ * operator, d ~ A32^-1 r by GMRES with the FP32 operator (Gram-Schmidt and
+ * Givens in FP64) to relative residual max(inner_tol, tol*||b||/(2||r||)),
+ * x += d, until ||b - A x|| <= tol ||b|| in FP64. The refinement follows the
+ * reference's GMRES cycle semantics (GMRES/GMRES.cpp:257-319) inside each
+ * inner solve; the refinement loop itself has no reference counterpart. */
+int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* opts, lsg_stats* stats);
 
 #ifdef __cplusplus
 }

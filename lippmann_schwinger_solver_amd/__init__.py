@@ -58,7 +58,17 @@ class lsg_info(C.Structure):
 
 class lsg_stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("restarts", C.c_int), ("converged", C.c_int),
-                ("rel_residual", C.c_double), ("est_residual", C.c_double), ("seconds", C.c_double)]
+                ("rel_residual", C.c_double), ("est_residual", C.c_double), ("seconds", C.c_double),
+                ("refinements", C.c_int)]
+
+
+LSG_SOLVE_FP64 = 0
+LSG_SOLVE_FP32_IR = 1
+
+
+class lsg_solve_opts(C.Structure):
+    _fields_ = [("mode", C.c_int), ("restart_len", C.c_int), ("max_restarts", C.c_int), ("tol", C.c_double),
+                ("inner_tol", C.c_double), ("max_outer", C.c_int), ("device_ptrs", C.c_int)]
 
 
 # (name, restype, argtypes) of every symbol in include/lsg.h and include/lsg_host.h
@@ -83,6 +93,9 @@ SYMBOLS = [
     ("lsg_profile_passes", C.c_int, [_P, C.c_int, _D, C.POINTER(C.c_int64), C.c_int]),
     ("lsg_time_matvec", C.c_int, [_P, C.c_int, _D]),
     ("lsg_gmres", C.c_int, [_P, _D, _D, C.c_int, C.c_int, C.c_double, C.POINTER(lsg_stats)]),
+    ("lsg_matvec_fp32", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
+    ("lsg_time_matvec_fp32", C.c_int, [_P, C.c_int, _D]),
+    ("lsg_gmres_ex", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.POINTER(lsg_stats)]),
     ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_build_tree", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_destroy", None, [_P]),
@@ -336,6 +349,44 @@ class DFMM:
             self._check(rc)
         stats["status"] = rc
         return x, stats
+
+    def gmres_ex(self, b, x0=None, mode="fp64", restart_len=50, max_restarts=20, tol=1e-8, inner_tol=1e-6,
+                 max_outer=10, device_ptrs=None):
+        """GMRES with solver options. mode "fp64" (= gmres) or "fp32ir" (FP64
+        iterative refinement around FP32-operator GMRES). With device_ptrs =
+        (b_dev, x_dev) (integer device addresses) nothing is copied and b, x0
+        are ignored; returns (x or None, stats)."""
+        o = lsg_solve_opts(mode={"fp64": LSG_SOLVE_FP64, "fp32ir": LSG_SOLVE_FP32_IR}[mode], restart_len=restart_len,
+                           max_restarts=max_restarts, tol=tol, inner_tol=inner_tol, max_outer=max_outer,
+                           device_ptrs=1 if device_ptrs else 0)
+        st = lsg_stats()
+        if device_ptrs:
+            x = None
+            rc = self._lib.lsg_gmres_ex(self._ctx, C.c_void_p(device_ptrs[0]), C.c_void_p(device_ptrs[1]), C.byref(o),
+                                        C.byref(st))
+        else:
+            b = np.ascontiguousarray(b, dtype=np.complex128)
+            x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+            rc = self._lib.lsg_gmres_ex(self._ctx, b.ctypes.data_as(_P), x.ctypes.data_as(_P), C.byref(o),
+                                        C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats["status"] = rc
+        return x, stats
+
+    def MatVecProduct_fp32(self, x):
+        """y = A32 x: FP32 operators and FP32 accumulation inside each pass, FP64 x and y."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        out = np.empty_like(x)
+        self._check(self._lib.lsg_matvec_fp32(self._ctx, _dptr(x.view(np.float64)), _dptr(out.view(np.float64)),
+                                              1, self.N))
+        return out
+
+    def time_matvec_fp32(self, reps=20):
+        ms = C.c_double()
+        self._check(self._lib.lsg_time_matvec_fp32(self._ctx, reps, C.byref(ms)))
+        return ms.value
 
     def profile_passes(self, reps=5):
         ms = np.zeros(64)

@@ -296,28 +296,72 @@ int lsg_time_matvec(lsg_ctx* ctx, int reps, double* ms) {
   return LSG_OK;
 }
 
-int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max_restarts, double tol,
-              lsg_stats* stats) {
+int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms) {
+  if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
+  std::string e = ctx->dev.enable_fp32();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  *ms = ctx->dev.time_matvec(reps, true);
+  if (*ms < 0) return ctx->set(LSG_ERR_DEVICE, "time_matvec allocation failed");
+  return LSG_OK;
+}
+
+int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld) {
   if (!ctx) return LSG_ERR_ARG;
-  if (!b || !x || restart_len < 1 || restart_len > 500 || max_restarts < 1)
-    return ctx->set(LSG_ERR_ARG, "bad gmres arguments (restart_len in [1,500], max_restarts >= 1)");
   const int64_t N = ctx->hb.plan.N;
-  for (int64_t i = 0; i < 2 * N; ++i)
-    if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+  if (!x || !y || nrhs < 1 || ld < N) return ctx->set(LSG_ERR_ARG, "bad matvec arguments");
+  std::string e = ctx->dev.enable_fp32();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   const size_t nb = sizeof(double2) * size_t(N);
-  double2 *db = nullptr, *dxs = nullptr;
-  if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
-    if (db) hipFree(db);
-    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+  for (int r = 0; r < nrhs; ++r) {
+    const double* xr = x + 2 * ld * r;
+    for (int64_t i = 0; i < 2 * N; ++i)
+      if (!std::isfinite(xr[i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    hipStream_t s = ctx->dev.stream;
+    if (hipMemcpyAsync(ctx->dx, xr, nb, hipMemcpyHostToDevice, s) != hipSuccess)
+      return ctx->set(LSG_ERR_DEVICE, "copy in failed");
+    ctx->dev.matvec32(ctx->dx, ctx->dy);
+    if (hipMemcpyAsync(y + 2 * ld * r, ctx->dy, nb, hipMemcpyDeviceToHost, s) != hipSuccess)
+      return ctx->set(LSG_ERR_DEVICE, "copy out failed");
+    hipError_t he = hipStreamSynchronize(s);
+    if (he != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("matvec: ") + hipGetErrorString(he));
   }
+  return LSG_OK;
+}
+
+int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* o, lsg_stats* stats) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (!o || !b || !x || o->restart_len < 1 || o->restart_len > 500 || o->max_restarts < 1)
+    return ctx->set(LSG_ERR_ARG, "bad gmres arguments (restart_len in [1,500], max_restarts >= 1)");
+  if (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR) return ctx->set(LSG_ERR_ARG, "unknown solver mode");
+  if (o->mode == LSG_SOLVE_FP32_IR && (!(o->tol > 0) || !(o->inner_tol > 0) || o->max_outer < 1))
+    return ctx->set(LSG_ERR_ARG, "FP32_IR needs tol > 0, inner_tol > 0, max_outer >= 1");
+  const int64_t N = ctx->hb.plan.N;
+  const size_t nb = sizeof(double2) * size_t(N);
   hipStream_t s = ctx->dev.stream;
-  hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
-  GmresResult r = device_gmres(ctx->dev, db, dxs, restart_len, max_restarts, tol);
-  hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
-  hipStreamSynchronize(s);
-  hipFree(db);
-  hipFree(dxs);
+  double2 *db = nullptr, *dxs = nullptr;
+  if (o->device_ptrs) {
+    db = const_cast<double2*>(reinterpret_cast<const double2*>(b));
+    dxs = reinterpret_cast<double2*>(x);
+  } else {
+    for (int64_t i = 0; i < 2 * N; ++i)
+      if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+    if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
+      if (db) hipFree(db);
+      return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+    }
+    hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
+  }
+  GmresResult r = o->mode == LSG_SOLVE_FP64
+                      ? device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol)
+                      : device_gmres_ir(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol, o->inner_tol,
+                                        o->max_outer);
+  if (!o->device_ptrs) {
+    hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    hipFree(db);
+    hipFree(dxs);
+  }
   if (stats) {
     stats->iterations = r.iterations;
     stats->restarts = r.restarts;
@@ -325,9 +369,20 @@ int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max
     stats->rel_residual = r.rel_residual;
     stats->est_residual = r.est_residual;
     stats->seconds = r.seconds;
+    stats->refinements = r.refinements;
   }
   if (r.status != 0) return ctx->set(r.status, r.error);
   return LSG_OK;
+}
+
+int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max_restarts, double tol,
+              lsg_stats* stats) {
+  lsg_solve_opts o{};
+  o.mode = LSG_SOLVE_FP64;
+  o.restart_len = restart_len;
+  o.max_restarts = max_restarts;
+  o.tol = tol;
+  return lsg_gmres_ex(ctx, b, x, &o, stats);
 }
 
 // ----------------------------------------------------------- host-only API

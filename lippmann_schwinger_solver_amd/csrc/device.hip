@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 namespace lsg {
 
@@ -49,16 +50,46 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Source value of a pass: state indices below N are the input vector x,
-// which is read in place (the state's x region is not filled).
-template <class C>
-__device__ __forceinline__ C load_src(const C* __restrict__ x, const C* __restrict__ state, int i, int N) {
-  return i < N ? x[i] : state[i];
+// Source value of a pass: state indices below N are the input vector x (FP64,
+// read in place and rounded to the pass precision; the state's x region is not
+// filled).
+template <class C, class XC>
+__device__ __forceinline__ C load_src(const XC* __restrict__ x, const C* __restrict__ state, int i, int N) {
+  C v;
+  if (i < N) {
+    const XC a = x[i];
+    v.x = a.x;
+    v.y = a.y;
+  } else {
+    v = state[i];
+  }
+  return v;
 }
-// Destination of a pass: [N, 2N) is the output vector y, written in place.
-template <class C>
-__device__ __forceinline__ C* dest_ptr(C* state, C* y, int d, int N) {
-  return (d >= N && d < 2 * N) ? y + (d - N) : state + d;
+
+// Write (or add) s to element r of destination d: [N, 2N) is the output
+// vector y (FP64, written in place), everything else the pass-precision state.
+template <bool ACC, class C, class XC>
+__device__ __forceinline__ void put_dst(C* state, XC* y, int d, int N, int r, C s) {
+  if (d >= N && d < 2 * N) {
+    XC* Y = y + (d - N) + r;
+    XC o;
+    if (ACC) {
+      o = *Y;
+      o.x += s.x;
+      o.y += s.y;
+    } else {
+      o.x = s.x;
+      o.y = s.y;
+    }
+    *Y = o;
+  } else {
+    C* Y = state + d + r;
+    if (ACC) {
+      s.x += Y->x;
+      s.y += Y->y;
+    }
+    *Y = s;
+  }
 }
 
 // One workgroup per destination: Y[0:R] (=|+=) A (R x C, col-major) * src[0..C).
@@ -69,7 +100,7 @@ __device__ __forceinline__ C* dest_ptr(C* state, C* y, int d, int N) {
 template <bool ACC, class R, int ROLE>
 __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
-    const typename cx<R>::t* __restrict__ x, typename cx<R>::t* y, int N, const int* __restrict__ rows,
+    const double2* __restrict__ x, double2* y, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
     const long long* __restrict__ src_off) {
   typedef typename cx<R>::t C;
@@ -80,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
-  C* Y = dest_ptr(state, y, dst[d], N);
+  const int dd = dst[d];
   const int t = threadIdx.x;
   for (int r0 = 0; r0 < Rw; r0 += kThreads) {
     const int Rt = min(kThreads, Rw - r0);
@@ -116,11 +147,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
         s.x += red[p * Rt + t].x;
         s.y += red[p * Rt + t].y;
       }
-      if (ACC) {
-        s.x += Y[r0 + t].x;
-        s.y += Y[r0 + t].y;
-      }
-      Y[r0 + t] = s;
+      put_dst<ACC>(state, y, dd, N, r0 + t, s);
     }
     __syncthreads();
   }
@@ -133,7 +160,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
 template <bool ACC, class R>
 __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
-    const typename cx<R>::t* __restrict__ x, typename cx<R>::t* y, int N, const int* __restrict__ rows,
+    const double2* __restrict__ x, double2* y, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
     const long long* __restrict__ src_off, int ndest) {
   typedef typename cx<R>::t C;
@@ -145,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
-  C* Y = dest_ptr(state, y, dst[d], N);
+  const int dd = dst[d];
   C* rw = red + wv * 64;
   for (int r0 = 0; r0 < Rw; r0 += 64) {
     const int Rt = min(64, Rw - r0);
@@ -189,11 +216,7 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
         s.x += rw[p * Rt + lane].x;
         s.y += rw[p * Rt + lane].y;
       }
-      if (ACC) {
-        s.x += Y[r0 + lane].x;
-        s.y += Y[r0 + lane].y;
-      }
-      Y[r0 + lane] = s;
+      put_dst<ACC>(state, y, dd, N, r0 + lane, s);
     }
     wave_sync();
   }
@@ -207,10 +230,9 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
 // current slot is being applied.
 template <class R, int RK>
 __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t* __restrict__ T,
-                                                        const int* __restrict__ nbr, const R* __restrict__ qk2,
-                                                        const typename cx<R>::t* __restrict__ x,
-                                                        typename cx<R>::t* __restrict__ y, int n_leaves,
-                                                        int find_phi) {
+                                                        const int* __restrict__ nbr, const double* __restrict__ qk2,
+                                                        const double2* __restrict__ x, double2* __restrict__ y,
+                                                        int n_leaves, int find_phi) {
   typedef typename cx<R>::t C;
   constexpr int LP = kThreads / RK;
   constexpr int LB = 2 * LP;
@@ -234,7 +256,13 @@ __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t*
       const int l = idx / RK, c = idx - l * RK;                               \
       const int leaf = leaf0 + l;                                             \
       const int nb = (idx < LB * RK && leaf < n_leaves) ? nbr[leaf * 9 + (slot)] : -1; \
-      xr[k] = nb >= 0 ? x[(size_t)nb * RK + c] : C{};                         \
+      if (nb >= 0) {                                                          \
+        const double2 xv = x[(size_t)nb * RK + c];                            \
+        xr[k].x = xv.x;                                                       \
+        xr[k].y = xv.y;                                                       \
+      } else {                                                                \
+        xr[k] = C{};                                                          \
+      }                                                                       \
     }                                                                         \
   }
   LSG_NEAR_FETCH(0)
@@ -272,10 +300,11 @@ __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t*
       const int leaf = leaf0 + 2 * p + q;
       if (leaf < n_leaves) {
         const size_t g = (size_t)leaf * RK + i;
-        C s = q == 0 ? a0 : a1;
+        const C a = q == 0 ? a0 : a1;
+        double2 s = make_double2(a.x, a.y);
         if (find_phi) {
-          const R qq = qk2[g];
-          const C xv = x[g];
+          const double qq = qk2[g];
+          const double2 xv = x[g];
           s.x = xv.x + qq * s.x;
           s.y = xv.y + qq * s.y;
         }
@@ -695,6 +724,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   HIP_OK(hipMemset(state, 0, size_t(n_state) * sizeof(double2)));
   static_assert(sizeof(cplx) == sizeof(double2), "layout");
   std::string e;
+  n_entries = int64_t(plan.ops.size());
   if (!(e = upload_vec(reinterpret_cast<const std::vector<double2>&>(plan.ops), &ops)).empty()) return e;
   if (!(e = upload_vec(plan.src, &src)).empty()) return e;
   std::vector<int> rows, ncol, dst;
@@ -758,7 +788,8 @@ std::string DeviceOperator::upload(const Plan& plan) {
 void DeviceOperator::release() {
   if (stream) hipStreamSynchronize(stream);
   if (aux) hipStreamSynchronize(aux);
-  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2};
+  work.release();
+  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32};
   for (void* p : ptrs)
     if (p) hipFree(p);
   state = ops = nullptr;
@@ -766,6 +797,7 @@ void DeviceOperator::release() {
   d_op = d_srcoff = nullptr;
   T = nullptr;
   qk2 = nullptr;
+  ops32 = T32 = state32 = nullptr;
   if (ev_fork) hipEventDestroy(ev_fork);
   if (ev_join) hipEventDestroy(ev_join);
   ev_fork = ev_join = nullptr;
@@ -774,73 +806,102 @@ void DeviceOperator::release() {
   aux = stream = nullptr;
 }
 
+template <class R>
 static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2* x, double2* y, hipStream_t s) {
   const int64_t b = p.dest_base;
-  typedef cx<double>::v V;
-  const V* ops = reinterpret_cast<const V*>(o.ops);
+  typedef typename cx<R>::v V;
+  typedef typename cx<R>::t C;
+  const V* ops = reinterpret_cast<const V*>(std::is_same<R, double>::value ? (const void*)o.ops : (const void*)o.ops32);
+  C* st = reinterpret_cast<C*>(std::is_same<R, double>::value ? (void*)o.state : (void*)o.state32);
   if (p.wave) {
     const dim3 grid((p.ndest + 3) / 4);
     if (p.kind == 0)
-      hipLaunchKernelGGL((pass_wave_kernel<false, double>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
-                         o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+      hipLaunchKernelGGL((pass_wave_kernel<false, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
     else
-      hipLaunchKernelGGL((pass_wave_kernel<true, double>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
-                         o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+      hipLaunchKernelGGL((pass_wave_kernel<true, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
     return;
   }
   const dim3 grid(p.ndest);
   if (p.kind == 0)
-    hipLaunchKernelGGL((pass_block_kernel<false, double, 0>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
-                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+    hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
   else if (p.role == 1)
-    hipLaunchKernelGGL((pass_block_kernel<true, double, 1>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
-                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+    hipLaunchKernelGGL((pass_block_kernel<true, R, 1>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
   else
-    hipLaunchKernelGGL((pass_block_kernel<true, double, 0>), grid, dim3(kThreads), 0, s, ops, o.src, o.state, x, y,
-                       o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+    hipLaunchKernelGGL((pass_block_kernel<true, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
 }
 
-template <int RK>
+template <class R, int RK>
 static void launch_near_rk(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
   constexpr int LB = 2 * (kThreads / RK);
-  hipLaunchKernelGGL((near_kernel<double, RK>), dim3((o.n_leaves + LB - 1) / LB), dim3(kThreads), 0, s, o.T, o.nbr,
-                     o.qk2, x, y, o.n_leaves, o.find_phi ? 1 : 0);
+  typedef typename cx<R>::t C;
+  const C* T = reinterpret_cast<const C*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
+  hipLaunchKernelGGL((near_kernel<R, RK>), dim3((o.n_leaves + LB - 1) / LB), dim3(kThreads), 0, s, T, o.nbr, o.qk2,
+                     x, y, o.n_leaves, o.find_phi ? 1 : 0);
 }
 
+template <class R>
 static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
   switch (o.rank) {
-    case 4: launch_near_rk<4>(o, x, y, s); break;
-    case 9: launch_near_rk<9>(o, x, y, s); break;
-    case 16: launch_near_rk<16>(o, x, y, s); break;
-    case 25: launch_near_rk<25>(o, x, y, s); break;
-    case 36: launch_near_rk<36>(o, x, y, s); break;
-    case 49: launch_near_rk<49>(o, x, y, s); break;
-    case 64: launch_near_rk<64>(o, x, y, s); break;
+    case 4: launch_near_rk<R, 4>(o, x, y, s); break;
+    case 9: launch_near_rk<R, 9>(o, x, y, s); break;
+    case 16: launch_near_rk<R, 16>(o, x, y, s); break;
+    case 25: launch_near_rk<R, 25>(o, x, y, s); break;
+    case 36: launch_near_rk<R, 36>(o, x, y, s); break;
+    case 49: launch_near_rk<R, 49>(o, x, y, s); break;
+    case 64: launch_near_rk<R, 64>(o, x, y, s); break;
     default: break;  // rejected at upload
   }
 }
 
-void DeviceOperator::matvec(const double2* x, double2* y) {
+template <class R>
+static void matvec_impl(DeviceOperator& o, const double2* x, double2* y) {
   if (x == y) {
     // in-place call: the passes read x while the output is written
-    hipMemcpyAsync(state, x, size_t(N) * sizeof(double2), hipMemcpyDeviceToDevice, stream);
-    x = state;
+    hipMemcpyAsync(o.state, x, size_t(o.N) * sizeof(double2), hipMemcpyDeviceToDevice, o.stream);
+    x = o.state;
   }
   // the first upward pass is the leaf level (the only one the leaf M2L reads)
   size_t u = 0;
-  if (!up.empty()) launch_pass(*this, passes[up[0]], x, y, stream), u = 1;
-  hipEventRecord(ev_fork, stream);
-  hipStreamWaitEvent(aux, ev_fork, 0);
-  launch_near(*this, x, y, aux);
-  for (int p : leaf_m2l) launch_pass(*this, passes[p], x, y, aux);
-  hipEventRecord(ev_join, aux);
-  for (; u < up.size(); ++u) launch_pass(*this, passes[up[u]], x, y, stream);
-  for (int p : down) launch_pass(*this, passes[p], x, y, stream);
-  hipStreamWaitEvent(stream, ev_join, 0);
-  for (int p : leaf_l2l) launch_pass(*this, passes[p], x, y, stream);
+  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream), u = 1;
+  hipEventRecord(o.ev_fork, o.stream);
+  hipStreamWaitEvent(o.aux, o.ev_fork, 0);
+  launch_near<R>(o, x, y, o.aux);
+  for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux);
+  hipEventRecord(o.ev_join, o.aux);
+  for (; u < o.up.size(); ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream);
+  for (int p : o.down) launch_pass<R>(o, o.passes[p], x, y, o.stream);
+  hipStreamWaitEvent(o.stream, o.ev_join, 0);
+  for (int p : o.leaf_l2l) launch_pass<R>(o, o.passes[p], x, y, o.stream);
 }
 
-double DeviceOperator::time_matvec(int reps) {
+void DeviceOperator::matvec(const double2* x, double2* y) { matvec_impl<double>(*this, x, y); }
+void DeviceOperator::matvec32(const double2* x, double2* y) { matvec_impl<float>(*this, x, y); }
+
+__global__ void to_f32_kernel(const double2* __restrict__ a, float2* __restrict__ b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = make_float2(float(a[i].x), float(a[i].y));
+}
+
+std::string DeviceOperator::enable_fp32() {
+  if (ops32) return "";
+  const long long n_ops = n_entries, n_T = 9LL * rank * rank;
+  HIP_OK(hipMalloc((void**)&ops32, sizeof(float2) * size_t(std::max(n_ops, 1LL))));
+  HIP_OK(hipMalloc((void**)&T32, sizeof(float2) * size_t(n_T)));
+  HIP_OK(hipMalloc((void**)&state32, sizeof(float2) * size_t(n_state)));
+  HIP_OK(hipMemsetAsync(state32, 0, sizeof(float2) * size_t(n_state), stream));
+  if (n_ops > 0)
+    hipLaunchKernelGGL(to_f32_kernel, dim3(unsigned((n_ops + 255) / 256)), dim3(256), 0, stream, ops, ops32, n_ops);
+  hipLaunchKernelGGL(to_f32_kernel, dim3(unsigned((n_T + 255) / 256)), dim3(256), 0, stream, T, T32, n_T);
+  HIP_OK(hipStreamSynchronize(stream));
+  return "";
+}
+
+double DeviceOperator::time_matvec(int reps, bool fp32) {
   double2 *x = nullptr, *y = nullptr;
   if (hipMalloc((void**)&x, sizeof(double2) * size_t(N)) != hipSuccess) return -1;
   if (hipMalloc((void**)&y, sizeof(double2) * size_t(N)) != hipSuccess) {
@@ -851,9 +912,13 @@ double DeviceOperator::time_matvec(int reps) {
   hipEvent_t a, b;
   hipEventCreate(&a);
   hipEventCreate(&b);
-  matvec(x, y);
+  auto mv = [&]() {
+    if (fp32) matvec32(x, y);
+    else matvec(x, y);
+  };
+  mv();
   hipEventRecord(a, stream);
-  for (int r = 0; r < reps; ++r) matvec(x, y);
+  for (int r = 0; r < reps; ++r) mv();
   hipEventRecord(b, stream);
   hipEventSynchronize(b);
   float ms = 0;
@@ -885,8 +950,8 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
   for (int r = 0; r < reps; ++r) {
     HIP_OK(hipEventRecord(ev[0], stream));
     for (int q = 0; q < np; ++q) {
-      if (order[q] < 0) launch_near(*this, x, y, stream);
-      else launch_pass(*this, passes[order[q]], x, y, stream);
+      if (order[q] < 0) launch_near<double>(*this, x, y, stream);
+      else launch_pass<double>(*this, passes[order[q]], x, y, stream);
       HIP_OK(hipEventRecord(ev[q + 1], stream));
     }
     HIP_OK(hipEventSynchronize(ev[np]));
@@ -910,8 +975,38 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
 }
 
 // ------------------------------------------------------------------ GMRES ---
+void GmresWork::release() {
+  void* ptrs[] = {V, w, partial, d1, d2, y, H, sn, g, cs, hn, beta};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (hostbuf) hipHostFree(hostbuf);
+  *this = GmresWork{};
+}
+
+std::string GmresWork::reserve(int N_, int m_) {
+  if (m == m_ && N == N_) return "";
+  release();
+  const int nrb = (N_ + kGsRows - 1) / kGsRows;
+  HIP_OK(hipMalloc((void**)&V, sizeof(double2) * size_t(N_) * (m_ + 1)));
+  HIP_OK(hipMalloc((void**)&w, sizeof(double2) * size_t(N_)));
+  HIP_OK(hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m_ + 2)));
+  HIP_OK(hipMalloc((void**)&d1, sizeof(double2) * (m_ + 2)));
+  HIP_OK(hipMalloc((void**)&d2, sizeof(double2) * (m_ + 2)));
+  HIP_OK(hipMalloc((void**)&y, sizeof(double2) * (m_ + 1)));
+  HIP_OK(hipMalloc((void**)&H, sizeof(double2) * size_t(m_ + 1) * m_));
+  HIP_OK(hipMalloc((void**)&cs, sizeof(double) * (m_ + 1)));
+  HIP_OK(hipMalloc((void**)&sn, sizeof(double2) * (m_ + 1)));
+  HIP_OK(hipMalloc((void**)&g, sizeof(double2) * (m_ + 2)));
+  HIP_OK(hipMalloc((void**)&hn, sizeof(double)));
+  HIP_OK(hipMalloc((void**)&beta, sizeof(double)));
+  HIP_OK(hipHostMalloc((void**)&hostbuf, sizeof(double) * 2 * (m_ + 2), hipHostMallocDefault));
+  m = m_;
+  N = N_;
+  return "";
+}
+
 GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts,
-                         double tol) {
+                         double tol, bool fp32) {
   GmresResult res;
   auto t0 = std::chrono::steady_clock::now();
   const int N = op.N;
@@ -921,31 +1016,31 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     res.error = "restart length must be in [1, 500]";
     return res;
   }
+  if (fp32 && !op.ops32) {
+    res.status = LSG_ERR_ARG;
+    res.error = "FP32 operators are not enabled";
+    return res;
+  }
   const int nrb = (N + kGsRows - 1) / kGsRows;   // row blocks of the dot kernels
   const int nb_vec = (N + kThreads - 1) / kThreads;
   const long long ldv = N;
-  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *y = nullptr;
-  GivensState st{};
-  double* hostbuf = nullptr;
-  std::vector<void*> owned;
-  auto alloc = [&](void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes);
-    if (e == hipSuccess) owned.push_back(*p);
-    return e == hipSuccess;
-  };
-  bool ok = alloc((void**)&V, sizeof(double2) * size_t(N) * (m + 1)) && alloc((void**)&w, sizeof(double2) * size_t(N)) &&
-            alloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m + 2)) &&
-            alloc((void**)&d1, sizeof(double2) * (m + 2)) && alloc((void**)&d2, sizeof(double2) * (m + 2)) &&
-            alloc((void**)&y, sizeof(double2) * (m + 1)) &&
-            alloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m) && alloc((void**)&st.cs, sizeof(double) * (m + 1)) &&
-            alloc((void**)&st.sn, sizeof(double2) * (m + 1)) && alloc((void**)&st.g, sizeof(double2) * (m + 2)) &&
-            alloc((void**)&st.hn, sizeof(double)) && alloc((void**)&st.beta, sizeof(double));
-  if (!ok || hipHostMalloc((void**)&hostbuf, sizeof(double) * 2 * (m + 2), hipHostMallocDefault) != hipSuccess) {
-    for (void* p : owned) hipFree(p);
-    res.status = LSG_ERR_DEVICE;
-    res.error = "GMRES workspace allocation failed";
-    return res;
+  {
+    std::string e = op.work.reserve(N, m);
+    if (!e.empty()) {
+      op.work.release();
+      res.status = LSG_ERR_DEVICE;
+      res.error = "GMRES workspace allocation failed: " + e;
+      return res;
+    }
   }
+  GmresWork& W = op.work;
+  double2 *V = W.V, *w = W.w, *partial = W.partial, *d1 = W.d1, *d2 = W.d2, *y = W.y;
+  GivensState st{W.H, W.cs, W.sn, W.g, W.hn, W.beta};
+  double* hostbuf = W.hostbuf;
+  auto matvec = [&](const double2* xin, double2* yout) {
+    if (fp32) op.matvec32(xin, yout);
+    else op.matvec(xin, yout);
+  };
   // partials of V[:, 0:nc]^H vec (+ |vec|^2 as column nc when norm)
   auto dot = [&](int nc, const double2* vec, bool norm) {
     const int ncx = nc + (norm ? 1 : 0);
@@ -973,7 +1068,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   // convergence test; the loop runs at most max_restarts Arnoldi cycles and
   // always ends on a residual evaluation.
   for (int cyc = 0; res.status == 0 && cyc <= max_restarts; ++cyc) {
-    op.matvec(x_dev, w);
+    matvec(x_dev, w);
     hipLaunchKernelGGL(sub_kernel, dim3(nb_vec), dim3(kThreads), 0, s, b_dev, w, w, N);
     dot(0, w, true);
     hipLaunchKernelGGL(gs_cycle_init_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, st, m, hostbuf);
@@ -1003,7 +1098,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
       // steps past an exit found inside the batch are discarded
       const int batch = std::min(m - j, (!ref_mode && last_est > 100.0 * tol) || ref_mode ? 4 : 1);
       for (int jj = j; jj < j + batch; ++jj) {
-        op.matvec(V + (size_t)jj * N, w);
+        matvec(V + (size_t)jj * N, w);
         const int nc = jj + 1;
         dot(nc, w, false);
         hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, nc, d1);
@@ -1056,8 +1151,104 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   res.iterations = total_it;
   res.converged = converged ? 1 : 0;
   hipStreamSynchronize(s);
-  for (void* p : owned) hipFree(p);
-  hipHostFree(hostbuf);
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+// ||v|| by the Gram-Schmidt dot kernels (norm column only), synchronous.
+static double device_norm(DeviceOperator& op, const double2* v, double2* partial, double2* out) {
+  const int N = op.N;
+  const int nrb = (N + kGsRows - 1) / kGsRows;
+  hipLaunchKernelGGL(gs_dot_kernel, dim3(nrb, 1), dim3(kThreads), 0, op.stream, (const double2*)nullptr, (long long)N,
+                     0, 1, v, N, partial);
+  hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(kThreads), 0, op.stream, partial, nrb, 1, out);
+  double2 h;
+  hipMemcpyAsync(&h, out, sizeof(double2), hipMemcpyDeviceToHost, op.stream);
+  if (hipStreamSynchronize(op.stream) != hipSuccess) return std::nan("");
+  return std::sqrt(std::fmax(h.x, 0.0));
+}
+
+__global__ void add_kernel(double2* __restrict__ x, const double2* __restrict__ d, int N) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) {
+    x[n].x += d[n].x;
+    x[n].y += d[n].y;
+  }
+}
+
+GmresResult device_gmres_ir(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts,
+                            double tol, double inner_tol, int max_outer) {
+  GmresResult res;
+  auto t0 = std::chrono::steady_clock::now();
+  const int N = op.N;
+  hipStream_t s = op.stream;
+  if (!(tol > 0) || !(inner_tol > 0) || max_outer < 1) {
+    res.status = LSG_ERR_ARG;
+    res.error = "iterative refinement needs tol > 0, inner_tol > 0 and max_outer >= 1";
+    return res;
+  }
+  std::string e = op.enable_fp32();
+  if (!e.empty()) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = e;
+    return res;
+  }
+  const int nrb = (N + kGsRows - 1) / kGsRows;
+  const int nb_vec = (N + kThreads - 1) / kThreads;
+  double2 *r = nullptr, *d = nullptr, *partial = nullptr, *out = nullptr;
+  bool ok = hipMalloc((void**)&r, sizeof(double2) * size_t(N)) == hipSuccess &&
+            hipMalloc((void**)&d, sizeof(double2) * size_t(N)) == hipSuccess &&
+            hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb)) == hipSuccess &&
+            hipMalloc((void**)&out, sizeof(double2)) == hipSuccess;
+  const double bnorm = ok ? device_norm(op, b_dev, partial, out) : 0.0;
+  if (!ok) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = "refinement workspace allocation failed";
+  } else if (!(bnorm > 0) || !std::isfinite(bnorm)) {
+    res.status = LSG_ERR_NUMERIC;
+    res.error = !std::isfinite(bnorm) ? "right-hand side holds NaN/Inf" : "right-hand side is zero";
+  }
+  // x <- x + A32^{-1} (b - A64 x), the FP64 residual deciding convergence
+  for (int outer = 0; res.status == 0; ++outer) {
+    op.matvec(x_dev, r);
+    hipLaunchKernelGGL(sub_kernel, dim3(nb_vec), dim3(kThreads), 0, s, b_dev, r, r, N);
+    const double rn = device_norm(op, r, partial, out);
+    res.rel_residual = rn / bnorm;
+    if (!std::isfinite(rn)) {
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "residual is NaN/Inf";
+      break;
+    }
+    if (rn <= tol * bnorm) {
+      res.converged = 1;
+      break;
+    }
+    if (outer == max_outer) break;
+    // inner target relative to ||r||: enough for this refinement to reach tol,
+    // but not below what the FP32 operator can resolve
+    const double eta = std::max(inner_tol, 0.5 * tol * bnorm / rn);
+    hipMemsetAsync(d, 0, sizeof(double2) * size_t(N), s);
+    GmresResult g = device_gmres(op, r, d, m, max_restarts, eta, true);
+    res.iterations += g.iterations;
+    res.restarts += g.restarts;
+    res.est_residual = g.est_residual;
+    if (g.status != 0 && g.status != LSG_ERR_NOCONV) {
+      res.status = g.status;
+      res.error = "inner FP32 solve: " + g.error;
+      break;
+    }
+    hipLaunchKernelGGL(add_kernel, dim3(nb_vec), dim3(kThreads), 0, s, x_dev, d, N);
+    res.refinements = outer + 1;
+  }
+  if (res.status == 0 && !res.converged) {
+    res.status = LSG_ERR_NOCONV;
+    res.error = "iterative refinement did not reach tol";
+  }
+  hipStreamSynchronize(s);
+  if (r) hipFree(r);
+  if (d) hipFree(d);
+  if (partial) hipFree(partial);
+  if (out) hipFree(out);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return res;
 }

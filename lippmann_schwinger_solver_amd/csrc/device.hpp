@@ -19,6 +19,18 @@ struct DevPass {
   int64_t bytes() const { return 16 * entries + 4 * n_src + 16 * n_src + 16 * n_dst; }
 };
 
+// Device workspace of the GMRES solver, kept between solves of the same
+// restart length (Krylov basis, Hessenberg, rotations, reduction partials).
+struct GmresWork {
+  int N = 0, m = -1;
+  double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *y = nullptr;
+  double2 *H = nullptr, *sn = nullptr, *g = nullptr;
+  double *cs = nullptr, *hn = nullptr, *beta = nullptr;
+  double* hostbuf = nullptr;  // pinned: per-step residual estimates and norms
+  std::string reserve(int N, int m);
+  void release();
+};
+
 // One matvec = upward passes -> downward passes -> leaf L2L on the main
 // stream, while the near field and the leaf M2L (which need only x and the
 // leaf-level outgoing charges) run on a second stream after the first upward
@@ -44,6 +56,12 @@ struct DeviceOperator {
   double2* T = nullptr;        // 9 x rank x rank, column-major per slot
   int* nbr = nullptr;          // n_leaves x 9
   double* qk2 = nullptr;       // N
+  int64_t n_entries = 0;        // operator entries of all passes
+  // FP32 copies (operators, near-field tables, pass state), made by enable_fp32
+  float2* ops32 = nullptr;
+  float2* T32 = nullptr;
+  float2* state32 = nullptr;
+  GmresWork work;
   std::vector<DevPass> passes;
   std::vector<int> up, down, leaf_m2l, leaf_l2l;  // pass indices by role
   int64_t near_bytes = 0;
@@ -52,20 +70,32 @@ struct DeviceOperator {
   void release();
   // y = A x on the main stream; x, y device pointers of N complex128 (may alias)
   void matvec(const double2* x, double2* y);
+  // same with FP32 operators and FP32 accumulation inside each pass (x, y FP64);
+  // enable_fp32 must have succeeded
+  void matvec32(const double2* x, double2* y);
+  std::string enable_fp32();
   // event-timed per-pass averages over reps matvecs (passes run serialised)
   std::string profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes, std::vector<std::string>& names);
   // average time of the full two-stream matvec (ms)
-  double time_matvec(int reps);
+  double time_matvec(int reps, bool fp32 = false);
 };
 
 // Restarted GMRES (classical Gram-Schmidt with one reorthogonalisation,
 // complex Givens) with device-resident Krylov basis, Hessenberg and rotations.
 struct GmresResult {
-  int iterations = 0, restarts = 0, converged = 0;
+  int iterations = 0, restarts = 0, converged = 0, refinements = 0;
   double rel_residual = 0, est_residual = 0, seconds = 0;
   int status = 0;  // 0 ok, else LSG_ERR_*
   std::string error;
 };
-GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts, double tol);
+// fp32: every matvec of the Krylov process uses the FP32 operators (FP64 vectors,
+// Gram-Schmidt and Givens)
+GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts, double tol,
+                         bool fp32 = false);
+// FP64 iterative refinement around FP32-operator GMRES: r = b - A x with the FP64
+// operator, d ~ A32^{-1} r by GMRES(m) to relative residual max(inner_tol, tol/2 ||b||/||r||),
+// x += d; stops when ||b - A x|| <= tol ||b|| (FP64) or after max_outer refinements.
+GmresResult device_gmres_ir(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts,
+                            double tol, double inner_tol, int max_outer);
 
 }  // namespace lsg
