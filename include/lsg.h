@@ -149,7 +149,8 @@ int lsg_incident(const lsg_ctx* ctx, double* u);
 int lsg_rhs(const lsg_ctx* ctx, double* b);
 
 /* y = A x for nrhs right-hand sides (column j at x + 2*ld*j doubles),
- * FP64 on the device. Replaces DFMM::MatVecProduct. */
+ * FP64 on the device. Replaces DFMM::MatVecProduct. nrhs >= 2 runs the block
+ * operator (64 right-hand sides per FP64 MFMA block matvec). */
 int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
 
 /* Device-pointer variant: x, y are device addresses of N complex128 each;
@@ -191,6 +192,19 @@ int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms);
  * reference's GMRES cycle semantics (GMRES/GMRES.cpp:257-319) inside each
  * inner solve; the refinement loop itself has no reference counterpart. */
 int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* opts, lsg_stats* stats);
+
+/* Batched GMRES for nrhs <= 64 systems A x_q = b_q sharing the operator (the
+ * reference solves one right-hand side per RestartedGMRES call,
+ * GMRES/GMRES.cpp:307-319): b and x hold nrhs columns at b + 2*ld*q (host, or
+ * device addresses when opts->device_ptrs), x the initial guesses. The systems
+ * advance in lockstep through one block matvec (FP64 MFMA) per Arnoldi step,
+ * each with its own Krylov basis (classical Gram-Schmidt), Givens rotations and
+ * stopping point; each cycle opens with the true FP64 residual of every
+ * system. opts->mode must be LSG_SOLVE_FP64. rel_residuals (host, nrhs, may be
+ * NULL) receives the final true relative residuals; stats->rel_residual is
+ * their maximum and stats->converged the number of systems at tol. */
+int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* opts,
+                    double* rel_residuals, lsg_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,8 @@ SYMBOLS = [
     ("lsg_matvec_fp32", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
     ("lsg_time_matvec_fp32", C.c_int, [_P, C.c_int, _D]),
     ("lsg_gmres_ex", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.POINTER(lsg_stats)]),
+    ("lsg_gmres_batch", C.c_int, [_P, _P, _P, C.c_int, C.c_int64, C.POINTER(lsg_solve_opts), _D,
+                                  C.POINTER(lsg_stats)]),
     ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_build_tree", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_destroy", None, [_P]),
@@ -374,6 +376,41 @@ class DFMM:
             self._check(rc)
         stats["status"] = rc
         return x, stats
+
+    def gmres_batch(self, B, X0=None, restart_len=50, max_restarts=20, tol=1e-8, device_ptrs=None, ld=None,
+                    nrhs=None):
+        """Batched GMRES on the columns of B (N x nrhs, nrhs <= 64); returns
+        (X, stats, rel_residuals). With device_ptrs = (b_dev, x_dev) the
+        column-major blocks (leading dimension ld) stay on the device and X is None."""
+        o = lsg_solve_opts(mode=LSG_SOLVE_FP64, restart_len=restart_len, max_restarts=max_restarts, tol=tol,
+                           inner_tol=0.0, max_outer=0, device_ptrs=1 if device_ptrs else 0)
+        st = lsg_stats()
+        if device_ptrs:
+            rel = np.zeros(nrhs)
+            rc = self._lib.lsg_gmres_batch(self._ctx, C.c_void_p(device_ptrs[0]), C.c_void_p(device_ptrs[1]), nrhs,
+                                           ld or self.N, C.byref(o), _dptr(rel), C.byref(st))
+            X = None
+        else:
+            Bt = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).T)     # nrhs x N
+            nrhs = Bt.shape[0]
+            Xt = np.zeros_like(Bt) if X0 is None else np.ascontiguousarray(np.asarray(X0, dtype=np.complex128).T)
+            rel = np.zeros(nrhs)
+            rc = self._lib.lsg_gmres_batch(self._ctx, Bt.ctypes.data_as(_P), Xt.ctypes.data_as(_P), nrhs, self.N,
+                                           C.byref(o), _dptr(rel), C.byref(st))
+            X = Xt.T
+        stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats["status"] = rc
+        return X, stats, rel
+
+    def rhs_angles(self, angles):
+        """Right-hand sides -k^2 q u_inc for plane waves at the given angles (N x len(angles))."""
+        x, y = self.gridPoints[:, 0], self.gridPoints[:, 1]
+        qk2 = self.contrast_k2()
+        k = self.inputs.kappa
+        a = np.asarray(angles, dtype=float)
+        return -qk2[:, None] * np.exp(1j * k * (np.outer(x, np.cos(a)) + np.outer(y, np.sin(a))))
 
     def MatVecProduct_fp32(self, x):
         """y = A32 x: FP32 operators and FP32 accumulation inside each pass, FP64 x and y."""

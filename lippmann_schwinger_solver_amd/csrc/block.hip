@@ -1,0 +1,642 @@
+// Block (multi right-hand-side) matvec on FP64 MFMA and batched GMRES, gfx950.
+#include "block.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace lsg {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kT = 256;
+constexpr int kNB = kBlockRhs;      // 64 right-hand sides = 4 column tiles of 16
+constexpr int kLD = 2 * kNB;        // doubles per block row
+constexpr int kBCK = 32;            // complex operator columns staged in LDS per step
+constexpr int kBMaxRows = 64;       // complex rows per destination (8 row tiles of 16 reals)
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return std::string(#expr ": ") + hipGetErrorString(e_); \
+  } while (0)
+
+// One workgroup per destination; wave w computes right-hand sides [16w, 16w+16)
+// for all row tiles. Real-ified operator fragment (16 x 4) per lane l:
+// row 16*mt + l%16, column 4*ks + l/16, taken from the complex entry in LDS;
+// the source fragment (4 x 16): real row 4*ks + l/16 (column c = 2ks + (l/16)/2,
+// Re or Im), right-hand side l%16. Result lane l holds rows l/16 + 4v, v < 4,
+// column l%16 of each 16 x 16 tile.
+template <bool ACC>
+__global__ __launch_bounds__(kT) void bpass_kernel(const double2* __restrict__ ops, const int* __restrict__ src,
+                                                   double* bst, const double* __restrict__ X, double* Y, int N,
+                                                   const int* __restrict__ rows, const int* __restrict__ ncol,
+                                                   const int* __restrict__ dst, const long long* __restrict__ op_off,
+                                                   const long long* __restrict__ src_off) {
+  __shared__ double2 As[kBMaxRows * kBCK];
+  __shared__ int Ss[kBCK];
+  const int d = blockIdx.x;
+  const int R = rows[d], C = ncol[d];
+  const double2* A = ops + op_off[d];
+  const int* S = src + src_off[d];
+  const int dd = dst[d];
+  const int t = threadIdx.x, lane = t & 63, nt = t >> 6;
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int MT = (2 * R + 15) >> 4;
+  const int q = (nt << 4) + i16;
+  d4 acc[8];
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int c0 = 0; c0 < C; c0 += kBCK) {
+    const int Cc = min(kBCK, C - c0);
+    __syncthreads();
+    const double2* Ac = A + (size_t)c0 * R;  // the chunk's columns are contiguous (column-major)
+    typedef double dv2 __attribute__((ext_vector_type(2)));
+    const dv2* Av = reinterpret_cast<const dv2*>(Ac);
+    for (int e = t; e < Cc * R; e += kT) {
+      const dv2 a = __builtin_nontemporal_load(Av + e);
+      As[e] = make_double2(a.x, a.y);
+    }
+    if (t < Cc) Ss[t] = S[c0 + t];
+    __syncthreads();
+    double bv[kBCK / 2];
+#pragma unroll
+    for (int ks = 0; ks < kBCK / 2; ++ks) {
+      const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
+      bv[ks] = 0.0;
+      if (cl < Cc) {
+        const int row = Ss[cl];
+        const double* base = row < N ? X + (size_t)row * kLD : bst + (size_t)row * kLD;
+        bv[ks] = base[b * kNB + q];
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < kBCK / 2; ++ks) {
+      if (2 * ks < Cc) {
+        const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+          if (mt < MT) {
+            const int ri = (mt << 4) + i16;
+            const int r = ri >> 1, a = ri & 1;
+            double av = 0.0;
+            if (r < R && cl < Cc) {
+              const double* e = reinterpret_cast<const double*>(&As[cl * R + r]);
+              av = e[a == b ? 0 : 1];
+              if (a == 0 && b == 1) av = -av;
+            }
+            acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks], acc[mt], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt) {
+    if (mt < MT) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ri = (mt << 4) + k4 + 4 * v;
+        const int r = ri >> 1, a = ri & 1;
+        if (r < R) {
+          const int row = dd + r;
+          double* p = (row >= N && row < 2 * N) ? Y + (size_t)(row - N) * kLD : bst + (size_t)row * kLD;
+          p += a * kNB + q;
+          double val = acc[mt][v];
+          if (ACC) val += *p;
+          *p = val;
+        }
+      }
+    }
+  }
+}
+
+// Y = X + qk2 .* Y on the output rows (find_phi); the near-field pass stored
+// the bare colleague sums in Y.
+__global__ void bnear_finish_kernel(const double* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2,
+                                    long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total) Y[i] = X[i] + qk2[i / kLD] * Y[i];
+}
+
+// ------------------------------------------------------- batched GMRES ----
+constexpr int kBRows = 256;   // rows per workgroup in the block dot
+constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
+
+// partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
+// conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
+__global__ __launch_bounds__(kT) void bdot_kernel(const double* __restrict__ V, long long vs, int nc, int ncx,
+                                                  const double* __restrict__ w, int N, double2* __restrict__ partial) {
+  __shared__ double2 red[kT / 64][kBCols][kNB];
+  const int t = threadIdx.x, q = t & 63, rl = t >> 6;
+  const int g0 = blockIdx.y * kBCols;
+  const long long n0 = (long long)blockIdx.x * kBRows;
+  double2 acc[kBCols];
+#pragma unroll
+  for (int u = 0; u < kBCols; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int r = rl; r < kBRows; r += kT / 64) {
+    const long long n = n0 + r;
+    if (n >= N) break;
+    const double wr = w[n * kLD + q], wi = w[n * kLD + kNB + q];
+#pragma unroll
+    for (int u = 0; u < kBCols; ++u) {
+      const int col = g0 + u;
+      double vr = 0.0, vi = 0.0;
+      if (col < nc) {
+        vr = V[col * vs + n * kLD + q];
+        vi = V[col * vs + n * kLD + kNB + q];
+      } else if (col == nc) {
+        vr = wr;
+        vi = wi;
+      }
+      acc[u].x = fma(vr, wr, fma(vi, wi, acc[u].x));
+      acc[u].y = fma(vr, wi, fma(-vi, wr, acc[u].y));
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kBCols; ++u) red[rl][u][q] = acc[u];
+  __syncthreads();
+  for (int e = t; e < kBCols * kNB; e += kT) {
+    const int u = e / kNB, qq = e % kNB;
+    if (g0 + u < ncx) {
+      double2 s = red[0][u][qq];
+      for (int k = 1; k < kT / 64; ++k) {
+        s.x += red[k][u][qq].x;
+        s.y += red[k][u][qq].y;
+      }
+      partial[((long long)blockIdx.x * ncx + g0 + u) * kNB + qq] = s;
+    }
+  }
+}
+
+// out[col * 64 + q] = sum over row blocks; one workgroup per column.
+__global__ __launch_bounds__(kT) void breduce_kernel(const double2* __restrict__ partial, int nrb, int ncx,
+                                                     double2* __restrict__ out) {
+  __shared__ double2 red[kT];
+  const int col = blockIdx.x, t = threadIdx.x, q = t & 63, sg = t >> 6;
+  double2 a = make_double2(0.0, 0.0);
+  for (int rb = sg; rb < nrb; rb += kT / 64) {
+    const double2 v = partial[((long long)rb * ncx + col) * kNB + q];
+    a.x += v.x;
+    a.y += v.y;
+  }
+  red[t] = a;
+  __syncthreads();
+  if (t < kNB) {
+    double2 s = red[t];
+    for (int k = 1; k < kT / 64; ++k) {
+      s.x += red[k * kNB + t].x;
+      s.y += red[k * kNB + t].y;
+    }
+    out[col * kNB + t] = s;
+  }
+}
+
+struct BGivens {
+  double2* H;     // [(j*(m+1) + i) * 64 + q]
+  double* cs;     // [j * 64 + q]
+  double2* sn;    // [j * 64 + q]
+  double2* g;     // [i * 64 + q], i <= m
+  double* hn;     // [q]
+  double* bnorm;  // [q]
+  int* active;    // [q]
+  int* kq;        // [q] Arnoldi steps applied in this cycle
+};
+
+// Cycle start per system q: beta = ||r_q|| (norm column of h), relative
+// residual to host_rel[q]; systems already at tol (or padding) are inactive.
+__global__ void bcycle_init_kernel(const double2* __restrict__ h, BGivens st, int m, int nrhs, double tol,
+                                   double* host_rel, int* host_active) {
+  const int q = threadIdx.x;
+  if (q >= kNB) return;
+  const double beta = sqrt(fmax(h[q].x, 0.0));
+  const double rel = beta / st.bnorm[q];
+  const int act = (q < nrhs && rel > tol && beta > 0.0) ? 1 : 0;
+  st.active[q] = act;
+  st.kq[q] = 0;
+  st.hn[q] = act ? beta : 0.0;
+  for (int i = 0; i <= m; ++i) st.g[i * kNB + q] = make_double2(i == 0 ? beta : 0.0, 0.0);
+  host_rel[q] = rel;
+  host_active[q] = act;
+}
+
+// V0[n, q] = w[n, q] / beta_q for active q, 0 otherwise (beta in st.hn)
+__global__ void bscale_kernel(const double* __restrict__ w, BGivens st, double* __restrict__ v, long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_total) return;
+  const int q = int(i % kNB);
+  v[i] = st.active[q] ? w[i] / st.hn[q] : 0.0;
+}
+
+// Arnoldi step j for every active system: h in [0, j], ||w||^2 at j+1 (from the
+// block dot); ||w - V h|| by Pythagoras; Givens rotation of H column j; the
+// relative estimate |g_{j+1}| / ||b_q|| goes to host_est[j*64 + q]. A system
+// whose estimate reaches 0.5 tol, or whose new vector vanishes, is frozen.
+__global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, int j, double tol, double* host_est,
+                             int* host_active) {
+  const int q = threadIdx.x;
+  if (q >= kNB) return;
+  if (!st.active[q]) {
+    st.hn[q] = 0.0;
+    host_est[j * kNB + q] = -1.0;
+    host_active[q] = 0;
+    return;
+  }
+  double s2 = 0.0;
+  double2* Hc = st.H + (long long)j * (m + 1) * kNB;
+  for (int i = 0; i <= j; ++i) {
+    const double2 v = h[i * kNB + q];
+    s2 += v.x * v.x + v.y * v.y;
+    Hc[i * kNB + q] = v;
+  }
+  const double wn2 = h[(j + 1) * kNB + q].x;
+  const double hn = sqrt(fmax(wn2 - s2, 0.0));
+  for (int i = 0; i < j; ++i) {
+    const double c = st.cs[i * kNB + q];
+    const double2 s = st.sn[i * kNB + q];
+    const double2 a = Hc[i * kNB + q], b = Hc[(i + 1) * kNB + q];
+    Hc[i * kNB + q] = make_double2(c * a.x + (s.x * b.x - s.y * b.y), c * a.y + (s.x * b.y + s.y * b.x));
+    Hc[(i + 1) * kNB + q] = make_double2(c * b.x - (s.x * a.x + s.y * a.y), c * b.y - (s.x * a.y - s.y * a.x));
+  }
+  const double2 a = Hc[j * kNB + q];
+  const double an = hypot(a.x, a.y);
+  double c;
+  double2 s;
+  if (an == 0.0) {
+    c = 0.0;
+    s = make_double2(1.0, 0.0);
+    Hc[j * kNB + q] = make_double2(hn, 0.0);
+  } else {
+    const double r = hypot(an, hn);
+    c = an / r;
+    const double2 ph = make_double2(a.x / an, a.y / an);
+    s = make_double2(ph.x * hn / r, ph.y * hn / r);
+    Hc[j * kNB + q] = make_double2(ph.x * r, ph.y * r);
+  }
+  Hc[(j + 1) * kNB + q] = make_double2(0.0, 0.0);
+  st.cs[j * kNB + q] = c;
+  st.sn[j * kNB + q] = s;
+  const double2 gj = st.g[j * kNB + q];
+  st.g[j * kNB + q] = make_double2(c * gj.x, c * gj.y);
+  const double2 g1 = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+  st.g[(j + 1) * kNB + q] = g1;
+  const double est = hypot(g1.x, g1.y) / st.bnorm[q];
+  st.kq[q] = j + 1;
+  st.hn[q] = hn;
+  const bool stop = est <= 0.5 * tol || !(hn > 1e-14 * sqrt(fmax(wn2, 0.0)));
+  if (stop) st.active[q] = 0;
+  host_est[j * kNB + q] = est;
+  host_active[q] = stop ? 0 : 1;
+}
+
+// v[n, q] = (w[n, q] - sum_i V_i[n, q] h[i, q]) / hn_q, zero for systems with hn_q = 0
+// (frozen before this step) — hn is the step's norm, set by btail.
+__global__ __launch_bounds__(kT) void bupdate_kernel(const double* __restrict__ V, long long vs, int nc,
+                                                     const double2* __restrict__ h, const double* __restrict__ w,
+                                                     const double* __restrict__ hn, double* __restrict__ vout,
+                                                     long long n_rows) {
+  const long long e = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long n = e / kNB;
+  const int q = int(e % kNB);
+  if (n >= n_rows) return;
+  double ar = 0.0, ai = 0.0;
+  const long long base = n * kLD + q;
+  int i = 0;
+  for (; i + 4 <= nc; i += 4) {
+    double vr[4], vi[4];
+    double2 hh[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      vr[u] = V[(i + u) * vs + base];
+      vi[u] = V[(i + u) * vs + base + kNB];
+      hh[u] = h[(i + u) * kNB + q];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ar = fma(vr[u], hh[u].x, fma(-vi[u], hh[u].y, ar));
+      ai = fma(vr[u], hh[u].y, fma(vi[u], hh[u].x, ai));
+    }
+  }
+  for (; i < nc; ++i) {
+    const double vr = V[i * vs + base], vi = V[i * vs + base + kNB];
+    const double2 hh = h[i * kNB + q];
+    ar = fma(vr, hh.x, fma(-vi, hh.y, ar));
+    ai = fma(vr, hh.y, fma(vi, hh.x, ai));
+  }
+  const double d = hn[q];
+  const double inv = d > 0.0 ? 1.0 / d : 0.0;
+  vout[base] = (w[base] - ar) * inv;
+  vout[base + kNB] = (w[base + kNB] - ai) * inv;
+}
+
+// y_q = R_q^{-1} g_q over the kq[q] steps of system q (zero beyond)
+__global__ void btrisolve_kernel(BGivens st, int m, double2* __restrict__ y) {
+  const int q = threadIdx.x;
+  if (q >= kNB) return;
+  const int k = st.kq[q];
+  for (int i = k; i <= m; ++i) y[i * kNB + q] = make_double2(0.0, 0.0);
+  for (int i = k - 1; i >= 0; --i) {
+    double2 s = st.g[i * kNB + q];
+    for (int c = i + 1; c < k; ++c) {
+      const double2 r = st.H[((long long)c * (m + 1) + i) * kNB + q];
+      const double2 yc = y[c * kNB + q];
+      s.x -= r.x * yc.x - r.y * yc.y;
+      s.y -= r.x * yc.y + r.y * yc.x;
+    }
+    const double2 dg = st.H[((long long)i * (m + 1) + i) * kNB + q];
+    const double den = dg.x * dg.x + dg.y * dg.y;
+    y[i * kNB + q] = den > 0.0 ? make_double2((s.x * dg.x + s.y * dg.y) / den, (s.y * dg.x - s.x * dg.y) / den)
+                               : make_double2(0.0, 0.0);
+  }
+}
+
+// X[n, q] += sum_i V_i[n, q] y[i, q], i < nc
+__global__ __launch_bounds__(kT) void baxpy_kernel(const double* __restrict__ V, long long vs, int nc,
+                                                   const double2* __restrict__ y, double* __restrict__ X,
+                                                   long long n_rows) {
+  const long long e = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long n = e / kNB;
+  const int q = int(e % kNB);
+  if (n >= n_rows) return;
+  const long long base = n * kLD + q;
+  double ar = 0.0, ai = 0.0;
+  for (int i = 0; i < nc; ++i) {
+    const double vr = V[i * vs + base], vi = V[i * vs + base + kNB];
+    const double2 yy = y[i * kNB + q];
+    ar = fma(vr, yy.x, fma(-vi, yy.y, ar));
+    ai = fma(vr, yy.y, fma(vi, yy.x, ai));
+  }
+  X[base] += ar;
+  X[base + kNB] += ai;
+}
+
+// R = B - R
+__global__ void bsub_kernel(const double* __restrict__ B, double* __restrict__ R, long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total) R[i] = B[i] - R[i];
+}
+
+__global__ void to_split_kernel(const double2* __restrict__ x, long long ld, int nrhs, int N, double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = e / kNB;
+  const int q = int(e % kNB);
+  if (n >= N) return;
+  double2 v = make_double2(0.0, 0.0);
+  if (q < nrhs) v = x[q * ld + n];
+  out[n * kLD + q] = v.x;
+  out[n * kLD + kNB + q] = v.y;
+}
+
+__global__ void from_split_kernel(const double* __restrict__ in, int N, int nrhs, double2* __restrict__ x, long long ld) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = e / kNB;
+  const int q = int(e % kNB);
+  if (n >= N || q >= nrhs) return;
+  x[q * ld + n] = make_double2(in[n * kLD + q], in[n * kLD + kNB + q]);
+}
+
+inline unsigned blocks_for(long long n, int per) { return unsigned((n + per - 1) / per); }
+
+}  // namespace
+
+void to_split(const double2* x, long long ld, int nrhs, int N, int nb, double* out, hipStream_t s) {
+  (void)nb;
+  hipLaunchKernelGGL(to_split_kernel, dim3(blocks_for((long long)N * kNB, kT)), dim3(kT), 0, s, x, ld, nrhs, N, out);
+}
+
+void from_split(const double* in, int N, int nb, int nrhs, double2* x, long long ld, hipStream_t s) {
+  (void)nb;
+  hipLaunchKernelGGL(from_split_kernel, dim3(blocks_for((long long)N * kNB, kT)), dim3(kT), 0, s, in, N, nrhs, x, ld);
+}
+
+std::string BlockOperator::setup(DeviceOperator& o) {
+  if (op == &o && bstate) return "";
+  release();
+  op = &o;
+  nb = kNB;
+  // every destination must fit the kernel's row tiles
+  {
+    std::vector<int> rows_h(size_t(o.passes.empty() ? 0 : o.passes.back().dest_base + o.passes.back().ndest));
+    if (!rows_h.empty())
+      HIP_OK(hipMemcpy(rows_h.data(), o.d_rows, rows_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int r : rows_h)
+      if (r > kBMaxRows) return "block matvec: a destination has more than 64 rows";
+    if (o.rank > kBMaxRows) return "block matvec: rank above 64";
+  }
+  HIP_OK(hipMalloc((void**)&bstate, sizeof(double) * size_t(o.n_state + 1) * kLD));
+  HIP_OK(hipMemset(bstate, 0, sizeof(double) * size_t(o.n_state + 1) * kLD));
+  // near field as one pass: leaf l reads the rank x (9 rank) side-by-side tables
+  const int L = o.n_leaves, rk = o.rank;
+  std::vector<int> nbr_h(size_t(L) * 9);
+  HIP_OK(hipMemcpy(nbr_h.data(), o.nbr, nbr_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> srcs(size_t(L) * 9 * rk), rws(L, rk), ncs(L, 9 * rk), dsts(L);
+  std::vector<long long> opo(L, 0), so(L);
+  for (int l = 0; l < L; ++l) {
+    for (int s = 0; s < 9; ++s) {
+      const int nbl = nbr_h[size_t(l) * 9 + s];
+      for (int c = 0; c < rk; ++c)
+        srcs[(size_t(l) * 9 + s) * rk + c] = nbl >= 0 ? nbl * rk + c : int(o.n_state);  // zero row
+    }
+    dsts[l] = o.N + l * rk;
+    so[l] = (long long)l * 9 * rk;
+  }
+  HIP_OK(hipMalloc((void**)&near_src, srcs.size() * sizeof(int)));
+  HIP_OK(hipMemcpy(near_src, srcs.data(), srcs.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_rows, L * sizeof(int)));
+  HIP_OK(hipMemcpy(near_rows, rws.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_ncol, L * sizeof(int)));
+  HIP_OK(hipMemcpy(near_ncol, ncs.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_dst, L * sizeof(int)));
+  HIP_OK(hipMemcpy(near_dst, dsts.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_op, L * sizeof(long long)));
+  HIP_OK(hipMemcpy(near_op, opo.data(), L * sizeof(long long), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_srcoff, L * sizeof(long long)));
+  HIP_OK(hipMemcpy(near_srcoff, so.data(), L * sizeof(long long), hipMemcpyHostToDevice));
+  return "";
+}
+
+void BlockOperator::release() {
+  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  bstate = nullptr;
+  near_src = near_rows = near_ncol = near_dst = nullptr;
+  near_op = near_srcoff = nullptr;
+  op = nullptr;
+}
+
+static void launch_bpass(BlockOperator& bo, const DevPass& p, const double* X, double* Y) {
+  DeviceOperator& o = *bo.op;
+  const int64_t b = p.dest_base;
+  if (p.kind == 0)
+    hipLaunchKernelGGL(bpass_kernel<false>, dim3(p.ndest), dim3(kT), 0, o.stream, o.ops, o.src, bo.bstate, X, Y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+  else
+    hipLaunchKernelGGL(bpass_kernel<true>, dim3(p.ndest), dim3(kT), 0, o.stream, o.ops, o.src, bo.bstate, X, Y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+}
+
+void BlockOperator::matvec(const double* X, double* Y) {
+  DeviceOperator& o = *op;
+  for (int p : o.up) launch_bpass(*this, o.passes[p], X, Y);
+  hipLaunchKernelGGL(bpass_kernel<false>, dim3(o.n_leaves), dim3(kT), 0, o.stream, o.T, near_src, bstate, X, Y, o.N,
+                     near_rows, near_ncol, near_dst, near_op, near_srcoff);
+  if (o.find_phi)
+    hipLaunchKernelGGL(bnear_finish_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
+                       o.qk2, (long long)o.N * kLD);
+  for (int p : o.down) launch_bpass(*this, o.passes[p], X, Y);
+  for (int p : o.leaf_m2l) launch_bpass(*this, o.passes[p], X, Y);
+  for (int p : o.leaf_l2l) launch_bpass(*this, o.passes[p], X, Y);
+}
+
+BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
+                        double* rel_residuals) {
+  BatchResult res;
+  auto t0 = std::chrono::steady_clock::now();
+  DeviceOperator& o = *bo.op;
+  hipStream_t s = o.stream;
+  const int N = o.N;
+  if (nrhs < 1 || nrhs > kNB || m < 1 || m > 500 || max_restarts < 1 || !(tol > 0)) {
+    res.status = LSG_ERR_ARG;
+    res.error = "batched GMRES: 1 <= nrhs <= 64, 1 <= m <= 500, max_restarts >= 1, tol > 0";
+    return res;
+  }
+  const long long vs = (long long)N * kLD;  // doubles per block vector
+  const int nrb = (N + kBRows - 1) / kBRows;
+  std::vector<void*> owned;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    owned.push_back(*p);
+    return true;
+  };
+  double *V = nullptr, *w = nullptr;
+  double2 *partial = nullptr, *h = nullptr, *y = nullptr;
+  BGivens st{};
+  double *host_rel = nullptr, *host_est = nullptr;
+  int* host_active = nullptr;
+  bool ok = alloc((void**)&V, sizeof(double) * size_t(vs) * (m + 1)) && alloc((void**)&w, sizeof(double) * size_t(vs)) &&
+            alloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m + 2) * kNB) &&
+            alloc((void**)&h, sizeof(double2) * size_t(m + 2) * kNB) &&
+            alloc((void**)&y, sizeof(double2) * size_t(m + 1) * kNB) &&
+            alloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m * kNB) &&
+            alloc((void**)&st.cs, sizeof(double) * size_t(m) * kNB) &&
+            alloc((void**)&st.sn, sizeof(double2) * size_t(m) * kNB) &&
+            alloc((void**)&st.g, sizeof(double2) * size_t(m + 1) * kNB) && alloc((void**)&st.hn, sizeof(double) * kNB) &&
+            alloc((void**)&st.bnorm, sizeof(double) * kNB) && alloc((void**)&st.active, sizeof(int) * kNB) &&
+            alloc((void**)&st.kq, sizeof(int) * kNB);
+  ok = ok && hipHostMalloc((void**)&host_rel, sizeof(double) * kNB, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&host_est, sizeof(double) * size_t(m) * kNB, hipHostMallocDefault) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&host_active, sizeof(int) * kNB, hipHostMallocDefault) == hipSuccess;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(s);
+    for (void* p : owned) hipFree(p);
+    if (host_rel) hipHostFree(host_rel);
+    if (host_est) hipHostFree(host_est);
+    if (host_active) hipHostFree(host_active);
+  };
+  if (!ok) {
+    cleanup();
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched GMRES workspace allocation failed";
+    return res;
+  }
+  auto dot = [&](int nc, const double* vec, bool norm) {
+    const int ncx = nc + (norm ? 1 : 0);
+    hipLaunchKernelGGL(bdot_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec, N,
+                       partial);
+    hipLaunchKernelGGL(breduce_kernel, dim3(ncx), dim3(kT), 0, s, partial, nrb, ncx, h);
+  };
+  // ||b_q||
+  dot(0, B, true);
+  std::vector<double2> bn2(kNB);
+  hipMemcpyAsync(bn2.data(), h, sizeof(double2) * kNB, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    cleanup();
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched GMRES: norm of b failed";
+    return res;
+  }
+  std::vector<double> bnorm(kNB, 1.0);
+  for (int q = 0; q < nrhs; ++q) {
+    bnorm[q] = std::sqrt(std::max(bn2[q].x, 0.0));
+    if (!(bnorm[q] > 0) || !std::isfinite(bnorm[q])) {
+      cleanup();
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched GMRES: a right-hand side is zero or holds NaN/Inf";
+      return res;
+    }
+  }
+  hipMemcpyAsync(st.bnorm, bnorm.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
+  const long long ntot = vs;                                  // doubles per block vector
+  const unsigned gv = blocks_for(ntot, kT);                   // one thread per double
+  const unsigned gq = blocks_for((long long)N * kNB, kT);     // one thread per (row, system)
+  int total_it = 0;
+  for (int cyc = 0; cyc <= max_restarts; ++cyc) {
+    bo.matvec(X, w);
+    hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, ntot);
+    dot(0, w, true);
+    hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, nrhs, tol, host_rel, host_active);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched GMRES: device error";
+      break;
+    }
+    int n_act = 0, n_conv = 0;
+    double worst = 0.0;
+    bool finite = true;
+    for (int q = 0; q < nrhs; ++q) {
+      n_act += host_active[q];
+      n_conv += host_rel[q] <= tol ? 1 : 0;
+      worst = std::max(worst, host_rel[q]);
+      finite = finite && std::isfinite(host_rel[q]);
+      if (rel_residuals) rel_residuals[q] = host_rel[q];
+    }
+    res.converged = n_conv;
+    res.max_rel_residual = worst;
+    if (!finite) {
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched GMRES: residual is NaN/Inf";
+      break;
+    }
+    if (n_act == 0 || cyc == max_restarts) break;
+    hipLaunchKernelGGL(bscale_kernel, dim3(gv), dim3(kT), 0, s, w, st, V, ntot);
+    int steps = 0;
+    for (int j = 0; j < m;) {
+      const int batch = std::min(m - j, 4);
+      for (int jj = j; jj < j + batch; ++jj) {
+        bo.matvec(V + size_t(jj) * vs, w);
+        dot(jj + 1, w, true);
+        hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, tol, host_est, host_active);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(gq), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
+                           V + size_t(jj + 1) * vs, (long long)N);
+      }
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        res.status = LSG_ERR_DEVICE;
+        res.error = "batched GMRES: device error";
+        break;
+      }
+      j += batch;
+      steps = j;
+      int still = 0;
+      for (int q = 0; q < nrhs; ++q) still += host_active[q];
+      if (still == 0) break;
+    }
+    if (res.status != 0) break;
+    total_it += steps;
+    hipLaunchKernelGGL(btrisolve_kernel, dim3(1), dim3(kNB), 0, s, st, m, y);
+    hipLaunchKernelGGL(baxpy_kernel, dim3(gq), dim3(kT), 0, s, V, vs, steps, y, X, (long long)N);
+    res.restarts = cyc + 1;
+  }
+  res.iterations = total_it;
+  if (res.status == 0 && res.converged < nrhs) {
+    res.status = LSG_ERR_NOCONV;
+    res.error = "batched GMRES: not every system reached tol";
+  }
+  cleanup();
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+}  // namespace lsg

@@ -1,0 +1,56 @@
+// Multi right-hand-side (block) matvec and batched GMRES on the device.
+//
+// Block vectors use a row-split layout: row n of an N x nb block holds
+// [Re v(n, 0..nb) | Im v(n, 0..nb)] (2*nb doubles), so one gathered source
+// row feeds all right-hand sides with coalesced loads. Every pass of the
+// single-vector plan (nca.cpp) becomes, per destination, a real GEMM
+//   [Yr; Yi] (2R x nb) = [[Ar, -Ai], [Ai, Ar]] (2R x 2C) [Xr; Xi] (2C x nb)
+// on FP64 MFMA tiles (v_mfma_f64_16x16x4f64); the near field is the same
+// GEMM with the 9 colleague tables side by side (36 x 324) for every leaf.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "device.hpp"
+
+namespace lsg {
+
+constexpr int kBlockRhs = 64;  // right-hand sides per block (4 MFMA column tiles)
+
+struct BlockOperator {
+  DeviceOperator* op = nullptr;
+  int nb = kBlockRhs;
+  double* bstate = nullptr;   // (n_state + 1) rows x 2nb; row n_state stays zero
+  int* near_src = nullptr;    // n_leaves x 9 rank source rows (zero row where a colleague is absent)
+  int* near_rows = nullptr;   // per leaf: rank
+  int* near_ncol = nullptr;   // per leaf: 9 rank
+  int* near_dst = nullptr;    // per leaf: N + leaf rank
+  long long* near_op = nullptr;   // per leaf: 0 (all leaves share the tables)
+  long long* near_srcoff = nullptr;
+  std::string setup(DeviceOperator& o);
+  void release();
+  // Y = A X for one block (row-split, nb columns), on op->stream
+  void matvec(const double* X, double* Y);
+};
+
+// Batched restarted GMRES: nb independent systems A x_q = b_q advanced in
+// lockstep with the block matvec; classical Gram-Schmidt (one projection,
+// ||w - V h||^2 = ||w||^2 - |h|^2), complex Givens per system; a system that
+// reaches 0.5 tol in its Givens estimate, or breaks down, is frozen for the
+// rest of the cycle. Every cycle opens with the true FP64 residual of each
+// system, which decides convergence (rel_residual <= tol for all nrhs).
+struct BatchResult {
+  int iterations = 0, restarts = 0, converged = 0;  // converged: systems meeting tol
+  double max_rel_residual = 0, seconds = 0;
+  int status = 0;
+  std::string error;
+};
+// B, X: N x nb row-split device blocks (X holds the initial guesses); only the
+// first nrhs columns are solved, the others must be zero.
+BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
+                        double* rel_residuals /* host, nrhs */);
+
+// column-major complex (x + 2*ld*j) <-> row-split block, nrhs <= nb columns
+void to_split(const double2* x, long long ld, int nrhs, int N, int nb, double* out, hipStream_t s);
+void from_split(const double* in, int N, int nb, int nrhs, double2* x, long long ld, hipStream_t s);
+
+}  // namespace lsg

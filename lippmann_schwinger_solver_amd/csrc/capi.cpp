@@ -1,13 +1,16 @@
 // C ABI of the solver (include/lsg.h, include/lsg_host.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/lsg.h"
 #include "../../include/lsg_host.h"
+#include "block.hpp"
 #include "device.hpp"
 #include "nca.hpp"
 #include "problem.hpp"
@@ -106,9 +109,23 @@ void fill_info(const HostBuild& hb, lsg_info* info) {
 struct lsg_ctx {
   HostBuild hb;
   DeviceOperator dev;
+  BlockOperator blk;          // multi right-hand-side operator, set up on first use
   std::string err;
   double2* dx = nullptr;
   double2* dy = nullptr;
+  double* bx = nullptr;       // two row-split blocks (N x 64) for block calls
+  double* by = nullptr;
+  double2* bcm = nullptr;     // column-major staging for up to 64 host vectors
+  std::string block_ready() {
+    std::string e = blk.setup(dev);
+    if (!e.empty()) return e;
+    const size_t nbk = sizeof(double) * size_t(hb.plan.N) * 2 * kBlockRhs;
+    if (!bx && hipMalloc((void**)&bx, nbk) != hipSuccess) return "hipMalloc of block buffers failed";
+    if (!by && hipMalloc((void**)&by, nbk) != hipSuccess) return "hipMalloc of block buffers failed";
+    if (!bcm && hipMalloc((void**)&bcm, sizeof(double2) * size_t(hb.plan.N) * kBlockRhs) != hipSuccess)
+      return "hipMalloc of block buffers failed";
+    return "";
+  }
   int set(int code, const std::string& m) {
     err = m;
     return code;
@@ -176,9 +193,13 @@ int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
 
 void lsg_destroy(lsg_ctx* ctx) {
   if (!ctx) return;
+  ctx->blk.release();
   ctx->dev.release();
   if (ctx->dx) hipFree(ctx->dx);
   if (ctx->dy) hipFree(ctx->dy);
+  if (ctx->bx) hipFree(ctx->bx);
+  if (ctx->by) hipFree(ctx->by);
+  if (ctx->bcm) hipFree(ctx->bcm);
   delete ctx;
 }
 
@@ -245,6 +266,28 @@ int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld) {
   const int64_t N = ctx->hb.plan.N;
   if (!x || !y || nrhs < 1 || ld < N) return ctx->set(LSG_ERR_ARG, "bad matvec arguments");
   const size_t nb = sizeof(double2) * size_t(N);
+  if (nrhs >= 2) {
+    // block path: 64 right-hand sides per block matvec (FP64 MFMA)
+    for (int r = 0; r < nrhs; ++r)
+      for (int64_t i = 0; i < 2 * N; ++i)
+        if (!std::isfinite(x[2 * ld * r + i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    std::string e = ctx->block_ready();
+    if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+    hipStream_t s = ctx->dev.stream;
+    for (int r0 = 0; r0 < nrhs; r0 += kBlockRhs) {
+      const int nr = std::min(kBlockRhs, nrhs - r0);
+      for (int r = 0; r < nr; ++r)
+        hipMemcpyAsync(ctx->bcm + size_t(r) * N, x + 2 * ld * (r0 + r), nb, hipMemcpyHostToDevice, s);
+      to_split(ctx->bcm, N, nr, int(N), kBlockRhs, ctx->bx, s);
+      ctx->blk.matvec(ctx->bx, ctx->by);
+      from_split(ctx->by, int(N), kBlockRhs, nr, ctx->bcm, N, s);
+      for (int r = 0; r < nr; ++r)
+        hipMemcpyAsync(y + 2 * ld * (r0 + r), ctx->bcm + size_t(r) * N, nb, hipMemcpyDeviceToHost, s);
+      hipError_t he = hipStreamSynchronize(s);
+      if (he != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("block matvec: ") + hipGetErrorString(he));
+    }
+    return LSG_OK;
+  }
   for (int r = 0; r < nrhs; ++r) {
     const double* xr = x + 2 * ld * r;
     for (int64_t i = 0; i < 2 * N; ++i)
@@ -370,6 +413,58 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
     stats->est_residual = r.est_residual;
     stats->seconds = r.seconds;
     stats->refinements = r.refinements;
+  }
+  if (r.status != 0) return ctx->set(r.status, r.error);
+  return LSG_OK;
+}
+
+int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* o,
+                    double* rel_residuals, lsg_stats* stats) {
+  if (!ctx) return LSG_ERR_ARG;
+  const int64_t N = ctx->hb.plan.N;
+  if (!o || !b || !x || nrhs < 1 || nrhs > kBlockRhs || ld < N || o->mode != LSG_SOLVE_FP64 || !(o->tol > 0) ||
+      o->restart_len < 1 || o->restart_len > 500 || o->max_restarts < 1)
+    return ctx->set(LSG_ERR_ARG,
+                    "bad batched gmres arguments (1 <= nrhs <= 64, ld >= N, mode FP64, tol > 0, restart_len in [1,500])");
+  std::string e = ctx->block_ready();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  hipStream_t s = ctx->dev.stream;
+  const size_t nb = sizeof(double2) * size_t(N);
+  // B -> bx, X -> by (row-split)
+  auto load = [&](const double* src, double* dst) {
+    if (o->device_ptrs) {
+      to_split(reinterpret_cast<const double2*>(src), ld, nrhs, int(N), kBlockRhs, dst, s);
+    } else {
+      for (int r = 0; r < nrhs; ++r) hipMemcpyAsync(ctx->bcm + size_t(r) * N, src + 2 * ld * r, nb, hipMemcpyHostToDevice, s);
+      to_split(ctx->bcm, N, nrhs, int(N), kBlockRhs, dst, s);
+    }
+  };
+  if (!o->device_ptrs)
+    for (int r = 0; r < nrhs; ++r)
+      for (int64_t i = 0; i < 2 * N; ++i)
+        if (!std::isfinite(b[2 * ld * r + i]) || !std::isfinite(x[2 * ld * r + i]))
+          return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+  load(b, ctx->bx);
+  load(x, ctx->by);
+  std::vector<double> rel(kBlockRhs, 0.0);
+  BatchResult r = block_gmres(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol, rel.data());
+  if (o->device_ptrs) {
+    from_split(ctx->by, int(N), kBlockRhs, nrhs, reinterpret_cast<double2*>(x), ld, s);
+  } else {
+    from_split(ctx->by, int(N), kBlockRhs, nrhs, ctx->bcm, N, s);
+    for (int q = 0; q < nrhs; ++q) hipMemcpyAsync(x + 2 * ld * q, ctx->bcm + size_t(q) * N, nb, hipMemcpyDeviceToHost, s);
+  }
+  hipStreamSynchronize(s);
+  if (rel_residuals)
+    for (int q = 0; q < nrhs; ++q) rel_residuals[q] = rel[q];
+  if (stats) {
+    stats->iterations = r.iterations;
+    stats->restarts = r.restarts;
+    stats->converged = r.converged;
+    stats->rel_residual = r.max_rel_residual;
+    stats->est_residual = 0.0;
+    stats->seconds = r.seconds;
+    stats->refinements = 0;
   }
   if (r.status != 0) return ctx->set(r.status, r.error);
   return LSG_OK;

@@ -1,0 +1,84 @@
+"""Block (multi right-hand-side) matvec on FP64 MFMA and batched GMRES."""
+import numpy as np
+import pytest
+
+import lippmann_schwinger_solver_amd as L
+import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10     # block matvec vs oracle, as for the single-vector matvec
+
+
+def make(inp):
+    S = L.DFMM(inp)
+    S.FMMTreeInitialisation()
+    return S
+
+
+@pytest.mark.parametrize("find_phi,nrhs", [(1, 5), (0, 3), (1, 70)])
+def test_block_matvec_matches_oracle(find_phi, nrhs):
+    inp = H.inputs(depth=4, kappa=30.0, q=6, find_phi=find_phi)
+    print(f"case find_phi={find_phi} nrhs={nrhs}", flush=True)
+    S = make(inp)
+    o = O.Oracle(inp.to_c())
+    X = np.stack([H.rand_vec(S.N, 100 + s) for s in range(nrhs)], axis=1)
+    Y = S.matvec_block(X)
+    worst_single = worst_oracle = 0.0
+    for j in sorted({0, 1, nrhs // 2, nrhs - 1, min(nrhs - 1, 64)}):
+        worst_single = max(worst_single, H.relerr(Y[:, j], S.MatVecProduct(X[:, j])))
+        worst_oracle = max(worst_oracle, H.relerr(Y[:, j], o.matvec(X[:, j])))
+    print(f"  block vs single {worst_single:.2e}, vs oracle {worst_oracle:.2e}", flush=True)
+    assert worst_single < 1e-13 and worst_oracle < TOL
+
+
+def test_rhs_angles_matches_rhs():
+    inp = H.inputs(depth=3, kappa=10.0, q=6)
+    inp.inc_angle = 0.7
+    S = make(inp)
+    b = S.rhs_angles([0.7])[:, 0]
+    assert H.relerr(b, S.rhs()) < 1e-14
+
+
+def test_batched_gmres_small():
+    inp = H.inputs(depth=4, kappa=20.0, q=6)
+    S = make(inp)
+    angles = 2 * np.pi * np.arange(8) / 8
+    B = S.rhs_angles(angles)
+    X, st, rel = S.gmres_batch(B, restart_len=30, max_restarts=10, tol=1e-8)
+    print(f"batched: {st}, rel {rel}", flush=True)
+    assert st["status"] == 0 and st["converged"] == 8
+    for j in range(8):
+        r = H.relerr(S.MatVecProduct(X[:, j]), B[:, j])       # independent FP64 residual
+        assert rel[j] <= 1e-8 and r <= 1.5e-8, (j, rel[j], r)
+        xs, _ = S.gmres(B[:, j], restart_len=30, max_restarts=10, tol=1e-8)
+        assert H.relerr(X[:, j], xs) < 1e-6
+
+
+def test_batched_gmres_c2_64_angles():
+    """64 plane waves at 2 pi j / 64 on the C2 geometry (depth 6, kappa 80, 5 bumps)."""
+    S = make(H.inputs(depth=6, kappa=80.0, q=6, seed=0))
+    angles = 2 * np.pi * np.arange(64) / 64
+    B = S.rhs_angles(angles)
+    X, st, rel = S.gmres_batch(B, restart_len=50, max_restarts=20, tol=1e-8)
+    print(f"C2 x 64 angles: {st['iterations']} steps, {st['restarts']} cycles, {st['seconds']:.3f} s, "
+          f"max rel residual {rel.max():.2e}", flush=True)
+    assert st["status"] == 0 and st["converged"] == 64 and rel.max() <= 1e-8
+    for j in (0, 17, 63):
+        assert H.relerr(S.MatVecProduct(X[:, j]), B[:, j]) <= 1.5e-8
+
+
+def test_batched_gmres_argument_errors():
+    S = make(H.inputs(depth=3, kappa=10.0))
+    B = np.stack([S.rhs()] * 2, axis=1)
+    with pytest.raises(L.LsgError) as e:
+        S.gmres_batch(np.zeros((S.N, 65), complex), tol=1e-8)
+    assert e.value.code == L.LSG_ERR_ARG
+    with pytest.raises(L.LsgError) as e:
+        S.gmres_batch(B, tol=0.0)
+    assert e.value.code == L.LSG_ERR_ARG
+    B[:, 1] = 0
+    with pytest.raises(L.LsgError) as e:
+        S.gmres_batch(B, tol=1e-8)
+    assert e.value.code == L.LSG_ERR_NUMERIC

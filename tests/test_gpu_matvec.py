@@ -49,11 +49,14 @@ def test_matvec_against_dense_small():
 
 
 def test_block_matvec_equals_single():
+    """nrhs >= 2 runs the FP64 MFMA block operator: same operators, different
+    summation order, so columns agree with the single-vector path to rounding."""
     S = make(H.inputs(depth=4, kappa=20.0))
     X = np.stack([H.rand_vec(S.N, s) for s in range(3)], axis=1)
     Y = S.matvec_block(X)
     for j in range(3):
-        assert np.array_equal(Y[:, j], S.MatVecProduct(X[:, j]))
+        e = H.relerr(Y[:, j], S.MatVecProduct(X[:, j]))
+        assert e < 1e-13, e
 
 
 def test_zero_contrast_is_identity():
