@@ -183,6 +183,19 @@ int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len,
 int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
 /* lsg_time_matvec for the FP32 operator. */
 int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms);
+/* Contrast override for a volume-potential context (lsg_inputs.find_phi = 0,
+ * whose compression does not depend on q): afterwards every matvec returns
+ * x + qk2 .* (V x), the Lippmann-Schwinger operator I + diag(k^2 q) V of the
+ * reference (G_FMM2DTree.hpp:2884-2960 with findPhi) for the contrast given
+ * here. qk2 holds ncol columns of N values k^2 q(x_i) (column-major): the
+ * single-vector calls use column 0; block calls and lsg_gmres_batch use column
+ * q for right-hand side q (column 0 when q >= ncol). qk2 = NULL removes it.
+ * No reference counterpart: the reference rebuilds the tree per contrast. */
+int lsg_set_contrast(lsg_ctx* ctx, const double* qk2, int ncol);
+
+/* Average device time (ms) of one block matvec (64 right-hand sides, FP64
+ * MFMA) over `reps` products of a device-resident block. */
+int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms);
 
 /* GMRES with solver options: lsg_gmres (mode LSG_SOLVE_FP64), or FP64
  * iterative refinement (mode LSG_SOLVE_FP32_IR): r = b - A x with the FP64

@@ -95,6 +95,24 @@ SYMBOLS = [
     ("lsg_gmres", C.c_int, [_P, _D, _D, C.c_int, C.c_int, C.c_double, C.POINTER(lsg_stats)]),
     ("lsg_matvec_fp32", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
     ("lsg_time_matvec_fp32", C.c_int, [_P, C.c_int, _D]),
+    ("lsg_time_matvec_block", C.c_int, [_P, C.c_int, _D]),
+    ("lsg_set_contrast", C.c_int, [_P, _D, C.c_int]),
+    # include/lsg_learn.h
+    ("lsgn_create", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    ("lsgn_destroy", None, [_P]),
+    ("lsgn_last_error", C.c_char_p, [_P]),
+    ("lsgn_num_params", C.c_int64, [_P]),
+    ("lsgn_get_params", C.c_int, [_P, C.POINTER(C.c_float)]),
+    ("lsgn_set_params", C.c_int, [_P, C.POINTER(C.c_float)]),
+    ("lsgn_forward", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+    ("lsgn_loss_grad", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
+    ("lsgn_set_training_data", C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]),
+    ("lsgn_train_step", C.c_int, [_P, C.POINTER(C.c_int), C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    ("lsg_attach_preconditioner", C.c_int, [_P, _P]),
+    ("lsg_apply_preconditioner", C.c_int, [_P, _D, _D]),
+    ("lsg_grid_resample", C.c_int, [_P, _D, _D, C.c_int]),
+    ("lsg_gmres_learned", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.c_int, C.POINTER(lsg_stats)]),
     ("lsg_gmres_ex", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.POINTER(lsg_stats)]),
     ("lsg_gmres_batch", C.c_int, [_P, _P, _P, C.c_int, C.c_int64, C.POINTER(lsg_solve_opts), _D,
                                   C.POINTER(lsg_stats)]),
@@ -419,6 +437,64 @@ class DFMM:
         self._check(self._lib.lsg_matvec_fp32(self._ctx, _dptr(x.view(np.float64)), _dptr(out.view(np.float64)),
                                               1, self.N))
         return out
+
+    def set_contrast(self, qk2):
+        """Volume-potential context only: use A = I + diag(qk2[:, q]) V for right-hand
+        side q (qk2 of shape (N,) or (N, ncol), ncol <= 64); None removes it."""
+        if qk2 is None:
+            self._check(self._lib.lsg_set_contrast(self._ctx, None, 0))
+            return
+        q = np.asarray(qk2, dtype=np.float64)
+        q = q.reshape(self.N, -1)
+        qc = np.ascontiguousarray(q.T)        # ncol x N: column j at j*N
+        self._check(self._lib.lsg_set_contrast(self._ctx, _dptr(qc), qc.shape[0]))
+
+    def attach_preconditioner(self, net):
+        """Attach a learned.LearnedNet (or None) as the learned preconditioner."""
+        self._check(self._lib.lsg_attach_preconditioner(self._ctx, net._h if net is not None else None))
+        self._net = net
+
+    def apply_preconditioner(self, v):
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        z = np.empty_like(v)
+        self._check(self._lib.lsg_apply_preconditioner(self._ctx, _dptr(v.view(np.float64)), _dptr(z.view(np.float64))))
+        return z
+
+    def to_grid(self, v):
+        """Leaf-node vector (N complex) -> G x G grid (complex), Chebyshev interpolation per leaf."""
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        G = self._net.grid
+        out = np.empty(G * G, dtype=np.complex128)
+        self._check(self._lib.lsg_grid_resample(self._ctx, _dptr(v.view(np.float64)), _dptr(out.view(np.float64)), 1))
+        return out.reshape(G, G)
+
+    def to_nodes(self, g):
+        """G x G grid (complex) -> leaf nodes, bilinear interpolation."""
+        g = np.ascontiguousarray(np.asarray(g, dtype=np.complex128).reshape(-1))
+        out = np.empty(self.N, dtype=np.complex128)
+        self._check(self._lib.lsg_grid_resample(self._ctx, _dptr(g.view(np.float64)), _dptr(out.view(np.float64)), 0))
+        return out
+
+    def gmres_learned(self, b, x0=None, guess=True, right=True, restart_len=50, max_restarts=20, tol=1e-8):
+        """Flexible GMRES with the attached network as initial guess and/or right preconditioner."""
+        o = lsg_solve_opts(mode=LSG_SOLVE_FP64, restart_len=restart_len, max_restarts=max_restarts, tol=tol)
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        st = lsg_stats()
+        flags = (1 if guess else 0) | (2 if right else 0)
+        rc = self._lib.lsg_gmres_learned(self._ctx, b.ctypes.data_as(_P), x.ctypes.data_as(_P), C.byref(o), flags,
+                                         C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats["status"] = rc
+        return x, stats
+
+    def time_matvec_block(self, reps=10):
+        """Average device time (ms) of one 64-right-hand-side block matvec."""
+        ms = C.c_double()
+        self._check(self._lib.lsg_time_matvec_block(self._ctx, reps, C.byref(ms)))
+        return ms.value
 
     def time_matvec_fp32(self, reps=20):
         ms = C.c_double()

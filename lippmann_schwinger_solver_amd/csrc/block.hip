@@ -121,6 +121,13 @@ __global__ void bnear_finish_kernel(const double* __restrict__ X, double* __rest
   if (i < n_total) Y[i] = X[i] + qk2[i / kLD] * Y[i];
 }
 
+// Y = X + qk2c .* Y with a contrast per right-hand side (qk2c[n * 64 + q])
+__global__ void bdiag_kernel(const double* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2c,
+                             long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total) Y[i] = X[i] + qk2c[(i / kLD) * kNB + (i % kNB)] * Y[i];
+}
+
 // ------------------------------------------------------- batched GMRES ----
 constexpr int kBRows = 256;   // rows per workgroup in the block dot
 constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
@@ -458,7 +465,8 @@ std::string BlockOperator::setup(DeviceOperator& o) {
 }
 
 void BlockOperator::release() {
-  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff};
+  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff, qk2_cols};
+  qk2_cols = nullptr;
   for (void* p : ptrs)
     if (p) hipFree(p);
   bstate = nullptr;
@@ -489,6 +497,9 @@ void BlockOperator::matvec(const double* X, double* Y) {
   for (int p : o.down) launch_bpass(*this, o.passes[p], X, Y);
   for (int p : o.leaf_m2l) launch_bpass(*this, o.passes[p], X, Y);
   for (int p : o.leaf_l2l) launch_bpass(*this, o.passes[p], X, Y);
+  if (qk2_cols)
+    hipLaunchKernelGGL(bdiag_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y, qk2_cols,
+                       (long long)o.N * kLD);
 }
 
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,

@@ -26,6 +26,9 @@ struct BlockOperator {
   int* near_dst = nullptr;    // per leaf: N + leaf rank
   long long* near_op = nullptr;   // per leaf: 0 (all leaves share the tables)
   long long* near_srcoff = nullptr;
+  // per right-hand-side contrast (N x 64, row-major [n][q]) for a volume-potential
+  // operator: Y[:, q] = X[:, q] + qk2_cols[:, q] .* (V X)[:, q]
+  double* qk2_cols = nullptr;
   std::string setup(DeviceOperator& o);
   void release();
   // Y = A X for one block (row-split, nb columns), on op->stream

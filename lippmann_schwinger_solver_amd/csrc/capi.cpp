@@ -10,8 +10,10 @@
 
 #include "../../include/lsg.h"
 #include "../../include/lsg_host.h"
+#include "../../include/lsg_learn.h"
 #include "block.hpp"
 #include "device.hpp"
+#include "learn.hpp"
 #include "nca.hpp"
 #include "problem.hpp"
 #include "tables.hpp"
@@ -110,6 +112,9 @@ struct lsg_ctx {
   HostBuild hb;
   DeviceOperator dev;
   BlockOperator blk;          // multi right-hand-side operator, set up on first use
+  GridMaps maps;              // node <-> grid maps of the attached learned preconditioner
+  LearnedPrecond prec;
+  std::vector<double> qk2_user_host;  // column 0 of lsg_set_contrast
   std::string err;
   double2* dx = nullptr;
   double2* dy = nullptr;
@@ -193,6 +198,7 @@ int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
 
 void lsg_destroy(lsg_ctx* ctx) {
   if (!ctx) return;
+  ctx->maps.release();
   ctx->blk.release();
   ctx->dev.release();
   if (ctx->dx) hipFree(ctx->dx);
@@ -336,6 +342,175 @@ int lsg_time_matvec(lsg_ctx* ctx, int reps, double* ms) {
   if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
   *ms = ctx->dev.time_matvec(reps);
   if (*ms < 0) return ctx->set(LSG_ERR_DEVICE, "time_matvec allocation failed");
+  return LSG_OK;
+}
+
+int lsg_set_contrast(lsg_ctx* ctx, const double* qk2, int ncol) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (ctx->dev.find_phi) return ctx->set(LSG_ERR_ARG, "lsg_set_contrast needs a volume-potential context (find_phi = 0)");
+  const int64_t N = ctx->hb.plan.N;
+  if (!qk2) {
+    if (ctx->dev.qk2_user) hipFree(ctx->dev.qk2_user);
+    if (ctx->blk.qk2_cols) hipFree(ctx->blk.qk2_cols);
+    ctx->dev.qk2_user = nullptr;
+    ctx->blk.qk2_cols = nullptr;
+    ctx->qk2_user_host.clear();
+    return LSG_OK;
+  }
+  if (ncol < 1 || ncol > kBlockRhs) return ctx->set(LSG_ERR_ARG, "ncol must be in [1, 64]");
+  for (int64_t i = 0; i < N * ncol; ++i)
+    if (!std::isfinite(qk2[i])) return ctx->set(LSG_ERR_NUMERIC, "contrast holds NaN/Inf");
+  std::string e = ctx->block_ready();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  if (!ctx->dev.qk2_user && hipMalloc((void**)&ctx->dev.qk2_user, sizeof(double) * N) != hipSuccess)
+    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+  if (!ctx->blk.qk2_cols && hipMalloc((void**)&ctx->blk.qk2_cols, sizeof(double) * N * kBlockRhs) != hipSuccess)
+    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+  std::vector<double> rowmajor(size_t(N) * kBlockRhs);
+  for (int64_t n = 0; n < N; ++n)
+    for (int q = 0; q < kBlockRhs; ++q) rowmajor[size_t(n) * kBlockRhs + q] = qk2[size_t(q < ncol ? q : 0) * N + n];
+  hipMemcpy(ctx->dev.qk2_user, qk2, sizeof(double) * N, hipMemcpyHostToDevice);
+  ctx->qk2_user_host.assign(qk2, qk2 + N);
+  if (hipMemcpy(ctx->blk.qk2_cols, rowmajor.data(), sizeof(double) * rowmajor.size(), hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return ctx->set(LSG_ERR_DEVICE, "copy of the contrast failed");
+  return LSG_OK;
+}
+
+int lsg_attach_preconditioner(lsg_ctx* ctx, lsgn_net* net) {
+  if (!ctx) return LSG_ERR_ARG;
+  ctx->maps.release();
+  ctx->prec.net = nullptr;
+  ctx->prec.maps = nullptr;
+  if (!net) return LSG_OK;
+  const Tree& t = ctx->hb.tree;
+  std::vector<double> cx(t.leaves.size()), cy(t.leaves.size());
+  double radius = 0;
+  for (size_t i = 0; i < t.leaves.size(); ++i) {
+    const Box& b = t.box(t.leaves[i]);
+    cx[i] = b.cx;
+    cy[i] = b.cy;
+    radius = b.radius;
+  }
+  std::vector<double> qk2;
+  if (!ctx->qk2_user_host.empty()) {
+    qk2 = ctx->qk2_user_host;
+  } else {
+    qk2.resize(size_t(t.N));
+    lsg_contrast_at_nodes(ctx, qk2.data());
+  }
+  std::string e = ctx->maps.build(cx, cy, radius, ctx->hb.in.nChebNodes, ctx->hb.in.L, net->net.G, qk2,
+                                  ctx->hb.in.kappa);
+  if (!e.empty()) {
+    ctx->maps.release();
+    return ctx->set(LSG_ERR_ARG, e);
+  }
+  ctx->prec.net = &net->net;
+  ctx->prec.maps = &ctx->maps;
+  return LSG_OK;
+}
+
+int lsg_apply_preconditioner(lsg_ctx* ctx, const double* v, double* z) {
+  if (!ctx || !v || !z) return LSG_ERR_ARG;
+  if (!ctx->prec.net) return ctx->set(LSG_ERR_ARG, "no preconditioner attached");
+  const size_t nb = sizeof(double2) * size_t(ctx->hb.plan.N);
+  hipStream_t s = ctx->dev.stream;
+  hipMemcpyAsync(ctx->dx, v, nb, hipMemcpyHostToDevice, s);
+  ctx->prec.apply(ctx->dx, ctx->dy, s);
+  hipMemcpyAsync(z, ctx->dy, nb, hipMemcpyDeviceToHost, s);
+  hipError_t he = hipStreamSynchronize(s);
+  return he == hipSuccess ? LSG_OK : ctx->set(LSG_ERR_DEVICE, hipGetErrorString(he));
+}
+
+int lsg_grid_resample(lsg_ctx* ctx, const double* in, double* out, int to_grid) {
+  if (!ctx || !in || !out) return LSG_ERR_ARG;
+  if (!ctx->prec.net) return ctx->set(LSG_ERR_ARG, "no preconditioner attached (the maps belong to it)");
+  const int N = int(ctx->hb.plan.N), GG = ctx->maps.G * ctx->maps.G;
+  const size_t nin = sizeof(double2) * size_t(to_grid ? N : GG), nout = sizeof(double2) * size_t(to_grid ? GG : N);
+  double2 *a = nullptr, *b = nullptr;
+  if (hipMalloc((void**)&a, nin) != hipSuccess || hipMalloc((void**)&b, nout) != hipSuccess) {
+    if (a) hipFree(a);
+    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+  }
+  hipStream_t s = ctx->dev.stream;
+  hipMemcpyAsync(a, in, nin, hipMemcpyHostToDevice, s);
+  if (to_grid) ctx->maps.to_grid(a, b, s);
+  else ctx->maps.to_nodes(a, b, s);
+  hipMemcpyAsync(out, b, nout, hipMemcpyDeviceToHost, s);
+  hipError_t he = hipStreamSynchronize(s);
+  hipFree(a);
+  hipFree(b);
+  return he == hipSuccess ? LSG_OK : ctx->set(LSG_ERR_DEVICE, hipGetErrorString(he));
+}
+
+int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* o, int flags, lsg_stats* stats) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (!o || !b || !x || o->mode != LSG_SOLVE_FP64 || o->restart_len < 1 || o->restart_len > 500 ||
+      o->max_restarts < 1 || !(o->tol > 0) || (flags & ~(LSG_PREC_GUESS | LSG_PREC_RIGHT)))
+    return ctx->set(LSG_ERR_ARG, "bad learned gmres arguments (mode FP64, tol > 0, restart_len in [1,500])");
+  if (flags && !ctx->prec.net) return ctx->set(LSG_ERR_ARG, "no preconditioner attached");
+  const int64_t N = ctx->hb.plan.N;
+  const size_t nb = sizeof(double2) * size_t(N);
+  hipStream_t s = ctx->dev.stream;
+  double2 *db = nullptr, *dxs = nullptr;
+  if (o->device_ptrs) {
+    db = const_cast<double2*>(reinterpret_cast<const double2*>(b));
+    dxs = reinterpret_cast<double2*>(x);
+  } else {
+    for (int64_t i = 0; i < 2 * N; ++i)
+      if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+    if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
+      if (db) hipFree(db);
+      return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
+    }
+    hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  if (flags & LSG_PREC_GUESS) ctx->prec.apply(db, dxs, s);
+  GmresResult r = device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol, false,
+                               (flags & LSG_PREC_RIGHT) ? &ctx->prec : nullptr);
+  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (!o->device_ptrs) {
+    hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
+    hipFree(db);
+    hipFree(dxs);
+  }
+  if (stats) {
+    stats->iterations = r.iterations;
+    stats->restarts = r.restarts;
+    stats->converged = r.converged;
+    stats->rel_residual = r.rel_residual;
+    stats->est_residual = r.est_residual;
+    stats->seconds = r.seconds;
+    stats->refinements = 0;
+  }
+  if (r.status != 0) return ctx->set(r.status, r.error);
+  return LSG_OK;
+}
+
+int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms) {
+  if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
+  std::string e = ctx->block_ready();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  hipStream_t s = ctx->dev.stream;
+  const size_t nbk = sizeof(double) * size_t(ctx->hb.plan.N) * 2 * kBlockRhs;
+  hipMemsetAsync(ctx->bx, 0, nbk, s);
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  ctx->blk.matvec(ctx->bx, ctx->by);
+  hipEventRecord(a, s);
+  for (int r = 0; r < reps; ++r) ctx->blk.matvec(ctx->bx, ctx->by);
+  hipEventRecord(b, s);
+  hipError_t he = hipEventSynchronize(b);
+  float t = 0;
+  hipEventElapsedTime(&t, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (he != hipSuccess) return ctx->set(LSG_ERR_DEVICE, hipGetErrorString(he));
+  *ms = double(t) / reps;
   return LSG_OK;
 }
 

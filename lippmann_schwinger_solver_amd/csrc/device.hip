@@ -789,7 +789,7 @@ void DeviceOperator::release() {
   if (stream) hipStreamSynchronize(stream);
   if (aux) hipStreamSynchronize(aux);
   work.release();
-  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32};
+  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user};
   for (void* p : ptrs)
     if (p) hipFree(p);
   state = ops = nullptr;
@@ -798,6 +798,7 @@ void DeviceOperator::release() {
   T = nullptr;
   qk2 = nullptr;
   ops32 = T32 = state32 = nullptr;
+  qk2_user = nullptr;
   if (ev_fork) hipEventDestroy(ev_fork);
   if (ev_join) hipEventDestroy(ev_join);
   ev_fork = ev_join = nullptr;
@@ -858,6 +859,17 @@ static void launch_near(const DeviceOperator& o, const double2* x, double2* y, h
   }
 }
 
+// y = x + qk2 .* y: the Lippmann-Schwinger operator from the volume potential
+__global__ void diag_kernel(const double2* __restrict__ x, double2* __restrict__ y, const double* __restrict__ qk2,
+                            int N) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) {
+    const double q = qk2[n];
+    const double2 a = x[n], v = y[n];
+    y[n] = make_double2(a.x + q * v.x, a.y + q * v.y);
+  }
+}
+
 template <class R>
 static void matvec_impl(DeviceOperator& o, const double2* x, double2* y) {
   if (x == y) {
@@ -877,6 +889,9 @@ static void matvec_impl(DeviceOperator& o, const double2* x, double2* y) {
   for (int p : o.down) launch_pass<R>(o, o.passes[p], x, y, o.stream);
   hipStreamWaitEvent(o.stream, o.ev_join, 0);
   for (int p : o.leaf_l2l) launch_pass<R>(o, o.passes[p], x, y, o.stream);
+  if (o.qk2_user)
+    hipLaunchKernelGGL(diag_kernel, dim3((o.N + kThreads - 1) / kThreads), dim3(kThreads), 0, o.stream, x, y,
+                       o.qk2_user, o.N);
 }
 
 void DeviceOperator::matvec(const double2* x, double2* y) { matvec_impl<double>(*this, x, y); }
@@ -976,16 +991,17 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
 
 // ------------------------------------------------------------------ GMRES ---
 void GmresWork::release() {
-  void* ptrs[] = {V, w, partial, d1, d2, y, H, sn, g, cs, hn, beta};
+  void* ptrs[] = {V, w, partial, d1, d2, y, H, sn, g, cs, hn, beta, Z};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (hostbuf) hipHostFree(hostbuf);
   *this = GmresWork{};
 }
 
-std::string GmresWork::reserve(int N_, int m_) {
-  if (m == m_ && N == N_) return "";
+std::string GmresWork::reserve(int N_, int m_, bool flexible) {
+  if (m == m_ && N == N_ && (!flexible || Z)) return "";
   release();
+  if (flexible) HIP_OK(hipMalloc((void**)&Z, sizeof(double2) * size_t(N_) * m_));
   const int nrb = (N_ + kGsRows - 1) / kGsRows;
   HIP_OK(hipMalloc((void**)&V, sizeof(double2) * size_t(N_) * (m_ + 1)));
   HIP_OK(hipMalloc((void**)&w, sizeof(double2) * size_t(N_)));
@@ -1006,7 +1022,7 @@ std::string GmresWork::reserve(int N_, int m_) {
 }
 
 GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts,
-                         double tol, bool fp32) {
+                         double tol, bool fp32, Precond* prec) {
   GmresResult res;
   auto t0 = std::chrono::steady_clock::now();
   const int N = op.N;
@@ -1025,7 +1041,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   const int nb_vec = (N + kThreads - 1) / kThreads;
   const long long ldv = N;
   {
-    std::string e = op.work.reserve(N, m);
+    std::string e = op.work.reserve(N, m, prec != nullptr);
     if (!e.empty()) {
       op.work.release();
       res.status = LSG_ERR_DEVICE;
@@ -1098,7 +1114,12 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
       // steps past an exit found inside the batch are discarded
       const int batch = std::min(m - j, (!ref_mode && last_est > 100.0 * tol) || ref_mode ? 4 : 1);
       for (int jj = j; jj < j + batch; ++jj) {
-        matvec(V + (size_t)jj * N, w);
+        if (prec) {
+          prec->apply(V + (size_t)jj * N, W.Z + (size_t)jj * N, s);
+          matvec(W.Z + (size_t)jj * N, w);
+        } else {
+          matvec(V + (size_t)jj * N, w);
+        }
         const int nc = jj + 1;
         dot(nc, w, false);
         hipLaunchKernelGGL(gs_reduce_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, nc, d1);
@@ -1138,7 +1159,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     if (res.status != 0) break;
     // x += V y, y = R^-1 g
     hipLaunchKernelGGL(trisolve_kernel, dim3(1), dim3(kThreads), 0, s, st, m, k, y);
-    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, k, y, x_dev, N);
+    hipLaunchKernelGGL(axpy_cols_kernel, dim3(nb_vec), dim3(kThreads), 0, s, prec ? W.Z : V, ldv, k, y, x_dev, N);
     res.restarts = cyc + 1;
   }
   if (res.status == 0) {

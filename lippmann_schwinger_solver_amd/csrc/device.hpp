@@ -24,11 +24,19 @@ struct DevPass {
 struct GmresWork {
   int N = 0, m = -1;
   double2 *V = nullptr, *w = nullptr, *partial = nullptr, *d1 = nullptr, *d2 = nullptr, *y = nullptr;
+  double2* Z = nullptr;       // flexible GMRES: preconditioned directions, m x N
   double2 *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *beta = nullptr;
   double* hostbuf = nullptr;  // pinned: per-step residual estimates and norms
-  std::string reserve(int N, int m);
+  std::string reserve(int N, int m, bool flexible = false);
   void release();
+};
+
+// Right preconditioner of flexible GMRES: z = M(v), device vectors of N
+// complex128, enqueued on stream s (M may be nonlinear).
+struct Precond {
+  virtual ~Precond() = default;
+  virtual void apply(const double2* v, double2* z, hipStream_t s) = 0;
 };
 
 // One matvec = upward passes -> downward passes -> leaf L2L on the main
@@ -56,6 +64,9 @@ struct DeviceOperator {
   double2* T = nullptr;        // 9 x rank x rank, column-major per slot
   int* nbr = nullptr;          // n_leaves x 9
   double* qk2 = nullptr;       // N
+  // contrast supplied by the caller for a volume-potential operator
+  // (find_phi = false): matvec then returns x + qk2_user .* (V x)
+  double* qk2_user = nullptr;
   int64_t n_entries = 0;        // operator entries of all passes
   // FP32 copies (operators, near-field tables, pass state), made by enable_fp32
   float2* ops32 = nullptr;
@@ -91,7 +102,7 @@ struct GmresResult {
 // fp32: every matvec of the Krylov process uses the FP32 operators (FP64 vectors,
 // Gram-Schmidt and Givens)
 GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts, double tol,
-                         bool fp32 = false);
+                         bool fp32 = false, Precond* prec = nullptr);
 // FP64 iterative refinement around FP32-operator GMRES: r = b - A x with the FP64
 // operator, d ~ A32^{-1} r by GMRES(m) to relative residual max(inner_tol, tol/2 ||b||/||r||),
 // x += d; stops when ||b - A x|| <= tol ||b|| (FP64) or after max_outer refinements.

@@ -1,0 +1,223 @@
+"""Learned preconditioner: the network handle, training-set generation with
+the solver itself, GPU training, and evaluation against plain GMRES.
+
+No counterpart in the reference (its preconditioner is HODLR, out of scope).
+All arithmetic runs in the HIP kernels of csrc/learn.hip and the solver; this
+module only moves data and drives the loop.
+
+Training pairs: for a contrast q, incident angle theta and wavenumber k, the
+solver computes x = A^-1 b with b = -k^2 q u_inc; with sigma = RMS(b) over the
+leaf nodes the network maps (q, Re b/sigma, Im b/sigma) on the G x G grid to
+the correction (Re, Im) of (x - b)/sigma; the preconditioner is
+M(v) = v + sigma * correction, the identity for an untrained network. Many contrasts share one compressed volume potential
+V per wavenumber (a find_phi = 0 context, whose compression does not depend on
+q); A = I + diag(k^2 q) V is applied per system through lsg_set_contrast.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import (DFMM, InputsToDFMM, LsgError, bumps_from_seed, load_library, LSG_OK)
+
+
+class LearnedNet:
+    """Handle of an lsgn_net (include/lsg_learn.h)."""
+
+    def __init__(self, n_layers=6, width=32, dilations=(1, 2, 4, 8, 4, 1), grid=128, max_batch=64, seed=0):
+        self._lib = load_library()
+        self.grid = grid
+        self.max_batch = max_batch
+        self.dilations = tuple(dilations)
+        d = (C.c_int * len(dilations))(*dilations)
+        h = C.c_void_p()
+        rc = self._lib.lsgn_create(n_layers, width, d, grid, max_batch, seed, C.byref(h))
+        if rc != LSG_OK:
+            raise LsgError(rc, "lsgn_create failed (no GPU, or bad sizes)")
+        self._h = h
+        self.nparams = int(self._lib.lsgn_num_params(h))
+
+    def _check(self, rc):
+        if rc != LSG_OK:
+            raise LsgError(rc, self._lib.lsgn_last_error(self._h).decode())
+
+    def params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        self._check(self._lib.lsgn_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self._lib.lsgn_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, x):
+        """x: (B, G, G, 3) float32 -> (B, G, G, 2)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty(x.shape[:3] + (2,), dtype=np.float32)
+        self._check(self._lib.lsgn_forward(self._h, x.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0],
+                                           out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def loss_grad(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = C.c_float()
+        g = np.empty(self.nparams, dtype=np.float32)
+        self._check(self._lib.lsgn_loss_grad(self._h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                             t.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0], C.byref(loss),
+                                             g.ctypes.data_as(C.POINTER(C.c_float))))
+        return float(loss.value), g
+
+    def set_training_data(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        self._check(self._lib.lsgn_set_training_data(self._h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                                     t.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0]))
+        self.n_data = x.shape[0]
+
+    def train_step(self, idx, lr):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_float()
+        self._check(self._lib.lsgn_train_step(self._h, idx.ctypes.data_as(C.POINTER(C.c_int)), idx.size, lr,
+                                              C.byref(loss)))
+        return float(loss.value)
+
+    def train(self, epochs=20, batch=None, lr=1e-3, seed=0, lr_decay=0.9, log=None):
+        """Adam on the resident training set, shuffled each epoch; returns the
+        per-epoch mean losses."""
+        batch = batch or self.max_batch
+        rng = np.random.default_rng(seed)
+        hist = []
+        for ep in range(epochs):
+            perm = rng.permutation(self.n_data)
+            losses = []
+            for s in range(0, self.n_data - batch + 1, batch):
+                losses.append(self.train_step(perm[s:s + batch], lr))
+            hist.append(float(np.mean(losses)))
+            if log:
+                log(f"epoch {ep}: loss {hist[-1]:.4e} lr {lr:.2e}")
+            lr *= lr_decay
+        return hist
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lsgn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def volume_context(kappa, depth, net):
+    """find_phi = 0 context (compressed volume potential, q-independent) with the
+    network attached for its grid maps."""
+    inp = InputsToDFMM(kappa=kappa, nLevelsUniform=depth, nChebNodes=6, degreeOfBases=6, TOL_POW=10,
+                       nCones_LFR=16, find_phi=0)
+    S = DFMM(inp)
+    S.FMMTreeInitialisation()
+    qk2 = np.zeros(S.N)
+    S.set_contrast(qk2)
+    S.attach_preconditioner(net)
+    return S
+
+
+def contrast_qk2(S, kappa, seed, nbumps):
+    """k^2 q at the leaf nodes for the seeded bump contrast (same generator as lsg_bumps_from_seed)."""
+    c = bumps_from_seed(seed, nbumps)
+    x, y = S.gridPoints[:, 0], S.gridPoints[:, 1]
+    q = np.zeros(S.N)
+    for i in range(c.n_bumps):
+        q += c.amp[i] * np.exp(-((x - c.cx[i]) ** 2 + (y - c.cy[i]) ** 2) / c.width[i] ** 2)
+    return kappa * kappa * q
+
+
+def sample_params(seed):
+    """(angle, nbumps) of training/evaluation sample `seed` (SplitMix64-free numpy stream keyed by seed)."""
+    r = np.random.default_rng(10_000 + seed)
+    return float(r.uniform(0, 2 * np.pi)), int(r.integers(3, 9))
+
+
+def random_field(S, kappa, seed, n_waves=6):
+    """Smooth random complex field: n_waves plane waves of wavenumber kappa with
+    random directions and complex amplitudes (the right-hand sides of the
+    'mixed' training set: -k^2 q times this field, like the Krylov vectors)."""
+    r = np.random.default_rng(50_000 + seed)
+    x, y = S.gridPoints[:, 0], S.gridPoints[:, 1]
+    f = np.zeros(S.N, dtype=complex)
+    for _ in range(n_waves):
+        a = r.uniform(0, 2 * np.pi)
+        f += (r.standard_normal() + 1j * r.standard_normal()) * np.exp(1j * kappa * (x * np.cos(a) + y * np.sin(a)))
+    return f
+
+
+def make_pairs(S, kappa, seeds, tol=1e-8, restart_len=50, mixed=True):
+    """Solve the systems of `seeds` on context S in batches of 64; return network
+    inputs (n, G, G, 3), targets (n, G, G, 2) and solve statistics. With mixed,
+    odd seeds use a random-field right-hand side instead of a single plane wave."""
+    G = S._net.grid
+    xs, ts, iters = [], [], []
+    for s0 in range(0, len(seeds), 64):
+        chunk = list(seeds[s0:s0 + 64])
+        qk2s, B = [], []
+        x, y = S.gridPoints[:, 0], S.gridPoints[:, 1]
+        for sd in chunk:
+            ang, nb = sample_params(sd)
+            qk2 = contrast_qk2(S, kappa, sd, nb)
+            qk2s.append(qk2)
+            if mixed and sd % 2:
+                B.append(-qk2 * random_field(S, kappa, sd))
+            else:
+                B.append(-qk2 * np.exp(1j * kappa * (x * np.cos(ang) + y * np.sin(ang))))
+        S.set_contrast(np.stack(qk2s, axis=1))
+        X, st, rel = S.gmres_batch(np.stack(B, axis=1), restart_len=restart_len, max_restarts=30, tol=tol)
+        if st["converged"] != len(chunk):
+            raise RuntimeError(f"training solves did not converge: {st}")
+        iters.append(st["iterations"])
+        for j, sd in enumerate(chunk):
+            b = B[j]
+            sigma = np.sqrt(np.mean(np.abs(b) ** 2))
+            gb = S.to_grid(b / sigma)
+            gx = S.to_grid((X[:, j] - b) / sigma)
+            gq = S.to_grid(qk2s[j].astype(complex) / kappa ** 2).real
+            xs.append(np.stack([gq, gb.real, gb.imag], axis=-1).astype(np.float32))
+            ts.append(np.stack([gx.real, gx.imag], axis=-1).astype(np.float32))
+    return np.stack(xs), np.stack(ts), iters
+
+
+def evaluate(S, kappa, seeds, tol=1e-8, restart_len=50):
+    """Per held-out system: plain GMRES vs learned initial guess + flexible GMRES;
+    returns a dict of iteration counts, times and FP64 true residuals."""
+    out = {"plain_it": [], "learned_it": [], "guess_only_it": [], "plain_s": [], "learned_s": [], "guess_only_s": [],
+           "initial_residual_learned": [], "max_rel_residual": 0.0}
+    x, y = S.gridPoints[:, 0], S.gridPoints[:, 1]
+    for sd in seeds:
+        ang, nb = sample_params(sd)
+        qk2 = contrast_qk2(S, kappa, sd, nb)
+        S.set_contrast(qk2)
+        S.attach_preconditioner(S._net)        # q channel of the grid maps follows the contrast
+        b = -qk2 * np.exp(1j * kappa * (x * np.cos(ang) + y * np.sin(ang)))
+        t0 = time.perf_counter()
+        _, sp = S.gmres_ex(b, mode="fp64", restart_len=restart_len, max_restarts=30, tol=tol)
+        t1 = time.perf_counter()
+        xl, sl = S.gmres_learned(b, guess=True, right=True, restart_len=restart_len, max_restarts=30, tol=tol)
+        t2 = time.perf_counter()
+        _, sg = S.gmres_learned(b, guess=True, right=False, restart_len=restart_len, max_restarts=30, tol=tol)
+        t3 = time.perf_counter()
+        x0 = S.apply_preconditioner(b)
+        r0 = float(np.linalg.norm(b - S.MatVecProduct(x0)) / np.linalg.norm(b))
+        for st in (sp, sl, sg):
+            if st["status"] != 0:
+                raise RuntimeError(f"evaluation solve failed: {st}")
+        out["plain_it"].append(sp["iterations"])
+        out["learned_it"].append(sl["iterations"])
+        out["guess_only_it"].append(sg["iterations"])
+        out["plain_s"].append(t1 - t0)
+        out["learned_s"].append(t2 - t1)
+        out["guess_only_s"].append(t3 - t2)
+        out["initial_residual_learned"].append(r0)
+        out["max_rel_residual"] = max(out["max_rel_residual"], sp["rel_residual"], sl["rel_residual"],
+                                      sg["rel_residual"])
+    return out

@@ -16,10 +16,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = []
-    for h in ("lsg.h", "lsg_host.h"):
+    for h in ("lsg.h", "lsg_host.h", "lsg_learn.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(lsg_[a-z_0-9]+)\s*\(", src, flags=re.M)
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(lsgn?_[a-z_0-9]+)\s*\(", src, flags=re.M)
     return sorted(set(names))
 
 
