@@ -169,6 +169,119 @@ def pmc_probe(S):
     log(f"pmc probe: copy {nb >> 20} MiB, matvec {ms * 1e3:.1f} us")
 
 
+def seeded_angle(seed):
+    """First draw of the SplitMix64 stream seeded with 1000 + seed, scaled to [0, 2 pi)."""
+    z = (1000 + seed + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    z ^= z >> 31
+    return 2 * np.pi * ((z >> 11) * 2.0 ** -53)
+
+
+def bench_c3(args):
+    """Config C3: depth 7 (16,384 leaves, 589,824 unknowns), kappa 160, 8 seeded
+    bumps. Plain FP64 GMRES, FP32-operator GMRES inside FP64 refinement, and 64
+    plane waves through batched GMRES; every result is FP64-verified."""
+    inp = lsg.InputsToDFMM(kappa=160.0, nLevelsUniform=7, L=0.5, nChebNodes=6, degreeOfBases=6, TOL_POW=10,
+                           nCones_LFR=16)
+    inp.contrast = lsg.bumps_from_seed(0, 8)
+    inp.inc_angle = seeded_angle(0)
+    t0 = time.time()
+    S = lsg.DFMM(inp)
+    S.FMMTreeInitialisation()
+    info = S.info()
+    log(f"C3 setup {time.time() - t0:.1f} s: N={info['N']} E_far={info['E_far']} bytes/matvec={info['bytes_matvec']}")
+    b = S.rhs()
+    S.gmres_ex(b, mode="fp64", restart_len=50, max_restarts=40, tol=args.tol)       # warm-up
+    t0 = time.perf_counter()
+    x64, s64 = S.gmres_ex(b, mode="fp64", restart_len=50, max_restarts=40, tol=args.tol)
+    t64 = time.perf_counter() - t0
+    log(f"C3 plain FP64: {t64:.3f} s {s64}")
+    S.gmres_ex(b, mode="fp32ir", restart_len=50, max_restarts=1, tol=args.tol, inner_tol=1e-5, max_outer=20)
+    t0 = time.perf_counter()
+    x32, s32 = S.gmres_ex(b, mode="fp32ir", restart_len=50, max_restarts=1, tol=args.tol, inner_tol=1e-5,
+                          max_outer=20)
+    t32 = time.perf_counter() - t0
+    log(f"C3 FP32-IR: {t32:.3f} s {s32}")
+    for st in (s64, s32):
+        if st["status"] != 0 or not st["rel_residual"] <= args.tol:
+            raise RuntimeError(f"C3 solve did not verify: {st}")
+    mv64 = S.time_matvec(reps=5)
+    mv32 = S.time_matvec_fp32(reps=5)
+    fp32_err = float(np.linalg.norm(S.MatVecProduct_fp32(b) - S.MatVecProduct(b)) / np.linalg.norm(S.MatVecProduct(b)))
+    plain_sps = 1.0 / t64
+    batched = batched_leg(S, info, args, st_plain=s64, plain_sps=plain_sps)
+    log(f"C3 batched: {batched}")
+    out = {"metric": METRIC, "value": plain_sps, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": 1e3 * t64, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c128", "data": "synthetic",
+           "config": {"workload": "C3: uniform depth 7 (16384 leaves x 36 = 589824 unknowns), kappa 160, 8 seeded "
+                                  f"Gaussian bumps, NCA tol 1e-10, GMRES tol {args.tol:g}", "N": int(info["N"]),
+                      "iterations_per_solve": s64["iterations"], "restarts_per_solve": s64["restarts"],
+                      "final_rel_residual": s64["rel_residual"]},
+           "matvec": {"ms": mv64, "algorithmic_bytes": int(info["bytes_matvec"]),
+                      "GBps": info["bytes_matvec"] / (mv64 * 1e-3) / 1e9,
+                      "frac_hbm": info["bytes_matvec"] / (mv64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "fp32_ms": mv32, "fp32_rel_err_vs_fp64": fp32_err},
+           "fp32_ir": {"solves_per_s": 1.0 / t32, "speedup_vs_plain_fp64": t64 / t32,
+                       "inner_iterations": s32["iterations"], "refinements": s32["refinements"],
+                       "final_rel_residual": s32["rel_residual"],
+                       "rel_diff_vs_fp64_solution": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))},
+           "batched_64_angles": batched}
+    print(json.dumps(out), flush=True)
+    S.close()
+
+
+def bench_learned(args):
+    """The learned preconditioner at the configuration it is specified for:
+    trained on 4096 solves the solver generates (depth 6, k in {40, 50, 60, 70,
+    80}, 3-8 seeded bumps, plane waves at random angles and random-field right-hand
+    sides), evaluated on 256 held-out systems against plain GMRES."""
+    from lippmann_schwinger_solver_amd import learned as LN
+    kappas = [40.0, 50.0, 60.0, 70.0, 80.0]
+    net = LN.LearnedNet(n_layers=6, width=32, dilations=(1, 2, 4, 8, 4, 1), grid=128, max_batch=64, seed=0)
+    t0 = time.time()
+    ctxs = {k: LN.volume_context(k, 6, net) for k in kappas}
+    log(f"volume potentials for {len(kappas)} wavenumbers: {time.time() - t0:.1f} s")
+    n_train, n_eval = 4096, 256
+    per = n_train // len(kappas)
+    xs, ts = [], []
+    t0 = time.time()
+    for i, k in enumerate(kappas):
+        n = per if i < len(kappas) - 1 else n_train - per * (len(kappas) - 1)
+        x, t, _ = LN.make_pairs(ctxs[k], k, list(range(i * 100_000, i * 100_000 + n)), tol=args.tol)
+        xs.append(x)
+        ts.append(t)
+    X, T = np.concatenate(xs), np.concatenate(ts)
+    t_data = time.time() - t0
+    log(f"training set: {X.shape[0]} solves in {t_data:.1f} s")
+    net.set_training_data(X, T)
+    t0 = time.time()
+    hist = net.train(epochs=30, batch=64, lr=2e-3, lr_decay=0.92, log=log)
+    t_train = time.time() - t0
+    ev = {"plain_it": [], "learned_it": [], "guess_only_it": [], "plain_s": [], "learned_s": [], "guess_only_s": [],
+          "initial_residual_learned": [], "max_rel_residual": 0.0}
+    for i, k in enumerate(kappas):
+        e = LN.evaluate(ctxs[k], k, list(range(900_000 + i * 1000, 900_000 + i * 1000 + n_eval // len(kappas) +
+                                               (1 if i < n_eval % len(kappas) else 0))), tol=args.tol)
+        for key in ev:
+            if key == "max_rel_residual":
+                ev[key] = max(ev[key], e[key])
+            else:
+                ev[key] += e[key]
+    res = {"train_solves": int(X.shape[0]), "train_data_s": t_data, "train_s": t_train, "loss_first_epoch": hist[0],
+           "loss_last_epoch": hist[-1], "eval_systems": len(ev["plain_it"]),
+           "plain_mean_iterations": float(np.mean(ev["plain_it"])),
+           "learned_fgmres_mean_iterations": float(np.mean(ev["learned_it"])),
+           "learned_guess_only_mean_iterations": float(np.mean(ev["guess_only_it"])),
+           "plain_solves_per_s": len(ev["plain_s"]) / float(np.sum(ev["plain_s"])),
+           "learned_fgmres_solves_per_s": len(ev["learned_s"]) / float(np.sum(ev["learned_s"])),
+           "learned_guess_only_solves_per_s": len(ev["guess_only_s"]) / float(np.sum(ev["guess_only_s"])),
+           "mean_initial_residual_learned_guess": float(np.mean(ev["initial_residual_learned"])),
+           "max_rel_residual": ev["max_rel_residual"]}
+    print(json.dumps({"workload": "learned preconditioner (C5)", "result": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,11 +292,18 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
+    ap.add_argument("--workload", choices=["c2", "c3", "learned"], default="c2",
+                    help="c2: the default line; c3: depth 7, kappa 160 mixed-precision and batched legs; "
+                         "learned: train the network on 4096 generated solves and evaluate on 256 held-out")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver is single-device (multi-GPU sharding is a design "
               "note, not implemented); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return bench_c3(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     inp = c2_inputs(0)
     t0 = time.time()
