@@ -126,6 +126,7 @@ def batched_leg(S, info, args, st_plain, plain_sps):
     ptrs = (Bd.ptr.value, Xd.ptr.value)
     Xd.zero()
     S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=50, max_restarts=20, tol=args.tol)  # warm-up
+    log("batched FP64: warm-up done")
     t = 0.0
     for _ in range(args.steps):
         Xd.zero()
@@ -133,9 +134,26 @@ def batched_leg(S, info, args, st_plain, plain_sps):
         _, st, rel = S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=50, max_restarts=20,
                                    tol=args.tol)
         t += time.perf_counter() - t0
+        log(f"batched FP64 batch: {st}")
         if st["status"] != 0 or st["converged"] != 64 or not rel.max() <= args.tol:
             raise RuntimeError(f"batched solve did not verify: {st}")
     X = Xd.download().T
+    # the same batch with the FP32 block operator inside FP64 refinement
+    Xd.zero()
+    S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=50, max_restarts=1, tol=args.tol,
+                  mode="fp32ir", inner_tol=1e-5)   # warm-up
+    log("batched FP32-IR: warm-up done")
+    t32 = 0.0
+    for _ in range(args.steps):
+        Xd.zero()
+        t0 = time.perf_counter()
+        _, st32, rel32 = S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=50, max_restarts=1,
+                                       tol=args.tol, mode="fp32ir", inner_tol=1e-5)
+        t32 += time.perf_counter() - t0
+        log(f"batched FP32-IR batch: {st32}")
+        if st32["status"] != 0 or st32["converged"] != 64 or not rel32.max() <= args.tol:
+            raise RuntimeError(f"batched FP32-IR solve did not verify: {st32}")
+    X32 = Xd.download().T
     Bd.free()
     Xd.free()
     # independent FP64 residuals of a few systems through the single-vector matvec
@@ -143,13 +161,23 @@ def batched_leg(S, info, args, st_plain, plain_sps):
     mv_block = S.time_matvec_block(reps=5)
     flops = 8.0 * 64 * (info["E_far"] + info["E_near"])
     sps = 64 * args.steps / t
+    sps32 = 64 * args.steps / t32
+    spot32 = max(float(np.linalg.norm(S.MatVecProduct(X32[:, j]) - Bh[j]) / np.linalg.norm(Bh[j]))
+                 for j in (0, 21, 42, 63))
     return {"solves_per_s": sps, "speedup_vs_plain_fp64": sps / plain_sps, "ms_per_batch": 1e3 * t / args.steps,
             "arnoldi_steps": st["iterations"], "cycles": st["restarts"], "max_rel_residual": float(rel.max()),
             "spot_check_rel_residual": spot, "block_matvec_ms": mv_block,
             "block_matvec_us_per_rhs": 1e3 * mv_block / 64,
             "block_matvec_mfma": {"achieved_tflops": flops / (mv_block * 1e-3) / 1e12, "peak": MFMA_F64_PEAK_TFLOPS,
                                   "frac": flops / (mv_block * 1e-3) / 1e12 / MFMA_F64_PEAK_TFLOPS,
-                                  "flops_per_matvec": flops}}
+                                  "flops_per_matvec": flops},
+            "fp32_inner": {"solves_per_s": sps32, "speedup_vs_plain_fp64": sps32 / plain_sps,
+                           "block_matvec_fp32_ms": S.time_matvec_block(reps=5, fp32=True),
+                           "ms_per_batch": 1e3 * t32 / args.steps, "inner_arnoldi_steps": st32["iterations"],
+                           "refinements": st32["refinements"], "max_rel_residual": float(rel32.max()),
+                           "spot_check_rel_residual": spot32,
+                           "max_rel_diff_vs_fp64_batch": max(float(np.linalg.norm(X32[:, j] - X[:, j]) /
+                                                                   np.linalg.norm(X[:, j])) for j in range(64))}}
 
 
 def pmc_probe(S):

@@ -196,6 +196,8 @@ int lsg_set_contrast(lsg_ctx* ctx, const double* qk2, int ncol);
 /* Average device time (ms) of one block matvec (64 right-hand sides, FP64
  * MFMA) over `reps` products of a device-resident block. */
 int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms);
+/* Same for the FP32 block operator (FP32 operators and FP32 MFMA). */
+int lsg_time_matvec_block_fp32(lsg_ctx* ctx, int reps, double* ms);
 
 /* GMRES with solver options: lsg_gmres (mode LSG_SOLVE_FP64), or FP64
  * iterative refinement (mode LSG_SOLVE_FP32_IR): r = b - A x with the FP64
@@ -213,7 +215,9 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
  * advance in lockstep through one block matvec (FP64 MFMA) per Arnoldi step,
  * each with its own Krylov basis (classical Gram-Schmidt), Givens rotations and
  * stopping point; each cycle opens with the true FP64 residual of every
- * system. opts->mode must be LSG_SOLVE_FP64. rel_residuals (host, nrhs, may be
+ * system. opts->mode LSG_SOLVE_FP32_IR refines the batch in FP64 around
+ * batched GMRES with the FP32 block operator (FP32 MFMA), each system to
+ * max(inner_tol, tol ||b|| / (2 ||r||)). rel_residuals (host, nrhs, may be
  * NULL) receives the final true relative residuals; stats->rel_residual is
  * their maximum and stats->converged the number of systems at tol. */
 int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* opts,

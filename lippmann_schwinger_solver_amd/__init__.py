@@ -96,6 +96,7 @@ SYMBOLS = [
     ("lsg_matvec_fp32", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
     ("lsg_time_matvec_fp32", C.c_int, [_P, C.c_int, _D]),
     ("lsg_time_matvec_block", C.c_int, [_P, C.c_int, _D]),
+    ("lsg_time_matvec_block_fp32", C.c_int, [_P, C.c_int, _D]),
     ("lsg_set_contrast", C.c_int, [_P, _D, C.c_int]),
     # include/lsg_learn.h
     ("lsgn_create", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
@@ -396,12 +397,13 @@ class DFMM:
         return x, stats
 
     def gmres_batch(self, B, X0=None, restart_len=50, max_restarts=20, tol=1e-8, device_ptrs=None, ld=None,
-                    nrhs=None):
+                    nrhs=None, mode="fp64", inner_tol=1e-5, max_outer=20):
         """Batched GMRES on the columns of B (N x nrhs, nrhs <= 64); returns
         (X, stats, rel_residuals). With device_ptrs = (b_dev, x_dev) the
         column-major blocks (leading dimension ld) stay on the device and X is None."""
-        o = lsg_solve_opts(mode=LSG_SOLVE_FP64, restart_len=restart_len, max_restarts=max_restarts, tol=tol,
-                           inner_tol=0.0, max_outer=0, device_ptrs=1 if device_ptrs else 0)
+        o = lsg_solve_opts(mode={"fp64": LSG_SOLVE_FP64, "fp32ir": LSG_SOLVE_FP32_IR}[mode], restart_len=restart_len,
+                           max_restarts=max_restarts, tol=tol, inner_tol=inner_tol, max_outer=max_outer,
+                           device_ptrs=1 if device_ptrs else 0)
         st = lsg_stats()
         if device_ptrs:
             rel = np.zeros(nrhs)
@@ -490,10 +492,11 @@ class DFMM:
         stats["status"] = rc
         return x, stats
 
-    def time_matvec_block(self, reps=10):
+    def time_matvec_block(self, reps=10, fp32=False):
         """Average device time (ms) of one 64-right-hand-side block matvec."""
         ms = C.c_double()
-        self._check(self._lib.lsg_time_matvec_block(self._ctx, reps, C.byref(ms)))
+        f = self._lib.lsg_time_matvec_block_fp32 if fp32 else self._lib.lsg_time_matvec_block
+        self._check(f(self._ctx, reps, C.byref(ms)))
         return ms.value
 
     def time_matvec_fp32(self, reps=20):

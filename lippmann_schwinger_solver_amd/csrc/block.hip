@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace lsg {
@@ -16,6 +17,8 @@ constexpr int kNB = kBlockRhs;      // 64 right-hand sides = 4 column tiles of 1
 constexpr int kLD = 2 * kNB;        // doubles per block row
 constexpr int kBCK = 32;            // complex operator columns staged in LDS per step
 constexpr int kBMaxRows = 64;       // complex rows per destination (8 row tiles of 16 reals)
+constexpr int kSplitCols = 256;     // columns per workgroup in split passes
+constexpr int kSplitMaxDest = 2048; // passes with fewer destinations (and longer rows) are split
 
 #define HIP_OK(expr)                                                              \
   do {                                                                            \
@@ -23,53 +26,81 @@ constexpr int kBMaxRows = 64;       // complex rows per destination (8 row tiles
     if (e_ != hipSuccess) return std::string(#expr ": ") + hipGetErrorString(e_); \
   } while (0)
 
+// MFMA tile traits per precision: 16 x 16 x 4 products; the result lane l holds
+// column l%16 and rows row(l/16, v), v < 4 (FP64: l/16 + 4v; FP32: 4(l/16) + v).
+template <class R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc __attribute__((ext_vector_type(4)));
+  typedef double2 c2;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int k4, int v) { return k4 + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc __attribute__((ext_vector_type(4)));
+  typedef float2 c2;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int k4, int v) { return 4 * k4 + v; }
+};
+
 // One workgroup per destination; wave w computes right-hand sides [16w, 16w+16)
 // for all row tiles. Real-ified operator fragment (16 x 4) per lane l:
 // row 16*mt + l%16, column 4*ks + l/16, taken from the complex entry in LDS;
 // the source fragment (4 x 16): real row 4*ks + l/16 (column c = 2ks + (l/16)/2,
-// Re or Im), right-hand side l%16. Result lane l holds rows l/16 + 4v, v < 4,
-// column l%16 of each 16 x 16 tile.
-template <bool ACC>
-__global__ __launch_bounds__(kT) void bpass_kernel(const double2* __restrict__ ops, const int* __restrict__ src,
-                                                   double* bst, const double* __restrict__ X, double* Y, int N,
-                                                   const int* __restrict__ rows, const int* __restrict__ ncol,
-                                                   const int* __restrict__ dst, const long long* __restrict__ op_off,
-                                                   const long long* __restrict__ src_off) {
-  __shared__ double2 As[kBMaxRows * kBCK];
+// Re or Im), right-hand side l%16. R is the pass precision (operators, pass
+// state, accumulation); the input X and output Y blocks are FP64. Source rows
+// below nx are read from X, the others from the pass state (the FP32 path
+// copies X into the state once, rounded, and passes nx = 0).
+template <bool ACC, class R, int MT>
+__global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* __restrict__ ops,
+                                                   const int* __restrict__ src, R* bst, const double* __restrict__ X,
+                                                   double* Y, int N, const int* __restrict__ rows,
+                                                   const int* __restrict__ ncol, const int* __restrict__ dst,
+                                                   const long long* __restrict__ op_off,
+                                                   const long long* __restrict__ src_off, int nx, int kseg, int nseg,
+                                                   R* __restrict__ part) {
+  typedef typename Mfma<R>::c2 C2;
+  typedef typename Mfma<R>::acc Acc;
+  __shared__ C2 As[kBMaxRows * kBCK];
   __shared__ int Ss[kBCK];
-  const int d = blockIdx.x;
-  const int R = rows[d], C = ncol[d];
-  const double2* A = ops + op_off[d];
+  // split passes (kseg > 0): workgroup (d, sg) takes columns [sg kseg, (sg+1) kseg) of
+  // destination d and writes its partial product to part; bsplit_reduce_kernel sums them
+  const int d = kseg > 0 ? blockIdx.x / nseg : blockIdx.x;
+  const int sg = kseg > 0 ? blockIdx.x % nseg : 0;
+  const int Rw = rows[d];
+  const int Cfull = ncol[d];
+  const int cbeg = kseg > 0 ? min(Cfull, sg * kseg) : 0;
+  const int C = kseg > 0 ? min(Cfull, cbeg + kseg) : Cfull;
+  const C2* A = ops + op_off[d];
   const int* S = src + src_off[d];
   const int dd = dst[d];
   const int t = threadIdx.x, lane = t & 63, nt = t >> 6;
   const int i16 = lane & 15, k4 = lane >> 4;
-  const int MT = (2 * R + 15) >> 4;
   const int q = (nt << 4) + i16;
-  d4 acc[8];
+  Acc acc[MT];   // MT row tiles of 16 reals: 2 * (rows of every destination of the pass) <= 16 MT
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt) acc[mt] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int c0 = 0; c0 < C; c0 += kBCK) {
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
+  for (int c0 = cbeg; c0 < C; c0 += kBCK) {
     const int Cc = min(kBCK, C - c0);
     __syncthreads();
-    const double2* Ac = A + (size_t)c0 * R;  // the chunk's columns are contiguous (column-major)
-    typedef double dv2 __attribute__((ext_vector_type(2)));
-    const dv2* Av = reinterpret_cast<const dv2*>(Ac);
-    for (int e = t; e < Cc * R; e += kT) {
-      const dv2 a = __builtin_nontemporal_load(Av + e);
-      As[e] = make_double2(a.x, a.y);
-    }
+    const C2* Ac = A + (size_t)c0 * Rw;  // the chunk's columns are contiguous (column-major)
+    for (int e = t; e < Cc * Rw; e += kT) As[e] = Ac[e];
     if (t < Cc) Ss[t] = S[c0 + t];
     __syncthreads();
-    double bv[kBCK / 2];
+    R bv[kBCK / 2];
 #pragma unroll
     for (int ks = 0; ks < kBCK / 2; ++ks) {
       const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
-      bv[ks] = 0.0;
+      bv[ks] = 0;
       if (cl < Cc) {
         const int row = Ss[cl];
-        const double* base = row < N ? X + (size_t)row * kLD : bst + (size_t)row * kLD;
-        bv[ks] = base[b * kNB + q];
+        bv[ks] = row < nx ? R(X[(size_t)row * kLD + b * kNB + q]) : bst[(size_t)row * kLD + b * kNB + q];
       }
     }
 #pragma unroll
@@ -77,38 +108,76 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const double2* __restrict__ o
       if (2 * ks < Cc) {
         const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt) {
-          if (mt < MT) {
-            const int ri = (mt << 4) + i16;
-            const int r = ri >> 1, a = ri & 1;
-            double av = 0.0;
-            if (r < R && cl < Cc) {
-              const double* e = reinterpret_cast<const double*>(&As[cl * R + r]);
-              av = e[a == b ? 0 : 1];
-              if (a == 0 && b == 1) av = -av;
-            }
-            acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks], acc[mt], 0, 0, 0);
+        for (int mt = 0; mt < MT; ++mt) {
+          const int ri = (mt << 4) + i16;
+          const int r = ri >> 1, a = ri & 1;
+          R av = 0;
+          if (r < Rw && cl < Cc) {
+            const R* e = reinterpret_cast<const R*>(&As[cl * Rw + r]);
+            av = e[a == b ? 0 : 1];
+            if (a == 0 && b == 1) av = -av;
           }
+          acc[mt] = Mfma<R>::op(av, bv[ks], acc[mt]);
         }
       }
     }
   }
+  if (kseg > 0) {
+    R* P = part + (size_t)blockIdx.x * (MT * 16) * kNB;
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt) {
-    if (mt < MT) {
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int ri = (mt << 4) + k4 + 4 * v;
-        const int r = ri >> 1, a = ri & 1;
-        if (r < R) {
-          const int row = dd + r;
-          double* p = (row >= N && row < 2 * N) ? Y + (size_t)(row - N) * kLD : bst + (size_t)row * kLD;
-          p += a * kNB + q;
-          double val = acc[mt][v];
+      for (int v = 0; v < 4; ++v) P[(size_t)((mt << 4) + Mfma<R>::row(k4, v)) * kNB + q] = acc[mt][v];
+    return;
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ri = (mt << 4) + Mfma<R>::row(k4, v);
+      const int r = ri >> 1, a = ri & 1;
+      if (r < Rw) {
+        const int row = dd + r;
+        if (row >= N && row < 2 * N) {
+          double* p = Y + (size_t)(row - N) * kLD + a * kNB + q;
+          double val = double(acc[mt][v]);
+          if (ACC) val += *p;
+          *p = val;
+        } else {
+          R* p = bst + (size_t)row * kLD + a * kNB + q;
+          R val = acc[mt][v];
           if (ACC) val += *p;
           *p = val;
         }
       }
+    }
+  }
+}
+
+// Sum of the nseg partial products of destination blockIdx.x (tiles of rt real
+// rows x 64), stored or accumulated into its rows (Y or the pass state).
+template <bool ACC, class R>
+__global__ __launch_bounds__(kT) void bsplit_reduce_kernel(const R* __restrict__ part, int nseg, int rt,
+                                                           const int* __restrict__ rows, const int* __restrict__ dst,
+                                                           R* bst, double* Y, int N) {
+  const int d = blockIdx.x;
+  const int Rw = rows[d], dd = dst[d];
+  const R* P = part + (size_t)d * nseg * rt * kNB;
+  for (int e = threadIdx.x; e < 2 * Rw * kNB; e += kT) {
+    const int ri = e / kNB, q = e - ri * kNB;
+    R s = 0;
+    for (int sg = 0; sg < nseg; ++sg) s += P[((size_t)sg * rt + ri) * kNB + q];
+    const int r = ri >> 1, a = ri & 1, row = dd + r;
+    if (row >= N && row < 2 * N) {
+      double* p = Y + (size_t)(row - N) * kLD + a * kNB + q;
+      double val = double(s);
+      if (ACC) val += *p;
+      *p = val;
+    } else {
+      R* p = bst + (size_t)row * kLD + a * kNB + q;
+      R val = s;
+      if (ACC) val += *p;
+      *p = val;
     }
   }
 }
@@ -210,17 +279,18 @@ struct BGivens {
   double* bnorm;  // [q]
   int* active;    // [q]
   int* kq;        // [q] Arnoldi steps applied in this cycle
+  double* tolq;   // [q] relative tolerance of system q
 };
 
 // Cycle start per system q: beta = ||r_q|| (norm column of h), relative
 // residual to host_rel[q]; systems already at tol (or padding) are inactive.
-__global__ void bcycle_init_kernel(const double2* __restrict__ h, BGivens st, int m, int nrhs, double tol,
+__global__ void bcycle_init_kernel(const double2* __restrict__ h, BGivens st, int m, int nrhs,
                                    double* host_rel, int* host_active) {
   const int q = threadIdx.x;
   if (q >= kNB) return;
   const double beta = sqrt(fmax(h[q].x, 0.0));
   const double rel = beta / st.bnorm[q];
-  const int act = (q < nrhs && rel > tol && beta > 0.0) ? 1 : 0;
+  const int act = (q < nrhs && rel > st.tolq[q] && beta > 0.0) ? 1 : 0;
   st.active[q] = act;
   st.kq[q] = 0;
   st.hn[q] = act ? beta : 0.0;
@@ -241,7 +311,7 @@ __global__ void bscale_kernel(const double* __restrict__ w, BGivens st, double* 
 // block dot); ||w - V h|| by Pythagoras; Givens rotation of H column j; the
 // relative estimate |g_{j+1}| / ||b_q|| goes to host_est[j*64 + q]. A system
 // whose estimate reaches 0.5 tol, or whose new vector vanishes, is frozen.
-__global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, int j, double tol, double* host_est,
+__global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, int j, double* host_est,
                              int* host_active) {
   const int q = threadIdx.x;
   if (q >= kNB) return;
@@ -292,7 +362,7 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
   const double est = hypot(g1.x, g1.y) / st.bnorm[q];
   st.kq[q] = j + 1;
   st.hn[q] = hn;
-  const bool stop = est <= 0.5 * tol || !(hn > 1e-14 * sqrt(fmax(wn2, 0.0)));
+  const bool stop = est <= 0.5 * st.tolq[q] || !(hn > 1e-14 * sqrt(fmax(wn2, 0.0)));
   if (stop) st.active[q] = 0;
   host_est[j * kNB + q] = est;
   host_active[q] = stop ? 0 : 1;
@@ -379,6 +449,18 @@ __global__ __launch_bounds__(kT) void baxpy_kernel(const double* __restrict__ V,
   X[base + kNB] += ai;
 }
 
+// zero the columns q of a block whose mask[q] is 0
+__global__ void bmask_kernel(double* __restrict__ V, const int* __restrict__ mask, long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total && !mask[i % kNB]) V[i] = 0.0;
+}
+
+// X += D
+__global__ void badd_kernel(double* __restrict__ X, const double* __restrict__ D, long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total) X[i] += D[i];
+}
+
 // R = B - R
 __global__ void bsub_kernel(const double* __restrict__ B, double* __restrict__ R, long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,6 +513,31 @@ std::string BlockOperator::setup(DeviceOperator& o) {
     for (int r : rows_h)
       if (r > kBMaxRows) return "block matvec: a destination has more than 64 rows";
     if (o.rank > kBMaxRows) return "block matvec: rank above 64";
+    pass_max_rows.assign(o.passes.size(), 1);
+    for (size_t pi = 0; pi < o.passes.size(); ++pi)
+      for (int d = 0; d < o.passes[pi].ndest; ++d)
+        pass_max_rows[pi] = std::max(pass_max_rows[pi], rows_h[size_t(o.passes[pi].dest_base + d)]);
+    // passes with few destinations and long operators are split over columns
+    std::vector<int> ncol_h(rows_h.size());
+    if (!ncol_h.empty())
+      HIP_OK(hipMemcpy(ncol_h.data(), o.d_ncol, ncol_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+    pass_nseg.assign(o.passes.size(), 1);
+    size_t part_elems = 0;
+    for (size_t pi = 0; pi < o.passes.size(); ++pi) {
+      const DevPass& p = o.passes[pi];
+      int64_t cols = 0;
+      int cmax = 0;
+      for (int d = 0; d < p.ndest; ++d) {
+        cols += ncol_h[size_t(p.dest_base + d)];
+        cmax = std::max(cmax, ncol_h[size_t(p.dest_base + d)]);
+      }
+      if (p.ndest > 0 && p.ndest < kSplitMaxDest && cols > int64_t(kSplitCols) * p.ndest) {
+        pass_nseg[pi] = (cmax + kSplitCols - 1) / kSplitCols;
+        const int rt = 16 * std::max(1, (2 * pass_max_rows[pi] + 15) / 16);
+        part_elems = std::max(part_elems, size_t(p.ndest) * pass_nseg[pi] * rt * kNB);
+      }
+    }
+    if (part_elems) HIP_OK(hipMalloc((void**)&split_part, part_elems * sizeof(double)));
   }
   HIP_OK(hipMalloc((void**)&bstate, sizeof(double) * size_t(o.n_state + 1) * kLD));
   HIP_OK(hipMemset(bstate, 0, sizeof(double) * size_t(o.n_state + 1) * kLD));
@@ -465,8 +572,12 @@ std::string BlockOperator::setup(DeviceOperator& o) {
 }
 
 void BlockOperator::release() {
-  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff, qk2_cols};
+  work.release();
+  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff, qk2_cols, bstate32,
+                  split_part};
   qk2_cols = nullptr;
+  bstate32 = nullptr;
+  split_part = nullptr;
   for (void* p : ptrs)
     if (p) hipFree(p);
   bstate = nullptr;
@@ -475,84 +586,174 @@ void BlockOperator::release() {
   op = nullptr;
 }
 
-static void launch_bpass(BlockOperator& bo, const DevPass& p, const double* X, double* Y) {
-  DeviceOperator& o = *bo.op;
-  const int64_t b = p.dest_base;
-  if (p.kind == 0)
-    hipLaunchKernelGGL(bpass_kernel<false>, dim3(p.ndest), dim3(kT), 0, o.stream, o.ops, o.src, bo.bstate, X, Y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
-  else
-    hipLaunchKernelGGL(bpass_kernel<true>, dim3(p.ndest), dim3(kT), 0, o.stream, o.ops, o.src, bo.bstate, X, Y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+template <bool ACC, class R, int MT>
+static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename Mfma<R>::c2* ops, const int* src,
+                            R* bst, const double* X, double* Y, const int* rows, const int* ncol, const int* dst,
+                            const long long* op_off, const long long* src_off, int nseg) {
+  const int nx = std::is_same<R, double>::value ? bo.op->N : 0;
+  hipStream_t s = bo.op->stream;
+  if (nseg <= 1) {
+    hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N, rows,
+                       ncol, dst, op_off, src_off, nx, 0, 1, (R*)nullptr);
+    return;
+  }
+  R* part = reinterpret_cast<R*>(bo.split_part);
+  hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest * nseg), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N,
+                     rows, ncol, dst, op_off, src_off, nx, kSplitCols, nseg, part);
+  hipLaunchKernelGGL((bsplit_reduce_kernel<ACC, R>), dim3(ndest), dim3(kT), 0, s, part, nseg, MT * 16, rows, dst, bst,
+                     Y, bo.op->N);
 }
 
-void BlockOperator::matvec(const double* X, double* Y) {
-  DeviceOperator& o = *op;
-  for (int p : o.up) launch_bpass(*this, o.passes[p], X, Y);
-  hipLaunchKernelGGL(bpass_kernel<false>, dim3(o.n_leaves), dim3(kT), 0, o.stream, o.T, near_src, bstate, X, Y, o.N,
-                     near_rows, near_ncol, near_dst, near_op, near_srcoff);
+// row tiles from the largest destination of the pass
+template <bool ACC, class R>
+static void launch_bpass_any(const BlockOperator& bo, int max_rows, int ndest, const typename Mfma<R>::c2* ops,
+                             const int* src, R* bst, const double* X, double* Y, const int* rows, const int* ncol,
+                             const int* dst, const long long* op_off, const long long* src_off, int nseg = 1) {
+  const int mt = std::max(1, (2 * max_rows + 15) / 16);
+#define LSG_BPASS_CASE(M)                                                                                       \
+  case M:                                                                                                       \
+    launch_bpass_mt<ACC, R, M>(bo, ndest, ops, src, bst, X, Y, rows, ncol, dst, op_off, src_off, nseg);        \
+    break;
+  switch (mt) {
+    LSG_BPASS_CASE(1)
+    LSG_BPASS_CASE(2)
+    LSG_BPASS_CASE(3)
+    LSG_BPASS_CASE(4)
+    LSG_BPASS_CASE(5)
+    LSG_BPASS_CASE(6)
+    LSG_BPASS_CASE(7)
+    default:
+      launch_bpass_mt<ACC, R, 8>(bo, ndest, ops, src, bst, X, Y, rows, ncol, dst, op_off, src_off, nseg);
+  }
+#undef LSG_BPASS_CASE
+}
+
+template <class R>
+static void launch_bpass(BlockOperator& bo, int pidx, const double* X, double* Y) {
+  DeviceOperator& o = *bo.op;
+  const DevPass& p = o.passes[size_t(pidx)];
+  const int64_t b = p.dest_base;
+  typedef typename Mfma<R>::c2 C2;
+  const C2* ops = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.ops : (const void*)o.ops32);
+  R* bst = reinterpret_cast<R*>(std::is_same<R, double>::value ? (void*)bo.bstate : (void*)bo.bstate32);
+  const int mr = bo.pass_max_rows[size_t(pidx)];
+  const int nseg = bo.pass_nseg[size_t(pidx)];
+  if (p.kind == 0)
+    launch_bpass_any<false, R>(bo, mr, p.ndest, ops, o.src, bst, X, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
+                               o.d_op + b, o.d_srcoff + b, nseg);
+  else
+    launch_bpass_any<true, R>(bo, mr, p.ndest, ops, o.src, bst, X, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
+                              o.d_op + b, o.d_srcoff + b, nseg);
+}
+
+// FP32 copy of the input block into the first N rows of the FP32 pass state
+__global__ void bround_kernel(const double* __restrict__ X, float* __restrict__ bst32, long long n_total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_total) bst32[i] = float(X[i]);
+}
+
+template <class R>
+static void block_matvec_impl(BlockOperator& bo, const double* X, double* Y) {
+  DeviceOperator& o = *bo.op;
+  typedef typename Mfma<R>::c2 C2;
+  const C2* T = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
+  R* bst = reinterpret_cast<R*>(std::is_same<R, double>::value ? (void*)bo.bstate : (void*)bo.bstate32);
+  if (!std::is_same<R, double>::value)
+    hipLaunchKernelGGL(bround_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
+                       bo.bstate32, (long long)o.N * kLD);
+  for (int p : o.up) launch_bpass<R>(bo, p, X, Y);
+  launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, X, Y, bo.near_rows, bo.near_ncol,
+                             bo.near_dst, bo.near_op, bo.near_srcoff);
   if (o.find_phi)
     hipLaunchKernelGGL(bnear_finish_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
                        o.qk2, (long long)o.N * kLD);
-  for (int p : o.down) launch_bpass(*this, o.passes[p], X, Y);
-  for (int p : o.leaf_m2l) launch_bpass(*this, o.passes[p], X, Y);
-  for (int p : o.leaf_l2l) launch_bpass(*this, o.passes[p], X, Y);
-  if (qk2_cols)
-    hipLaunchKernelGGL(bdiag_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y, qk2_cols,
-                       (long long)o.N * kLD);
+  for (int p : o.down) launch_bpass<R>(bo, p, X, Y);
+  for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, X, Y);
+  for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, X, Y);
+  if (bo.qk2_cols)
+    hipLaunchKernelGGL(bdiag_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
+                       bo.qk2_cols, (long long)o.N * kLD);
+}
+
+void BlockOperator::matvec(const double* X, double* Y) { block_matvec_impl<double>(*this, X, Y); }
+void BlockOperator::matvec32(const double* X, double* Y) { block_matvec_impl<float>(*this, X, Y); }
+
+std::string BlockOperator::enable_fp32() {
+  std::string e = op->enable_fp32();
+  if (!e.empty()) return e;
+  if (bstate32) return "";
+  HIP_OK(hipMalloc((void**)&bstate32, sizeof(float) * size_t(op->n_state + 1) * kLD));
+  HIP_OK(hipMemset(bstate32, 0, sizeof(float) * size_t(op->n_state + 1) * kLD));
+  return "";
+}
+
+std::string BlockWork::reserve(int N_, int m_) {
+  const long long vs = (long long)N_ * kLD;
+  const int nrb = (N_ + kBRows - 1) / kBRows;
+  HIP_OK(hipMalloc((void**)&V, sizeof(double) * size_t(vs) * (m_ + 1)));
+  HIP_OK(hipMalloc((void**)&w, sizeof(double) * size_t(vs)));
+  HIP_OK(hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m_ + 2) * kNB));
+  HIP_OK(hipMalloc((void**)&h, sizeof(double2) * size_t(m_ + 2) * kNB));
+  HIP_OK(hipMalloc((void**)&y, sizeof(double2) * size_t(m_ + 1) * kNB));
+  HIP_OK(hipMalloc((void**)&H, sizeof(double2) * size_t(m_ + 1) * m_ * kNB));
+  HIP_OK(hipMalloc((void**)&cs, sizeof(double) * size_t(m_) * kNB));
+  HIP_OK(hipMalloc((void**)&sn, sizeof(double2) * size_t(m_) * kNB));
+  HIP_OK(hipMalloc((void**)&g, sizeof(double2) * size_t(m_ + 1) * kNB));
+  HIP_OK(hipMalloc((void**)&hn, sizeof(double) * kNB));
+  HIP_OK(hipMalloc((void**)&bnorm, sizeof(double) * kNB));
+  HIP_OK(hipMalloc((void**)&active, sizeof(int) * kNB));
+  HIP_OK(hipMalloc((void**)&kq, sizeof(int) * kNB));
+  HIP_OK(hipMalloc((void**)&tolq, sizeof(double) * kNB));
+  HIP_OK(hipHostMalloc((void**)&host_rel, sizeof(double) * kNB, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&host_est, sizeof(double) * size_t(m_) * kNB, hipHostMallocDefault));
+  HIP_OK(hipHostMalloc((void**)&host_active, sizeof(int) * kNB, hipHostMallocDefault));
+  m = m_;
+  N = N_;
+  return "";
+}
+
+void BlockWork::release() {
+  void* ptrs[] = {V, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (host_rel) (void)hipHostFree(host_rel);
+  if (host_est) (void)hipHostFree(host_est);
+  if (host_active) (void)hipHostFree(host_active);
+  *this = BlockWork{};
 }
 
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
-                        double* rel_residuals) {
+                        double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
   BatchResult res;
   auto t0 = std::chrono::steady_clock::now();
   DeviceOperator& o = *bo.op;
   hipStream_t s = o.stream;
   const int N = o.N;
-  if (nrhs < 1 || nrhs > kNB || m < 1 || m > 500 || max_restarts < 1 || !(tol > 0)) {
+  if (nrhs < 1 || nrhs > kNB || m < 1 || m > 500 || max_restarts < 1 || (!(tol > 0) && !tol_per)) {
     res.status = LSG_ERR_ARG;
     res.error = "batched GMRES: 1 <= nrhs <= 64, 1 <= m <= 500, max_restarts >= 1, tol > 0";
     return res;
   }
   const long long vs = (long long)N * kLD;  // doubles per block vector
   const int nrb = (N + kBRows - 1) / kBRows;
-  std::vector<void*> owned;
-  auto alloc = [&](void** p, size_t bytes) {
-    if (hipMalloc(p, bytes) != hipSuccess) return false;
-    owned.push_back(*p);
-    return true;
-  };
-  double *V = nullptr, *w = nullptr;
-  double2 *partial = nullptr, *h = nullptr, *y = nullptr;
-  BGivens st{};
-  double *host_rel = nullptr, *host_est = nullptr;
-  int* host_active = nullptr;
-  bool ok = alloc((void**)&V, sizeof(double) * size_t(vs) * (m + 1)) && alloc((void**)&w, sizeof(double) * size_t(vs)) &&
-            alloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m + 2) * kNB) &&
-            alloc((void**)&h, sizeof(double2) * size_t(m + 2) * kNB) &&
-            alloc((void**)&y, sizeof(double2) * size_t(m + 1) * kNB) &&
-            alloc((void**)&st.H, sizeof(double2) * size_t(m + 1) * m * kNB) &&
-            alloc((void**)&st.cs, sizeof(double) * size_t(m) * kNB) &&
-            alloc((void**)&st.sn, sizeof(double2) * size_t(m) * kNB) &&
-            alloc((void**)&st.g, sizeof(double2) * size_t(m + 1) * kNB) && alloc((void**)&st.hn, sizeof(double) * kNB) &&
-            alloc((void**)&st.bnorm, sizeof(double) * kNB) && alloc((void**)&st.active, sizeof(int) * kNB) &&
-            alloc((void**)&st.kq, sizeof(int) * kNB);
-  ok = ok && hipHostMalloc((void**)&host_rel, sizeof(double) * kNB, hipHostMallocDefault) == hipSuccess;
-  ok = ok && hipHostMalloc((void**)&host_est, sizeof(double) * size_t(m) * kNB, hipHostMallocDefault) == hipSuccess;
-  ok = ok && hipHostMalloc((void**)&host_active, sizeof(int) * kNB, hipHostMallocDefault) == hipSuccess;
-  auto cleanup = [&]() {
-    hipStreamSynchronize(s);
-    for (void* p : owned) hipFree(p);
-    if (host_rel) hipHostFree(host_rel);
-    if (host_est) hipHostFree(host_est);
-    if (host_active) hipHostFree(host_active);
-  };
-  if (!ok) {
-    cleanup();
-    res.status = LSG_ERR_DEVICE;
-    res.error = "batched GMRES workspace allocation failed";
-    return res;
+  // workspace kept on the block operator between solves of the same restart length
+  BlockWork& W = bo.work;
+  if (W.m != m || W.N != N) {
+    W.release();
+    std::string e = W.reserve(N, m);
+    if (!e.empty()) {
+      W.release();
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched GMRES workspace allocation failed: " + e;
+      return res;
+    }
   }
+  double *V = W.V, *w = W.w;
+  double2 *partial = W.partial, *h = W.h, *y = W.y;
+  BGivens st{W.H, W.cs, W.sn, W.g, W.hn, W.bnorm, W.active, W.kq, W.tolq};
+  double *host_rel = W.host_rel, *host_est = W.host_est;
+  int* host_active = W.host_active;
+  auto cleanup = [&]() { hipStreamSynchronize(s); };
   auto dot = [&](int nc, const double* vec, bool norm) {
     const int ncx = nc + (norm ? 1 : 0);
     hipLaunchKernelGGL(bdot_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec, N,
@@ -569,9 +770,14 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
     res.error = "batched GMRES: norm of b failed";
     return res;
   }
-  std::vector<double> bnorm(kNB, 1.0);
+  std::vector<double> bnorm(kNB, 1.0), tolv(kNB, tol);
   for (int q = 0; q < nrhs; ++q) {
     bnorm[q] = std::sqrt(std::max(bn2[q].x, 0.0));
+    if (tol_per) tolv[q] = tol_per[q];
+    if (allow_zero && bnorm[q] == 0.0) {
+      bnorm[q] = 1.0;   // a zero system: x stays as given, counted as converged
+      continue;
+    }
     if (!(bnorm[q] > 0) || !std::isfinite(bnorm[q])) {
       cleanup();
       res.status = LSG_ERR_NUMERIC;
@@ -579,16 +785,30 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
       return res;
     }
   }
+  if (fp32) {
+    std::string e = bo.enable_fp32();
+    if (!e.empty()) {
+      cleanup();
+      res.status = LSG_ERR_DEVICE;
+      res.error = e;
+      return res;
+    }
+  }
+  auto amv = [&](const double* xin, double* yout) {
+    if (fp32) bo.matvec32(xin, yout);
+    else bo.matvec(xin, yout);
+  };
   hipMemcpyAsync(st.bnorm, bnorm.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(st.tolq, tolv.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
   const long long ntot = vs;                                  // doubles per block vector
   const unsigned gv = blocks_for(ntot, kT);                   // one thread per double
   const unsigned gq = blocks_for((long long)N * kNB, kT);     // one thread per (row, system)
   int total_it = 0;
   for (int cyc = 0; cyc <= max_restarts; ++cyc) {
-    bo.matvec(X, w);
+    amv(X, w);
     hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, ntot);
     dot(0, w, true);
-    hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, nrhs, tol, host_rel, host_active);
+    hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, nrhs, host_rel, host_active);
     if (hipStreamSynchronize(s) != hipSuccess) {
       res.status = LSG_ERR_DEVICE;
       res.error = "batched GMRES: device error";
@@ -599,7 +819,7 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
     bool finite = true;
     for (int q = 0; q < nrhs; ++q) {
       n_act += host_active[q];
-      n_conv += host_rel[q] <= tol ? 1 : 0;
+      n_conv += host_rel[q] <= tolv[q] ? 1 : 0;
       worst = std::max(worst, host_rel[q]);
       finite = finite && std::isfinite(host_rel[q]);
       if (rel_residuals) rel_residuals[q] = host_rel[q];
@@ -617,9 +837,9 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
       for (int jj = j; jj < j + batch; ++jj) {
-        bo.matvec(V + size_t(jj) * vs, w);
+        amv(V + size_t(jj) * vs, w);
         dot(jj + 1, w, true);
-        hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, tol, host_est, host_active);
+        hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
         hipLaunchKernelGGL(bupdate_kernel, dim3(gq), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
                            V + size_t(jj + 1) * vs, (long long)N);
       }
@@ -644,6 +864,126 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
   if (res.status == 0 && res.converged < nrhs) {
     res.status = LSG_ERR_NOCONV;
     res.error = "batched GMRES: not every system reached tol";
+  }
+  cleanup();
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
+                           double tol, double inner_tol, int max_outer, double* rel_residuals) {
+  BatchResult res;
+  auto t0 = std::chrono::steady_clock::now();
+  DeviceOperator& o = *bo.op;
+  hipStream_t s = o.stream;
+  const int N = o.N;
+  if (nrhs < 1 || nrhs > kNB || !(tol > 0) || !(inner_tol > 0) || max_outer < 1) {
+    res.status = LSG_ERR_ARG;
+    res.error = "batched refinement: 1 <= nrhs <= 64, tol > 0, inner_tol > 0, max_outer >= 1";
+    return res;
+  }
+  std::string e = bo.enable_fp32();
+  if (!e.empty()) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = e;
+    return res;
+  }
+  const long long vs = (long long)N * kLD;
+  const int nrb = (N + kBRows - 1) / kBRows;
+  double *R = nullptr, *D = nullptr;
+  double2 *partial = nullptr, *h = nullptr;
+  int* mask = nullptr;
+  bool ok = hipMalloc((void**)&R, sizeof(double) * size_t(vs)) == hipSuccess &&
+            hipMalloc((void**)&D, sizeof(double) * size_t(vs)) == hipSuccess &&
+            hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * kNB) == hipSuccess &&
+            hipMalloc((void**)&h, sizeof(double2) * kNB) == hipSuccess &&
+            hipMalloc((void**)&mask, sizeof(int) * kNB) == hipSuccess;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(s);
+    void* ptrs[] = {R, D, partial, h, mask};
+    for (void* p : ptrs)
+      if (p) hipFree(p);
+  };
+  if (!ok) {
+    cleanup();
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched refinement workspace allocation failed";
+    return res;
+  }
+  const unsigned gv = blocks_for(vs, kT);
+  // per-system 2-norms of a block
+  auto norms = [&](const double* v, std::vector<double>& out) {
+    hipLaunchKernelGGL(bdot_kernel, dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N, partial);
+    hipLaunchKernelGGL(breduce_kernel, dim3(1), dim3(kT), 0, s, partial, nrb, 1, h);
+    std::vector<double2> hh(kNB);
+    hipMemcpyAsync(hh.data(), h, sizeof(double2) * kNB, hipMemcpyDeviceToHost, s);
+    const bool good = hipStreamSynchronize(s) == hipSuccess;
+    out.assign(kNB, 0.0);
+    for (int q = 0; q < kNB; ++q) out[q] = std::sqrt(std::max(hh[q].x, 0.0));
+    return good;
+  };
+  std::vector<double> bn, rn;
+  if (!norms(B, bn)) {
+    cleanup();
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched refinement: device error";
+    return res;
+  }
+  for (int q = 0; q < nrhs; ++q)
+    if (!(bn[q] > 0) || !std::isfinite(bn[q])) {
+      cleanup();
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched refinement: a right-hand side is zero or holds NaN/Inf";
+      return res;
+    }
+  std::vector<int> maskh(kNB, 0);
+  std::vector<double> tol_in(kNB, 1.0);
+  for (int outer = 0; res.status == 0; ++outer) {
+    bo.matvec(X, R);   // FP64 residual
+    hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, R, vs);
+    if (!norms(R, rn)) {
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched refinement: device error";
+      break;
+    }
+    int n_conv = 0;
+    double worst = 0.0;
+    bool finite = true;
+    for (int q = 0; q < nrhs; ++q) {
+      const double rel = rn[q] / bn[q];
+      finite = finite && std::isfinite(rel);
+      worst = std::max(worst, rel);
+      if (rel_residuals) rel_residuals[q] = rel;
+      maskh[q] = rel > tol ? 1 : 0;
+      n_conv += 1 - maskh[q];
+      tol_in[q] = maskh[q] ? std::max(inner_tol, 0.5 * tol * bn[q] / rn[q]) : 1.0;
+    }
+    res.converged = n_conv;
+    res.max_rel_residual = worst;
+    if (!finite) {
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched refinement: residual is NaN/Inf";
+      break;
+    }
+    if (n_conv == nrhs || outer == max_outer) break;
+    // inner FP32 solves of A d = r for the unconverged systems (converged columns zeroed)
+    hipMemcpyAsync(mask, maskh.data(), sizeof(int) * kNB, hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(bmask_kernel, dim3(gv), dim3(kT), 0, s, R, mask, vs);
+    hipMemsetAsync(D, 0, sizeof(double) * size_t(vs), s);
+    BatchResult g = block_gmres(bo, R, D, nrhs, m, max_restarts, 0.0, nullptr, true, tol_in.data(), true);
+    res.iterations += g.iterations;
+    res.restarts += g.restarts;
+    if (g.status != 0 && g.status != LSG_ERR_NOCONV) {
+      res.status = g.status;
+      res.error = "inner FP32 batched solve: " + g.error;
+      break;
+    }
+    hipLaunchKernelGGL(badd_kernel, dim3(gv), dim3(kT), 0, s, X, D, vs);
+    res.refinements = outer + 1;
+  }
+  if (res.status == 0 && res.converged < nrhs) {
+    res.status = LSG_ERR_NOCONV;
+    res.error = "batched refinement: not every system reached tol";
   }
   cleanup();
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

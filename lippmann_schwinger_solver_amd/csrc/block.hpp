@@ -10,11 +10,26 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 #include "device.hpp"
 
 namespace lsg {
 
 constexpr int kBlockRhs = 64;  // right-hand sides per block (4 MFMA column tiles)
+
+// Device workspace of the batched GMRES, kept between solves of the same
+// restart length (64-system Krylov basis, Hessenbergs, rotations, partials).
+struct BlockWork {
+  int N = 0, m = -1;
+  double *V = nullptr, *w = nullptr;
+  double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
+  double *cs = nullptr, *hn = nullptr, *bnorm = nullptr, *tolq = nullptr;
+  int *active = nullptr, *kq = nullptr;
+  double *host_rel = nullptr, *host_est = nullptr;  // pinned
+  int* host_active = nullptr;                        // pinned
+  std::string reserve(int N, int m);
+  void release();
+};
 
 struct BlockOperator {
   DeviceOperator* op = nullptr;
@@ -29,10 +44,18 @@ struct BlockOperator {
   // per right-hand-side contrast (N x 64, row-major [n][q]) for a volume-potential
   // operator: Y[:, q] = X[:, q] + qk2_cols[:, q] .* (V X)[:, q]
   double* qk2_cols = nullptr;
+  float* bstate32 = nullptr;  // FP32 pass state for matvec32
+  std::vector<int> pass_max_rows;  // largest destination of each pass (selects the row-tile count)
+  std::vector<int> pass_nseg;      // column segments per destination (1: not split)
+  void* split_part = nullptr;      // partial products of split passes
+  BlockWork work;
   std::string setup(DeviceOperator& o);
+  std::string enable_fp32();  // FP32 operators (DeviceOperator::enable_fp32) and pass state
   void release();
   // Y = A X for one block (row-split, nb columns), on op->stream
   void matvec(const double* X, double* Y);
+  // same with FP32 operators, FP32 MFMA and FP32 pass state (X, Y FP64)
+  void matvec32(const double* X, double* Y);
 };
 
 // Batched restarted GMRES: nb independent systems A x_q = b_q advanced in
@@ -43,14 +66,24 @@ struct BlockOperator {
 // system, which decides convergence (rel_residual <= tol for all nrhs).
 struct BatchResult {
   int iterations = 0, restarts = 0, converged = 0;  // converged: systems meeting tol
+  int refinements = 0;
   double max_rel_residual = 0, seconds = 0;
   int status = 0;
   std::string error;
 };
 // B, X: N x nb row-split device blocks (X holds the initial guesses); only the
 // first nrhs columns are solved, the others must be zero.
+// fp32: the block matvecs use FP32 operators and FP32 MFMA; tol_per (host, nrhs):
+// per-system tolerances instead of tol; allow_zero: zero right-hand sides are
+// accepted (their systems are converged with x unchanged).
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
-                        double* rel_residuals /* host, nrhs */);
+                        double* rel_residuals /* host, nrhs */, bool fp32 = false, const double* tol_per = nullptr,
+                        bool allow_zero = false);
+// FP64 iterative refinement of the batch around FP32 batched GMRES:
+// R = B - A X (FP64 block matvec); systems above tol solve A D = R with the FP32
+// block operator to max(inner_tol, tol ||b_q|| / (2 ||r_q||)); X += D.
+BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
+                           double tol, double inner_tol, int max_outer, double* rel_residuals);
 
 // column-major complex (x + 2*ld*j) <-> row-split block, nrhs <= nb columns
 void to_split(const double2* x, long long ld, int nrhs, int N, int nb, double* out, hipStream_t s);

@@ -490,19 +490,24 @@ int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_
   return LSG_OK;
 }
 
-int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms) {
+static int time_block(lsg_ctx* ctx, int reps, double* ms, bool fp32) {
   if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
   std::string e = ctx->block_ready();
+  if (e.empty() && fp32) e = ctx->blk.enable_fp32();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  auto mv = [&]() {
+    if (fp32) ctx->blk.matvec32(ctx->bx, ctx->by);
+    else ctx->blk.matvec(ctx->bx, ctx->by);
+  };
   hipStream_t s = ctx->dev.stream;
   const size_t nbk = sizeof(double) * size_t(ctx->hb.plan.N) * 2 * kBlockRhs;
   hipMemsetAsync(ctx->bx, 0, nbk, s);
   hipEvent_t a, b;
   hipEventCreate(&a);
   hipEventCreate(&b);
-  ctx->blk.matvec(ctx->bx, ctx->by);
+  mv();
   hipEventRecord(a, s);
-  for (int r = 0; r < reps; ++r) ctx->blk.matvec(ctx->bx, ctx->by);
+  for (int r = 0; r < reps; ++r) mv();
   hipEventRecord(b, s);
   hipError_t he = hipEventSynchronize(b);
   float t = 0;
@@ -513,6 +518,9 @@ int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms) {
   *ms = double(t) / reps;
   return LSG_OK;
 }
+
+int lsg_time_matvec_block(lsg_ctx* ctx, int reps, double* ms) { return time_block(ctx, reps, ms, false); }
+int lsg_time_matvec_block_fp32(lsg_ctx* ctx, int reps, double* ms) { return time_block(ctx, reps, ms, true); }
 
 int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms) {
   if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
@@ -597,10 +605,12 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
                     double* rel_residuals, lsg_stats* stats) {
   if (!ctx) return LSG_ERR_ARG;
   const int64_t N = ctx->hb.plan.N;
-  if (!o || !b || !x || nrhs < 1 || nrhs > kBlockRhs || ld < N || o->mode != LSG_SOLVE_FP64 || !(o->tol > 0) ||
-      o->restart_len < 1 || o->restart_len > 500 || o->max_restarts < 1)
+  if (!o || !b || !x || nrhs < 1 || nrhs > kBlockRhs || ld < N || !(o->tol > 0) || o->restart_len < 1 ||
+      o->restart_len > 500 || o->max_restarts < 1 || (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR) ||
+      (o->mode == LSG_SOLVE_FP32_IR && (!(o->inner_tol > 0) || o->max_outer < 1)))
     return ctx->set(LSG_ERR_ARG,
-                    "bad batched gmres arguments (1 <= nrhs <= 64, ld >= N, mode FP64, tol > 0, restart_len in [1,500])");
+                    "bad batched gmres arguments (1 <= nrhs <= 64, ld >= N, tol > 0, restart_len in [1,500]; "
+                    "FP32_IR: inner_tol > 0, max_outer >= 1)");
   std::string e = ctx->block_ready();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   hipStream_t s = ctx->dev.stream;
@@ -622,7 +632,11 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
   load(b, ctx->bx);
   load(x, ctx->by);
   std::vector<double> rel(kBlockRhs, 0.0);
-  BatchResult r = block_gmres(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol, rel.data());
+  BatchResult r = o->mode == LSG_SOLVE_FP64
+                      ? block_gmres(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
+                                    rel.data())
+                      : block_gmres_ir(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
+                                       o->inner_tol, o->max_outer, rel.data());
   if (o->device_ptrs) {
     from_split(ctx->by, int(N), kBlockRhs, nrhs, reinterpret_cast<double2*>(x), ld, s);
   } else {
@@ -639,7 +653,7 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
     stats->rel_residual = r.max_rel_residual;
     stats->est_residual = 0.0;
     stats->seconds = r.seconds;
-    stats->refinements = 0;
+    stats->refinements = r.refinements;
   }
   if (r.status != 0) return ctx->set(r.status, r.error);
   return LSG_OK;

@@ -82,3 +82,19 @@ def test_batched_gmres_argument_errors():
     with pytest.raises(L.LsgError) as e:
         S.gmres_batch(B, tol=1e-8)
     assert e.value.code == L.LSG_ERR_NUMERIC
+
+
+def test_fp32_block_matvec_and_batched_refinement():
+    """FP32 block operator (FP32 MFMA) inside FP64 refinement of the batch: every
+    system reaches 1e-8 in FP64, checked again through the single-vector matvec."""
+    S = make(H.inputs(depth=5, kappa=40.0, q=6, seed=1))
+    angles = 2 * np.pi * np.arange(16) / 16
+    B = S.rhs_angles(angles)
+    X, st, rel = S.gmres_batch(B, restart_len=50, max_restarts=1, tol=1e-8, mode="fp32ir", inner_tol=1e-5)
+    X64, s64, _ = S.gmres_batch(B, restart_len=50, max_restarts=20, tol=1e-8)
+    print(f"batched FP32-IR: {st}; FP64: {s64['iterations']} steps", flush=True)
+    assert st["status"] == 0 and st["converged"] == 16 and rel.max() <= 1e-8
+    assert st["refinements"] >= 1
+    for j in (0, 7, 15):
+        assert H.relerr(S.MatVecProduct(X[:, j]), B[:, j]) <= 1.5e-8
+        assert H.relerr(X[:, j], X64[:, j]) < 1e-6
