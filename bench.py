@@ -219,6 +219,8 @@ def bench_c3(args):
     S.FMMTreeInitialisation()
     info = S.info()
     log(f"C3 setup {time.time() - t0:.1f} s: N={info['N']} E_far={info['E_far']} bytes/matvec={info['bytes_matvec']}")
+    n_changed, report = S.autotune(reps=3)
+    log(f"autotune ({n_changed} changed):\n{report}")
     b = S.rhs()
     S.gmres_ex(b, mode="fp64", restart_len=50, max_restarts=40, tol=args.tol)       # warm-up
     t0 = time.perf_counter()
@@ -310,6 +312,47 @@ def bench_learned(args):
     print(json.dumps({"workload": "learned preconditioner (C5)", "result": res}), flush=True)
 
 
+def bench_ragged(args):
+    """Ragged batch: 48 independent scatterers, tree depth uniform in {4, 5, 6},
+    kappa uniform in [20, 80], 3-8 seeded bumps and a seeded angle each;
+    autotuned kernel variants, graph-captured matvecs, solved concurrently
+    (lsg_gmres_ragged) and, for comparison, one after the other."""
+    rng = np.random.default_rng(2024)
+    solvers, rhs, cases = [], [], []
+    t0 = time.time()
+    for i in range(48):
+        depth, kappa, nb = int(rng.integers(4, 7)), float(rng.uniform(20, 80)), int(rng.integers(3, 9))
+        inp = lsg.InputsToDFMM(kappa=kappa, nLevelsUniform=depth, L=0.5, nChebNodes=6, degreeOfBases=6, TOL_POW=10,
+                               nCones_LFR=16)
+        inp.contrast = lsg.bumps_from_seed(100 + i, nb)
+        inp.inc_angle = seeded_angle(100 + i)
+        S = lsg.DFMM(inp)
+        S.FMMTreeInitialisation()
+        S.autotune(reps=2)
+        S.use_graph(True)
+        solvers.append(S)
+        rhs.append(S.rhs())
+        cases.append((depth, round(kappa, 2), nb))
+        log(f"scatterer {i}: depth {depth} kappa {kappa:.1f} bumps {nb} N {S.N} ({time.time() - t0:.0f} s)")
+    lsg.gmres_ragged(solvers, rhs, tol=args.tol)          # warm-up
+    t0 = time.perf_counter()
+    xs, stats, rc = lsg.gmres_ragged(solvers, rhs, tol=args.tol)
+    t_con = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"ragged batch did not verify: {[s['rel_residual'] for s in stats]}")
+    t0 = time.perf_counter()
+    for S, b in zip(solvers, rhs):
+        S.gmres(b, restart_len=50, max_restarts=20, tol=args.tol)
+    t_seq = time.perf_counter() - t0
+    worst = max(float(np.linalg.norm(S.MatVecProduct(x) - b) / np.linalg.norm(b)) for S, b, x in zip(solvers, rhs, xs))
+    out = {"workload": "C4b ragged batch: 48 scatterers, depth {4,5,6}, kappa [20,80], 3-8 bumps",
+           "solves_per_s_concurrent": 48 / t_con, "solves_per_s_sequential": 48 / t_seq,
+           "concurrency_speedup": t_seq / t_con, "max_rel_residual": max(s["rel_residual"] for s in stats),
+           "recomputed_max_rel_residual": worst, "total_unknowns": int(sum(S.N for S in solvers)),
+           "iterations": [s["iterations"] for s in stats], "cases": cases}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,7 +363,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
-    ap.add_argument("--workload", choices=["c2", "c3", "learned"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged"], default="c2",
                     help="c2: the default line; c3: depth 7, kappa 160 mixed-precision and batched legs; "
                          "learned: train the network on 4096 generated solves and evaluate on 256 held-out")
     args = ap.parse_args()
@@ -332,6 +375,8 @@ def main():
         return bench_c3(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "ragged":
+        return bench_ragged(args)
 
     inp = c2_inputs(0)
     t0 = time.time()
@@ -339,6 +384,8 @@ def main():
     S.FMMTreeInitialisation()
     info = S.info()
     log(f"setup {time.time() - t0:.1f} s: N={info['N']} E_far={info['E_far']} bytes/matvec={info['bytes_matvec']}")
+    n_changed, report = S.autotune(reps=3)
+    log(f"autotune ({n_changed} changed):\n{report}")
     b = S.rhs()
     if args.pmc_probe:
         pmc_probe(S)

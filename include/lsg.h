@@ -223,6 +223,29 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
 int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* opts,
                     double* rel_residuals, lsg_stats* stats);
 
+/* Autotuner (no reference counterpart): for every pass, time the one-wavefront
+ * and one-workgroup-per-destination kernels, and 1 or 2 leaves per thread in
+ * the near field, on this operator with HIP events (best of `reps`); keep the
+ * faster. n_changed (may be NULL) receives the number of changed choices;
+ * lsg_tuning() returns one line per decision. */
+int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed);
+const char* lsg_tuning(const lsg_ctx* ctx);
+
+/* on = 1: capture the FP64 matvec (both streams) into a hipGraph, replayed by
+ * every later FP64 matvec of the context (one launch plus a pointer update);
+ * on = 0: back to individual launches. */
+int lsg_use_graph(lsg_ctx* ctx, int on);
+
+/* Ragged batch: n independent systems, each with its own context (tree depth,
+ * wavenumber and contrast may all differ), solved concurrently by
+ * lsg_gmres_ex with the same options, one host thread and one pair of streams
+ * per context (matvecs replayed from each context's graph when lsg_use_graph
+ * is on). b[i], x[i] are the vectors of context i; stats (may be NULL) has n
+ * entries. Returns the first error other than LSG_ERR_NOCONV, else
+ * LSG_ERR_NOCONV if any system missed tol, else LSG_OK. */
+int lsg_gmres_ragged(lsg_ctx* const* ctxs, int n, const double* const* b, double* const* x,
+                     const lsg_solve_opts* opts, lsg_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

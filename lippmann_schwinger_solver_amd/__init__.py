@@ -98,6 +98,11 @@ SYMBOLS = [
     ("lsg_time_matvec_block", C.c_int, [_P, C.c_int, _D]),
     ("lsg_time_matvec_block_fp32", C.c_int, [_P, C.c_int, _D]),
     ("lsg_set_contrast", C.c_int, [_P, _D, C.c_int]),
+    ("lsg_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("lsg_tuning", C.c_char_p, [_P]),
+    ("lsg_use_graph", C.c_int, [_P, C.c_int]),
+    ("lsg_gmres_ragged", C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(lsg_solve_opts),
+                                   C.POINTER(lsg_stats)]),
     # include/lsg_learn.h
     ("lsgn_create", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
     ("lsgn_destroy", None, [_P]),
@@ -492,6 +497,16 @@ class DFMM:
         stats["status"] = rc
         return x, stats
 
+    def autotune(self, reps=3):
+        """Pick the faster kernel variant per pass and for the near field; returns (n_changed, report)."""
+        n = C.c_int()
+        self._check(self._lib.lsg_autotune(self._ctx, reps, C.byref(n)))
+        return n.value, self._lib.lsg_tuning(self._ctx).decode()
+
+    def use_graph(self, on=True):
+        """Replay the FP64 matvec from a captured hipGraph (on) or launch kernel by kernel (off)."""
+        self._check(self._lib.lsg_use_graph(self._ctx, 1 if on else 0))
+
     def time_matvec_block(self, reps=10, fp32=False):
         """Average device time (ms) of one 64-right-hand-side block matvec."""
         ms = C.c_double()
@@ -541,3 +556,23 @@ def RestartedGMRES(restart, S, m, inputs, b):
     """`restart` cycles of GMRES(m) from x0 = 0 (GMRES/GMRES.cpp:307-319)."""
     x, _ = S.gmres(b, x0=None, restart_len=m, max_restarts=restart, tol=0.0)
     return x
+
+
+def gmres_ragged(solvers, rhs, restart_len=50, max_restarts=20, tol=1e-8, mode="fp64", inner_tol=1e-5, max_outer=20):
+    """Solve one system per DFMM context concurrently (lsg_gmres_ragged): the
+    contexts may differ in depth, wavenumber and contrast. Returns (xs, stats list)."""
+    n = len(solvers)
+    lib = solvers[0]._lib
+    bs = [np.ascontiguousarray(b, dtype=np.complex128) for b in rhs]
+    xs = [np.zeros(S.N, dtype=np.complex128) for S in solvers]
+    ctxs = (C.c_void_p * n)(*[S._ctx for S in solvers])
+    bp = (C.c_void_p * n)(*[b.ctypes.data for b in bs])
+    xp = (C.c_void_p * n)(*[x.ctypes.data for x in xs])
+    o = lsg_solve_opts(mode={"fp64": LSG_SOLVE_FP64, "fp32ir": LSG_SOLVE_FP32_IR}[mode], restart_len=restart_len,
+                       max_restarts=max_restarts, tol=tol, inner_tol=inner_tol, max_outer=max_outer, device_ptrs=0)
+    st = (lsg_stats * n)()
+    rc = lib.lsg_gmres_ragged(ctxs, n, bp, xp, C.byref(o), st)
+    stats = [{k: getattr(st[i], k) for k, _ in lsg_stats._fields_} for i in range(n)]
+    if rc != 0 and rc != LSG_ERR_NOCONV:
+        raise LsgError(rc, "ragged batch failed: " + "; ".join(S._lib.lsg_last_error(S._ctx).decode() for S in solvers))
+    return xs, stats, rc

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lsg.h"
@@ -115,6 +116,7 @@ struct lsg_ctx {
   GridMaps maps;              // node <-> grid maps of the attached learned preconditioner
   LearnedPrecond prec;
   std::vector<double> qk2_user_host;  // column 0 of lsg_set_contrast
+  std::string tuning;                 // last autotuner report
   std::string err;
   double2* dx = nullptr;
   double2* dy = nullptr;
@@ -657,6 +659,50 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
   }
   if (r.status != 0) return ctx->set(r.status, r.error);
   return LSG_OK;
+}
+
+int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed) {
+  if (!ctx || reps < 1) return LSG_ERR_ARG;
+  const bool had_graph = ctx->dev.graph_exec != nullptr;
+  std::string e = ctx->dev.autotune(reps, n_changed);
+  if (e.empty() && had_graph) e = ctx->dev.capture_graph();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  ctx->tuning.clear();
+  for (const std::string& t : ctx->dev.tuning) ctx->tuning += t + "\n";
+  return LSG_OK;
+}
+
+const char* lsg_tuning(const lsg_ctx* ctx) { return ctx ? ctx->tuning.c_str() : ""; }
+
+int lsg_use_graph(lsg_ctx* ctx, int on) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (!on) {
+    hipStreamSynchronize(ctx->dev.stream);
+    ctx->dev.drop_graph();
+    return LSG_OK;
+  }
+  std::string e = ctx->dev.capture_graph();
+  return e.empty() ? LSG_OK : ctx->set(LSG_ERR_DEVICE, e);
+}
+
+int lsg_gmres_ragged(lsg_ctx* const* ctxs, int n, const double* const* b, double* const* x, const lsg_solve_opts* o,
+                     lsg_stats* stats) {
+  if (!ctxs || n < 1 || !b || !x || !o) return LSG_ERR_ARG;
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i] || !b[i] || !x[i]) return LSG_ERR_ARG;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (ctxs[i] == ctxs[j]) return ctxs[i]->set(LSG_ERR_ARG, "a context appears twice in the ragged batch");
+  std::vector<int> rc(size_t(n), LSG_OK);
+  std::vector<std::thread> th;
+  th.reserve(size_t(n));
+  for (int i = 0; i < n; ++i)
+    th.emplace_back([&, i]() { rc[size_t(i)] = lsg_gmres_ex(ctxs[i], b[i], x[i], o, stats ? &stats[i] : nullptr); });
+  for (auto& t : th) t.join();
+  int worst = LSG_OK;
+  for (int i = 0; i < n; ++i)
+    if (rc[size_t(i)] != LSG_OK && (worst == LSG_OK || rc[size_t(i)] != LSG_ERR_NOCONV)) worst = rc[size_t(i)];
+  return worst;
 }
 
 int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len, int max_restarts, double tol,

@@ -100,9 +100,12 @@ __device__ __forceinline__ void put_dst(C* state, XC* y, int d, int N, int r, C 
 template <bool ACC, class R, int ROLE>
 __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
-    const double2* __restrict__ x, double2* y, int N, const int* __restrict__ rows,
+    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off) {
+    const long long* __restrict__ src_off, const double2* const* __restrict__ xyp) {
+  // pointers from device memory when the matvec runs as a captured graph
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
   typedef typename cx<R>::t C;
   typedef typename cx<R>::v V;
   __shared__ C xs[kChunk];
@@ -160,9 +163,12 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
 template <bool ACC, class R>
 __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
-    const double2* __restrict__ x, double2* y, int N, const int* __restrict__ rows,
+    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off, int ndest) {
+    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp) {
+  // pointers from device memory when the matvec runs as a captured graph
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
   typedef typename cx<R>::t C;
   typedef typename cx<R>::v V;
   __shared__ C red[kThreads];
@@ -223,19 +229,22 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
 }
 
 // Near field of a uniform tree as one GEMM over the 9 colleague slots:
-// y_l = [x_l +] qk2_l * sum_s T_s x_nbr(l,s). A workgroup owns LB = 2*(256/RK)
-// leaves; thread (row i, pair p) accumulates row i of leaves 2p and 2p+1, so
-// each table entry read from LDS feeds two complex multiply-adds. The next
+// y_l = [x_l +] qk2_l * sum_s T_s x_nbr(l,s). A workgroup owns LB = NL*(256/RK)
+// leaves; thread (row i, group p) accumulates row i of leaves NL p .. NL p + NL-1,
+// so each table entry read from LDS feeds NL complex multiply-adds (NL = 1 or
+// 2, chosen per operator by the autotuner). The next
 // slot's table and neighbour vectors are fetched into registers while the
 // current slot is being applied.
-template <class R, int RK>
+template <class R, int RK, int NL>
 __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t* __restrict__ T,
                                                         const int* __restrict__ nbr, const double* __restrict__ qk2,
-                                                        const double2* __restrict__ x, double2* __restrict__ y,
-                                                        int n_leaves, int find_phi) {
+                                                        const double2* __restrict__ x_arg, double2* __restrict__ y_arg,
+                                                        int n_leaves, int find_phi, const double2* const* __restrict__ xyp) {
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* __restrict__ y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
   typedef typename cx<R>::t C;
-  constexpr int LP = kThreads / RK;
-  constexpr int LB = 2 * LP;
+  constexpr int LP = kThreads / RK;   // leaf groups per workgroup
+  constexpr int LB = NL * LP;         // leaves per workgroup (NL per thread)
   constexpr int NT = (RK * RK + kThreads - 1) / kThreads;
   constexpr int NX = (LB * RK + kThreads - 1) / kThreads;
   __shared__ C Ts[RK * RK];
@@ -266,8 +275,9 @@ __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t*
     }                                                                         \
   }
   LSG_NEAR_FETCH(0)
-  C a0, a1;
-  a0.x = a0.y = a1.x = a1.y = 0;
+  C a[NL];
+#pragma unroll
+  for (int u = 0; u < NL; ++u) a[u].x = a[u].y = 0;
   for (int s = 0; s < 9; ++s) {
     __syncthreads();
 #pragma unroll
@@ -283,25 +293,23 @@ __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t*
     __syncthreads();
     if (s + 1 < 9) LSG_NEAR_FETCH(s + 1)
     if (p < LP) {
-      const C* xa = Xs + 2 * p * RK;
-      const C* xb = xa + RK;
+      const C* xa = Xs + NL * p * RK;
 #pragma unroll 6
       for (int c = 0; c < RK; ++c) {
         const C tv = Ts[c * RK + i];
-        cmac(a0, tv, xa[c]);
-        cmac(a1, tv, xb[c]);
+#pragma unroll
+        for (int u = 0; u < NL; ++u) cmac(a[u], tv, xa[u * RK + c]);
       }
     }
   }
 #undef LSG_NEAR_FETCH
   if (p < LP) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int leaf = leaf0 + 2 * p + q;
+    for (int u = 0; u < NL; ++u) {
+      const int leaf = leaf0 + NL * p + u;
       if (leaf < n_leaves) {
         const size_t g = (size_t)leaf * RK + i;
-        const C a = q == 0 ? a0 : a1;
-        double2 s = make_double2(a.x, a.y);
+        double2 s = make_double2(a[u].x, a[u].y);
         if (find_phi) {
           const double qq = qk2[g];
           const double2 xv = x[g];
@@ -789,6 +797,9 @@ void DeviceOperator::release() {
   if (stream) hipStreamSynchronize(stream);
   if (aux) hipStreamSynchronize(aux);
   work.release();
+  drop_graph();
+  if (xy_dev) hipFree(xy_dev);
+  xy_dev = nullptr;
   void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -808,7 +819,8 @@ void DeviceOperator::release() {
 }
 
 template <class R>
-static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2* x, double2* y, hipStream_t s) {
+static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2* x, double2* y, hipStream_t s,
+                        const double2* const* xyp = nullptr) {
   const int64_t b = p.dest_base;
   typedef typename cx<R>::v V;
   typedef typename cx<R>::t C;
@@ -818,50 +830,61 @@ static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2
     const dim3 grid((p.ndest + 3) / 4);
     if (p.kind == 0)
       hipLaunchKernelGGL((pass_wave_kernel<false, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
     else
       hipLaunchKernelGGL((pass_wave_kernel<true, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest);
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
     return;
   }
   const dim3 grid(p.ndest);
   if (p.kind == 0)
     hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
   else if (p.role == 1)
     hipLaunchKernelGGL((pass_block_kernel<true, R, 1>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
   else
     hipLaunchKernelGGL((pass_block_kernel<true, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
 }
 
-template <class R, int RK>
-static void launch_near_rk(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
-  constexpr int LB = 2 * (kThreads / RK);
+template <class R, int RK, int NL>
+static void launch_near_rk(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                           const double2* const* xyp) {
+  constexpr int LB = NL * (kThreads / RK);
   typedef typename cx<R>::t C;
   const C* T = reinterpret_cast<const C*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
-  hipLaunchKernelGGL((near_kernel<R, RK>), dim3((o.n_leaves + LB - 1) / LB), dim3(kThreads), 0, s, T, o.nbr, o.qk2,
-                     x, y, o.n_leaves, o.find_phi ? 1 : 0);
+  hipLaunchKernelGGL((near_kernel<R, RK, NL>), dim3((o.n_leaves + LB - 1) / LB), dim3(kThreads), 0, s, T, o.nbr,
+                     o.qk2, x, y, o.n_leaves, o.find_phi ? 1 : 0, xyp);
 }
 
-template <class R>
-static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s) {
+template <class R, int NL>
+static void launch_near_nl(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                           const double2* const* xyp) {
   switch (o.rank) {
-    case 4: launch_near_rk<R, 4>(o, x, y, s); break;
-    case 9: launch_near_rk<R, 9>(o, x, y, s); break;
-    case 16: launch_near_rk<R, 16>(o, x, y, s); break;
-    case 25: launch_near_rk<R, 25>(o, x, y, s); break;
-    case 36: launch_near_rk<R, 36>(o, x, y, s); break;
-    case 49: launch_near_rk<R, 49>(o, x, y, s); break;
-    case 64: launch_near_rk<R, 64>(o, x, y, s); break;
+    case 4: launch_near_rk<R, 4, NL>(o, x, y, s, xyp); break;
+    case 9: launch_near_rk<R, 9, NL>(o, x, y, s, xyp); break;
+    case 16: launch_near_rk<R, 16, NL>(o, x, y, s, xyp); break;
+    case 25: launch_near_rk<R, 25, NL>(o, x, y, s, xyp); break;
+    case 36: launch_near_rk<R, 36, NL>(o, x, y, s, xyp); break;
+    case 49: launch_near_rk<R, 49, NL>(o, x, y, s, xyp); break;
+    case 64: launch_near_rk<R, 64, NL>(o, x, y, s, xyp); break;
     default: break;  // rejected at upload
   }
 }
 
+template <class R>
+static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                        const double2* const* xyp = nullptr) {
+  if (o.near_leaves_per_thread == 1) launch_near_nl<R, 1>(o, x, y, s, xyp);
+  else launch_near_nl<R, 2>(o, x, y, s, xyp);
+}
+
 // y = x + qk2 .* y: the Lippmann-Schwinger operator from the volume potential
-__global__ void diag_kernel(const double2* __restrict__ x, double2* __restrict__ y, const double* __restrict__ qk2,
-                            int N) {
+__global__ void diag_kernel(const double2* __restrict__ x_arg, double2* __restrict__ y_arg,
+                            const double* __restrict__ qk2, int N, const double2* const* __restrict__ xyp) {
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* __restrict__ y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) {
     const double q = qk2[n];
@@ -871,30 +894,118 @@ __global__ void diag_kernel(const double2* __restrict__ x, double2* __restrict__
 }
 
 template <class R>
-static void matvec_impl(DeviceOperator& o, const double2* x, double2* y) {
-  if (x == y) {
+static void matvec_impl(DeviceOperator& o, const double2* x, double2* y, const double2* const* xyp = nullptr) {
+  if (!xyp && x == y) {
     // in-place call: the passes read x while the output is written
     hipMemcpyAsync(o.state, x, size_t(o.N) * sizeof(double2), hipMemcpyDeviceToDevice, o.stream);
     x = o.state;
   }
   // the first upward pass is the leaf level (the only one the leaf M2L reads)
   size_t u = 0;
-  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream), u = 1;
+  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream, xyp), u = 1;
   hipEventRecord(o.ev_fork, o.stream);
   hipStreamWaitEvent(o.aux, o.ev_fork, 0);
-  launch_near<R>(o, x, y, o.aux);
-  for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux);
+  launch_near<R>(o, x, y, o.aux, xyp);
+  for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
   hipEventRecord(o.ev_join, o.aux);
-  for (; u < o.up.size(); ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream);
-  for (int p : o.down) launch_pass<R>(o, o.passes[p], x, y, o.stream);
+  for (; u < o.up.size(); ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream, xyp);
+  for (int p : o.down) launch_pass<R>(o, o.passes[p], x, y, o.stream, xyp);
   hipStreamWaitEvent(o.stream, o.ev_join, 0);
-  for (int p : o.leaf_l2l) launch_pass<R>(o, o.passes[p], x, y, o.stream);
+  for (int p : o.leaf_l2l) launch_pass<R>(o, o.passes[p], x, y, o.stream, xyp);
   if (o.qk2_user)
     hipLaunchKernelGGL(diag_kernel, dim3((o.N + kThreads - 1) / kThreads), dim3(kThreads), 0, o.stream, x, y,
-                       o.qk2_user, o.N);
+                       o.qk2_user, o.N, xyp);
 }
 
-void DeviceOperator::matvec(const double2* x, double2* y) { matvec_impl<double>(*this, x, y); }
+__global__ void set_xy_kernel(const double2** xy, const double2* x, double2* y) {
+  xy[0] = x;
+  xy[1] = y;
+}
+
+void DeviceOperator::matvec(const double2* x, double2* y) {
+  if (graph_exec && x != y) {
+    // the captured sequence reads its input and output addresses from xy_dev
+    hipLaunchKernelGGL(set_xy_kernel, dim3(1), dim3(1), 0, stream, xy_dev, x, y);
+    hipGraphLaunch(graph_exec, stream);
+    return;
+  }
+  matvec_impl<double>(*this, x, y);
+}
+
+std::string DeviceOperator::capture_graph() {
+  if (graph_exec) return "";
+  if (!xy_dev) HIP_OK(hipMalloc((void**)&xy_dev, 2 * sizeof(double2*)));
+  HIP_OK(hipStreamSynchronize(stream));
+  HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  matvec_impl<double>(*this, nullptr, nullptr, const_cast<const double2* const*>(xy_dev));
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamEndCapture(stream, &g));
+  hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    graph_exec = nullptr;
+    return std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+  }
+  return "";
+}
+
+void DeviceOperator::drop_graph() {
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
+  graph_exec = nullptr;
+}
+
+// Per-pass choice between one wavefront and one workgroup per destination, and
+// 1 or 2 leaves per thread in the near field: each candidate is timed with HIP
+// events on this operator (best of `reps`) and the faster one kept.
+std::string DeviceOperator::autotune(int reps, int* changed) {
+  drop_graph();
+  const double2* x = state;
+  double2* y = state + N;
+  hipEvent_t a, b;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  auto time_it = [&](auto&& launch) {
+    float best = 1e30f;
+    launch();   // warm-up
+    for (int r = 0; r < reps; ++r) {
+      hipEventRecord(a, stream);
+      launch();
+      hipEventRecord(b, stream);
+      hipEventSynchronize(b);
+      float ms = 0;
+      hipEventElapsedTime(&ms, a, b);
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  int nchg = 0;
+  tuning.clear();
+  for (DevPass& p : passes) {
+    const bool before = p.wave;
+    p.wave = false;
+    const float tb = time_it([&]() { launch_pass<double>(*this, p, x, y, stream); });
+    p.wave = true;
+    const float tw = time_it([&]() { launch_pass<double>(*this, p, x, y, stream); });
+    p.wave = tw < tb;
+    nchg += p.wave != before;
+    tuning.push_back(p.name + (p.wave ? ": wave" : ": block") + " (" + std::to_string(tb * 1e3f) + " us block, " +
+                     std::to_string(tw * 1e3f) + " us wave)");
+  }
+  const int nl_before = near_leaves_per_thread;
+  near_leaves_per_thread = 1;
+  const float t1 = time_it([&]() { launch_near<double>(*this, x, y, stream); });
+  near_leaves_per_thread = 2;
+  const float t2 = time_it([&]() { launch_near<double>(*this, x, y, stream); });
+  near_leaves_per_thread = t1 < t2 ? 1 : 2;
+  nchg += near_leaves_per_thread != nl_before;
+  tuning.push_back("near_field: " + std::to_string(near_leaves_per_thread) + " leaves per thread (" +
+                   std::to_string(t1 * 1e3f) + " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2)");
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  HIP_OK(hipStreamSynchronize(stream));
+  if (changed) *changed = nchg;
+  return "";
+}
 void DeviceOperator::matvec32(const double2* x, double2* y) { matvec_impl<float>(*this, x, y); }
 
 __global__ void to_f32_kernel(const double2* __restrict__ a, float2* __restrict__ b, long long n) {

@@ -73,6 +73,10 @@ struct DeviceOperator {
   float2* T32 = nullptr;
   float2* state32 = nullptr;
   GmresWork work;
+  int near_leaves_per_thread = 2;     // near-field variant (autotuned)
+  hipGraphExec_t graph_exec = nullptr;  // captured FP64 matvec (capture_graph)
+  const double2** xy_dev = nullptr;     // its input/output addresses
+  std::vector<std::string> tuning;      // autotuner decisions
   std::vector<DevPass> passes;
   std::vector<int> up, down, leaf_m2l, leaf_l2l;  // pass indices by role
   int64_t near_bytes = 0;
@@ -85,6 +89,12 @@ struct DeviceOperator {
   // enable_fp32 must have succeeded
   void matvec32(const double2* x, double2* y);
   std::string enable_fp32();
+  // capture the FP64 matvec into a hipGraph; matvec() then sets the addresses
+  // and replays it (one launch)
+  std::string capture_graph();
+  void drop_graph();
+  // time the kernel variants of every pass and of the near field, keep the faster
+  std::string autotune(int reps, int* changed);
   // event-timed per-pass averages over reps matvecs (passes run serialised)
   std::string profile(int reps, std::vector<double>& ms, std::vector<int64_t>& bytes, std::vector<std::string>& names);
   // average time of the full two-stream matvec (ms)
