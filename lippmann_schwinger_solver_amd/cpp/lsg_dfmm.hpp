@@ -125,6 +125,44 @@ class DFMM {
     if (rc != LSG_OK && !(rc == LSG_ERR_NOCONV && tol > 0)) check(rc);
     return st;
   }
+  // FP64 iterative refinement around GMRES with the FP32 operator (no reference
+  // counterpart; the FP64 true residual decides convergence)
+  lsg_stats gmres_mixed(const Vec& b, Vec& x, int m, double tol, double inner_tol = 1e-5, int max_outer = 20) {
+    if (int(x.size()) != N) x.assign(N, 0.0);
+    lsg_solve_opts o{};
+    o.mode = LSG_SOLVE_FP32_IR;
+    o.restart_len = m;
+    o.max_restarts = 1;
+    o.tol = tol;
+    o.inner_tol = inner_tol;
+    o.max_outer = max_outer;
+    lsg_stats st{};
+    int rc = lsg_gmres_ex(ctx_, reinterpret_cast<const double*>(b.data()), reinterpret_cast<double*>(x.data()), &o, &st);
+    if (rc != LSG_OK && rc != LSG_ERR_NOCONV) check(rc);
+    return st;
+  }
+  // batched GMRES for up to 64 right-hand sides sharing the operator (block
+  // matvec on FP64 MFMA); B[q] and X[q] are the systems' vectors
+  lsg_stats gmres_batch(const std::vector<Vec>& B, std::vector<Vec>& X, int m, int max_restarts, double tol,
+                        std::vector<double>* rel_residuals = nullptr) {
+    const int nrhs = int(B.size());
+    std::vector<std::complex<double>> bb(size_t(nrhs) * N), xx(size_t(nrhs) * N, 0.0);
+    for (int q = 0; q < nrhs; ++q) std::copy(B[q].begin(), B[q].end(), bb.begin() + size_t(q) * N);
+    lsg_solve_opts o{};
+    o.mode = LSG_SOLVE_FP64;
+    o.restart_len = m;
+    o.max_restarts = max_restarts;
+    o.tol = tol;
+    std::vector<double> rel(static_cast<size_t>(nrhs), 0.0);
+    lsg_stats st{};
+    int rc = lsg_gmres_batch(ctx_, reinterpret_cast<const double*>(bb.data()), reinterpret_cast<double*>(xx.data()),
+                             nrhs, N, &o, rel.data(), &st);
+    if (rc != LSG_OK && rc != LSG_ERR_NOCONV) check(rc);
+    X.assign(size_t(nrhs), Vec());
+    for (int q = 0; q < nrhs; ++q) X[q].assign(xx.begin() + size_t(q) * N, xx.begin() + size_t(q + 1) * N);
+    if (rel_residuals) *rel_residuals = rel;
+    return st;
+  }
   lsg_ctx* handle() const { return ctx_; }
 
  private:
