@@ -290,13 +290,16 @@ def bench_learned(args):
     hist = net.train(epochs=30, batch=64, lr=2e-3, lr_decay=0.92, log=log)
     t_train = time.time() - t0
     ev = {"plain_it": [], "learned_it": [], "guess_only_it": [], "plain_s": [], "learned_s": [], "guess_only_s": [],
-          "initial_residual_learned": [], "max_rel_residual": 0.0}
+          "initial_residual_learned": [], "max_rel_residual": 0.0, "failed": 0}
     for i, k in enumerate(kappas):
+        log(f"evaluating kappa {k}")
         e = LN.evaluate(ctxs[k], k, list(range(900_000 + i * 1000, 900_000 + i * 1000 + n_eval // len(kappas) +
                                                (1 if i < n_eval % len(kappas) else 0))), tol=args.tol)
         for key in ev:
             if key == "max_rel_residual":
                 ev[key] = max(ev[key], e[key])
+            elif key == "failed":
+                ev[key] += e[key]
             else:
                 ev[key] += e[key]
     res = {"train_solves": int(X.shape[0]), "train_data_s": t_data, "train_s": t_train, "loss_first_epoch": hist[0],
@@ -308,7 +311,8 @@ def bench_learned(args):
            "learned_fgmres_solves_per_s": len(ev["learned_s"]) / float(np.sum(ev["learned_s"])),
            "learned_guess_only_solves_per_s": len(ev["guess_only_s"]) / float(np.sum(ev["guess_only_s"])),
            "mean_initial_residual_learned_guess": float(np.mean(ev["initial_residual_learned"])),
-           "max_rel_residual": ev["max_rel_residual"]}
+           "max_rel_residual": ev["max_rel_residual"], "solves_not_at_tol": ev["failed"]}
+    log(f"learned: {res}")
     print(json.dumps({"workload": "learned preconditioner (C5)", "result": res}), flush=True)
 
 

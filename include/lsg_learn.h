@@ -75,8 +75,10 @@ int lsg_grid_resample(lsg_ctx* ctx, const double* in, double* out, int to_grid);
 #define LSG_PREC_GUESS 1          /* x0 = M(b) when x is zero on entry */
 #define LSG_PREC_RIGHT 2          /* flexible GMRES with right preconditioner M */
 /* Flexible GMRES (FP64 operator, FP64 Gram-Schmidt) with the attached
- * network: flags = LSG_PREC_GUESS | LSG_PREC_RIGHT; the true FP64 residual
- * decides convergence as in lsg_gmres_ex. */
+ * network: flags = LSG_PREC_GUESS | LSG_PREC_RIGHT. With LSG_PREC_RIGHT the
+ * network preconditions the first restart cycle; the remaining cycles are
+ * plain GMRES from the current iterate. The true FP64 residual decides
+ * convergence as in lsg_gmres_ex. */
 int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* opts, int flags,
                       lsg_stats* stats);
 

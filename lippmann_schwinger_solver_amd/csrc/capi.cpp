@@ -470,8 +470,21 @@ int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_
   }
   auto t0 = std::chrono::steady_clock::now();
   if (flags & LSG_PREC_GUESS) ctx->prec.apply(db, dxs, s);
-  GmresResult r = device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol, false,
-                               (flags & LSG_PREC_RIGHT) ? &ctx->prec : nullptr);
+  // The network preconditions the first restart cycle only; later cycles are
+  // plain FP64 GMRES from the current iterate, so convergence does not depend
+  // on the network (a nonlinear M can stall flexible GMRES near its own noise).
+  GmresResult r;
+  if (flags & LSG_PREC_RIGHT) {
+    r = device_gmres(ctx->dev, db, dxs, o->restart_len, 1, o->tol, false, &ctx->prec);
+    if ((r.status == 0 || r.status == LSG_ERR_NOCONV) && !r.converged && o->max_restarts > 1) {
+      GmresResult r2 = device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts - 1, o->tol, false, nullptr);
+      r2.iterations += r.iterations;
+      r2.restarts += r.restarts;
+      r = r2;
+    }
+  } else {
+    r = device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol, false, nullptr);
+  }
   r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (!o->device_ptrs) {
     hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);

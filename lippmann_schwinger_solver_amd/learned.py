@@ -208,9 +208,8 @@ def evaluate(S, kappa, seeds, tol=1e-8, restart_len=50):
         t3 = time.perf_counter()
         x0 = S.apply_preconditioner(b)
         r0 = float(np.linalg.norm(b - S.MatVecProduct(x0)) / np.linalg.norm(b))
-        for st in (sp, sl, sg):
-            if st["status"] != 0:
-                raise RuntimeError(f"evaluation solve failed: {st}")
+        out.setdefault("failed", 0)
+        out["failed"] += sum(1 for st in (sp, sl, sg) if st["status"] != 0)
         out["plain_it"].append(sp["iterations"])
         out["learned_it"].append(sl["iterations"])
         out["guess_only_it"].append(sg["iterations"])
@@ -220,4 +219,5 @@ def evaluate(S, kappa, seeds, tol=1e-8, restart_len=50):
         out["initial_residual_learned"].append(r0)
         out["max_rel_residual"] = max(out["max_rel_residual"], sp["rel_residual"], sl["rel_residual"],
                                       sg["rel_residual"])
+    out.setdefault("failed", 0)
     return out

@@ -143,5 +143,5 @@ def test_trained_network_gain():
           f"learned it {np.mean(ev['learned_it']):.1f}, guess-only it {np.mean(ev['guess_only_it']):.1f}, "
           f"initial residual of the learned guess {np.mean(ev['initial_residual_learned']):.3f}", flush=True)
     assert hist[-1] < hist[0]
-    assert ev["max_rel_residual"] <= 1e-8
+    assert ev["max_rel_residual"] <= 1e-8 and ev["failed"] == 0
     assert np.mean(ev["initial_residual_learned"]) < 1.0     # better than the zero guess
