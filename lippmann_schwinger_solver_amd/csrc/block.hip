@@ -369,43 +369,45 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
 }
 
 // v[n, q] = (w[n, q] - sum_i V_i[n, q] h[i, q]) / hn_q, zero for systems with hn_q = 0
-// (frozen before this step) — hn is the step's norm, set by btail.
+// (frozen before this step) — hn is the step's norm, set by btail. Thread
+// (q, row group) owns kUpdRows consecutive rows of system q, so each
+// coefficient h[i, q] is loaded once per kUpdRows rows.
+constexpr int kUpdRows = 8;
 __global__ __launch_bounds__(kT) void bupdate_kernel(const double* __restrict__ V, long long vs, int nc,
                                                      const double2* __restrict__ h, const double* __restrict__ w,
                                                      const double* __restrict__ hn, double* __restrict__ vout,
                                                      long long n_rows) {
-  const long long e = (long long)blockIdx.x * kT + threadIdx.x;
-  const long long n = e / kNB;
-  const int q = int(e % kNB);
-  if (n >= n_rows) return;
-  double ar = 0.0, ai = 0.0;
-  const long long base = n * kLD + q;
-  int i = 0;
-  for (; i + 4 <= nc; i += 4) {
-    double vr[4], vi[4];
-    double2 hh[4];
+  const int q = threadIdx.x & 63;
+  const long long n0 = ((long long)blockIdx.x * (kT / 64) + (threadIdx.x >> 6)) * kUpdRows;
+  if (n0 >= n_rows) return;
+  const int nr = int(min((long long)kUpdRows, n_rows - n0));
+  double ar[kUpdRows], ai[kUpdRows];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      vr[u] = V[(i + u) * vs + base];
-      vi[u] = V[(i + u) * vs + base + kNB];
-      hh[u] = h[(i + u) * kNB + q];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      ar = fma(vr[u], hh[u].x, fma(-vi[u], hh[u].y, ar));
-      ai = fma(vr[u], hh[u].y, fma(vi[u], hh[u].x, ai));
-    }
-  }
-  for (; i < nc; ++i) {
-    const double vr = V[i * vs + base], vi = V[i * vs + base + kNB];
+  for (int r = 0; r < kUpdRows; ++r) ar[r] = ai[r] = 0.0;
+  const long long base = n0 * kLD + q;
+#pragma unroll 2
+  for (int i = 0; i < nc; ++i) {
     const double2 hh = h[i * kNB + q];
-    ar = fma(vr, hh.x, fma(-vi, hh.y, ar));
-    ai = fma(vr, hh.y, fma(vi, hh.x, ai));
+    const double* Vi = V + i * vs + base;
+#pragma unroll
+    for (int r = 0; r < kUpdRows; ++r) {
+      if (r < nr) {
+        const double vr = Vi[r * kLD], vi = Vi[r * kLD + kNB];
+        ar[r] = fma(vr, hh.x, fma(-vi, hh.y, ar[r]));
+        ai[r] = fma(vr, hh.y, fma(vi, hh.x, ai[r]));
+      }
+    }
   }
   const double d = hn[q];
   const double inv = d > 0.0 ? 1.0 / d : 0.0;
-  vout[base] = (w[base] - ar) * inv;
-  vout[base + kNB] = (w[base + kNB] - ai) * inv;
+#pragma unroll
+  for (int r = 0; r < kUpdRows; ++r) {
+    if (r < nr) {
+      const long long o = base + r * kLD;
+      vout[o] = (w[o] - ar[r]) * inv;
+      vout[o + kNB] = (w[o + kNB] - ai[r]) * inv;
+    }
+  }
 }
 
 // y_q = R_q^{-1} g_q over the kq[q] steps of system q (zero beyond)
@@ -803,6 +805,7 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
   const long long ntot = vs;                                  // doubles per block vector
   const unsigned gv = blocks_for(ntot, kT);                   // one thread per double
   const unsigned gq = blocks_for((long long)N * kNB, kT);     // one thread per (row, system)
+  const unsigned gu = blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 64);   // kUpdRows rows per (thread, system)
   int total_it = 0;
   for (int cyc = 0; cyc <= max_restarts; ++cyc) {
     amv(X, w);
@@ -840,7 +843,7 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
         amv(V + size_t(jj) * vs, w);
         dot(jj + 1, w, true);
         hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(gq), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
+        hipLaunchKernelGGL(bupdate_kernel, dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
                            V + size_t(jj + 1) * vs, (long long)N);
       }
       if (hipStreamSynchronize(s) != hipSuccess) {
