@@ -126,20 +126,40 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     acc.y = 0;
     for (int c0 = 0; c0 < Cn; c0 += kChunk) {
       const int Cc = min(kChunk, Cn - c0);
+      const V* Ap = A + (size_t)c0 * Rw + r0 + row;
+      // the first 8 operator columns of this thread are requested before the
+      // source gather, so the operator stream does not wait for it
+      V a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = ph + u * nph;
+        if (active && c < Cc) a[u] = __builtin_nontemporal_load(Ap + (size_t)c * Rw);
+        else a[u] = V{0, 0};
+      }
       __syncthreads();
       for (int i = t; i < Cc; i += kThreads) xs[i] = load_src(x, state, S[c0 + i], N);
       __syncthreads();
       if (active) {
-        const V* Ap = A + (size_t)c0 * Rw + r0 + row;
         int c = ph;
         for (; c + 7 * nph < Cc; c += 8 * nph) {
-          V a[8];
+          // next group in flight while this one is multiplied
+          V nx[8];
+          const int cn = c + 8 * nph;
 #pragma unroll
-          for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(Ap + (size_t)(c + u * nph) * Rw);
+          for (int u = 0; u < 8; ++u) {
+            const int cc = cn + u * nph;
+            if (cc < Cc) nx[u] = __builtin_nontemporal_load(Ap + (size_t)cc * Rw);
+            else nx[u] = V{0, 0};
+          }
 #pragma unroll
           for (int u = 0; u < 8; ++u) cmac(acc, a[u], xs[c + u * nph]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) a[u] = nx[u];
         }
-        for (; c < Cc; c += nph) cmac(acc, __builtin_nontemporal_load(Ap + (size_t)c * Rw), xs[c]);
+        // tail: fewer than 8 columns left, already in a[]
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (c + u * nph < Cc) cmac(acc, a[u], xs[c + u * nph]);
       }
     }
     red[t] = acc;
