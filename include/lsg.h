@@ -92,6 +92,7 @@ typedef struct lsg_info {
   int64_t n_state;        /* complex slots in the device state vector */
   int max_rank;           /* largest NCA rank */
   double setup_seconds;   /* host operator build time */
+  double setup_phase_seconds[4]; /* of which: far-field tables, NCA skeletons, operator fill, near-field tables */
 } lsg_info;
 
 /* GMRES statistics. */

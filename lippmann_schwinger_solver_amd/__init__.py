@@ -53,7 +53,13 @@ class lsg_info(C.Structure):
     _fields_ = [("N", C.c_int64), ("nLevels", C.c_int), ("level_LFR", C.c_int), ("n_leaves", C.c_int),
                 ("E_far", C.c_int64), ("E_near", C.c_int64), ("bytes_matvec", C.c_int64),
                 ("flops_matvec", C.c_int64), ("n_state", C.c_int64), ("max_rank", C.c_int),
-                ("setup_seconds", C.c_double)]
+                ("setup_seconds", C.c_double), ("setup_phase_seconds", C.c_double * 4)]
+
+
+def _info_dict(inf):
+    d = {k: getattr(inf, k) for k, _ in lsg_info._fields_}
+    d["setup_phase_seconds"] = list(d["setup_phase_seconds"])
+    return d
 
 
 class lsg_stats(C.Structure):
@@ -225,7 +231,7 @@ class HostBuild:
     def info(self):
         inf = lsg_info()
         self._lib.lsg_host_info(self._h, C.byref(inf))
-        return {k: getattr(inf, k) for k, _ in lsg_info._fields_}
+        return _info_dict(inf)
 
     def far_table(self):
         n = self._lib.lsg_host_far_table(self._h, None)
@@ -325,7 +331,7 @@ class DFMM:
     def info(self):
         inf = lsg_info()
         self._check(self._lib.lsg_get_info(self._ctx, C.byref(inf)))
-        return {k: getattr(inf, k) for k, _ in lsg_info._fields_}
+        return _info_dict(inf)
 
     def MatVecProduct(self, b, output=None):
         """output = A b (complex128, length N)."""

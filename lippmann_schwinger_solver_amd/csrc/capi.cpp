@@ -105,6 +105,7 @@ void fill_info(const HostBuild& hb, lsg_info* info) {
   info->n_state = hb.plan.n_state;
   info->max_rank = hb.plan.max_rank;
   info->setup_seconds = hb.setup_seconds;
+  for (int i = 0; i < 4; ++i) info->setup_phase_seconds[i] = hb.plan.phase_seconds[i];
 }
 
 }  // namespace

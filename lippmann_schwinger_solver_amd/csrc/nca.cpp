@@ -93,9 +93,38 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
     dst.resize(N2);
     K.block(&rows[i], 1, cols.data(), N2, dst.data());
   };
+  // Column j needs h(rows[r], leaf of cols[j]) for every row. Pivot columns
+  // often share a source leaf (a leaf box's own 36 nodes, a parent's four
+  // children), so these series vectors are kept per source leaf for the
+  // duration of the call (first in, first out, about 16 MB at most). The
+  // entries are the same arithmetic as EntryKernel::block.
+  constexpr int kH = 2 * kSeries + 1;
+  const size_t hbytes = size_t(N1) * kH * sizeof(cplx);
+  const int hcap = int(std::max<size_t>(1, std::min<size_t>(64, (size_t(16) << 20) / std::max<size_t>(hbytes, 1))));
+  std::vector<int> hleaf;
+  std::vector<std::vector<cplx>> hval;
+  int hnext = 0;
   auto getCol = [&](int j, std::vector<cplx>& dst) {
     dst.resize(N1);
-    K.block(rows.data(), N1, &cols[j], 1, dst.data());
+    const int leaf = cols[j] / K.rank;
+    int slot = -1;
+    for (size_t q = 0; q < hleaf.size(); ++q)
+      if (hleaf[q] == leaf) slot = int(q);
+    if (slot < 0) {
+      if (int(hleaf.size()) < hcap) {
+        hleaf.push_back(leaf);
+        hval.emplace_back(size_t(N1) * kH);
+        slot = int(hleaf.size()) - 1;
+      } else {
+        slot = hnext;
+        hnext = (hnext + 1) % hcap;
+        hleaf[size_t(slot)] = leaf;
+      }
+      cplx* h = hval[size_t(slot)].data();
+      for (int r = 0; r < N1; ++r) K.hvec(rows[r], leaf, h + size_t(r) * kH);
+    }
+    const cplx* h = hval[size_t(slot)].data();
+    for (int r = 0; r < N1; ++r) dst[r] = K.entry(h + size_t(r) * kH, rows[r], cols[j]);
   };
   int rank = 0;
   int row_index = 0, col_index = 0;
@@ -596,6 +625,10 @@ int64_t contrib_cols(const Contrib& c) { return int64_t(c.cols->size()); }
 }  // namespace
 
 std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
+  typedef std::chrono::steady_clock clk;
+  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+  double phase[4] = {0, 0, 0, 0};
+  auto tp = clk::now();
   EntryKernel K;
   K.tree = &t;
   K.rank = bs.rank;
@@ -615,8 +648,12 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     if (K.M[t.leaves[i].l].empty()) K.M[t.leaves[i].l] = far_table(bs, prob.kappa, t.boxRadius[t.leaves[i].l]);
   }
 
+  phase[0] = since(tp);
+  tp = clk::now();
   Builder B(t, bs, K, std::pow(10, -1.0 * opt.TOL_POW));
   B.run_nca();
+  phase[1] = since(tp);
+  tp = clk::now();
 
   plan = Plan();
   plan.far_leaf = K.M[t.nLevels];
@@ -789,6 +826,8 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     plan.passes.push_back(std::move(ph));
   }
 
+  phase[2] = since(tp);
+  tp = clk::now();
   // near field: colleague tables of the leaf level (uniform tree)
   const int lvl = t.nLevels;
   const double rA = t.boxRadius[lvl];
@@ -806,6 +845,8 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         plan.E_near += int64_t(bs.rank) * bs.rank;
       }
   }
+  phase[3] = since(tp);
+  for (int i = 0; i < 4; ++i) plan.phase_seconds[i] = phase[i];
   return "";
 }
 

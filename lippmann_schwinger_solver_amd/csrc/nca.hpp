@@ -80,6 +80,8 @@ struct Plan {
   // the five lists (node ids).
   std::vector<int> skeleton;
   std::vector<cplx> far_leaf;  // far-field table of the leaf level (after rounding)
+  // build time of: far-field tables, NCA skeletons, operator fill, near-field tables
+  double phase_seconds[4] = {0, 0, 0, 0};
 };
 
 struct BuildOptions {
