@@ -205,7 +205,8 @@ def setup_record(wall_s, info):
     """Operator build time (host, OpenMP) and its phases as the library reports them."""
     ph = info["setup_phase_seconds"]
     return {"wall_s": wall_s, "host_build_s": info["setup_seconds"], "far_tables_s": ph[0], "nca_skeletons_s": ph[1],
-            "operator_fill_s": ph[2], "near_tables_s": ph[3], "threads": os.cpu_count()}
+            "operator_fill_s": ph[2], "near_tables_s": ph[3],
+            "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))}
 
 
 def seeded_angle(seed):

@@ -77,6 +77,10 @@ typedef struct lsg_inputs {
   int find_phi;        /* 1: A = I + diag(k^2 q) V (solve); 0: V (field) */
   double inc_angle;    /* plane-wave direction; 0 = +x as in the reference */
   lsg_contrast contrast;
+  int setup_device;    /* 0: NCA skeletons and operator entries on the host (skeletons
+                          identical to the reference algorithm's); 1: on the GPU (batched ACA
+                          and entry blocks; pivots agree with the host's except where two
+                          candidates tie to rounding) */
 } lsg_inputs;
 
 /* Work and size figures of a built operator (for roofline accounting). */

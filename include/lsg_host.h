@@ -65,6 +65,17 @@ int64_t lsg_host_plan_qk2(const lsg_host* h, double* out);
 /* Per-level largest NCA rank, levels 0..nLevels. */
 int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels);
 
+/* Checks of the GPU operator build (lsg_inputs.setup_device = 1) against the
+ * host one; device = 0 evaluates on the host, 1 on the GPU.
+ * lsg_host_entries: K(rows, cols) column-major (nr x nc interleaved complex),
+ * the far-field entry of getMatrixEntry (G_FMM2DTree.hpp:603-658).
+ * lsg_host_aca: one ACA_only_nodes (ACA.hpp:253-481) of K(rows, cols) at the
+ * build's tolerance; rank, row and column pivots (positions in rows / cols;
+ * room for min(n1, n2) each). */
+int lsg_host_entries(const lsg_host* h, const int* rows, int nr, const int* cols, int nc, int device, double* out);
+int lsg_host_aca(const lsg_host* h, const int* rows, int n1, const int* cols, int n2, int device, int* rank,
+                 int* row_piv, int* col_piv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,7 +46,8 @@ class lsg_inputs(C.Structure):
                 ("L", C.c_double), ("kappa", C.c_double), ("yes2DFMM", C.c_int),
                 ("degreeOfBases", C.c_int), ("TOL_POW", C.c_int), ("m", C.c_int), ("restart", C.c_int),
                 ("nLevelsUniform", C.c_int), ("Qchoice", C.c_int), ("tree_mode", C.c_int),
-                ("find_phi", C.c_int), ("inc_angle", C.c_double), ("contrast", lsg_contrast)]
+                ("find_phi", C.c_int), ("inc_angle", C.c_double), ("contrast", lsg_contrast),
+                ("setup_device", C.c_int)]
 
 
 class lsg_info(C.Structure):
@@ -137,6 +138,9 @@ SYMBOLS = [
     ("lsg_host_near_tables", C.c_int64, [_P, _D]),
     ("lsg_host_skeleton", C.c_int64, [_P, C.POINTER(C.c_int)]),
     ("lsg_host_level_ranks", C.c_int, [_P, C.POINTER(C.c_int), C.c_int]),
+    ("lsg_host_entries", C.c_int, [_P, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, _P]),
+    ("lsg_host_aca", C.c_int, [_P, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("lsg_host_il_pairs", C.c_int, [_P, C.POINTER(C.c_int64), C.c_int]),
     ("lsg_host_qpinv", C.c_int64, [_P, _D]),
     ("lsg_host_plan_ops", C.c_int64, [_P, _D]),
@@ -185,7 +189,7 @@ class InputsToDFMM:
 
     def __init__(self, nCones_LFR=16, nChebNodes=6, treeAdaptivity=6, L=0.5, kappa=20.0, yes2DFMM=1,
                  degreeOfBases=6, TOL_POW=10, m=50, restart=1, nLevelsUniform=4, Qchoice=0,
-                 tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None):
+                 tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None, setup_device=0):
         self.nCones_LFR = nCones_LFR
         self.nChebNodes = nChebNodes
         self.treeAdaptivity = treeAdaptivity
@@ -202,11 +206,13 @@ class InputsToDFMM:
         self.find_phi = find_phi
         self.inc_angle = inc_angle
         self.contrast = contrast
+        self.setup_device = setup_device   # 1: NCA skeletons and operator entries built on the GPU
 
     def to_c(self):
         s = lsg_inputs()
         for k in ("nCones_LFR", "nChebNodes", "treeAdaptivity", "L", "kappa", "yes2DFMM", "degreeOfBases",
-                  "TOL_POW", "m", "restart", "nLevelsUniform", "Qchoice", "tree_mode", "find_phi", "inc_angle"):
+                  "TOL_POW", "m", "restart", "nLevelsUniform", "Qchoice", "tree_mode", "find_phi", "inc_angle",
+                  "setup_device"):
             setattr(s, k, getattr(self, k))
         if self.contrast is not None:
             s.contrast = self.contrast
@@ -250,6 +256,31 @@ class HostBuild:
         out = np.zeros(n, dtype=np.int32)
         self._lib.lsg_host_skeleton(self._h, out.ctypes.data_as(C.POINTER(C.c_int)))
         return out
+
+    def entries(self, rows, cols, device=False):
+        """K(rows, cols) from the host entry kernel or the GPU one (complex nr x nc)."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.zeros(2 * r.size * c.size)
+        rc = self._lib.lsg_host_entries(self._h, r.ctypes.data_as(C.POINTER(C.c_int)), r.size,
+                                        c.ctypes.data_as(C.POINTER(C.c_int)), c.size, 1 if device else 0, _dptr(out))
+        if rc != 0:
+            raise LsgError(rc, "lsg_host_entries failed")
+        return out.view(np.complex128).reshape(c.size, r.size).T
+
+    def aca(self, rows, cols, device=False):
+        """One ACA of K(rows, cols): (rank, row pivots, column pivots)."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        k = C.c_int()
+        rp = np.zeros(min(r.size, c.size), dtype=np.int32)
+        cp = np.zeros(min(r.size, c.size), dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        rc = self._lib.lsg_host_aca(self._h, r.ctypes.data_as(ip), r.size, c.ctypes.data_as(ip), c.size,
+                                    1 if device else 0, C.byref(k), rp.ctypes.data_as(ip), cp.ctypes.data_as(ip))
+        if rc != 0:
+            raise LsgError(rc, "lsg_host_aca failed")
+        return k.value, rp[:k.value], cp[:k.value]
 
     def level_ranks(self):
         out = np.zeros(16, dtype=np.int32)

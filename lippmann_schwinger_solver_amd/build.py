@@ -16,7 +16,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HOST_SRC = ["tables.cpp", "tree.cpp", "nca.cpp"]
-HIP_SRC = ["device.hip", "block.hip", "learn.hip", "capi.cpp"]
+HIP_SRC = ["device.hip", "block.hip", "learn.hip", "nca_device.hip", "capi.cpp"]
+# device code that restates the host numerics keeps their rounding (no contraction)
+HIP_NO_CONTRACT = {"nca_device.hip"}
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -56,7 +58,7 @@ def build(verbose=False):
         lang = ["-x", "hip"] if s.endswith(".hip") else []
         # device kernels may fuse multiply-add; host code in the C ABI keeps
         # the host numerics' rounding (no contraction)
-        contract = ["-ffp-contract=fast"] if s.endswith(".hip") else ["-ffp-contract=off"]
+        contract = ["-ffp-contract=fast"] if s.endswith(".hip") and s not in HIP_NO_CONTRACT else ["-ffp-contract=off"]
         cmds.append([HIPCC] + HIP_FLAGS + contract + inc + lang + ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     if verbose:

@@ -51,6 +51,7 @@ std::string validate(const lsg_inputs* in) {
   if (in->contrast.kind < 0 || in->contrast.kind > 6) return "contrast.kind must be in [0, 6]";
   if (in->contrast.kind == 6 && (in->contrast.n_bumps < 0 || in->contrast.n_bumps > LSG_MAX_BUMPS))
     return "contrast.n_bumps out of range";
+  if (in->setup_device != 0 && in->setup_device != 1) return "setup_device must be 0 or 1";
   return "";
 }
 
@@ -79,6 +80,7 @@ int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_
   BuildOptions bo;
   bo.TOL_POW = in->TOL_POW;
   bo.find_phi = in->find_phi != 0;
+  if (in->setup_device) bo.device = make_device_nca;
   e = build_plan(hb.tree, hb.bs, hb.prob, bo, hb.plan);
   if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
   hb.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -834,6 +836,64 @@ int64_t lsg_host_plan_dests(const lsg_host* h, int64_t* out) {
         o[4] = ps[p].dst[d]; o[5] = ps[p].op_off[d]; o[6] = ps[p].src_off[d];
       }
   return n;
+}
+
+namespace {
+// Entry kernel of a host build, and its GPU counterpart when asked for.
+int probe_kernel(const lsg_host* h, int device, EntryKernel& K, std::unique_ptr<DeviceNca>& dev) {
+  K = make_entry_kernel(h->hb.tree, h->hb.bs, h->hb.prob, h->hb.in.find_phi != 0);
+  if (device) {
+    std::string e;
+    dev.reset(make_device_nca(K, e));
+    if (!dev) return LSG_ERR_DEVICE;
+  }
+  return LSG_OK;
+}
+}  // namespace
+
+int lsg_host_entries(const lsg_host* h, const int* rows, int nr, const int* cols, int nc, int device, double* out) {
+  if (!h || !rows || !cols || !out || nr < 1 || nc < 1) return LSG_ERR_ARG;
+  for (int i = 0; i < nr; ++i)
+    if (rows[i] < 0 || rows[i] >= h->hb.tree.N) return LSG_ERR_ARG;
+  for (int i = 0; i < nc; ++i)
+    if (cols[i] < 0 || cols[i] >= h->hb.tree.N) return LSG_ERR_ARG;
+  EntryKernel K;
+  std::unique_ptr<DeviceNca> dev;
+  int rc = probe_kernel(h, device, K, dev);
+  if (rc != LSG_OK) return rc;
+  cplx* o = reinterpret_cast<cplx*>(out);
+  if (!dev) {
+    K.block(rows, nr, cols, nc, o);
+    return LSG_OK;
+  }
+  std::vector<BlockJob> jobs{BlockJob{rows, nr, cols, nc, o}};
+  return dev->blocks(jobs).empty() ? LSG_OK : LSG_ERR_DEVICE;
+}
+
+int lsg_host_aca(const lsg_host* h, const int* rows, int n1, const int* cols, int n2, int device, int* rank,
+                 int* row_piv, int* col_piv) {
+  if (!h || !rows || !cols || !rank || !row_piv || !col_piv || n1 < 1 || n2 < 1) return LSG_ERR_ARG;
+  for (int i = 0; i < n1; ++i)
+    if (rows[i] < 0 || rows[i] >= h->hb.tree.N) return LSG_ERR_ARG;
+  for (int i = 0; i < n2; ++i)
+    if (cols[i] < 0 || cols[i] >= h->hb.tree.N) return LSG_ERR_ARG;
+  EntryKernel K;
+  std::unique_ptr<DeviceNca> dev;
+  int rc = probe_kernel(h, device, K, dev);
+  if (rc != LSG_OK) return rc;
+  const std::vector<int> r(rows, rows + n1), c(cols, cols + n2);
+  const double eps = std::pow(10.0, -1.0 * h->hb.in.TOL_POW);
+  AcaResult res;
+  if (dev) {
+    std::vector<AcaJob> jobs{AcaJob{&r, &c, &res}};
+    if (!dev->aca(jobs, eps).empty()) return LSG_ERR_DEVICE;
+  } else {
+    aca_only_nodes(K, r, c, eps, res);
+  }
+  *rank = res.rank;
+  std::copy(res.row_bases.begin(), res.row_bases.end(), row_piv);
+  std::copy(res.col_bases.begin(), res.col_bases.end(), col_piv);
+  return LSG_OK;
 }
 
 int64_t lsg_host_plan_nbr(const lsg_host* h, int* out) {

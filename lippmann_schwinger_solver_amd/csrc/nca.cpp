@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <set>
 
 namespace lsg {
@@ -342,99 +343,109 @@ struct Builder {
     o.in_qrT.factor(At.data(), r, r);
   }
 
-  // getNodes_LFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:2083-2212, 2294-2358)
-  void lfr_out(int j, int k) {
-    Owner& o = L[j][k];
-    std::vector<int> cols, rows;
-    lfr_children(j, k, 0, cols);
-    for (const BoxRef& b : t.lv[j][k].il) lfr_children(b.l, b.k, 1, rows);
-    if (!rows.empty() && !cols.empty()) {
-      o.out_active = true;
-      AcaResult res;
-      aca_only_nodes(K, rows, cols, eps, res);
-      if (res.rank > 0) finish_out(o, rows, cols, res);
-    } else {
-      o.out_active = false;
-      o.out_charge = cols;
-    }
-  }
-  void lfr_in(int j, int k) {
-    Owner& o = L[j][k];
+  // One skeleton construction: an owner (LFR box or HFR cone), the direction
+  // (outgoing or incoming) and its ACA operands.
+  struct Job {
+    Owner* o = nullptr;
+    bool out = true;
     std::vector<int> rows, cols;
-    lfr_children(j, k, 1, rows);
-    for (const BoxRef& b : t.lv[j][k].il) lfr_children(b.l, b.k, 0, cols);
-    if (!rows.empty() && !cols.empty()) {
-      o.in_active = true;
-      AcaResult res;
-      aca_only_nodes(K, rows, cols, eps, res);
-      if (res.rank > 0) finish_in(o, rows, cols, res);
-    } else {
-      o.in_active = false;
-      o.in_check = rows;
-    }
+    AcaResult res;
+  };
+  // getNodes_LFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:2083-2212, 2294-2358):
+  // outgoing: columns = the box's charges (children's outgoing charges or leaf
+  // nodes), rows = the incoming check points of its interaction list; incoming:
+  // rows = the box's check points, columns = the interaction list's charges.
+  void lfr_job(int j, int k, bool out, Job& jb) {
+    jb.o = &L[j][k];
+    jb.out = out;
+    lfr_children(j, k, out ? 0 : 1, out ? jb.cols : jb.rows);
+    for (const BoxRef& b : t.lv[j][k].il) lfr_children(b.l, b.k, out ? 1 : 0, out ? jb.rows : jb.cols);
   }
   // getNodes_HFR_outgoing_box / _incoming_box (G_FMM2DTree.hpp:3322-3392, 3485-3546)
-  void hfr_out(int j, int k, int cone) {
-    Owner& o = hc(j, k, cone);
-    std::vector<int> cols, rows;
-    hfr_children(j, k, cone, 0, cols);
+  void hfr_job(int j, int k, int cone, bool out, Job& jb) {
+    jb.o = &hc(j, k, cone);
+    jb.out = out;
+    hfr_children(j, k, cone, out ? 0 : 1, out ? jb.cols : jb.rows);
     for (const BoxRef& b : t.lv[j][k].cone_il[cone]) {
       int coneB = t.cone_toward(t.lv[b.l][b.k], t.lv[j][k], b.l);
-      hfr_children(b.l, b.k, coneB, 1, rows);
-    }
-    if (!rows.empty() && !cols.empty()) {
-      o.out_active = true;
-      AcaResult res;
-      aca_only_nodes(K, rows, cols, eps, res);
-      if (res.rank > 0) finish_out(o, rows, cols, res);
-    } else {
-      o.out_active = false;
-      o.out_charge = cols;
+      hfr_children(b.l, b.k, coneB, out ? 1 : 0, out ? jb.rows : jb.cols);
     }
   }
-  void hfr_in(int j, int k, int cone) {
-    Owner& o = hc(j, k, cone);
-    std::vector<int> rows, cols;
-    hfr_children(j, k, cone, 1, rows);
-    for (const BoxRef& b : t.lv[j][k].cone_il[cone]) {
-      int coneB = t.cone_toward(t.lv[b.l][b.k], t.lv[j][k], b.l);
-      hfr_children(b.l, b.k, coneB, 0, cols);
-    }
-    if (!rows.empty() && !cols.empty()) {
-      o.in_active = true;
-      AcaResult res;
-      aca_only_nodes(K, rows, cols, eps, res);
-      if (res.rank > 0) finish_in(o, rows, cols, res);
+  static bool job_active(const Job& jb) { return !jb.rows.empty() && !jb.cols.empty(); }
+  void complete(Job& jb) {
+    Owner& o = *jb.o;
+    if (jb.out) {
+      if (job_active(jb)) {
+        o.out_active = true;
+        if (jb.res.rank > 0) finish_out(o, jb.rows, jb.cols, jb.res);
+      } else {
+        o.out_active = false;
+        o.out_charge = jb.cols;
+      }
     } else {
-      o.in_active = false;
-      o.in_check = rows;
+      if (job_active(jb)) {
+        o.in_active = true;
+        if (jb.res.rank > 0) finish_in(o, jb.rows, jb.cols, jb.res);
+      } else {
+        o.in_active = false;
+        o.in_check = jb.rows;
+      }
     }
+  }
+  // Skeletons of one level: operand lists (from the level below), the ACAs
+  // (host threads or one batched device launch sequence), then the QR finish.
+  std::string run_level(std::vector<Job>& jobs) {
+    const int nj = int(jobs.size());
+    if (dev) {
+      std::vector<AcaJob> aj;
+      for (Job& jb : jobs)
+        if (job_active(jb)) aj.push_back(AcaJob{&jb.rows, &jb.cols, &jb.res});
+      std::string e = dev->aca(aj, eps);
+      if (!e.empty()) return e;
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int q = 0; q < nj; ++q)
+        if (job_active(jobs[q])) aca_only_nodes(K, jobs[q].rows, jobs[q].cols, eps, jobs[q].res);
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < nj; ++q) complete(jobs[q]);
+    return "";
   }
 
-  void run_nca() {
+  DeviceNca* dev = nullptr;
+
+  std::string run_nca() {
     // getNodes_LFR (G_FMM2DTree.hpp:2062-2081)
     for (int j = t.nLevels; j >= t.level_LFR; --j) {
       const int nb = int(t.lv[j].size());
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int k = 0; k < nb; ++k) lfr_out(j, k);
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int k = 0; k < nb; ++k) lfr_in(j, k);
+      std::vector<Job> jobs(size_t(2) * nb);
+#pragma omp parallel for schedule(dynamic, 8)
+      for (int q = 0; q < 2 * nb; ++q) lfr_job(j, q >> 1, (q & 1) == 0, jobs[size_t(q)]);
+      std::string e = run_level(jobs);
+      if (!e.empty()) return e;
     }
-    // getNodes_HFR (G_FMM2DTree.hpp:3232-3255)
+    // getNodes_HFR (G_FMM2DTree.hpp:3232-3255); leaf boxes above level_LFR keep LFR skeletons
     for (int j = t.level_LFR - 1; j >= 2; --j) {
       const int nb = int(t.lv[j].size()), nc = t.nCones[j];
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int w = 0; w < nb * nc; ++w) {
-        int k = w / nc, c = w % nc;
-        if (t.lv[j][k].leaf) { if (c == 0) lfr_out(j, k); }
-        else hfr_out(j, k, c);
+      std::vector<Job> jobs(size_t(2) * nb * nc);
+      std::vector<char> used(jobs.size(), 0);
+#pragma omp parallel for schedule(dynamic, 8)
+      for (int q = 0; q < 2 * nb * nc; ++q) {
+        const int w = q >> 1, k = w / nc, c = w % nc;
+        const bool out = (q & 1) == 0;
+        if (t.lv[j][k].leaf) {
+          if (c == 0) { lfr_job(j, k, out, jobs[size_t(q)]); used[size_t(q)] = 1; }
+        } else {
+          hfr_job(j, k, c, out, jobs[size_t(q)]);
+          used[size_t(q)] = 1;
+        }
       }
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int w = 0; w < nb * nc; ++w) {
-        int k = w / nc, c = w % nc;
-        if (t.lv[j][k].leaf) { if (c == 0) lfr_in(j, k); }
-        else hfr_in(j, k, c);
-      }
+      std::vector<Job> live;
+      live.reserve(jobs.size());
+      for (size_t q = 0; q < jobs.size(); ++q)
+        if (used[q]) live.push_back(std::move(jobs[q]));
+      std::string e = run_level(live);
+      if (!e.empty()) return e;
     }
     // getUserCheckPoints (G_FMM2DTree.hpp:3146-3230)
     for (int j = t.nLevels; j >= t.level_LFR; --j)
@@ -458,6 +469,7 @@ struct Builder {
           }
         }
       }
+    return "";
   }
 
   // ------------------------------------------------------------------ plan
@@ -624,16 +636,12 @@ int64_t contrib_cols(const Contrib& c) { return int64_t(c.cols->size()); }
 
 }  // namespace
 
-std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
-  typedef std::chrono::steady_clock clk;
-  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
-  double phase[4] = {0, 0, 0, 0};
-  auto tp = clk::now();
+EntryKernel make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& prob, bool find_phi) {
   EntryKernel K;
   K.tree = &t;
   K.rank = bs.rank;
   K.kappa = prob.kappa;
-  K.find_phi = opt.find_phi;
+  K.find_phi = find_phi;
   K.qk2.resize(t.N);
   for (int i = 0; i < t.N; ++i) K.qk2[i] = prob.kappa * prob.kappa * prob.q(t.gx[i], t.gy[i]);
   K.leaf_level.resize(t.leaves.size());
@@ -648,10 +656,29 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     if (K.M[t.leaves[i].l].empty()) K.M[t.leaves[i].l] = far_table(bs, prob.kappa, t.boxRadius[t.leaves[i].l]);
   }
 
+  return K;
+}
+
+std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
+  typedef std::chrono::steady_clock clk;
+  auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+  double phase[4] = {0, 0, 0, 0};
+  auto tp = clk::now();
+  EntryKernel K = make_entry_kernel(t, bs, prob, opt.find_phi);
   phase[0] = since(tp);
   tp = clk::now();
   Builder B(t, bs, K, std::pow(10, -1.0 * opt.TOL_POW));
-  B.run_nca();
+  std::unique_ptr<DeviceNca> dev;
+  if (opt.device) {
+    std::string e;
+    dev.reset(opt.device(K, e));
+    if (!dev) return "device NCA: " + e;
+    B.dev = dev.get();
+  }
+  {
+    std::string e = B.run_nca();
+    if (!e.empty()) return "NCA: " + e;
+  }
   phase[1] = since(tp);
   tp = clk::now();
 
@@ -784,11 +811,39 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   plan.ops.resize(op_total);
   plan.src.resize(src_total);
 
-  // fill operators
+  // fill operators: M2L blocks K(check points, charges) and L2L blocks
+  // K Atilde_in^-1 (row solves with Atilde_in^T)
+  const bool on_dev = B.dev != nullptr;
   for (auto& pr : passes) {
     PassHost& ph = pr.first;
     std::vector<Dest>& ds = pr.second;
     const int nd = int(ds.size());
+    // on the device every entry block of the pass is generated in one batch:
+    // M2L blocks in place, L2L blocks into zbuf for the solve below
+    std::vector<std::vector<cplx>> zbuf;
+    if (on_dev) {
+      std::vector<BlockJob> jobs;
+      zbuf.resize(size_t(nd));
+      for (int di = 0; di < nd; ++di) {
+        Dest& d = ds[di];
+        if (d.P) continue;
+        size_t zs = 0;
+        for (const Contrib& c : *d.contribs)
+          if (c.Zt && int(c.rows.size()) == d.rows) zs += size_t(d.rows) * c.cols->size();
+        zbuf[size_t(di)].resize(zs);
+        cplx* op = plan.ops.data() + ph.op_off[di];
+        cplx* zp = zbuf[size_t(di)].data();
+        for (const Contrib& c : *d.contribs) {
+          const int nc = int(c.cols->size());
+          if (int(c.rows.size()) != d.rows) continue;
+          jobs.push_back(BlockJob{c.rows.data(), d.rows, c.cols->data(), nc, c.Zt ? zp : op});
+          if (c.Zt) zp += size_t(nc) * d.rows;
+          op += size_t(nc) * d.rows;
+        }
+      }
+      std::string e = B.dev->blocks(jobs);
+      if (!e.empty()) return "device NCA: " + e;
+    }
 #pragma omp parallel for schedule(dynamic, 4)
     for (int di = 0; di < nd; ++di) {
       Dest& d = ds[di];
@@ -800,23 +855,31 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         std::copy(d.up_src.begin(), d.up_src.end(), sp);
         continue;
       }
-      std::vector<cplx> kb, zb;
+      std::vector<cplx> kb;
+      const cplx* zp = on_dev ? zbuf[size_t(di)].data() : nullptr;
       for (const Contrib& c : *d.contribs) {
         const int nc = int(c.cols->size());
         if (int(c.rows.size()) != nr) continue;  // cannot happen: rows are the destination's ucp
-        kb.resize(size_t(nr) * nc);
-        K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());
+        const cplx* kblk;
+        if (on_dev) {
+          kblk = c.Zt ? zp : op;
+          if (c.Zt) zp += size_t(nc) * nr;
+        } else {
+          kb.resize(size_t(nr) * nc);
+          K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());
+          kblk = kb.data();
+        }
         if (c.Zt) {
           // op = K Atilde^-1: rows of op solve Atilde^T op_row^T = K_row^T
-          std::vector<cplx> B(size_t(nc) * nr), X(size_t(nc) * nr);
+          std::vector<cplx> Bm(size_t(nc) * nr), X(size_t(nc) * nr);
           for (int r = 0; r < nr; ++r)
-            for (int p = 0; p < nc; ++p) B[size_t(r) * nc + p] = kb[size_t(p) * nr + r];
-          c.Zt->solve(B.data(), nr, X.data());
+            for (int p = 0; p < nc; ++p) Bm[size_t(r) * nc + p] = kblk[size_t(p) * nr + r];
+          c.Zt->solve(Bm.data(), nr, X.data());
           for (int q = 0; q < nc; ++q)
             for (int r = 0; r < nr; ++r) op[size_t(q) * nr + r] = X[size_t(r) * nc + q];
           for (int q = 0; q < nc; ++q) sp[q] = int(c.src_base + q);
         } else {
-          std::copy(kb.begin(), kb.end(), op);
+          if (!on_dev) std::copy(kb.begin(), kb.end(), op);
           std::copy(c.src_list->begin(), c.src_list->end(), sp);
         }
         op += size_t(nc) * nr;

@@ -84,10 +84,45 @@ struct Plan {
   double phase_seconds[4] = {0, 0, 0, 0};
 };
 
+// One ACA of K(rows, cols) (rows and cols are node lists).
+struct AcaJob {
+  const std::vector<int>* rows = nullptr;
+  const std::vector<int>* cols = nullptr;
+  AcaResult* out = nullptr;
+};
+// One entry block K(rows, cols), written column-major (nr x nc) to out.
+struct BlockJob {
+  const int* rows = nullptr;
+  int nr = 0;
+  const int* cols = nullptr;
+  int nc = 0;
+  cplx* out = nullptr;
+};
+
+// Entry generation and ACA on the GPU: the entry kernel, the pivot order and
+// the stopping rules of EntryKernel and aca_only_nodes, evaluated in device
+// arithmetic (results agree with the host to rounding; pivots may differ
+// where two candidates tie to rounding).
+class DeviceNca {
+ public:
+  virtual ~DeviceNca() {}
+  virtual std::string aca(const std::vector<AcaJob>& jobs, double eps) = 0;
+  virtual std::string blocks(const std::vector<BlockJob>& jobs) = 0;
+};
+typedef DeviceNca* (*DeviceNcaFactory)(const EntryKernel& K, std::string& err);
+
 struct BuildOptions {
   int TOL_POW = 10;
   bool find_phi = true;
+  // when set, NCA skeletons and operator entries are computed on the GPU
+  DeviceNcaFactory device = nullptr;
 };
+
+// Factory of the HIP implementation (csrc/nca_device.hip).
+DeviceNca* make_device_nca(const EntryKernel& K, std::string& err);
+
+// The entry kernel of a built tree (far-field tables of the leaf levels, k^2 q at the nodes).
+EntryKernel make_entry_kernel(const Tree& tree, const BasisSet& bs, const Problem& prob, bool find_phi);
 
 // Build tables, NCA skeletons and the plan. Returns "" on success.
 std::string build_plan(const Tree& tree, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan);

@@ -104,3 +104,37 @@ def test_fused_plan_matches_oracle(depth, kappa, q, find_phi):
     y = P.apply_plan(h.plan(), x, h.info()["n_state"], find_phi=bool(find_phi))
     e = H.relerr(y, o.matvec(x))
     assert e < 1e-10, e
+
+
+@pytest.mark.parametrize("depth,kappa,q", [(3, 10.0, 0), (4, 30.0, 6)])
+def test_entry_hook_matches_oracle_far_entry(depth, kappa, q):
+    """lsg_host_entries (host kernel, the one the GPU entry kernel is checked
+    against) equals the oracle's getMatrixEntry on well-separated blocks."""
+    inp = H.inputs(depth=depth, kappa=kappa, q=q)
+    h = L.HostBuild(inp)
+    o = O.Oracle(inp.to_c())
+    N = h.info()["N"]
+    rows = np.array([0, 5, 17, 35, 36, 71])
+    cols = np.arange(N - 72, N, 7)
+    E = h.entries(rows, cols)
+    ref = np.array([[o.far_entry(int(i), int(j)) for j in cols] for i in rows])
+    assert np.max(np.abs(E - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_aca_hook_host_rank_and_pivots_are_valid():
+    """lsg_host_aca (host path): pivots are distinct positions in range and the
+    pivot block K(row pivots, column pivots) is nonsingular."""
+    inp = H.inputs(depth=4, kappa=30.0, q=6)
+    h = L.HostBuild(inp)
+    N = h.info()["N"]
+    # check points on the leaf of largest contrast (the kernel rows scale with k^2 q)
+    lead = int(np.argmax(h.plan()["qk2"])) // 36
+    rows = np.arange(36 * lead, 36 * lead + 36)
+    far = 0 if lead >= N // 72 else N // 36 - 1
+    cols = np.arange(36 * far, 36 * far + 36)
+    k, rp, cp = h.aca(rows, cols)
+    assert 0 < k <= 36
+    assert len(set(rp.tolist())) == k and len(set(cp.tolist())) == k
+    assert rp.max() < rows.size and cp.max() < cols.size
+    S = h.entries(rows[rp], cols[cp])
+    assert np.linalg.matrix_rank(S) == k
