@@ -41,6 +41,7 @@ struct inputsToDFMM {
   int Qchoice = 0;
   double inc_angle = 0.0;
   lsg_contrast contrast{};
+  int setup_device = 0;  // 1: NCA skeletons and operator entries built on the GPU
 };
 
 class DFMM {
@@ -73,6 +74,7 @@ class DFMM {
     c.inc_angle = inputs.inc_angle;
     c.contrast = inputs.contrast;
     c.contrast.kind = inputs.Qchoice;
+    c.setup_device = inputs.setup_device;
     int rc = lsg_create(&c, &ctx_);
     if (rc != LSG_OK) throw Error(rc, lsg_create_error());
     lsg_info info;
