@@ -310,16 +310,24 @@ struct Builder {
     }
   }
 
-  void qr_of(const std::vector<int>& r, const std::vector<int>& c, ColPivQR& qr) {
-    std::vector<cplx> A(r.size() * c.size());
-    K.block(r.data(), int(r.size()), c.data(), int(c.size()), A.data());
-    qr.factor(A.data(), int(r.size()), int(c.size()));
+  // K(row pivots, column pivots), column-major, gathered from the ACA's raw
+  // pivot rows Ar = K(row pivots, all columns): the same entries K.block of the
+  // skeleton lists computes (same series vector per row and source leaf).
+  static std::vector<cplx> pivot_block(const AcaResult& res, int ncols) {
+    const int r = res.rank;
+    std::vector<cplx> A(size_t(r) * r);
+    for (int j = 0; j < r; ++j)
+      for (int i = 0; i < r; ++i) A[size_t(j) * r + i] = res.Ar[size_t(i) * ncols + res.col_bases[size_t(j)]];
+    return A;
   }
   void finish_out(Owner& o, const std::vector<int>& rows, const std::vector<int>& cols, const AcaResult& res) {
     for (int r : res.row_bases) o.out_check.push_back(rows[r]);
     for (int c : res.col_bases) o.out_charge.push_back(cols[c]);
     ColPivQR qr;
-    qr_of(o.out_check, o.out_charge, qr);
+    {
+      const std::vector<cplx> A = pivot_block(res, int(cols.size()));
+      qr.factor(A.data(), res.rank, res.rank);
+    }
     const int r = res.rank, ns = int(cols.size());
     std::vector<cplx> B(size_t(r) * ns);
     for (int i = 0; i < r; ++i)
@@ -334,8 +342,7 @@ struct Builder {
     // The downward operator K(child, in_charge) Atilde_in^-1 is formed row by
     // row from solves with Atilde_in^T (backward stable), never through an
     // explicit inverse, whose entries grow like cond(Atilde_in).
-    std::vector<cplx> A(o.in_check.size() * o.in_charge.size());
-    K.block(o.in_check.data(), int(o.in_check.size()), o.in_charge.data(), int(o.in_charge.size()), A.data());
+    const std::vector<cplx> A = pivot_block(res, int(cols.size()));
     const int r = res.rank;
     std::vector<cplx> At(size_t(r) * r);
     for (int i = 0; i < r; ++i)
@@ -818,26 +825,18 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     PassHost& ph = pr.first;
     std::vector<Dest>& ds = pr.second;
     const int nd = int(ds.size());
-    // on the device every entry block of the pass is generated in one batch:
-    // M2L blocks in place, L2L blocks into zbuf for the solve below
-    std::vector<std::vector<cplx>> zbuf;
+    // on the device every entry block of the pass is generated in one batch,
+    // in place; the L2L blocks are turned into K Atilde^-1 by the loop below
     if (on_dev) {
       std::vector<BlockJob> jobs;
-      zbuf.resize(size_t(nd));
       for (int di = 0; di < nd; ++di) {
         Dest& d = ds[di];
         if (d.P) continue;
-        size_t zs = 0;
-        for (const Contrib& c : *d.contribs)
-          if (c.Zt && int(c.rows.size()) == d.rows) zs += size_t(d.rows) * c.cols->size();
-        zbuf[size_t(di)].resize(zs);
         cplx* op = plan.ops.data() + ph.op_off[di];
-        cplx* zp = zbuf[size_t(di)].data();
         for (const Contrib& c : *d.contribs) {
           const int nc = int(c.cols->size());
           if (int(c.rows.size()) != d.rows) continue;
-          jobs.push_back(BlockJob{c.rows.data(), d.rows, c.cols->data(), nc, c.Zt ? zp : op});
-          if (c.Zt) zp += size_t(nc) * d.rows;
+          jobs.push_back(BlockJob{c.rows.data(), d.rows, c.cols->data(), nc, op});
           op += size_t(nc) * d.rows;
         }
       }
@@ -856,14 +855,12 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         continue;
       }
       std::vector<cplx> kb;
-      const cplx* zp = on_dev ? zbuf[size_t(di)].data() : nullptr;
       for (const Contrib& c : *d.contribs) {
         const int nc = int(c.cols->size());
         if (int(c.rows.size()) != nr) continue;  // cannot happen: rows are the destination's ucp
         const cplx* kblk;
         if (on_dev) {
-          kblk = c.Zt ? zp : op;
-          if (c.Zt) zp += size_t(nc) * nr;
+          kblk = op;  // read into Bm below before op is overwritten
         } else {
           kb.resize(size_t(nr) * nc);
           K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());

@@ -600,13 +600,21 @@ class HipNca : public DeviceNca {
         hipLaunchKernelGGL(block_kernel, dim3(unsigned(dd.size())), dim3(kBT), 0, 0, dk_, bdesc_.p, idx_.p, out_.p);
         NCA_HIP(hipGetLastError());
       }
-      host_.resize(size_t(off));
-      if (off) NCA_HIP(hipMemcpy(host_.data(), out_.p, size_t(off) * sizeof(double2), hipMemcpyDeviceToHost));
-      const long long nq = (long long)(q1 - q0);
-      for (long long q = 0; q < nq; ++q) {
-        const BlockJob& j = jobs[q0 + size_t(q)];
-        std::memcpy(reinterpret_cast<void*>(j.out), host_.data() + dd[size_t(q)].out,
-                    size_t(j.nr) * j.nc * sizeof(double2));
+      // blocks that follow each other in the caller's memory come back in one copy
+      bool contiguous = true;
+      for (size_t q = q0 + 1; q < q1 && contiguous; ++q)
+        contiguous = jobs[q].out == jobs[q - 1].out + size_t(jobs[q - 1].nr) * jobs[q - 1].nc;
+      if (contiguous) {
+        if (off) NCA_HIP(hipMemcpy(reinterpret_cast<void*>(jobs[q0].out), out_.p, size_t(off) * sizeof(double2),
+                                   hipMemcpyDeviceToHost));
+      } else {
+        host_.resize(size_t(off));
+        if (off) NCA_HIP(hipMemcpy(host_.data(), out_.p, size_t(off) * sizeof(double2), hipMemcpyDeviceToHost));
+        for (size_t q = q0; q < q1; ++q) {
+          const BlockJob& j = jobs[q];
+          std::memcpy(reinterpret_cast<void*>(j.out), host_.data() + dd[q - q0].out,
+                      size_t(j.nr) * j.nc * sizeof(double2));
+        }
       }
       q0 = q1;
     }
