@@ -232,10 +232,7 @@ def bench_c3(args):
     """Config C3: depth 7 (16,384 leaves, 589,824 unknowns), kappa 160, 8 seeded
     bumps. Plain FP64 GMRES, FP32-operator GMRES inside FP64 refinement, and 64
     plane waves through batched GMRES; every result is FP64-verified."""
-    inp = lsg.InputsToDFMM(kappa=160.0, nLevelsUniform=7, L=0.5, nChebNodes=6, degreeOfBases=6, TOL_POW=10,
-                           nCones_LFR=16)
-    inp.contrast = lsg.bumps_from_seed(0, 8)
-    inp.inc_angle = seeded_angle(0)
+    inp = c3_inputs(0)
     t0 = time.time()
     S = lsg.DFMM(inp)
     S.FMMTreeInitialisation()
@@ -284,6 +281,38 @@ def bench_c3(args):
            "batched_64_angles": batched}
     print(json.dumps(out), flush=True)
     S.close()
+
+
+def c3_inputs(seed=0):
+    inp = lsg.InputsToDFMM(kappa=160.0, nLevelsUniform=7, L=0.5, nChebNodes=6, degreeOfBases=6, TOL_POW=10,
+                           nCones_LFR=16)
+    inp.contrast = lsg.bumps_from_seed(seed, 8)
+    inp.inc_angle = seeded_angle(seed)
+    return inp
+
+
+def bench_setup(args):
+    """Operator build (tree, tables, NCA skeletons, fused operators) of the C2
+    and C3 geometries on the host (OpenMP) and with skeletons and entries on the
+    GPU; both builds report their phases."""
+    out = {"workload": "operator build: C2 (N = 147456, kappa 80) and C3 (N = 589824, kappa 160), host vs GPU",
+           "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    for name, mk in (("C2", c2_inputs), ("C3", c3_inputs)):
+        for dev in (1, 0):
+            inp = mk(0)
+            inp.setup_device = dev
+            t0 = time.time()
+            h = lsg.HostBuild(inp)
+            wall = time.time() - t0
+            i = h.info()
+            ph = i["setup_phase_seconds"]
+            rec = {"wall_s": wall, "far_tables_s": ph[0], "nca_skeletons_s": ph[1], "operator_fill_s": ph[2],
+                   "near_tables_s": ph[3], "E_far": int(i["E_far"]), "max_rank": int(i["max_rank"])}
+            log(f"{name} {'GPU' if dev else 'host'} build: {rec}")
+            out[f"{name}_{'gpu' if dev else 'host'}"] = rec
+            del h
+        out[f"{name}_speedup"] = out[f"{name}_host"]["wall_s"] / out[f"{name}_gpu"]["wall_s"]
+    print(json.dumps(out), flush=True)
 
 
 def bench_learned(args):
@@ -391,9 +420,10 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
-    ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged", "setup"], default="c2",
                     help="c2: the default line; c3: depth 7, kappa 160 mixed-precision and batched legs; "
-                         "learned: train the network on 4096 generated solves and evaluate on 256 held-out")
+                         "learned: train the network on 4096 generated solves and evaluate on 256 held-out; "
+                         "ragged: 48 independent scatterers; setup: operator build on the host vs the GPU")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver is single-device (multi-GPU sharding is a design "
@@ -401,6 +431,8 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload == "setup":
+        return bench_setup(args)
     if args.workload == "learned":
         return bench_learned(args)
     if args.workload == "ragged":
