@@ -153,6 +153,17 @@ int lsg_incident(const lsg_ctx* ctx, double* u);
 /* b_i = -k^2 q(x_i) u_inc(x_i) (RHSFunction). */
 int lsg_rhs(const lsg_ctx* ctx, double* b);
 
+/* Re-interpolation of a node field f (N complex, leaf-major as every vector of
+ * this API) onto npts points (x[i], y[i]): per leaf the least-squares
+ * coefficients c = Q_pinv f_leaf of the total-degree Chebyshev basis
+ * T_a(x) T_b(y), a + b < degreeOfBases, evaluated at each point in the leaf box
+ * [centre - r, centre + r) holding it. Restates the Phi -> Psi step of
+ * GMRES_Hybrid/LippmannSchwinger/main/hybridTrueValidation_gmres.cpp:186-209
+ * (the field of one tree at the nodes of another). Host arrays; out: npts
+ * complex. A point outside [-L, L)^2 gives LSG_ERR_ARG (the reference falls
+ * back to the last leaf there). Uniform trees. */
+int lsg_interpolate(lsg_ctx* ctx, const double* f, const double* x, const double* y, int64_t npts, double* out);
+
 /* y = A x for nrhs right-hand sides (column j at x + 2*ld*j doubles),
  * FP64 on the device. Replaces DFMM::MatVecProduct. nrhs >= 2 runs the block
  * operator (64 right-hand sides per FP64 MFMA block matvec). */

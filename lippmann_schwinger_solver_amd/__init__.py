@@ -94,6 +94,7 @@ SYMBOLS = [
     ("lsg_contrast_at_nodes", C.c_int, [_P, _D]),
     ("lsg_incident", C.c_int, [_P, _D]),
     ("lsg_rhs", C.c_int, [_P, _D]),
+    ("lsg_interpolate", C.c_int, [_P, _D, _D, _D, C.c_int64, _D]),
     ("lsg_matvec", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
     ("lsg_matvec_device", C.c_int, [_P, _P, _P]),
     ("lsg_sync", C.c_int, [_P]),
@@ -465,6 +466,21 @@ class DFMM:
             self._check(rc)
         stats["status"] = rc
         return X, stats, rel
+
+    def interpolate(self, f, x, y):
+        """Field f at the nodes (length N) re-interpolated at the points (x[i], y[i])
+        through each leaf's total-degree Chebyshev expansion (hybridTrueValidation_gmres.cpp:186-209)."""
+        f = np.ascontiguousarray(f, dtype=np.complex128)
+        if f.shape != (self.N,):
+            raise ValueError("f must have shape (N,)")
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if x.shape != y.shape or x.ndim != 1:
+            raise ValueError("x and y must be 1-D of equal length")
+        out = np.empty(x.size, dtype=np.complex128)
+        self._check(self._lib.lsg_interpolate(self._ctx, _dptr(f.view(np.float64)), _dptr(x), _dptr(y), x.size,
+                                              _dptr(out.view(np.float64))))
+        return out
 
     def rhs_angles(self, angles):
         """Right-hand sides -k^2 q u_inc for plane waves at the given angles (N x len(angles))."""

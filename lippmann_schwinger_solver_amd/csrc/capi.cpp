@@ -14,6 +14,7 @@
 #include "../../include/lsg_learn.h"
 #include "block.hpp"
 #include "device.hpp"
+#include "field.hpp"
 #include "learn.hpp"
 #include "nca.hpp"
 #include "problem.hpp"
@@ -117,6 +118,7 @@ struct lsg_ctx {
   DeviceOperator dev;
   BlockOperator blk;          // multi right-hand-side operator, set up on first use
   GridMaps maps;              // node <-> grid maps of the attached learned preconditioner
+  FieldInterp field;          // re-interpolation tables, set up on first use
   LearnedPrecond prec;
   std::vector<double> qk2_user_host;  // column 0 of lsg_set_contrast
   std::string tuning;                 // last autotuner report
@@ -204,6 +206,7 @@ int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
 void lsg_destroy(lsg_ctx* ctx) {
   if (!ctx) return;
   ctx->maps.release();
+  ctx->field.release();
   ctx->blk.release();
   ctx->dev.release();
   if (ctx->dx) hipFree(ctx->dx);
@@ -258,6 +261,41 @@ int lsg_incident(const lsg_ctx* ctx, double* u) {
     u[2 * i] = v.real();
     u[2 * i + 1] = v.imag();
   }
+  return LSG_OK;
+}
+
+int lsg_interpolate(lsg_ctx* ctx, const double* f, const double* x, const double* y, int64_t npts, double* out) {
+  if (!ctx || !f || (npts > 0 && (!x || !y || !out)) || npts < 0) return LSG_ERR_ARG;
+  if (npts == 0) return LSG_OK;
+  std::string e = ctx->field.setup(ctx->hb.tree, ctx->hb.bs);
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  const size_t N = size_t(ctx->hb.plan.N), n = size_t(npts);
+  double2* df = nullptr;
+  double *dx = nullptr, *dy = nullptr;
+  double2* dout = nullptr;
+  bool outside = false;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)df, (void*)dx, (void*)dy, (void*)dout})
+      if (p) (void)hipFree(p);
+  };
+  if (hipMalloc((void**)&df, sizeof(double2) * N) != hipSuccess || hipMalloc((void**)&dx, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&dy, sizeof(double) * n) != hipSuccess || hipMalloc((void**)&dout, sizeof(double2) * n) != hipSuccess) {
+    cleanup();
+    return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed in lsg_interpolate");
+  }
+  hipStream_t s = ctx->dev.stream;
+  if (hipMemcpyAsync(df, f, sizeof(double2) * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+    cleanup();
+    return ctx->set(LSG_ERR_DEVICE, "copy to the device failed in lsg_interpolate");
+  }
+  e = ctx->field.run(df, dx, dy, (long long)npts, dout, s, &outside);
+  if (e.empty() && hipMemcpy(out, dout, sizeof(double2) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    e = "copy to the host failed in lsg_interpolate";
+  cleanup();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  if (outside) return ctx->set(LSG_ERR_ARG, "a point lies outside [-L, L)^2");
   return LSG_OK;
 }
 
