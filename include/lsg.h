@@ -81,6 +81,10 @@ typedef struct lsg_inputs {
                           identical to the reference algorithm's); 1: on the GPU (batched ACA
                           and entry blocks; pivots agree with the host's except where two
                           candidates tie to rounding) */
+  const char* precomputations_dir; /* NULL: compute the far-field and near-field tables;
+                          else read them from <dir>/precomputations/M and /Neighbor in the
+                          reference's cache layout and text format (G_FMM2DTree.hpp:1301-1313,
+                          1911-1950); a missing or malformed file fails lsg_create */
 } lsg_inputs;
 
 /* Work and size figures of a built operator (for roofline accounting). */
@@ -163,6 +167,13 @@ int lsg_rhs(const lsg_ctx* ctx, double* b);
  * complex. A point outside [-L, L)^2 gives LSG_ERR_ARG (the reference falls
  * back to the last leaf there). Uniform trees. */
 int lsg_interpolate(lsg_ctx* ctx, const double* f, const double* x, const double* y, int64_t npts, double* out);
+
+/* Writes the table cache of these inputs under <dir>/precomputations in the
+ * reference's layout and text format: far-field tables M of levels
+ * 0..nLevelsUniform (writeMToFile, G_FMM2DTree.hpp:1252-1269) and the 53
+ * neighbour tables of levels 2..10 (writeNeighborInteractionsForLevel,
+ * :1813-1909). Host only; errors in lsg_create_error(). */
+int lsg_write_precomputations(const lsg_inputs* in, const char* dir);
 
 /* y = A x for nrhs right-hand sides (column j at x + 2*ld*j doubles),
  * FP64 on the device. Replaces DFMM::MatVecProduct. nrhs >= 2 runs the block

@@ -47,7 +47,7 @@ class lsg_inputs(C.Structure):
                 ("degreeOfBases", C.c_int), ("TOL_POW", C.c_int), ("m", C.c_int), ("restart", C.c_int),
                 ("nLevelsUniform", C.c_int), ("Qchoice", C.c_int), ("tree_mode", C.c_int),
                 ("find_phi", C.c_int), ("inc_angle", C.c_double), ("contrast", lsg_contrast),
-                ("setup_device", C.c_int)]
+                ("setup_device", C.c_int), ("precomputations_dir", C.c_char_p)]
 
 
 class lsg_info(C.Structure):
@@ -95,6 +95,7 @@ SYMBOLS = [
     ("lsg_incident", C.c_int, [_P, _D]),
     ("lsg_rhs", C.c_int, [_P, _D]),
     ("lsg_interpolate", C.c_int, [_P, _D, _D, _D, C.c_int64, _D]),
+    ("lsg_write_precomputations", C.c_int, [_P, C.c_char_p]),
     ("lsg_matvec", C.c_int, [_P, _D, _D, C.c_int, C.c_int64]),
     ("lsg_matvec_device", C.c_int, [_P, _P, _P]),
     ("lsg_sync", C.c_int, [_P]),
@@ -190,7 +191,8 @@ class InputsToDFMM:
 
     def __init__(self, nCones_LFR=16, nChebNodes=6, treeAdaptivity=6, L=0.5, kappa=20.0, yes2DFMM=1,
                  degreeOfBases=6, TOL_POW=10, m=50, restart=1, nLevelsUniform=4, Qchoice=0,
-                 tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None, setup_device=0):
+                 tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None, setup_device=0,
+                 precomputations_dir=None):
         self.nCones_LFR = nCones_LFR
         self.nChebNodes = nChebNodes
         self.treeAdaptivity = treeAdaptivity
@@ -208,6 +210,8 @@ class InputsToDFMM:
         self.inc_angle = inc_angle
         self.contrast = contrast
         self.setup_device = setup_device   # 1: NCA skeletons and operator entries built on the GPU
+        # read the tables from <dir>/precomputations (the reference's cache) instead of computing them
+        self.precomputations_dir = precomputations_dir
 
     def to_c(self):
         s = lsg_inputs()
@@ -215,12 +219,25 @@ class InputsToDFMM:
                   "TOL_POW", "m", "restart", "nLevelsUniform", "Qchoice", "tree_mode", "find_phi", "inc_angle",
                   "setup_device"):
             setattr(s, k, getattr(self, k))
+        if self.precomputations_dir:
+            s.precomputations_dir = str(self.precomputations_dir).encode()
         if self.contrast is not None:
             s.contrast = self.contrast
             s.Qchoice = self.contrast.kind
         else:
             s.contrast.kind = self.Qchoice
         return s
+
+
+def write_precomputations(inputs, directory):
+    """Write the table cache of `inputs` under <directory>/precomputations in the
+    reference's layout (M tables of levels 0..nLevelsUniform, neighbour tables of
+    levels 2..10)."""
+    lib = load_library()
+    c = inputs.to_c()
+    rc = lib.lsg_write_precomputations(C.byref(c), str(directory).encode())
+    if rc != LSG_OK:
+        raise LsgError(rc, lib.lsg_create_error().decode())
 
 
 class HostBuild:

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "liblsg.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HOST_SRC = ["tables.cpp", "tree.cpp", "nca.cpp"]
+HOST_SRC = ["tables.cpp", "tree.cpp", "nca.cpp", "precomp.cpp"]
 HIP_SRC = ["device.hip", "block.hip", "learn.hip", "nca_device.hip", "field.hip", "capi.cpp"]
 # device code that restates the host numerics keeps their rounding (no contraction)
 HIP_NO_CONTRACT = {"nca_device.hip"}

@@ -17,6 +17,7 @@
 #include "field.hpp"
 #include "learn.hpp"
 #include "nca.hpp"
+#include "precomp.hpp"
 #include "problem.hpp"
 #include "tables.hpp"
 #include "tree.hpp"
@@ -35,6 +36,7 @@ struct HostBuild {
   Problem prob;
   Plan plan;
   double setup_seconds = 0;
+  std::string table_dir;  // copy of lsg_inputs.precomputations_dir ("" = compute the tables)
   std::vector<cplx> far;  // leaf-level M table
 };
 
@@ -63,6 +65,8 @@ int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_
   std::string e;
   auto t0 = std::chrono::steady_clock::now();
   hb.in = *in;
+  hb.table_dir = in->precomputations_dir ? in->precomputations_dir : "";
+  hb.in.precomputations_dir = nullptr;  // the caller's string need not outlive the call
   hb.bs.init(in->nChebNodes, in->degreeOfBases);
   TreeParams tp;
   tp.nCones_LFR = in->nCones_LFR;
@@ -82,6 +86,7 @@ int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_
   bo.TOL_POW = in->TOL_POW;
   bo.find_phi = in->find_phi != 0;
   if (in->setup_device) bo.device = make_device_nca;
+  bo.table_dir = hb.table_dir;
   e = build_plan(hb.tree, hb.bs, hb.prob, bo, hb.plan);
   if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
   hb.setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -879,7 +884,8 @@ int64_t lsg_host_plan_dests(const lsg_host* h, int64_t* out) {
 namespace {
 // Entry kernel of a host build, and its GPU counterpart when asked for.
 int probe_kernel(const lsg_host* h, int device, EntryKernel& K, std::unique_ptr<DeviceNca>& dev) {
-  K = make_entry_kernel(h->hb.tree, h->hb.bs, h->hb.prob, h->hb.in.find_phi != 0);
+  if (!make_entry_kernel(h->hb.tree, h->hb.bs, h->hb.prob, h->hb.in.find_phi != 0, h->hb.table_dir, K).empty())
+    return LSG_ERR_SETUP;
   if (device) {
     std::string e;
     dev.reset(make_device_nca(K, e));
@@ -888,6 +894,23 @@ int probe_kernel(const lsg_host* h, int device, EntryKernel& K, std::unique_ptr<
   return LSG_OK;
 }
 }  // namespace
+
+int lsg_write_precomputations(const lsg_inputs* in, const char* dir) {
+  if (!in || !dir) return LSG_ERR_ARG;
+  std::string e = validate(in);
+  if (!e.empty()) {
+    g_create_error = e;
+    return LSG_ERR_ARG;
+  }
+  BasisSet bs;
+  bs.init(in->nChebNodes, in->degreeOfBases);
+  e = write_precomputations(dir, bs, in->kappa, in->L, in->nLevelsUniform, 10);
+  if (!e.empty()) {
+    g_create_error = e;
+    return LSG_ERR_SETUP;
+  }
+  return LSG_OK;
+}
 
 int lsg_host_entries(const lsg_host* h, const int* rows, int nr, const int* cols, int nc, int device, double* out) {
   if (!h || !rows || !cols || !out || nr < 1 || nc < 1) return LSG_ERR_ARG;

@@ -1,5 +1,7 @@
 #include "nca.hpp"
 
+#include "precomp.hpp"
+
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -643,8 +645,9 @@ int64_t contrib_cols(const Contrib& c) { return int64_t(c.cols->size()); }
 
 }  // namespace
 
-EntryKernel make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& prob, bool find_phi) {
-  EntryKernel K;
+std::string make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& prob, bool find_phi,
+                              const std::string& table_dir, EntryKernel& K) {
+  K = EntryKernel();
   K.tree = &t;
   K.rank = bs.rank;
   K.kappa = prob.kappa;
@@ -660,10 +663,16 @@ EntryKernel make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& 
     K.leaf_level[i] = t.leaves[i].l;
     K.leaf_cx[i] = b.cx;
     K.leaf_cy[i] = b.cy;
-    if (K.M[t.leaves[i].l].empty()) K.M[t.leaves[i].l] = far_table(bs, prob.kappa, t.boxRadius[t.leaves[i].l]);
+    const int l = t.leaves[i].l;
+    if (!K.M[l].empty()) continue;
+    if (table_dir.empty()) {
+      K.M[l] = far_table(bs, prob.kappa, t.boxRadius[l]);
+    } else {
+      std::string e = read_far_table(table_dir, bs, prob.kappa, l, K.M[l]);
+      if (!e.empty()) return e;
+    }
   }
-
-  return K;
+  return "";
 }
 
 std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
@@ -671,7 +680,11 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
   double phase[4] = {0, 0, 0, 0};
   auto tp = clk::now();
-  EntryKernel K = make_entry_kernel(t, bs, prob, opt.find_phi);
+  EntryKernel K;
+  {
+    std::string e = make_entry_kernel(t, bs, prob, opt.find_phi, opt.table_dir, K);
+    if (!e.empty()) return "tables: " + e;
+  }
   phase[0] = since(tp);
   tp = clk::now();
   Builder B(t, bs, K, std::pow(10, -1.0 * opt.TOL_POW));
@@ -891,10 +904,16 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   // near field: colleague tables of the leaf level (uniform tree)
   const int lvl = t.nLevels;
   const double rA = t.boxRadius[lvl];
-  plan.T.resize(size_t(9) * bs.rank * bs.rank);
-  for (int s = 0; s < 9; ++s) {
-    std::vector<cplx> Ts = near_table(bs, prob.kappa, rA, rA, kColleagueCenters[s][0] * rA, kColleagueCenters[s][1] * rA);
-    std::copy(Ts.begin(), Ts.end(), plan.T.begin() + size_t(s) * bs.rank * bs.rank);
+  if (opt.table_dir.empty()) {
+    plan.T.resize(size_t(9) * bs.rank * bs.rank);
+    for (int s = 0; s < 9; ++s) {
+      std::vector<cplx> Ts =
+          near_table(bs, prob.kappa, rA, rA, kColleagueCenters[s][0] * rA, kColleagueCenters[s][1] * rA);
+      std::copy(Ts.begin(), Ts.end(), plan.T.begin() + size_t(s) * bs.rank * bs.rank);
+    }
+  } else {
+    std::string e = read_colleague_tables(opt.table_dir, bs, prob.kappa, lvl, plan.T);
+    if (!e.empty()) return "tables: " + e;
   }
   plan.nbr.assign(size_t(plan.n_leaves) * 9, -1);
   for (int li = 0; li < plan.n_leaves; ++li) {

@@ -116,13 +116,19 @@ struct BuildOptions {
   bool find_phi = true;
   // when set, NCA skeletons and operator entries are computed on the GPU
   DeviceNcaFactory device = nullptr;
+  // when not empty, the far-field and colleague tables are read from the
+  // reference's cache files under <table_dir>/precomputations
+  std::string table_dir;
 };
 
 // Factory of the HIP implementation (csrc/nca_device.hip).
 DeviceNca* make_device_nca(const EntryKernel& K, std::string& err);
 
-// The entry kernel of a built tree (far-field tables of the leaf levels, k^2 q at the nodes).
-EntryKernel make_entry_kernel(const Tree& tree, const BasisSet& bs, const Problem& prob, bool find_phi);
+// The entry kernel of a built tree (far-field tables of the leaf levels, k^2 q
+// at the nodes). With table_dir set the tables are read from the reference's
+// cache files under it instead of computed. Returns "" on success.
+std::string make_entry_kernel(const Tree& tree, const BasisSet& bs, const Problem& prob, bool find_phi,
+                              const std::string& table_dir, EntryKernel& K);
 
 // Build tables, NCA skeletons and the plan. Returns "" on success.
 std::string build_plan(const Tree& tree, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan);
