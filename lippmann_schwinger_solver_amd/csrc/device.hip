@@ -96,13 +96,16 @@ __device__ __forceinline__ void put_dst(C* state, XC* y, int d, int N, int r, C 
 // Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
 // step of the workgroup reads nph*R consecutive complex entries. The operator is
 // streamed once per matvec, so it is read with non-temporal loads. ROLE only
-// names the leaf M2L instantiation apart in kernel traces.
+// names the leaf M2L instantiation apart in kernel traces. A grid smaller than
+// the destination count walks the destinations with a stride of gridDim.x: the
+// leaf M2L runs that way beside the upward/downward chain, so that it holds
+// only part of each CU and the chain's small passes are dispatched at once.
 template <bool ACC, class R, int ROLE>
 __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
     const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off, const double2* const* __restrict__ xyp) {
+    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp) {
   // pointers from device memory when the matvec runs as a captured graph
   const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
   double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
   typedef typename cx<R>::v V;
   __shared__ C xs[kChunk];
   __shared__ C red[kThreads];
-  const int d = blockIdx.x;
+  for (int d = blockIdx.x; d < ndest; d += gridDim.x) {
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
@@ -173,6 +176,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
       put_dst<ACC>(state, y, dd, N, r0 + t, s);
     }
     __syncthreads();
+  }
   }
 }
 
@@ -748,6 +752,8 @@ std::string DeviceOperator::upload(const Plan& plan) {
   HIP_OK(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&ev_leaf, hipEventDisableTiming));
+  HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
   HIP_OK(hipMalloc((void**)&state, size_t(n_state) * sizeof(double2)));
   HIP_OK(hipMemset(state, 0, size_t(n_state) * sizeof(double2)));
   static_assert(sizeof(cplx) == sizeof(double2), "layout");
@@ -832,10 +838,28 @@ void DeviceOperator::release() {
   qk2_user = nullptr;
   if (ev_fork) hipEventDestroy(ev_fork);
   if (ev_join) hipEventDestroy(ev_join);
-  ev_fork = ev_join = nullptr;
+  if (ev_leaf) hipEventDestroy(ev_leaf);
+  ev_fork = ev_join = ev_leaf = nullptr;
   if (aux) hipStreamDestroy(aux);
   if (stream) hipStreamDestroy(stream);
   aux = stream = nullptr;
+}
+
+template <class R>
+static void launch_block(const DeviceOperator& o, const DevPass& p, const typename cx<R>::v* ops,
+                         typename cx<R>::t* st, const double2* x, double2* y, hipStream_t s,
+                         const double2* const* xyp) {
+  const int64_t b = p.dest_base;
+  const dim3 grid(p.grid_cap > 0 ? std::min(p.ndest, p.grid_cap) : p.ndest);
+  if (p.kind == 0)
+    hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+  else if (p.role == 1)
+    hipLaunchKernelGGL((pass_block_kernel<true, R, 1>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+  else
+    hipLaunchKernelGGL((pass_block_kernel<true, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
 }
 
 template <class R>
@@ -856,16 +880,7 @@ static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2
                          o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
     return;
   }
-  const dim3 grid(p.ndest);
-  if (p.kind == 0)
-    hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
-  else if (p.role == 1)
-    hipLaunchKernelGGL((pass_block_kernel<true, R, 1>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
-  else
-    hipLaunchKernelGGL((pass_block_kernel<true, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, xyp);
+  launch_block<R>(o, p, ops, st, x, y, s, xyp);
 }
 
 template <class R, int RK, int NL>
@@ -920,12 +935,16 @@ static void matvec_impl(DeviceOperator& o, const double2* x, double2* y, const d
     hipMemcpyAsync(o.state, x, size_t(o.N) * sizeof(double2), hipMemcpyDeviceToDevice, o.stream);
     x = o.state;
   }
-  // the first upward pass is the leaf level (the only one the leaf M2L reads)
-  size_t u = 0;
-  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream, xyp), u = 1;
+  // The near field needs only x: it starts on the second stream at once. The
+  // leaf M2L there waits for the first upward pass (the leaf level, the only
+  // charges it reads).
   hipEventRecord(o.ev_fork, o.stream);
   hipStreamWaitEvent(o.aux, o.ev_fork, 0);
   launch_near<R>(o, x, y, o.aux, xyp);
+  size_t u = 0;
+  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream, xyp), u = 1;
+  hipEventRecord(o.ev_leaf, o.stream);
+  hipStreamWaitEvent(o.aux, o.ev_leaf, 0);
   for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
   hipEventRecord(o.ev_join, o.aux);
   for (; u < o.up.size(); ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream, xyp);
@@ -1020,6 +1039,28 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   nchg += near_leaves_per_thread != nl_before;
   tuning.push_back("near_field: " + std::to_string(near_leaves_per_thread) + " leaves per thread (" +
                    std::to_string(t1 * 1e3f) + " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2)");
+  // Workgroup cap of the leaf M2L: it runs beside the upward/downward chain,
+  // so each cap is timed on the whole two-stream matvec.
+  for (int pi : leaf_m2l) {
+    DevPass& lp = passes[pi];
+    if (lp.wave) continue;
+    const int before = lp.grid_cap;
+    const int caps[] = {0, 6 * n_cu, 4 * n_cu, 3 * n_cu, 2 * n_cu};
+    int best_cap = 0;
+    float best_t = 1e30f;
+    std::string rep = lp.name + " workgroups:";
+    for (int c : caps) {
+      if (c >= lp.ndest) continue;
+      lp.grid_cap = c;
+      const float tm = time_it([&]() { matvec_impl<double>(*this, x, y); });
+      rep += " " + (c ? std::to_string(c) : std::string("all")) + " " + std::to_string(tm * 1e3f) + " us;";
+      if (tm < best_t) best_t = tm, best_cap = c;
+    }
+    lp.grid_cap = best_cap;
+    nchg += best_cap != before;
+    tuning.push_back(rep + " kept " + (best_cap ? std::to_string(best_cap) : std::string("all")) +
+                     " (full matvec)");
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
   HIP_OK(hipStreamSynchronize(stream));

@@ -13,6 +13,7 @@ struct DevPass {
   int kind = 0;        // 0 store, 1 accumulate
   int role = 0;        // 1: leaf M2L
   bool wave = false;   // one wavefront (not one workgroup) per destination
+  int grid_cap = 0;    // block variant: at most this many workgroups (0 = one per destination)
   int ndest = 0;
   int64_t dest_base = 0;  // index into the concatenated destination arrays
   int64_t entries = 0, n_src = 0, n_dst = 0;
@@ -40,15 +41,16 @@ struct Precond {
 };
 
 // One matvec = upward passes -> downward passes -> leaf L2L on the main
-// stream, while the near field and the leaf M2L (which need only x and the
-// leaf-level outgoing charges) run on a second stream after the first upward
-// pass and join before the leaf L2L. x is read and y written in place; the
+// stream, while the near field (from the start) and the leaf M2L (after the
+// first upward pass, whose leaf-level charges it reads) run on a second stream
+// and join before the leaf L2L. x is read and y written in place; the
 // state's own x and y regions are used only by profile() and in-place calls.
 struct DeviceOperator {
   hipStream_t stream = nullptr;   // main stream (all public calls are ordered on it)
   hipStream_t aux = nullptr;      // second stream, same default priority
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_leaf = nullptr;
   int device = 0;
+  int n_cu = 256;                 // compute units of the device
   int max_lds = 65536;            // LDS bytes per workgroup on this device
   int N = 0, rank = 0, n_leaves = 0;
   bool find_phi = true;
