@@ -8,6 +8,8 @@
  *
  *   reference                                         inspected by
  *   createUniformTree / assign_Tree_Interactions       lsg_host_info
+ *   createAdaptiveTree / makeLevelRestriction_Tree     lsg_host_level_boxes,
+ *     GMRES/G_FMM2DTree.hpp:776-1137                     lsg_host_leaf_boxes
  *     GMRES/G_FMM2DTree.hpp:833-941, 1356-1498
  *   evaluateM / writeMToFile (6-digit tables)          lsg_host_far_table
  *     GMRES/FarFieldInteraction.hpp:152-160, G_FMM2DTree.hpp:1252-1271
@@ -64,6 +66,19 @@ int64_t lsg_host_plan_nbr(const lsg_host* h, int* out);
 int64_t lsg_host_plan_qk2(const lsg_host* h, double* out);
 /* Per-level largest NCA rank, levels 0..nLevels. */
 int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels);
+/* Boxes and leaf boxes per level, levels 0..nLevels (the tree's tree[j].size()
+ * and the leaves among them; createAdaptiveTree + makeLevelRestriction_Tree,
+ * GMRES/G_FMM2DTree.hpp:776-831, 891-915, 1019-1137, for tree_mode 1). */
+int lsg_host_level_boxes(const lsg_host* h, int* boxes, int* leaves, int max_levels);
+/* The leaf order (leafNodes): per leaf its level and box index in the level;
+ * 2 ints per leaf, returns the leaf count. */
+int64_t lsg_host_leaf_boxes(const lsg_host* h, int* out);
+/* The NCA's ACA operands per skeleton owner, in the skeleton dump's order:
+ * 9 ints {level, box, cone (-1 = LFR), outgoing rows, columns, rank, incoming
+ * rows, columns, rank}, rank -1 where an operand list was empty (the n_rows,
+ * n_cols and ComputedRank of getNodes_*_box, G_FMM2DTree.hpp:2083-2212,
+ * 2294-2358, 3322-3392, 3485-3546). Returns the int count. */
+int64_t lsg_host_aca_shapes(const lsg_host* h, int* out);
 
 /* Checks of the GPU operator build (lsg_inputs.setup_device = 1) against the
  * host one; device = 0 evaluates on the host, 1 on the GPU.

@@ -138,6 +138,9 @@ SYMBOLS = [
     ("lsg_host_info", C.c_int, [_P, C.POINTER(lsg_info)]),
     ("lsg_host_far_table", C.c_int64, [_P, _D]),
     ("lsg_host_near_tables", C.c_int64, [_P, _D]),
+    ("lsg_host_level_boxes", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
+    ("lsg_host_leaf_boxes", C.c_int64, [_P, C.POINTER(C.c_int)]),
+    ("lsg_host_aca_shapes", C.c_int64, [_P, C.POINTER(C.c_int)]),
     ("lsg_host_skeleton", C.c_int64, [_P, C.POINTER(C.c_int)]),
     ("lsg_host_level_ranks", C.c_int, [_P, C.POINTER(C.c_int), C.c_int]),
     ("lsg_host_entries", C.c_int, [_P, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, _P]),
@@ -304,6 +307,28 @@ class HostBuild:
         out = np.zeros(16, dtype=np.int32)
         n = self._lib.lsg_host_level_ranks(self._h, out.ctypes.data_as(C.POINTER(C.c_int)), 16)
         return out[:n]
+
+    def level_boxes(self):
+        """(boxes per level, leaf boxes per level), levels 0..nLevels."""
+        b = np.zeros(16, dtype=np.int32)
+        lf = np.zeros(16, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        n = self._lib.lsg_host_level_boxes(self._h, b.ctypes.data_as(ip), lf.ctypes.data_as(ip), 16)
+        return b[:n], lf[:n]
+
+    def leaf_boxes(self):
+        """Leaf order: (n_leaves, 2) array of (level, box index in the level)."""
+        n = self._lib.lsg_host_leaf_boxes(self._h, None)
+        out = np.zeros(2 * n, dtype=np.int32)
+        self._lib.lsg_host_leaf_boxes(self._h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out.reshape(-1, 2)
+
+    def aca_shapes(self):
+        """Per skeleton owner: (level, box, cone, out rows, out cols, out rank, in rows, in cols, in rank)."""
+        n = self._lib.lsg_host_aca_shapes(self._h, None)
+        out = np.zeros(n, dtype=np.int32)
+        self._lib.lsg_host_aca_shapes(self._h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out.reshape(-1, 9)
 
     def il_pairs(self):
         out = np.zeros(16, dtype=np.int64)

@@ -543,7 +543,9 @@ std::string BlockOperator::setup(DeviceOperator& o) {
   }
   HIP_OK(hipMalloc((void**)&bstate, sizeof(double) * size_t(o.n_state + 1) * kLD));
   HIP_OK(hipMemset(bstate, 0, sizeof(double) * size_t(o.n_state + 1) * kLD));
-  // near field as one pass: leaf l reads the rank x (9 rank) side-by-side tables
+  // near field as one pass: leaf l reads the rank x (9 rank) side-by-side
+  // tables (uniform tree; an adaptive tree's near field is already a pass)
+  if (!o.T) return "";
   const int L = o.n_leaves, rk = o.rank;
   std::vector<int> nbr_h(size_t(L) * 9);
   HIP_OK(hipMemcpy(nbr_h.data(), o.nbr, nbr_h.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -664,11 +666,14 @@ static void block_matvec_impl(BlockOperator& bo, const double* X, double* Y) {
     hipLaunchKernelGGL(bround_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
                        bo.bstate32, (long long)o.N * kLD);
   for (int p : o.up) launch_bpass<R>(bo, p, X, Y);
-  launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, X, Y, bo.near_rows, bo.near_ncol,
-                             bo.near_dst, bo.near_op, bo.near_srcoff);
-  if (o.find_phi)
-    hipLaunchKernelGGL(bnear_finish_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
-                       o.qk2, (long long)o.N * kLD);
+  if (o.T) {
+    launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, X, Y, bo.near_rows, bo.near_ncol,
+                               bo.near_dst, bo.near_op, bo.near_srcoff);
+    if (o.find_phi)
+      hipLaunchKernelGGL(bnear_finish_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
+                         Y, o.qk2, (long long)o.N * kLD);
+  }
+  for (int p : o.near) launch_bpass<R>(bo, p, X, Y);
   for (int p : o.down) launch_bpass<R>(bo, p, X, Y);
   for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, X, Y);
   for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, X, Y);

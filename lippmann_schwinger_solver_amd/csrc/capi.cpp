@@ -48,8 +48,11 @@ std::string validate(const lsg_inputs* in) {
   if (!(in->kappa > 0) || !std::isfinite(in->kappa)) return "kappa must be positive";
   if (in->nCones_LFR < 1) return "nCones_LFR must be >= 1";
   if (in->TOL_POW < 1 || in->TOL_POW > 16) return "TOL_POW must be in [1, 16]";
-  if (in->tree_mode != 0) return "tree_mode must be 0: only the uniform tree is supported";
-  if (in->nLevelsUniform < 2 || in->nLevelsUniform > 9) return "nLevelsUniform must be in [2, 9]";
+  if (in->tree_mode != 0 && in->tree_mode != 1) return "tree_mode must be 0 (uniform) or 1 (adaptive)";
+  if (in->tree_mode == 0 && (in->nLevelsUniform < 2 || in->nLevelsUniform > 9))
+    return "nLevelsUniform must be in [2, 9]";
+  if (in->tree_mode == 1 && (in->treeAdaptivity < 1 || in->treeAdaptivity > 12))
+    return "treeAdaptivity must be in [1, 12]";
   if (in->Qchoice != in->contrast.kind) return "Qchoice must equal contrast.kind";
   if (in->contrast.kind < 0 || in->contrast.kind > 6) return "contrast.kind must be in [0, 6]";
   if (in->contrast.kind == 6 && (in->contrast.n_bumps < 0 || in->contrast.n_bumps > LSG_MAX_BUMPS))
@@ -76,11 +79,15 @@ int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_
   tp.yes2DFMM = in->yes2DFMM;
   tp.nLevelsUniform = in->nLevelsUniform;
   tp.tree_mode = in->tree_mode;
-  e = hb.tree.build(tp, hb.bs);
-  if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
+  tp.treeAdaptivity = in->treeAdaptivity;
   hb.prob.kappa = in->kappa;
   hb.prob.inc_angle = in->inc_angle;
   hb.prob.contrast = in->contrast;
+  const Problem* prob = &hb.prob;
+  tp.contrast = [prob](double x, double y) { return prob->q(x, y); };
+  e = hb.tree.build(tp, hb.bs);
+  hb.tree.p.contrast = nullptr;  // refers to hb.prob, used only while the tree grows
+  if (!e.empty()) { err = e; return LSG_ERR_SETUP; }
   if (tree_only) return LSG_OK;
   BuildOptions bo;
   bo.TOL_POW = in->TOL_POW;
@@ -272,6 +279,7 @@ int lsg_incident(const lsg_ctx* ctx, double* u) {
 int lsg_interpolate(lsg_ctx* ctx, const double* f, const double* x, const double* y, int64_t npts, double* out) {
   if (!ctx || !f || (npts > 0 && (!x || !y || !out)) || npts < 0) return LSG_ERR_ARG;
   if (npts == 0) return LSG_OK;
+  if (ctx->hb.in.tree_mode != 0) return ctx->set(LSG_ERR_ARG, "lsg_interpolate needs a uniform tree (tree_mode 0)");
   std::string e = ctx->field.setup(ctx->hb.tree, ctx->hb.bs);
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   const size_t N = size_t(ctx->hb.plan.N), n = size_t(npts);
@@ -431,6 +439,8 @@ int lsg_attach_preconditioner(lsg_ctx* ctx, lsgn_net* net) {
   ctx->prec.net = nullptr;
   ctx->prec.maps = nullptr;
   if (!net) return LSG_OK;
+  if (ctx->hb.in.tree_mode != 0)
+    return ctx->set(LSG_ERR_ARG, "the learned preconditioner's grid maps need a uniform tree (tree_mode 0)");
   const Tree& t = ctx->hb.tree;
   std::vector<double> cx(t.leaves.size()), cy(t.leaves.size());
   double radius = 0;
@@ -967,6 +977,36 @@ int64_t lsg_host_plan_qk2(const lsg_host* h, double* out) {
   if (!h) return LSG_ERR_ARG;
   if (out) std::memcpy(out, h->hb.plan.qk2.data(), sizeof(double) * h->hb.plan.qk2.size());
   return int64_t(h->hb.plan.qk2.size());
+}
+
+int lsg_host_level_boxes(const lsg_host* h, int* boxes, int* leaves, int max_levels) {
+  if (!h || !boxes || !leaves) return LSG_ERR_ARG;
+  const Tree& t = h->hb.tree;
+  const int n = std::min<int>(max_levels, t.nLevels + 1);
+  for (int l = 0; l < n; ++l) {
+    boxes[l] = int(t.lv[l].size());
+    int nl = 0;
+    for (const Box& b : t.lv[l]) nl += b.leaf ? 1 : 0;
+    leaves[l] = nl;
+  }
+  return n;
+}
+
+int64_t lsg_host_leaf_boxes(const lsg_host* h, int* out) {
+  if (!h) return LSG_ERR_ARG;
+  const Tree& t = h->hb.tree;
+  if (out)
+    for (size_t i = 0; i < t.leaves.size(); ++i) {
+      out[2 * i] = t.leaves[i].l;
+      out[2 * i + 1] = t.leaves[i].k;
+    }
+  return int64_t(t.leaves.size());
+}
+
+int64_t lsg_host_aca_shapes(const lsg_host* h, int* out) {
+  if (!h) return LSG_ERR_ARG;
+  if (out) std::memcpy(out, h->hb.plan.aca.data(), sizeof(int) * h->hb.plan.aca.size());
+  return int64_t(h->hb.plan.aca.size());
 }
 
 int lsg_host_level_ranks(const lsg_host* h, int* out, int max_levels) {

@@ -741,7 +741,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (rank > 64) return "nChebNodes^2 > 64 is not supported by the near-field kernel";
   HIP_OK(hipGetDevice(&device));
   HIP_OK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-  {
+  if (!plan.T.empty()) {
     const int nc = int(std::lround(std::sqrt(double(rank))));
     if (nc * nc != rank || nc < 2 || nc > 8) return "rank must be nChebNodes^2 with nChebNodes in [2, 8]";
     // the near kernel holds one rank x rank table and 2*(256/rank) leaf vectors in LDS
@@ -764,7 +764,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   std::vector<int> rows, ncol, dst;
   std::vector<long long> opo, srco;
   passes.clear();
-  up.clear(); down.clear(); leaf_m2l.clear(); leaf_l2l.clear();
+  up.clear(); down.clear(); leaf_m2l.clear(); leaf_l2l.clear(); near.clear();
   for (const PassHost& ph : plan.passes) {
     DevPass dp;
     dp.name = ph.name;
@@ -783,6 +783,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
     if (ph.name.rfind("up_", 0) == 0) up.push_back(idx);
     else if (ph.name == "leaf_m2l") leaf_m2l.push_back(idx), dp.role = 1;
     else if (ph.name == "leaf_l2l") leaf_l2l.push_back(idx);
+    else if (ph.name == "near") near.push_back(idx);
     else down.push_back(idx);
     dp.wave = dp.ndest > 0 && ph.entries <= int64_t(kWaveEntries) * dp.ndest;
     for (size_t q = 0; q < ph.rows.size(); ++q) {
@@ -798,14 +799,38 @@ std::string DeviceOperator::upload(const Plan& plan) {
       }
     passes.push_back(dp);
   }
+  {
+    // the leaf M2L waits for the last upward pass that writes one of its sources
+    std::vector<int> writer(size_t(n_state), -1);
+    for (size_t u = 0; u < up.size(); ++u) {
+      const PassHost& ph = plan.passes[size_t(up[u])];
+      for (size_t q = 0; q < ph.rows.size(); ++q)
+        for (int r = 0; r < ph.rows[q]; ++r) writer[size_t(ph.dst[q]) + r] = int(u);
+    }
+    leaf_after = 0;
+    for (int pi : leaf_m2l) {
+      const PassHost& ph = plan.passes[size_t(pi)];
+      for (int64_t q = 0; q < int64_t(ph.src_off.size()); ++q)
+        for (int c = 0; c < ph.ncol[q]; ++c) leaf_after = std::max(leaf_after, writer[size_t(plan.src[ph.src_off[q] + c])]);
+    }
+    // the near field runs beside the upward passes: it may read only x
+    for (int pi : near) {
+      const PassHost& ph = plan.passes[size_t(pi)];
+      for (int64_t q = 0; q < int64_t(ph.src_off.size()); ++q)
+        for (int c = 0; c < ph.ncol[q]; ++c)
+          if (plan.src[ph.src_off[q] + c] >= N) return "near-field pass reads beyond the input vector";
+    }
+  }
   if (!(e = upload_vec(rows, &d_rows)).empty()) return e;
   if (!(e = upload_vec(ncol, &d_ncol)).empty()) return e;
   if (!(e = upload_vec(dst, &d_dst)).empty()) return e;
   if (!(e = upload_vec(opo, &d_op)).empty()) return e;
   if (!(e = upload_vec(srco, &d_srcoff)).empty()) return e;
-  // near-field tables transposed to column-major per slot
+  // near-field tables transposed to column-major per slot (uniform tree only:
+  // an adaptive tree's near field is a pass)
   std::vector<double2> Tc(plan.T.size());
-  for (int s = 0; s < 9; ++s)
+  if (plan.T.size() != size_t(9) * rank * rank && !plan.T.empty()) return "near-field tables: expected 9 slots";
+  for (int s = 0; s < (plan.T.empty() ? 0 : 9); ++s)
     for (int i = 0; i < rank; ++i)
       for (int c = 0; c < rank; ++c) {
         const cplx v = plan.T[size_t(s) * rank * rank + size_t(i) * rank + c];
@@ -814,7 +839,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (!(e = upload_vec(Tc, &T)).empty()) return e;
   if (!(e = upload_vec(plan.nbr, &nbr)).empty()) return e;
   if (!(e = upload_vec(plan.qk2, &qk2)).empty()) return e;
-  near_bytes = 16 * int64_t(9) * rank * rank + 16 * 10 * int64_t(N) + 8 * int64_t(N);
+  near_bytes = plan.T.empty() ? 0 : 16 * int64_t(9) * rank * rank + 16 * 10 * int64_t(N) + 8 * int64_t(N);
   HIP_OK(hipDeviceSynchronize());
   return "";
 }
@@ -940,9 +965,10 @@ static void matvec_impl(DeviceOperator& o, const double2* x, double2* y, const d
   // charges it reads).
   hipEventRecord(o.ev_fork, o.stream);
   hipStreamWaitEvent(o.aux, o.ev_fork, 0);
-  launch_near<R>(o, x, y, o.aux, xyp);
+  if (o.T) launch_near<R>(o, x, y, o.aux, xyp);
+  for (int p : o.near) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
   size_t u = 0;
-  if (!o.up.empty()) launch_pass<R>(o, o.passes[o.up[0]], x, y, o.stream, xyp), u = 1;
+  for (; u < o.up.size() && int(u) <= o.leaf_after; ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream, xyp);
   hipEventRecord(o.ev_leaf, o.stream);
   hipStreamWaitEvent(o.aux, o.ev_leaf, 0);
   for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
@@ -1031,6 +1057,9 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
                      std::to_string(tw * 1e3f) + " us wave)");
   }
   const int nl_before = near_leaves_per_thread;
+  if (!T) {
+    tuning.push_back("near_field: passes (adaptive tree)");
+  } else {
   near_leaves_per_thread = 1;
   const float t1 = time_it([&]() { launch_near<double>(*this, x, y, stream); });
   near_leaves_per_thread = 2;
@@ -1039,6 +1068,7 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   nchg += near_leaves_per_thread != nl_before;
   tuning.push_back("near_field: " + std::to_string(near_leaves_per_thread) + " leaves per thread (" +
                    std::to_string(t1 * 1e3f) + " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2)");
+  }
   // Workgroup cap of the leaf M2L: it runs beside the upward/downward chain,
   // so each cap is timed on the whole two-stream matvec.
   for (int pi : leaf_m2l) {
@@ -1076,14 +1106,15 @@ __global__ void to_f32_kernel(const double2* __restrict__ a, float2* __restrict_
 
 std::string DeviceOperator::enable_fp32() {
   if (ops32) return "";
-  const long long n_ops = n_entries, n_T = 9LL * rank * rank;
+  const long long n_ops = n_entries, n_T = T ? 9LL * rank * rank : 0;
   HIP_OK(hipMalloc((void**)&ops32, sizeof(float2) * size_t(std::max(n_ops, 1LL))));
-  HIP_OK(hipMalloc((void**)&T32, sizeof(float2) * size_t(n_T)));
+  if (n_T) HIP_OK(hipMalloc((void**)&T32, sizeof(float2) * size_t(n_T)));
   HIP_OK(hipMalloc((void**)&state32, sizeof(float2) * size_t(n_state)));
   HIP_OK(hipMemsetAsync(state32, 0, sizeof(float2) * size_t(n_state), stream));
   if (n_ops > 0)
     hipLaunchKernelGGL(to_f32_kernel, dim3(unsigned((n_ops + 255) / 256)), dim3(256), 0, stream, ops, ops32, n_ops);
-  hipLaunchKernelGGL(to_f32_kernel, dim3(unsigned((n_T + 255) / 256)), dim3(256), 0, stream, T, T32, n_T);
+  if (n_T)
+    hipLaunchKernelGGL(to_f32_kernel, dim3(unsigned((n_T + 255) / 256)), dim3(256), 0, stream, T, T32, n_T);
   HIP_OK(hipStreamSynchronize(stream));
   return "";
 }
@@ -1124,7 +1155,8 @@ std::string DeviceOperator::profile(int reps, std::vector<double>& ms, std::vect
   const double2* x = state;
   double2* y = state + N;
   std::vector<int> order(up.begin(), up.end());
-  order.push_back(-1);
+  if (T) order.push_back(-1);
+  order.insert(order.end(), near.begin(), near.end());
   order.insert(order.end(), down.begin(), down.end());
   order.insert(order.end(), leaf_m2l.begin(), leaf_m2l.end());
   order.insert(order.end(), leaf_l2l.begin(), leaf_l2l.end());

@@ -42,8 +42,9 @@ struct Precond {
 
 // One matvec = upward passes -> downward passes -> leaf L2L on the main
 // stream, while the near field (from the start) and the leaf M2L (after the
-// first upward pass, whose leaf-level charges it reads) run on a second stream
-// and join before the leaf L2L. x is read and y written in place; the
+// upward passes that write the charges it reads: the first one, the leaf
+// level, on a uniform tree) run on a second stream and join before the leaf
+// L2L. x is read and y written in place; the
 // state's own x and y regions are used only by profile() and in-place calls.
 struct DeviceOperator {
   hipStream_t stream = nullptr;   // main stream (all public calls are ordered on it)
@@ -81,6 +82,12 @@ struct DeviceOperator {
   std::vector<std::string> tuning;      // autotuner decisions
   std::vector<DevPass> passes;
   std::vector<int> up, down, leaf_m2l, leaf_l2l;  // pass indices by role
+  // near field of an adaptive tree as passes (stored operators); empty for a
+  // uniform tree, whose near field is the 9-slot table kernel
+  std::vector<int> near;
+  // the leaf M2L starts once upward passes 0..leaf_after (in `up` order) are
+  // done: the last one writing a charge it reads
+  int leaf_after = 0;
   int64_t near_bytes = 0;
 
   std::string upload(const Plan& plan);

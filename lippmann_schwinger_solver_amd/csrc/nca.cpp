@@ -2,6 +2,7 @@
 
 #include "precomp.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -252,6 +253,9 @@ struct Contrib {
 
 struct Owner {
   bool out_active = false, in_active = false;
+  // ACA operand sizes and rank (-1: not run, an operand was empty), outgoing
+  // then incoming: {rows, cols, rank}
+  int aca_out[3] = {0, 0, -1}, aca_in[3] = {0, 0, -1};
   std::vector<int> out_check, out_charge, in_check, in_charge, ucp;
   std::vector<int> src_nodes;  // outgoing ACA columns
   std::vector<cplx> Ar;        // rank x src, row-major
@@ -383,6 +387,10 @@ struct Builder {
   static bool job_active(const Job& jb) { return !jb.rows.empty() && !jb.cols.empty(); }
   void complete(Job& jb) {
     Owner& o = *jb.o;
+    int* shape = jb.out ? o.aca_out : o.aca_in;
+    shape[0] = int(jb.rows.size());
+    shape[1] = int(jb.cols.size());
+    shape[2] = job_active(jb) ? jb.res.rank : -1;
     if (jb.out) {
       if (job_active(jb)) {
         o.out_active = true;
@@ -675,6 +683,125 @@ std::string make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& 
   return "";
 }
 
+// Near field of an adaptive tree (evaluate_NearField_Using_Precomputations_LS,
+// G_FMM2DTree.hpp:2884-2960) as one store pass named "near": leaf l's
+// destination is its rank rows, its columns the nodes of its neighbour leaves
+// slot by slot (colleagues, fine, separated fine, coarse), and its operator the
+// side-by-side blocks diag(k^2 q) T (+ I on the self slot; T alone for the
+// volume potential). T of slot s at leaf level j is getOperator for a source
+// box of level j, j+1 or j-1 centred at the slot centre times r_j (the
+// writeNeighborInteractionsForLevel geometry, :1812-1857); each table in use
+// is computed once (or read from the table cache).
+static std::string adaptive_near_pass(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt,
+                                      const std::vector<double>& qk2, Plan& plan) {
+  const int rk = bs.rank, nL = int(t.leaves.size());
+  // kinds in the table cache's numbering: 0 colleague, 1 fine, 2 separated fine, 3 coarse
+  static const int kSlots[4] = {9, 12, 20, 12};
+  static const int kDl[4] = {0, 1, 1, -1};
+  auto centres = [](int kind, int s) -> const double* {
+    switch (kind) {
+      case 0: return kColleagueCenters[s];
+      case 1: return kFineCenters[s];
+      case 2: return kSepFineCenters[s];
+      default: return kCoarseCenters[s];
+    }
+  };
+  auto slot_box = [](const Box& b, int kind, int s) {
+    switch (kind) {
+      case 0: return b.colleague[s];
+      case 1: return b.fine[s];
+      case 2: return b.sepfine[s];
+      default: return b.coarse[s];
+    }
+  };
+  // tables in use, keyed by (level, kind, slot)
+  auto key = [](int l, int kind, int s) { return (l * 4 + kind) * 32 + s; };
+  std::vector<int> need;
+  for (int li = 0; li < nL; ++li) {
+    const Box& b = t.box(t.leaves[li]);
+    const int l = t.leaves[li].l;
+    for (int kind = 0; kind < 4; ++kind)
+      for (int s = 0; s < kSlots[kind]; ++s) {
+        const int nk = slot_box(b, kind, s);
+        if (nk < 0) continue;
+        const int nl = l + kDl[kind];
+        if (nl < 0 || nl > t.nLevels || nk >= int(t.lv[nl].size()) || !t.lv[nl][nk].leaf)
+          return "near field: a neighbour slot of leaf " + std::to_string(li) + " is not a leaf";
+        need.push_back(key(l, kind, s));
+      }
+  }
+  std::sort(need.begin(), need.end());
+  need.erase(std::unique(need.begin(), need.end()), need.end());
+  std::vector<std::vector<cplx>> tabs(need.size());
+  std::vector<std::string> errs(need.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int q = 0; q < int(need.size()); ++q) {
+    const int l = need[q] / 128, kind = (need[q] / 32) % 4, s = need[q] % 32;
+    const double rA = t.boxRadius[l], rB = t.boxRadius[l + kDl[kind]];
+    const double* c = centres(kind, s);
+    if (opt.table_dir.empty()) tabs[q] = near_table(bs, prob.kappa, rA, rB, c[0] * rA, c[1] * rA);
+    else errs[q] = read_neighbor_table(opt.table_dir, bs, prob.kappa, l, kind, s, tabs[q]);
+  }
+  for (const std::string& e : errs)
+    if (!e.empty()) return "tables: " + e;
+  auto table = [&](int l, int kind, int s) -> const std::vector<cplx>& {
+    return tabs[std::lower_bound(need.begin(), need.end(), key(l, kind, s)) - need.begin()];
+  };
+
+  PassHost ph;
+  ph.name = "near";
+  ph.level = t.nLevels;
+  ph.kind = 0;
+  int64_t op_total = int64_t(plan.ops.size()), src_total = int64_t(plan.src.size());
+  for (int li = 0; li < nL; ++li) {
+    const Box& b = t.box(t.leaves[li]);
+    const int l = t.leaves[li].l;
+    int ns = 0;
+    for (int kind = 0; kind < 4; ++kind)
+      for (int s = 0; s < kSlots[kind]; ++s) ns += slot_box(b, kind, s) >= 0;
+    if (ns == 0) continue;
+    const int nc = ns * rk;
+    ph.rows.push_back(rk);
+    ph.ncol.push_back(nc);
+    ph.dst.push_back(t.N + li * rk);
+    ph.op_off.push_back(op_total);
+    ph.src_off.push_back(src_total);
+    plan.ops.resize(size_t(op_total) + size_t(nc) * rk);
+    plan.src.resize(size_t(src_total) + size_t(nc));
+    cplx* op = plan.ops.data() + op_total;
+    int* sp = plan.src.data() + src_total;
+    int col = 0;
+    // slot order of the reference: colleagues, fine, separated fine, coarse
+    for (int kind = 0; kind < 4; ++kind)
+      for (int s = 0; s < kSlots[kind]; ++s) {
+        const int nk = slot_box(b, kind, s);
+        if (nk < 0) continue;
+        const int nleaf = t.lv[l + kDl[kind]][nk].leaf_index;
+        const std::vector<cplx>& T = table(l, kind, s);   // row-major rk x rk
+        for (int c = 0; c < rk; ++c) {
+          sp[col + c] = nleaf * rk + c;
+          for (int i = 0; i < rk; ++i) {
+            cplx v = T[size_t(i) * rk + c];
+            if (opt.find_phi) {
+              v *= qk2[size_t(li) * rk + i];
+              if (kind == 0 && s == 8 && i == c) v += 1.0;
+            }
+            op[size_t(col + c) * rk + i] = v;
+          }
+        }
+        col += rk;
+      }
+    op_total += int64_t(nc) * rk;
+    src_total += nc;
+    ph.entries += int64_t(nc) * rk;
+    ph.n_src += nc;
+    ph.n_dst += rk;
+  }
+  plan.E_near = ph.entries;
+  if (!ph.rows.empty()) plan.passes.push_back(std::move(ph));
+  return "";
+}
+
 std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt, Plan& plan) {
   typedef std::chrono::steady_clock clk;
   auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
@@ -717,6 +844,8 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
                        int(o.out_charge.size()), int(o.in_check.size()), int(o.in_charge.size()), int(o.ucp.size())});
     for (const std::vector<int>* v : {&o.out_check, &o.out_charge, &o.in_check, &o.in_charge, &o.ucp})
       d.insert(d.end(), v->begin(), v->end());
+    plan.aca.insert(plan.aca.end(), {l, k, cone, o.aca_out[0], o.aca_out[1], o.aca_out[2], o.aca_in[0], o.aca_in[1],
+                                     o.aca_in[2]});
   };
   for (int l = t.nLevels; l >= 2; --l) {
     for (int k = 0; k < int(t.lv[l].size()); ++k)
@@ -795,7 +924,7 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
     if (!ds.empty()) passes.push_back({ph, std::move(ds)});
   }
   {
-    // uniform tree: every leaf M2L source is a leaf-level box
+    // leaf rows: M2L from the interaction lists, then L2L from the parents
     PassHost ph;
     ph.name = "leaf_m2l";
     ph.level = t.nLevels;
@@ -901,6 +1030,13 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
 
   phase[2] = since(tp);
   tp = clk::now();
+  if (t.p.tree_mode == 1) {
+    std::string e = adaptive_near_pass(t, bs, prob, opt, K.qk2, plan);
+    if (!e.empty()) return e;
+    phase[3] = since(tp);
+    for (int i = 0; i < 4; ++i) plan.phase_seconds[i] = phase[i];
+    return "";
+  }
   // near field: colleague tables of the leaf level (uniform tree)
   const int lvl = t.nLevels;
   const double rA = t.boxRadius[lvl];

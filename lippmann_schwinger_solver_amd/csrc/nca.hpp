@@ -79,6 +79,10 @@ struct Plan {
   // in_active, |out_check|, |out_charge|, |in_check|, |in_charge|, |ucp|, then
   // the five lists (node ids).
   std::vector<int> skeleton;
+  // Per owner, in the skeleton dump's order: level, box, cone (-1 = LFR), then
+  // the outgoing and the incoming ACA's rows, columns and rank (-1 when an
+  // operand list was empty and no ACA ran).
+  std::vector<int> aca;
   std::vector<cplx> far_leaf;  // far-field table of the leaf level (after rounding)
   // build time of: far-field tables, NCA skeletons, operator fill, near-field tables
   double phase_seconds[4] = {0, 0, 0, 0};

@@ -152,4 +152,17 @@ std::string read_colleague_tables(const std::string& root, const BasisSet& bs, d
   return "";
 }
 
+std::string read_neighbor_table(const std::string& root, const BasisSet& bs, double kappa, int level, int kind,
+                                int slot, std::vector<cplx>& T) {
+  if (kind < 0 || kind > 3) return "neighbour table kind must be in [0, 3]";
+  int r = 0, c = 0;
+  const std::string p = near_table_dir(root, bs.n, kappa, bs.deg) + "/" + kNeighborKinds[kind] + "_" +
+                        std::to_string(level) + "_" + std::to_string(slot);
+  std::string e = read_matrix(p, T, r, c);
+  if (!e.empty()) return e;
+  if (r != bs.rank || c != bs.rank)
+    return p + ": expected a " + std::to_string(bs.rank) + " x " + std::to_string(bs.rank) + " table";
+  return "";
+}
+
 }  // namespace lsg

@@ -34,5 +34,9 @@ std::string read_far_table(const std::string& root, const BasisSet& bs, double k
                            std::vector<cplx>& M);
 std::string read_colleague_tables(const std::string& root, const BasisSet& bs, double kappa, int level,
                                   std::vector<cplx>& T);
+// One neighbour table of a level: kind 0 colleague, 1 fine, 2 separated fine,
+// 3 coarse (readNeighborInteractions, G_FMM2DTree.hpp:1911-1950).
+std::string read_neighbor_table(const std::string& root, const BasisSet& bs, double kappa, int level, int kind,
+                                int slot, std::vector<cplx>& T);
 
 }  // namespace lsg

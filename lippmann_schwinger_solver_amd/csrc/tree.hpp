@@ -4,12 +4,16 @@
 //
 // Reference: GMRES/G_FMM2DTree.hpp
 //   FMM2DTree ctor / level_LFR            :293-328
+//   get_Transfer_Matrix / resolveContrast :669-757
+//   createTree / createAdaptiveTree       :776-831, 891-915
 //   createTreeU / createUniformTree       :833-889, 917-941
+//   makeLevelRestriction_Tree (2:1)       :943-1137
 //   assign_Child_Interaction (IL rules)   :1356-1461
 //   getNeighbors (slot numbering)         :1500-1745
 //   createCones                           :1952-1974
 //   assignLeafChargeLocations (node order):1150-1163
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 #include "numerics.hpp"
@@ -32,7 +36,13 @@ struct Box {
   std::vector<BoxRef> nbrs;                  // neighborNumbers (incl. self)
   std::vector<BoxRef> il;                    // LFR interaction list
   std::vector<std::vector<BoxRef>> cone_il;  // HFR: interaction list per cone
+  // near-field slots (box index at the slot's level, -1 = empty): colleagues
+  // (same level, 8 = self), coarse (level - 1), fine and separated fine
+  // (level + 1); only adaptive trees fill the last three
   int colleague[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int coarse[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int fine[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int sepfine[20] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   int leaf_index = -1;                       // position in leaf order
 };
 
@@ -43,7 +53,9 @@ struct TreeParams {
   double kappa = 1.0;
   int yes2DFMM = 1;
   int nLevelsUniform = 4;
-  int tree_mode = 0;  // 0 uniform; adaptive trees are not built by this library
+  int tree_mode = 0;       // 0 uniform (createUniformTree), 1 adaptive (createAdaptiveTree + 2:1 restriction)
+  int treeAdaptivity = 6;  // adaptive: contrast resolved to 10^-treeAdaptivity
+  std::function<double(double, double)> contrast;  // adaptive: q(x, y)
 };
 
 struct Tree {

@@ -43,6 +43,7 @@ def _load():
             "or_far_table": (I64, [P, D]),
             "or_near_tables": (I64, [P, D]),
             "or_skeleton": (I64, [P, C.POINTER(C.c_int)]),
+            "or_neighbor_table": (I64, [P, C.c_int, C.c_int, C.c_int, D]),
             "or_bessel": (C.c_int, [C.c_int, C.c_double, D, D]),
             "or_qpinv": (I64, [P, D]),
             "or_box_center": (C.c_int, [P, C.c_int, C.c_int, D]),
@@ -137,6 +138,17 @@ class Oracle:
         out = np.zeros(2 * n)
         self._lib.or_near_tables(self._h, _d(out))
         return out.view(np.complex128)
+
+    def neighbor_table(self, level, kind, slot):
+        """Near-field table in use at a leaf level (kind 0 colleague, 1 coarse,
+        2 fine, 3 separated fine), or None when no leaf uses the slot."""
+        n = self._lib.or_neighbor_table(self._h, level, kind, slot, None)
+        if n < 0:
+            return None
+        out = np.zeros(2 * n)
+        self._lib.or_neighbor_table(self._h, level, kind, slot, _d(out))
+        r = int(round(np.sqrt(n)))
+        return out.view(np.complex128).reshape(r, r)
 
     def skeleton(self):
         n = self._lib.or_skeleton(self._h, None)

@@ -438,6 +438,10 @@ struct FMM2DBox {  // GMRES/G_FMM2DTree.hpp:154-227
   bool isLeaf = false;
   std::vector<orderedPair> neighborNumbers, InteractionList;
   int colleagueNeighbors[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int coarseNeighbors[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int fineNeighbors[12] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  int separatedFineNeighbors[20] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                                    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   Vec outgoing_charges, outgoing_potential, incoming_charges, incoming_potential, multipoles;
   std::vector<int> incoming_chargePoints, incoming_checkPoints, user_checkPoints, outgoing_chargePoints,
       outgoing_checkPoints, chargeLocations;
@@ -448,14 +452,26 @@ struct FMM2DBox {  // GMRES/G_FMM2DTree.hpp:154-227
   std::vector<FMM2DCone> ConeTree;
 };
 
+// neighbour slot centres in units of the target box radius (G_FMM2DTree.hpp:19-72)
 const double N_centers[9][2] = {{-2, -2}, {0, -2}, {2, -2}, {2, 0}, {2, 2}, {0, 2}, {-2, 2}, {-2, 0}, {0, 0}};
+const double SFN_centers[20][2] = {{-2.5, -2.5}, {-1.5, -2.5}, {-0.5, -2.5}, {0.5, -2.5}, {1.5, -2.5},
+                                   {2.5, -2.5},  {2.5, -1.5},  {2.5, -0.5},  {2.5, 0.5},  {2.5, 1.5},
+                                   {2.5, 2.5},   {1.5, 2.5},   {0.5, 2.5},   {-0.5, 2.5}, {-1.5, 2.5},
+                                   {-2.5, 2.5},  {-2.5, 1.5},  {-2.5, 0.5},  {-2.5, -0.5}, {-2.5, -1.5}};
+const double FN_centers[12][2] = {{-1.5, -1.5}, {-0.5, -1.5}, {0.5, -1.5}, {1.5, -1.5}, {1.5, -0.5}, {1.5, 0.5},
+                                  {1.5, 1.5},   {0.5, 1.5},   {-0.5, 1.5}, {-1.5, 1.5}, {-1.5, 0.5}, {-1.5, -0.5}};
+const double CN_centers[12][2] = {{-3, -3}, {-1, -3}, {1, -3}, {3, -3}, {3, -1}, {3, 1},
+                                  {3, 3},   {1, 3},   {-1, 3}, {-3, 3}, {-3, 1}, {-3, -1}};
+// neighbour kinds as getIfNeighbor numbers them (G_FMM2DTree.hpp:335-516)
+enum { kColleague = 0, kCoarse = 1, kFine = 2, kSepFine = 3 };
 const double epsilonRoundOff = 1e-8;
 
 struct FMM2DTree {
   Problem Q;
   bool findPhi = true;
   int nLevels = 0, nChebNodes = 0, rank = 0, N = 0, level_LFR = 2, nCones_LFR = 16, TOL_POW = 10, degreeOfBases = 6,
-      nLevelsUniform = 4, yes2DFMM = 1;
+      nLevelsUniform = 4, yes2DFMM = 1, treeAdaptivity = 6, tree_mode = 0;
+  double epsilonTree = 1e-6;
   double L = 0.5, kappa = 1.0;
   std::vector<double> boxRadius, ConeAperture;
   std::vector<int> nCones;
@@ -468,12 +484,18 @@ struct FMM2DTree {
   std::vector<double> Q_pinv;        // NumberOfBases x rank, row-major
   std::vector<Mat> M;                // per level (only leaf levels filled)
   Mat colleagueNeighborInteraction[9];  // leaf level (uniform tree)
+  // neighbour tables in use, keyed (level * 4 + kind) * 32 + slot
+  std::vector<int> tabKeys;
+  std::vector<Mat> tabs;
+  std::vector<double> L2Lc;  // get_Transfer_Matrix: 4 x rank x rank
   std::vector<double> contrastK2;    // kappa*kappa*q at nodes
 
   // ---------------------------------------------------------- setup
   std::string setup(const lsg_inputs& in) {
     nCones_LFR = in.nCones_LFR; nChebNodes = in.nChebNodes; L = in.L; kappa = in.kappa; yes2DFMM = in.yes2DFMM;
     degreeOfBases = in.degreeOfBases; TOL_POW = in.TOL_POW; nLevelsUniform = in.nLevelsUniform;
+    treeAdaptivity = in.treeAdaptivity; tree_mode = in.tree_mode;
+    epsilonTree = std::pow(10, -treeAdaptivity);  // :301
     findPhi = in.find_phi != 0;
     Q.kappa = kappa; Q.angle = in.inc_angle; Q.con = in.contrast;
     rank = nChebNodes * nChebNodes;
@@ -486,7 +508,14 @@ struct FMM2DTree {
       level_LFR = 2;
     }
     set_Standard_Cheb_Nodes();
-    createUniformTree();
+    if (tree_mode == 1) {
+      get_Transfer_Matrix();
+      std::string e = createAdaptiveTree();
+      if (!e.empty()) return e;
+      makeLevelRestriction_Tree();
+    } else {
+      createUniformTree();
+    }
     if (nLevels < 2) return "tree too shallow";
     createCones();
     assign_Tree_Interactions();
@@ -494,12 +523,16 @@ struct FMM2DTree {
     evaluateQ();
     getNeighbors();
     for (int i = 0; i < N; ++i) contrastK2.push_back(kappa * kappa * Q.ContrastFunction(gpx[i], gpy[i]));
-    // tables of the leaf level, with the 6-digit text round trip
+    // tables of the leaf levels, with the 6-digit text round trip
     M.assign(nLevels + 1, Mat());
-    M[nLevels] = evaluateM(nLevels);
-    for (int n = 0; n < 9; ++n)
-      colleagueNeighborInteraction[n] = getOperator(nLevels, nLevels, N_centers[n][0] * boxRadius[nLevels],
-                                                    N_centers[n][1] * boxRadius[nLevels]);
+    {
+      std::set<int> leafLevels;
+      for (auto& lf : leafNodes) leafLevels.insert(lf.x);
+      for (int l : leafLevels) M[l] = evaluateM(l);
+    }
+    evaluateNeighborTables();
+    if (tree_mode == 0)
+      for (int n = 0; n < 9; ++n) colleagueNeighborInteraction[n] = table(nLevels, kColleague, n);
     getNodes_LFR();
     getNodes_HFR();
     getUserCheckPoints();
@@ -517,6 +550,150 @@ struct FMM2DTree {
       for (int k = 0; k < nChebNodes; ++k) { scx.push_back(standardChebNodes1D[k]); scy.push_back(standardChebNodes1D[j]); }
     for (int d = 0; d < degreeOfBases; ++d)
       for (int a = 0; a < d + 1; ++a) degreeOfBasesVec.push_back({a, d - a});
+  }
+
+  // get_ChebPoly / get_S / get_Transfer_Matrix (:670-733): L2L[c](j, k) =
+  // S(x_k, xc_j) S(y_k, yc_j), xc the nodes of child c in the parent's frame
+  double get_S(double x, double y, int n) const {
+    double S = 0.5;
+    for (int k = 1; k < n; ++k) S += get_ChebPoly(x, k) * get_ChebPoly(y, k);
+    return 2.0 / n * S;
+  }
+  void childNode(int c, int j, double& x, double& y) const {  // standardChebNodesChild, :696-719
+    const double sx = (c == 0 || c == 3) ? -0.5 : 0.5, sy = (c < 2) ? -0.5 : 0.5;
+    x = sx < 0 ? 0.5 * scx[j] - 0.5 : 0.5 * scx[j] + 0.5;
+    y = sy < 0 ? 0.5 * scy[j] - 0.5 : 0.5 * scy[j] + 0.5;
+  }
+  void get_Transfer_Matrix() {
+    L2Lc.assign(size_t(4) * rank * rank, 0.0);
+    for (int l = 0; l < 4; ++l)
+      for (int j = 0; j < rank; ++j) {
+        double x, y;
+        childNode(l, j, x, y);
+        for (int k = 0; k < rank; ++k)
+          L2Lc[(size_t(l) * rank + j) * rank + k] = get_S(scx[k], x, nChebNodes) * get_S(scy[k], y, nChebNodes);
+      }
+  }
+  // resolveContrast (:738-754): max |q - L2L q| over the children's nodes
+  double resolveContrast(int j, int k) const {
+    const FMM2DBox& b = tree[j][k];
+    std::vector<double> qa(rank);
+    for (int i = 0; i < rank; ++i) qa[i] = Q.ContrastFunction(b.radius * scx[i] + b.cx, b.radius * scy[i] + b.cy);
+    double err = 0.0;
+    for (int c = 0; c < 4; ++c)
+      for (int i = 0; i < rank; ++i) {
+        double ap = 0.0;
+        for (int q = 0; q < rank; ++q) ap += L2Lc[(size_t(c) * rank + i) * rank + q] * qa[q];
+        double x, y;
+        childNode(c, i, x, y);
+        err = std::max(err, std::fabs(Q.ContrastFunction(b.radius * x + b.cx, b.radius * y + b.cy) - ap));
+      }
+    return err;
+  }
+  void addChildren(int j, int k, bool leaves) {  // box creation of createTree / refineBox
+    if (int(tree.size()) == j + 1) tree.emplace_back();
+    const int n = int(tree[j + 1].size());
+    const int b = tree[j][k].boxNumber;
+    for (int c = 0; c < 4; ++c) {
+      FMM2DBox box;
+      box.level = j + 1;
+      box.boxNumber = b * 4 + c;
+      box.parentNumber = b;
+      box.radius = 0.5 * tree[j][k].radius;
+      box.isLeaf = leaves;
+      const double r = tree[j][k].radius;
+      if (c == 0) { box.cx = tree[j][k].cx - 0.5 * r; box.cy = tree[j][k].cy - 0.5 * r; }
+      else if (c == 1) { box.cx = tree[j][k].cx + 0.5 * r; box.cy = tree[j][k].cy - 0.5 * r; }
+      else if (c == 2) { box.cx = tree[j][k].cx + 0.5 * r; box.cy = tree[j][k].cy + 0.5 * r; }
+      else { box.cx = tree[j][k].cx - 0.5 * r; box.cy = tree[j][k].cy + 0.5 * r; }
+      tree[j][k].child[c] = n + c;
+      tree[j + 1].push_back(box);
+    }
+  }
+  // createTree (:776-831): leaf when the contrast is resolved and the box
+  // spans at most one wavelength (the right-hand-side test is always true)
+  bool createTree(int j, int k) {
+    const bool condition1Leaf = resolveContrast(j, k) < epsilonTree;
+    const bool condition3Leaf = kappa * tree[j][k].radius / 2 / PI <= 1.0;
+    if (condition1Leaf && condition3Leaf) {
+      tree[j][k].isLeaf = true;
+      leafNodes.push_back({j, k});
+      return true;
+    }
+    if (j >= 12) return false;  // the reference has no depth limit
+    const int n = int(j + 1 < int(tree.size()) ? tree[j + 1].size() : 0);
+    addChildren(j, k, false);
+    for (int c = 0; c < 4; ++c)
+      if (!createTree(j + 1, n + c)) return false;
+    return true;
+  }
+  std::string createAdaptiveTree() {  // :891-915
+    FMM2DBox root;
+    root.level = 0; root.boxNumber = 0; root.radius = L;
+    tree.assign(1, std::vector<FMM2DBox>(1, root));
+    if (!createTree(0, 0)) return "adaptive tree deeper than 12 levels";
+    nLevels = int(tree.size()) - 1;
+    for (int j = 1; j <= std::max(10, nLevels + 1); ++j) boxRadius.push_back(boxRadius[j - 1] / 2.0);
+    if (level_LFR >= nLevels) level_LFR = nLevels - 1;
+    return "";
+  }
+  // assign_Tree_Interactions_LFR and helpers (:966-1017): for every box, the
+  // leaves near it
+  void assign_Tree_Interactions_LFR(std::vector<std::vector<std::vector<orderedPair>>>& TN) {
+    for (int c = 0; c < 4; ++c)
+      for (int n = 0; n < 4; ++n) TN[1][c].push_back({1, n});
+    for (int j = 1; j <= nLevels - 1; ++j)
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        if (tree[j][k].isLeaf) continue;
+        for (int c = 0; c < 4; ++c) {
+          const int a = tree[j][k].child[c];
+          for (size_t n = 0; n < TN[j][k].size(); ++n) {
+            std::vector<orderedPair> fg;
+            getFutureGeneration_LFR(j + 1, a, TN[j][k][n].x, TN[j][k][n].y, fg);
+            for (auto& f : fg) TN[j + 1][a].push_back(f);
+          }
+        }
+      }
+  }
+  int getIndexOfInLeafNodes(orderedPair op) const {  // :1070-1076
+    for (size_t l = 0; l < leafNodes.size(); ++l)
+      if (op.x == leafNodes[l].x && op.y == leafNodes[l].y) return int(l);
+    return -1;
+  }
+  void refineBox(int l, orderedPair leaf, std::vector<orderedPair>& leafNodesOld) {  // :1102-1137
+    const int j = leaf.x, k = leaf.y;
+    leafNodesOld[l].x = -1;
+    tree[j][k].isLeaf = false;
+    const int n = int(tree[j + 1].size());
+    addChildren(j, k, true);
+    for (int c = 0; c < 4; ++c) leafNodes.push_back({j + 1, n + c});
+  }
+  void makeLevelRestriction_Tree() {  // :1019-1068, 1078-1100
+    for (int sweep = 0; sweep < nLevels; ++sweep) {
+      std::vector<std::vector<std::vector<orderedPair>>> TN(nLevels + 1);
+      for (int j = 0; j <= nLevels; ++j) TN[j].resize(tree[j].size());
+      assign_Tree_Interactions_LFR(TN);
+      std::vector<orderedPair> leafNodesOld = leafNodes;
+      for (size_t l = 0; l < leafNodesOld.size(); ++l) {
+        const orderedPair leaf = leafNodes[l];
+        const int j = leaf.x, k = leaf.y;
+        for (size_t i = 0; i < TN[j][k].size(); ++i) {
+          const int nj = TN[j][k][i].x;
+          if (j - nj >= 2) {
+            const int idx = getIndexOfInLeafNodes(TN[j][k][i]);
+            if (leafNodesOld[idx].x != -1) refineBox(idx, TN[j][k][i], leafNodesOld);
+          } else if (nj - j >= 2) {
+            if (leafNodesOld[l].x != -1) refineBox(int(l), leaf, leafNodesOld);
+          }
+        }
+      }
+      int cnt = 0;
+      for (size_t l = 0; l < leafNodesOld.size(); ++l)
+        if (leafNodesOld[l].x == -1) {
+          leafNodes.erase(leafNodes.begin() + (long)l - cnt);
+          cnt++;
+        }
+    }
   }
 
   void createTreeU(int j, int k) {  // :833-889
@@ -649,22 +826,96 @@ struct FMM2DTree {
       }
     }
   }
-  void getNeighbors() {  // colleague slots, :1567-1601
-    double N_angles[9];
-    for (int i = 0; i < 9; ++i) N_angles[i] = std::atan2(N_centers[i][1], N_centers[i][0]);
+  // getIfNeighbor's slot rule (:335-516), shared with getNeighbors
+  // (:1500-1745): kind and slot of box (nj, nk) seen from (j, k), or kind -1
+  void neighborSlot(int j, int k, int nj, int nk, int& kind, int& slot) const {
+    const FMM2DBox& A = tree[j][k];
+    const FMM2DBox& B = tree[nj][nk];
+    kind = -1;
+    slot = -1;
+    if (std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[j] + boxRadius[nj] - epsilonRoundOff ||
+        std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[j] + boxRadius[nj] - epsilonRoundOff)
+      return;
+    const double angle = std::atan2(B.cy - A.cy, B.cx - A.cx);
+    if (A.radius == B.radius) {
+      kind = kColleague;
+      if (std::fabs(B.cy - A.cy) < epsilonRoundOff && std::fabs(B.cx - A.cx) < epsilonRoundOff) slot = 8;
+      else
+        for (int i = 0; i < 8; ++i)
+          if (std::fabs(std::atan2(N_centers[i][1], N_centers[i][0]) - angle) < epsilonRoundOff) { slot = i; break; }
+    } else if (A.radius < B.radius) {
+      kind = kCoarse;
+      for (int i = 0; i < 12; ++i)
+        if (std::fabs(std::atan2(CN_centers[i][1], CN_centers[i][0]) - angle) < epsilonRoundOff) { slot = i; break; }
+    } else {
+      const double d = std::sqrt((A.cx - B.cx) * (A.cx - B.cx) + (A.cy - B.cy) * (A.cy - B.cy));
+      if (d > 2.5 * A.radius) {
+        kind = kSepFine;
+        slot = 19;
+        for (int i = 0; i < 19; ++i)
+          if (std::fabs(std::atan2(SFN_centers[i][1], SFN_centers[i][0]) - angle) < epsilonRoundOff) { slot = i; break; }
+      } else {
+        kind = kFine;
+        for (int i = 0; i < 12; ++i)
+          if (std::fabs(std::atan2(FN_centers[i][1], FN_centers[i][0]) - angle) < epsilonRoundOff) { slot = i; break; }
+      }
+    }
+  }
+  void getNeighbors() {  // :1567-1745
     for (int j = 1; j <= nLevels; ++j)
-      for (auto& A : tree[j])
+      for (int k = 0; k < int(tree[j].size()); ++k) {
+        FMM2DBox& A = tree[j][k];
         for (auto& nb : A.neighborNumbers) {
           const FMM2DBox& B = tree[nb.x][nb.y];
-          double angle = std::atan2(B.cy - A.cy, B.cx - A.cx);
+          int kind, slot;
           if (A.radius == B.radius) {
+            // colleagues are matched without the separation test
+            const double angle = std::atan2(B.cy - A.cy, B.cx - A.cx);
             if (std::fabs(B.cy - A.cy) < epsilonRoundOff && std::fabs(B.cx - A.cx) < epsilonRoundOff)
               A.colleagueNeighbors[8] = nb.y;
             else
               for (int i = 0; i < 8; ++i)
-                if (std::fabs(N_angles[i] - angle) < epsilonRoundOff) { A.colleagueNeighbors[i] = nb.y; break; }
+                if (std::fabs(std::atan2(N_centers[i][1], N_centers[i][0]) - angle) < epsilonRoundOff) {
+                  A.colleagueNeighbors[i] = nb.y;
+                  break;
+                }
+            continue;
           }
+          neighborSlot(j, k, nb.x, nb.y, kind, slot);
+          if (kind == kCoarse && slot >= 0) A.coarseNeighbors[slot] = nb.y;
+          else if (kind == kFine && slot >= 0) A.fineNeighbors[slot] = nb.y;
+          else if (kind == kSepFine) A.separatedFineNeighbors[slot] = nb.y;
         }
+      }
+  }
+  // the neighbour table of a slot (writeNeighborInteractionsForLevel geometry,
+  // :1812-1857): source box of level j (colleague), j-1 (coarse) or j+1 (fine,
+  // separated fine) centred at the slot centre times r_j
+  static int tabKey(int l, int kind, int s) { return (l * 4 + kind) * 32 + s; }
+  const Mat& table(int l, int kind, int s) const {
+    return tabs[std::lower_bound(tabKeys.begin(), tabKeys.end(), tabKey(l, kind, s)) - tabKeys.begin()];
+  }
+  void evaluateNeighborTables() {
+    std::set<int> need;
+    for (auto& lf : leafNodes) {
+      const FMM2DBox& A = tree[lf.x][lf.y];
+      for (int s = 0; s < 9; ++s) if (A.colleagueNeighbors[s] >= 0) need.insert(tabKey(lf.x, kColleague, s));
+      for (int s = 0; s < 12; ++s) if (A.coarseNeighbors[s] >= 0) need.insert(tabKey(lf.x, kCoarse, s));
+      for (int s = 0; s < 12; ++s) if (A.fineNeighbors[s] >= 0) need.insert(tabKey(lf.x, kFine, s));
+      for (int s = 0; s < 20; ++s) if (A.separatedFineNeighbors[s] >= 0) need.insert(tabKey(lf.x, kSepFine, s));
+    }
+    if (tree_mode == 0)
+      for (int s = 0; s < 9; ++s) need.insert(tabKey(nLevels, kColleague, s));
+    tabKeys.assign(need.begin(), need.end());
+    tabs.assign(tabKeys.size(), Mat());
+    for (size_t q = 0; q < tabKeys.size(); ++q) {
+      const int l = tabKeys[q] / 128, kind = (tabKeys[q] / 32) % 4, s = tabKeys[q] % 32;
+      const double rA = boxRadius[l];
+      const double* c = kind == kColleague ? N_centers[s] : kind == kCoarse ? CN_centers[s]
+                      : kind == kFine ? FN_centers[s] : SFN_centers[s];
+      const int lB = kind == kColleague ? l : kind == kCoarse ? l - 1 : l + 1;
+      tabs[q] = getOperator(l, lB, c[0] * rA, c[1] * rA);
+    }
   }
 
   // ------------------------------------------------------- tables
@@ -768,14 +1019,12 @@ struct FMM2DTree {
     const int li = i / rank, lj = j / rank;
     const FMM2DBox& A = tree[leafNodes[li].x][leafNodes[li].y];
     const FMM2DBox& B = tree[leafNodes[lj].x][leafNodes[lj].y];
-    const int lev = leafNodes[li].x;
-    const bool touch = !(std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff ||
-                         std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff);
-    if (touch) {
-      int slot = -1;
-      for (int s = 0; s < 9; ++s)
-        if (A.colleagueNeighbors[s] == leafNodes[lj].y) slot = s;
-      cd entry = colleagueNeighborInteraction[slot](i % rank, j % rank);
+    (void)A;
+    (void)B;
+    int kind, slot;
+    neighborSlot(leafNodes[li].x, leafNodes[li].y, leafNodes[lj].x, leafNodes[lj].y, kind, slot);
+    if (kind >= 0) {
+      cd entry = table(leafNodes[li].x, kind, slot)(i % rank, j % rank);
       if (findPhi) {
         entry = entry * contrastK2[i];
         if (i == j) entry = 1.0 + entry;
@@ -789,10 +1038,10 @@ struct FMM2DTree {
     const int li = i / rank;
     const FMM2DBox& A = tree[leafNodes[li].x][leafNodes[li].y];
     const FMM2DBox& B = tree[leafNodes[lj].x][leafNodes[lj].y];
-    const int lev = leafNodes[li].x;
-    const bool touch = !(std::fabs(A.cx - B.cx) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff ||
-                         std::fabs(A.cy - B.cy) >= 3.0 * boxRadius[lev] + boxRadius[leafNodes[lj].x] - epsilonRoundOff);
-    if (touch) {
+    (void)A;
+    int kind, slot;
+    neighborSlot(leafNodes[li].x, leafNodes[li].y, leafNodes[lj].x, leafNodes[lj].y, kind, slot);
+    if (kind >= 0) {
       for (int q = 0; q < rank; ++q) out[q] = getMatrixEntry2(i, lj * rank + q);
       return;
     }
@@ -1353,20 +1602,25 @@ struct FMM2DTree {
   void evaluate_NearField_Using_Precomputations_LS() {  // :2884-2960
 #pragma omp parallel for schedule(dynamic, 4)
     for (int t = 0; t < int(leafNodes.size()); ++t) {
-      FMM2DBox& A = tree[leafNodes[t].x][leafNodes[t].y];
-      for (int n = 0; n < 9; ++n) {
-        const int nk = A.colleagueNeighbors[n];
-        if (nk == -1) continue;
-        const FMM2DBox& B = tree[leafNodes[t].x][nk];
-        Mat op = colleagueNeighborInteraction[n];
+      const int j = leafNodes[t].x;
+      FMM2DBox& A = tree[j][leafNodes[t].y];
+      // colleagues, fine, separated fine, coarse, in the reference's order
+      auto apply = [&](int kind, int n, int nj, int nk) {
+        if (nk == -1) return;
+        const FMM2DBox& B = tree[nj][nk];
+        Mat op = table(j, kind, n);
         if (findPhi) {
           for (int i = 0; i < rank; ++i)
             for (int c = 0; c < rank; ++c) op(i, c) = contrastK2[A.chargeLocations[i]] * op(i, c);
-          if (n == 8)
+          if (kind == kColleague && n == 8)
             for (int i = 0; i < rank; ++i) op(i, i) = op(i, i) + 1.0;
         }
         addto(A.incoming_potential, matvec(op, B.multipoles));
-      }
+      };
+      for (int n = 0; n < 9; ++n) apply(kColleague, n, j, A.colleagueNeighbors[n]);
+      for (int n = 0; n < 12; ++n) apply(kFine, n, j + 1, A.fineNeighbors[n]);
+      for (int n = 0; n < 20; ++n) apply(kSepFine, n, j + 1, A.separatedFineNeighbors[n]);
+      for (int n = 0; n < 12; ++n) apply(kCoarse, n, j - 1, A.coarseNeighbors[n]);
     }
   }
   void collectPotential(Vec& potential) {  // :3045-3054
@@ -1536,6 +1790,24 @@ int64_t or_near_tables(const or_ctx* c, double* out) {
           out[2 * o + 1] = c->t.colleagueNeighborInteraction[s](i, j).imag();
         }
   return int64_t(9) * r * r;
+}
+
+// A neighbour table in use (kind 0 colleague, 1 coarse, 2 fine, 3 separated
+// fine) of leaf level `level`, row-major; returns rank^2, or -1 when no leaf
+// of that level uses the slot.
+int64_t or_neighbor_table(const or_ctx* c, int level, int kind, int slot, double* out) {
+  const oracle::FMM2DTree& t = c->t;
+  const int key = oracle::FMM2DTree::tabKey(level, kind, slot);
+  auto it = std::lower_bound(t.tabKeys.begin(), t.tabKeys.end(), key);
+  if (it == t.tabKeys.end() || *it != key) return -1;
+  const oracle::Mat& T = t.tabs[size_t(it - t.tabKeys.begin())];
+  if (out)
+    for (int i = 0; i < T.r; ++i)
+      for (int j = 0; j < T.c; ++j) {
+        out[2 * (size_t(i) * T.c + j)] = T(i, j).real();
+        out[2 * (size_t(i) * T.c + j) + 1] = T(i, j).imag();
+      }
+  return int64_t(T.r) * T.c;
 }
 
 // same layout as the product's skeleton dump

@@ -107,14 +107,31 @@ def fan_quad(f, X, nphi=20, nr=20, grade=0.25, levels=14):
     return total
 
 
-def near_entry_indep(n, deg, kappa, r, slot_center, i, j):
-    """Colleague near-field operator entry T(i, j) (target node i of the box at the
-    origin, source node j of the neighbour centred at slot_center * r)."""
+def near_entry_indep(n, deg, kappa, r, slot_center, i, j, rB=None):
+    """Near-field operator entry T(i, j): target node i of the box of radius r at
+    the origin, source node j of the neighbour of radius rB (default r) centred
+    at slot_center * r."""
+    rB = r if rB is None else rB
     Q, X, Y = vandermonde(n, deg)
     P = np.linalg.pinv(Q)
-    Xi = np.array([(r * X[i] - slot_center[0] * r) / r, (r * Y[i] - slot_center[1] * r) / r])
+    Xi = np.array([(r * X[i] - slot_center[0] * r) / rB, (r * Y[i] - slot_center[1] * r) / rB])
     val = 0j
     for l, (a, b) in enumerate(basis_ab(deg)):
-        f = lambda px, py: sp.hankel1(0, kappa * r * np.hypot(Xi[0] - px, Xi[1] - py)) * cheb(px, a) * cheb(py, b)
+        f = lambda px, py: sp.hankel1(0, kappa * rB * np.hypot(Xi[0] - px, Xi[1] - py)) * cheb(px, a) * cheb(py, b)
         val += fan_quad(f, Xi) * P[l, j]
-    return 1j * r * r / 4.0 * val
+    return 1j * rB * rB / 4.0 * val
+
+
+def near_entry_gl(n, deg, kappa, r, slot_center, i, j, rB=None, ng=64):
+    """near_entry_indep by a tensor Gauss-Legendre rule over the source box, for
+    sources well separated from the target node (smooth integrand)."""
+    rB = r if rB is None else rB
+    Q, X, Y = vandermonde(n, deg)
+    P = np.linalg.pinv(Q)
+    Xi = np.array([(r * X[i] - slot_center[0] * r) / rB, (r * Y[i] - slot_center[1] * r) / rB])
+    g, w = np.polynomial.legendre.leggauss(ng)
+    PX, PY = np.meshgrid(g, g, indexing="ij")
+    W = np.outer(w, w)
+    Hk = sp.hankel1(0, kappa * rB * np.hypot(Xi[0] - PX, Xi[1] - PY))
+    val = sum(np.sum(W * Hk * cheb(PX, a) * cheb(PY, b)) * P[l, j] for l, (a, b) in enumerate(basis_ab(deg)))
+    return 1j * rB * rB / 4.0 * val

@@ -81,7 +81,7 @@ def test_bumps_are_seeded_and_in_range():
 
 
 def test_invalid_inputs_are_rejected():
-    for bad in (dict(nChebNodes=1), dict(kappa=-1.0), dict(TOL_POW=0), dict(nLevelsUniform=1), dict(tree_mode=1)):
+    for bad in (dict(nChebNodes=1), dict(kappa=-1.0), dict(TOL_POW=0), dict(nLevelsUniform=1), dict(tree_mode=2)):
         inp = H.inputs(depth=3, kappa=10.0)
         for k, v in bad.items():
             setattr(inp, k, v)
