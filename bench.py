@@ -315,6 +315,65 @@ def bench_setup(args):
     print(json.dumps(out), flush=True)
 
 
+# GMRES/Readme.md:43-115: the reference's sample run, the only timings it
+# publishes for this path (hardware and thread count unstated)
+README_ARGS = dict(nCones_LFR=16, nChebNodes=6, treeAdaptivity=5, L=0.5, kappa=40.0, yes2DFMM=1, degreeOfBases=6,
+                   TOL_POW=8, m=10, restart=1, nLevelsUniform=5, Qchoice=0, tree_mode=1)
+README_SWEEP = [((1, 10), 0.277782, 4.59412e-06), ((1, 50), 1.47135, 1.40294e-15), ((1, 100), 2.90013, 1.30732e-15),
+                ((1, 200), 6.22192, 1.39836e-15), ((1, 500), 19.1974, 1.40164e-15), ((2, 10), 0.751761, 1.28443e-10),
+                ((2, 50), 2.63477, 8.96364e-16), ((2, 100), 5.29393, 7.94185e-16), ((2, 200), 11.7579, 8.37373e-16),
+                ((2, 500), 38.9819, 7.82761e-16)]
+README_SETUP_S, README_MATVEC_V_S = 285.211, 0.011503
+
+
+def bench_readme(args):
+    """The reference's own sample run (adaptive tree, 5328 unknowns, kappa 40,
+    TOL_POW 8; `testFMM2D_GMRES 16 6 5 0.5 40.0 1 6 8 10 1 5 0`) on this GPU:
+    setup, every entry of its GMRES sweep (fixed cycles from x0 = 0, true FP64
+    residual) and the volume-potential matvec, beside the README's times."""
+    inp = lsg.InputsToDFMM(**README_ARGS)
+    t0 = time.time()
+    S = lsg.DFMM(inp)
+    S.FMMTreeInitialisation()
+    setup_s = time.time() - t0
+    info = S.info()
+    n_changed, report = S.autotune(reps=3)
+    log(f"autotune ({n_changed} changed):\n{report}")
+    b = S.rhs()
+    bn = np.linalg.norm(b)
+    sweep = []
+    for (restart, m), t_ref, r_ref in README_SWEEP:
+        x = lsg.RestartedGMRES(restart, S, m, inp, b)     # warm-up (workspace of this m)
+        S._check(S._lib.lsg_sync(S._ctx))
+        reps = 5
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            x = lsg.RestartedGMRES(restart, S, m, inp, b)
+        S._check(S._lib.lsg_sync(S._ctx))
+        t = (time.perf_counter() - t1) / reps
+        r = float(np.linalg.norm(b - S.MatVecProduct(x)) / bn)
+        rec = {"restart": restart, "m": m, "seconds": t, "readme_seconds": t_ref, "speedup": t_ref / t,
+               "true_rel_residual": r, "readme_rel_residual": r_ref}
+        log(f"GMRES({restart},{m}): {t * 1e3:.2f} ms (README {t_ref} s, {t_ref / t:.0f}x), residual {r:.5e} "
+            f"(README {r_ref:.5e})")
+        sweep.append(rec)
+    V = lsg.DFMM(lsg.InputsToDFMM(**dict(README_ARGS, find_phi=0)))
+    V.FMMTreeInitialisation()
+    V.autotune(reps=3)
+    mv_v = V.time_matvec(reps=50) * 1e-3
+    out = {"workload": "README run: adaptive tree (nLevels 5, 148 leaves, 5328 unknowns), kappa 40, Gaussian "
+                       "contrast, TOL_POW 8, GMRES sweep of GMRES.cpp main",
+           "N": int(info["N"]), "nLevels": int(info["nLevels"]), "level_LFR": int(info["level_LFR"]),
+           "setup_s": setup_s, "readme_setup_s": README_SETUP_S, "setup_speedup": README_SETUP_S / setup_s,
+           "matvec_ms": S.time_matvec(reps=50), "matvec_volume_potential_ms": mv_v * 1e3,
+           "readme_matvec_volume_potential_ms": README_MATVEC_V_S * 1e3,
+           "matvec_volume_potential_speedup": README_MATVEC_V_S / mv_v, "sweep": sweep,
+           "readme_hardware": "unstated CPU (GMRES/Readme.md)"}
+    print(json.dumps(out), flush=True)
+    S.close()
+    V.close()
+
+
 def bench_learned(args):
     """The learned preconditioner at the configuration it is specified for:
     trained on 4096 solves the solver generates (depth 6, k in {40, 50, 60, 70,
@@ -420,10 +479,11 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
-    ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged", "setup"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged", "setup", "readme"], default="c2",
                     help="c2: the default line; c3: depth 7, kappa 160 mixed-precision and batched legs; "
                          "learned: train the network on 4096 generated solves and evaluate on 256 held-out; "
-                         "ragged: 48 independent scatterers; setup: operator build on the host vs the GPU")
+                         "ragged: 48 independent scatterers; setup: operator build on the host vs the GPU; "
+                         "readme: the reference's README run (adaptive tree) beside its published times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver is single-device (multi-GPU sharding is a design "
@@ -437,6 +497,8 @@ def main():
         return bench_learned(args)
     if args.workload == "ragged":
         return bench_ragged(args)
+    if args.workload == "readme":
+        return bench_readme(args)
 
     inp = c2_inputs(0)
     t0 = time.time()

@@ -36,6 +36,10 @@ struct inputsToDFMM {
   int degreeOfBases = 6;
   int treeAdaptivity = 6;
   int nLevelsUniform = 4;
+  // GMRES/GMRES.cpp:164-166: the reference builds createAdaptiveTree +
+  // makeLevelRestriction_Tree (1); createUniformTree (0) is its commented-out
+  // alternative, used by the uniform benchmark configurations
+  int tree_mode = 1;
   // not in the reference struct: kappa and Qchoice are globals there
   double kappa = 20.0;
   int Qchoice = 0;
@@ -50,6 +54,7 @@ class DFMM {
   std::vector<pts2D> gridPoints;
   bool findPhi = true;  // K->findPhi of the reference (GMRES/GMRES.cpp:583)
   int N = 0;
+  int nLevels = 0, level_LFR = 0;  // K->nLevels, K->level_LFR
 
   explicit DFMM(const inputsToDFMM& in) : inputs(in) {}
   ~DFMM() { lsg_destroy(ctx_); }
@@ -69,7 +74,7 @@ class DFMM {
     c.TOL_POW = inputs.TOL_POW;
     c.nLevelsUniform = inputs.nLevelsUniform;
     c.Qchoice = inputs.Qchoice;
-    c.tree_mode = 0;
+    c.tree_mode = inputs.tree_mode;
     c.find_phi = findPhi ? 1 : 0;
     c.inc_angle = inputs.inc_angle;
     c.contrast = inputs.contrast;
@@ -80,6 +85,8 @@ class DFMM {
     lsg_info info;
     lsg_get_info(ctx_, &info);
     N = int(info.N);
+    nLevels = info.nLevels;
+    level_LFR = info.level_LFR;
     std::vector<double> x(N), y(N);
     lsg_grid_points(ctx_, x.data(), y.data());
     gridPoints.resize(N);

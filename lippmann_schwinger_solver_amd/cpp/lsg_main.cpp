@@ -5,9 +5,12 @@
 //   gridPointsX, gridPointsY, solutionR, solutionI, leftRightBoundary,
 //   bottomTopBoundary
 // plus full-precision sidecars Phi_<restart>_<m>.f64 (binary complex128).
-// The tree is uniform of depth nLevelsUniform (the reference switches to it by
-// editing GMRES.cpp:164-165). Extra optional arguments after the 12:
-//   argv[13] incidence angle, argv[14] bump seed (Qchoice 6), argv[15] #bumps.
+// The tree is the reference's adaptive one (createAdaptiveTree +
+// makeLevelRestriction_Tree, GMRES.cpp:165-166, leaves resolving the contrast to
+// 10^-treeAdaptivity). Extra optional arguments after the 12:
+//   argv[13] incidence angle, argv[14] bump seed (Qchoice 6), argv[15] #bumps,
+//   argv[16] tree: 1 adaptive (default), 0 uniform of depth nLevelsUniform (the
+//   reference's commented-out createUniformTree, GMRES.cpp:164).
 #include <sys/stat.h>
 
 #include <chrono>
@@ -36,7 +39,7 @@ int main(int argc, char* argv[]) {
   if (argc < 13) {
     std::cerr << "usage: " << argv[0]
               << " nCones_LFR nChebNodes treeAdaptivity L kappa yes2DFMM degreeOfBases TOL_POW m restart "
-                 "nLevelsUniform Qchoice [angle seed nbumps]\n";
+                 "nLevelsUniform Qchoice [angle seed nbumps tree]\n";
     return 1;
   }
   inputsToDFMM in;
@@ -53,6 +56,7 @@ int main(int argc, char* argv[]) {
   in.nLevelsUniform = std::atoi(argv[11]);
   in.Qchoice = std::atoi(argv[12]);
   if (argc > 13) in.inc_angle = std::atof(argv[13]);
+  if (argc > 16) in.tree_mode = std::atoi(argv[16]);
   if (in.Qchoice == 6) {
     unsigned long long seed = argc > 14 ? std::strtoull(argv[14], nullptr, 10) : 0;
     int nb = argc > 15 ? std::atoi(argv[15]) : 5;
@@ -61,12 +65,15 @@ int main(int argc, char* argv[]) {
   const double PI = 3.1415926535897932384;
   std::cout << "Wavenumber:\t\t" << in.kappa << std::endl;
   std::cout << "Wavelength:\t\t" << 2 * PI / in.kappa << std::endl;
+  std::cout << "no. of full cycles:\t" << in.L * in.kappa / 2 * PI << std::endl;
   try {
     double t0 = now();
     DFMM S(in);
     S.FMMTreeInitialisation();
     std::cout << std::endl << "Time taken to assemble: " << now() - t0 << std::endl;
     const int N = S.N;
+    std::cout << "nLevels: " << S.nLevels << std::endl;
+    std::cout << "level_LFR: " << S.level_LFR << std::endl;
     std::cout << "Number of particles: " << N << std::endl;
     Vec b = S.rhs();
     mkdir("result", 0775);
