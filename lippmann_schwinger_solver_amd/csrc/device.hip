@@ -96,10 +96,9 @@ __device__ __forceinline__ void put_dst(C* state, XC* y, int d, int N, int r, C 
 // Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
 // step of the workgroup reads nph*R consecutive complex entries. The operator is
 // streamed once per matvec, so it is read with non-temporal loads. ROLE only
-// names the leaf M2L instantiation apart in kernel traces. A grid smaller than
-// the destination count walks the destinations with a stride of gridDim.x: the
-// leaf M2L runs that way beside the upward/downward chain, so that it holds
-// only part of each CU and the chain's small passes are dispatched at once.
+// names the leaf M2L instantiation apart in kernel traces. Workgroups walk the
+// destinations with a stride of gridDim.x (one destination each when the grid
+// covers them all).
 template <bool ACC, class R, int ROLE>
 __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
@@ -354,7 +353,8 @@ __global__ __launch_bounds__(kThreads) void near_kernel(const typename cx<R>::t*
 //   gs_proj_dot w' = w - V h1, partials of V^H w' and |w'|^2
 //   gs_tail     h2, h = h1 + h2, ||w''||^2 = |w'|^2 - |h2|^2, Givens update
 //   gs_update   v_{j+1} = (w' - V h2) / ||w''||
-constexpr int kGsRows = 2 * kThreads;  // rows per workgroup in the dot kernels
+constexpr int kGsRpt = 2;                  // rows per thread in the dot kernels
+constexpr int kGsRows = kGsRpt * kThreads;  // rows per workgroup in the dot kernels
 constexpr int kGsCols = 8;             // columns per workgroup in gs_dot
 
 __device__ __forceinline__ double2 wave_sum(double2 v) {
@@ -395,31 +395,36 @@ __device__ __forceinline__ void store_block_partials(double2 (*ws)[kGsCols], con
 }
 
 // partial[blockIdx.x*ncx + col] for col in [blockIdx.y*8, +8): conj(V[:,col]).w over
-// the workgroup's 512 rows for col < nc, |w|^2 for col == nc (when ncx = nc+1).
+// the workgroup's kGsRows rows for col < nc, |w|^2 for col == nc (when ncx = nc+1).
 __global__ __launch_bounds__(kThreads) void gs_dot_kernel(const double2* __restrict__ V, long long ldv, int nc,
                                                           int ncx, const double2* __restrict__ w, int N,
                                                           double2* __restrict__ partial) {
   __shared__ double2 ws[kThreads / 64][kGsCols];
   const int t = threadIdx.x;
-  const long long n0 = (long long)blockIdx.x * kGsRows + t, n1 = n0 + kThreads;
   const int g0 = blockIdx.y * kGsCols;
   const double2 z = make_double2(0.0, 0.0);
-  const double2 w0 = n0 < N ? w[n0] : z, w1 = n1 < N ? w[n1] : z;
+  long long n[kGsRpt];
+  double2 wv[kGsRpt];
+#pragma unroll
+  for (int r = 0; r < kGsRpt; ++r) {
+    n[r] = (long long)blockIdx.x * kGsRows + t + r * kThreads;
+    wv[r] = n[r] < N ? w[n[r]] : z;
+  }
   double2 acc[kGsCols];
 #pragma unroll
   for (int u = 0; u < kGsCols; ++u) {
     const int col = g0 + u;
-    double2 v0 = z, v1 = z;
-    if (col < nc) {
-      if (n0 < N) v0 = V[col * ldv + n0];
-      if (n1 < N) v1 = V[col * ldv + n1];
-    } else if (col == nc) {
-      v0 = w0;
-      v1 = w1;
-    }
     acc[u] = z;
-    cdot(acc[u], v0, w0);
-    cdot(acc[u], v1, w1);
+#pragma unroll
+    for (int r = 0; r < kGsRpt; ++r) {
+      double2 v = z;
+      if (col < nc) {
+        if (n[r] < N) v = V[col * ldv + n[r]];
+      } else if (col == nc) {
+        v = wv[r];
+      }
+      cdot(acc[u], v, wv[r]);
+    }
   }
   store_block_partials(ws, acc, g0, ncx, partial, blockIdx.x);
 }
@@ -463,7 +468,7 @@ __global__ __launch_bounds__(kThreads) void gs_reduce_kernel(const double2* __re
 }
 
 // w <- w - V h (h = first nc entries of hg); then partials of conj(V[:,c]).w for c < nc and
-// |w|^2 (column nc) per workgroup of 512 rows.
+// |w|^2 (column nc) per workgroup of kGsRows rows.
 __global__ __launch_bounds__(kThreads) void gs_proj_dot_kernel(const double2* __restrict__ V, long long ldv, int nc,
                                                                const double2* __restrict__ hg, double2* __restrict__ w,
                                                                int N, double2* __restrict__ partial) {
@@ -472,36 +477,42 @@ __global__ __launch_bounds__(kThreads) void gs_proj_dot_kernel(const double2* __
   const int t = threadIdx.x;
   for (int i = t; i < nc; i += kThreads) hs[i] = hg[i];
   __syncthreads();
-  const long long n0 = (long long)blockIdx.x * kGsRows + t, n1 = n0 + kThreads;
-  const bool ok0 = n0 < N, ok1 = n1 < N;
   const double2 z = make_double2(0.0, 0.0);
-  double2 w0 = ok0 ? w[n0] : z, w1 = ok1 ? w[n1] : z;
+  long long n[kGsRpt];
+  bool ok[kGsRpt];
+  double2 wv[kGsRpt];
+#pragma unroll
+  for (int r = 0; r < kGsRpt; ++r) {
+    n[r] = (long long)blockIdx.x * kGsRows + t + r * kThreads;
+    ok[r] = n[r] < N;
+    wv[r] = ok[r] ? w[n[r]] : z;
+  }
   {
-    double2 a0 = z, a1 = z;
+    double2 a[kGsRpt];
+#pragma unroll
+    for (int r = 0; r < kGsRpt; ++r) a[r] = z;
     int i = 0;
     for (; i + 4 <= nc; i += 4) {
-      double2 v0[4], v1[4];
+      double2 v[kGsRpt][4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        v0[u] = ok0 ? V[(i + u) * ldv + n0] : z;
-        v1[u] = ok1 ? V[(i + u) * ldv + n1] : z;
-      }
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cmac(a0, v0[u], hs[i + u]);
-        cmac(a1, v1[u], hs[i + u]);
-      }
+        for (int r = 0; r < kGsRpt; ++r) v[r][u] = ok[r] ? V[(i + u) * ldv + n[r]] : z;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < kGsRpt; ++r) cmac(a[r], v[r][u], hs[i + u]);
     }
-    for (; i < nc; ++i) {
-      if (ok0) cmac(a0, V[i * ldv + n0], hs[i]);
-      if (ok1) cmac(a1, V[i * ldv + n1], hs[i]);
+    for (; i < nc; ++i)
+#pragma unroll
+      for (int r = 0; r < kGsRpt; ++r)
+        if (ok[r]) cmac(a[r], V[i * ldv + n[r]], hs[i]);
+#pragma unroll
+    for (int r = 0; r < kGsRpt; ++r) {
+      wv[r].x -= a[r].x;
+      wv[r].y -= a[r].y;
+      if (ok[r]) w[n[r]] = wv[r];
     }
-    w0.x -= a0.x;
-    w0.y -= a0.y;
-    w1.x -= a1.x;
-    w1.y -= a1.y;
-    if (ok0) w[n0] = w0;
-    if (ok1) w[n1] = w1;
   }
   const int ncx = nc + 1;
   for (int g0 = 0; g0 < ncx; g0 += kGsCols) {
@@ -509,17 +520,17 @@ __global__ __launch_bounds__(kThreads) void gs_proj_dot_kernel(const double2* __
 #pragma unroll
     for (int u = 0; u < kGsCols; ++u) {
       const int col = g0 + u;
-      double2 v0 = z, v1 = z;
-      if (col < nc) {
-        if (ok0) v0 = V[col * ldv + n0];
-        if (ok1) v1 = V[col * ldv + n1];
-      } else if (col == nc) {
-        v0 = w0;
-        v1 = w1;
-      }
       acc[u] = z;
-      cdot(acc[u], v0, w0);
-      cdot(acc[u], v1, w1);
+#pragma unroll
+      for (int r = 0; r < kGsRpt; ++r) {
+        double2 v = z;
+        if (col < nc) {
+          if (ok[r]) v = V[col * ldv + n[r]];
+        } else if (col == nc) {
+          v = wv[r];
+        }
+        cdot(acc[u], v, wv[r]);
+      }
     }
     store_block_partials(ws, acc, g0, ncx, partial, blockIdx.x);
   }
@@ -875,7 +886,7 @@ static void launch_block(const DeviceOperator& o, const DevPass& p, const typena
                          typename cx<R>::t* st, const double2* x, double2* y, hipStream_t s,
                          const double2* const* xyp) {
   const int64_t b = p.dest_base;
-  const dim3 grid(p.grid_cap > 0 ? std::min(p.ndest, p.grid_cap) : p.ndest);
+  const dim3 grid(p.ndest);
   if (p.kind == 0)
     hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
                        o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
@@ -1068,28 +1079,6 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   nchg += near_leaves_per_thread != nl_before;
   tuning.push_back("near_field: " + std::to_string(near_leaves_per_thread) + " leaves per thread (" +
                    std::to_string(t1 * 1e3f) + " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2)");
-  }
-  // Workgroup cap of the leaf M2L: it runs beside the upward/downward chain,
-  // so each cap is timed on the whole two-stream matvec.
-  for (int pi : leaf_m2l) {
-    DevPass& lp = passes[pi];
-    if (lp.wave) continue;
-    const int before = lp.grid_cap;
-    const int caps[] = {0, 6 * n_cu, 4 * n_cu, 3 * n_cu, 2 * n_cu};
-    int best_cap = 0;
-    float best_t = 1e30f;
-    std::string rep = lp.name + " workgroups:";
-    for (int c : caps) {
-      if (c >= lp.ndest) continue;
-      lp.grid_cap = c;
-      const float tm = time_it([&]() { matvec_impl<double>(*this, x, y); });
-      rep += " " + (c ? std::to_string(c) : std::string("all")) + " " + std::to_string(tm * 1e3f) + " us;";
-      if (tm < best_t) best_t = tm, best_cap = c;
-    }
-    lp.grid_cap = best_cap;
-    nchg += best_cap != before;
-    tuning.push_back(rep + " kept " + (best_cap ? std::to_string(best_cap) : std::string("all")) +
-                     " (full matvec)");
   }
   hipEventDestroy(a);
   hipEventDestroy(b);

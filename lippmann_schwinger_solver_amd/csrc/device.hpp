@@ -13,7 +13,6 @@ struct DevPass {
   int kind = 0;        // 0 store, 1 accumulate
   int role = 0;        // 1: leaf M2L
   bool wave = false;   // one wavefront (not one workgroup) per destination
-  int grid_cap = 0;    // block variant: at most this many workgroups (0 = one per destination)
   int ndest = 0;
   int64_t dest_base = 0;  // index into the concatenated destination arrays
   int64_t entries = 0, n_src = 0, n_dst = 0;
