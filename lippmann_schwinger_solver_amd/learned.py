@@ -187,6 +187,39 @@ def make_pairs(S, kappa, seeds, tol=1e-8, restart_len=50, mixed=True):
     return np.stack(xs), np.stack(ts), iters
 
 
+def make_inverse_pairs(S, kappa, seeds, n_waves=6):
+    """Forward pairs for the right preconditioner, one block matvec per 64
+    samples instead of 64 solves: sample s has a contrast q_s (as in make_pairs)
+    and a field y_s = (q_s/max q) times a random superposition of plane waves of
+    wavenumber kappa; v_s = A_s y_s. The network maps (q, v/sigma) to
+    (y - v)/sigma, so that M(v) = v + sigma * net ~ A^-1 v on fields supported
+    where q is, which holds for every Krylov vector of A from b = -k^2 q u_inc
+    (A - I = diag(k^2 q) V maps into that support). Measured at depth 5, kappa 40
+    (grid 64, 16 held-out systems): the training loss falls to 5.5e-3 (0.11
+    with make_pairs), yet flexible GMRES takes 40 (512 pairs) to 47 (4096
+    pairs) steps against 30 plain, where make_pairs' network stays at 30."""
+    xs, ts = [], []
+    for s0 in range(0, len(seeds), 64):
+        chunk = list(seeds[s0:s0 + 64])
+        qk2s, Y = [], []
+        for sd in chunk:
+            _, nb = sample_params(sd)
+            qk2 = contrast_qk2(S, kappa, sd, nb)
+            qk2s.append(qk2)
+            Y.append(qk2 / np.max(qk2) * random_field(S, kappa, 70_000 + sd, n_waves))
+        S.set_contrast(np.stack(qk2s, axis=1))
+        V = S.matvec_block(np.stack(Y, axis=1))
+        for j in range(len(chunk)):
+            v, y = V[:, j], Y[j]
+            sigma = np.sqrt(np.mean(np.abs(v) ** 2))
+            gv = S.to_grid(v / sigma)
+            gt = S.to_grid((y - v) / sigma)
+            gq = S.to_grid(qk2s[j].astype(complex) / kappa ** 2).real
+            xs.append(np.stack([gq, gv.real, gv.imag], axis=-1).astype(np.float32))
+            ts.append(np.stack([gt.real, gt.imag], axis=-1).astype(np.float32))
+    return np.stack(xs), np.stack(ts)
+
+
 def evaluate(S, kappa, seeds, tol=1e-8, restart_len=50):
     """Per held-out system: plain GMRES vs learned initial guess + flexible GMRES;
     returns a dict of iteration counts, times and FP64 true residuals."""

@@ -194,8 +194,28 @@ def test_bump_contrast_plan_matches_oracle():
     assert H.relerr(y, o.matvec(x)) < 1e-10
 
 
+# the reference's other contrast functions (GMRES/GMRES.cpp:10-98, Qchoice 1-5):
+# (Qchoice, treeAdaptivity, kappa) giving 576-4680 unknowns, leaves on 1-3 levels
+CONTRAST_CASES = [(1, 2, 20.0), (2, 5, 40.0), (3, 5, 40.0), (4, 2, 20.0), (5, 3, 20.0)]
+
+
+@pytest.mark.parametrize("q,tadapt,kappa", CONTRAST_CASES)
+def test_reference_contrasts_plan_matches_oracle(q, tadapt, kappa):
+    inp = L.InputsToDFMM(**dict(README, Qchoice=q, treeAdaptivity=tadapt, kappa=kappa))
+    h = L.HostBuild(inp)
+    o = oracle.Oracle(inp.to_c())
+    assert np.array_equal(h.plan()["qk2"], o.contrast_k2())
+    assert np.array_equal(h.skeleton(), o.skeleton())
+    x = H.rand_vec(o.N, 11)
+    y = apply_plan(h.plan(), x, h.info()["n_state"], find_phi=True)
+    assert H.relerr(y, o.matvec(x)) < 1e-10
+
+
 def test_adaptive_inputs_validated():
     with pytest.raises(L.LsgError):
         L.HostBuild(L.InputsToDFMM(**dict(README, treeAdaptivity=0)), tree_only=True)
     with pytest.raises(L.LsgError):
         L.HostBuild(L.InputsToDFMM(**dict(README, tree_mode=2)), tree_only=True)
+    # the flat bump at kappa 20 is resolved by the level-1 boxes: no tree for the NCA
+    with pytest.raises(L.LsgError):
+        L.HostBuild(L.InputsToDFMM(**dict(README, Qchoice=3, treeAdaptivity=4, kappa=20.0)), tree_only=True)
