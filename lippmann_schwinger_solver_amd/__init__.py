@@ -234,8 +234,8 @@ class InputsToDFMM:
 
 def write_precomputations(inputs, directory):
     """Write the table cache of `inputs` under <directory>/precomputations in the
-    reference's layout (M tables of levels 0..nLevelsUniform, neighbour tables of
-    levels 2..10)."""
+    reference's layout (M tables of levels 0..nLevels of the tree the inputs
+    build, neighbour tables of levels 2..10)."""
     lib = load_library()
     c = inputs.to_c()
     rc = lib.lsg_write_precomputations(C.byref(c), str(directory).encode())

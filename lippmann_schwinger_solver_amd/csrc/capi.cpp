@@ -914,7 +914,18 @@ int lsg_write_precomputations(const lsg_inputs* in, const char* dir) {
   }
   BasisSet bs;
   bs.init(in->nChebNodes, in->degreeOfBases);
-  e = write_precomputations(dir, bs, in->kappa, in->L, in->nLevelsUniform, 10);
+  // the far-field tables cover the tree's levels (writeMToFile runs after the
+  // tree is built, G_FMM2DTree.hpp:1252-1269): an adaptive tree is grown first
+  int n_levels = in->nLevelsUniform;
+  if (in->tree_mode == 1) {
+    HostBuild hb;
+    if (host_build(in, hb, e, true) != LSG_OK) {
+      g_create_error = e;
+      return LSG_ERR_SETUP;
+    }
+    n_levels = hb.tree.nLevels;
+  }
+  e = write_precomputations(dir, bs, in->kappa, in->L, n_levels, 10);
   if (!e.empty()) {
     g_create_error = e;
     return LSG_ERR_SETUP;

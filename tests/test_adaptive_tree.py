@@ -211,6 +211,21 @@ def test_reference_contrasts_plan_matches_oracle(q, tadapt, kappa):
     assert H.relerr(y, o.matvec(x)) < 1e-10
 
 
+def test_table_cache_round_trip_on_adaptive_tree(tmp_path):
+    """precomputations/ written for an adaptive input (M of every level of the
+    grown tree, neighbour tables of every slot kind) and read back: the same
+    plan and skeletons as computing the tables."""
+    inp = L.InputsToDFMM(**dict(README, Qchoice=2, treeAdaptivity=5, kappa=40.0))
+    L.write_precomputations(inp, tmp_path)
+    h = L.HostBuild(inp)
+    inp2 = L.InputsToDFMM(**dict(README, Qchoice=2, treeAdaptivity=5, kappa=40.0))
+    inp2.precomputations_dir = str(tmp_path)
+    h2 = L.HostBuild(inp2)
+    assert np.array_equal(h2.skeleton(), h.skeleton())
+    p, p2 = h.plan(), h2.plan()
+    assert np.array_equal(p2["ops"], p["ops"]) and np.array_equal(p2["src"], p["src"])
+
+
 def test_adaptive_inputs_validated():
     with pytest.raises(L.LsgError):
         L.HostBuild(L.InputsToDFMM(**dict(README, treeAdaptivity=0)), tree_only=True)

@@ -134,6 +134,24 @@ def test_fp32_block_and_graph_paths(readme):
     assert all(np.array_equal(v, y) for v in yg)
 
 
+def test_gpu_operator_build_on_adaptive_tree(readme):
+    """setup_device = 1 (entries with per-leaf-level far tables and the ACA on
+    the GPU) on the README run: the operator stays within the pivot sensitivity
+    of the algorithm from the host build, and GMRES(1,10) lands on the README's
+    residual."""
+    S, o = readme
+    inp = L.InputsToDFMM(**dict(README, setup_device=1))
+    Sd = make(inp)
+    x = H.rand_vec(S.N, 6)
+    diff = H.relerr(Sd.MatVecProduct(x), S.MatVecProduct(x))
+    b = Sd.rhs()
+    r = H.relerr(o.matvec(L.RestartedGMRES(1, Sd, 10, inp, b)), b)
+    print(f"device-built vs host-built operator {diff:.2e}; GMRES(1,10) residual {r:.5e}", flush=True)
+    assert diff < 1e-4
+    assert 4.59412e-06 / 2 <= r <= 2 * 4.59412e-06
+    Sd.close()
+
+
 def test_uniform_only_features_rejected(readme):
     S, _ = readme
     with pytest.raises(L.LsgError):
