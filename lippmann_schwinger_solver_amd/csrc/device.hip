@@ -764,7 +764,6 @@ std::string DeviceOperator::upload(const Plan& plan) {
   HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev_leaf, hipEventDisableTiming));
-  HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
   HIP_OK(hipMalloc((void**)&state, size_t(n_state) * sizeof(double2)));
   HIP_OK(hipMemset(state, 0, size_t(n_state) * sizeof(double2)));
   static_assert(sizeof(cplx) == sizeof(double2), "layout");

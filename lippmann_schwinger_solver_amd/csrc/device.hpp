@@ -50,7 +50,6 @@ struct DeviceOperator {
   hipStream_t aux = nullptr;      // second stream, same default priority
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_leaf = nullptr;
   int device = 0;
-  int n_cu = 256;                 // compute units of the device
   int max_lds = 65536;            // LDS bytes per workgroup on this device
   int N = 0, rank = 0, n_leaves = 0;
   bool find_phi = true;
