@@ -102,8 +102,12 @@ def test_reference_gmres_readme_residuals(readme):
     for restart, printed in ((1, 4.59412e-06), (2, 1.28443e-10)):
         x = L.RestartedGMRES(restart, S, 10, S.inputs, b)
         r = H.relerr(o.matvec(x), b)
-        print(f"GMRES({restart},10): true residual {r:.5e} (README {printed:.5e})", flush=True)
+        xo, _ = o.gmres(b, restart, 10)
+        dx = H.relerr(x, xo)
+        print(f"GMRES({restart},10): true residual {r:.5e} (README {printed:.5e}); Phi vs oracle Phi {dx:.2e}",
+              flush=True)
         assert printed / 2 <= r <= 2 * printed
+        assert dx < 100 * printed        # the two iterates differ far less than their residuals
 
 
 def test_tolerance_gmres_fp64_verified(readme):

@@ -45,6 +45,21 @@ def test_tolerance_mode_true_residual(c1, tol):
     assert H.relerr(o.matvec(x), b) <= 2 * tol
 
 
+def test_solution_agrees_with_oracle_solution(c1):
+    """Phi of the device GMRES at tol 1e-10 against the oracle's restarted MGS
+    GMRES run past it (4 cycles of 50): the solutions differ by the tolerance
+    times the conditioning of A, far below the 1e-8 bound asserted."""
+    S, o = c1
+    b = S.rhs()
+    xg, st = S.gmres(b, restart_len=50, max_restarts=10, tol=1e-10)
+    xo, _ = o.gmres(b, 4, 50)
+    ro = H.relerr(o.matvec(xo), b)
+    diff = H.relerr(xg, xo)
+    print(f"oracle residual {ro:.2e}; |Phi_gpu - Phi_oracle| / |Phi_oracle| = {diff:.2e}", flush=True)
+    assert st["converged"] == 1 and ro < 1e-10
+    assert diff < 1e-8
+
+
 def test_gmres_failure_modes(c1):
     S, _ = c1
     with pytest.raises(L.LsgError) as e:
