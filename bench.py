@@ -202,6 +202,15 @@ def pmc_probe(S):
     log(f"pmc probe: copy {nb >> 20} MiB, matvec {ms * 1e3:.1f} us")
 
 
+def pmc_probe_block(S):
+    """3 block matvecs (64 right-hand sides, FP64 MFMA) for counter runs, after
+    one warm-up; the algorithmic flops of one are printed for the ratio."""
+    ms = S.time_matvec_block(reps=3)
+    info = S.info()
+    flops = 8.0 * 64 * (info["E_far"] + info["E_near"])
+    log(f"pmc block probe: block matvec {ms:.3f} ms, {flops:.4e} algorithmic flops each")
+
+
 def setup_record(wall_s, info, inp):
     """Operator build time and its phases as the library reports them: the
     host build (OpenMP; the operator every timed solve uses) and, beside it,
@@ -477,6 +486,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--pmc-probe-block", action="store_true",
+                    help="only run block (64-column, MFMA) matvecs, for rocprofv3 --pmc")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
     ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged", "setup", "readme"], default="c2",
@@ -510,6 +521,10 @@ def main():
     n_changed, report = S.autotune(reps=3)
     log(f"autotune ({n_changed} changed):\n{report}")
     b = S.rhs()
+    if args.pmc_probe_block:
+        pmc_probe_block(S)
+        S.close()
+        return
     if args.pmc_probe:
         pmc_probe(S)
         S.close()
