@@ -693,7 +693,7 @@ std::string make_entry_kernel(const Tree& t, const BasisSet& bs, const Problem& 
 // writeNeighborInteractionsForLevel geometry, :1812-1857); each table in use
 // is computed once (or read from the table cache).
 static std::string adaptive_near_pass(const Tree& t, const BasisSet& bs, const Problem& prob, const BuildOptions& opt,
-                                      const std::vector<double>& qk2, Plan& plan) {
+                                      const std::vector<double>& qk2, DeviceNca* dev, Plan& plan) {
   const int rk = bs.rank, nL = int(t.leaves.size());
   // kinds in the table cache's numbering: 0 colleague, 1 fine, 2 separated fine, 3 coarse
   static const int kSlots[4] = {9, 12, 20, 12};
@@ -734,13 +734,28 @@ static std::string adaptive_near_pass(const Tree& t, const BasisSet& bs, const P
   need.erase(std::unique(need.begin(), need.end()), need.end());
   std::vector<std::vector<cplx>> tabs(need.size());
   std::vector<std::string> errs(need.size());
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int q = 0; q < int(need.size()); ++q) {
+  std::vector<NearGeom> geom(need.size());
+  for (size_t q = 0; q < need.size(); ++q) {
     const int l = need[q] / 128, kind = (need[q] / 32) % 4, s = need[q] % 32;
     const double rA = t.boxRadius[l], rB = t.boxRadius[l + kDl[kind]];
     const double* c = centres(kind, s);
-    if (opt.table_dir.empty()) tabs[q] = near_table(bs, prob.kappa, rA, rB, c[0] * rA, c[1] * rA);
-    else errs[q] = read_neighbor_table(opt.table_dir, bs, prob.kappa, l, kind, s, tabs[q]);
+    geom[q] = NearGeom{rA, rB, c[0] * rA, c[1] * rA};
+  }
+  if (dev && opt.table_dir.empty()) {
+    // every table's integrals in one device launch, assembled and rounded here
+    std::vector<std::vector<cplx>> V1;
+    std::string e = dev->near_integrals(bs, prob.kappa, geom, V1);
+    if (!e.empty()) return "device tables: " + e;
+    for (size_t q = 0; q < need.size(); ++q) tabs[q] = near_table_from_integrals(bs, geom[q].rB, V1[q]);
+  } else {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < int(need.size()); ++q) {
+      const int l = need[q] / 128, kind = (need[q] / 32) % 4, s = need[q] % 32;
+      if (opt.table_dir.empty())
+        tabs[q] = near_table(bs, prob.kappa, geom[q].rA, geom[q].rB, geom[q].bx, geom[q].by);
+      else
+        errs[q] = read_neighbor_table(opt.table_dir, bs, prob.kappa, l, kind, s, tabs[q]);
+    }
   }
   for (const std::string& e : errs)
     if (!e.empty()) return "tables: " + e;
@@ -1031,7 +1046,7 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   phase[2] = since(tp);
   tp = clk::now();
   if (t.p.tree_mode == 1) {
-    std::string e = adaptive_near_pass(t, bs, prob, opt, K.qk2, plan);
+    std::string e = adaptive_near_pass(t, bs, prob, opt, K.qk2, B.dev, plan);
     if (!e.empty()) return e;
     phase[3] = since(tp);
     for (int i = 0; i < 4; ++i) plan.phase_seconds[i] = phase[i];
@@ -1042,9 +1057,16 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   const double rA = t.boxRadius[lvl];
   if (opt.table_dir.empty()) {
     plan.T.resize(size_t(9) * bs.rank * bs.rank);
+    std::vector<NearGeom> geom(9);
+    for (int s = 0; s < 9; ++s) geom[s] = NearGeom{rA, rA, kColleagueCenters[s][0] * rA, kColleagueCenters[s][1] * rA};
+    std::vector<std::vector<cplx>> V1;
+    if (B.dev) {
+      std::string e = B.dev->near_integrals(bs, prob.kappa, geom, V1);
+      if (!e.empty()) return "device tables: " + e;
+    }
     for (int s = 0; s < 9; ++s) {
-      std::vector<cplx> Ts =
-          near_table(bs, prob.kappa, rA, rA, kColleagueCenters[s][0] * rA, kColleagueCenters[s][1] * rA);
+      std::vector<cplx> Ts = B.dev ? near_table_from_integrals(bs, rA, V1[s])
+                                   : near_table(bs, prob.kappa, rA, rA, geom[s].bx, geom[s].by);
       std::copy(Ts.begin(), Ts.end(), plan.T.begin() + size_t(s) * bs.rank * bs.rank);
     }
   } else {

@@ -112,6 +112,10 @@ class DeviceNca {
   virtual ~DeviceNca() {}
   virtual std::string aca(const std::vector<AcaJob>& jobs, double eps) = 0;
   virtual std::string blocks(const std::vector<BlockJob>& jobs) = 0;
+  // The integrals of near_integrals for every table in `tabs` (each rank x nb,
+  // row-major), by the same adaptive quadrature on the GPU.
+  virtual std::string near_integrals(const BasisSet& bs, double kappa, const std::vector<NearGeom>& tabs,
+                                     std::vector<std::vector<cplx>>& V1) = 0;
 };
 typedef DeviceNca* (*DeviceNcaFactory)(const EntryKernel& K, std::string& err);
 

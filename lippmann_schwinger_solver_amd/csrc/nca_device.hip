@@ -1,13 +1,15 @@
 // NCA skeletons and far-field operator entries on the GPU (gfx950).
 //
-// Three pieces, all driven by the host build in nca.cpp:
+// Four pieces, all driven by the host build in nca.cpp:
 //   - the entry kernel: the Graf series of (i/4) H0 against the projected
 //     source basis, with the Bessel recurrences of bessel.hpp evaluated on the
 //     device (GMRES/G_FMM2DTree.hpp:603-658, getMatrixEntry);
 //   - a batched partially pivoted ACA, one workgroup per skeleton, with the
 //     pivot order and stopping rules of GMRES/ACA.hpp:253-481 (ACA_only_nodes);
 //   - batched entry blocks K(check points, charges) for the fused M2L and L2L
-//     operators (G_FMM2DTree.hpp:2502-2575, 3829-3880).
+//     operators (G_FMM2DTree.hpp:2502-2575, 3829-3880);
+//   - the adaptive quadrature of the near-field tables' integrals
+//     (NearFieldInteraction1.hpp:98-128 over GMRES/domain2D.hpp:46-138).
 // The arithmetic follows the host code operation by operation (this file is
 // compiled without multiply-add contraction); the device's sin, cos, atan2,
 // log and hypot may differ from the host's in the last bit, so a pivot can
@@ -482,6 +484,127 @@ __global__ __launch_bounds__(kBT) void block_kernel(DevEntry K, const BlockDesc*
   }
 }
 
+// ------------------------------------------- near-field table quadrature ---
+// V1(i, l) = int_[-1,1]^2 H0(kappa rB |X_i - P|) T_a(P_x) T_b(P_y) dP by the
+// adaptive tensor Gauss-Legendre-10 rule of adaptive_quad (tables.cpp, after
+// GMRES/domain2D.hpp:46-138): a panel is accepted when its own value differs
+// from the sum of its four children's by less than eps, accepted panels are
+// summed in depth-first order (children SW, SE, NE, NW). One wavefront per
+// integral: the 400 points of a refinement are spread over the lanes and
+// summed by shuffles; the depth-first panel stack lives in LDS (every lane
+// writes the same, wave-uniform values). A panel is also accepted at depth
+// kQMaxDepth or after kQMaxSteps refinements, so every wavefront finishes;
+// such a cap is reported and fails the build.
+constexpr int kQT = 256;                 // threads per quadrature workgroup (4 integrals)
+constexpr int kQMaxDepth = 50;
+constexpr int kQStack = 3 * kQMaxDepth + 4;
+constexpr int kQMaxSteps = 20000;       // a log-singular integral needs a few hundred
+
+struct QuadJob {
+  double Xx, Xy, krb;  // target node in source-box units, kappa rB
+  int a, b;            // T_a(P_x) T_b(P_y)
+};
+
+__device__ __forceinline__ double q_cheb(double x, int n) { return cos(n * acos(x)); }
+
+__device__ __forceinline__ void q_f(const QuadJob& q, double px, double py, double& re, double& im) {
+  const double R = sqrt((q.Xx - px) * (q.Xx - px) + (q.Xy - py) * (q.Xy - py));
+  double J[2], Y[2];
+  bessel_jy(q.krb * R, 0, J, Y);
+  const double ta = q_cheb(px, q.a), tb = q_cheb(py, q.b);
+  re = J[0] * ta * tb;
+  im = Y[0] * ta * tb;
+}
+
+__device__ __forceinline__ double q_wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kQT) void quad_kernel(const QuadJob* __restrict__ jobs, int njobs,
+                                                   const double* __restrict__ glx, const double* __restrict__ glw,
+                                                   double eps, double2* __restrict__ out, int* __restrict__ capped) {
+  __shared__ double stk[kQT / 64][kQStack][6];  // cx, cy, half width, re, im, depth
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jid = blockIdx.x * (kQT / 64) + wv;
+  if (jid >= njobs) return;   // whole wavefront: no workgroup barrier follows
+  const QuadJob q = jobs[jid];
+  double (*st)[6] = stk[wv];
+  // the root panel [-1,1]^2
+  double rr = 0.0, ri = 0.0;
+  for (int p = lane; p < 100; p += 64) {
+    const int m = p / 10, n = p % 10;
+    double fr, fi;
+    q_f(q, glx[m], glx[n], fr, fi);
+    const double w = glw[m] * glw[n];
+    rr += w * fr;
+    ri += w * fi;
+  }
+  rr = q_wsum(rr);
+  ri = q_wsum(ri);
+  int sp = 0;
+  st[0][0] = 0.0; st[0][1] = 0.0; st[0][2] = 1.0; st[0][3] = rr; st[0][4] = ri; st[0][5] = 0.0;
+  sp = 1;
+  double ar = 0.0, ai = 0.0;
+  int steps = 0, cap = 0;
+  while (sp > 0) {
+    --sp;
+    const double cx = st[sp][0], cy = st[sp][1], h = st[sp][2], pr = st[sp][3], pi = st[sp][4];
+    const int d = int(st[sp][5]);
+    if (d >= kQMaxDepth || steps >= kQMaxSteps) {
+      ar += pr;
+      ai += pi;
+      cap = 1;
+      continue;
+    }
+    ++steps;
+    const double r = 0.5 * h;
+    double cr[4] = {0.0, 0.0, 0.0, 0.0}, ci[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double ccx = (c == 1 || c == 2) ? cx + r : cx - r;
+      const double ccy = (c >= 2) ? cy + r : cy - r;
+      double sr = 0.0, si = 0.0;
+      for (int p = lane; p < 100; p += 64) {
+        const int m = p / 10, n = p % 10;
+        double fr, fi;
+        q_f(q, ccx + r * glx[m], ccy + r * glx[n], fr, fi);
+        const double w = glw[m] * glw[n];
+        sr += w * fr;
+        si += w * fi;
+      }
+      cr[c] = q_wsum(sr) * (r * r);
+      ci[c] = q_wsum(si) * (r * r);
+    }
+    double er = pr, ei = pi;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      er -= cr[c];
+      ei -= ci[c];
+    }
+    if (hypot(er, ei) < eps) {
+      ar += pr;
+      ai += pi;
+    } else {
+      // children pushed NW, NE, SE, SW so that SW is refined first
+#pragma unroll
+      for (int c = 3; c >= 0; --c) {
+        st[sp][0] = (c == 1 || c == 2) ? cx + r : cx - r;
+        st[sp][1] = (c >= 2) ? cy + r : cy - r;
+        st[sp][2] = r;
+        st[sp][3] = cr[c];
+        st[sp][4] = ci[c];
+        st[sp][5] = double(d + 1);
+        ++sp;
+      }
+    }
+  }
+  if (lane == 0) {
+    out[jid] = make_double2(ar, ai);
+    capped[jid] = cap;
+  }
+}
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -563,6 +686,51 @@ class HipNca : public DeviceNca {
       todo.swap(again);
     }
     if (!todo.empty()) return "ACA rank capacity exceeded";
+    return "";
+  }
+
+  std::string near_integrals(const BasisSet& bs, double kappa, const std::vector<NearGeom>& tabs,
+                             std::vector<std::vector<cplx>>& V1) override {
+    V1.assign(tabs.size(), std::vector<cplx>());
+    if (tabs.empty()) return "";
+    std::vector<QuadJob> jobs;
+    jobs.reserve(tabs.size() * size_t(bs.rank) * bs.nb);
+    for (const NearGeom& g : tabs)
+      for (int i = 0; i < bs.rank; ++i)
+        for (int l = 0; l < bs.nb; ++l) {
+          QuadJob q;
+          q.Xx = (g.rA * bs.sx[i] + 0.0 - g.bx) / g.rB;
+          q.Xy = (g.rA * bs.sy[i] + 0.0 - g.by) / g.rB;
+          q.krb = kappa * g.rB;
+          q.a = bs.ab[l][0];
+          q.b = bs.ab[l][1];
+          jobs.push_back(q);
+        }
+    std::vector<double> gx(kQuadNodes), gw(kQuadNodes);
+    gauss_legendre(kQuadNodes, gx.data(), gw.data());
+    DevBuf<QuadJob> dj;
+    DevBuf<double> dx, dw;
+    DevBuf<double2> dout;
+    DevBuf<int> dcap;
+    std::string e;
+    if (!(e = dj.put(jobs)).empty() || !(e = dx.put(gx)).empty() || !(e = dw.put(gw)).empty() ||
+        !(e = dout.reserve(jobs.size())).empty() || !(e = dcap.reserve(jobs.size())).empty())
+      return e;
+    const int n = int(jobs.size());
+    hipLaunchKernelGGL(quad_kernel, dim3(unsigned((n + kQT / 64 - 1) / (kQT / 64))), dim3(kQT), 0, 0, dj.p, n, dx.p,
+                       dw.p, kQuadEps, dout.p, dcap.p);
+    NCA_HIP(hipGetLastError());
+    std::vector<double2> h(jobs.size());
+    std::vector<int> cap(jobs.size());
+    NCA_HIP(hipMemcpy(h.data(), dout.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    NCA_HIP(hipMemcpy(cap.data(), dcap.p, cap.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int c : cap)
+      if (c) return "table quadrature reached its depth or step limit";
+    const size_t per = size_t(bs.rank) * bs.nb;
+    for (size_t t = 0; t < tabs.size(); ++t) {
+      V1[t].resize(per);
+      for (size_t k = 0; k < per; ++k) V1[t][k] = cplx(h[t * per + k].x, h[t * per + k].y);
+    }
     return "";
   }
 

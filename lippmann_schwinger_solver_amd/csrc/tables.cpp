@@ -121,7 +121,7 @@ std::vector<cplx> far_table(const BasisSet& bs, double kappa, double radius) {
 // T = (i rB^2 / 4) V1 Q_pinv, V1(i,l) = int H0(kappa rB |X_i - P|) T_a T_b dP with
 // X_i = (node_i of A - centre_B)/rB.  NearFieldInteraction1.hpp:98-128,
 // G_FMM2DTree.hpp:1773-1805.
-std::vector<cplx> near_table(const BasisSet& bs, double kappa, double rA, double rB, double bx, double by) {
+std::vector<cplx> near_integrals(const BasisSet& bs, double kappa, double rA, double rB, double bx, double by) {
   std::vector<cplx> V(size_t(bs.rank) * bs.nb);
 #pragma omp parallel for schedule(dynamic)
   for (int t = 0; t < bs.rank * bs.nb; ++t) {
@@ -138,6 +138,10 @@ std::vector<cplx> near_table(const BasisSet& bs, double kappa, double rA, double
     };
     V[t] = adaptive_quad(f, kQuadEps);
   }
+  return V;
+}
+
+std::vector<cplx> near_table_from_integrals(const BasisSet& bs, double rB, const std::vector<cplx>& V) {
   std::vector<cplx> T(size_t(bs.rank) * bs.rank);
   for (int i = 0; i < bs.rank; ++i)
     for (int j = 0; j < bs.rank; ++j) {
@@ -149,6 +153,10 @@ std::vector<cplx> near_table(const BasisSet& bs, double kappa, double rA, double
       T[size_t(i) * bs.rank + j] = round6(s);
     }
   return T;
+}
+
+std::vector<cplx> near_table(const BasisSet& bs, double kappa, double rA, double rB, double bx, double by) {
+  return near_table_from_integrals(bs, rB, near_integrals(bs, kappa, rA, rB, bx, by));
 }
 
 }  // namespace lsg

@@ -54,5 +54,15 @@ std::vector<cplx> far_table(const BasisSet& bs, double kappa, double radius);
 // source box B of radius rB centred at (bx,by) to the nodes of a target box of
 // radius rA centred at the origin.
 std::vector<cplx> near_table(const BasisSet& bs, double kappa, double rA, double rB, double bx, double by);
+// Its two halves: the integrals V1(i, l) = int H0(kappa rB |X_i - P|) T_a T_b dP
+// (rank x nb, row-major, by adaptive_quad), and T = round6((i rB^2/4) V1 Q_pinv).
+std::vector<cplx> near_integrals(const BasisSet& bs, double kappa, double rA, double rB, double bx, double by);
+std::vector<cplx> near_table_from_integrals(const BasisSet& bs, double rB, const std::vector<cplx>& V1);
+
+// Geometry of one near-field table: target box radius rA at the origin, source
+// box radius rB centred at (bx, by).
+struct NearGeom {
+  double rA = 0, rB = 0, bx = 0, by = 0;
+};
 
 }  // namespace lsg

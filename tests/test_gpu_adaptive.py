@@ -139,10 +139,10 @@ def test_fp32_block_and_graph_paths(readme):
 
 
 def test_gpu_operator_build_on_adaptive_tree(readme):
-    """setup_device = 1 (entries with per-leaf-level far tables and the ACA on
-    the GPU) on the README run: the operator stays within the pivot sensitivity
-    of the algorithm from the host build, and GMRES(1,10) lands on the README's
-    residual."""
+    """setup_device = 1 (entries with per-leaf-level far tables, the ACA and the
+    quadrature of every neighbour table on the GPU) on the README run: the
+    operator stays within the pivot sensitivity of the algorithm from the host
+    build, and GMRES(1,10) lands on the README's residual."""
     S, o = readme
     inp = L.InputsToDFMM(**dict(README, setup_device=1))
     Sd = make(inp)
