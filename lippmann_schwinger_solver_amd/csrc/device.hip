@@ -743,6 +743,8 @@ std::string upload_vec(const std::vector<T>& h, T** d) {
 
 }  // namespace
 
+static std::vector<double> near_fragments(const std::vector<cplx>& T, int rk);
+
 std::string DeviceOperator::upload(const Plan& plan) {
   N = plan.N;
   rank = plan.rank;
@@ -847,6 +849,10 @@ std::string DeviceOperator::upload(const Plan& plan) {
         Tc[size_t(s) * rank * rank + size_t(c) * rank + i] = make_double2(v.real(), v.imag());
       }
   if (!(e = upload_vec(Tc, &T)).empty()) return e;
+  if (!plan.T.empty() && (rank == 16 || rank == 25 || rank == 36)) {
+    if (!(e = upload_vec(near_fragments(plan.T, rank), &Tm)).empty()) return e;
+    near_mfma = true;
+  }
   if (!(e = upload_vec(plan.nbr, &nbr)).empty()) return e;
   if (!(e = upload_vec(plan.qk2, &qk2)).empty()) return e;
   near_bytes = plan.T.empty() ? 0 : 16 * int64_t(9) * rank * rank + 16 * 10 * int64_t(N) + 8 * int64_t(N);
@@ -861,7 +867,8 @@ void DeviceOperator::release() {
   drop_graph();
   if (xy_dev) hipFree(xy_dev);
   xy_dev = nullptr;
-  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user};
+  void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user,
+                  Tm};
   for (void* p : ptrs)
     if (p) hipFree(p);
   state = ops = nullptr;
@@ -871,6 +878,8 @@ void DeviceOperator::release() {
   qk2 = nullptr;
   ops32 = T32 = state32 = nullptr;
   qk2_user = nullptr;
+  Tm = nullptr;
+  near_mfma = false;
   if (ev_fork) hipEventDestroy(ev_fork);
   if (ev_join) hipEventDestroy(ev_join);
   if (ev_leaf) hipEventDestroy(ev_leaf);
@@ -943,11 +952,144 @@ static void launch_near_nl(const DeviceOperator& o, const double2* x, double2* y
   }
 }
 
+static bool launch_near_mfma(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                             const double2* const* xyp);
+
 template <class R>
 static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
                         const double2* const* xyp = nullptr) {
+  if (std::is_same<R, double>::value && o.near_mfma && o.Tm && launch_near_mfma(o, x, y, s, xyp)) return;
   if (o.near_leaves_per_thread == 1) launch_near_nl<R, 1>(o, x, y, s, xyp);
   else launch_near_nl<R, 2>(o, x, y, s, xyp);
+}
+
+// Near field of a uniform tree on FP64 MFMA (16 x 16 x 4): per slot s the
+// product T_s [x_nbr(l,s)] over all leaves l is a GEMM with the leaves as
+// columns. Complex arithmetic is real-ified: real row 2r + a (a = 0 Re, 1 Im)
+// of table row r, real column 2c + b of table column c, entries
+// [[Re T, -Im T], [Im T, Re T]]. A workgroup owns 16 leaves (the MFMA
+// columns); wave w owns real rows [16w, 16w + 16). Lane l holds the operand
+// fragments A(row l%16, k l/16), read from Tm (pre-arranged per slot, row
+// tile, k-step and lane: one coalesced load per k-step), and B(k l/16, leaf
+// l%16), read from the slot's neighbour vectors staged in LDS; the result
+// rows l/16 + 4v of leaf l%16. The next slot's neighbour vectors and
+// fragments are loaded into registers while the current slot's products run.
+template <int RK>
+__global__ __launch_bounds__(64 * ((2 * RK + 15) / 16)) void near_mfma_kernel(
+    const double* __restrict__ Tm, const int* __restrict__ nbr, const double* __restrict__ qk2,
+    const double2* __restrict__ x_arg, double2* __restrict__ y_arg, int n_leaves, int find_phi,
+    const double2* const* __restrict__ xyp) {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  constexpr int MT = (2 * RK + 15) / 16;  // real row tiles = waves
+  constexpr int KS = (2 * RK + 3) / 4;    // k-steps of 4 real columns
+  constexpr int NT = 64 * MT;
+  constexpr int NE = 16 * 2 * RK;         // reals of one slot's 16 neighbour vectors
+  constexpr int NG = (NE + NT - 1) / NT;  // of them per thread
+  constexpr int LS = 2 * RK + 2;          // LDS row stride (reals) of one leaf
+  const double* __restrict__ x = reinterpret_cast<const double*>(xyp ? xyp[0] : x_arg);
+  double* __restrict__ y = reinterpret_cast<double*>(xyp ? const_cast<double2*>(xyp[1]) : y_arg);
+  __shared__ double Xs[2][16 * LS];
+  __shared__ int nbs[16 * 9];
+  const int t = threadIdx.x, lane = t & 63, mt = t >> 6;
+  const int j = lane & 15, k4 = lane >> 4;
+  const int leaf0 = blockIdx.x * 16;
+  for (int e = t; e < 16 * 9; e += NT) nbs[e] = leaf0 + e / 9 < n_leaves ? nbr[leaf0 * 9 + e] : -1;
+  __syncthreads();
+  double g[NG], a_cur[KS], a_nxt[KS];
+  auto gather = [&](int s) {
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      const int e = t + q * NT;
+      const int l = e / (2 * RK), d = e - l * (2 * RK);
+      const int nb = e < NE ? nbs[l * 9 + s] : -1;
+      g[q] = nb >= 0 ? x[size_t(nb) * 2 * RK + d] : 0.0;
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      const int e = t + q * NT;
+      const int l = e / (2 * RK), d = e - l * (2 * RK);
+      if (e < NE) Xs[buf][l * LS + d] = g[q];
+    }
+  };
+  auto frags = [&](int s, double* a) {
+    const double* A = Tm + (size_t(s) * MT + mt) * KS * 64 + lane;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) a[ks] = A[ks * 64];
+  };
+  gather(0);
+  stage(0);
+  frags(0, a_cur);
+  acc_t acc = {0, 0, 0, 0};
+  for (int s = 0; s < 9; ++s) {
+    __syncthreads();  // slot s staged; the other buffer is no longer read
+    if (s + 1 < 9) {
+      gather(s + 1);
+      frags(s + 1, a_nxt);
+    }
+    const double* B = Xs[s & 1] + j * LS + k4;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const double bv = 4 * ks + k4 < 2 * RK ? B[4 * ks] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[ks], bv, acc, 0, 0, 0);
+    }
+    if (s + 1 < 9) {
+      stage((s + 1) & 1);
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) a_cur[ks] = a_nxt[ks];
+    }
+  }
+  const int leaf = leaf0 + j;
+  if (leaf >= n_leaves) return;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int ri = 16 * mt + k4 + 4 * v;
+    if (ri < 2 * RK) {
+      const size_t g2 = size_t(leaf) * RK + (ri >> 1);
+      double val = acc[v];
+      if (find_phi) val = x[2 * g2 + (ri & 1)] + qk2[g2] * val;
+      y[2 * g2 + (ri & 1)] = val;
+    }
+  }
+}
+
+template <int RK>
+static void launch_near_mfma_rk(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                                const double2* const* xyp) {
+  constexpr int MT = (2 * RK + 15) / 16;
+  hipLaunchKernelGGL((near_mfma_kernel<RK>), dim3((o.n_leaves + 15) / 16), dim3(64 * MT), 0, s, o.Tm, o.nbr, o.qk2,
+                     x, y, o.n_leaves, o.find_phi ? 1 : 0, xyp);
+}
+
+static bool launch_near_mfma(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
+                             const double2* const* xyp) {
+  switch (o.rank) {
+    case 16: launch_near_mfma_rk<16>(o, x, y, s, xyp); return true;
+    case 25: launch_near_mfma_rk<25>(o, x, y, s, xyp); return true;
+    case 36: launch_near_mfma_rk<36>(o, x, y, s, xyp); return true;
+    default: return false;
+  }
+}
+
+// Tm from the 9 tables (T: row-major rank x rank per slot): fragment value of
+// lane l at (slot s, row tile mt, k-step ks) is the real-ified entry at real
+// row 16 mt + l%16, real column 4 ks + l/16 (zero beyond 2 rank)
+static std::vector<double> near_fragments(const std::vector<cplx>& T, int rk) {
+  const int MT = (2 * rk + 15) / 16, KS = (2 * rk + 3) / 4;
+  std::vector<double> out(size_t(9) * MT * KS * 64, 0.0);
+  for (int s = 0; s < 9; ++s)
+    for (int mt = 0; mt < MT; ++mt)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int ri = 16 * mt + (l & 15), rc = 4 * ks + (l >> 4);
+          if (ri >= 2 * rk || rc >= 2 * rk) continue;
+          const int r = ri >> 1, a = ri & 1, c = rc >> 1, b = rc & 1;
+          const cplx v = T[size_t(s) * rk * rk + size_t(r) * rk + c];
+          const double e = a == b ? v.real() : (a == 1 ? v.imag() : -v.imag());
+          out[((size_t(s) * MT + mt) * KS + ks) * 64 + l] = e;
+        }
+  return out;
 }
 
 // y = x + qk2 .* y: the Lippmann-Schwinger operator from the volume potential
@@ -1070,14 +1212,24 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   if (!T) {
     tuning.push_back("near_field: passes (adaptive tree)");
   } else {
+  const bool mfma_before = near_mfma;
+  near_mfma = false;
   near_leaves_per_thread = 1;
   const float t1 = time_it([&]() { launch_near<double>(*this, x, y, stream); });
   near_leaves_per_thread = 2;
   const float t2 = time_it([&]() { launch_near<double>(*this, x, y, stream); });
   near_leaves_per_thread = t1 < t2 ? 1 : 2;
-  nchg += near_leaves_per_thread != nl_before;
-  tuning.push_back("near_field: " + std::to_string(near_leaves_per_thread) + " leaves per thread (" +
-                   std::to_string(t1 * 1e3f) + " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2)");
+  float tm = 1e30f;
+  if (Tm) {
+    near_mfma = true;
+    tm = time_it([&]() { launch_near<double>(*this, x, y, stream); });
+    near_mfma = tm < std::min(t1, t2);
+  }
+  nchg += (near_leaves_per_thread != nl_before) + (near_mfma != mfma_before);
+  tuning.push_back("near_field: " + std::string(near_mfma ? "MFMA" : "FMA") + "; FMA kernel " +
+                   std::to_string(near_leaves_per_thread) + " leaves per thread (" + std::to_string(t1 * 1e3f) +
+                   " us with 1, " + std::to_string(t2 * 1e3f) + " us with 2, " +
+                   (Tm ? std::to_string(tm * 1e3f) + " us on MFMA)" : std::string("no MFMA variant)")));
   }
   hipEventDestroy(a);
   hipEventDestroy(b);

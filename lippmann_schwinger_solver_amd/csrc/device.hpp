@@ -74,7 +74,11 @@ struct DeviceOperator {
   float2* T32 = nullptr;
   float2* state32 = nullptr;
   GmresWork work;
-  int near_leaves_per_thread = 2;     // near-field variant (autotuned)
+  int near_leaves_per_thread = 2;     // near-field variant of the FMA kernel (autotuned)
+  // FP64 near field on MFMA (autotuned against the FMA kernel): the tables as
+  // real-ified 16 x 4 fragments per (slot, row tile, k-step, lane)
+  double* Tm = nullptr;
+  bool near_mfma = false;
   hipGraphExec_t graph_exec = nullptr;  // captured FP64 matvec (capture_graph)
   const double2** xy_dev = nullptr;     // its input/output addresses
   std::vector<std::string> tuning;      // autotuner decisions
