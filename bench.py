@@ -10,6 +10,7 @@ import argparse
 import copy
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -478,6 +479,25 @@ def bench_ragged(args):
     print(json.dumps(out), flush=True)
 
 
+def stream_ceiling(op_bytes=443880000):
+    """Practical streaming-read ceiling of the device: lsg's stream_probe (a
+    read-and-sum kernel over buffers of the leaf M2L operator's size and of
+    2 GiB, non-temporal and plain loads, grids of 1024-16384 workgroups), run
+    as a child process. Returns the best rates, or None if the probe is absent."""
+    exe = os.path.join(ROOT, "lippmann_schwinger_solver_amd", "stream_probe")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, str(op_bytes)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log("stream_probe failed: " + r.stderr.strip())
+        return None
+    cases = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    best = max(cases, key=lambda c: c["GBps"])
+    best_op = max((c for c in cases if c["bytes"] == op_bytes // 16 * 16), key=lambda c: c["GBps"])
+    return {"best_GBps": best["GBps"], "best_case": best, "best_GBps_at_leaf_m2l_size": best_op["GBps"],
+            "cases": cases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,6 +510,10 @@ def main():
                     help="only run block (64-column, MFMA) matvecs, for rocprofv3 --pmc")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run a 256 MiB device copy (counter calibration) and 3 matvecs, for rocprofv3 --pmc")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only run the streaming-read ceiling probe and print its cases")
+    ap.add_argument("--no-stream-ceiling", action="store_true",
+                    help="leave the streaming-read ceiling probe out of the default line (profiled runs)")
     ap.add_argument("--workload", choices=["c2", "c3", "learned", "ragged", "setup", "readme"], default="c2",
                     help="c2: the default line; c3: depth 7, kappa 160 mixed-precision and batched legs; "
                          "learned: train the network on 4096 generated solves and evaluate on 256 held-out; "
@@ -500,6 +524,13 @@ def main():
         print("bench.py runs on one GPU only: the solver is single-device (multi-GPU sharding is a design "
               "note, not implemented); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.stream_ceiling:
+        c = stream_ceiling()
+        if c is None:
+            sys.exit("stream_probe is not built (run __graft_entry__.build())")
+        for case in c["cases"]:
+            print(json.dumps(case), flush=True)
+        return
     if args.workload == "c3":
         return bench_c3(args)
     if args.workload == "setup":
@@ -511,6 +542,8 @@ def main():
     if args.workload == "readme":
         return bench_readme(args)
 
+    # the probe runs before this process touches the device
+    ceiling = None if (args.pmc_probe or args.pmc_probe_block or args.no_stream_ceiling) else stream_ceiling()
     inp = c2_inputs(0)
     t0 = time.time()
     S = lsg.DFMM(inp)
@@ -619,6 +652,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom[0])},
         "cpu_baseline": None,
+        "stream_read_ceiling": None if ceiling is None else {
+            "GBps": ceiling["best_GBps_at_leaf_m2l_size"], "best_any_size_GBps": ceiling["best_GBps"],
+            "frac_of_peak": ceiling["best_GBps_at_leaf_m2l_size"] / HBM_PEAK_GBS,
+            "dominant_kernel_frac_of_ceiling": achieved / ceiling["best_GBps_at_leaf_m2l_size"],
+            "probe": "read-and-sum kernel (lsg stream_probe), best over grids 1024-16384 and load kinds"},
         "fp32_ir": fp32_ir,
         "batched_64_angles": batched,
     }
