@@ -251,9 +251,10 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
                     double* rel_residuals, lsg_stats* stats);
 
 /* Autotuner (no reference counterpart): for every pass, time the one-wavefront
- * and one-workgroup-per-destination kernels, and 1 or 2 leaves per thread in
- * the near field, on this operator with HIP events (best of `reps`); keep the
- * faster. n_changed (may be NULL) receives the number of changed choices;
+ * and one-workgroup-per-destination kernels, and for the near field of a
+ * uniform tree the FP64 MFMA kernel and the FMA kernel with 1 or 2 leaves per
+ * thread, on this operator with HIP events (best of `reps`); keep the
+ * fastest. n_changed (may be NULL) receives the number of changed choices;
  * lsg_tuning() returns one line per decision. */
 int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed);
 const char* lsg_tuning(const lsg_ctx* ctx);
