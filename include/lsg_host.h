@@ -35,6 +35,15 @@ int lsg_host_build(const lsg_inputs* in, lsg_host** out);
  * operator queries then return empty results. */
 int lsg_host_build_tree(const lsg_inputs* in, lsg_host** out);
 void lsg_host_destroy(lsg_host* h);
+/* A device context from a host build: the build's tree, tables and fused NCA
+ * operators are uploaded as they are, with no second build. No reference
+ * counterpart: the reference builds and applies the operator in one object
+ * (DFMM::FMMTreeInitialisation, GMRES/GMRES.cpp:161-199). One host build can
+ * serve several contexts, on one device or on several (call with each device
+ * current); the host build stays valid. Errors as lsg_create
+ * (lsg_create_error): LSG_ERR_ARG for a NULL or tree-only build,
+ * LSG_ERR_DEVICE without a HIP device. */
+int lsg_create_from_host(const lsg_host* h, lsg_ctx** out);
 const char* lsg_host_error(void);
 int lsg_host_info(const lsg_host* h, lsg_info* info);
 /* Far-field table of the leaf level: (2*15+1) x nChebNodes^2, row-major,

@@ -132,6 +132,7 @@ SYMBOLS = [
     ("lsg_gmres_batch", C.c_int, [_P, _P, _P, C.c_int, C.c_int64, C.POINTER(lsg_solve_opts), _D,
                                   C.POINTER(lsg_stats)]),
     ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
+    ("lsg_create_from_host", C.c_int, [_P, C.POINTER(_P)]),
     ("lsg_host_build_tree", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
     ("lsg_host_destroy", None, [_P]),
     ("lsg_host_error", C.c_char_p, []),
@@ -384,10 +385,15 @@ class DFMM:
         self.N = 0
         self.gridPoints = None
 
-    def FMMTreeInitialisation(self):
-        c = self.inputs.to_c()
+    def FMMTreeInitialisation(self, host=None):
+        """Build the operator and upload it; with `host` (a HostBuild of the
+        same inputs), upload that build instead of building again."""
         ctx = C.c_void_p()
-        rc = self._lib.lsg_create(C.byref(c), C.byref(ctx))
+        if host is None:
+            c = self.inputs.to_c()
+            rc = self._lib.lsg_create(C.byref(c), C.byref(ctx))
+        else:
+            rc = self._lib.lsg_create_from_host(host._h, C.byref(ctx))
         if rc != 0:
             raise LsgError(rc, self._lib.lsg_create_error().decode())
         self._ctx = ctx

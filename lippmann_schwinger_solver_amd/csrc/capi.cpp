@@ -178,10 +178,9 @@ int lsg_bumps_from_seed(uint64_t seed, int n, double cmin, double cmax, double w
   return LSG_OK;
 }
 
-int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
-  g_create_error.clear();
-  if (!out) { g_create_error = "out is NULL"; return LSG_ERR_ARG; }
-  *out = nullptr;
+static int finish_create(std::unique_ptr<lsg_ctx> ctx, lsg_ctx** out);
+
+static int device_present() {
   int ndev = 0;
   hipError_t he = hipGetDeviceCount(&ndev);
   if (he != hipSuccess || ndev == 0) {
@@ -189,6 +188,14 @@ int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
                      "); this library has no CPU execution path";
     return LSG_ERR_DEVICE;
   }
+  return LSG_OK;
+}
+
+int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  if (int rc = device_present(); rc != LSG_OK) return rc;
   auto ctx = std::make_unique<lsg_ctx>();
   std::string e;
   int rc = host_build(in, ctx->hb, e);
@@ -196,7 +203,12 @@ int lsg_create(const lsg_inputs* in, lsg_ctx** out) {
     g_create_error = e;
     return rc;
   }
-  e = ctx->dev.upload(ctx->hb.plan);
+  return finish_create(std::move(ctx), out);
+}
+
+// upload a context's host build and allocate its matvec buffers
+static int finish_create(std::unique_ptr<lsg_ctx> ctx, lsg_ctx** out) {
+  std::string e = ctx->dev.upload(ctx->hb.plan);
   if (!e.empty()) {
     ctx->dev.release();
     g_create_error = e;
@@ -815,6 +827,24 @@ int lsg_host_build_tree(const lsg_inputs* in, lsg_host** out) {
   h->hb.plan.N = h->hb.tree.N;
   *out = h.release();
   return LSG_OK;
+}
+
+int lsg_create_from_host(const lsg_host* h, lsg_ctx** out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  if (!h) { g_create_error = "host build is NULL"; return LSG_ERR_ARG; }
+  if (h->hb.plan.passes.empty() && h->hb.plan.T.empty())
+    { g_create_error = "the host build has no operators (lsg_host_build_tree builds only the tree)"; return LSG_ERR_ARG; }
+  int64_t entries = 0;
+  for (const PassHost& ph : h->hb.plan.passes) entries += ph.entries;
+  if (int64_t(h->hb.plan.ops.size()) != entries)
+    { g_create_error = "the host build's operators are incomplete"; return LSG_ERR_ARG; }
+  if (int rc = device_present(); rc != LSG_OK) return rc;
+  auto ctx = std::make_unique<lsg_ctx>();
+  ctx->hb = h->hb;
+  std::vector<cplx>().swap(ctx->hb.far);
+  return finish_create(std::move(ctx), out);
 }
 
 void lsg_host_destroy(lsg_host* h) { delete h; }

@@ -69,6 +69,30 @@ def test_matvec_other_node_counts(n, depth, kappa, tol64, tol32):
     S.close()
 
 
+def test_context_from_host_build():
+    """One host build uploaded to two contexts (lsg_create_from_host): both
+    apply the operator of a context that builds its own, bit for bit, and
+    solve with it; the host build stays usable."""
+    inp = H.inputs(depth=4, kappa=20.0, q=6, seed=1)
+    hb = L.HostBuild(inp)
+    S0 = make(inp)
+    x = H.rand_vec(S0.N, 8)
+    y0 = S0.MatVecProduct(x)
+    ctxs = []
+    for _ in range(2):
+        S = L.DFMM(inp)
+        S.FMMTreeInitialisation(host=hb)
+        assert S.N == S0.N and np.array_equal(S.gridPoints, S0.gridPoints)
+        assert np.array_equal(S.MatVecProduct(x), y0)
+        ctxs.append(S)
+    b = ctxs[1].rhs()
+    xs, st = ctxs[1].gmres(b, restart_len=50, max_restarts=4, tol=1e-10)
+    assert st["converged"] == 1 and H.relerr(S0.MatVecProduct(xs), b) < 2e-10
+    assert hb.info()["N"] == S0.N and hb.plan()["ops"].size > 0
+    for S in ctxs + [S0]:
+        S.close()
+
+
 def test_matvec_against_dense_small():
     """GPU matvec vs the dense getMatrixEntry2 operator (depth 3, N = 2304)."""
     inp = H.inputs(depth=3, kappa=10.0)

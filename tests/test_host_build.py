@@ -42,6 +42,21 @@ def test_create_without_gpu_fails_loudly():
         S.FMMTreeInitialisation()
 
 
+def test_context_from_host_build_checks():
+    """lsg_create_from_host rejects a tree-only build before it looks for a
+    device, and without a GPU it fails loudly like lsg_create."""
+    inp = H.inputs(depth=3, kappa=10.0)
+    tree_only = L.HostBuild(inp, tree_only=True)
+    with pytest.raises(L.LsgError) as e:
+        L.DFMM(inp).FMMTreeInitialisation(host=tree_only)
+    assert e.value.code == L.LSG_ERR_ARG and "tree" in str(e.value)
+    import torch  # noqa: F401  (only to learn whether a GPU exists)
+    if not torch.cuda.is_available():
+        with pytest.raises(L.LsgError) as e:
+            L.DFMM(inp).FMMTreeInitialisation(host=L.HostBuild(inp))
+        assert e.value.code == L.LSG_ERR_DEVICE
+
+
 @pytest.mark.parametrize("depth,kappa,expect", [
     (4, 20.0, (3, [0, 0, 116, 1756, 5628])),
     (6, 80.0, (5, [0, 0, 0, 2600, 20532, 45628, 105276])),
