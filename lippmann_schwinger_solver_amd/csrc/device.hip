@@ -1452,11 +1452,24 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     hipLaunchKernelGGL(scale_kernel, dim3(nb_vec), dim3(kThreads), 0, s, w, st.beta, V, N);
     int k = 0;
     bool stop = false;
-    double last_est = 1.0;
+    double last_est = beta / bnorm, rate = 0.0;
     for (int j = 0; j < m && !stop;) {
-      // enqueue a batch of Arnoldi steps before reading their estimates back;
-      // steps past an exit found inside the batch are discarded
-      const int batch = std::min(m - j, (!ref_mode && last_est > 100.0 * tol) || ref_mode ? 4 : 1);
+      // Enqueue a batch of Arnoldi steps before reading their estimates back
+      // (each read is a host synchronisation, about 40 us of idle device);
+      // steps past an exit found inside the batch are discarded. In reference
+      // mode only a breakdown ends a cycle early: batches of 16. With a
+      // tolerance, half the steps the last batch's rate of decrease still
+      // predicts before tol (at most 16), and 4 before a rate is known.
+      int batch = 4;
+      if (ref_mode) {
+        batch = 16;
+      } else if (rate > 0.0 && rate < 1.0 && last_est > 0.5 * tol) {
+        const double left = std::log(0.5 * tol / last_est) / std::log(rate);
+        batch = int(std::max(1.0, std::min(16.0, std::floor(0.5 * left))));
+      } else if (rate >= 1.0) {
+        batch = 1;
+      }
+      batch = std::min(batch, m - j);
       for (int jj = j; jj < j + batch; ++jj) {
         if (prec) {
           prec->apply(V + (size_t)jj * N, W.Z + (size_t)jj * N, s);
@@ -1477,9 +1490,12 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
         res.error = hipGetErrorString(e);
         break;
       }
+      const double est_before = last_est;
+      int n_read = 0;
       for (int jj = j; jj < j + batch; ++jj) {
         const double est = hostbuf[2 * jj], hn = hostbuf[2 * jj + 1];
         ++total_it;
+        ++n_read;
         k = jj + 1;
         res.est_residual = est / bnorm;
         last_est = est / bnorm;
@@ -1498,6 +1514,10 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
           break;
         }
       }
+      // geometric mean decrease per step over this batch (the first batch of a
+      // cycle starts from the cycle's residual, 1 relative to itself)
+      if (n_read > 0 && est_before > 0.0 && last_est > 0.0 && std::isfinite(last_est))
+        rate = std::pow(last_est / est_before, 1.0 / n_read);
       j += batch;
     }
     if (res.status != 0) break;
