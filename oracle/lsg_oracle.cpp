@@ -22,6 +22,24 @@
 // and the Hankel expansion above 25; the QR by Eigen 3.3's documented
 // column-pivoting Householder algorithm; the Vandermonde pseudo-inverse by
 // its closed form (the columns are orthogonal at Chebyshev nodes).
+//
+// How it is pinned (tests/):
+// - Bessel values against scipy/mpmath fixtures (tests/golden/bessel_jy.npz);
+// - Gauss-Legendre and Q_pinv against numpy;
+// - the far and near tables (every slot kind of uniform and adaptive trees)
+//   against independent quadratures;
+// - the tree, interaction lists and skeleton sizes against the README run's
+//   printout and the closed-form pair counts;
+// - the matvec against the dense getMatrixEntry2 operator and against an
+//   independent numpy restatement of the passes;
+// - the README's GMRES residuals, reproduced to six digits;
+// - the whole chain against the exact field of the radial Gaussian (one ODE
+//   per Fourier mode).
+// Parity unpinned: the exact NCA pivots and the 6-digit table values where a
+// rounding boundary meets the last bits of the reference's Boost/Eigen
+// arithmetic, since the reference cannot be built here (Eigen and Boost
+// headers are absent). Those follow from the restated algorithm, checked
+// against the README's printed ranks (within one ACA step).
 #include <algorithm>
 #include <cmath>
 #include <complex>
