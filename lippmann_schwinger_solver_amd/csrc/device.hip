@@ -92,6 +92,27 @@ __device__ __forceinline__ void put_dst(C* state, XC* y, int d, int N, int r, C 
   }
 }
 
+// Sum over the nph phases of red[ph * rw + row] into red[row]: a halving tree
+// when there are many phases (few rows), else left to the caller's serial
+// loop. nph and rw are the same for the whole workgroup, so every thread
+// reaches the same barriers. Returns whether the tree ran.
+template <class C>
+__device__ __forceinline__ bool phase_tree(C* red, int nph, int rw, int ph, int row) {
+  if (nph <= 8) return false;
+  for (int n = nph; n > 1;) {
+    const int h = (n + 1) / 2;
+    if (ph < n - h) {
+      C& a = red[ph * rw + row];
+      const C b = red[(ph + h) * rw + row];
+      a.x += b.x;
+      a.y += b.y;
+    }
+    __syncthreads();
+    n = h;
+  }
+  return true;
+}
+
 // One workgroup per destination: Y[0:R] (=|+=) A (R x C, col-major) * src[0..C).
 // Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
 // step of the workgroup reads nph*R consecutive complex entries. The operator is
@@ -166,12 +187,14 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     }
     red[t] = acc;
     __syncthreads();
+    const bool tree = phase_tree(red, nph, Rt, ph, row);
     if (t < Rt) {
       C s = red[t];
-      for (int p = 1; p < nph; ++p) {
-        s.x += red[p * Rt + t].x;
-        s.y += red[p * Rt + t].y;
-      }
+      if (!tree)
+        for (int p = 1; p < nph; ++p) {
+          s.x += red[p * Rt + t].x;
+          s.y += red[p * Rt + t].y;
+        }
       put_dst<ACC>(state, y, dd, N, r0 + t, s);
     }
     __syncthreads();
