@@ -61,6 +61,18 @@ std::string validate(const lsg_inputs* in) {
   return "";
 }
 
+// No entry of v[0, n) is NaN or Inf: an IEEE double is not finite exactly
+// when its exponent bits are all ones; the integer test vectorises.
+bool all_finite(const double* v, int64_t n) {
+  uint64_t bad = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t u;
+    std::memcpy(&u, v + i, sizeof(u));
+    bad |= uint64_t((u & 0x7ff0000000000000ull) == 0x7ff0000000000000ull);
+  }
+  return bad == 0;
+}
+
 // Returns LSG_OK, LSG_ERR_ARG (invalid inputs) or LSG_ERR_SETUP; message in err.
 int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_only = false) {
   err = validate(in);
@@ -140,6 +152,38 @@ struct lsg_ctx {
   double* bx = nullptr;       // two row-split blocks (N x 64) for block calls
   double* by = nullptr;
   double2* bcm = nullptr;     // column-major staging for up to 64 host vectors
+  // pinned host staging of two vectors for host-buffer calls: the caller's
+  // (pageable) data is copied here by the CPU and moved by DMA at full rate
+  double2* pin = nullptr;
+  double2* pinned() {
+    if (!pin && hipHostMalloc((void**)&pin, 2 * sizeof(double2) * size_t(hb.plan.N), hipHostMallocDefault) != hipSuccess)
+      pin = nullptr;
+    return pin;
+  }
+  // host -> device through the pinned staging (slot 0 or 1), on stream s
+  bool to_device(double2* d, const double* h, int slot, hipStream_t s) {
+    const size_t nb = sizeof(double2) * size_t(hb.plan.N);
+    double2* p = pinned();
+    if (!p) return hipMemcpyAsync(d, h, nb, hipMemcpyHostToDevice, s) == hipSuccess;
+    double2* ps = p + size_t(slot) * size_t(hb.plan.N);
+    // the previous use of this slot has finished (every host-buffer call ends synchronised)
+    std::memcpy(ps, h, nb);
+    return hipMemcpyAsync(d, ps, nb, hipMemcpyHostToDevice, s) == hipSuccess;
+  }
+  // device -> host through the pinned staging (slot 1); synchronises s
+  hipError_t to_host(double* h, const double2* d, hipStream_t s) {
+    const size_t nb = sizeof(double2) * size_t(hb.plan.N);
+    double2* p = pinned();
+    if (!p) {
+      hipError_t e = hipMemcpyAsync(h, d, nb, hipMemcpyDeviceToHost, s);
+      return e != hipSuccess ? e : hipStreamSynchronize(s);
+    }
+    double2* ps = p + size_t(hb.plan.N);
+    hipError_t e = hipMemcpyAsync(ps, d, nb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) std::memcpy(h, ps, nb);
+    return e;
+  }
   std::string block_ready() {
     std::string e = blk.setup(dev);
     if (!e.empty()) return e;
@@ -238,6 +282,7 @@ void lsg_destroy(lsg_ctx* ctx) {
   if (ctx->bx) hipFree(ctx->bx);
   if (ctx->by) hipFree(ctx->by);
   if (ctx->bcm) hipFree(ctx->bcm);
+  if (ctx->pin) hipHostFree(ctx->pin);
   delete ctx;
 }
 
@@ -343,8 +388,7 @@ int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld) {
   if (nrhs >= 2) {
     // block path: 64 right-hand sides per block matvec (FP64 MFMA)
     for (int r = 0; r < nrhs; ++r)
-      for (int64_t i = 0; i < 2 * N; ++i)
-        if (!std::isfinite(x[2 * ld * r + i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+      if (!all_finite(x + 2 * ld * r, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
     std::string e = ctx->block_ready();
     if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
     hipStream_t s = ctx->dev.stream;
@@ -364,15 +408,11 @@ int lsg_matvec(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld) {
   }
   for (int r = 0; r < nrhs; ++r) {
     const double* xr = x + 2 * ld * r;
-    for (int64_t i = 0; i < 2 * N; ++i)
-      if (!std::isfinite(xr[i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    if (!all_finite(xr, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
     hipStream_t s = ctx->dev.stream;
-    if (hipMemcpyAsync(ctx->dx, xr, nb, hipMemcpyHostToDevice, s) != hipSuccess)
-      return ctx->set(LSG_ERR_DEVICE, "copy in failed");
+    if (!ctx->to_device(ctx->dx, xr, 0, s)) return ctx->set(LSG_ERR_DEVICE, "copy in failed");
     ctx->dev.matvec(ctx->dx, ctx->dy);
-    if (hipMemcpyAsync(y + 2 * ld * r, ctx->dy, nb, hipMemcpyDeviceToHost, s) != hipSuccess)
-      return ctx->set(LSG_ERR_DEVICE, "copy out failed");
-    hipError_t e = hipStreamSynchronize(s);
+    hipError_t e = ctx->to_host(y + 2 * ld * r, ctx->dy, s);
     if (e != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("matvec: ") + hipGetErrorString(e));
   }
   return LSG_OK;
@@ -527,14 +567,11 @@ int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_
     db = const_cast<double2*>(reinterpret_cast<const double2*>(b));
     dxs = reinterpret_cast<double2*>(x);
   } else {
-    for (int64_t i = 0; i < 2 * N; ++i)
-      if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
-    if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
-      if (db) hipFree(db);
-      return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
-    }
-    hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
+    if (!all_finite(b, 2 * N) || !all_finite(x, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+    // the context's own vectors (one call at a time per context)
+    db = ctx->dx;
+    dxs = ctx->dy;
+    if (!ctx->to_device(db, b, 0, s) || !ctx->to_device(dxs, x, 1, s)) return ctx->set(LSG_ERR_DEVICE, "copy in failed");
   }
   auto t0 = std::chrono::steady_clock::now();
   if (flags & LSG_PREC_GUESS) ctx->prec.apply(db, dxs, s);
@@ -555,10 +592,11 @@ int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_
   }
   r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (!o->device_ptrs) {
-    hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
-    hipFree(db);
-    hipFree(dxs);
+    hipError_t ce = ctx->to_host(x, dxs, s);
+    if (ce != hipSuccess && r.status == 0) {
+      r.status = LSG_ERR_DEVICE;
+      r.error = std::string("copy out failed: ") + hipGetErrorString(ce);
+    }
   }
   if (stats) {
     stats->iterations = r.iterations;
@@ -623,8 +661,7 @@ int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t 
   const size_t nb = sizeof(double2) * size_t(N);
   for (int r = 0; r < nrhs; ++r) {
     const double* xr = x + 2 * ld * r;
-    for (int64_t i = 0; i < 2 * N; ++i)
-      if (!std::isfinite(xr[i])) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    if (!all_finite(xr, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
     hipStream_t s = ctx->dev.stream;
     if (hipMemcpyAsync(ctx->dx, xr, nb, hipMemcpyHostToDevice, s) != hipSuccess)
       return ctx->set(LSG_ERR_DEVICE, "copy in failed");
@@ -652,24 +689,22 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
     db = const_cast<double2*>(reinterpret_cast<const double2*>(b));
     dxs = reinterpret_cast<double2*>(x);
   } else {
-    for (int64_t i = 0; i < 2 * N; ++i)
-      if (!std::isfinite(b[i]) || !std::isfinite(x[i])) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
-    if (hipMalloc((void**)&db, nb) != hipSuccess || hipMalloc((void**)&dxs, nb) != hipSuccess) {
-      if (db) hipFree(db);
-      return ctx->set(LSG_ERR_DEVICE, "hipMalloc failed");
-    }
-    hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(dxs, x, nb, hipMemcpyHostToDevice, s);
+    if (!all_finite(b, 2 * N) || !all_finite(x, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+    // the context's own vectors (one call at a time per context)
+    db = ctx->dx;
+    dxs = ctx->dy;
+    if (!ctx->to_device(db, b, 0, s) || !ctx->to_device(dxs, x, 1, s)) return ctx->set(LSG_ERR_DEVICE, "copy in failed");
   }
   GmresResult r = o->mode == LSG_SOLVE_FP64
                       ? device_gmres(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol)
                       : device_gmres_ir(ctx->dev, db, dxs, o->restart_len, o->max_restarts, o->tol, o->inner_tol,
                                         o->max_outer);
   if (!o->device_ptrs) {
-    hipMemcpyAsync(x, dxs, nb, hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
-    hipFree(db);
-    hipFree(dxs);
+    hipError_t ce = ctx->to_host(x, dxs, s);
+    if (ce != hipSuccess && r.status == 0) {
+      r.status = LSG_ERR_DEVICE;
+      r.error = std::string("copy out failed: ") + hipGetErrorString(ce);
+    }
   }
   if (stats) {
     stats->iterations = r.iterations;
@@ -709,9 +744,8 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
   };
   if (!o->device_ptrs)
     for (int r = 0; r < nrhs; ++r)
-      for (int64_t i = 0; i < 2 * N; ++i)
-        if (!std::isfinite(b[2 * ld * r + i]) || !std::isfinite(x[2 * ld * r + i]))
-          return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+      if (!all_finite(b + 2 * ld * r, 2 * N) || !all_finite(x + 2 * ld * r, 2 * N))
+        return ctx->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
   load(b, ctx->bx);
   load(x, ctx->by);
   std::vector<double> rel(kBlockRhs, 0.0);
