@@ -7,8 +7,8 @@ Known answers from GMRES/Readme.md:43-69, the run
 treeAdaptivity 5, TOL_POW 8):
   nLevels 5, level_LFR 4, 5328 unknowns, boxes per level 16/64/64/48;
   per-level NCA printout (O/I lines, G_FMM2DTree.hpp:2238, 2384, 3418, 3570);
-  true relative residual of GMRES(m=10), one cycle: 4.59412e-06, two cycles
-  (BASELINE.md table): 1.28443e-10.
+  true relative residual of GMRES(m=10), one cycle: 4.59412e-06, two cycles:
+  1.28443e-10.
 """
 import numpy as np
 import pytest

@@ -1,5 +1,5 @@
 """FP32-operator matvec and FP64 iterative refinement around FP32 GMRES
-(the mixed-precision solver mode); the FP64 true residual is the judge."""
+(the mixed-precision solver mode); the FP64 true residual decides convergence."""
 import numpy as np
 import pytest
 
