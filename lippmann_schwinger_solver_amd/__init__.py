@@ -408,6 +408,17 @@ class DFMM:
         if rc != 0:
             raise LsgError(rc, self._lib.lsg_last_error(self._ctx).decode())
 
+    def _vec(self, v, name):
+        """v as a contiguous complex128 copy-or-view of shape (N,); anything else
+        raises ValueError (the C side reads and writes exactly N entries)."""
+        a = np.ascontiguousarray(v, dtype=np.complex128)
+        if a.shape != (self.N,):
+            raise ValueError(f"{name} must have shape ({self.N},), got {a.shape}")
+        return a
+
+    def _x0(self, x0):
+        return np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(self._vec(x0, "x0"))
+
     def info(self):
         inf = lsg_info()
         self._check(self._lib.lsg_get_info(self._ctx, C.byref(inf)))
@@ -451,8 +462,8 @@ class DFMM:
 
     def gmres(self, b, x0=None, restart_len=50, max_restarts=20, tol=1e-10):
         """Device GMRES; returns (x, stats dict). tol <= 0 runs the reference's fixed cycles."""
-        b = np.ascontiguousarray(b, dtype=np.complex128)
-        x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        b = self._vec(b, "b")
+        x = self._x0(x0)
         st = lsg_stats()
         rc = self._lib.lsg_gmres(self._ctx, _dptr(b.view(np.float64)), _dptr(x.view(np.float64)), restart_len,
                                  max_restarts, tol, C.byref(st))
@@ -477,8 +488,8 @@ class DFMM:
             rc = self._lib.lsg_gmres_ex(self._ctx, C.c_void_p(device_ptrs[0]), C.c_void_p(device_ptrs[1]), C.byref(o),
                                         C.byref(st))
         else:
-            b = np.ascontiguousarray(b, dtype=np.complex128)
-            x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+            b = self._vec(b, "b")
+            x = self._x0(x0)
             rc = self._lib.lsg_gmres_ex(self._ctx, b.ctypes.data_as(_P), x.ctypes.data_as(_P), C.byref(o),
                                         C.byref(st))
         stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
@@ -502,9 +513,16 @@ class DFMM:
                                            ld or self.N, C.byref(o), _dptr(rel), C.byref(st))
             X = None
         else:
-            Bt = np.ascontiguousarray(np.asarray(B, dtype=np.complex128).T)     # nrhs x N
+            Ba = np.asarray(B, dtype=np.complex128)
+            if Ba.ndim == 1:
+                Ba = Ba[:, None]
+            if Ba.ndim != 2 or Ba.shape[0] != self.N:
+                raise ValueError(f"B must have shape ({self.N}, nrhs), got {Ba.shape}")
+            Bt = np.ascontiguousarray(Ba.T)     # nrhs x N
             nrhs = Bt.shape[0]
-            Xt = np.zeros_like(Bt) if X0 is None else np.ascontiguousarray(np.asarray(X0, dtype=np.complex128).T)
+            if X0 is not None and np.shape(X0) != Ba.shape:
+                raise ValueError(f"X0 must have the shape of B {Ba.shape}, got {np.shape(X0)}")
+            Xt = np.zeros_like(Bt) if X0 is None else np.ascontiguousarray(np.asarray(X0, dtype=np.complex128).reshape(Ba.shape).T)
             rel = np.zeros(nrhs)
             rc = self._lib.lsg_gmres_batch(self._ctx, Bt.ctypes.data_as(_P), Xt.ctypes.data_as(_P), nrhs, self.N,
                                            C.byref(o), _dptr(rel), C.byref(st))
@@ -540,7 +558,7 @@ class DFMM:
 
     def MatVecProduct_fp32(self, x):
         """y = A32 x: FP32 operators and FP32 accumulation inside each pass, FP64 x and y."""
-        x = np.ascontiguousarray(x, dtype=np.complex128)
+        x = self._vec(x, "x")
         out = np.empty_like(x)
         self._check(self._lib.lsg_matvec_fp32(self._ctx, _dptr(x.view(np.float64)), _dptr(out.view(np.float64)),
                                               1, self.N))
@@ -563,14 +581,14 @@ class DFMM:
         self._net = net
 
     def apply_preconditioner(self, v):
-        v = np.ascontiguousarray(v, dtype=np.complex128)
+        v = self._vec(v, "v")
         z = np.empty_like(v)
         self._check(self._lib.lsg_apply_preconditioner(self._ctx, _dptr(v.view(np.float64)), _dptr(z.view(np.float64))))
         return z
 
     def to_grid(self, v):
         """Leaf-node vector (N complex) -> G x G grid (complex), Chebyshev interpolation per leaf."""
-        v = np.ascontiguousarray(v, dtype=np.complex128)
+        v = self._vec(v, "v")
         G = self._net.grid
         out = np.empty(G * G, dtype=np.complex128)
         self._check(self._lib.lsg_grid_resample(self._ctx, _dptr(v.view(np.float64)), _dptr(out.view(np.float64)), 1))
@@ -586,8 +604,8 @@ class DFMM:
     def gmres_learned(self, b, x0=None, guess=True, right=True, restart_len=50, max_restarts=20, tol=1e-8):
         """Flexible GMRES with the attached network as initial guess and/or right preconditioner."""
         o = lsg_solve_opts(mode=LSG_SOLVE_FP64, restart_len=restart_len, max_restarts=max_restarts, tol=tol)
-        b = np.ascontiguousarray(b, dtype=np.complex128)
-        x = np.zeros(self.N, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+        b = self._vec(b, "b")
+        x = self._x0(x0)
         st = lsg_stats()
         flags = (1 if guess else 0) | (2 if right else 0)
         rc = self._lib.lsg_gmres_learned(self._ctx, b.ctypes.data_as(_P), x.ctypes.data_as(_P), C.byref(o), flags,
@@ -663,8 +681,10 @@ def gmres_ragged(solvers, rhs, restart_len=50, max_restarts=20, tol=1e-8, mode="
     """Solve one system per DFMM context concurrently (lsg_gmres_ragged): the
     contexts may differ in depth, wavenumber and contrast. Returns (xs, stats list)."""
     n = len(solvers)
+    if n < 1 or len(rhs) != n:
+        raise ValueError(f"one right-hand side per solver: {len(rhs)} for {n} solvers")
     lib = solvers[0]._lib
-    bs = [np.ascontiguousarray(b, dtype=np.complex128) for b in rhs]
+    bs = [S._vec(b, f"rhs[{i}]") for i, (S, b) in enumerate(zip(solvers, rhs))]
     xs = [np.zeros(S.N, dtype=np.complex128) for S in solvers]
     ctxs = (C.c_void_p * n)(*[S._ctx for S in solvers])
     bp = (C.c_void_p * n)(*[b.ctypes.data for b in bs])

@@ -78,10 +78,6 @@ def build(verbose=False):
     _run(["g++", "-O2", "-std=c++17", "-o", exe + ".tmp", os.path.join(HERE, "cpp", "lsg_main.cpp"),
           "-L" + HERE, "-llsg", "-Wl,-rpath,$ORIGIN"])
     os.replace(exe + ".tmp", exe)
-    # streaming-read ceiling probe (bench.py --stream-ceiling)
-    probe = os.path.join(HERE, "stream_probe")
-    _run([HIPCC] + HIP_FLAGS + ["-x", "hip", os.path.join(CSRC, "stream_probe.hip"), "-o", probe + ".tmp"])
-    os.replace(probe + ".tmp", probe)
     return LIB
 
 

@@ -59,7 +59,7 @@ struct Mfma<float> {
 // copies X into the state once, rounded, and passes nx = 0).
 template <bool ACC, class R, int MT>
 __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* __restrict__ ops,
-                                                   const int* __restrict__ src, R* bst, const double* __restrict__ X,
+                                                   const int* __restrict__ src, R* bst, const R* __restrict__ X,
                                                    double* Y, int N, const int* __restrict__ rows,
                                                    const int* __restrict__ ncol, const int* __restrict__ dst,
                                                    const long long* __restrict__ op_off,
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* _
       bv[ks] = 0;
       if (cl < Cc) {
         const int row = Ss[cl];
-        bv[ks] = row < nx ? R(X[(size_t)row * kLD + b * kNB + q]) : bst[(size_t)row * kLD + b * kNB + q];
+        bv[ks] = row < nx ? X[(size_t)row * kLD + b * kNB + q] : bst[(size_t)row * kLD + b * kNB + q];
       }
     }
 #pragma unroll
@@ -184,17 +184,19 @@ __global__ __launch_bounds__(kT) void bsplit_reduce_kernel(const R* __restrict__
 
 // Y = X + qk2 .* Y on the output rows (find_phi); the near-field pass stored
 // the bare colleague sums in Y.
-__global__ void bnear_finish_kernel(const double* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2,
+template <class XT>
+__global__ void bnear_finish_kernel(const XT* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2,
                                     long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_total) Y[i] = X[i] + qk2[i / kLD] * Y[i];
+  if (i < n_total) Y[i] = double(X[i]) + qk2[i / kLD] * Y[i];
 }
 
 // Y = X + qk2c .* Y with a contrast per right-hand side (qk2c[n * 64 + q])
-__global__ void bdiag_kernel(const double* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2c,
+template <class XT>
+__global__ void bdiag_kernel(const XT* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2c,
                              long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_total) Y[i] = X[i] + qk2c[(i / kLD) * kNB + (i % kNB)] * Y[i];
+  if (i < n_total) Y[i] = double(X[i]) + qk2c[(i / kLD) * kNB + (i % kNB)] * Y[i];
 }
 
 // ------------------------------------------------------- batched GMRES ----
@@ -203,7 +205,8 @@ constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
 
 // partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
 // conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
-__global__ __launch_bounds__(kT) void bdot_kernel(const double* __restrict__ V, long long vs, int nc, int ncx,
+template <class TV>
+__global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long long vs, int nc, int ncx,
                                                   const double* __restrict__ w, int N, double2* __restrict__ partial) {
   __shared__ double2 red[kT / 64][kBCols][kNB];
   const int t = threadIdx.x, q = t & 63, rl = t >> 6;
@@ -221,8 +224,8 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const double* __restrict__ V, 
       const int col = g0 + u;
       double vr = 0.0, vi = 0.0;
       if (col < nc) {
-        vr = V[col * vs + n * kLD + q];
-        vi = V[col * vs + n * kLD + kNB + q];
+        vr = double(V[col * vs + n * kLD + q]);
+        vi = double(V[col * vs + n * kLD + kNB + q]);
       } else if (col == nc) {
         vr = wr;
         vi = wi;
@@ -300,11 +303,12 @@ __global__ void bcycle_init_kernel(const double2* __restrict__ h, BGivens st, in
 }
 
 // V0[n, q] = w[n, q] / beta_q for active q, 0 otherwise (beta in st.hn)
-__global__ void bscale_kernel(const double* __restrict__ w, BGivens st, double* __restrict__ v, long long n_total) {
+template <class TV>
+__global__ void bscale_kernel(const double* __restrict__ w, BGivens st, TV* __restrict__ v, long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_total) return;
   const int q = int(i % kNB);
-  v[i] = st.active[q] ? w[i] / st.hn[q] : 0.0;
+  v[i] = TV(st.active[q] ? w[i] / st.hn[q] : 0.0);
 }
 
 // Arnoldi step j for every active system: h in [0, j], ||w||^2 at j+1 (from the
@@ -373,9 +377,10 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
 // (q, row group) owns kUpdRows consecutive rows of system q, so each
 // coefficient h[i, q] is loaded once per kUpdRows rows.
 constexpr int kUpdRows = 8;
-__global__ __launch_bounds__(kT) void bupdate_kernel(const double* __restrict__ V, long long vs, int nc,
+template <class TV>
+__global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, long long vs, int nc,
                                                      const double2* __restrict__ h, const double* __restrict__ w,
-                                                     const double* __restrict__ hn, double* __restrict__ vout,
+                                                     const double* __restrict__ hn, TV* __restrict__ vout,
                                                      long long n_rows) {
   const int q = threadIdx.x & 63;
   const long long n0 = ((long long)blockIdx.x * (kT / 64) + (threadIdx.x >> 6)) * kUpdRows;
@@ -388,11 +393,11 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const double* __restrict__ 
 #pragma unroll 2
   for (int i = 0; i < nc; ++i) {
     const double2 hh = h[i * kNB + q];
-    const double* Vi = V + i * vs + base;
+    const TV* Vi = V + i * vs + base;
 #pragma unroll
     for (int r = 0; r < kUpdRows; ++r) {
       if (r < nr) {
-        const double vr = Vi[r * kLD], vi = Vi[r * kLD + kNB];
+        const double vr = double(Vi[r * kLD]), vi = double(Vi[r * kLD + kNB]);
         ar[r] = fma(vr, hh.x, fma(-vi, hh.y, ar[r]));
         ai[r] = fma(vr, hh.y, fma(vi, hh.x, ai[r]));
       }
@@ -404,8 +409,8 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const double* __restrict__ 
   for (int r = 0; r < kUpdRows; ++r) {
     if (r < nr) {
       const long long o = base + r * kLD;
-      vout[o] = (w[o] - ar[r]) * inv;
-      vout[o + kNB] = (w[o + kNB] - ai[r]) * inv;
+      vout[o] = TV((w[o] - ar[r]) * inv);
+      vout[o + kNB] = TV((w[o + kNB] - ai[r]) * inv);
     }
   }
 }
@@ -432,7 +437,8 @@ __global__ void btrisolve_kernel(BGivens st, int m, double2* __restrict__ y) {
 }
 
 // X[n, q] += sum_i V_i[n, q] y[i, q], i < nc
-__global__ __launch_bounds__(kT) void baxpy_kernel(const double* __restrict__ V, long long vs, int nc,
+template <class TV>
+__global__ __launch_bounds__(kT) void baxpy_kernel(const TV* __restrict__ V, long long vs, int nc,
                                                    const double2* __restrict__ y, double* __restrict__ X,
                                                    long long n_rows) {
   const long long e = (long long)blockIdx.x * kT + threadIdx.x;
@@ -442,7 +448,7 @@ __global__ __launch_bounds__(kT) void baxpy_kernel(const double* __restrict__ V,
   const long long base = n * kLD + q;
   double ar = 0.0, ai = 0.0;
   for (int i = 0; i < nc; ++i) {
-    const double vr = V[i * vs + base], vi = V[i * vs + base + kNB];
+    const double vr = double(V[i * vs + base]), vi = double(V[i * vs + base + kNB]);
     const double2 yy = y[i * kNB + q];
     ar = fma(vr, yy.x, fma(-vi, yy.y, ar));
     ai = fma(vr, yy.y, fma(vi, yy.x, ai));
@@ -592,9 +598,8 @@ void BlockOperator::release() {
 
 template <bool ACC, class R, int MT>
 static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename Mfma<R>::c2* ops, const int* src,
-                            R* bst, const double* X, double* Y, const int* rows, const int* ncol, const int* dst,
+                            R* bst, const R* X, int nx, double* Y, const int* rows, const int* ncol, const int* dst,
                             const long long* op_off, const long long* src_off, int nseg) {
-  const int nx = std::is_same<R, double>::value ? bo.op->N : 0;
   hipStream_t s = bo.op->stream;
   if (nseg <= 1) {
     hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N, rows,
@@ -611,12 +616,12 @@ static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename M
 // row tiles from the largest destination of the pass
 template <bool ACC, class R>
 static void launch_bpass_any(const BlockOperator& bo, int max_rows, int ndest, const typename Mfma<R>::c2* ops,
-                             const int* src, R* bst, const double* X, double* Y, const int* rows, const int* ncol,
+                             const int* src, R* bst, const R* X, int nx, double* Y, const int* rows, const int* ncol,
                              const int* dst, const long long* op_off, const long long* src_off, int nseg = 1) {
   const int mt = std::max(1, (2 * max_rows + 15) / 16);
 #define LSG_BPASS_CASE(M)                                                                                       \
   case M:                                                                                                       \
-    launch_bpass_mt<ACC, R, M>(bo, ndest, ops, src, bst, X, Y, rows, ncol, dst, op_off, src_off, nseg);        \
+    launch_bpass_mt<ACC, R, M>(bo, ndest, ops, src, bst, X, nx, Y, rows, ncol, dst, op_off, src_off, nseg);    \
     break;
   switch (mt) {
     LSG_BPASS_CASE(1)
@@ -627,13 +632,13 @@ static void launch_bpass_any(const BlockOperator& bo, int max_rows, int ndest, c
     LSG_BPASS_CASE(6)
     LSG_BPASS_CASE(7)
     default:
-      launch_bpass_mt<ACC, R, 8>(bo, ndest, ops, src, bst, X, Y, rows, ncol, dst, op_off, src_off, nseg);
+      launch_bpass_mt<ACC, R, 8>(bo, ndest, ops, src, bst, X, nx, Y, rows, ncol, dst, op_off, src_off, nseg);
   }
 #undef LSG_BPASS_CASE
 }
 
 template <class R>
-static void launch_bpass(BlockOperator& bo, int pidx, const double* X, double* Y) {
+static void launch_bpass(BlockOperator& bo, int pidx, const R* X, int nx, double* Y) {
   DeviceOperator& o = *bo.op;
   const DevPass& p = o.passes[size_t(pidx)];
   const int64_t b = p.dest_base;
@@ -643,10 +648,10 @@ static void launch_bpass(BlockOperator& bo, int pidx, const double* X, double* Y
   const int mr = bo.pass_max_rows[size_t(pidx)];
   const int nseg = bo.pass_nseg[size_t(pidx)];
   if (p.kind == 0)
-    launch_bpass_any<false, R>(bo, mr, p.ndest, ops, o.src, bst, X, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
+    launch_bpass_any<false, R>(bo, mr, p.ndest, ops, o.src, bst, X, nx, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
                                o.d_op + b, o.d_srcoff + b, nseg);
   else
-    launch_bpass_any<true, R>(bo, mr, p.ndest, ops, o.src, bst, X, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
+    launch_bpass_any<true, R>(bo, mr, p.ndest, ops, o.src, bst, X, nx, Y, o.d_rows + b, o.d_ncol + b, o.d_dst + b,
                               o.d_op + b, o.d_srcoff + b, nseg);
 }
 
@@ -656,34 +661,46 @@ __global__ void bround_kernel(const double* __restrict__ X, float* __restrict__ 
   if (i < n_total) bst32[i] = float(X[i]);
 }
 
-template <class R>
-static void block_matvec_impl(BlockOperator& bo, const double* X, double* Y) {
+// R: pass precision; XT: type of the input block. An FP64 input to the FP32
+// passes is rounded into the pass state's x rows first; an input already in
+// the pass precision is read in place.
+template <class R, class XT>
+static void block_matvec_impl(BlockOperator& bo, const XT* X, double* Y) {
   DeviceOperator& o = *bo.op;
   typedef typename Mfma<R>::c2 C2;
   const C2* T = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
   R* bst = reinterpret_cast<R*>(std::is_same<R, double>::value ? (void*)bo.bstate : (void*)bo.bstate32);
-  if (!std::is_same<R, double>::value)
-    hipLaunchKernelGGL(bround_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
-                       bo.bstate32, (long long)o.N * kLD);
-  for (int p : o.up) launch_bpass<R>(bo, p, X, Y);
+  const R* Xr;
+  int nx;
+  if (std::is_same<R, XT>::value) {
+    Xr = reinterpret_cast<const R*>(X);
+    nx = o.N;
+  } else {
+    hipLaunchKernelGGL(bround_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream,
+                       reinterpret_cast<const double*>(X), bo.bstate32, (long long)o.N * kLD);
+    Xr = bst;
+    nx = 0;
+  }
+  for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Y);
   if (o.T) {
-    launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, X, Y, bo.near_rows, bo.near_ncol,
+    launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, Xr, nx, Y, bo.near_rows, bo.near_ncol,
                                bo.near_dst, bo.near_op, bo.near_srcoff);
     if (o.find_phi)
-      hipLaunchKernelGGL(bnear_finish_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
-                         Y, o.qk2, (long long)o.N * kLD);
+      hipLaunchKernelGGL(bnear_finish_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream,
+                         X, Y, o.qk2, (long long)o.N * kLD);
   }
-  for (int p : o.near) launch_bpass<R>(bo, p, X, Y);
-  for (int p : o.down) launch_bpass<R>(bo, p, X, Y);
-  for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, X, Y);
-  for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, X, Y);
+  for (int p : o.near) launch_bpass<R>(bo, p, Xr, nx, Y);
+  for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
+  for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, Xr, nx, Y);
+  for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, Xr, nx, Y);
   if (bo.qk2_cols)
-    hipLaunchKernelGGL(bdiag_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
+    hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
                        bo.qk2_cols, (long long)o.N * kLD);
 }
 
-void BlockOperator::matvec(const double* X, double* Y) { block_matvec_impl<double>(*this, X, Y); }
-void BlockOperator::matvec32(const double* X, double* Y) { block_matvec_impl<float>(*this, X, Y); }
+void BlockOperator::matvec(const double* X, double* Y) { block_matvec_impl<double, double>(*this, X, Y); }
+void BlockOperator::matvec32(const double* X, double* Y) { block_matvec_impl<float, double>(*this, X, Y); }
+void BlockOperator::matvec32(const float* X, double* Y) { block_matvec_impl<float, float>(*this, X, Y); }
 
 std::string BlockOperator::enable_fp32() {
   std::string e = op->enable_fp32();
@@ -697,7 +714,6 @@ std::string BlockOperator::enable_fp32() {
 std::string BlockWork::reserve(int N_, int m_) {
   const long long vs = (long long)N_ * kLD;
   const int nrb = (N_ + kBRows - 1) / kBRows;
-  HIP_OK(hipMalloc((void**)&V, sizeof(double) * size_t(vs) * (m_ + 1)));
   HIP_OK(hipMalloc((void**)&w, sizeof(double) * size_t(vs)));
   HIP_OK(hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m_ + 2) * kNB));
   HIP_OK(hipMalloc((void**)&h, sizeof(double2) * size_t(m_ + 2) * kNB));
@@ -719,8 +735,15 @@ std::string BlockWork::reserve(int N_, int m_) {
   return "";
 }
 
+std::string BlockWork::basis(bool f32) {
+  const size_t n = size_t(N) * kLD * size_t(m + 1);
+  if (f32 && !V32) HIP_OK(hipMalloc((void**)&V32, sizeof(float) * n));
+  if (!f32 && !V) HIP_OK(hipMalloc((void**)&V, sizeof(double) * n));
+  return "";
+}
+
 void BlockWork::release() {
-  void* ptrs[] = {V, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  void* ptrs[] = {V, V32, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (host_rel) (void)hipHostFree(host_rel);
@@ -729,8 +752,12 @@ void BlockWork::release() {
   *this = BlockWork{};
 }
 
-BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
-                        double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
+// TV: storage of the Krylov basis (double; float for the FP32 inner solves,
+// whose accuracy target is far above FP32 rounding). Dots, norms, Givens and
+// the solution update accumulate in FP64 either way.
+template <class TV>
+static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
+                                 double tol, double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
   BatchResult res;
   auto t0 = std::chrono::steady_clock::now();
   DeviceOperator& o = *bo.op;
@@ -755,7 +782,17 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
       return res;
     }
   }
-  double *V = W.V, *w = W.w;
+  {
+    std::string e = W.basis(std::is_same<TV, float>::value);
+    if (!e.empty()) {
+      W.release();
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched GMRES basis allocation failed: " + e;
+      return res;
+    }
+  }
+  TV* V = reinterpret_cast<TV*>(std::is_same<TV, float>::value ? (void*)W.V32 : (void*)W.V);
+  double* w = W.w;
   double2 *partial = W.partial, *h = W.h, *y = W.y;
   BGivens st{W.H, W.cs, W.sn, W.g, W.hn, W.bnorm, W.active, W.kq, W.tolq};
   double *host_rel = W.host_rel, *host_est = W.host_est;
@@ -763,8 +800,8 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
   auto cleanup = [&]() { hipStreamSynchronize(s); };
   auto dot = [&](int nc, const double* vec, bool norm) {
     const int ncx = nc + (norm ? 1 : 0);
-    hipLaunchKernelGGL(bdot_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec, N,
-                       partial);
+    hipLaunchKernelGGL(bdot_kernel<TV>, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
+                       N, partial);
     hipLaunchKernelGGL(breduce_kernel, dim3(ncx), dim3(kT), 0, s, partial, nrb, ncx, h);
   };
   // ||b_q||
@@ -801,9 +838,14 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
       return res;
     }
   }
-  auto amv = [&](const double* xin, double* yout) {
-    if (fp32) bo.matvec32(xin, yout);
-    else bo.matvec(xin, yout);
+  auto amv = [&](const auto* xin, double* yout) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(xin)>> XT;
+    if constexpr (std::is_same<XT, float>::value) {
+      bo.matvec32(xin, yout);
+    } else {
+      if (fp32) bo.matvec32(xin, yout);
+      else bo.matvec(xin, yout);
+    }
   };
   hipMemcpyAsync(st.bnorm, bnorm.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
   hipMemcpyAsync(st.tolq, tolv.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
@@ -840,15 +882,15 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
       break;
     }
     if (n_act == 0 || cyc == max_restarts) break;
-    hipLaunchKernelGGL(bscale_kernel, dim3(gv), dim3(kT), 0, s, w, st, V, ntot);
+    hipLaunchKernelGGL(bscale_kernel<TV>, dim3(gv), dim3(kT), 0, s, w, st, V, ntot);
     int steps = 0;
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
       for (int jj = j; jj < j + batch; ++jj) {
-        amv(V + size_t(jj) * vs, w);
+        amv((const TV*)(V + size_t(jj) * vs), w);
         dot(jj + 1, w, true);
         hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
+        hipLaunchKernelGGL(bupdate_kernel<TV>, dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
                            V + size_t(jj + 1) * vs, (long long)N);
       }
       if (hipStreamSynchronize(s) != hipSuccess) {
@@ -865,7 +907,7 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
     if (res.status != 0) break;
     total_it += steps;
     hipLaunchKernelGGL(btrisolve_kernel, dim3(1), dim3(kNB), 0, s, st, m, y);
-    hipLaunchKernelGGL(baxpy_kernel, dim3(gq), dim3(kT), 0, s, V, vs, steps, y, X, (long long)N);
+    hipLaunchKernelGGL(baxpy_kernel<TV>, dim3(gq), dim3(kT), 0, s, V, vs, steps, y, X, (long long)N);
     res.restarts = cyc + 1;
   }
   res.iterations = total_it;
@@ -876,6 +918,15 @@ BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs,
   cleanup();
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return res;
+}
+
+BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
+                        double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
+  // the FP32 inner solves keep their Krylov basis in FP32 (half the
+  // Gram-Schmidt traffic); the FP64 solver keeps it in FP64
+  if (fp32)
+    return block_gmres_t<float>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, true, tol_per, allow_zero);
+  return block_gmres_t<double>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, false, tol_per, allow_zero);
 }
 
 BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
@@ -921,7 +972,8 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
   const unsigned gv = blocks_for(vs, kT);
   // per-system 2-norms of a block
   auto norms = [&](const double* v, std::vector<double>& out) {
-    hipLaunchKernelGGL(bdot_kernel, dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N, partial);
+    hipLaunchKernelGGL(bdot_kernel<double>, dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N,
+                       partial);
     hipLaunchKernelGGL(breduce_kernel, dim3(1), dim3(kT), 0, s, partial, nrb, 1, h);
     std::vector<double2> hh(kNB);
     hipMemcpyAsync(hh.data(), h, sizeof(double2) * kNB, hipMemcpyDeviceToHost, s);

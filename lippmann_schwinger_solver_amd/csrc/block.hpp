@@ -22,12 +22,14 @@ constexpr int kBlockRhs = 64;  // right-hand sides per block (4 MFMA column tile
 struct BlockWork {
   int N = 0, m = -1;
   double *V = nullptr, *w = nullptr;
+  float* V32 = nullptr;   // FP32 basis of the FP32 inner solves
   double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *bnorm = nullptr, *tolq = nullptr;
   int *active = nullptr, *kq = nullptr;
   double *host_rel = nullptr, *host_est = nullptr;  // pinned
   int* host_active = nullptr;                        // pinned
-  std::string reserve(int N, int m);
+  std::string reserve(int N, int m);   // everything but the basis
+  std::string basis(bool f32);         // the FP64 or FP32 basis, on first use
   void release();
 };
 
@@ -54,8 +56,10 @@ struct BlockOperator {
   void release();
   // Y = A X for one block (row-split, nb columns), on op->stream
   void matvec(const double* X, double* Y);
-  // same with FP32 operators, FP32 MFMA and FP32 pass state (X, Y FP64)
+  // same with FP32 operators, FP32 MFMA and FP32 pass state (Y FP64; X FP64,
+  // rounded into the pass state, or FP32 and read in place)
   void matvec32(const double* X, double* Y);
+  void matvec32(const float* X, double* Y);
 };
 
 // Batched restarted GMRES: nb independent systems A x_q = b_q advanced in

@@ -457,17 +457,31 @@ int lsg_set_contrast(lsg_ctx* ctx, const double* qk2, int ncol) {
   if (!ctx) return LSG_ERR_ARG;
   if (ctx->dev.find_phi) return ctx->set(LSG_ERR_ARG, "lsg_set_contrast needs a volume-potential context (find_phi = 0)");
   const int64_t N = ctx->hb.plan.N;
+  if (qk2) {
+    if (ncol < 1 || ncol > kBlockRhs) return ctx->set(LSG_ERR_ARG, "ncol must be in [1, 64]");
+    for (int64_t i = 0; i < N * ncol; ++i)
+      if (!std::isfinite(qk2[i])) return ctx->set(LSG_ERR_NUMERIC, "contrast holds NaN/Inf");
+  }
+  // A captured matvec graph decided at capture time whether it applies the
+  // contrast and which buffer it reads: drop it (after in-flight replays)
+  // and capture again once the new contrast is in place.
+  const bool had_graph = ctx->dev.graph_exec != nullptr;
+  ctx->dev.drop_graph();
+  hipError_t se = hipStreamSynchronize(ctx->dev.stream);
+  if (se != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("lsg_set_contrast: ") + hipGetErrorString(se));
+  auto recapture = [&]() {
+    if (!had_graph) return LSG_OK;
+    std::string ge = ctx->dev.capture_graph();
+    return ge.empty() ? LSG_OK : ctx->set(LSG_ERR_DEVICE, ge);
+  };
   if (!qk2) {
     if (ctx->dev.qk2_user) hipFree(ctx->dev.qk2_user);
     if (ctx->blk.qk2_cols) hipFree(ctx->blk.qk2_cols);
     ctx->dev.qk2_user = nullptr;
     ctx->blk.qk2_cols = nullptr;
     ctx->qk2_user_host.clear();
-    return LSG_OK;
+    return recapture();
   }
-  if (ncol < 1 || ncol > kBlockRhs) return ctx->set(LSG_ERR_ARG, "ncol must be in [1, 64]");
-  for (int64_t i = 0; i < N * ncol; ++i)
-    if (!std::isfinite(qk2[i])) return ctx->set(LSG_ERR_NUMERIC, "contrast holds NaN/Inf");
   std::string e = ctx->block_ready();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   if (!ctx->dev.qk2_user && hipMalloc((void**)&ctx->dev.qk2_user, sizeof(double) * N) != hipSuccess)
@@ -477,12 +491,13 @@ int lsg_set_contrast(lsg_ctx* ctx, const double* qk2, int ncol) {
   std::vector<double> rowmajor(size_t(N) * kBlockRhs);
   for (int64_t n = 0; n < N; ++n)
     for (int q = 0; q < kBlockRhs; ++q) rowmajor[size_t(n) * kBlockRhs + q] = qk2[size_t(q < ncol ? q : 0) * N + n];
-  hipMemcpy(ctx->dev.qk2_user, qk2, sizeof(double) * N, hipMemcpyHostToDevice);
+  if (hipMemcpy(ctx->dev.qk2_user, qk2, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+    return ctx->set(LSG_ERR_DEVICE, "copy of the contrast failed");
   ctx->qk2_user_host.assign(qk2, qk2 + N);
   if (hipMemcpy(ctx->blk.qk2_cols, rowmajor.data(), sizeof(double) * rowmajor.size(), hipMemcpyHostToDevice) !=
       hipSuccess)
     return ctx->set(LSG_ERR_DEVICE, "copy of the contrast failed");
-  return LSG_OK;
+  return recapture();
 }
 
 int lsg_attach_preconditioner(lsg_ctx* ctx, lsgn_net* net) {
@@ -679,6 +694,8 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
   if (!o || !b || !x || o->restart_len < 1 || o->restart_len > 500 || o->max_restarts < 1)
     return ctx->set(LSG_ERR_ARG, "bad gmres arguments (restart_len in [1,500], max_restarts >= 1)");
   if (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR) return ctx->set(LSG_ERR_ARG, "unknown solver mode");
+  // tol <= 0 is the reference's fixed-cycle mode; NaN or Inf is an error
+  if (!std::isfinite(o->tol)) return ctx->set(LSG_ERR_ARG, "tol must be finite");
   if (o->mode == LSG_SOLVE_FP32_IR && (!(o->tol > 0) || !(o->inner_tol > 0) || o->max_outer < 1))
     return ctx->set(LSG_ERR_ARG, "FP32_IR needs tol > 0, inner_tol > 0, max_outer >= 1");
   const int64_t N = ctx->hb.plan.N;
@@ -760,7 +777,11 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
     from_split(ctx->by, int(N), kBlockRhs, nrhs, ctx->bcm, N, s);
     for (int q = 0; q < nrhs; ++q) hipMemcpyAsync(x + 2 * ld * q, ctx->bcm + size_t(q) * N, nb, hipMemcpyDeviceToHost, s);
   }
-  hipStreamSynchronize(s);
+  hipError_t se = hipStreamSynchronize(s);
+  if (se != hipSuccess && r.status == 0) {
+    r.status = LSG_ERR_DEVICE;
+    r.error = std::string("batched GMRES: copy out failed: ") + hipGetErrorString(se);
+  }
   if (rel_residuals)
     for (int q = 0; q < nrhs; ++q) rel_residuals[q] = rel[q];
   if (stats) {

@@ -1190,7 +1190,11 @@ std::string DeviceOperator::capture_graph() {
 }
 
 void DeviceOperator::drop_graph() {
-  if (graph_exec) hipGraphExecDestroy(graph_exec);
+  if (!graph_exec) return;
+  // a replay may still be in flight after an asynchronous call
+  (void)hipStreamSynchronize(stream);
+  (void)hipStreamSynchronize(aux);
+  hipGraphExecDestroy(graph_exec);
   graph_exec = nullptr;
 }
 

@@ -138,7 +138,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
       row_index = l_local;
       getRow(row_index, row);
       col_index = max_abs(row, rem_col);
-      if (std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0)) break;
+      if (std::abs(row[col_index]) > eps) break;
     }
     if (l_local == N1) return;
     v.resize(N2);
@@ -153,7 +153,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
     rank = 1;
     double normS = 0.0, prev_normS = 1000.0;
     row_index = max_abs(col, rem_row);
-    while (row_index >= 0 && std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > eps &&
+    while (row_index >= 0 && std::abs(row[col_index]) > eps && std::abs(col[row_index]) > eps &&
            std::fabs(prev_normS - normS) >= eps * prev_normS && rank < mn) {
       out.row_bases.push_back(row_index);
       getRow(row_index, row);
@@ -188,7 +188,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
       col_index = l_local;
       getCol(col_index, col);
       row_index = max_abs(col, rem_row);
-      if (std::abs(col[row_index]) > eps * std::pow(10, -1.0 * 0)) break;
+      if (std::abs(col[row_index]) > eps) break;
     }
     if (l_local == N2) return;
     out.col_bases.push_back(col_index);
@@ -203,7 +203,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
     rem_row.erase(row_index); rem_col.erase(col_index);
     double normS = 0.0, prev_normS = 1000.0;
     col_index = max_abs(row, rem_col);
-    while (col_index >= 0 && std::abs(row[col_index]) > eps * std::pow(10, -1.0 * 0) && std::abs(col[row_index]) > eps &&
+    while (col_index >= 0 && std::abs(row[col_index]) > eps && std::abs(col[row_index]) > eps &&
            std::fabs(prev_normS - normS) >= eps * prev_normS && rank < mn) {
       out.col_bases.push_back(col_index);
       getCol(col_index, col);

@@ -40,6 +40,7 @@ def _load():
             "or_rhs": (C.c_int, [P, D]),
             "or_grid": (C.c_int, [P, D, D]),
             "or_gmres": (C.c_int, [P, D, D, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+            "or_gmres_cycle": (C.c_int, [P, D, D, C.c_int, C.POINTER(C.c_int)]),
             "or_far_table": (I64, [P, D]),
             "or_near_tables": (I64, [P, D]),
             "or_skeleton": (I64, [P, C.POINTER(C.c_int)]),
@@ -125,6 +126,14 @@ class Oracle:
         x = np.empty(self.N, dtype=np.complex128)
         it = C.c_int()
         self._lib.or_gmres(self._h, _d(b.view(np.float64)), _d(x.view(np.float64)), restart, m, C.byref(it))
+        return x, it.value
+
+    def gmres_cycle(self, b, x0, m):
+        """One GMRES(m) cycle from x0 (GMRES/GMRES.cpp:257-305); returns (x, steps)."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        x = np.array(x0, dtype=np.complex128)
+        it = C.c_int()
+        self._lib.or_gmres_cycle(self._h, _d(b.view(np.float64)), _d(x.view(np.float64)), m, C.byref(it))
         return x, it.value
 
     def far_table(self):

@@ -1005,9 +1005,12 @@ struct FMM2DTree {
 
   // ------------------------------------------------- matrix entries
   int leafOf(int j) const { return j / rank; }
-  // getMatrixEntry (G_FMM2DTree.hpp:603-658)
-  cd getMatrixEntry(int i, int j) const {
-    const int l = j / rank, chargeIndex = j % rank;
+  // getMatrixEntry (G_FMM2DTree.hpp:603-658) in two steps: the Graf series
+  // vector h of target i and source leaf l (31 values (J_n + i Y_n) e^{i n theta}),
+  // then the entry sum M(n+15, j mod 36) h_n. Rows of a block that share a
+  // source leaf reuse h; every entry is the same arithmetic in the same order
+  // as one evaluated alone.
+  void grafVector(int i, int l, cd* h) const {
     const int tj = leafNodes[l].x, tk = leafNodes[l].y;
     const FMM2DBox& B = tree[tj][tk];
     const double polar_angle_i = std::atan2(gpy[i] - B.cy, gpx[i] - B.cx);
@@ -1016,20 +1019,42 @@ struct FMM2DTree {
     // J_n, Y_n of one argument computed once for all orders
     double J[16], Y[16];
     jy_all(R, 15, J, Y);
-    cd tempSum(0.0, 0.0);
     for (int n = -15; n <= 15; ++n) {
       const int a = n < 0 ? -n : n;
       double jn = J[a], yn = Y[a];
       if (n < 0 && (a % 2)) { jn = -jn; yn = -yn; }
-      cd temp = (jn + I * yn) * std::exp(I * double(n) * polar_angle_i);
-      tempSum += M[tj](n + 15, chargeIndex) * temp;
+      h[n + 15] = (jn + I * yn) * std::exp(I * double(n) * polar_angle_i);
     }
+  }
+  cd entryFromGraf(const cd* h, int i, int j) const {
+    const int l = j / rank, chargeIndex = j % rank;
+    const Mat& Ml = M[leafNodes[l].x];
+    cd tempSum(0.0, 0.0);
+    for (int n = -15; n <= 15; ++n) tempSum += Ml(n + 15, chargeIndex) * h[n + 15];
     tempSum *= I / 4.0;
     if (findPhi) {
       tempSum *= contrastK2[i];
       if (i == j) tempSum += 1.0;
     }
     return tempSum;
+  }
+  cd getMatrixEntry(int i, int j) const {
+    cd h[31];
+    grafVector(i, j / rank, h);
+    return entryFromGraf(h, i, j);
+  }
+  // K(i, cols[k]) for all k, one Graf vector per run of columns in one leaf
+  void entriesOfRow(int i, const std::vector<int>& cols, cd* out) const {
+    cd h[31];
+    int last = -1;
+    for (size_t k = 0; k < cols.size(); ++k) {
+      const int l = cols[k] / rank;
+      if (l != last) {
+        grafVector(i, l, h);
+        last = l;
+      }
+      out[k] = entryFromGraf(h, i, cols[k]);
+    }
   }
   // getMatrixEntry2 (G_FMM2DTree.hpp:518-601): near-field table if the leaves
   // touch (colleagues in a uniform tree), else the far-field entry.
@@ -1088,7 +1113,7 @@ struct FMM2DTree {
   }
   Vec getRow(int j, const std::vector<int>& col_indices) const {
     Vec row(col_indices.size());
-    for (size_t k = 0; k < col_indices.size(); ++k) row[k] = getMatrixEntry(j, col_indices[k]);
+    entriesOfRow(j, col_indices, row.data());
     return row;
   }
   Vec getCol(int k, const std::vector<int>& row_indices) const {
@@ -1098,8 +1123,11 @@ struct FMM2DTree {
   }
   Mat getMatrix(const std::vector<int>& r, const std::vector<int>& c) const {
     Mat m(int(r.size()), int(c.size()));
-    for (size_t j = 0; j < r.size(); ++j)
-      for (size_t k = 0; k < c.size(); ++k) m(int(j), int(k)) = getMatrixEntry(r[j], c[k]);
+    std::vector<cd> row(c.size());
+    for (size_t j = 0; j < r.size(); ++j) {
+      entriesOfRow(r[j], c, row.data());
+      for (size_t k = 0; k < c.size(); ++k) m(int(j), int(k)) = row[k];
+    }
     return m;
   }
 
@@ -1784,6 +1812,22 @@ int or_gmres(or_ctx* c, const double* b, double* x, int restart, int m, int* ite
   for (int i = 0; i < N; ++i) bb[i] = cd(b[2 * i], b[2 * i + 1]);
   int it = 0;
   c->t.RestartedGMRES(restart, m, bb, sol, &it);
+  for (int i = 0; i < N; ++i) { x[2 * i] = sol[i].real(); x[2 * i + 1] = sol[i].imag(); }
+  if (iters) *iters = it;
+  return 0;
+}
+
+// One GMRES(m) cycle from the initial guess held in x (GMRES/GMRES.cpp:257-305);
+// x receives the new iterate.
+int or_gmres_cycle(or_ctx* c, const double* b, double* x, int m, int* iters) {
+  const int N = c->t.N;
+  oracle::Vec bb(N), x0(N), sol;
+  for (int i = 0; i < N; ++i) {
+    bb[i] = cd(b[2 * i], b[2 * i + 1]);
+    x0[i] = cd(x[2 * i], x[2 * i + 1]);
+  }
+  int it = 0;
+  c->t.GMRES(x0, m, bb, sol, &it);
   for (int i = 0; i < N; ++i) { x[2 * i] = sol[i].real(); x[2 * i + 1] = sol[i].imag(); }
   if (iters) *iters = it;
   return 0;

@@ -128,6 +128,20 @@ def test_edge_inputs_rejected():
     assert e.value.code == L.LSG_ERR_NUMERIC
     with pytest.raises(ValueError):
         S.MatVecProduct(np.zeros(S.N - 1, complex))
+    # every solve wrapper checks lengths before the C side reads N entries
+    b = S.rhs()
+    short = np.ones(S.N - 1, complex)
+    for call in (lambda: S.gmres(short), lambda: S.gmres(b, x0=short), lambda: S.gmres_ex(short),
+                 lambda: S.gmres_ex(b, x0=short, mode="fp32ir"), lambda: S.gmres_batch(np.ones((S.N + 1, 2))),
+                 lambda: S.gmres_batch(np.stack([b, b], 1), X0=np.zeros((S.N, 3))),
+                 lambda: S.MatVecProduct_fp32(short), lambda: L.gmres_ragged([S], [short]),
+                 lambda: L.gmres_ragged([S], [b, b])):
+        with pytest.raises(ValueError):
+            call()
+    # a non-finite tolerance is an argument error, not the fixed-cycle mode
+    with pytest.raises(L.LsgError) as e:
+        S.gmres_ex(b, tol=float("nan"))
+    assert e.value.code == L.LSG_ERR_ARG
 
 
 def test_full_size_c2_linearity_and_determinism():

@@ -40,6 +40,24 @@ def test_gmres_with_graph_matches_without():
     assert s0["iterations"] == s1["iterations"] and np.array_equal(x0, x1)
 
 
+def test_set_contrast_after_graph_capture():
+    """A contrast set or removed after use_graph(True) reaches the replayed
+    matvec: the graph is captured again around the new contrast buffer."""
+    S = make(H.inputs(depth=4, kappa=20.0, q=6, seed=5, find_phi=0))
+    x = H.rand_vec(S.N, 6)
+    v = S.MatVecProduct(x)                           # V x (no contrast yet)
+    qk2 = np.abs(H.rand_vec(S.N, 7).real) * 100.0
+    S.use_graph(True)
+    S.set_contrast(qk2)
+    assert H.relerr(S.MatVecProduct(x), x + qk2 * v) < 1e-13
+    S.set_contrast(2.0 * qk2)
+    assert H.relerr(S.MatVecProduct(x), x + 2.0 * qk2 * v) < 1e-13
+    S.set_contrast(None)
+    assert np.array_equal(S.MatVecProduct(x), v)
+    S.use_graph(False)
+    assert np.array_equal(S.MatVecProduct(x), v)
+
+
 def test_autotune_keeps_the_operator():
     inp = H.inputs(depth=4, kappa=30.0, q=6, seed=4)
     S = make(inp)
