@@ -235,6 +235,20 @@ int lsg_time_matvec_block_fp32(lsg_ctx* ctx, int reps, double* ms);
  * inner solve; the refinement loop itself has no reference counterpart. */
 int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts* opts, lsg_stats* stats);
 
+/* Allocation-free variant of lsg_gmres_ex (mode LSG_SOLVE_FP64, device
+ * pointers). No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops).
+ * b_dev and x_dev are device addresses of N complex128 (x holds the initial
+ * guess); the Krylov workspace comes from one scratch array of 128 MiB per
+ * device, held by the library until the process ends, and nothing is
+ * allocated or freed. restart_len * N is bounded by that size (about 8.3 M
+ * complex128 in total: GMRES(50) up to N = 161,000); larger requests fail
+ * with LSG_ERR_ARG. Every call on one device shares its array, so calls that
+ * overlap on one device (from two host threads or two contexts) race. Same
+ * iterates and statistics as lsg_gmres_ex. Synchronous. */
+int lsg_gmres_noalloc(lsg_ctx* ctx, const void* b_dev, void* x_dev, int restart_len, int max_restarts, double tol,
+                      lsg_stats* stats);
+
 /* Batched GMRES for nrhs <= 64 systems A x_q = b_q sharing the operator (the
  * reference solves one right-hand side per RestartedGMRES call,
  * GMRES/GMRES.cpp:307-319): b and x hold nrhs columns at b + 2*ld*q (host, or

@@ -129,6 +129,7 @@ SYMBOLS = [
     ("lsg_grid_resample", C.c_int, [_P, _D, _D, C.c_int]),
     ("lsg_gmres_learned", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.c_int, C.POINTER(lsg_stats)]),
     ("lsg_gmres_ex", C.c_int, [_P, _P, _P, C.POINTER(lsg_solve_opts), C.POINTER(lsg_stats)]),
+    ("lsg_gmres_noalloc", C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_double, C.POINTER(lsg_stats)]),
     ("lsg_gmres_batch", C.c_int, [_P, _P, _P, C.c_int, C.c_int64, C.POINTER(lsg_solve_opts), _D,
                                   C.POINTER(lsg_stats)]),
     ("lsg_host_build", C.c_int, [C.POINTER(lsg_inputs), C.POINTER(_P)]),
@@ -497,6 +498,19 @@ class DFMM:
             self._check(rc)
         stats["status"] = rc
         return x, stats
+
+    def gmres_noalloc(self, b_dev, x_dev, restart_len=50, max_restarts=20, tol=1e-8):
+        """lsg_gmres_noalloc: FP64 GMRES on device vectors (integer addresses of
+        N complex128; x holds the initial guess) with the library's scratch array
+        as workspace, allocating nothing; returns the stats dict."""
+        st = lsg_stats()
+        rc = self._lib.lsg_gmres_noalloc(self._ctx, C.c_void_p(b_dev), C.c_void_p(x_dev), restart_len, max_restarts,
+                                         tol, C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in lsg_stats._fields_}
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats["status"] = rc
+        return stats
 
     def gmres_batch(self, B, X0=None, restart_len=50, max_restarts=20, tol=1e-8, device_ptrs=None, ld=None,
                     nrhs=None, mode="fp64", inner_tol=1e-5, max_outer=20):

@@ -149,6 +149,11 @@ struct lsg_ctx {
   std::string err;
   double2* dx = nullptr;
   double2* dy = nullptr;
+  // allocation-free GMRES (lsg_gmres_noalloc): the device's scratch array,
+  // looked up at creation, and the host copy of its per-step estimates
+  void* scratch = nullptr;
+  std::string scratch_err;
+  std::vector<double> noalloc_host;
   double* bx = nullptr;       // two row-split blocks (N x 64) for block calls
   double* by = nullptr;
   double2* bcm = nullptr;     // column-major staging for up to 64 host vectors
@@ -267,6 +272,10 @@ static int finish_create(std::unique_ptr<lsg_ctx> ctx, lsg_ctx** out) {
     g_create_error = "hipMalloc of matvec buffers failed";
     return LSG_ERR_DEVICE;
   }
+  // the allocation-free entry point's workspace: the scratch array's address
+  // on this device (once per device) and a host buffer for its estimates
+  ctx->scratch_err = scratch_workspace(ctx->dev.device, &ctx->scratch);
+  ctx->noalloc_host.assign(2 * (500 + 2), 0.0);
   *out = ctx.release();
   return LSG_OK;
 }
@@ -731,6 +740,37 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
     stats->est_residual = r.est_residual;
     stats->seconds = r.seconds;
     stats->refinements = r.refinements;
+  }
+  if (r.status != 0) return ctx->set(r.status, r.error);
+  return LSG_OK;
+}
+
+int lsg_gmres_noalloc(lsg_ctx* ctx, const void* b_dev, void* x_dev, int restart_len, int max_restarts, double tol,
+                      lsg_stats* stats) {
+  if (!ctx) return LSG_ERR_ARG;
+  if (!b_dev || !x_dev || restart_len < 1 || restart_len > 500 || max_restarts < 1 || !std::isfinite(tol))
+    return ctx->set(LSG_ERR_ARG, "bad gmres arguments (device b and x, restart_len in [1,500], max_restarts >= 1, "
+                                 "finite tol)");
+  if (!ctx->scratch) return ctx->set(LSG_ERR_DEVICE, "no scratch workspace on this device: " + ctx->scratch_err);
+  const int N = int(ctx->hb.plan.N);
+  const size_t need = gmres_work_bytes(N, restart_len);
+  if (need > kScratchBytes)
+    return ctx->set(LSG_ERR_ARG, "GMRES(" + std::to_string(restart_len) + ") on " + std::to_string(N) +
+                                     " unknowns needs " + std::to_string(need) + " bytes of scratch; the maximum is " +
+                                     std::to_string(kScratchBytes) + " (use lsg_gmres_ex)");
+  GmresWork W;
+  std::string e = W.carve(ctx->scratch, kScratchBytes, N, restart_len, ctx->noalloc_host.data());
+  if (!e.empty()) return ctx->set(LSG_ERR_ARG, e);
+  GmresResult r = device_gmres(ctx->dev, static_cast<const double2*>(b_dev), static_cast<double2*>(x_dev), restart_len,
+                               max_restarts, tol, false, nullptr, &W);
+  if (stats) {
+    stats->iterations = r.iterations;
+    stats->restarts = r.restarts;
+    stats->converged = r.converged;
+    stats->rel_residual = r.rel_residual;
+    stats->est_residual = r.est_residual;
+    stats->seconds = r.seconds;
+    stats->refinements = 0;
   }
   if (r.status != 0) return ctx->set(r.status, r.error);
   return LSG_OK;

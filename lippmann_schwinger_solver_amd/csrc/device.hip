@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <type_traits>
@@ -1392,8 +1393,86 @@ std::string GmresWork::reserve(int N_, int m_, bool flexible) {
   return "";
 }
 
+// ------------------------------------------------ allocation-free workspace
+// One array per device for the whole process (the library's scratch): calls
+// that use it on one device must not overlap. Its host-side name stands for
+// no device address; scratch_workspace() returns the address on a device.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::string scratch_workspace(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  if (device < 0 || device >= 64) return "device index out of range";
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    HIP_OK(hipGetDevice(&cur));
+    if (cur != device) return "scratch_workspace: the device must be current";
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return "hipGetSymbolAddress returned no address for the scratch array";
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return "";
+}
+
+namespace {
+// sub-buffer sizes of a GMRES(m) workspace, each rounded to 256 bytes
+struct WorkLayout {
+  size_t off[13] = {};
+  size_t total = 0;
+  WorkLayout(int N, int m) {
+    const int nrb = (N + kGsRows - 1) / kGsRows;
+    const size_t sz[13] = {sizeof(double2) * size_t(N) * (m + 1),   // V
+                           sizeof(double2) * size_t(N),             // w
+                           sizeof(double2) * size_t(nrb) * (m + 2), // partial
+                           sizeof(double2) * (m + 2),               // d1
+                           sizeof(double2) * (m + 2),               // d2
+                           sizeof(double2) * (m + 1),               // y
+                           sizeof(double2) * size_t(m + 1) * m,     // H
+                           sizeof(double) * (m + 1),                // cs
+                           sizeof(double2) * (m + 1),               // sn
+                           sizeof(double2) * (m + 2),               // g
+                           sizeof(double),                          // hn
+                           sizeof(double),                          // beta
+                           sizeof(double) * 2 * (m + 2)};           // hostbuf (device side)
+    for (int i = 0; i < 13; ++i) {
+      off[i] = total;
+      total += (sz[i] + 255) / 256 * 256;
+    }
+  }
+};
+}  // namespace
+
+size_t gmres_work_bytes(int N, int m) { return WorkLayout(N, m).total; }
+
+std::string GmresWork::carve(void* base, size_t cap_bytes, int N_, int m_, double* mirror) {
+  const WorkLayout L(N_, m_);
+  if (L.total > cap_bytes) return "workspace does not fit";
+  char* b = static_cast<char*>(base);
+  V = reinterpret_cast<double2*>(b + L.off[0]);
+  w = reinterpret_cast<double2*>(b + L.off[1]);
+  partial = reinterpret_cast<double2*>(b + L.off[2]);
+  d1 = reinterpret_cast<double2*>(b + L.off[3]);
+  d2 = reinterpret_cast<double2*>(b + L.off[4]);
+  y = reinterpret_cast<double2*>(b + L.off[5]);
+  H = reinterpret_cast<double2*>(b + L.off[6]);
+  cs = reinterpret_cast<double*>(b + L.off[7]);
+  sn = reinterpret_cast<double2*>(b + L.off[8]);
+  g = reinterpret_cast<double2*>(b + L.off[9]);
+  hn = reinterpret_cast<double*>(b + L.off[10]);
+  beta = reinterpret_cast<double*>(b + L.off[11]);
+  hostbuf = reinterpret_cast<double*>(b + L.off[12]);
+  host_mirror = mirror;
+  Z = nullptr;
+  N = N_;
+  m = m_;
+  return "";
+}
+
 GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts,
-                         double tol, bool fp32, Precond* prec) {
+                         double tol, bool fp32, Precond* prec, GmresWork* ext) {
   GmresResult res;
   auto t0 = std::chrono::steady_clock::now();
   const int N = op.N;
@@ -1411,7 +1490,12 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
   const int nrb = (N + kGsRows - 1) / kGsRows;   // row blocks of the dot kernels
   const int nb_vec = (N + kThreads - 1) / kThreads;
   const long long ldv = N;
-  {
+  if (ext && (ext->N != N || ext->m != m || prec)) {
+    res.status = LSG_ERR_ARG;
+    res.error = "external GMRES workspace does not match (N, m), or has no room for a preconditioner";
+    return res;
+  }
+  if (!ext) {
     std::string e = op.work.reserve(N, m, prec != nullptr);
     if (!e.empty()) {
       op.work.release();
@@ -1420,10 +1504,18 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
       return res;
     }
   }
-  GmresWork& W = op.work;
+  GmresWork& W = ext ? *ext : op.work;
   double2 *V = W.V, *w = W.w, *partial = W.partial, *d1 = W.d1, *d2 = W.d2, *y = W.y;
   GivensState st{W.H, W.cs, W.sn, W.g, W.hn, W.beta};
+  // the kernels write hostbuf; the host reads hb (the same pinned buffer, or
+  // a host copy of a device buffer made after each synchronisation)
   double* hostbuf = W.hostbuf;
+  const double* hb = W.host_mirror ? W.host_mirror : W.hostbuf;
+  auto pull = [&](int n) -> hipError_t {
+    if (!W.host_mirror) return hipSuccess;
+    hipError_t pe = hipMemcpyAsync(W.host_mirror, W.hostbuf, sizeof(double) * size_t(n), hipMemcpyDeviceToHost, s);
+    return pe != hipSuccess ? pe : hipStreamSynchronize(s);
+  };
   auto matvec = [&](const double2* xin, double2* yout) {
     if (fp32) op.matvec32(xin, yout);
     else op.matvec(xin, yout);
@@ -1459,12 +1551,12 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
     hipLaunchKernelGGL(sub_kernel, dim3(nb_vec), dim3(kThreads), 0, s, b_dev, w, w, N);
     dot(0, w, true);
     hipLaunchKernelGGL(gs_cycle_init_kernel, dim3(1), dim3(kThreads), 0, s, partial, nrb, st, m, hostbuf);
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+    if ((e = hipStreamSynchronize(s)) != hipSuccess || (e = pull(1)) != hipSuccess) {
       res.status = LSG_ERR_DEVICE;
       res.error = hipGetErrorString(e);
       break;
     }
-    const double beta = hostbuf[0];
+    const double beta = hb[0];
     res.rel_residual = beta / bnorm;
     if (!std::isfinite(beta)) {
       res.status = LSG_ERR_NUMERIC;
@@ -1512,7 +1604,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
         hipLaunchKernelGGL(gs_update_kernel, dim3(nb_vec), dim3(kThreads), 0, s, V, ldv, nc, d2, w, st.hn,
                            V + (size_t)(jj + 1) * N, N);
       }
-      if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+      if ((e = hipStreamSynchronize(s)) != hipSuccess || (e = pull(2 * (j + batch))) != hipSuccess) {
         res.status = LSG_ERR_DEVICE;
         res.error = hipGetErrorString(e);
         break;
@@ -1520,7 +1612,7 @@ GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_de
       const double est_before = last_est;
       int n_read = 0;
       for (int jj = j; jj < j + batch; ++jj) {
-        const double est = hostbuf[2 * jj], hn = hostbuf[2 * jj + 1];
+        const double est = hb[2 * jj], hn = hb[2 * jj + 1];
         ++total_it;
         ++n_read;
         k = jj + 1;

@@ -28,9 +28,23 @@ struct GmresWork {
   double2 *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *beta = nullptr;
   double* hostbuf = nullptr;  // pinned: per-step residual estimates and norms
+  // Set when the workspace lives in device memory the caller provides (carve):
+  // hostbuf is then a device buffer, copied here after each synchronisation.
+  double* host_mirror = nullptr;
   std::string reserve(int N, int m, bool flexible = false);
+  // point every buffer into [base, base + cap_bytes) (device memory, not owned);
+  // host_mirror must hold 2 (m + 2) doubles; fails if the workspace does not fit
+  std::string carve(void* base, size_t cap_bytes, int N, int m, double* host_mirror);
   void release();
 };
+// bytes carve() needs for GMRES(m) on N unknowns
+size_t gmres_work_bytes(int N, int m);
+
+// The library's allocation-free workspace on `device`: the device address of a
+// file-scope array of kScratchBytes, looked up once per device (the device
+// must be current at the first call).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+std::string scratch_workspace(int device, void** ptr);
 
 // Right preconditioner of flexible GMRES: z = M(v), device vectors of N
 // complex128, enqueued on stream s (M may be nonlinear).
@@ -122,8 +136,10 @@ struct GmresResult {
 };
 // fp32: every matvec of the Krylov process uses the FP32 operators (FP64 vectors,
 // Gram-Schmidt and Givens)
+// ext: a workspace carved from caller-provided memory (nothing is allocated);
+// otherwise op.work, reserved on first use
 GmresResult device_gmres(DeviceOperator& op, const double2* b_dev, double2* x_dev, int m, int max_restarts, double tol,
-                         bool fp32 = false, Precond* prec = nullptr);
+                         bool fp32 = false, Precond* prec = nullptr, GmresWork* ext = nullptr);
 // FP64 iterative refinement around FP32-operator GMRES: r = b - A x with the FP64
 // operator, d ~ A32^{-1} r by GMRES(m) to relative residual max(inner_tol, tol/2 ||b||/||r||),
 // x += d; stops when ||b - A x|| <= tol ||b|| (FP64) or after max_outer refinements.
