@@ -545,66 +545,26 @@ def bench_readme(args):
 
 
 def bench_learned(args):
-    """The learned preconditioner at the configuration it is specified for:
-    trained on 4096 solves the solver generates (depth 6, k in {40, 50, 60, 70,
-    80}, 3-8 seeded bumps, plane waves at random angles and random-field right-hand
-    sides), evaluated on 256 held-out systems against plain GMRES."""
+    """The learned preconditioner at the configuration it is specified for
+    (learned.c5_experiment): trained on 4096 solves the solver generates,
+    evaluated on held-out systems (256 by default) against plain GMRES.
+    --nets lists the dilation sets to train on the same data."""
     from lippmann_schwinger_solver_amd import learned as LN
-    kappas = [40.0, 50.0, 60.0, 70.0, 80.0]
-    net = LN.LearnedNet(n_layers=6, width=32, dilations=(1, 2, 4, 8, 4, 1), grid=128, max_batch=64, seed=0)
-    t0 = time.time()
-    ctxs = {k: LN.volume_context(k, 6, net) for k in kappas}
-    log(f"volume potentials for {len(kappas)} wavenumbers: {time.time() - t0:.1f} s")
-    n_train, n_eval = 4096, 256
-    per = n_train // len(kappas)
-    xs, ts = [], []
-    t0 = time.time()
-    for i, k in enumerate(kappas):
-        n = per if i < len(kappas) - 1 else n_train - per * (len(kappas) - 1)
-        x, t, _ = LN.make_pairs(ctxs[k], k, list(range(i * 100_000, i * 100_000 + n)), tol=args.tol)
-        xs.append(x)
-        ts.append(t)
-    X, T = np.concatenate(xs), np.concatenate(ts)
-    t_data = time.time() - t0
-    log(f"training set: {X.shape[0]} solves in {t_data:.1f} s")
-    net.set_training_data(X, T)
-    t0 = time.time()
-    hist = net.train(epochs=30, batch=64, lr=2e-3, lr_decay=0.92, log=log)
-    t_train = time.time() - t0
-    ev = {"plain_it": [], "learned_it": [], "guess_only_it": [], "plain_s": [], "learned_s": [], "guess_only_s": [],
-          "initial_residual_learned": [], "max_rel_residual": 0.0, "failed": 0}
-    for i, k in enumerate(kappas):
-        log(f"evaluating kappa {k}")
-        e = LN.evaluate(ctxs[k], k, list(range(900_000 + i * 1000, 900_000 + i * 1000 + n_eval // len(kappas) +
-                                               (1 if i < n_eval % len(kappas) else 0))), tol=args.tol)
-        for key in ev:
-            if key == "max_rel_residual":
-                ev[key] = max(ev[key], e[key])
-            elif key == "failed":
-                ev[key] += e[key]
-            else:
-                ev[key] += e[key]
-    res = {"train_solves": int(X.shape[0]), "train_data_s": t_data, "train_s": t_train, "loss_first_epoch": hist[0],
-           "loss_last_epoch": hist[-1], "eval_systems": len(ev["plain_it"]),
-           "plain_mean_iterations": float(np.mean(ev["plain_it"])),
-           "learned_fgmres_mean_iterations": float(np.mean(ev["learned_it"])),
-           "learned_guess_only_mean_iterations": float(np.mean(ev["guess_only_it"])),
-           "plain_solves_per_s": len(ev["plain_s"]) / float(np.sum(ev["plain_s"])),
-           "learned_fgmres_solves_per_s": len(ev["learned_s"]) / float(np.sum(ev["learned_s"])),
-           "learned_guess_only_solves_per_s": len(ev["guess_only_s"]) / float(np.sum(ev["guess_only_s"])),
-           "mean_initial_residual_learned_guess": float(np.mean(ev["initial_residual_learned"])),
-           "max_rel_residual": ev["max_rel_residual"], "solves_not_at_tol": ev["failed"]}
-    log(f"learned: {res}")
-    print(json.dumps({"workload": "learned preconditioner (C5)", "result": res}), flush=True)
+    nets_spec = tuple(tuple(int(d) for d in spec.split(",")) for spec in args.nets.split(";"))
+    for res in LN.c5_experiment(nets_spec, n_eval=args.n_eval, epochs=args.epochs, tol=args.tol, log=log):
+        res.pop("per_system")
+        log(f"learned {res['dilations']}: {res}")
+        print(json.dumps({"workload": "learned preconditioner (C5)", "result": res}), flush=True)
 
 
 def bench_ragged(args):
     """Ragged batch: 48 independent scatterers, tree depth uniform in {4, 5, 6},
-    kappa uniform in [20, 80], 3-8 seeded bumps and a seeded angle each;
-    autotuned kernel variants, graph-captured matvecs, solved concurrently
-    (lsg_gmres_ragged) and, for comparison, one after the other."""
+    kappa uniform in [20, 80], 3-8 seeded bumps and a seeded angle each. The
+    fused batch (one plan, one launch sequence per matvec, hipGraph-captured,
+    autotuned, per-system Krylov bases) against the same systems solved one
+    after the other and by concurrent contexts (one host thread each)."""
     rng = np.random.default_rng(2024)
-    solvers, rhs, cases = [], [], []
+    hosts, solvers, rhs, cases = [], [], [], []
     t0 = time.time()
     for i in range(48):
         depth, kappa, nb = int(rng.integers(4, 7)), float(rng.uniform(20, 80)), int(rng.integers(3, 9))
@@ -612,31 +572,42 @@ def bench_ragged(args):
                                nCones_LFR=16)
         inp.contrast = lsg.bumps_from_seed(100 + i, nb)
         inp.inc_angle = seeded_angle(100 + i)
+        h = lsg.HostBuild(inp)
         S = lsg.DFMM(inp)
-        S.FMMTreeInitialisation()
+        S.FMMTreeInitialisation(host=h)
         S.autotune(reps=2)
         S.use_graph(True)
+        hosts.append(h)
         solvers.append(S)
         rhs.append(S.rhs())
         cases.append((depth, round(kappa, 2), nb))
-        log(f"scatterer {i}: depth {depth} kappa {kappa:.1f} bumps {nb} N {S.N} ({time.time() - t0:.0f} s)")
-    lsg.gmres_ragged(solvers, rhs, tol=args.tol)          # warm-up
+    log(f"48 scatterers built in {time.time() - t0:.0f} s")
+    R = lsg.RaggedBatch(hosts)
+    R.autotune(reps=2)
+    R.use_graph(True)
+    R.gmres(rhs, tol=args.tol)                              # warm-up
     t0 = time.perf_counter()
-    xs, stats, rc = lsg.gmres_ragged(solvers, rhs, tol=args.tol)
-    t_con = time.perf_counter() - t0
+    xs, stats, rc = R.gmres(rhs, tol=args.tol)
+    t_fused = time.perf_counter() - t0
     if rc != 0:
-        raise RuntimeError(f"ragged batch did not verify: {[s['rel_residual'] for s in stats]}")
+        raise RuntimeError(f"fused ragged batch did not verify: {[s['rel_residual'] for s in stats]}")
+    worst = max(float(np.linalg.norm(S.MatVecProduct(x) - b) / np.linalg.norm(b)) for S, b, x in zip(solvers, rhs, xs))
+    lsg.gmres_ragged(solvers, rhs, tol=args.tol)            # warm-up
+    t0 = time.perf_counter()
+    _, st_con, rc_con = lsg.gmres_ragged(solvers, rhs, tol=args.tol)
+    t_con = time.perf_counter() - t0
     t0 = time.perf_counter()
     for S, b in zip(solvers, rhs):
         S.gmres(b, restart_len=50, max_restarts=20, tol=args.tol)
     t_seq = time.perf_counter() - t0
-    worst = max(float(np.linalg.norm(S.MatVecProduct(x) - b) / np.linalg.norm(b)) for S, b, x in zip(solvers, rhs, xs))
-    out = {"workload": "C4b ragged batch: 48 scatterers, depth {4,5,6}, kappa [20,80], 3-8 bumps",
-           "solves_per_s_concurrent": 48 / t_con, "solves_per_s_sequential": 48 / t_seq,
-           "concurrency_speedup": t_seq / t_con, "max_rel_residual": max(s["rel_residual"] for s in stats),
-           "recomputed_max_rel_residual": worst, "total_unknowns": int(sum(S.N for S in solvers)),
-           "iterations": [s["iterations"] for s in stats], "cases": cases}
+    out = {"workload": "C4b ragged batch: 48 scatterers, depth {4,5,6}, kappa [20,80], 3-8 bumps, tol %g" % args.tol,
+           "solves_per_s_fused": 48 / t_fused, "solves_per_s_concurrent_contexts": 48 / t_con,
+           "solves_per_s_sequential": 48 / t_seq, "fused_speedup_vs_sequential": t_seq / t_fused,
+           "concurrent_speedup_vs_sequential": t_seq / t_con, "fused_matvec_ms": R.time_matvec(reps=5),
+           "max_rel_residual": max(s["rel_residual"] for s in stats), "recomputed_max_rel_residual": worst,
+           "total_unknowns": int(sum(R.sizes)), "iterations": [s["iterations"] for s in stats], "cases": cases}
     print(json.dumps(out), flush=True)
+    R.close()
 
 
 def main():
@@ -647,6 +618,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones (default 1; c2: 2)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--restart", type=int, default=30, help="restart length of the batched solves")
+    ap.add_argument("--nets", default="1,2,4,8,4,1",
+                    help="learned workload: dilation sets of the networks to train, ';'-separated")
+    ap.add_argument("--n-eval", type=int, default=256, help="learned workload: held-out systems")
+    ap.add_argument("--epochs", type=int, default=30, help="learned workload: training epochs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--pmc-probe-block", action="store_true",

@@ -85,6 +85,11 @@ typedef struct lsg_inputs {
                           else read them from <dir>/precomputations/M and /Neighbor in the
                           reference's cache layout and text format (G_FMM2DTree.hpp:1301-1313,
                           1911-1950); a missing or malformed file fails lsg_create */
+  int aca_exit_pow;    /* 0: the reference's absolute exit of the ACA on the new cross vectors'
+                          norms, 1e-8 (GMRES/ACA.hpp:350, 453); p in [8, 20]: 10^-p instead. That
+                          fixed exit, not TOL_POW, bounds how close the DAFMM comes to the dense
+                          operator (5.7e-6 at depth 3 for any TOL_POW); with TOL_POW 14 and p 16 the
+                          DAFMM matches the dense matvec to ~1e-12 */
 } lsg_inputs;
 
 /* Work and size figures of a built operator (for roofline accounting). */

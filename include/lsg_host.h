@@ -100,6 +100,34 @@ int lsg_host_entries(const lsg_host* h, const int* rows, int nr, const int* cols
 int lsg_host_aca(const lsg_host* h, const int* rows, int n1, const int* cols, int n2, int device, int* rank,
                  int* row_piv, int* col_piv);
 
+/* Fused ragged batch (no reference counterpart: the reference builds one tree
+ * and solves one system per RestartedGMRES call, GMRES/GMRES.cpp:152-199,
+ * 307-319). n host builds of independent scatterers (uniform trees, any
+ * depth, wavenumber and contrast, the same nChebNodes) become ONE plan on the
+ * current device: one state vector, each matvec pass covering every system,
+ * the near field on MFMA with one table set per system. Vectors of system i
+ * are N_i complex128 (lsg_ragged_size), host arrays, leaf-major as
+ * everywhere. Errors of lsg_ragged_create in lsg_create_error(). */
+typedef struct lsg_ragged lsg_ragged;
+int lsg_ragged_create(const lsg_host* const* hosts, int n, lsg_ragged** out);
+void lsg_ragged_destroy(lsg_ragged* r);
+const char* lsg_ragged_error(const lsg_ragged* r);
+/* N_i of system i; i = -1: the total */
+int64_t lsg_ragged_size(const lsg_ragged* r, int i);
+/* y_i = A_i x_i for every system with one fused matvec */
+int lsg_ragged_matvec(lsg_ragged* r, const double* const* x, double* const* y);
+/* GMRES(restart_len) of every system at once, each with its own Krylov
+ * basis, Givens rotations and stopping point (estimate at tol/2, breakdown),
+ * every cycle opening with its true FP64 residual: x_i holds the initial
+ * guess and receives the solution; stats (n entries, may be NULL) per system.
+ * LSG_ERR_NOCONV if a system missed tol after max_restarts cycles. */
+int lsg_ragged_gmres(lsg_ragged* r, const double* const* b, double* const* x, int restart_len, int max_restarts,
+                     double tol, lsg_stats* stats);
+/* lsg_autotune / lsg_use_graph / lsg_time_matvec for the fused operator */
+int lsg_ragged_autotune(lsg_ragged* r, int reps, int* n_changed);
+int lsg_ragged_use_graph(lsg_ragged* r, int on);
+int lsg_ragged_time_matvec(lsg_ragged* r, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

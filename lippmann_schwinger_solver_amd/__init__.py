@@ -47,7 +47,7 @@ class lsg_inputs(C.Structure):
                 ("degreeOfBases", C.c_int), ("TOL_POW", C.c_int), ("m", C.c_int), ("restart", C.c_int),
                 ("nLevelsUniform", C.c_int), ("Qchoice", C.c_int), ("tree_mode", C.c_int),
                 ("find_phi", C.c_int), ("inc_angle", C.c_double), ("contrast", lsg_contrast),
-                ("setup_device", C.c_int), ("precomputations_dir", C.c_char_p)]
+                ("setup_device", C.c_int), ("precomputations_dir", C.c_char_p), ("aca_exit_pow", C.c_int)]
 
 
 class lsg_info(C.Structure):
@@ -155,6 +155,16 @@ SYMBOLS = [
     ("lsg_host_plan_dests", C.c_int64, [_P, C.POINTER(C.c_int64)]),
     ("lsg_host_plan_nbr", C.c_int64, [_P, C.POINTER(C.c_int)]),
     ("lsg_host_plan_qk2", C.c_int64, [_P, _D]),
+    ("lsg_ragged_create", C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P)]),
+    ("lsg_ragged_destroy", None, [_P]),
+    ("lsg_ragged_error", C.c_char_p, [_P]),
+    ("lsg_ragged_size", C.c_int64, [_P, C.c_int]),
+    ("lsg_ragged_matvec", C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    ("lsg_ragged_gmres", C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.c_int, C.c_int, C.c_double,
+                                   C.POINTER(lsg_stats)]),
+    ("lsg_ragged_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("lsg_ragged_use_graph", C.c_int, [_P, C.c_int]),
+    ("lsg_ragged_time_matvec", C.c_int, [_P, C.c_int, _D]),
 ]
 
 _lib = None
@@ -197,7 +207,7 @@ class InputsToDFMM:
     def __init__(self, nCones_LFR=16, nChebNodes=6, treeAdaptivity=6, L=0.5, kappa=20.0, yes2DFMM=1,
                  degreeOfBases=6, TOL_POW=10, m=50, restart=1, nLevelsUniform=4, Qchoice=0,
                  tree_mode=0, find_phi=1, inc_angle=0.0, contrast=None, setup_device=0,
-                 precomputations_dir=None):
+                 precomputations_dir=None, aca_exit_pow=0):
         self.nCones_LFR = nCones_LFR
         self.nChebNodes = nChebNodes
         self.treeAdaptivity = treeAdaptivity
@@ -217,12 +227,14 @@ class InputsToDFMM:
         self.setup_device = setup_device   # 1: NCA skeletons and operator entries built on the GPU
         # read the tables from <dir>/precomputations (the reference's cache) instead of computing them
         self.precomputations_dir = precomputations_dir
+        # 0: the reference's absolute ACA exit 1e-8; p: 10^-p (see lsg_inputs.aca_exit_pow)
+        self.aca_exit_pow = aca_exit_pow
 
     def to_c(self):
         s = lsg_inputs()
         for k in ("nCones_LFR", "nChebNodes", "treeAdaptivity", "L", "kappa", "yes2DFMM", "degreeOfBases",
                   "TOL_POW", "m", "restart", "nLevelsUniform", "Qchoice", "tree_mode", "find_phi", "inc_angle",
-                  "setup_device"):
+                  "setup_device", "aca_exit_pow"):
             setattr(s, k, getattr(self, k))
         if self.precomputations_dir:
             s.precomputations_dir = str(self.precomputations_dir).encode()
@@ -689,6 +701,84 @@ def RestartedGMRES(restart, S, m, inputs, b):
     """`restart` cycles of GMRES(m) from x0 = 0 (GMRES/GMRES.cpp:307-319)."""
     x, _ = S.gmres(b, x0=None, restart_len=m, max_restarts=restart, tol=0.0)
     return x
+
+
+class RaggedBatch:
+    """Independent scatterers fused into one operator on the device
+    (lsg_ragged_*): `hosts` are HostBuild objects of uniform trees (any depth,
+    wavenumber and contrast). matvec and gmres take and return one vector per
+    system; gmres solves all systems at once, each with its own Krylov basis
+    and stopping point."""
+
+    def __init__(self, hosts):
+        self._lib = load_library()
+        self._hosts = list(hosts)
+        arr = (_P * len(self._hosts))(*[h._h for h in self._hosts])
+        self._r = C.c_void_p()
+        rc = self._lib.lsg_ragged_create(arr, len(self._hosts), C.byref(self._r))
+        if rc != 0:
+            raise LsgError(rc, self._lib.lsg_create_error().decode())
+        self.n = len(self._hosts)
+        self.sizes = [int(self._lib.lsg_ragged_size(self._r, i)) for i in range(self.n)]
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LsgError(rc, self._lib.lsg_ragged_error(self._r).decode())
+
+    def _vecs(self, vs, name):
+        if len(vs) != self.n:
+            raise ValueError(f"{name}: one vector per system ({self.n}), got {len(vs)}")
+        out = []
+        for i, (v, N) in enumerate(zip(vs, self.sizes)):
+            a = np.ascontiguousarray(v, dtype=np.complex128)
+            if a.shape != (N,):
+                raise ValueError(f"{name}[{i}] must have shape ({N},), got {a.shape}")
+            out.append(a)
+        return out
+
+    def matvec(self, xs):
+        xs = self._vecs(xs, "x")
+        ys = [np.empty(N, dtype=np.complex128) for N in self.sizes]
+        self._check(self._lib.lsg_ragged_matvec(self._r, (_P * self.n)(*[x.ctypes.data for x in xs]),
+                                                (_P * self.n)(*[y.ctypes.data for y in ys])))
+        return ys
+
+    def gmres(self, rhs, x0=None, restart_len=50, max_restarts=20, tol=1e-8):
+        """Returns (xs, per-system stats dicts, rc); rc LSG_ERR_NOCONV if a system missed tol."""
+        bs = self._vecs(rhs, "rhs")
+        xs = [np.zeros(N, dtype=np.complex128) for N in self.sizes] if x0 is None else \
+            [np.array(a) for a in self._vecs(x0, "x0")]
+        st = (lsg_stats * self.n)()
+        rc = self._lib.lsg_ragged_gmres(self._r, (_P * self.n)(*[b.ctypes.data for b in bs]),
+                                        (_P * self.n)(*[x.ctypes.data for x in xs]), restart_len, max_restarts, tol, st)
+        if rc != 0 and rc != LSG_ERR_NOCONV:
+            self._check(rc)
+        stats = [{k: getattr(st[i], k) for k, _ in lsg_stats._fields_} for i in range(self.n)]
+        return xs, stats, rc
+
+    def autotune(self, reps=3):
+        n = C.c_int()
+        self._check(self._lib.lsg_ragged_autotune(self._r, reps, C.byref(n)))
+        return n.value
+
+    def use_graph(self, on=True):
+        self._check(self._lib.lsg_ragged_use_graph(self._r, 1 if on else 0))
+
+    def time_matvec(self, reps=10):
+        ms = C.c_double()
+        self._check(self._lib.lsg_ragged_time_matvec(self._r, reps, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_r", None):
+            self._lib.lsg_ragged_destroy(self._r)
+            self._r = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gmres_ragged(solvers, rhs, restart_len=50, max_restarts=20, tol=1e-8, mode="fp64", inner_tol=1e-5, max_outer=20):

@@ -16,7 +16,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HOST_SRC = ["tables.cpp", "tree.cpp", "nca.cpp", "precomp.cpp"]
-HIP_SRC = ["device.hip", "block.hip", "learn.hip", "nca_device.hip", "field.hip", "capi.cpp"]
+HIP_SRC = ["device.hip", "block.hip", "ragged.hip", "learn.hip", "nca_device.hip", "field.hip", "capi.cpp"]
 # device code that restates the host numerics keeps their rounding (no contraction)
 HIP_NO_CONTRACT = {"nca_device.hip"}
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -46,6 +46,7 @@ struct inputsToDFMM {
   double inc_angle = 0.0;
   lsg_contrast contrast{};
   int setup_device = 0;  // 1: NCA skeletons and operator entries built on the GPU
+  int aca_exit_pow = 0;  // 0: the reference's absolute ACA exit 1e-8; p: 10^-p
 };
 
 class DFMM {
@@ -80,6 +81,7 @@ class DFMM {
     c.contrast = inputs.contrast;
     c.contrast.kind = inputs.Qchoice;
     c.setup_device = inputs.setup_device;
+    c.aca_exit_pow = inputs.aca_exit_pow;
     int rc = lsg_create(&c, &ctx_);
     if (rc != LSG_OK) throw Error(rc, lsg_create_error());
     lsg_info info;

@@ -510,6 +510,7 @@ void from_split(const double* in, int N, int nb, int nrhs, double2* x, long long
 
 std::string BlockOperator::setup(DeviceOperator& o) {
   if (op == &o && bstate) return "";
+  if (o.n_table_sets > 1) return "block matvec: the operator holds several near-field table sets (a fused plan)";
   release();
   op = &o;
   nb = kNB;

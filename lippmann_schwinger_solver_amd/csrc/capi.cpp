@@ -19,6 +19,7 @@
 #include "nca.hpp"
 #include "precomp.hpp"
 #include "problem.hpp"
+#include "ragged.hpp"
 #include "tables.hpp"
 #include "tree.hpp"
 
@@ -58,6 +59,8 @@ std::string validate(const lsg_inputs* in) {
   if (in->contrast.kind == 6 && (in->contrast.n_bumps < 0 || in->contrast.n_bumps > LSG_MAX_BUMPS))
     return "contrast.n_bumps out of range";
   if (in->setup_device != 0 && in->setup_device != 1) return "setup_device must be 0 or 1";
+  if (in->aca_exit_pow != 0 && (in->aca_exit_pow < 8 || in->aca_exit_pow > 20))
+    return "aca_exit_pow must be 0 (the reference's 1e-8) or in [8, 20]";
   return "";
 }
 
@@ -103,6 +106,7 @@ int host_build(const lsg_inputs* in, HostBuild& hb, std::string& err, bool tree_
   if (tree_only) return LSG_OK;
   BuildOptions bo;
   bo.TOL_POW = in->TOL_POW;
+  bo.aca_abs_exit = in->aca_exit_pow ? std::pow(10.0, -in->aca_exit_pow) : 1e-8;
   bo.find_phi = in->find_phi != 0;
   if (in->setup_device) bo.device = make_device_nca;
   bo.table_dir = hb.table_dir;
@@ -207,6 +211,37 @@ struct lsg_ctx {
 
 struct lsg_host {
   HostBuild hb;
+};
+
+struct lsg_ragged {
+  RaggedOperator R;
+  std::string err;
+  std::string tuning;
+  double2 *db = nullptr, *dx = nullptr;   // concatenated device vectors
+  double2* pin = nullptr;                  // pinned staging of one concatenated vector
+  int set(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+  // host vectors of every system -> concatenated device vector d (through the staging)
+  bool gather(double2* d, const double* const* v) {
+    const int n = R.n;
+    for (int i = 0; i < n; ++i)
+      std::memcpy(pin + R.off[size_t(i)], v[i], sizeof(double2) * size_t(R.off[size_t(i) + 1] - R.off[size_t(i)]));
+    return hipMemcpyAsync(d, pin, sizeof(double2) * size_t(R.off[size_t(n)]), hipMemcpyHostToDevice, R.op.stream) ==
+               hipSuccess &&
+           hipStreamSynchronize(R.op.stream) == hipSuccess;
+  }
+  bool scatter(double* const* v, const double2* d) {
+    const int n = R.n;
+    if (hipMemcpyAsync(pin, d, sizeof(double2) * size_t(R.off[size_t(n)]), hipMemcpyDeviceToHost, R.op.stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(R.op.stream) != hipSuccess)
+      return false;
+    for (int i = 0; i < n; ++i)
+      std::memcpy(v[i], pin + R.off[size_t(i)], sizeof(double2) * size_t(R.off[size_t(i) + 1] - R.off[size_t(i)]));
+    return true;
+  }
 };
 
 extern "C" {
@@ -943,6 +978,127 @@ int lsg_create_from_host(const lsg_host* h, lsg_ctx** out) {
 }
 
 void lsg_host_destroy(lsg_host* h) { delete h; }
+
+int lsg_ragged_create(const lsg_host* const* hosts, int n, lsg_ragged** out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "out is NULL"; return LSG_ERR_ARG; }
+  *out = nullptr;
+  if (!hosts || n < 1) { g_create_error = "need at least one host build"; return LSG_ERR_ARG; }
+  std::vector<const Plan*> plans;
+  for (int i = 0; i < n; ++i) {
+    if (!hosts[i]) { g_create_error = "host build " + std::to_string(i) + " is NULL"; return LSG_ERR_ARG; }
+    for (int j = 0; j < i; ++j)
+      if (hosts[j] == hosts[i]) { g_create_error = "a host build appears twice"; return LSG_ERR_ARG; }
+    if (hosts[i]->hb.in.tree_mode != 0) { g_create_error = "the fused batch needs uniform trees"; return LSG_ERR_ARG; }
+    plans.push_back(&hosts[i]->hb.plan);
+  }
+  if (int rc = device_present(); rc != LSG_OK) return rc;
+  Plan fused;
+  std::vector<int> off;
+  std::string e = fuse_plans(plans, fused, off);
+  if (!e.empty()) { g_create_error = e; return LSG_ERR_ARG; }
+  auto r = std::make_unique<lsg_ragged>();
+  e = r->R.setup(fused, off);
+  const size_t nb = sizeof(double2) * size_t(off.back());
+  if (e.empty() && (hipMalloc((void**)&r->db, nb) != hipSuccess || hipMalloc((void**)&r->dx, nb) != hipSuccess ||
+                    hipHostMalloc((void**)&r->pin, nb, hipHostMallocDefault) != hipSuccess))
+    e = "allocation of the fused vectors failed";
+  if (!e.empty()) {
+    lsg_ragged_destroy(r.release());
+    g_create_error = e;
+    return LSG_ERR_DEVICE;
+  }
+  *out = r.release();
+  return LSG_OK;
+}
+
+void lsg_ragged_destroy(lsg_ragged* r) {
+  if (!r) return;
+  r->R.release();
+  if (r->db) (void)hipFree(r->db);
+  if (r->dx) (void)hipFree(r->dx);
+  if (r->pin) (void)hipHostFree(r->pin);
+  delete r;
+}
+
+const char* lsg_ragged_error(const lsg_ragged* r) { return r ? r->err.c_str() : "ragged batch is NULL"; }
+
+int64_t lsg_ragged_size(const lsg_ragged* r, int i) {
+  if (!r || i < -1 || i >= r->R.n) return LSG_ERR_ARG;
+  if (i < 0) return r->R.off.back();
+  return r->R.off[size_t(i) + 1] - r->R.off[size_t(i)];
+}
+
+int lsg_ragged_matvec(lsg_ragged* r, const double* const* x, double* const* y) {
+  if (!r) return LSG_ERR_ARG;
+  if (!x || !y) return r->set(LSG_ERR_ARG, "x and y must hold one vector per system");
+  for (int i = 0; i < r->R.n; ++i) {
+    if (!x[i] || !y[i]) return r->set(LSG_ERR_ARG, "a vector is NULL");
+    if (!all_finite(x[i], 2 * lsg_ragged_size(r, i))) return r->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+  }
+  if (!r->gather(r->dx, x)) return r->set(LSG_ERR_DEVICE, "copy in failed");
+  r->R.all_active();
+  r->R.op.matvec(r->dx, r->db);
+  if (!r->scatter(y, r->db)) return r->set(LSG_ERR_DEVICE, "matvec or copy out failed");
+  return LSG_OK;
+}
+
+int lsg_ragged_gmres(lsg_ragged* r, const double* const* b, double* const* x, int restart_len, int max_restarts,
+                     double tol, lsg_stats* stats) {
+  if (!r) return LSG_ERR_ARG;
+  if (!b || !x) return r->set(LSG_ERR_ARG, "b and x must hold one vector per system");
+  for (int i = 0; i < r->R.n; ++i) {
+    if (!b[i] || !x[i]) return r->set(LSG_ERR_ARG, "a vector is NULL");
+    const int64_t Ni = lsg_ragged_size(r, i);
+    if (!all_finite(b[i], 2 * Ni) || !all_finite(x[i], 2 * Ni)) return r->set(LSG_ERR_NUMERIC, "b or x holds NaN/Inf");
+  }
+  if (!r->gather(r->db, b) || !r->gather(r->dx, x)) return r->set(LSG_ERR_DEVICE, "copy in failed");
+  RaggedResult res = ragged_gmres(r->R, r->db, r->dx, restart_len, max_restarts, tol);
+  if (res.status != LSG_ERR_ARG && !r->scatter(x, r->dx) && (res.status == 0 || res.status == LSG_ERR_NOCONV)) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = "copy out failed";
+  }
+  if (stats)
+    for (int i = 0; i < r->R.n; ++i) {
+      const RaggedStats& q = res.sys[size_t(i)];
+      stats[i] = lsg_stats{};
+      stats[i].iterations = q.iterations;
+      stats[i].restarts = q.restarts;
+      stats[i].converged = q.converged;
+      stats[i].rel_residual = q.rel_residual;
+      stats[i].seconds = res.seconds;
+    }
+  if (res.status != 0) return r->set(res.status, res.error);
+  return LSG_OK;
+}
+
+int lsg_ragged_autotune(lsg_ragged* r, int reps, int* n_changed) {
+  if (!r || reps < 1) return LSG_ERR_ARG;
+  const bool had_graph = r->R.op.graph_exec != nullptr;
+  r->R.all_active();
+  std::string e = r->R.op.autotune(reps, n_changed);
+  if (e.empty() && had_graph) e = r->R.op.capture_graph();
+  if (!e.empty()) return r->set(LSG_ERR_DEVICE, e);
+  return LSG_OK;
+}
+
+int lsg_ragged_use_graph(lsg_ragged* r, int on) {
+  if (!r) return LSG_ERR_ARG;
+  if (!on) {
+    r->R.op.drop_graph();
+    return LSG_OK;
+  }
+  std::string e = r->R.op.capture_graph();
+  return e.empty() ? LSG_OK : r->set(LSG_ERR_DEVICE, e);
+}
+
+int lsg_ragged_time_matvec(lsg_ragged* r, int reps, double* ms) {
+  if (!r || reps < 1 || !ms) return LSG_ERR_ARG;
+  r->R.all_active();
+  *ms = r->R.op.time_matvec(reps);
+  if (*ms < 0) return r->set(LSG_ERR_DEVICE, "time_matvec allocation failed");
+  return LSG_OK;
+}
 const char* lsg_host_error(void) { return g_host_error.c_str(); }
 
 int lsg_host_info(const lsg_host* h, lsg_info* info) {
@@ -1090,12 +1246,13 @@ int lsg_host_aca(const lsg_host* h, const int* rows, int n1, const int* cols, in
   if (rc != LSG_OK) return rc;
   const std::vector<int> r(rows, rows + n1), c(cols, cols + n2);
   const double eps = std::pow(10.0, -1.0 * h->hb.in.TOL_POW);
+  const double abs_exit = h->hb.in.aca_exit_pow ? std::pow(10.0, -h->hb.in.aca_exit_pow) : 1e-8;
   AcaResult res;
   if (dev) {
     std::vector<AcaJob> jobs{AcaJob{&r, &c, &res}};
-    if (!dev->aca(jobs, eps).empty()) return LSG_ERR_DEVICE;
+    if (!dev->aca(jobs, eps, abs_exit).empty()) return LSG_ERR_DEVICE;
   } else {
-    aca_only_nodes(K, r, c, eps, res);
+    aca_only_nodes(K, r, c, eps, res, abs_exit);
   }
   *rank = res.rank;
   std::copy(res.row_bases.begin(), res.row_bases.end(), row_piv);

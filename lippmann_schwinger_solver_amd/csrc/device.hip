@@ -114,6 +114,15 @@ __device__ __forceinline__ bool phase_tree(C* red, int nph, int rw, int ph, int 
   return true;
 }
 
+// Destinations of inactive systems skipped (a fused plan of several systems):
+// destination d belongs to system dsys[d], active while act[system] != 0.
+// dsys == nullptr: every destination runs.
+struct DestMask {
+  const int* dsys;
+  const int* act;
+  __device__ __forceinline__ bool skip(int d) const { return dsys && !act[dsys[d]]; }
+};
+
 // One workgroup per destination: Y[0:R] (=|+=) A (R x C, col-major) * src[0..C).
 // Threads are split into (row, phase); phase p walks columns p, p+nph, ... so each
 // step of the workgroup reads nph*R consecutive complex entries. The operator is
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
     const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp) {
+    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp, DestMask mk) {
   // pointers from device memory when the matvec runs as a captured graph
   const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
   double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
@@ -135,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
   __shared__ C xs[kChunk];
   __shared__ C red[kThreads];
   for (int d = blockIdx.x; d < ndest; d += gridDim.x) {
+  if (mk.skip(d)) continue;  // uniform over the workgroup
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
     const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
     const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
     const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp) {
+    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp, DestMask mk) {
   // pointers from device memory when the matvec runs as a captured graph
   const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
   double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
@@ -221,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
   __shared__ C red[kThreads];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int d = blockIdx.x * (kThreads / 64) + wv;
-  if (d >= ndest) return;
+  if (d >= ndest || mk.skip(d)) return;
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
@@ -864,18 +874,42 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (!(e = upload_vec(srco, &d_srcoff)).empty()) return e;
   // near-field tables transposed to column-major per slot (uniform tree only:
   // an adaptive tree's near field is a pass)
+  const size_t set_sz = size_t(9) * rank * rank;
+  n_table_sets = plan.T.empty() ? 1 : int(plan.T.size() / set_sz);
+  if (!plan.T.empty() && plan.T.size() != set_sz * size_t(n_table_sets)) return "near-field tables: expected 9 slots per set";
+  if (n_table_sets > 1) {
+    // one set per group of 16 leaves (the MFMA kernel's workgroup)
+    if (n_leaves % 16 || plan.near_group_set.size() != size_t(n_leaves / 16))
+      return "near-field table sets: one set index per 16 leaves expected";
+    for (int g : plan.near_group_set)
+      if (g < 0 || g >= n_table_sets) return "near-field table set index out of range";
+    if (!(rank == 16 || rank == 25 || rank == 36)) return "several near-field table sets need rank 16, 25 or 36";
+  }
   std::vector<double2> Tc(plan.T.size());
-  if (plan.T.size() != size_t(9) * rank * rank && !plan.T.empty()) return "near-field tables: expected 9 slots";
-  for (int s = 0; s < (plan.T.empty() ? 0 : 9); ++s)
-    for (int i = 0; i < rank; ++i)
-      for (int c = 0; c < rank; ++c) {
-        const cplx v = plan.T[size_t(s) * rank * rank + size_t(i) * rank + c];
-        Tc[size_t(s) * rank * rank + size_t(c) * rank + i] = make_double2(v.real(), v.imag());
-      }
+  for (int ts = 0; ts < (plan.T.empty() ? 0 : n_table_sets); ++ts)
+    for (int s = 0; s < 9; ++s)
+      for (int i = 0; i < rank; ++i)
+        for (int c = 0; c < rank; ++c) {
+          const size_t b = ts * set_sz + size_t(s) * rank * rank;
+          const cplx v = plan.T[b + size_t(i) * rank + c];
+          Tc[b + size_t(c) * rank + i] = make_double2(v.real(), v.imag());
+        }
   if (!(e = upload_vec(Tc, &T)).empty()) return e;
   if (!plan.T.empty() && (rank == 16 || rank == 25 || rank == 36)) {
-    if (!(e = upload_vec(near_fragments(plan.T, rank), &Tm)).empty()) return e;
+    std::vector<double> frag;
+    for (int ts = 0; ts < n_table_sets; ++ts) {
+      const std::vector<cplx> one(plan.T.begin() + ts * set_sz, plan.T.begin() + (ts + 1) * set_sz);
+      std::vector<double> f = near_fragments(one, rank);
+      tm_set_stride = (long long)f.size();
+      frag.insert(frag.end(), f.begin(), f.end());
+    }
+    if (!(e = upload_vec(frag, &Tm)).empty()) return e;
     near_mfma = true;
+  }
+  if (n_table_sets > 1 && !(e = upload_vec(plan.near_group_set, &near_set)).empty()) return e;
+  if (!plan.dest_sys.empty()) {
+    if (plan.dest_sys.size() != rows.size()) return "plan: one system index per destination expected";
+    if (!(e = upload_vec(plan.dest_sys, &d_dest_sys)).empty()) return e;
   }
   if (!(e = upload_vec(plan.nbr, &nbr)).empty()) return e;
   if (!(e = upload_vec(plan.qk2, &qk2)).empty()) return e;
@@ -892,7 +926,7 @@ void DeviceOperator::release() {
   if (xy_dev) hipFree(xy_dev);
   xy_dev = nullptr;
   void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user,
-                  Tm};
+                  Tm, near_set, d_dest_sys};
   for (void* p : ptrs)
     if (p) hipFree(p);
   state = ops = nullptr;
@@ -903,6 +937,9 @@ void DeviceOperator::release() {
   ops32 = T32 = state32 = nullptr;
   qk2_user = nullptr;
   Tm = nullptr;
+  near_set = d_dest_sys = nullptr;
+  sys_active = nullptr;
+  n_table_sets = 1;
   near_mfma = false;
   if (ev_fork) hipEventDestroy(ev_fork);
   if (ev_join) hipEventDestroy(ev_join);
@@ -919,15 +956,16 @@ static void launch_block(const DeviceOperator& o, const DevPass& p, const typena
                          const double2* const* xyp) {
   const int64_t b = p.dest_base;
   const dim3 grid(p.ndest);
+  const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys + b : nullptr, o.sys_active};
   if (p.kind == 0)
     hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp, mk);
   else if (p.role == 1)
     hipLaunchKernelGGL((pass_block_kernel<true, R, 1>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp, mk);
   else
     hipLaunchKernelGGL((pass_block_kernel<true, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+                       o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp, mk);
 }
 
 template <class R>
@@ -940,12 +978,13 @@ static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2
   C* st = reinterpret_cast<C*>(std::is_same<R, double>::value ? (void*)o.state : (void*)o.state32);
   if (p.wave) {
     const dim3 grid((p.ndest + 3) / 4);
+    const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys + b : nullptr, o.sys_active};
     if (p.kind == 0)
       hipLaunchKernelGGL((pass_wave_kernel<false, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp, mk);
     else
       hipLaunchKernelGGL((pass_wave_kernel<true, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
-                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp);
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, p.ndest, xyp, mk);
     return;
   }
   launch_block<R>(o, p, ops, st, x, y, s, xyp);
@@ -982,7 +1021,9 @@ static bool launch_near_mfma(const DeviceOperator& o, const double2* x, double2*
 template <class R>
 static void launch_near(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
                         const double2* const* xyp = nullptr) {
-  if (std::is_same<R, double>::value && o.near_mfma && o.Tm && launch_near_mfma(o, x, y, s, xyp)) return;
+  if (std::is_same<R, double>::value && (o.near_mfma || o.n_table_sets > 1) && o.Tm && launch_near_mfma(o, x, y, s, xyp))
+    return;
+  if (o.n_table_sets > 1) return;  // rejected at upload (FP32 and FMA variants hold one table set)
   if (o.near_leaves_per_thread == 1) launch_near_nl<R, 1>(o, x, y, s, xyp);
   else launch_near_nl<R, 2>(o, x, y, s, xyp);
 }
@@ -1000,9 +1041,10 @@ static void launch_near(const DeviceOperator& o, const double2* x, double2* y, h
 // fragments are loaded into registers while the current slot's products run.
 template <int RK>
 __global__ __launch_bounds__(64 * ((2 * RK + 15) / 16)) void near_mfma_kernel(
-    const double* __restrict__ Tm, const int* __restrict__ nbr, const double* __restrict__ qk2,
+    const double* __restrict__ Tm_all, const int* __restrict__ nbr, const double* __restrict__ qk2,
     const double2* __restrict__ x_arg, double2* __restrict__ y_arg, int n_leaves, int find_phi,
-    const double2* const* __restrict__ xyp) {
+    const double2* const* __restrict__ xyp, const int* __restrict__ tset, long long set_stride,
+    const int* __restrict__ set_active) {
   typedef double acc_t __attribute__((ext_vector_type(4)));
   constexpr int MT = (2 * RK + 15) / 16;  // real row tiles = waves
   constexpr int KS = (2 * RK + 3) / 4;    // k-steps of 4 real columns
@@ -1017,6 +1059,10 @@ __global__ __launch_bounds__(64 * ((2 * RK + 15) / 16)) void near_mfma_kernel(
   const int t = threadIdx.x, lane = t & 63, mt = t >> 6;
   const int j = lane & 15, k4 = lane >> 4;
   const int leaf0 = blockIdx.x * 16;
+  // the workgroup's 16 leaves share one table set, one per system of a fused
+  // plan; the leaves of inactive systems are skipped
+  if (tset && set_active && !set_active[tset[blockIdx.x]]) return;
+  const double* __restrict__ Tm = Tm_all + (tset ? tset[blockIdx.x] * set_stride : 0);
   for (int e = t; e < 16 * 9; e += NT) nbs[e] = leaf0 + e / 9 < n_leaves ? nbr[leaf0 * 9 + e] : -1;
   __syncthreads();
   double g[NG], a_cur[KS], a_nxt[KS];
@@ -1083,7 +1129,7 @@ static void launch_near_mfma_rk(const DeviceOperator& o, const double2* x, doubl
                                 const double2* const* xyp) {
   constexpr int MT = (2 * RK + 15) / 16;
   hipLaunchKernelGGL((near_mfma_kernel<RK>), dim3((o.n_leaves + 15) / 16), dim3(64 * MT), 0, s, o.Tm, o.nbr, o.qk2,
-                     x, y, o.n_leaves, o.find_phi ? 1 : 0, xyp);
+                     x, y, o.n_leaves, o.find_phi ? 1 : 0, xyp, o.near_set, o.tm_set_stride, o.sys_active);
 }
 
 static bool launch_near_mfma(const DeviceOperator& o, const double2* x, double2* y, hipStream_t s,
@@ -1239,6 +1285,8 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   const int nl_before = near_leaves_per_thread;
   if (!T) {
     tuning.push_back("near_field: passes (adaptive tree)");
+  } else if (n_table_sets > 1) {
+    tuning.push_back("near_field: MFMA (" + std::to_string(n_table_sets) + " table sets)");
   } else {
   const bool mfma_before = near_mfma;
   near_mfma = false;
@@ -1274,6 +1322,7 @@ __global__ void to_f32_kernel(const double2* __restrict__ a, float2* __restrict_
 
 std::string DeviceOperator::enable_fp32() {
   if (ops32) return "";
+  if (n_table_sets > 1) return "the FP32 operator holds one near-field table set (not a fused plan)";
   const long long n_ops = n_entries, n_T = T ? 9LL * rank * rank : 0;
   HIP_OK(hipMalloc((void**)&ops32, sizeof(float2) * size_t(std::max(n_ops, 1LL))));
   if (n_T) HIP_OK(hipMalloc((void**)&T32, sizeof(float2) * size_t(n_T)));

@@ -93,6 +93,16 @@ struct DeviceOperator {
   // real-ified 16 x 4 fragments per (slot, row tile, k-step, lane)
   double* Tm = nullptr;
   bool near_mfma = false;
+  // several table sets (fused plans): set of each 16-leaf group, and the
+  // fragment stride of one set; the near field then runs on MFMA only
+  int n_table_sets = 1;
+  int* near_set = nullptr;
+  long long tm_set_stride = 0;
+  // fused plans: system of every destination, and the per-system activity
+  // flags (device, owned by the caller) that make the passes skip the
+  // destinations of inactive systems; nullptr: every destination runs
+  int* d_dest_sys = nullptr;
+  const int* sys_active = nullptr;
   hipGraphExec_t graph_exec = nullptr;  // captured FP64 matvec (capture_graph)
   const double2** xy_dev = nullptr;     // its input/output addresses
   std::vector<std::string> tuning;      // autotuner decisions

@@ -98,6 +98,217 @@ __global__ __launch_bounds__(kT) void conv_bwd_data_kernel(const float* __restri
   for (int ci = 0; ci < CIN; ++ci) op[ci] = acc[ci];
 }
 
+// ---------------------------------------------------------------------------
+// Hidden 32 -> 32 layers on the matrix cores. A wave owns a tile of 16
+// consecutive pixels of one image row (G is a multiple of 16) and forms each
+// 3x3 convolution as 9 shifted GEMMs, one per tap.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kTilesPerWg = kT / 64;
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p) {
+  bf16x8 v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+// Forward on v_mfma_f32_16x16x32_bf16: per tap, C[16 pixels x 16 cout] +=
+// A[16 pixels x 32 cin] B[32 cin x 16 cout]; lane l supplies A(pixel l%16,
+// cin 8(l/16)..+8) from the NHWC input (one 16-byte load) and B(cin
+// 8(l/16)..+8, cout l%16) from the weights [cout][tap][cin] (one 16-byte
+// load, kept in registers for all tiles: 9 taps x 2 cout tiles); the result
+// lane l holds pixels 4(l/16)+i, cout l%16. FP32 accumulation, bias and ReLU
+// in FP32, output rounded to BF16 (and kept in FP32 when out_f is given).
+__global__ __launch_bounds__(kT) void conv_fwd_mfma_kernel(const uint16_t* __restrict__ in,
+                                                           const uint16_t* __restrict__ w,
+                                                           const float* __restrict__ bias, int G, int dil, int relu,
+                                                           long long ntiles, uint16_t* __restrict__ out_bf,
+                                                           float* __restrict__ out_f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  bf16x8 wf[9][2];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) wf[t][ct] = load_bf16x8(w + ((ct * 16 + r16) * 9 + t) * kWidth + 8 * kg);
+  const float b0 = bias[r16], b1 = bias[16 + r16];
+  const int GG = G * G;
+  const bf16x8 zero = {};
+  for (long long tile = (long long)blockIdx.x * kTilesPerWg + wave; tile < ntiles;
+       tile += (long long)gridDim.x * kTilesPerWg) {
+    const long long p0 = tile * 16;
+    const int b = int(p0 / GG), pix0 = int(p0 - (long long)b * GG);
+    const int y = pix0 / G, x = pix0 - y * G + r16;
+    const uint16_t* img = in + (size_t)b * GG * kWidth;
+    f32x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int yy = y + (t / 3 - 1) * dil, xx = x + (t % 3 - 1) * dil;
+      const bool ok = yy >= 0 && yy < G && xx >= 0 && xx < G;
+      const bf16x8 a = ok ? load_bf16x8(img + ((size_t)yy * G + xx) * kWidth + 8 * kg) : zero;
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wf[t][0], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wf[t][1], acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t o = (size_t)(p0 + 4 * kg + i) * kWidth;
+      float v0 = acc0[i] + b0, v1 = acc1[i] + b1;
+      if (relu) {
+        v0 = fmaxf(v0, 0.0f);
+        v1 = fmaxf(v1, 0.0f);
+      }
+      if (out_bf) {
+        out_bf[o + r16] = f2bf(v0);
+        out_bf[o + 16 + r16] = f2bf(v1);
+      }
+      if (out_f) {
+        out_f[o + r16] = v0;
+        out_f[o + 16 + r16] = v1;
+      }
+    }
+  }
+}
+
+// Data gradient on v_mfma_f32_16x16x4f32: gin[p, ci] = sum_{tap, co}
+// g[p - off_tap, co] w[co, tap, ci]. Per tap, 8 k-steps of 4 cout: in k-step
+// s, lane l supplies A(pixel l%16, k l/16) = g(pixel - off, cout 8(l/16) + s)
+// (its 8 cout values are one contiguous 32-byte read) and B(k l/16, cin l%16)
+// = w(cout 8(l/16) + s, tap, cin) from the weights staged in LDS as FP32;
+// the result lane l holds pixels 4(l/16)+v, cin l%16 (+16 for the second tile).
+__global__ __launch_bounds__(kT) void conv_bwd_data_mfma_kernel(const float* __restrict__ g,
+                                                                const uint16_t* __restrict__ w, int G, int dil,
+                                                                long long ntiles, float* __restrict__ gin) {
+  __shared__ float ws[kWidth * 9 * kWidth];
+  for (int e = threadIdx.x; e < kWidth * 9 * kWidth; e += kT) ws[e] = bf2f(w[e]);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const int GG = G * G;
+  for (long long tile = (long long)blockIdx.x * kTilesPerWg + wave; tile < ntiles;
+       tile += (long long)gridDim.x * kTilesPerWg) {
+    const long long p0 = tile * 16;
+    const int b = int(p0 / GG), pix0 = int(p0 - (long long)b * GG);
+    const int y = pix0 / G, x = pix0 - y * G + r16;
+    const float* img = g + (size_t)b * GG * kWidth;
+    f32x4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    for (int t = 0; t < 9; ++t) {
+      const int yy = y - (t / 3 - 1) * dil, xx = x - (t % 3 - 1) * dil;
+      float a[8];
+      if (yy >= 0 && yy < G && xx >= 0 && xx < G) {
+        const float4* gp = reinterpret_cast<const float4*>(img + ((size_t)yy * G + xx) * kWidth + 8 * kg);
+        const float4 u = gp[0], v = gp[1];
+        a[0] = u.x; a[1] = u.y; a[2] = u.z; a[3] = u.w;
+        a[4] = v.x; a[5] = v.y; a[6] = v.z; a[7] = v.w;
+      } else {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) a[s] = 0.0f;
+      }
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const float* wr = ws + ((8 * kg + s) * 9 + t) * kWidth;
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], wr[r16], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], wr[16 + r16], acc1, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float* op = gin + (size_t)(p0 + 4 * kg + v) * kWidth;
+      op[r16] = acc0[v];
+      op[16 + r16] = acc1[v];
+    }
+  }
+}
+
+// Weight and bias gradients on v_mfma_f32_16x16x4f32, per workgroup of
+// kWChunk pixels of one image (the partial layout of conv_bwd_w_kernel):
+// dW[co, tap, ci] = sum_p g[p, co] in[p + off_tap, ci]. In a k-step of 4
+// pixels, lane l supplies A(cout l%16 (+16), k l/16) = g(p0 + l/16, cout) and
+// B(k l/16, cin l%16 (+16)) = in(p0 + l/16 + off, cin); wave w owns taps w,
+// w+4, w+8 (2 x 2 cout/cin tiles each); wave 0 also sums the bias gradient.
+__global__ __launch_bounds__(kT) void conv_bwd_w_mfma_kernel(const uint16_t* __restrict__ in,
+                                                             const float* __restrict__ g, int G, int dil,
+                                                             float* __restrict__ part) {
+  constexpr int NW = kWidth * 9 * kWidth;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const int GG = G * G;
+  const int chunks = (GG + kWChunk - 1) / kWChunk;
+  const int b = blockIdx.x / chunks, c0 = (blockIdx.x % chunks) * kWChunk;
+  const int c1 = min(GG, c0 + kWChunk);
+  const float* gi = g + (size_t)b * GG * kWidth;
+  const uint16_t* ii = in + (size_t)b * GG * kWidth;
+  constexpr int kMaxTaps = 3;
+  int taps[kMaxTaps];
+  int ntap = 0;
+  for (int t = wave; t < 9; t += 4) taps[ntap++] = t;
+  f32x4 acc[kMaxTaps][2][2];
+#pragma unroll
+  for (int q = 0; q < kMaxTaps; ++q)
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[q][mt][ct] = f32x4{0, 0, 0, 0};
+  float bsum0 = 0.0f, bsum1 = 0.0f;
+  for (int p0 = c0; p0 < c1; p0 += 4) {
+    const int p = p0 + kg;
+    float a0 = 0.0f, a1 = 0.0f;
+    int y = 0, x = 0;
+    if (p < c1) {
+      a0 = gi[(size_t)p * kWidth + r16];
+      a1 = gi[(size_t)p * kWidth + 16 + r16];
+      y = p / G;
+      x = p - y * G;
+    }
+    bsum0 += a0;
+    bsum1 += a1;
+#pragma unroll
+    for (int q = 0; q < kMaxTaps; ++q) {
+      if (q < ntap) {
+        const int t = taps[q];
+        const int yy = y + (t / 3 - 1) * dil, xx = x + (t % 3 - 1) * dil;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (p < c1 && yy >= 0 && yy < G && xx >= 0 && xx < G) {
+          const uint16_t* ip = ii + ((size_t)yy * G + xx) * kWidth;
+          v0 = bf2f(ip[r16]);
+          v1 = bf2f(ip[16 + r16]);
+        }
+        acc[q][0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, v0, acc[q][0][0], 0, 0, 0);
+        acc[q][0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, v1, acc[q][0][1], 0, 0, 0);
+        acc[q][1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, v0, acc[q][1][0], 0, 0, 0);
+        acc[q][1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, v1, acc[q][1][1], 0, 0, 0);
+      }
+    }
+  }
+  float* P = part + (size_t)blockIdx.x * (NW + kWidth);
+#pragma unroll
+  for (int q = 0; q < kMaxTaps; ++q) {
+    if (q < ntap) {
+      const int t = taps[q];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int co = 16 * mt + 4 * kg + v, ci = 16 * ct + r16;
+            P[(co * 9 + t) * kWidth + ci] = acc[q][mt][ct][v];
+          }
+    }
+  }
+  if (wave == 0) {
+    // bias: sum over the 4 lanes sharing r16 (one per pixel of a k-step)
+#pragma unroll
+    for (int o = 16; o < 64; o <<= 1) {
+      bsum0 += __shfl_xor(bsum0, o);
+      bsum1 += __shfl_xor(bsum1, o);
+    }
+    if (kg == 0) {
+      P[NW + r16] = bsum0;
+      P[NW + 16 + r16] = bsum1;
+    }
+  }
+}
+
 // g *= (act > 0): gradient through ReLU (act is the layer's stored output)
 __global__ void relu_bwd_kernel(float* __restrict__ g, const uint16_t* __restrict__ act, long long n) {
   const long long i = (long long)blockIdx.x * kT + threadIdx.x;
@@ -378,9 +589,17 @@ void Net::release() {
   stream = nullptr;
 }
 
+// the 32 -> 32 layers run on the matrix cores when their weights are 16-byte
+// aligned (the 16-byte fragment loads); G is a multiple of 16 by construction
+static bool mfma_ok(const Net::Layer& L) { return L.cin == kWidth && L.cout == kWidth && L.w_off % 8 == 0; }
+static unsigned mfma_grid(long long ntiles) {
+  return unsigned(std::max(1LL, std::min(4096LL, (ntiles + kTilesPerWg - 1) / kTilesPerWg)));
+}
+
 void Net::forward(int B, hipStream_t s) {
   const size_t GG = size_t(G) * G;
   const dim3 grid(nblk((long long)GG), B);
+  const long long ntiles = (long long)B * (long long)GG / 16;
   for (size_t l = 0; l < layers.size(); ++l) {
     const Layer& L = layers[l];
     const bool last = l + 1 == layers.size();
@@ -392,6 +611,9 @@ void Net::forward(int B, hipStream_t s) {
     else if (L.cout == 2)
       hipLaunchKernelGGL((conv_fwd_kernel<kWidth, 2>), grid, dim3(kT), 0, s, act[l], pw + L.w_off, p + L.b_off, G,
                          L.dil, L.relu ? 1 : 0, o_bf, o_f);
+    else if (mfma_ok(L))
+      hipLaunchKernelGGL(conv_fwd_mfma_kernel, dim3(mfma_grid(ntiles)), dim3(kT), 0, s, act[l], pw + L.w_off,
+                         p + L.b_off, G, L.dil, L.relu ? 1 : 0, ntiles, o_bf, o_f);
     else
       hipLaunchKernelGGL((conv_fwd_kernel<kWidth, kWidth>), grid, dim3(kT), 0, s, act[l], pw + L.w_off,
                          p + L.b_off, G, L.dil, L.relu ? 1 : 0, o_bf, o_f);
@@ -406,6 +628,7 @@ void Net::backward(int B, hipStream_t s) {
   hipLaunchKernelGGL(mse_kernel, dim3(nlb), dim3(kT), 0, s, out, tgt, nout, g0, lpart);
   const int chunks = int((GG + kWChunk - 1) / kWChunk);
   const dim3 grid(nblk((long long)GG), B);
+  const long long ntiles = (long long)B * (long long)GG / 16;
   float* gcur = g0;
   float* gnext = g1;
   for (int l = int(layers.size()) - 1; l >= 0; --l) {
@@ -420,6 +643,8 @@ void Net::backward(int B, hipStream_t s) {
     else if (L.cout == 2)
       hipLaunchKernelGGL((conv_bwd_w_kernel<kWidth, 2>), dim3(B * chunks), dim3(kT), 0, s, act[l], gcur, G, L.dil,
                          wpart);
+    else if (mfma_ok(L))
+      hipLaunchKernelGGL(conv_bwd_w_mfma_kernel, dim3(B * chunks), dim3(kT), 0, s, act[l], gcur, G, L.dil, wpart);
     else
       hipLaunchKernelGGL((conv_bwd_w_kernel<kWidth, kWidth>), dim3(B * chunks), dim3(kT), 0, s, act[l], gcur, G,
                          L.dil, wpart);
@@ -428,6 +653,9 @@ void Net::backward(int B, hipStream_t s) {
     if (L.cout == 2)
       hipLaunchKernelGGL((conv_bwd_data_kernel<kWidth, 2>), grid, dim3(kT), 0, s, gcur, pw + L.w_off, G, L.dil,
                          gnext);
+    else if (mfma_ok(L))
+      hipLaunchKernelGGL(conv_bwd_data_mfma_kernel, dim3(mfma_grid(ntiles)), dim3(kT), 0, s, gcur, pw + L.w_off, G,
+                         L.dil, ntiles, gnext);
     else
       hipLaunchKernelGGL((conv_bwd_data_kernel<kWidth, kWidth>), grid, dim3(kT), 0, s, gcur, pw + L.w_off, G, L.dil,
                          gnext);

@@ -83,7 +83,7 @@ cplx vdot(const std::vector<cplx>& a, const std::vector<cplx>& b) {  // Eigen a.
 }  // namespace
 
 void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const std::vector<int>& cols, double eps,
-                    AcaResult& out) {
+                    AcaResult& out, double abs_exit) {
   const int N1 = int(rows.size()), N2 = int(cols.size());
   out.row_bases.clear(); out.col_bases.clear(); out.Ar.clear(); out.rank = 0;
   const int mn = std::min(N1, N2);
@@ -168,7 +168,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
       for (int l = 0; l <= rank - 1; ++l)
         for (size_t s = 0; s < U[l].size(); ++s) col[s] -= V[l][col_index] * U[l][s];
       u = col;
-      if (vnorm(u) < 1e-8 || vnorm(v) < 1e-8) {
+      if (vnorm(u) < abs_exit || vnorm(v) < abs_exit) {
         out.row_bases.pop_back(); out.col_bases.pop_back();
         break;
       }
@@ -218,7 +218,7 @@ void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const st
       for (int l = 0; l <= rank - 1; ++l)
         for (size_t s = 0; s < U[l].size(); ++s) row[s] -= V[l][row_index] * U[l][s];
       u = row;
-      if (vnorm(u) < 1e-8 || vnorm(v) < 1e-8) {
+      if (vnorm(u) < abs_exit || vnorm(v) < abs_exit) {
         out.col_bases.pop_back(); out.row_bases.pop_back();
         break;
       }
@@ -247,6 +247,7 @@ struct Contrib {
   std::vector<int> rows;                 // target nodes
   const std::vector<int>* cols = nullptr;  // source charge nodes
   const ColPivQR* Zt = nullptr;            // L2L only: QR of Atilde_in^T; operator = K Atilde_in^-1
+  const std::vector<cplx>* At = nullptr;   // L2L, device build: Atilde_in^T itself (factored on the GPU)
   const std::vector<int>* src_list = nullptr;  // source state indices (M2L)
   int64_t src_base = -1;                       // contiguous source (L2L)
 };
@@ -261,6 +262,7 @@ struct Owner {
   std::vector<cplx> Ar;        // rank x src, row-major
   std::vector<cplx> P;         // rank x src, col-major (Atilde^-1 Ar)
   ColPivQR in_qrT;             // QR of Atilde_in^T (incoming check x charge points)
+  std::vector<cplx> in_At;     // device build: Atilde_in^T (r x r, col-major), factored with each solve
   std::vector<int> fout;       // state indices aligned with out_charge
   int64_t uin = -1;            // state offset of the incoming potential (size ucp)
   std::vector<Contrib> contribs;
@@ -272,6 +274,7 @@ struct Builder {
   const BasisSet& bs;
   const EntryKernel& K;
   double eps;
+  double abs_exit = 1e-8;
   std::vector<std::vector<Owner>> L;  // L[l][k]: LFR owners (and leaves)
   std::vector<std::vector<Owner>> H;  // H[l][k*nc+c]: HFR cones
   int rank;
@@ -329,6 +332,12 @@ struct Builder {
   void finish_out(Owner& o, const std::vector<int>& rows, const std::vector<int>& cols, const AcaResult& res) {
     for (int r : res.row_bases) o.out_check.push_back(rows[r]);
     for (int c : res.col_bases) o.out_charge.push_back(cols[c]);
+    if (dev) {
+      // P = Atilde^-1 Ar is solved on the GPU for the whole level (qr_level)
+      o.P.resize(size_t(res.rank) * cols.size());
+      o.src_nodes = cols;
+      return;
+    }
     ColPivQR qr;
     {
       const std::vector<cplx> A = pivot_block(res, int(cols.size()));
@@ -353,6 +362,10 @@ struct Builder {
     std::vector<cplx> At(size_t(r) * r);
     for (int i = 0; i < r; ++i)
       for (int j = 0; j < r; ++j) At[size_t(i) * r + j] = A[size_t(j) * r + i];
+    if (dev) {
+      o.in_At = std::move(At);   // factored on the GPU with every L2L solve
+      return;
+    }
     o.in_qrT.factor(At.data(), r, r);
   }
 
@@ -409,6 +422,23 @@ struct Builder {
       }
     }
   }
+  // Device build: P = Atilde_out^-1 Ar of every outgoing skeleton of a level,
+  // one batched QR factor + solve on the GPU.
+  std::string qr_level(std::vector<Job>& jobs) {
+    std::vector<QrSolveJob> qj;
+    std::vector<std::vector<cplx>> As(jobs.size()), Bs(jobs.size());
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      Job& jb = jobs[q];
+      if (!jb.out || !job_active(jb) || jb.res.rank == 0) continue;
+      const int r = jb.res.rank, ns = int(jb.cols.size());
+      As[q] = pivot_block(jb.res, ns);
+      Bs[q].resize(size_t(r) * ns);
+      for (int i = 0; i < r; ++i)
+        for (int j = 0; j < ns; ++j) Bs[q][size_t(j) * r + i] = jb.res.Ar[size_t(i) * ns + j];
+      qj.push_back(QrSolveJob{As[q].data(), r, Bs[q].data(), ns, jb.o->P.data()});
+    }
+    return qj.empty() ? "" : dev->qr_solve(qj);
+  }
   // Skeletons of one level: operand lists (from the level below), the ACAs
   // (host threads or one batched device launch sequence), then the QR finish.
   std::string run_level(std::vector<Job>& jobs) {
@@ -417,16 +447,16 @@ struct Builder {
       std::vector<AcaJob> aj;
       for (Job& jb : jobs)
         if (job_active(jb)) aj.push_back(AcaJob{&jb.rows, &jb.cols, &jb.res});
-      std::string e = dev->aca(aj, eps);
+      std::string e = dev->aca(aj, eps, abs_exit);
       if (!e.empty()) return e;
     } else {
 #pragma omp parallel for schedule(dynamic, 1)
       for (int q = 0; q < nj; ++q)
-        if (job_active(jobs[q])) aca_only_nodes(K, jobs[q].rows, jobs[q].cols, eps, jobs[q].res);
+        if (job_active(jobs[q])) aca_only_nodes(K, jobs[q].rows, jobs[q].cols, eps, jobs[q].res, abs_exit);
     }
 #pragma omp parallel for schedule(dynamic, 1)
     for (int q = 0; q < nj; ++q) complete(jobs[q]);
-    return "";
+    return dev ? qr_level(jobs) : "";
   }
 
   DeviceNca* dev = nullptr;
@@ -616,7 +646,7 @@ struct Builder {
           if (n > 0) {
             Contrib s;
             s.rows.assign(c.rows.begin() + off, c.rows.begin() + off + n);
-            s.cols = c.cols; s.Zt = c.Zt; s.src_list = c.src_list; s.src_base = c.src_base;
+            s.cols = c.cols; s.Zt = c.Zt; s.At = c.At; s.src_list = c.src_list; s.src_base = c.src_base;
             tg.o->contribs.push_back(std::move(s));
           }
           off += n;
@@ -642,7 +672,8 @@ struct Builder {
       Contrib c;
       c.rows = tg.o->ucp;
       c.cols = &o.in_charge;
-      c.Zt = &o.in_qrT;
+      if (dev) c.At = &o.in_At;
+      else c.Zt = &o.in_qrT;
       c.src_base = o.uin;
       tg.o->contribs.push_back(std::move(c));
     }
@@ -830,6 +861,7 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
   phase[0] = since(tp);
   tp = clk::now();
   Builder B(t, bs, K, std::pow(10, -1.0 * opt.TOL_POW));
+  B.abs_exit = opt.aca_abs_exit;
   std::unique_ptr<DeviceNca> dev;
   if (opt.device) {
     std::string e;
@@ -1023,7 +1055,10 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
           K.block(c.rows.data(), nr, c.cols->data(), nc, kb.data());
           kblk = kb.data();
         }
-        if (c.Zt) {
+        if (c.At) {
+          // device build: solved below for the whole pass in one GPU batch
+          for (int q = 0; q < nc; ++q) sp[q] = int(c.src_base + q);
+        } else if (c.Zt) {
           // op = K Atilde^-1: rows of op solve Atilde^T op_row^T = K_row^T
           std::vector<cplx> Bm(size_t(nc) * nr), X(size_t(nc) * nr);
           for (int r = 0; r < nr; ++r)
@@ -1038,6 +1073,45 @@ std::string build_plan(const Tree& t, const BasisSet& bs, const Problem& prob, c
         }
         op += size_t(nc) * nr;
         sp += nc;
+      }
+    }
+    if (on_dev) {
+      // L2L blocks of the device build: op = K Atilde_in^-1, i.e. X = Atilde_in^-T K^T
+      // for the K block in op, one batched QR factor + solve on the GPU
+      struct L2L { cplx* op; int nr, nc; const std::vector<cplx>* At; };
+      std::vector<L2L> items;
+      for (int di = 0; di < nd; ++di) {
+        Dest& d = ds[di];
+        if (d.P) continue;
+        cplx* op = plan.ops.data() + ph.op_off[di];
+        for (const Contrib& c : *d.contribs) {
+          const int nc = int(c.cols->size());
+          if (int(c.rows.size()) != d.rows) continue;
+          if (c.At) items.push_back(L2L{op, d.rows, nc, c.At});
+          op += size_t(nc) * d.rows;
+        }
+      }
+      if (!items.empty()) {
+        std::vector<std::vector<cplx>> Bm(items.size()), X(items.size());
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int q = 0; q < int(items.size()); ++q) {
+          const L2L& it = items[size_t(q)];
+          Bm[size_t(q)].resize(size_t(it.nc) * it.nr);
+          X[size_t(q)].resize(size_t(it.nc) * it.nr);
+          for (int r = 0; r < it.nr; ++r)
+            for (int p = 0; p < it.nc; ++p) Bm[size_t(q)][size_t(r) * it.nc + p] = it.op[size_t(p) * it.nr + r];
+        }
+        std::vector<QrSolveJob> qj(items.size());
+        for (size_t q = 0; q < items.size(); ++q)
+          qj[q] = QrSolveJob{items[q].At->data(), items[q].nc, Bm[q].data(), items[q].nr, X[q].data()};
+        std::string e = B.dev->qr_solve(qj);
+        if (!e.empty()) return "device NCA: " + e;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int q = 0; q < int(items.size()); ++q) {
+          const L2L& it = items[size_t(q)];
+          for (int c = 0; c < it.nc; ++c)
+            for (int r = 0; r < it.nr; ++r) it.op[size_t(c) * it.nr + r] = X[size_t(q)][size_t(r) * it.nc + c];
+        }
       }
     }
     plan.passes.push_back(std::move(ph));

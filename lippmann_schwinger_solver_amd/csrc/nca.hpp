@@ -46,8 +46,9 @@ struct AcaResult {
   std::vector<cplx> Ar;  // rank x ncols, row-major: K(row pivots, all cols)
 };
 // Partially pivoted ACA of K(rows, cols) with the reference's stopping rules.
+// abs_exit: the absolute exit on the new cross vectors' norms (the reference's 1e-8)
 void aca_only_nodes(const EntryKernel& K, const std::vector<int>& rows, const std::vector<int>& cols, double eps,
-                    AcaResult& out);
+                    AcaResult& out, double abs_exit = 1e-8);
 
 // One destination of a device pass: rows x ncol column-major operator,
 // ncol source state indices, contiguous destination [dst, dst+rows).
@@ -67,7 +68,14 @@ struct Plan {
   std::vector<int> src;     // source state indices of every pass
   std::vector<PassHost> passes;   // execution order: upward, downward, leaf (after near field)
   // near field (leaf level of a uniform tree): y = x + qk2 .* sum_s T_s x_nbr(s)
-  std::vector<cplx> T;      // 9 x rank x rank, row-major per slot
+  std::vector<cplx> T;      // 9 x rank x rank, row-major per slot (per table set)
+  // several near-field table sets (a fused plan of independent trees): T holds
+  // the sets back to back and near_group_set names the set of each group of
+  // 16 consecutive leaves; empty for one set
+  std::vector<int> near_group_set;
+  // fused plans: the system of every destination (passes in order); empty
+  // for one system
+  std::vector<int> dest_sys;
   std::vector<int> nbr;     // n_leaves x 9 leaf indices, -1 where absent
   std::vector<double> qk2;  // N
   bool find_phi = true;
@@ -103,6 +111,16 @@ struct BlockJob {
   cplx* out = nullptr;
 };
 
+// X (n x nrhs) = A^-1 B for a square A (n x n), all column-major: the factor
+// and basic solve of ColPivQR (column-pivoted Householder QR).
+struct QrSolveJob {
+  const cplx* A = nullptr;
+  int n = 0;
+  const cplx* B = nullptr;
+  int nrhs = 0;
+  cplx* X = nullptr;
+};
+
 // Entry generation and ACA on the GPU: the entry kernel, the pivot order and
 // the stopping rules of EntryKernel and aca_only_nodes, evaluated in device
 // arithmetic (results agree with the host to rounding; pivots may differ
@@ -110,8 +128,11 @@ struct BlockJob {
 class DeviceNca {
  public:
   virtual ~DeviceNca() {}
-  virtual std::string aca(const std::vector<AcaJob>& jobs, double eps) = 0;
+  virtual std::string aca(const std::vector<AcaJob>& jobs, double eps, double abs_exit = 1e-8) = 0;
   virtual std::string blocks(const std::vector<BlockJob>& jobs) = 0;
+  // Batched column-pivoted QR factor + solve (ColPivQR semantics in device
+  // arithmetic); jobs whose n exceeds the device limit are solved on the host.
+  virtual std::string qr_solve(const std::vector<QrSolveJob>& jobs) = 0;
   // The integrals of near_integrals for every table in `tabs` (each rank x nb,
   // row-major), by the same adaptive quadrature on the GPU.
   virtual std::string near_integrals(const BasisSet& bs, double kappa, const std::vector<NearGeom>& tabs,
@@ -121,6 +142,7 @@ typedef DeviceNca* (*DeviceNcaFactory)(const EntryKernel& K, std::string& err);
 
 struct BuildOptions {
   int TOL_POW = 10;
+  double aca_abs_exit = 1e-8;   // the ACA's absolute exit (the reference's 1e-8)
   bool find_phi = true;
   // when set, NCA skeletons and operator entries are computed on the GPU
   DeviceNcaFactory device = nullptr;

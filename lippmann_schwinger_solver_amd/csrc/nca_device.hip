@@ -192,7 +192,8 @@ __device__ double block_norm(const double2* v, int n, double* sh) {
 __global__ __launch_bounds__(kAT) void aca_kernel(DevEntry K, const AcaDesc* __restrict__ desc,
                                                   const int* __restrict__ idx, const int* __restrict__ runs,
                                                   double2* __restrict__ wsp, unsigned char* __restrict__ flg,
-                                                  int* __restrict__ piv, int* __restrict__ res, double eps) {
+                                                  int* __restrict__ piv, int* __restrict__ res, double eps,
+                                                  double abs_exit) {
   __shared__ double shv[kAT / 64];
   __shared__ int shi[kAT / 64];
   __shared__ double shs[kAT / 64];
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(kAT) void aca_kernel(DevEntry K, const AcaDesc* __r
         __syncthreads();
         const double nu = block_norm(U + (size_t)rank * N1, N1, shs);
         const double nv = block_norm(V + (size_t)rank * N2, N2, shs);
-        if (nu < 1e-8 || nv < 1e-8) break;
+        if (nu < abs_exit || nv < abs_exit) break;
         if (t == 0) {
           rbase[rank] = ri;
           cbase[rank] = ci;
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(kAT) void aca_kernel(DevEntry K, const AcaDesc* __r
         __syncthreads();
         const double nu = block_norm(U + (size_t)rank * N2, N2, shs);
         const double nv = block_norm(V + (size_t)rank * N1, N1, shs);
-        if (nu < 1e-8 || nv < 1e-8) break;
+        if (nu < abs_exit || nv < abs_exit) break;
         if (t == 0) {
           cbase[rank] = ci;
           rbase[rank] = ri;
@@ -605,6 +606,198 @@ __global__ __launch_bounds__(kQT) void quad_kernel(const QuadJob* __restrict__ j
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched column-pivoted Householder QR and basic solve (ColPivQR of the host
+// build, step by step: the pivot is the first largest downdated column norm,
+// pivots below eps max_norm sqrt(n - k) end the nonzero block, H_k = I - tau
+// v v^H with v = [1; essential], Q^H b applies H_0, H_1, ... with tau itself,
+// back substitution on the nonzero block, the rest zero). One wavefront per
+// system: the matrix lives in LDS; thread j owns column j in the trailing
+// update and the norm downdates, thread 0 the sequential reductions, so every
+// sum is formed in the host's order. Thread r solves right-hand sides r,
+// r + 64, ... in the global workspace W.
+constexpr int kQrMax = 60;     // largest system solved on the device (A in 64 KB of LDS)
+struct QrDesc {
+  int n, nrhs;
+  long long a, b;   // offsets of A (n x n) and of B / X / W (n x nrhs), complex
+};
+
+__global__ __launch_bounds__(64) void qr_solve_kernel(const QrDesc* __restrict__ desc, const double2* __restrict__ Aall,
+                                                     const double2* __restrict__ Ball, double2* __restrict__ Xall,
+                                                     double2* __restrict__ Wall) {
+  extern __shared__ double2 qr[];   // n x n, column-major
+  __shared__ double nu[kQrMax], nd[kQrMax];
+  __shared__ double2 tau[kQrMax];
+  __shared__ int transp[kQrMax], perm[kQrMax];
+  __shared__ int nonzero_s, big_s, zero_s;
+  __shared__ double2 d_s, t_s;
+  __shared__ double beta_s;
+  const QrDesc D = desc[blockIdx.x];
+  const int n = D.n, t = threadIdx.x;
+  const double2* A = Aall + D.a;
+  for (int e = t; e < n * n; e += 64) qr[e] = A[e];
+  __syncthreads();
+  auto at = [&](int i, int j) -> double2& { return qr[j * n + i]; };
+  if (t < n) {
+    double sq = 0.0;
+    for (int i = 0; i < n; ++i) sq += at(i, t).x * at(i, t).x + at(i, t).y * at(i, t).y;
+    nd[t] = nu[t] = sqrt(sq);
+  }
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  double maxn = 0.0;
+  for (int k = 0; k < n; ++k) maxn = fmax(maxn, nu[k]);
+  const double thr_helper = (maxn * eps) * (maxn * eps) / double(n);
+  const double down_thr = sqrt(eps);
+  if (t == 0) nonzero_s = n;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (t == 0) {
+      int big = k;
+      double bv = nu[k];
+      for (int j = k + 1; j < n; ++j)
+        if (nu[j] > bv) {
+          bv = nu[j];
+          big = j;
+        }
+      if (nonzero_s == n && bv * bv < thr_helper * double(n - k)) nonzero_s = k;
+      transp[k] = big;
+      big_s = big;
+    }
+    __syncthreads();
+    const int big = big_s;
+    if (big != k) {
+      for (int i = t; i < n; i += 64) {
+        const double2 a = at(i, k);
+        at(i, k) = at(i, big);
+        at(i, big) = a;
+      }
+      if (t == 0) {
+        double a = nu[k]; nu[k] = nu[big]; nu[big] = a;
+        a = nd[k]; nd[k] = nd[big]; nd[big] = a;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      const double2 c0 = at(k, k);
+      double tail = 0.0;
+      for (int i = k + 1; i < n; ++i) tail += at(i, k).x * at(i, k).x + at(i, k).y * at(i, k).y;
+      const double tiny = 2.2250738585072014e-308;
+      if (tail <= tiny && c0.y * c0.y <= tiny) {
+        zero_s = 1;
+        t_s = make_double2(0.0, 0.0);
+        beta_s = c0.x;
+      } else {
+        zero_s = 0;
+        double beta = sqrt(c0.x * c0.x + c0.y * c0.y + tail);
+        if (c0.x >= 0) beta = -beta;
+        d_s = make_double2(c0.x - beta, c0.y);
+        const double2 q = make_double2((beta - c0.x) / beta, -c0.y / beta);
+        t_s = make_double2(q.x, -q.y);   // conj((beta - c0) / beta)
+        beta_s = beta;
+      }
+    }
+    __syncthreads();
+    for (int i = k + 1 + t; i < n; i += 64) at(i, k) = zero_s ? make_double2(0.0, 0.0) : cdiv(at(i, k), d_s);
+    __syncthreads();
+    const double2 tk = t_s;
+    if (t == 0) {
+      tau[k] = tk;
+      at(k, k) = make_double2(beta_s, 0.0);
+    }
+    __syncthreads();
+    // H_k applied to the trailing columns, one column per thread
+    const int j = k + 1 + t;
+    if (j < n) {
+      if (n - k == 1) {
+        at(k, j) = cmul(at(k, j), make_double2(1.0 - tk.x, -tk.y));
+      } else if (tk.x != 0.0 || tk.y != 0.0) {
+        double2 sv = at(k, j);
+        for (int i = k + 1; i < n; ++i) {
+          const double2 p = cjmul(at(i, k), at(i, j));
+          sv.x += p.x;
+          sv.y += p.y;
+        }
+        const double2 ts = cmul(tk, sv);
+        at(k, j).x -= ts.x;
+        at(k, j).y -= ts.y;
+        for (int i = k + 1; i < n; ++i) {
+          const double2 p = cmul(at(i, k), ts);
+          at(i, j).x -= p.x;
+          at(i, j).y -= p.y;
+        }
+      }
+      if (nu[j] != 0.0) {
+        double tt = cabs2(at(k, j)) / nu[j];
+        tt = (1.0 + tt) * (1.0 - tt);
+        if (tt < 0) tt = 0;
+        const double r = nu[j] / nd[j];
+        const double t2 = tt * r * r;
+        if (t2 <= down_thr) {
+          double sq = 0.0;
+          for (int i = k + 1; i < n; ++i) sq += at(i, j).x * at(i, j).x + at(i, j).y * at(i, j).y;
+          nd[j] = nu[j] = sqrt(sq);
+        } else {
+          nu[j] *= sqrt(tt);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+      const int a = perm[k];
+      perm[k] = perm[transp[k]];
+      perm[transp[k]] = a;
+    }
+  }
+  __syncthreads();
+  const int nz = nonzero_s;
+  for (int r = t; r < D.nrhs; r += 64) {
+    double2* c = Wall + D.b + (long long)r * n;
+    const double2* bcol = Ball + D.b + (long long)r * n;
+    double2* xcol = Xall + D.b + (long long)r * n;
+    for (int i = 0; i < n; ++i) {
+      c[i] = bcol[i];
+      xcol[i] = make_double2(0.0, 0.0);
+    }
+    if (nz == 0) continue;
+    for (int k = 0; k < nz; ++k) {
+      const double2 tk = tau[k];
+      if (n - k == 1) {
+        c[k] = cmul(c[k], make_double2(1.0 - tk.x, -tk.y));
+        continue;
+      }
+      if (tk.x == 0.0 && tk.y == 0.0) continue;
+      double2 sv = c[k];
+      for (int i = k + 1; i < n; ++i) {
+        const double2 p = cjmul(at(i, k), c[i]);
+        sv.x += p.x;
+        sv.y += p.y;
+      }
+      const double2 ts = cmul(tk, sv);
+      c[k].x -= ts.x;
+      c[k].y -= ts.y;
+      for (int i = k + 1; i < n; ++i) {
+        const double2 p = cmul(at(i, k), ts);
+        c[i].x -= p.x;
+        c[i].y -= p.y;
+      }
+    }
+    for (int i = nz - 1; i >= 0; --i) {
+      double2 sv = c[i];
+      for (int jj = i + 1; jj < nz; ++jj) {
+        const double2 p = cmul(at(i, jj), c[jj]);
+        sv.x -= p.x;
+        sv.y -= p.y;
+      }
+      c[i] = cdiv(sv, at(i, i));
+    }
+    for (int i = 0; i < nz; ++i) xcol[perm[i]] = c[i];
+  }
+}
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -659,7 +852,7 @@ class HipNca : public DeviceNca {
     return "";
   }
 
-  std::string aca(const std::vector<AcaJob>& jobs, double eps) override {
+  std::string aca(const std::vector<AcaJob>& jobs, double eps, double abs_exit) override {
     std::vector<int> caps(jobs.size());
     for (size_t q = 0; q < jobs.size(); ++q)
       caps[q] = std::min<int>(kCapFirst, int(std::min(jobs[q].rows->size(), jobs[q].cols->size())));
@@ -678,7 +871,7 @@ class HipNca : public DeviceNca {
           bytes += b;
           ++q1;
         }
-        std::string e = aca_chunk(jobs, todo, q0, q1, caps, eps, again);
+        std::string e = aca_chunk(jobs, todo, q0, q1, caps, eps, abs_exit, again);
         if (!e.empty()) return e;
         q0 = q1;
       }
@@ -730,6 +923,59 @@ class HipNca : public DeviceNca {
     for (size_t t = 0; t < tabs.size(); ++t) {
       V1[t].resize(per);
       for (size_t k = 0; k < per; ++k) V1[t][k] = cplx(h[t * per + k].x, h[t * per + k].y);
+    }
+    return "";
+  }
+
+  std::string qr_solve(const std::vector<QrSolveJob>& jobs) override {
+    // systems above the device limit: the host ColPivQR
+    std::vector<size_t> host_jobs, dev_jobs;
+    for (size_t q = 0; q < jobs.size(); ++q) (jobs[q].n > kQrMax || jobs[q].n < 1 ? host_jobs : dev_jobs).push_back(q);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int h = 0; h < int(host_jobs.size()); ++h) {
+      const QrSolveJob& j = jobs[host_jobs[size_t(h)]];
+      if (j.n < 1) continue;
+      ColPivQR qr;
+      qr.factor(j.A, j.n, j.n);
+      qr.solve(j.B, j.nrhs, j.X);
+    }
+    size_t q0 = 0;
+    while (q0 < dev_jobs.size()) {
+      size_t q1 = q0, ea = 0, eb = 0;
+      while (q1 < dev_jobs.size()) {
+        const QrSolveJob& j = jobs[dev_jobs[q1]];
+        const size_t na = size_t(j.n) * j.n, nb = size_t(j.n) * j.nrhs;
+        if (q1 > q0 && (ea + na + 3 * (eb + nb)) * sizeof(double2) > kChunkBytes / 4) break;
+        ea += na;
+        eb += nb;
+        ++q1;
+      }
+      std::vector<QrDesc> dd(q1 - q0);
+      std::vector<double2> Ah(ea), Bh(eb);
+      long long oa = 0, ob = 0;
+      for (size_t q = q0; q < q1; ++q) {
+        const QrSolveJob& j = jobs[dev_jobs[q]];
+        dd[q - q0] = QrDesc{j.n, j.nrhs, oa, ob};
+        std::memcpy(Ah.data() + oa, j.A, sizeof(double2) * size_t(j.n) * j.n);
+        std::memcpy(Bh.data() + ob, j.B, sizeof(double2) * size_t(j.n) * j.nrhs);
+        oa += (long long)j.n * j.n;
+        ob += (long long)j.n * j.nrhs;
+      }
+      std::string e;
+      if (!(e = qdesc_.put(dd)).empty() || !(e = qa_.put(Ah)).empty() || !(e = qb_.put(Bh)).empty() ||
+          !(e = qx_.reserve(eb)).empty() || !(e = qw_.reserve(eb)).empty())
+        return e;
+      int nmax = 1;
+      for (const QrDesc& d : dd) nmax = std::max(nmax, d.n);
+      hipLaunchKernelGGL(qr_solve_kernel, dim3(unsigned(dd.size())), dim3(64), sizeof(double2) * nmax * nmax, 0,
+                         qdesc_.p, qa_.p, qb_.p, qx_.p, qw_.p);
+      NCA_HIP(hipGetLastError());
+      NCA_HIP(hipMemcpy(Bh.data(), qx_.p, sizeof(double2) * eb, hipMemcpyDeviceToHost));
+      for (size_t q = q0; q < q1; ++q) {
+        const QrSolveJob& j = jobs[dev_jobs[q]];
+        std::memcpy(j.X, Bh.data() + dd[q - q0].b, sizeof(double2) * size_t(j.n) * j.nrhs);
+      }
+      q0 = q1;
     }
     return "";
   }
@@ -796,7 +1042,7 @@ class HipNca : public DeviceNca {
   }
 
   std::string aca_chunk(const std::vector<AcaJob>& jobs, const std::vector<size_t>& todo, size_t q0, size_t q1,
-                        const std::vector<int>& caps, double eps, std::vector<size_t>& again) {
+                        const std::vector<int>& caps, double eps, double abs_exit, std::vector<size_t>& again) {
     const size_t nj = q1 - q0;
     std::vector<AcaDesc> dd(nj);
     std::vector<int> ids, runs;
@@ -840,7 +1086,7 @@ class HipNca : public DeviceNca {
         !(e = piv_.reserve(size_t(pv))).empty() || !(e = res_.reserve(2 * nj)).empty())
       return e;
     hipLaunchKernelGGL(aca_kernel, dim3(unsigned(nj)), dim3(kAT), 0, 0, dk_, adesc_.p, idx_.p, runs_.p, ws_.p,
-                       fl_.p, piv_.p, res_.p, eps);
+                       fl_.p, piv_.p, res_.p, eps, abs_exit);
     NCA_HIP(hipGetLastError());
     std::vector<int> res(2 * nj), pivh(static_cast<size_t>(pv));
     NCA_HIP(hipMemcpy(res.data(), res_.p, res.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -885,6 +1131,8 @@ class HipNca : public DeviceNca {
   DevBuf<unsigned char> fl_;
   DevBuf<double2> ws_, out_;
   DevBuf<long long> off_;
+  DevBuf<QrDesc> qdesc_;
+  DevBuf<double2> qa_, qb_, qx_, qw_;
   std::vector<double2> host_;
 };
 

@@ -254,3 +254,78 @@ def evaluate(S, kappa, seeds, tol=1e-8, restart_len=50):
                                       sg["rel_residual"])
     out.setdefault("failed", 0)
     return out
+
+
+C5_KAPPAS = (40.0, 50.0, 60.0, 70.0, 80.0)
+
+
+def c5_experiment(dilation_sets=((1, 2, 4, 8, 4, 1),), n_train=4096, n_eval=256, epochs=30, tol=1e-8, log=None):
+    """The learned preconditioner at its configuration: depth 6 contexts for
+    k in {40, 50, 60, 70, 80} (one compressed volume potential each), training
+    pairs from n_train solves the solver generates (3-8 seeded bumps, plane
+    waves at random angles and random-field right-hand sides; sample s uses
+    k = C5_KAPPAS[s * 5 // n_train]), one network of 3x3 convolutions (32
+    channels, BF16 weights and activations, 128 x 128 grid) per dilation set,
+    trained by Adam on the GPU, then n_eval held-out systems solved by plain
+    GMRES(50), by GMRES(50) from the network's guess, and by flexible GMRES
+    with the network as right preconditioner in the first cycle. Returns one
+    result dict per dilation set."""
+    log = log or (lambda m: None)
+    net0 = LearnedNet(n_layers=len(dilation_sets[0]), width=32, dilations=dilation_sets[0], grid=128, max_batch=64,
+                      seed=0)
+    t0 = time.time()
+    ctxs = {k: volume_context(k, 6, net0) for k in C5_KAPPAS}
+    log(f"volume potentials for {len(C5_KAPPAS)} wavenumbers: {time.time() - t0:.1f} s")
+    per = n_train // len(C5_KAPPAS)
+    xs, ts = [], []
+    t0 = time.time()
+    for i, k in enumerate(C5_KAPPAS):
+        n = per if i < len(C5_KAPPAS) - 1 else n_train - per * (len(C5_KAPPAS) - 1)
+        x, t, _ = make_pairs(ctxs[k], k, list(range(i * 100_000, i * 100_000 + n)), tol=tol)
+        xs.append(x)
+        ts.append(t)
+    X, T = np.concatenate(xs), np.concatenate(ts)
+    del xs, ts
+    t_data = time.time() - t0
+    log(f"training set: {X.shape[0]} solves in {t_data:.1f} s")
+    results = []
+    for spec in dilation_sets:
+        net = net0 if spec == dilation_sets[0] else LearnedNet(n_layers=len(spec), width=32, dilations=spec, grid=128,
+                                                               max_batch=64, seed=0)
+        net.set_training_data(X, T)
+        t0 = time.time()
+        hist = net.train(epochs=epochs, batch=64, lr=2e-3, lr_decay=0.92, log=log)
+        t_train = time.time() - t0
+        for S in ctxs.values():
+            S.attach_preconditioner(net)
+        ev = {"plain_it": [], "learned_it": [], "guess_only_it": [], "plain_s": [], "learned_s": [],
+              "guess_only_s": [], "initial_residual_learned": [], "max_rel_residual": 0.0, "failed": 0}
+        for i, k in enumerate(C5_KAPPAS):
+            cnt = n_eval // len(C5_KAPPAS) + (1 if i < n_eval % len(C5_KAPPAS) else 0)
+            e = evaluate(ctxs[k], k, list(range(900_000 + i * 1000, 900_000 + i * 1000 + cnt)), tol=tol)
+            for key in ev:
+                if key == "max_rel_residual":
+                    ev[key] = max(ev[key], e[key])
+                else:
+                    ev[key] += e[key]
+        res = {"dilations": list(spec), "receptive_field": 1 + 2 * sum(spec), "train_solves": int(X.shape[0]),
+               "train_data_s": t_data, "train_s": t_train, "epochs": epochs, "loss_first_epoch": hist[0],
+               "loss_last_epoch": hist[-1], "eval_systems": len(ev["plain_it"]),
+               "plain_mean_iterations": float(np.mean(ev["plain_it"])),
+               "learned_fgmres_mean_iterations": float(np.mean(ev["learned_it"])),
+               "learned_guess_only_mean_iterations": float(np.mean(ev["guess_only_it"])),
+               "plain_solves_per_s": len(ev["plain_s"]) / float(np.sum(ev["plain_s"])),
+               "learned_fgmres_solves_per_s": len(ev["learned_s"]) / float(np.sum(ev["learned_s"])),
+               "learned_guess_only_solves_per_s": len(ev["guess_only_s"]) / float(np.sum(ev["guess_only_s"])),
+               "mean_initial_residual_learned_guess": float(np.mean(ev["initial_residual_learned"])),
+               "max_rel_residual": ev["max_rel_residual"], "solves_not_at_tol": ev["failed"],
+               "per_system": {k: ev[k] for k in ("plain_it", "learned_it", "guess_only_it")}}
+        results.append(res)
+        if net is not net0:
+            for S in ctxs.values():
+                S.attach_preconditioner(net0)
+            net.close()
+    for S in ctxs.values():
+        S.close()
+    net0.close()
+    return results

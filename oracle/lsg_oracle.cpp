@@ -483,6 +483,10 @@ const double CN_centers[12][2] = {{-3, -3}, {-1, -3}, {1, -3}, {3, -3}, {3, -1},
 // neighbour kinds as getIfNeighbor numbers them (G_FMM2DTree.hpp:335-516)
 enum { kColleague = 0, kCoarse = 1, kFine = 2, kSepFine = 3 };
 const double epsilonRoundOff = 1e-8;
+// Absolute exit of ACA_only_nodes on the norms of the new cross vectors
+// (GMRES/ACA.hpp:350, 453: 1e-8, the default); lsg_inputs.aca_exit_pow = p
+// replaces it by 10^-p for the experiment showing that this fixed threshold,
+// not TOL_POW, limits the DAFMM's agreement with the dense operator.
 
 struct FMM2DTree {
   Problem Q;
@@ -490,6 +494,7 @@ struct FMM2DTree {
   int nLevels = 0, nChebNodes = 0, rank = 0, N = 0, level_LFR = 2, nCones_LFR = 16, TOL_POW = 10, degreeOfBases = 6,
       nLevelsUniform = 4, yes2DFMM = 1, treeAdaptivity = 6, tree_mode = 0;
   double epsilonTree = 1e-6;
+  double aca_abs_exit = 1e-8;  // ACA.hpp:350, 453
   double L = 0.5, kappa = 1.0;
   std::vector<double> boxRadius, ConeAperture;
   std::vector<int> nCones;
@@ -514,6 +519,7 @@ struct FMM2DTree {
     degreeOfBases = in.degreeOfBases; TOL_POW = in.TOL_POW; nLevelsUniform = in.nLevelsUniform;
     treeAdaptivity = in.treeAdaptivity; tree_mode = in.tree_mode;
     epsilonTree = std::pow(10, -treeAdaptivity);  // :301
+    aca_abs_exit = in.aca_exit_pow ? std::pow(10.0, -in.aca_exit_pow) : 1e-8;
     findPhi = in.find_phi != 0;
     Q.kappa = kappa; Q.angle = in.inc_angle; Q.con = in.contrast;
     rank = nChebNodes * nChebNodes;
@@ -1191,7 +1197,7 @@ struct FMM2DTree {
         for (int l = 0; l <= computed_rank - 1; ++l)
           for (size_t s = 0; s < Uvec[l].size(); ++s) col[s] -= Vvec[l][col_index] * Uvec[l][s];
         u = col;
-        if (norm(u) < 1e-8 || norm(v) < 1e-8) { row_bases.pop_back(); col_bases.pop_back(); break; }
+        if (norm(u) < aca_abs_exit || norm(v) < aca_abs_exit) { row_bases.pop_back(); col_bases.pop_back(); break; }
         Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row_temp);
         ++computed_rank;
         rc.erase(col_index); rr.erase(row_index);
@@ -1236,7 +1242,7 @@ struct FMM2DTree {
         for (int l = 0; l <= computed_rank - 1; ++l)
           for (size_t s = 0; s < Uvec[l].size(); ++s) row[s] -= Vvec[l][row_index] * Uvec[l][s];
         u = row;
-        if (norm(u) < 1e-8 || norm(v) < 1e-8) { col_bases.pop_back(); row_bases.pop_back(); break; }
+        if (norm(u) < aca_abs_exit || norm(v) < aca_abs_exit) { col_bases.pop_back(); row_bases.pop_back(); break; }
         Uvec.push_back(u); Vvec.push_back(v); Arows.push_back(row_temp);
         ++computed_rank;
         rr.erase(row_index); rc.erase(col_index);

@@ -145,3 +145,23 @@ def test_trained_network_gain():
     assert hist[-1] < hist[0]
     assert ev["max_rel_residual"] <= 1e-8 and ev["failed"] == 0
     assert np.mean(ev["initial_residual_learned"]) < 1.0     # better than the zero guess
+
+
+def test_c5_learned_preconditioner_at_config():
+    """C5 at its configured sizes: the 6-layer network (32 channels, 3x3,
+    BF16, 128 x 128 grid) trained on 4096 solves the solver generates (depth 6,
+    k in [40, 80], 3-8 bumps, random angles), evaluated on 256 held-out
+    systems at tol 1e-8. Measured: the network's initial guess saves steps
+    (56.9 against 58.3 plain GMRES(50) on average; its initial residual is
+    0.57), while flexible GMRES with the network as right preconditioner
+    takes many more (89.2): the network's correction does not cluster the
+    spectrum of the strongly scattering operator. Every solve is FP64-verified."""
+    res = LN.c5_experiment(n_train=4096, n_eval=256, log=lambda m: None)[0]
+    print({k: v for k, v in res.items() if k != "per_system"}, flush=True)
+    assert res["eval_systems"] == 256 and res["train_solves"] == 4096
+    assert res["max_rel_residual"] <= 1e-8 and res["solves_not_at_tol"] == 0
+    assert res["loss_last_epoch"] < 0.5 * res["loss_first_epoch"]
+    assert res["mean_initial_residual_learned_guess"] < 0.8
+    # the gain of the guess, and the loss of the right preconditioner
+    assert res["learned_guess_only_mean_iterations"] < res["plain_mean_iterations"]
+    assert res["learned_fgmres_mean_iterations"] > res["plain_mean_iterations"]

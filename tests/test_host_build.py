@@ -85,6 +85,19 @@ def test_tables_and_skeletons_match_oracle(depth, kappa, q):
     assert np.array_equal(h.skeleton(), o.skeleton())
 
 
+def test_skeletons_match_oracle_with_a_smaller_aca_exit():
+    """lsg_inputs.aca_exit_pow (the ACA's absolute exit 10^-p instead of the
+    reference's 1e-8) changes the skeletons the same way in the product's
+    build and in the oracle."""
+    inp = H.inputs(depth=4, kappa=30.0, q=6, seed=2, tol_pow=14)
+    ref = L.HostBuild(inp).skeleton()
+    inp.aca_exit_pow = 16
+    h = L.HostBuild(inp)
+    o = O.Oracle(inp.to_c())
+    assert np.array_equal(h.skeleton(), o.skeleton())
+    assert h.skeleton().size > ref.size          # the ACAs run longer: larger skeletons
+
+
 def test_bumps_are_seeded_and_in_range():
     a = L.bumps_from_seed(7, 8)
     b = L.bumps_from_seed(7, 8)

@@ -51,6 +51,31 @@ def test_dafmm_matvec_matches_dense_operator(depth, kappa, q, find_phi, tol):
     assert e < tol, e
 
 
+def dense_gap(depth, kappa, q, find_phi, tol_pow, exit_pow):
+    inp = H.inputs(depth=depth, kappa=kappa, q=q, seed=0, find_phi=find_phi, tol_pow=tol_pow)
+    inp.aca_exit_pow = exit_pow
+    o = O.Oracle(inp.to_c())
+    x = H.rand_vec(o.N, 1)
+    e = H.relerr(o.matvec(x), o.dense_matvec(x))
+    o.close()
+    return e
+
+
+def test_aca_absolute_exit_sets_the_dense_floor():
+    """What limits the DAFMM against the dense operator is the ACA's fixed
+    absolute exit on the new cross vectors' norms, 1e-8 (GMRES/ACA.hpp:350,
+    453), not its relative stopping tolerance 10^-TOL_POW: with the
+    reference's exit, V alone at depth 3 stays at 5.7e-6 whether TOL_POW is
+    10 or 14; with the exit at 1e-16 it falls with TOL_POW, to 6.6e-9 at
+    TOL_POW 14, and I + D V to 2.7e-13 (Gaussian) and 4.7e-12 (bumps)."""
+    ref10 = dense_gap(3, 10.0, 0, 0, 10, 0)
+    ref14 = dense_gap(3, 10.0, 0, 0, 14, 0)
+    assert ref10 > 1e-6 and abs(ref14 - ref10) <= 1e-2 * ref10       # no dependence on TOL_POW
+    assert dense_gap(3, 10.0, 0, 0, 14, 16) < 1e-8
+    assert dense_gap(3, 10.0, 0, 1, 14, 16) < 1e-11
+    assert dense_gap(3, 10.0, 6, 1, 14, 16) < 1e-10
+
+
 @pytest.mark.parametrize("depth,kappa,q,find_phi", [(3, 10.0, 0, 1), (3, 10.0, 6, 0), (4, 20.0, 0, 1), (4, 30.0, 6, 1)])
 def test_dafmm_passes_match_numpy_restatement(depth, kappa, q, find_phi):
     """Every pass of the oracle's matvec (M2M, cone M2L, L2L with pass-throughs,
