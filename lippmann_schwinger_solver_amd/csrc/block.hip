@@ -11,13 +11,11 @@ namespace lsg {
 
 namespace {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int kT = 256;
 constexpr int kNB = kBlockRhs;      // 64 right-hand sides = 4 column tiles of 16
 constexpr int kLD = 2 * kNB;        // doubles per block row
-constexpr int kBCK = 32;            // complex operator columns staged in LDS per step
 constexpr int kBMaxRows = 64;       // complex rows per destination (8 row tiles of 16 reals)
-constexpr int kSplitCols = 256;     // columns per workgroup in split passes
+constexpr int kSplitCols = 256;     // columns per wavefront in split passes
 constexpr int kSplitMaxDest = 2048; // passes with fewer destinations (and longer rows) are split
 
 #define HIP_OK(expr)                                                              \
@@ -49,87 +47,271 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int k4, int v) { return 4 * k4 + v; }
 };
 
-// One workgroup per destination; wave w computes right-hand sides [16w, 16w+16)
-// for all row tiles. Real-ified operator fragment (16 x 4) per lane l:
-// row 16*mt + l%16, column 4*ks + l/16, taken from the complex entry in LDS;
-// the source fragment (4 x 16): real row 4*ks + l/16 (column c = 2ks + (l/16)/2,
-// Re or Im), right-hand side l%16. R is the pass precision (operators, pass
-// state, accumulation); the input X and output Y blocks are FP64. Source rows
-// below nx are read from X, the others from the pass state (the FP32 path
-// copies X into the state once, rounded, and passes nx = 0).
+// Workgroups are dispatched round-robin over the 8 XCDs. With kXcdGroup = G
+// > 0, position b of nb is mapped so that each XCD takes runs of G
+// consecutive positions (neighbouring destinations share source rows, which
+// then meet in one XCD's L2) while the runs still interleave over the XCDs
+// (destinations of unequal work stay spread over the chip); G = 0 keeps the
+// dispatch order. A bijection on [0, nb).
+#ifndef LSG_XCD_GROUP
+#define LSG_XCD_GROUP 0
+#endif
+constexpr int kXcdGroup = LSG_XCD_GROUP;
+__device__ __forceinline__ int xcd_order(int b, int nb) {
+  if (kXcdGroup <= 0) return b;
+  constexpr int span = 8 * (kXcdGroup > 0 ? kXcdGroup : 1);
+  const int full = nb - nb % span;   // positions in complete spans
+  if (b >= full) return b;
+  const int x = b & 7, i = b >> 3;    // XCD, its i-th workgroup
+  return (i / kXcdGroup) * span + x * kXcdGroup + i % kXcdGroup;
+}
+
+// complex operator columns per LDS chunk
+#ifndef LSG_BLOCK_KC
+#define LSG_BLOCK_KC 16
+#endif
+constexpr int kKC = LSG_BLOCK_KC;
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// gfx950 LDS-direct loads: lane l of the wavefront copies 4 (16) bytes from g
+// to wave_base + 4 l (16 l), wave_base the same for every lane; completion is
+// counted by vmcnt.
+__device__ __forceinline__ void lds_copy4(const void* g, void* wave_base) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)wave_base, 4, 0, 0);
+}
+__device__ __forceinline__ void lds_copy16(const void* g, void* wave_base) {
+  __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)wave_base, 16, 0, 0);
+}
+
+// Operand stream of one workgroup: up to three segments, each a complex
+// column-major block A_k (Rw rows, C_k columns) whose column c reads source
+// row S_k[c]; walked in chunks of kKC columns.
+template <class R>
+struct Segments {
+  const typename Mfma<R>::c2 *A0, *A1, *A2;
+  const int *S0, *S1, *S2;
+  int C0, C1, C2;
+  int n0, n01, nch;  // chunk counts: segment 0, segments 0-1, all
+};
+// a segment as the workgroup reads it from LDS: operator, indices, columns, first chunk
+struct SegDesc {
+  const void* A;
+  const int* S;
+  int C, first;
+};
+
+// Block product of one workgroup on 64 right-hand sides: acc (wave w: RHS
+// 16 w .. 16 w + 15) += sum over the segments of A_k X(rows S_k).
+// Real-ified product [Yr; Yi] = [[Ar, -Ai], [Ai, Ar]] [Xr; Xi] with
+// interleaved real rows 2r + a and real columns 2c + b; MFMA 16 x 16 x 4:
+// lane l feeds A(real row 16 mt + l%16, real column 4 ks + l/16), i.e. the
+// entry (row 8 mt + (l%16)/2, column 2 ks + (l/16)/2), Re when
+// l%2 == (l/16)%2, else Im, times rm[mt] (sign -1 for (a, b) = (0, 1), 0 for
+// rows past Rw, segment 0's row scale where asked), and X(real row
+// 4 ks + l/16, RHS 16 w + l%16). Pipeline per chunk i: the operator columns
+// of chunk i + 1 are copied into LDS by LDS-direct loads (double buffered,
+// shared by the 4 waves), the column indices of chunk i + 2 into a second
+// pair of LDS buffers, and each wave loads its source values of chunk i + 1
+// into registers, while it runs the MFMAs of chunk i; one wait and one
+// barrier per chunk. Columns past C_k read zeros (operator: the zero row of
+// the pass state; source: row zrow).
+template <class R, int MT, int MTU>
+__device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], const Segments<R>& sg, int Rw,
+                                              const R (&rm0)[MT], const R (&rm1)[MT], const R* __restrict__ X,
+                                              const R* __restrict__ bst, int nx, int zrow, R* As0, R* As1,
+                                              int* Sr, const SegDesc* sd) {
+  constexpr int KS = kKC / 2;                        // MFMA k-steps per chunk
+  constexpr int PB = sizeof(R) == 8 ? 16 : 4;        // bytes per operator copy (one complex FP64, or one float)
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i16 = lane & 15, k4 = lane >> 4, kb = k4 & 1, kc = k4 >> 1;
+  const int sel = (i16 & 1) == kb ? 0 : 1;
+  int roff[MTU];
+#pragma unroll
+  for (int mt = 0; mt < MTU; ++mt) roff[mt] = 2 * min(8 * mt + (i16 >> 1), Rw - 1) + sel;
+  const R* zero = bst + (size_t)zrow * kLD;
+  const int per_col = 2 * Rw * int(sizeof(R)) / PB;  // copies per operator column
+  const int n_cp = kKC * per_col;
+  // segment of chunk i (descriptors in LDS, picked by a uniform index)
+  struct Pick {
+    const char* A;
+    const int* S;
+    int C, c0;
+    bool seg0;
+  };
+  auto seg_of = [=](int i) __attribute__((always_inline)) {
+    const SegDesc& d = sd[(i >= sg.n0 ? 1 : 0) + (i >= sg.n01 ? 1 : 0)];
+    Pick p;
+    p.A = reinterpret_cast<const char*>(d.A);
+    p.S = d.S;
+    p.C = d.C;
+    p.c0 = (i - d.first) * kKC;
+    p.seg0 = i < sg.n0;
+    return p;
+  };
+  // operator columns of chunk i into As (every thread), indices into Sb (wave 0)
+  auto copy_ops = [=](int i, R* As) __attribute__((always_inline)) {
+    const Pick p = seg_of(i);
+    const int c0 = p.c0, Ck = p.C;
+    const char* src = p.A + (size_t)c0 * per_col * PB;
+    const char* z = reinterpret_cast<const char*>(zero);
+    const int valid = min(kKC, Ck - c0) * per_col;
+    for (int q = 0; q * kT < n_cp; ++q) {
+      const int e = q * kT + t;
+      const char* g = e < valid ? src + (size_t)e * PB : z + (e & (kLD * int(sizeof(R)) / PB - 1)) * PB;
+      char* wb = reinterpret_cast<char*>(As) + (size_t)(q * kT + 64 * w) * PB;
+      if (e < n_cp) {
+        if (PB == 16) lds_copy16(g, wb);
+        else lds_copy4(g, wb);
+      }
+    }
+  };
+  auto copy_idx = [=](int i, int* Sb) __attribute__((always_inline)) {
+    if (w != 0 || i >= sg.nch) return;
+    const Pick p = seg_of(i);
+    const int c0 = p.c0, Ck = p.C;
+    const int* Sk = p.S;
+    if (lane < kKC) lds_copy4(Sk + min(c0 + lane, Ck - 1), Sb);
+  };
+  // this lane's source values of chunk i (indices in Sb)
+  auto load_src = [=](int i, const int* Sb, R (&bv)[KS]) __attribute__((always_inline)) {
+    const Pick p = seg_of(i);
+    const int c0 = p.c0, Ck = p.C;
+    int sv[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) sv[ks] = Sb[2 * ks + kc];  // every index of the chunk was copied
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int r = c0 + 2 * ks + kc < Ck ? sv[ks] : zrow;
+      bv[ks] = (r < nx ? X : bst)[(size_t)r * kLD + kb * kNB + 16 * w + i16];
+    }
+  };
+  // index buffers: Sc holds chunk i + 1's indices, Sf receives chunk i + 2's
+  int* Sc = Sr + kKC;
+  int* Sf = Sr;
+  R bcur[KS], bnext[KS];
+  copy_idx(0, Sf);
+  copy_idx(1, Sc);
+  copy_ops(0, As0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __syncthreads();
+  load_src(0, Sf, bcur);
+  // nothing is in flight when the loop starts, so the first MFMAs wait only for LDS
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  for (int i = 0; i < sg.nch; ++i) {
+    const bool odd = i & 1;
+    const R* As = odd ? As1 : As0;
+    const bool seg0 = i < sg.n0;
+    if (i + 1 < sg.nch) copy_ops(i + 1, odd ? As0 : As1);
+    copy_idx(i + 2, Sf);
+    if (i + 1 < sg.nch) load_src(i + 1, Sc, bnext);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int cl = 2 * ks + kc;
+#pragma unroll
+      for (int mt = 0; mt < MTU; ++mt)
+        acc[mt] = Mfma<R>::op(As[cl * 2 * Rw + roff[mt]] * (seg0 ? rm0[mt] : rm1[mt]), bcur[ks], acc[mt]);
+    }
+    // the MFMAs stay ahead of the wait for the next chunk's loads
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) bcur[ks] = bnext[ks];
+    int* tmp = Sc;
+    Sc = Sf;
+    Sf = tmp;
+  }
+}
+
+// The lane's operator multipliers: the real-ification sign, 0 for rows past
+// Rw, times the row's scale (scale: per complex row, or null).
+template <class R, int MT>
+__device__ __forceinline__ void lane_rmul(R (&rmul)[MT], int Rw, int lane, const double* scale) {
+  const int i16 = lane & 15, kb = (lane >> 4) & 1;
+  const R sgn = ((i16 & 1) == 0 && kb == 1) ? R(-1) : R(1);
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int r = 8 * mt + (i16 >> 1);
+    rmul[mt] = r < Rw ? (scale ? R(double(sgn) * scale[r]) : sgn) : R(0);
+  }
+}
+
+// block_product over the row tiles that hold rows of the destinations (a
+// workgroup-uniform choice); the LDS stage buffers are sized for MT tiles.
+template <class R, int MT>
+__device__ __forceinline__ void block_product_rows(typename Mfma<R>::acc (&acc)[MT], const Segments<R>& sg, int Rw,
+                                                   const R (&rm0)[MT], const R (&rm1)[MT], const R* __restrict__ X,
+                                                   const R* __restrict__ bst, int nx, int zrow) {
+  __shared__ R As0[2 * 8 * MT * kKC], As1[2 * 8 * MT * kKC];
+  __shared__ int Sr[2 * kKC];
+  __shared__ SegDesc sd[3];
+  if (sg.nch <= 0) return;
+  if (threadIdx.x == 0) {
+    sd[0] = SegDesc{sg.A0, sg.S0, sg.C0, 0};
+    sd[1] = SegDesc{sg.A1, sg.S1, sg.C1, sg.n0};
+    sd[2] = SegDesc{sg.A2, sg.S2, sg.C2, sg.n01};
+  }
+  __syncthreads();
+  const int mtu = (2 * Rw + 15) >> 4;
+  if (mtu >= MT) {
+    block_product<R, MT, MT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+  } else if (MT > 1 && mtu == MT - 1) {
+    block_product<R, MT, (MT > 1 ? MT - 1 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+  } else if (MT > 2 && mtu == MT - 2) {
+    block_product<R, MT, (MT > 2 ? MT - 2 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+  } else {
+    block_product<R, MT, (MT > 3 ? MT - 3 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+  }
+}
+
+// One pass of the plan on 64 right-hand sides: one workgroup per destination
+// (or per column segment of a destination in split passes: kseg > 0, item =
+// d * nseg + segment, partial products to part, summed by
+// bsplit_reduce_kernel). R is the pass precision (operators, pass state,
+// accumulation); Y is FP64. MT row tiles of 16 reals: 2 * (rows of every
+// destination of the pass) <= 16 MT.
 template <bool ACC, class R, int MT>
 __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* __restrict__ ops,
                                                    const int* __restrict__ src, R* bst, const R* __restrict__ X,
                                                    double* Y, int N, const int* __restrict__ rows,
                                                    const int* __restrict__ ncol, const int* __restrict__ dst,
                                                    const long long* __restrict__ op_off,
-                                                   const long long* __restrict__ src_off, int nx, int kseg, int nseg,
-                                                   R* __restrict__ part) {
-  typedef typename Mfma<R>::c2 C2;
+                                                   const long long* __restrict__ src_off, int nx, int zrow, int kseg,
+                                                   int nseg, R* __restrict__ part) {
   typedef typename Mfma<R>::acc Acc;
-  __shared__ C2 As[kBMaxRows * kBCK];
-  __shared__ int Ss[kBCK];
-  // split passes (kseg > 0): workgroup (d, sg) takes columns [sg kseg, (sg+1) kseg) of
-  // destination d and writes its partial product to part; bsplit_reduce_kernel sums them
-  const int d = kseg > 0 ? blockIdx.x / nseg : blockIdx.x;
-  const int sg = kseg > 0 ? blockIdx.x % nseg : 0;
+  const int item = xcd_order(blockIdx.x, gridDim.x);
+  const int d = kseg > 0 ? item / nseg : item;
+  const int sgi = kseg > 0 ? item % nseg : 0;
   const int Rw = rows[d];
   const int Cfull = ncol[d];
-  const int cbeg = kseg > 0 ? min(Cfull, sg * kseg) : 0;
+  const int cbeg = kseg > 0 ? min(Cfull, sgi * kseg) : 0;
   const int C = kseg > 0 ? min(Cfull, cbeg + kseg) : Cfull;
-  const C2* A = ops + op_off[d];
-  const int* S = src + src_off[d];
-  const int dd = dst[d];
-  const int t = threadIdx.x, lane = t & 63, nt = t >> 6;
-  const int i16 = lane & 15, k4 = lane >> 4;
-  const int q = (nt << 4) + i16;
-  Acc acc[MT];   // MT row tiles of 16 reals: 2 * (rows of every destination of the pass) <= 16 MT
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Segments<R> sg;
+  sg.A0 = sg.A1 = sg.A2 = ops + op_off[d] + (size_t)cbeg * Rw;
+  sg.S0 = sg.S1 = sg.S2 = src + src_off[d] + cbeg;
+  sg.C0 = C - cbeg;
+  sg.C1 = sg.C2 = 0;
+  sg.n0 = sg.n01 = sg.nch = (sg.C0 + kKC - 1) / kKC;
+  Acc acc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
-  for (int c0 = cbeg; c0 < C; c0 += kBCK) {
-    const int Cc = min(kBCK, C - c0);
-    __syncthreads();
-    const C2* Ac = A + (size_t)c0 * Rw;  // the chunk's columns are contiguous (column-major)
-    for (int e = t; e < Cc * Rw; e += kT) As[e] = Ac[e];
-    if (t < Cc) Ss[t] = S[c0 + t];
-    __syncthreads();
-    R bv[kBCK / 2];
-#pragma unroll
-    for (int ks = 0; ks < kBCK / 2; ++ks) {
-      const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
-      bv[ks] = 0;
-      if (cl < Cc) {
-        const int row = Ss[cl];
-        bv[ks] = row < nx ? X[(size_t)row * kLD + b * kNB + q] : bst[(size_t)row * kLD + b * kNB + q];
-      }
-    }
-#pragma unroll
-    for (int ks = 0; ks < kBCK / 2; ++ks) {
-      if (2 * ks < Cc) {
-        const int cl = 2 * ks + (k4 >> 1), b = k4 & 1;
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const int ri = (mt << 4) + i16;
-          const int r = ri >> 1, a = ri & 1;
-          R av = 0;
-          if (r < Rw && cl < Cc) {
-            const R* e = reinterpret_cast<const R*>(&As[cl * Rw + r]);
-            av = e[a == b ? 0 : 1];
-            if (a == 0 && b == 1) av = -av;
-          }
-          acc[mt] = Mfma<R>::op(av, bv[ks], acc[mt]);
-        }
-      }
-    }
-  }
+  R rm[MT];
+  lane_rmul<R, MT>(rm, Rw, lane, nullptr);
+  block_product_rows<R, MT>(acc, sg, Rw, rm, rm, X, bst, nx, zrow);
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int q = 16 * w + i16;
   if (kseg > 0) {
-    R* P = part + (size_t)blockIdx.x * (MT * 16) * kNB;
+    R* P = part + (size_t)item * (MT * 16) * kNB;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
       for (int v = 0; v < 4; ++v) P[(size_t)((mt << 4) + Mfma<R>::row(k4, v)) * kNB + q] = acc[mt][v];
     return;
   }
+  const int dd = dst[d];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
@@ -150,6 +332,66 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* _
           *p = val;
         }
       }
+    }
+  }
+}
+
+// Output rows of the leaves of a uniform tree, one workgroup per leaf:
+//   near field: the 9 colleague tables T (rank x 9 rank, shared by all leaves)
+//   applied to the x rows of the colleagues (near_src: 9 rank rows per leaf,
+//   the zero row for an absent colleague); find_phi: scaled row-wise by qk2
+//   (applied to the table rows as they are loaded);
+// then the leaf's destinations in the fused leaf passes (leaf_dest[2 leaf + k],
+// -1 for none; rank rows at N + leaf rank) accumulated on top, and the rows
+// stored once: Y = (find_phi ? X : 0) + qk2 .* (T x) + leaf passes.
+template <class R, class XT, int MT>
+__global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
+                                                   const int* __restrict__ near_src, int rank,
+                                                   const typename Mfma<R>::c2* __restrict__ ops,
+                                                   const int* __restrict__ src, const int* __restrict__ ncol,
+                                                   const long long* __restrict__ op_off,
+                                                   const long long* __restrict__ src_off,
+                                                   const int* __restrict__ leaf_dest, const R* __restrict__ X,
+                                                   const R* __restrict__ bst, int nx, const XT* __restrict__ Xin,
+                                                   const double* __restrict__ qk2, int find_phi, double* __restrict__ Y,
+                                                   int zrow) {
+  typedef typename Mfma<R>::acc Acc;
+  const int leaf = xcd_order(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  Segments<R> sg;
+  sg.A0 = T;
+  sg.S0 = near_src + (size_t)leaf * 9 * rank;
+  sg.C0 = 9 * rank;
+  const int d1 = leaf_dest[2 * leaf], d2 = leaf_dest[2 * leaf + 1];
+  sg.A1 = d1 >= 0 ? ops + op_off[d1] : T;
+  sg.S1 = d1 >= 0 ? src + src_off[d1] : sg.S0;
+  sg.C1 = d1 >= 0 ? ncol[d1] : 0;
+  sg.A2 = d2 >= 0 ? ops + op_off[d2] : T;
+  sg.S2 = d2 >= 0 ? src + src_off[d2] : sg.S0;
+  sg.C2 = d2 >= 0 ? ncol[d2] : 0;
+  sg.n0 = (sg.C0 + kKC - 1) / kKC;
+  sg.n01 = sg.n0 + (sg.C1 + kKC - 1) / kKC;
+  sg.nch = sg.n01 + (sg.C2 + kKC - 1) / kKC;
+  Acc acc[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
+  R rm0[MT], rm1[MT];
+  lane_rmul<R, MT>(rm0, rank, lane, find_phi ? qk2 + (size_t)leaf * rank : nullptr);
+  lane_rmul<R, MT>(rm1, rank, lane, nullptr);
+  block_product_rows<R, MT>(acc, sg, rank, rm0, rm1, X, bst, nx, zrow);
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int q = 16 * w + i16;
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ri = (mt << 4) + Mfma<R>::row(k4, v);
+      const int r = ri >> 1, a = ri & 1;
+      if (r >= rank) continue;
+      const size_t o = ((size_t)leaf * rank + r) * kLD + a * kNB + q;
+      double val = double(acc[mt][v]);
+      if (find_phi) val += double(Xin[o]);
+      Y[o] = val;
     }
   }
 }
@@ -180,15 +422,6 @@ __global__ __launch_bounds__(kT) void bsplit_reduce_kernel(const R* __restrict__
       *p = val;
     }
   }
-}
-
-// Y = X + qk2 .* Y on the output rows (find_phi); the near-field pass stored
-// the bare colleague sums in Y.
-template <class XT>
-__global__ void bnear_finish_kernel(const XT* __restrict__ X, double* __restrict__ Y, const double* __restrict__ qk2,
-                                    long long n_total) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_total) Y[i] = double(X[i]) + qk2[i / kLD] * Y[i];
 }
 
 // Y = X + qk2c .* Y with a contrast per right-hand side (qk2c[n * 64 + q])
@@ -550,50 +783,77 @@ std::string BlockOperator::setup(DeviceOperator& o) {
   }
   HIP_OK(hipMalloc((void**)&bstate, sizeof(double) * size_t(o.n_state + 1) * kLD));
   HIP_OK(hipMemset(bstate, 0, sizeof(double) * size_t(o.n_state + 1) * kLD));
-  // near field as one pass: leaf l reads the rank x (9 rank) side-by-side
-  // tables (uniform tree; an adaptive tree's near field is already a pass)
   if (!o.T) return "";
+  // Uniform tree: the leaf kernel writes every y row (near field, qk2, x). The
+  // leaf M2L and leaf L2L passes are folded into it when each of them is one
+  // accumulating pass whose destinations are whole leaves (rank rows at
+  // N + leaf rank, at most one per leaf) and no other pass writes y rows;
+  // otherwise they run as passes after it.
   const int L = o.n_leaves, rk = o.rank;
+  if (int64_t(L) * rk != o.N) return "block matvec: a uniform tree must have N = leaves x rank";
+  std::vector<int> ld(size_t(L) * 2, -1);
+  bool fuse = o.near.empty() && o.leaf_m2l.size() <= 1 && o.leaf_l2l.size() <= 1;
+  {
+    const int64_t nd = o.passes.empty() ? 0 : o.passes.back().dest_base + o.passes.back().ndest;
+    std::vector<int> rows_h(static_cast<size_t>(nd)), dst_h(static_cast<size_t>(nd));
+    if (nd) {
+      HIP_OK(hipMemcpy(rows_h.data(), o.d_rows, rows_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(dst_h.data(), o.d_dst, dst_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    auto writes_y = [&](const DevPass& p) {
+      for (int d = 0; d < p.ndest && fuse; ++d) {
+        const int64_t a = dst_h[size_t(p.dest_base + d)], b = a + rows_h[size_t(p.dest_base + d)];
+        if (a < 2 * int64_t(o.N) && b > o.N) return true;
+      }
+      return false;
+    };
+    for (int pi : o.up) fuse = fuse && !writes_y(o.passes[size_t(pi)]);
+    for (int pi : o.down) fuse = fuse && !writes_y(o.passes[size_t(pi)]);
+    const std::vector<int>* lists[2] = {&o.leaf_m2l, &o.leaf_l2l};
+    for (int k = 0; k < 2 && fuse; ++k) {
+      if (lists[k]->empty()) continue;
+      const DevPass& p = o.passes[size_t(lists[k]->front())];
+      fuse = p.kind == 1;
+      for (int d = 0; d < p.ndest && fuse; ++d) {
+        const int64_t g = p.dest_base + d;
+        const int64_t off = int64_t(dst_h[size_t(g)]) - o.N;
+        const int64_t leaf = off / rk;
+        fuse = rows_h[size_t(g)] == rk && off >= 0 && off % rk == 0 && leaf < L && ld[size_t(2 * leaf + k)] < 0;
+        if (fuse) ld[size_t(2 * leaf + k)] = int(g);
+      }
+    }
+  }
+  leaf_fused = fuse;
+  if (!fuse) std::fill(ld.begin(), ld.end(), -1);
+  HIP_OK(hipMalloc((void**)&leaf_dest, ld.size() * sizeof(int)));
+  HIP_OK(hipMemcpy(leaf_dest, ld.data(), ld.size() * sizeof(int), hipMemcpyHostToDevice));
+  // near-field source rows: leaf l reads colleague s's x rows (nbl rank + c),
+  // the state's zero row where the colleague is absent
   std::vector<int> nbr_h(size_t(L) * 9);
   HIP_OK(hipMemcpy(nbr_h.data(), o.nbr, nbr_h.size() * sizeof(int), hipMemcpyDeviceToHost));
-  std::vector<int> srcs(size_t(L) * 9 * rk), rws(L, rk), ncs(L, 9 * rk), dsts(L);
-  std::vector<long long> opo(L, 0), so(L);
-  for (int l = 0; l < L; ++l) {
-    for (int s = 0; s < 9; ++s) {
-      const int nbl = nbr_h[size_t(l) * 9 + s];
-      for (int c = 0; c < rk; ++c)
-        srcs[(size_t(l) * 9 + s) * rk + c] = nbl >= 0 ? nbl * rk + c : int(o.n_state);  // zero row
+  std::vector<int> ns(size_t(L) * 9 * rk);
+  for (int l = 0; l < L; ++l)
+    for (int sl = 0; sl < 9; ++sl) {
+      const int nbl = nbr_h[size_t(l) * 9 + sl];
+      for (int c = 0; c < rk; ++c) ns[(size_t(l) * 9 + sl) * rk + c] = nbl >= 0 ? nbl * rk + c : int(o.n_state);
     }
-    dsts[l] = o.N + l * rk;
-    so[l] = (long long)l * 9 * rk;
-  }
-  HIP_OK(hipMalloc((void**)&near_src, srcs.size() * sizeof(int)));
-  HIP_OK(hipMemcpy(near_src, srcs.data(), srcs.size() * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMalloc((void**)&near_rows, L * sizeof(int)));
-  HIP_OK(hipMemcpy(near_rows, rws.data(), L * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMalloc((void**)&near_ncol, L * sizeof(int)));
-  HIP_OK(hipMemcpy(near_ncol, ncs.data(), L * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMalloc((void**)&near_dst, L * sizeof(int)));
-  HIP_OK(hipMemcpy(near_dst, dsts.data(), L * sizeof(int), hipMemcpyHostToDevice));
-  HIP_OK(hipMalloc((void**)&near_op, L * sizeof(long long)));
-  HIP_OK(hipMemcpy(near_op, opo.data(), L * sizeof(long long), hipMemcpyHostToDevice));
-  HIP_OK(hipMalloc((void**)&near_srcoff, L * sizeof(long long)));
-  HIP_OK(hipMemcpy(near_srcoff, so.data(), L * sizeof(long long), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc((void**)&near_src, ns.size() * sizeof(int)));
+  HIP_OK(hipMemcpy(near_src, ns.data(), ns.size() * sizeof(int), hipMemcpyHostToDevice));
   return "";
 }
 
 void BlockOperator::release() {
   work.release();
-  void* ptrs[] = {bstate, near_src, near_rows, near_ncol, near_dst, near_op, near_srcoff, qk2_cols, bstate32,
-                  split_part};
+  void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part};
   qk2_cols = nullptr;
   bstate32 = nullptr;
   split_part = nullptr;
+  leaf_dest = nullptr;
+  near_src = nullptr;
+  leaf_fused = false;
   for (void* p : ptrs)
     if (p) hipFree(p);
   bstate = nullptr;
-  near_src = near_rows = near_ncol = near_dst = nullptr;
-  near_op = near_srcoff = nullptr;
   op = nullptr;
 }
 
@@ -602,14 +862,15 @@ static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename M
                             R* bst, const R* X, int nx, double* Y, const int* rows, const int* ncol, const int* dst,
                             const long long* op_off, const long long* src_off, int nseg) {
   hipStream_t s = bo.op->stream;
+  const int zrow = int(bo.op->n_state);  // the pass state's all-zero row
   if (nseg <= 1) {
     hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N, rows,
-                       ncol, dst, op_off, src_off, nx, 0, 1, (R*)nullptr);
+                       ncol, dst, op_off, src_off, nx, zrow, 0, 1, (R*)nullptr);
     return;
   }
   R* part = reinterpret_cast<R*>(bo.split_part);
   hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest * nseg), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N,
-                     rows, ncol, dst, op_off, src_off, nx, kSplitCols, nseg, part);
+                     rows, ncol, dst, op_off, src_off, nx, zrow, kSplitCols, nseg, part);
   hipLaunchKernelGGL((bsplit_reduce_kernel<ACC, R>), dim3(ndest), dim3(kT), 0, s, part, nseg, MT * 16, rows, dst, bst,
                      Y, bo.op->N);
 }
@@ -682,18 +943,44 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, double* Y) {
     Xr = bst;
     nx = 0;
   }
+  // the leaf kernel: near field, qk2 and x, and the leaf passes when folded in
+  auto leaf = [&]() {
+    const C2* ops = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.ops
+                                                                                : (const void*)o.ops32);
+    const int mt = std::max(1, (2 * o.rank + 15) / 16);
+    const dim3 grid(o.n_leaves);
+#define LSG_LEAF_CASE(M)                                                                                        \
+  case M:                                                                                                       \
+    hipLaunchKernelGGL((bleaf_kernel<R, XT, M>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,     \
+                       o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y, \
+                       int(o.n_state));                                                                         \
+    break;
+    switch (mt) {
+      LSG_LEAF_CASE(1)
+      LSG_LEAF_CASE(2)
+      LSG_LEAF_CASE(3)
+      LSG_LEAF_CASE(4)
+      LSG_LEAF_CASE(5)
+      LSG_LEAF_CASE(6)
+      LSG_LEAF_CASE(7)
+      default:
+        hipLaunchKernelGGL((bleaf_kernel<R, XT, 8>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,
+                           o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y,
+                           int(o.n_state));
+    }
+#undef LSG_LEAF_CASE
+  };
   for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Y);
-  if (o.T) {
-    launch_bpass_any<false, R>(bo, o.rank, o.n_leaves, T, bo.near_src, bst, Xr, nx, Y, bo.near_rows, bo.near_ncol,
-                               bo.near_dst, bo.near_op, bo.near_srcoff);
-    if (o.find_phi)
-      hipLaunchKernelGGL(bnear_finish_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream,
-                         X, Y, o.qk2, (long long)o.N * kLD);
+  if (o.T && bo.leaf_fused) {
+    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
+    leaf();
+  } else {
+    if (o.T) leaf();
+    for (int p : o.near) launch_bpass<R>(bo, p, Xr, nx, Y);
+    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
+    for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, Xr, nx, Y);
+    for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, Xr, nx, Y);
   }
-  for (int p : o.near) launch_bpass<R>(bo, p, Xr, nx, Y);
-  for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
-  for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, Xr, nx, Y);
-  for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, Xr, nx, Y);
   if (bo.qk2_cols)
     hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
                        bo.qk2_cols, (long long)o.N * kLD);

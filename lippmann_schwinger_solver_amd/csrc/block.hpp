@@ -5,8 +5,11 @@
 // row feeds all right-hand sides with coalesced loads. Every pass of the
 // single-vector plan (nca.cpp) becomes, per destination, a real GEMM
 //   [Yr; Yi] (2R x nb) = [[Ar, -Ai], [Ai, Ar]] (2R x 2C) [Xr; Xi] (2C x nb)
-// on FP64 MFMA tiles (v_mfma_f64_16x16x4f64); the near field is the same
-// GEMM with the 9 colleague tables side by side (36 x 324) for every leaf.
+// on MFMA tiles (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for the
+// FP32 operator), one wavefront per destination. On a uniform tree one
+// kernel per leaf computes the near field (the 9 colleague tables side by
+// side, rank x 9 rank, shared by all leaves), the leaf M2L and leaf L2L
+// products and the x term, and writes the leaf's output rows once.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
@@ -37,12 +40,11 @@ struct BlockOperator {
   DeviceOperator* op = nullptr;
   int nb = kBlockRhs;
   double* bstate = nullptr;   // (n_state + 1) rows x 2nb; row n_state stays zero
-  int* near_src = nullptr;    // n_leaves x 9 rank source rows (zero row where a colleague is absent)
-  int* near_rows = nullptr;   // per leaf: rank
-  int* near_ncol = nullptr;   // per leaf: 9 rank
-  int* near_dst = nullptr;    // per leaf: N + leaf rank
-  long long* near_op = nullptr;   // per leaf: 0 (all leaves share the tables)
-  long long* near_srcoff = nullptr;
+  // uniform tree: per leaf, its destination in the leaf M2L and leaf L2L
+  // passes (2 per leaf, -1 for none) when the leaf kernel folds them in
+  int* leaf_dest = nullptr;
+  bool leaf_fused = false;
+  int* near_src = nullptr;    // n_leaves x 9 rank near-field source rows (zero row for an absent colleague)
   // per right-hand-side contrast (N x 64, row-major [n][q]) for a volume-potential
   // operator: Y[:, q] = X[:, q] + qk2_cols[:, q] .* (V X)[:, q]
   double* qk2_cols = nullptr;
