@@ -617,7 +617,7 @@ def main():
                     help="timed steps (default: 2 batches of 64 angles for c3, 10 solves for c2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones (default 1; c2: 2)")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--restart", type=int, default=30, help="restart length of the batched solves")
+    ap.add_argument("--restart", type=int, default=24, help="restart length of the batched solves")
     ap.add_argument("--nets", default="1,2,4,8,4,1",
                     help="learned workload: dilation sets of the networks to train, ';'-separated")
     ap.add_argument("--n-eval", type=int, default=256, help="learned workload: held-out systems")

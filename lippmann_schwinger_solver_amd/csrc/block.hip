@@ -435,6 +435,7 @@ __global__ void bdiag_kernel(const XT* __restrict__ X, double* __restrict__ Y, c
 // ------------------------------------------------------- batched GMRES ----
 constexpr int kBRows = 256;   // rows per workgroup in the block dot
 constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
+constexpr int kRS = 16;       // first-stage segments of the row-block sums
 
 // partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
 // conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
@@ -483,13 +484,14 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
   }
 }
 
-// out[col * 64 + q] = sum over row blocks; one workgroup per column.
-__global__ __launch_bounds__(kT) void breduce_kernel(const double2* __restrict__ partial, int nrb, int ncx,
-                                                     double2* __restrict__ out) {
+// First stage of the row-block sums: workgroup (col, sg) adds the row blocks
+// rb = sg, sg + kRS, ... into part2[(sg * ncx + col) * 64 + q].
+__global__ __launch_bounds__(kT) void breduce1_kernel(const double2* __restrict__ partial, int nrb, int ncx,
+                                                      double2* __restrict__ part2) {
   __shared__ double2 red[kT];
-  const int col = blockIdx.x, t = threadIdx.x, q = t & 63, sg = t >> 6;
+  const int col = blockIdx.x, sg = blockIdx.y, t = threadIdx.x, q = t & 63, ln = t >> 6;
   double2 a = make_double2(0.0, 0.0);
-  for (int rb = sg; rb < nrb; rb += kT / 64) {
+  for (int rb = sg + kRS * ln; rb < nrb; rb += kRS * (kT / 64)) {
     const double2 v = partial[((long long)rb * ncx + col) * kNB + q];
     a.x += v.x;
     a.y += v.y;
@@ -502,8 +504,26 @@ __global__ __launch_bounds__(kT) void breduce_kernel(const double2* __restrict__
       s.x += red[k * kNB + t].x;
       s.y += red[k * kNB + t].y;
     }
-    out[col * kNB + t] = s;
+    part2[((long long)sg * ncx + col) * kNB + t] = s;
   }
+}
+
+// Second stage: out[col * 64 + q] = sum over the kRS segments (fixed order).
+__global__ void breduce2_kernel(const double2* __restrict__ part2, int ncx, double2* __restrict__ out) {
+  const int col = blockIdx.x, q = threadIdx.x;
+  double2 s = part2[(long long)col * kNB + q];
+  for (int sg = 1; sg < kRS; ++sg) {
+    const double2 v = part2[((long long)sg * ncx + col) * kNB + q];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  out[col * kNB + q] = s;
+}
+
+// out = sums over the nrb row blocks of partial (ncx columns); part2 holds kRS x ncx x 64
+void block_reduce(const double2* partial, int nrb, int ncx, double2* part2, double2* out, hipStream_t s) {
+  hipLaunchKernelGGL(breduce1_kernel, dim3(ncx, kRS), dim3(kT), 0, s, partial, nrb, ncx, part2);
+  hipLaunchKernelGGL(breduce2_kernel, dim3(ncx), dim3(kNB), 0, s, part2, ncx, out);
 }
 
 struct BGivens {
@@ -1003,7 +1023,7 @@ std::string BlockWork::reserve(int N_, int m_) {
   const long long vs = (long long)N_ * kLD;
   const int nrb = (N_ + kBRows - 1) / kBRows;
   HIP_OK(hipMalloc((void**)&w, sizeof(double) * size_t(vs)));
-  HIP_OK(hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * (m_ + 2) * kNB));
+  HIP_OK(hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb + kRS) * (m_ + 2) * kNB));
   HIP_OK(hipMalloc((void**)&h, sizeof(double2) * size_t(m_ + 2) * kNB));
   HIP_OK(hipMalloc((void**)&y, sizeof(double2) * size_t(m_ + 1) * kNB));
   HIP_OK(hipMalloc((void**)&H, sizeof(double2) * size_t(m_ + 1) * m_ * kNB));
@@ -1090,7 +1110,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     const int ncx = nc + (norm ? 1 : 0);
     hipLaunchKernelGGL(bdot_kernel<TV>, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
                        N, partial);
-    hipLaunchKernelGGL(breduce_kernel, dim3(ncx), dim3(kT), 0, s, partial, nrb, ncx, h);
+    block_reduce(partial, nrb, ncx, partial + size_t(nrb) * (m + 2) * kNB, h, s);
   };
   // ||b_q||
   dot(0, B, true);
@@ -1242,7 +1262,7 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
   int* mask = nullptr;
   bool ok = hipMalloc((void**)&R, sizeof(double) * size_t(vs)) == hipSuccess &&
             hipMalloc((void**)&D, sizeof(double) * size_t(vs)) == hipSuccess &&
-            hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb) * kNB) == hipSuccess &&
+            hipMalloc((void**)&partial, sizeof(double2) * size_t(nrb + kRS) * kNB) == hipSuccess &&
             hipMalloc((void**)&h, sizeof(double2) * kNB) == hipSuccess &&
             hipMalloc((void**)&mask, sizeof(int) * kNB) == hipSuccess;
   auto cleanup = [&]() {
@@ -1262,7 +1282,7 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
   auto norms = [&](const double* v, std::vector<double>& out) {
     hipLaunchKernelGGL(bdot_kernel<double>, dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N,
                        partial);
-    hipLaunchKernelGGL(breduce_kernel, dim3(1), dim3(kT), 0, s, partial, nrb, 1, h);
+    block_reduce(partial, nrb, 1, partial + size_t(nrb) * kNB, h, s);
     std::vector<double2> hh(kNB);
     hipMemcpyAsync(hh.data(), h, sizeof(double2) * kNB, hipMemcpyDeviceToHost, s);
     const bool good = hipStreamSynchronize(s) == hipSuccess;
