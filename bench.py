@@ -144,14 +144,15 @@ MFMA_F64_PEAK_TFLOPS = 78.6
 MFMA_F32_PEAK_TFLOPS = 157.3
 
 
-def batched_leg(S, info, args, st_plain, plain_sps, restart=50):
+def batched_leg(S, info, args, st_plain, plain_sps, restart=50, restart32=30):
     """64 plane waves at 2 pi j / 64 on the same geometry, solved together by
     batched GMRES (one FP64 MFMA block matvec per Arnoldi step); the right-hand
     sides are resident in HBM before timing; every system's FP64 true residual
-    must reach tol. `restart` is the batched restart length: the Gram-Schmidt
-    of 64 systems streams 64 basis vectors per column, so the batched solve
-    prefers shorter cycles than the single-vector one (restart sweeps of both
-    are recorded with the workload results under profiles/)."""
+    must reach tol. `restart` is the FP64 batched restart length, `restart32`
+    the FP32 inner solves' (one cycle per refinement): the Gram-Schmidt of 64
+    systems streams 64 basis vectors per column, so the FP32-inner batch
+    prefers shorter cycles (restart sweeps are recorded with the workload
+    results under profiles/)."""
     angles = 2 * np.pi * np.arange(64) / 64
     Bh = np.ascontiguousarray(S.rhs_angles(angles).T)      # 64 x N: column q at q*N
     Bd, Xd = DeviceBlock(Bh), DeviceBlock(np.zeros_like(Bh))
@@ -172,14 +173,14 @@ def batched_leg(S, info, args, st_plain, plain_sps, restart=50):
     X = Xd.download().T
     # the same batch with the FP32 block operator inside FP64 refinement
     Xd.zero()
-    S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=restart, max_restarts=1, tol=args.tol,
+    S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=restart32, max_restarts=1, tol=args.tol,
                   mode="fp32ir", inner_tol=1e-5, max_outer=40)   # warm-up
     log("batched FP32-IR: warm-up done")
     t32 = 0.0
     for _ in range(args.steps):
         Xd.zero()
         t0 = time.perf_counter()
-        _, st32, rel32 = S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=restart,
+        _, st32, rel32 = S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=restart32,
                                        max_restarts=1, tol=args.tol, mode="fp32ir", inner_tol=1e-5, max_outer=40)
         t32 += time.perf_counter() - t0
         log(f"batched FP32-IR batch: {st32}")
@@ -204,7 +205,8 @@ def batched_leg(S, info, args, st_plain, plain_sps, restart=50):
             "block_matvec_mfma": {"achieved_tflops": flops / (mv_block * 1e-3) / 1e12, "peak": MFMA_F64_PEAK_TFLOPS,
                                   "frac": flops / (mv_block * 1e-3) / 1e12 / MFMA_F64_PEAK_TFLOPS,
                                   "flops_per_matvec": flops},
-            "fp32_inner": {"solves_per_s": sps32, "speedup_vs_plain_fp64": sps32 / plain_sps,
+            "fp32_inner": {"solves_per_s": sps32, "speedup_vs_plain_fp64": sps32 / plain_sps, "restart_len": restart32,
+                           "speedup_vs_batched_fp64": sps32 / sps,
                            "block_matvec_fp32_ms": S.time_matvec_block(reps=5, fp32=True),
                            "ms_per_batch": 1e3 * t32 / args.steps, "inner_arnoldi_steps": st32["iterations"],
                            "refinements": st32["refinements"], "max_rel_residual": float(rel32.max()),
@@ -231,12 +233,14 @@ def pmc_probe(S):
 
 
 def pmc_probe_block(S):
-    """3 block matvecs (64 right-hand sides, FP64 MFMA) for counter runs, after
-    one warm-up; the algorithmic flops of one are printed for the ratio."""
+    """3 block matvecs (64 right-hand sides) on FP64 MFMA, then 3 on FP32 MFMA,
+    each after one warm-up, for counter runs; the algorithmic flops of one are
+    printed for the ratio."""
     ms = S.time_matvec_block(reps=3)
+    ms32 = S.time_matvec_block(reps=3, fp32=True)
     info = S.info()
     flops = 8.0 * 64 * (info["E_far"] + info["E_near"])
-    log(f"pmc block probe: block matvec {ms:.3f} ms, {flops:.4e} algorithmic flops each")
+    log(f"pmc block probe: block matvec {ms:.3f} ms FP64, {ms32:.3f} ms FP32, {flops:.4e} algorithmic flops each")
 
 
 def setup_record(wall_s, info, inp):

@@ -269,14 +269,25 @@ int lsg_gmres_noalloc(lsg_ctx* ctx, const void* b_dev, void* x_dev, int restart_
 int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* opts,
                     double* rel_residuals, lsg_stats* stats);
 
-/* Autotuner (no reference counterpart): for every pass, time the one-wavefront
- * and one-workgroup-per-destination kernels, and for the near field of a
+/* Autotuner (no reference counterpart): for every pass, time the one-wavefront,
+ * one-workgroup-per-destination and one-lane-per-row kernels, the first upward
+ * passes fused per subtree against their separate launches, and for the near field of a
  * uniform tree the FP64 MFMA kernel and the FMA kernel with 1 or 2 leaves per
  * thread, on this operator with HIP events (best of `reps`); keep the
  * fastest. n_changed (may be NULL) receives the number of changed choices;
  * lsg_tuning() returns one line per decision. */
 int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed);
 const char* lsg_tuning(const lsg_ctx* ctx);
+
+/* Pass kernel choice without timing (no reference counterpart): every pass of
+ * the single-vector operator runs the given kernel, 0 one workgroup per
+ * destination, 1 one wavefront per destination, 2 one lane per output row
+ * (passes whose rows exceed 256 keep the workgroup kernel); 3, 4 and 5 are
+ * the same with the leaf level's upward pass and the next two, and the last
+ * two small downward passes, each group fused into one launch of one
+ * workgroup per group of subtrees (where the operator has such groups). For
+ * tests and measurements of each kernel; lsg_autotune replaces the choice. */
+int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel);
 
 /* on = 1: capture the FP64 matvec (both streams) into a hipGraph, replayed by
  * every later FP64 matvec of the context (one launch plus a pointer update);

@@ -109,6 +109,7 @@ SYMBOLS = [
     ("lsg_set_contrast", C.c_int, [_P, _D, C.c_int]),
     ("lsg_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("lsg_tuning", C.c_char_p, [_P]),
+    ("lsg_set_pass_kernel", C.c_int, [_P, C.c_int]),
     ("lsg_use_graph", C.c_int, [_P, C.c_int]),
     ("lsg_gmres_ragged", C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(lsg_solve_opts),
                                    C.POINTER(lsg_stats)]),
@@ -647,6 +648,15 @@ class DFMM:
         n = C.c_int()
         self._check(self._lib.lsg_autotune(self._ctx, reps, C.byref(n)))
         return n.value, self._lib.lsg_tuning(self._ctx).decode()
+
+    def set_pass_kernel(self, kernel, fused=False):
+        """Run every pass with one kernel: "block" (one workgroup per
+        destination), "wave" (one wavefront per destination) or "lane" (one
+        lane per output row); with fused=True the first three upward passes,
+        and the last two small downward passes, each run as one launch of one
+        workgroup per group of subtrees. autotune() replaces the choice."""
+        k = {"block": 0, "wave": 1, "lane": 2}[kernel] + (3 if fused else 0)
+        self._check(self._lib.lsg_set_pass_kernel(self._ctx, k))
 
     def use_graph(self, on=True):
         """Replay the FP64 matvec from a captured hipGraph (on) or launch kernel by kernel (off)."""

@@ -343,8 +343,9 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* _
 //   (applied to the table rows as they are loaded);
 // then the leaf's destinations in the fused leaf passes (leaf_dest[2 leaf + k],
 // -1 for none; rank rows at N + leaf rank) accumulated on top, and the rows
-// stored once: Y = (find_phi ? X : 0) + qk2 .* (T x) + leaf passes.
-template <class R, class XT, int MT>
+// stored once: Y = (find_phi ? X : 0) + qk2 .* (T x) + leaf passes (in FP64,
+// or rounded to FP32 for the FP32 inner solves' Krylov product).
+template <class R, class XT, class YT, int MT>
 __global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
                                                    const int* __restrict__ near_src, int rank,
                                                    const typename Mfma<R>::c2* __restrict__ ops,
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* _
                                                    const long long* __restrict__ src_off,
                                                    const int* __restrict__ leaf_dest, const R* __restrict__ X,
                                                    const R* __restrict__ bst, int nx, const XT* __restrict__ Xin,
-                                                   const double* __restrict__ qk2, int find_phi, double* __restrict__ Y,
+                                                   const double* __restrict__ qk2, int find_phi, YT* __restrict__ Y,
                                                    int zrow) {
   typedef typename Mfma<R>::acc Acc;
   const int leaf = xcd_order(blockIdx.x, gridDim.x);
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* _
       const size_t o = ((size_t)leaf * rank + r) * kLD + a * kNB + q;
       double val = double(acc[mt][v]);
       if (find_phi) val += double(Xin[o]);
-      Y[o] = val;
+      Y[o] = YT(val);
     }
   }
 }
@@ -439,9 +440,9 @@ constexpr int kRS = 16;       // first-stage segments of the row-block sums
 
 // partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
 // conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
-template <class TV>
+template <class TV, class TW>
 __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long long vs, int nc, int ncx,
-                                                  const double* __restrict__ w, int N, double2* __restrict__ partial) {
+                                                  const TW* __restrict__ w, int N, double2* __restrict__ partial) {
   __shared__ double2 red[kT / 64][kBCols][kNB];
   const int t = threadIdx.x, q = t & 63, rl = t >> 6;
   const int g0 = blockIdx.y * kBCols;
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
   for (int r = rl; r < kBRows; r += kT / 64) {
     const long long n = n0 + r;
     if (n >= N) break;
-    const double wr = w[n * kLD + q], wi = w[n * kLD + kNB + q];
+    const double wr = double(w[n * kLD + q]), wi = double(w[n * kLD + kNB + q]);
 #pragma unroll
     for (int u = 0; u < kBCols; ++u) {
       const int col = g0 + u;
@@ -630,9 +631,9 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
 // (q, row group) owns kUpdRows consecutive rows of system q, so each
 // coefficient h[i, q] is loaded once per kUpdRows rows.
 constexpr int kUpdRows = 8;
-template <class TV>
+template <class TV, class TW>
 __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, long long vs, int nc,
-                                                     const double2* __restrict__ h, const double* __restrict__ w,
+                                                     const double2* __restrict__ h, const TW* __restrict__ w,
                                                      const double* __restrict__ hn, TV* __restrict__ vout,
                                                      long long n_rows) {
   const int q = threadIdx.x & 63;
@@ -662,8 +663,8 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, l
   for (int r = 0; r < kUpdRows; ++r) {
     if (r < nr) {
       const long long o = base + r * kLD;
-      vout[o] = TV((w[o] - ar[r]) * inv);
-      vout[o + kNB] = TV((w[o + kNB] - ai[r]) * inv);
+      vout[o] = TV((double(w[o]) - ar[r]) * inv);
+      vout[o + kNB] = TV((double(w[o + kNB]) - ai[r]) * inv);
     }
   }
 }
@@ -946,8 +947,12 @@ __global__ void bround_kernel(const double* __restrict__ X, float* __restrict__ 
 // R: pass precision; XT: type of the input block. An FP64 input to the FP32
 // passes is rounded into the pass state's x rows first; an input already in
 // the pass precision is read in place.
-template <class R, class XT>
-static void block_matvec_impl(BlockOperator& bo, const XT* X, double* Y) {
+template <class R, class XT, class YT = double>
+static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
+  // an FP32 product is stored by the leaf kernel alone (can_store_f32): the
+  // other passes then write only the pass state, and get no output pointer
+  constexpr bool y64 = std::is_same<YT, double>::value;
+  double* Yd = y64 ? reinterpret_cast<double*>(Y) : nullptr;
   DeviceOperator& o = *bo.op;
   typedef typename Mfma<R>::c2 C2;
   const C2* T = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
@@ -971,7 +976,7 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, double* Y) {
     const dim3 grid(o.n_leaves);
 #define LSG_LEAF_CASE(M)                                                                                        \
   case M:                                                                                                       \
-    hipLaunchKernelGGL((bleaf_kernel<R, XT, M>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,     \
+    hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, M>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src, \
                        o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y, \
                        int(o.n_state));                                                                         \
     break;
@@ -984,31 +989,35 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, double* Y) {
       LSG_LEAF_CASE(6)
       LSG_LEAF_CASE(7)
       default:
-        hipLaunchKernelGGL((bleaf_kernel<R, XT, 8>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,
+        hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, 8>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,
                            o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y,
                            int(o.n_state));
     }
 #undef LSG_LEAF_CASE
   };
-  for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Y);
+  if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
+  for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Yd);
   if (o.T && bo.leaf_fused) {
-    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
+    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Yd);
     leaf();
-  } else {
+  } else if constexpr (y64) {
     if (o.T) leaf();
-    for (int p : o.near) launch_bpass<R>(bo, p, Xr, nx, Y);
-    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Y);
-    for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, Xr, nx, Y);
-    for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, Xr, nx, Y);
+    for (int p : o.near) launch_bpass<R>(bo, p, Xr, nx, Yd);
+    for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Yd);
+    for (int p : o.leaf_m2l) launch_bpass<R>(bo, p, Xr, nx, Yd);
+    for (int p : o.leaf_l2l) launch_bpass<R>(bo, p, Xr, nx, Yd);
   }
-  if (bo.qk2_cols)
-    hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Y,
-                       bo.qk2_cols, (long long)o.N * kLD);
+  if constexpr (y64) {
+    if (bo.qk2_cols)
+      hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Yd,
+                         bo.qk2_cols, (long long)o.N * kLD);
+  }
 }
 
 void BlockOperator::matvec(const double* X, double* Y) { block_matvec_impl<double, double>(*this, X, Y); }
 void BlockOperator::matvec32(const double* X, double* Y) { block_matvec_impl<float, double>(*this, X, Y); }
 void BlockOperator::matvec32(const float* X, double* Y) { block_matvec_impl<float, float>(*this, X, Y); }
+void BlockOperator::matvec32(const float* X, float* Y) { block_matvec_impl<float, float, float>(*this, X, Y); }
 
 std::string BlockOperator::enable_fp32() {
   std::string e = op->enable_fp32();
@@ -1046,12 +1055,13 @@ std::string BlockWork::reserve(int N_, int m_) {
 std::string BlockWork::basis(bool f32) {
   const size_t n = size_t(N) * kLD * size_t(m + 1);
   if (f32 && !V32) HIP_OK(hipMalloc((void**)&V32, sizeof(float) * n));
+  if (f32 && !w32) HIP_OK(hipMalloc((void**)&w32, sizeof(float) * size_t(N) * kLD));
   if (!f32 && !V) HIP_OK(hipMalloc((void**)&V, sizeof(double) * n));
   return "";
 }
 
 void BlockWork::release() {
-  void* ptrs[] = {V, V32, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  void* ptrs[] = {V, V32, w32, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (host_rel) (void)hipHostFree(host_rel);
@@ -1101,15 +1111,19 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   }
   TV* V = reinterpret_cast<TV*>(std::is_same<TV, float>::value ? (void*)W.V32 : (void*)W.V);
   double* w = W.w;
+  // the FP32 inner solves keep the Arnoldi product A v_j in FP32 too (the
+  // FP32 operator computed it in FP32) where the leaf kernel stores it
+  float* w32 = std::is_same<TV, float>::value && fp32 && bo.can_store_f32() ? W.w32 : nullptr;
   double2 *partial = W.partial, *h = W.h, *y = W.y;
   BGivens st{W.H, W.cs, W.sn, W.g, W.hn, W.bnorm, W.active, W.kq, W.tolq};
   double *host_rel = W.host_rel, *host_est = W.host_est;
   int* host_active = W.host_active;
   auto cleanup = [&]() { hipStreamSynchronize(s); };
-  auto dot = [&](int nc, const double* vec, bool norm) {
+  auto dot = [&](int nc, const auto* vec, bool norm) {
+    typedef std::remove_cv_t<std::remove_pointer_t<decltype(vec)>> TW;
     const int ncx = nc + (norm ? 1 : 0);
-    hipLaunchKernelGGL(bdot_kernel<TV>, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
-                       N, partial);
+    hipLaunchKernelGGL((bdot_kernel<TV, TW>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx,
+                       vec, N, partial);
     block_reduce(partial, nrb, ncx, partial + size_t(nrb) * (m + 2) * kNB, h, s);
   };
   // ||b_q||
@@ -1195,11 +1209,21 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
       for (int jj = j; jj < j + batch; ++jj) {
+        if constexpr (std::is_same<TV, float>::value) {
+          if (w32) {
+            bo.matvec32((const float*)(V + size_t(jj) * vs), w32);
+            dot(jj + 1, (const float*)w32, true);
+            hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
+            hipLaunchKernelGGL((bupdate_kernel<TV, float>), dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h,
+                               (const float*)w32, st.hn, V + size_t(jj + 1) * vs, (long long)N);
+            continue;
+          }
+        }
         amv((const TV*)(V + size_t(jj) * vs), w);
-        dot(jj + 1, w, true);
+        dot(jj + 1, (const double*)w, true);
         hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
-        hipLaunchKernelGGL(bupdate_kernel<TV>, dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h, w, st.hn,
-                           V + size_t(jj + 1) * vs, (long long)N);
+        hipLaunchKernelGGL((bupdate_kernel<TV, double>), dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h,
+                           (const double*)w, st.hn, V + size_t(jj + 1) * vs, (long long)N);
       }
       if (hipStreamSynchronize(s) != hipSuccess) {
         res.status = LSG_ERR_DEVICE;
@@ -1280,7 +1304,7 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
   const unsigned gv = blocks_for(vs, kT);
   // per-system 2-norms of a block
   auto norms = [&](const double* v, std::vector<double>& out) {
-    hipLaunchKernelGGL(bdot_kernel<double>, dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N,
+    hipLaunchKernelGGL((bdot_kernel<double, double>), dim3(nrb, 1), dim3(kT), 0, s, (const double*)nullptr, vs, 0, 1, v, N,
                        partial);
     block_reduce(partial, nrb, 1, partial + size_t(nrb) * kNB, h, s);
     std::vector<double2> hh(kNB);

@@ -26,6 +26,7 @@ struct BlockWork {
   int N = 0, m = -1;
   double *V = nullptr, *w = nullptr;
   float* V32 = nullptr;   // FP32 basis of the FP32 inner solves
+  float* w32 = nullptr;   // FP32 product A v_j of the FP32 inner solves (when the leaf kernel writes every output row)
   double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *bnorm = nullptr, *tolq = nullptr;
   int *active = nullptr, *kq = nullptr;
@@ -62,6 +63,10 @@ struct BlockOperator {
   // rounded into the pass state, or FP32 and read in place)
   void matvec32(const double* X, double* Y);
   void matvec32(const float* X, double* Y);
+  // same with the product stored in FP32; only where the leaf kernel writes
+  // every output row (can_store_f32)
+  void matvec32(const float* X, float* Y);
+  bool can_store_f32() const { return op && leaf_fused && !qk2_cols; }
 };
 
 // Batched restarted GMRES: nb independent systems A x_q = b_q advanced in

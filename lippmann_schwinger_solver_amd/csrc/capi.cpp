@@ -885,6 +885,21 @@ int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed) {
 
 const char* lsg_tuning(const lsg_ctx* ctx) { return ctx ? ctx->tuning.c_str() : ""; }
 
+int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel) {
+  if (!ctx || kernel < 0 || kernel > 5) return LSG_ERR_ARG;
+  const bool had_graph = ctx->dev.graph_exec != nullptr;
+  ctx->dev.drop_graph();
+  const int k = kernel % 3;
+  for (DevPass& p : ctx->dev.passes) {
+    p.lane = k == 2 && p.lane_ok;
+    p.wave = k == 1;
+  }
+  ctx->dev.fup.on = kernel >= 3 && ctx->dev.fup.tasks > 0;
+  ctx->dev.fdown.on = kernel >= 3 && ctx->dev.fdown.tasks > 0;
+  std::string e = had_graph ? ctx->dev.capture_graph() : std::string();
+  return e.empty() ? LSG_OK : ctx->set(LSG_ERR_DEVICE, e);
+}
+
 int lsg_use_graph(lsg_ctx* ctx, int on) {
   if (!ctx) return LSG_ERR_ARG;
   if (!on) {

@@ -7,6 +7,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <mutex>
 #include <cmath>
@@ -213,30 +214,24 @@ __global__ __launch_bounds__(kThreads) void pass_block_kernel(
   }
 }
 
-// One wavefront per destination (four per workgroup) for passes of small
-// operators: lanes split into (row, phase) as above and gather their source
-// values straight from memory; the phases are summed through the wave's own
-// LDS slice, so no workgroup barrier is needed and waves retire independently.
-template <bool ACC, class R>
-__global__ __launch_bounds__(kThreads) void pass_wave_kernel(
-    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
-    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
-    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
-    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp, DestMask mk) {
-  // pointers from device memory when the matvec runs as a captured graph
-  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
-  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
+// Destination d of a pass computed by one wavefront: lanes split into (row,
+// phase), phase p walks columns p, p + nph, ... WC at a time (indices, then
+// operator entries, then sources in flight together); the phases are summed
+// through the wave's own LDS slice rw (64 entries), so no workgroup barrier
+// is needed.
+template <bool ACC, class R, int WC>
+__device__ __forceinline__ void wave_dest(const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src,
+                                          typename cx<R>::t* state, const double2* __restrict__ x, double2* y, int N,
+                                          const int* __restrict__ rows, const int* __restrict__ ncol,
+                                          const int* __restrict__ dst, const long long* __restrict__ op_off,
+                                          const long long* __restrict__ src_off, int d, typename cx<R>::t* rw,
+                                          int lane) {
   typedef typename cx<R>::t C;
   typedef typename cx<R>::v V;
-  __shared__ C red[kThreads];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int d = blockIdx.x * (kThreads / 64) + wv;
-  if (d >= ndest || mk.skip(d)) return;
   const int Rw = rows[d], Cn = ncol[d];
   const V* A = ops + op_off[d];
   const int* S = src + src_off[d];
   const int dd = dst[d];
-  C* rw = red + wv * 64;
   for (int r0 = 0; r0 < Rw; r0 += 64) {
     const int Rt = min(64, Rw - r0);
     const int nph = 64 / Rt;
@@ -246,24 +241,22 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
     acc.y = 0;
     if (ph < nph) {
       const V* Ap = A + r0 + row;
-      // up to 16 columns per lane per step: all index loads, then all operand
-      // and source loads, then the multiply-adds (two memory round trips)
-      for (int c = ph; c < Cn; c += kWaveCols * nph) {
-        int si[kWaveCols];
-        V a[kWaveCols];
+      for (int c = ph; c < Cn; c += WC * nph) {
+        int si[WC];
+        V a[WC];
 #pragma unroll
-        for (int u = 0; u < kWaveCols; ++u) {
+        for (int u = 0; u < WC; ++u) {
           const int cc = c + u * nph;
           si[u] = cc < Cn ? S[cc] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < kWaveCols; ++u) {
+        for (int u = 0; u < WC; ++u) {
           const int cc = c + u * nph;
           if (cc < Cn) a[u] = __builtin_nontemporal_load(Ap + (size_t)cc * Rw);
           else a[u] = V{0, 0};
         }
 #pragma unroll
-        for (int u = 0; u < kWaveCols; ++u) {
+        for (int u = 0; u < WC; ++u) {
           C xv;
           if (si[u] >= 0) xv = load_src(x, state, si[u], N);
           else xv.x = xv.y = 0;
@@ -283,6 +276,102 @@ __global__ __launch_bounds__(kThreads) void pass_wave_kernel(
     }
     wave_sync();
   }
+}
+
+// One wavefront per destination (four per workgroup) for passes of small
+// operators (wave_dest); waves retire independently.
+template <bool ACC, class R>
+__global__ __launch_bounds__(kThreads) void pass_wave_kernel(
+    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
+    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
+    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
+    const long long* __restrict__ src_off, int ndest, const double2* const* __restrict__ xyp, DestMask mk) {
+  // pointers from device memory when the matvec runs as a captured graph
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
+  typedef typename cx<R>::t C;
+  __shared__ C red[kThreads];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int d = blockIdx.x * (kThreads / 64) + wv;
+  if (d >= ndest || mk.skip(d)) return;
+  wave_dest<ACC, R, kWaveCols>(ops, src, state, x, y, N, rows, ncol, dst, op_off, src_off, d, red + wv * 64, lane);
+}
+
+// Upward passes fused per subtree. Workgroup t runs the destinations of task
+// t stage by stage (stage s: its destinations of the s-th fused upward pass,
+// fdest[task_stage[t (nstage + 1) + s] ...]), one wavefront per destination
+// round robin over the 16 wavefronts, with a workgroup barrier between
+// stages. Every source a destination reads is x or a charge written by an
+// earlier stage of the same task (checked when the tasks are built), so no
+// other workgroup is waited for; the barrier makes the charges written by one
+// wavefront visible to the others.
+constexpr int kFusedThreads = 1024;
+template <class R>
+__global__ __launch_bounds__(kFusedThreads) void pass_fused_kernel(
+    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
+    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
+    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
+    const long long* __restrict__ src_off, const int* __restrict__ task_stage, const int* __restrict__ fdest,
+    int nstage, const double2* const* __restrict__ xyp, DestMask mk) {
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
+  typedef typename cx<R>::t C;
+  __shared__ C red[kFusedThreads];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int* ts = task_stage + (size_t)blockIdx.x * (nstage + 1);
+  for (int s = 0; s < nstage; ++s) {
+    const int e = ts[s + 1];
+    for (int i = ts[s] + wv; i < e; i += kFusedThreads / 64) {
+      const int d = fdest[i];
+      if (!mk.skip(d))
+        wave_dest<false, R, 8>(ops, src, state, x, y, N, rows, ncol, dst, op_off, src_off, d, red + wv * 64, lane);
+    }
+    __syncthreads();
+  }
+}
+
+// One lane per output row, for passes of short rows: lane k takes row
+// rowmap[k] & 255 of destination rowmap[k] >> 8 and walks all its columns
+// alone (8 columns in flight: indices, then operator entries, then sources),
+// so there is no phase reduction and no barrier; lanes of one destination
+// read each operator column as one contiguous run and share its source loads.
+// The row map orders destinations by decreasing column count, so the lanes of
+// a wavefront run loops of similar length.
+template <bool ACC, class R>
+__global__ __launch_bounds__(kThreads) void pass_lane_kernel(
+    const typename cx<R>::v* __restrict__ ops, const int* __restrict__ src, typename cx<R>::t* state,
+    const double2* __restrict__ x_arg, double2* y_arg, int N, const int* __restrict__ rows,
+    const int* __restrict__ ncol, const int* __restrict__ dst, const long long* __restrict__ op_off,
+    const long long* __restrict__ src_off, const unsigned* __restrict__ rowmap, int nrows,
+    const double2* const* __restrict__ xyp, DestMask mk) {
+  const double2* __restrict__ x = xyp ? xyp[0] : x_arg;
+  double2* y = xyp ? const_cast<double2*>(xyp[1]) : y_arg;
+  typedef typename cx<R>::t C;
+  typedef typename cx<R>::v V;
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= nrows) return;
+  const unsigned pm = rowmap[k];
+  const int d = int(pm >> 8), r = int(pm & 255u);
+  if (mk.skip(d)) return;
+  const int Rw = rows[d], Cn = ncol[d];
+  const V* A = ops + op_off[d] + r;
+  const int* S = src + src_off[d];
+  C acc;
+  acc.x = 0;
+  acc.y = 0;
+  int c = 0;
+  for (; c + 8 <= Cn; c += 8) {
+    int si[8];
+    V a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) si[u] = S[c + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = __builtin_nontemporal_load(A + (size_t)(c + u) * Rw);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cmac(acc, a[u], load_src(x, state, si[u], N));
+  }
+  for (; c < Cn; ++c) cmac(acc, __builtin_nontemporal_load(A + (size_t)c * Rw), load_src(x, state, S[c], N));
+  put_dst<ACC>(state, y, dst[d], N, r, acc);
 }
 
 // Near field of a uniform tree as one GEMM over the 9 colleague slots:
@@ -779,6 +868,105 @@ std::string upload_vec(const std::vector<T>& h, T** d) {
 
 static std::vector<double> near_fragments(const std::vector<cplx>& T, int rk);
 
+// Tasks of a fused group: the destinations of the nf passes plist[0..nf)
+// (stage s = pass plist[s]) grouped into the connected components of "reads a
+// slot written by", i.e. one box's subtree over nf levels, HFR cones of one
+// box included (upward: children's charges; downward: the parent's incoming
+// potential). Each component is one task; its stage-s destinations are listed
+// in fdest from task_stage[t (nf + 1) + s]. Returns an empty string and no
+// tasks when a destination reads a slot of its own or a later stage, a pass
+// accumulates, or a task grows beyond a few subtrees' worth of destinations
+// (one workgroup would serialise them).
+static std::string fused_tasks(int nf, const int* plist, const std::vector<DevPass>& passes,
+                                  const std::vector<int>& rows, const std::vector<int>& ncol,
+                                  const std::vector<int>& dst, const std::vector<long long>& srco,
+                                  const std::vector<int>& src, int64_t n_state, std::vector<int>& task_stage,
+                                  std::vector<int>& fdest) {
+  task_stage.clear();
+  fdest.clear();
+  std::vector<int> gl, stage;
+  for (int s = 0; s < nf; ++s) {
+    const DevPass& P = passes[size_t(plist[s])];
+    if (P.kind != 0) return "";
+    for (int q = 0; q < P.ndest; ++q) {
+      gl.push_back(int(P.dest_base) + q);
+      stage.push_back(s);
+    }
+  }
+  const int n = int(gl.size());
+  if (n == 0) return "";
+  std::vector<int> wr(static_cast<size_t>(n_state), -1), parent(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    parent[size_t(i)] = i;
+    const int g = gl[size_t(i)];
+    for (int r = 0; r < rows[size_t(g)]; ++r) wr[size_t(dst[size_t(g)]) + r] = i;
+  }
+  std::function<int(int)> find = [&](int a) {
+    while (parent[size_t(a)] != a) a = parent[size_t(a)] = parent[size_t(parent[size_t(a)])];
+    return a;
+  };
+  for (int i = 0; i < n; ++i) {
+    const int g = gl[size_t(i)];
+    for (int c = 0; c < ncol[size_t(g)]; ++c) {
+      const int si = src[size_t(srco[size_t(g)]) + c];
+      if (si < 0 || si >= n_state) return "fused upward passes: source index out of range";
+      const int j = wr[size_t(si)];
+      if (j < 0) continue;                             // x, or a slot written before the group
+      if (stage[size_t(j)] >= stage[size_t(i)]) return "";
+      const int a = find(i), b = find(j);
+      if (a != b) parent[size_t(std::max(a, b))] = std::min(a, b);
+    }
+  }
+  // components in order of their first destination; destinations by stage
+  std::vector<int> task_of(size_t(n), -1), root_task(size_t(n), -1);
+  int ntask = 0;
+  for (int i = 0; i < n; ++i) {
+    const int r = find(i);
+    if (root_task[size_t(r)] < 0) root_task[size_t(r)] = ntask++;
+    task_of[size_t(i)] = root_task[size_t(r)];
+  }
+  {
+    // consecutive components share a task while every stage keeps at most
+    // one destination per wavefront of the workgroup
+    constexpr int kPack = kFusedThreads / 64;
+    std::vector<int> ccnt(size_t(ntask) * nf, 0), packed(static_cast<size_t>(ntask));
+    for (int i = 0; i < n; ++i) ++ccnt[size_t(task_of[size_t(i)]) * nf + stage[size_t(i)]];
+    std::vector<int> cur(size_t(nf), 0);
+    int np = 0;
+    for (int c = 0; c < ntask; ++c) {
+      bool fits = c > 0;
+      for (int s = 0; s < nf && fits; ++s) fits = cur[size_t(s)] + ccnt[size_t(c) * nf + s] <= kPack;
+      if (!fits) {
+        std::fill(cur.begin(), cur.end(), 0);
+        ++np;
+      }
+      for (int s = 0; s < nf; ++s) cur[size_t(s)] += ccnt[size_t(c) * nf + s];
+      packed[size_t(c)] = np - 1;
+    }
+    for (int i = 0; i < n; ++i) task_of[size_t(i)] = packed[size_t(task_of[size_t(i)])];
+    ntask = np;
+  }
+  std::vector<int> cnt(size_t(ntask) * nf, 0);
+  for (int i = 0; i < n; ++i) ++cnt[size_t(task_of[size_t(i)]) * nf + stage[size_t(i)]];
+  constexpr int kMaxStage = 256;
+  for (int c : cnt)
+    if (c > kMaxStage) return "";
+  task_stage.assign(size_t(ntask) * (nf + 1), 0);
+  std::vector<int> pos(size_t(ntask) * nf);
+  int off = 0;
+  for (int t = 0; t < ntask; ++t) {
+    for (int s = 0; s < nf; ++s) {
+      task_stage[size_t(t) * (nf + 1) + s] = off;
+      pos[size_t(t) * nf + s] = off;
+      off += cnt[size_t(t) * nf + s];
+    }
+    task_stage[size_t(t) * (nf + 1) + nf] = off;
+  }
+  fdest.assign(size_t(n), -1);
+  for (int i = 0; i < n; ++i) fdest[size_t(pos[size_t(task_of[size_t(i)]) * nf + stage[size_t(i)]]++)] = gl[size_t(i)];
+  return "";
+}
+
 std::string DeviceOperator::upload(const Plan& plan) {
   N = plan.N;
   rank = plan.rank;
@@ -809,6 +997,7 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (!(e = upload_vec(plan.src, &src)).empty()) return e;
   std::vector<int> rows, ncol, dst;
   std::vector<long long> opo, srco;
+  std::vector<unsigned> rowmap;
   passes.clear();
   up.clear(); down.clear(); leaf_m2l.clear(); leaf_l2l.clear(); near.clear();
   for (const PassHost& ph : plan.passes) {
@@ -832,6 +1021,25 @@ std::string DeviceOperator::upload(const Plan& plan) {
     else if (ph.name == "near") near.push_back(idx);
     else down.push_back(idx);
     dp.wave = dp.ndest > 0 && ph.entries <= int64_t(kWaveEntries) * dp.ndest;
+    {
+      // row map of the one-lane-per-row variant (pass_lane_kernel)
+      bool ok = dp.ndest > 0 && dp.ndest < (1 << 24) && ph.n_dst < (int64_t(1) << 31);
+      for (int r : ph.rows) ok = ok && r >= 0 && r <= 256;
+      dp.lane_ok = ok;
+      dp.row_base = int64_t(rowmap.size());
+      if (ok) {
+        std::vector<int> order(size_t(dp.ndest));
+        for (int q = 0; q < dp.ndest; ++q) order[size_t(q)] = q;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ph.ncol[size_t(a)] > ph.ncol[size_t(b)]; });
+        for (int q : order)
+          for (int r = 0; r < ph.rows[size_t(q)]; ++r) rowmap.push_back((unsigned(q) << 8) | unsigned(r));
+        // the kernel runs one lane per entry: n_dst of them
+        if (int64_t(rowmap.size()) - dp.row_base != dp.n_dst) {
+          rowmap.resize(size_t(dp.row_base));
+          dp.lane_ok = false;
+        }
+      }
+    }
     for (size_t q = 0; q < ph.rows.size(); ++q) {
       // a destination lies wholly in the output vector or wholly in the charge/potential slots
       const int64_t d0 = ph.dst[q], d1 = d0 + ph.rows[q];
@@ -872,6 +1080,34 @@ std::string DeviceOperator::upload(const Plan& plan) {
   if (!(e = upload_vec(dst, &d_dst)).empty()) return e;
   if (!(e = upload_vec(opo, &d_op)).empty()) return e;
   if (!(e = upload_vec(srco, &d_srcoff)).empty()) return e;
+  if (!(e = upload_vec(rowmap, &d_rowmap)).empty()) return e;
+  {
+    // fused groups (each kept or not by the autotuner): the leaf level's
+    // upward pass and the two above it; the last two consecutive downward
+    // passes of wavefront size (at most kWaveEntries entries per destination
+    // on average) before the first larger one
+    auto make_group = [&](const std::vector<int>& list, int first, int nf, FusedPasses& g) -> std::string {
+      g = FusedPasses();
+      if (nf < 2) return "";
+      std::vector<int> ts, fd;
+      std::string e2 = fused_tasks(nf, list.data() + first, passes, rows, ncol, dst, srco, plan.src, n_state, ts, fd);
+      if (!e2.empty() || ts.empty()) return e2;
+      if (!(e2 = upload_vec(ts, &g.d_task_stage)).empty()) return e2;
+      if (!(e2 = upload_vec(fd, &g.d_fdest)).empty()) return e2;
+      g.first = first;
+      g.n = nf;
+      g.tasks = int(ts.size()) / (nf + 1);
+      return "";
+    };
+    if (!(e = make_group(up, 0, int(std::min<size_t>(3, up.size())), fup)).empty()) return e;
+    int last = -1;
+    for (size_t i = 0; i < down.size(); ++i) {
+      const DevPass& P = passes[size_t(down[i])];
+      if (P.ndest > 0 && P.entries > int64_t(kWaveEntries) * P.ndest) break;
+      last = int(i);
+    }
+    if (last >= 1 && !(e = make_group(down, last - 1, 2, fdown)).empty()) return e;
+  }
   // near-field tables transposed to column-major per slot (uniform tree only:
   // an adaptive tree's near field is a pass)
   const size_t set_sz = size_t(9) * rank * rank;
@@ -926,12 +1162,16 @@ void DeviceOperator::release() {
   if (xy_dev) hipFree(xy_dev);
   xy_dev = nullptr;
   void* ptrs[] = {state, ops, src, d_rows, d_ncol, d_dst, d_op, d_srcoff, T, nbr, qk2, ops32, T32, state32, qk2_user,
-                  Tm, near_set, d_dest_sys};
+                  Tm, near_set, d_dest_sys, d_rowmap, fup.d_task_stage, fup.d_fdest, fdown.d_task_stage,
+                  fdown.d_fdest};
   for (void* p : ptrs)
     if (p) hipFree(p);
   state = ops = nullptr;
   src = d_rows = d_ncol = d_dst = nbr = nullptr;
   d_op = d_srcoff = nullptr;
+  d_rowmap = nullptr;
+  fup = FusedPasses();
+  fdown = FusedPasses();
   T = nullptr;
   qk2 = nullptr;
   ops32 = T32 = state32 = nullptr;
@@ -955,7 +1195,7 @@ static void launch_block(const DeviceOperator& o, const DevPass& p, const typena
                          typename cx<R>::t* st, const double2* x, double2* y, hipStream_t s,
                          const double2* const* xyp) {
   const int64_t b = p.dest_base;
-  const dim3 grid(p.ndest);
+  const dim3 grid(p.grid_cap > 0 ? std::min(p.ndest, p.grid_cap) : p.ndest);
   const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys + b : nullptr, o.sys_active};
   if (p.kind == 0)
     hipLaunchKernelGGL((pass_block_kernel<false, R, 0>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
@@ -976,6 +1216,19 @@ static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2
   typedef typename cx<R>::t C;
   const V* ops = reinterpret_cast<const V*>(std::is_same<R, double>::value ? (const void*)o.ops : (const void*)o.ops32);
   C* st = reinterpret_cast<C*>(std::is_same<R, double>::value ? (void*)o.state : (void*)o.state32);
+  if (p.lane && p.lane_ok) {
+    const int nrows = int(p.n_dst);
+    const dim3 grid((nrows + kThreads - 1) / kThreads);
+    const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys + b : nullptr, o.sys_active};
+    const unsigned* rm = o.d_rowmap + p.row_base;
+    if (p.kind == 0)
+      hipLaunchKernelGGL((pass_lane_kernel<false, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, rm, nrows, xyp, mk);
+    else
+      hipLaunchKernelGGL((pass_lane_kernel<true, R>), grid, dim3(kThreads), 0, s, ops, o.src, st, x, y, o.N,
+                         o.d_rows + b, o.d_ncol + b, o.d_dst + b, o.d_op + b, o.d_srcoff + b, rm, nrows, xyp, mk);
+    return;
+  }
   if (p.wave) {
     const dim3 grid((p.ndest + 3) / 4);
     const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys + b : nullptr, o.sys_active};
@@ -988,6 +1241,19 @@ static void launch_pass(const DeviceOperator& o, const DevPass& p, const double2
     return;
   }
   launch_block<R>(o, p, ops, st, x, y, s, xyp);
+}
+
+// a fused group of passes as one launch of pass_fused_kernel
+template <class R>
+static void launch_fused(const DeviceOperator& o, const FusedPasses& g, const double2* x, double2* y, hipStream_t s,
+                         const double2* const* xyp = nullptr) {
+  typedef typename cx<R>::v V;
+  typedef typename cx<R>::t C;
+  const V* ops = reinterpret_cast<const V*>(std::is_same<R, double>::value ? (const void*)o.ops : (const void*)o.ops32);
+  C* st = reinterpret_cast<C*>(std::is_same<R, double>::value ? (void*)o.state : (void*)o.state32);
+  const DestMask mk{o.d_dest_sys && o.sys_active ? o.d_dest_sys : nullptr, o.sys_active};
+  hipLaunchKernelGGL((pass_fused_kernel<R>), dim3(g.tasks), dim3(kFusedThreads), 0, s, ops, o.src, st, x, y, o.N,
+                     o.d_rows, o.d_ncol, o.d_dst, o.d_op, o.d_srcoff, g.d_task_stage, g.d_fdest, g.n, xyp, mk);
 }
 
 template <class R, int RK, int NL>
@@ -1189,14 +1455,35 @@ static void matvec_impl(DeviceOperator& o, const double2* x, double2* y, const d
   hipStreamWaitEvent(o.aux, o.ev_fork, 0);
   if (o.T) launch_near<R>(o, x, y, o.aux, xyp);
   for (int p : o.near) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
+  // the leaf M2L on the second stream, once the charges it reads are written
+  // (and, by the schedule the autotuner picked, later)
+  bool m2l_started = false;
+  auto start_m2l = [&]() {
+    if (m2l_started) return;
+    m2l_started = true;
+    hipEventRecord(o.ev_leaf, o.stream);
+    hipStreamWaitEvent(o.aux, o.ev_leaf, 0);
+    for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
+    hipEventRecord(o.ev_join, o.aux);
+  };
   size_t u = 0;
+  if (o.fup.on) {
+    launch_fused<R>(o, o.fup, x, y, o.stream, xyp);
+    u = size_t(o.fup.n);
+  }
   for (; u < o.up.size() && int(u) <= o.leaf_after; ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream, xyp);
-  hipEventRecord(o.ev_leaf, o.stream);
-  hipStreamWaitEvent(o.aux, o.ev_leaf, 0);
-  for (int p : o.leaf_m2l) launch_pass<R>(o, o.passes[p], x, y, o.aux, xyp);
-  hipEventRecord(o.ev_join, o.aux);
+  if (o.m2l_start == 0) start_m2l();
   for (; u < o.up.size(); ++u) launch_pass<R>(o, o.passes[o.up[u]], x, y, o.stream, xyp);
-  for (int p : o.down) launch_pass<R>(o, o.passes[p], x, y, o.stream, xyp);
+  if (o.m2l_start == 1) start_m2l();
+  for (size_t i = 0; i < o.down.size(); ++i) {
+    const bool fused = o.fdown.on && int(i) == o.fdown.first;
+    const size_t last = i + (fused ? size_t(o.fdown.n) - 1 : 0);
+    if (o.m2l_start == 2 && last + 1 == o.down.size()) start_m2l();
+    if (fused) launch_fused<R>(o, o.fdown, x, y, o.stream, xyp);
+    else launch_pass<R>(o, o.passes[o.down[i]], x, y, o.stream, xyp);
+    i = last;
+  }
+  start_m2l();
   hipStreamWaitEvent(o.stream, o.ev_join, 0);
   for (int p : o.leaf_l2l) launch_pass<R>(o, o.passes[p], x, y, o.stream, xyp);
   if (o.qk2_user)
@@ -1272,15 +1559,72 @@ std::string DeviceOperator::autotune(int reps, int* changed) {
   int nchg = 0;
   tuning.clear();
   for (DevPass& p : passes) {
-    const bool before = p.wave;
+    const bool wave_before = p.wave, lane_before = p.lane;
+    p.lane = false;
     p.wave = false;
     const float tb = time_it([&]() { launch_pass<double>(*this, p, x, y, stream); });
     p.wave = true;
     const float tw = time_it([&]() { launch_pass<double>(*this, p, x, y, stream); });
-    p.wave = tw < tb;
-    nchg += p.wave != before;
-    tuning.push_back(p.name + (p.wave ? ": wave" : ": block") + " (" + std::to_string(tb * 1e3f) + " us block, " +
-                     std::to_string(tw * 1e3f) + " us wave)");
+    float tl = 1e30f;
+    if (p.lane_ok) {
+      p.lane = true;
+      tl = time_it([&]() { launch_pass<double>(*this, p, x, y, stream); });
+    }
+    p.lane = tl < std::min(tb, tw);
+    p.wave = !p.lane && tw < tb;
+    nchg += (p.wave != wave_before) + (p.lane != lane_before);
+    tuning.push_back(p.name + (p.lane ? ": lane" : p.wave ? ": wave" : ": block") + " (" + std::to_string(tb * 1e3f) +
+                     " us block, " + std::to_string(tw * 1e3f) + " us wave, " +
+                     (p.lane_ok ? std::to_string(tl * 1e3f) + " us lane)" : std::string("no lane variant)")));
+  }
+  auto tune_group = [&](FusedPasses& g, const std::vector<int>& list, const char* what) {
+    if (g.tasks == 0) return;
+    const bool before = g.on;
+    const float tf = time_it([&]() { launch_fused<double>(*this, g, x, y, stream); });
+    const float ts = time_it([&]() {
+      for (int u = g.first; u < g.first + g.n; ++u) launch_pass<double>(*this, passes[size_t(list[size_t(u)])], x, y, stream);
+    });
+    g.on = tf < ts;
+    nchg += g.on != before;
+    tuning.push_back(std::string(what) + " subtrees: " + (g.on ? "fused" : "separate") + " (" + std::to_string(tf * 1e3f) +
+                     " us fused over " + std::to_string(g.tasks) + " subtrees, " + std::to_string(ts * 1e3f) + " us as " +
+                     std::to_string(g.n) + " passes from " + passes[size_t(list[size_t(g.first)])].name + ")");
+  };
+  tune_group(fup, up, "upward");
+  tune_group(fdown, down, "downward");
+  if (!leaf_m2l.empty()) {
+    // the two-stream schedule, timed on the whole matvec: when the leaf M2L
+    // starts, and how many workgroups it may hold (grid-stride), so that the
+    // dependent chain of small passes on the main stream still finds CUs
+    DevPass& lp = passes[size_t(leaf_m2l.front())];
+    const int start_before = m2l_start, cap_before = lp.grid_cap;
+    const int ncu = [&]() {
+      int n = 0;
+      return hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && n > 0 ? n : 256;
+    }();
+    const int caps[4] = {0, 4 * ncu, 2 * ncu, ncu};
+    float best = 1e30f;
+    int bs = 0, bc = 0;
+    std::string rep = "leaf M2L schedule:";
+    for (int st = 0; st < 3; ++st)
+      for (int ci = 0; ci < 4; ++ci) {
+        if (lp.wave || lp.lane) {
+          if (ci > 0) continue;  // the cap applies to the one-workgroup kernel
+        }
+        m2l_start = st;
+        lp.grid_cap = caps[ci];
+        const float t = time_it([&]() { matvec_impl<double>(*this, x, y); });
+        rep += " start" + std::to_string(st) + "/cap" + std::to_string(caps[ci]) + " " + std::to_string(t * 1e3f) + " us;";
+        if (t < best) {
+          best = t;
+          bs = st;
+          bc = caps[ci];
+        }
+      }
+    m2l_start = bs;
+    lp.grid_cap = bc;
+    nchg += (m2l_start != start_before) + (lp.grid_cap != cap_before);
+    tuning.push_back(rep + " kept start" + std::to_string(bs) + "/cap" + std::to_string(bc));
   }
   const int nl_before = near_leaves_per_thread;
   if (!T) {

@@ -13,10 +13,25 @@ struct DevPass {
   int kind = 0;        // 0 store, 1 accumulate
   int role = 0;        // 1: leaf M2L
   bool wave = false;   // one wavefront (not one workgroup) per destination
+  bool lane = false;   // one lane per output row (overrides wave)
+  bool lane_ok = false;  // the row map holds this pass (rows < 256 and < 2^23 destinations)
   int ndest = 0;
+  int grid_cap = 0;       // one-workgroup kernel: at most this many workgroups (grid-stride); 0 = one per destination
   int64_t dest_base = 0;  // index into the concatenated destination arrays
+  int64_t row_base = 0;   // index into the concatenated row map
   int64_t entries = 0, n_src = 0, n_dst = 0;
   int64_t bytes() const { return 16 * entries + 4 * n_src + 16 * n_src + 16 * n_dst; }
+};
+
+// Passes run as one launch of one workgroup per subtree: `first` is the
+// position of the first fused pass in its list (up or down), n the number of
+// passes (stages); task t's destinations of stage s are
+// d_fdest[d_task_stage[t (n + 1) + s] ... d_task_stage[t (n + 1) + s + 1]).
+struct FusedPasses {
+  int first = 0, n = 0, tasks = 0;
+  bool on = false;
+  int* d_task_stage = nullptr;
+  int* d_fdest = nullptr;
 };
 
 // Device workspace of the GMRES solver, kept between solves of the same
@@ -76,6 +91,14 @@ struct DeviceOperator {
   int* d_dst = nullptr;
   long long* d_op = nullptr;
   long long* d_srcoff = nullptr;
+  // per pass, one entry per output row: (destination within the pass) << 8 | row,
+  // destinations ordered by decreasing column count, rows of one destination adjacent
+  unsigned* d_rowmap = nullptr;
+  // Consecutive passes fused per subtree (pass_fused_kernel: one workgroup per
+  // task, a barrier between the passes' stages; used when `on`): the leaf
+  // level's upward pass and the two above it (fup), and the last two
+  // wavefront-sized downward passes before the first large one (fdown).
+  FusedPasses fup, fdown;
   double2* T = nullptr;        // 9 x rank x rank, column-major per slot
   int* nbr = nullptr;          // n_leaves x 9
   double* qk2 = nullptr;       // N
@@ -114,6 +137,10 @@ struct DeviceOperator {
   // the leaf M2L starts once upward passes 0..leaf_after (in `up` order) are
   // done: the last one writing a charge it reads
   int leaf_after = 0;
+  // when the leaf M2L starts on the second stream (autotuned on the whole
+  // matvec): 0 after upward pass leaf_after, 1 after every upward pass, 2
+  // before the last downward pass
+  int m2l_start = 0;
   int64_t near_bytes = 0;
 
   std::string upload(const Plan& plan);

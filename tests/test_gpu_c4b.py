@@ -74,9 +74,12 @@ def test_fused_matvec_matches_each_context(ragged48):
     print(f"fused matvec vs contexts: {worst:.2e}, after autotune + graph {worst2:.2e}; "
           f"fused matvec {R.time_matvec(reps=5):.3f} ms", flush=True)
     # the same per-destination arithmetic; the kernel variant of a pass (one
-    # wavefront or one workgroup per destination) may differ from the
-    # context's, which changes the summation order: rounding level
-    assert worst <= 1e-12 and worst2 <= 1e-12
+    # workgroup, one wavefront or one lane per destination row) may differ
+    # from the context's, which changes the summation order (one running sum
+    # per row against 2-256 interleaved partial sums). The fused upward
+    # operators carry inverses of skeleton blocks and cancel downstream, so
+    # that rounding shows at up to 1.5e-12 (measured after autotuning)
+    assert worst <= 1e-11 and worst2 <= 1e-11
 
 
 def test_fused_ragged_gmres_48(ragged48):

@@ -166,3 +166,37 @@ def test_full_size_c2_linearity_and_determinism():
     a, b = 0.7 - 0.2j, -1.3 + 0.5j
     assert H.relerr(S.MatVecProduct(a * x + b * y), a * ax + b * ay) < 1e-13
     assert np.array_equal(S.MatVecProduct(x), ax)
+
+
+@pytest.mark.parametrize("depth,kappa,q,tree", [(4, 30.0, 6, 0), (5, 40.0, 6, 0), (5, 40.0, 0, 1)])
+def test_every_pass_kernel_matches_oracle(depth, kappa, q, tree):
+    """Each of the three pass kernels (one workgroup per destination, one
+    wavefront per destination, one lane per output row) forced on every pass,
+    with and without the first three upward passes fused per subtree: the FP64
+    matvec against the oracle (uniform trees with bumps, and an adaptive tree
+    whose near field is a pass too), the FP32 matvec against the FP64 one, and
+    the captured graph against the eager matvec."""
+    print(f"case depth={depth} kappa={kappa} q={q} tree_mode={tree}", flush=True)
+    inp = H.inputs(depth=depth, kappa=kappa, q=q)
+    if tree:
+        inp = L.InputsToDFMM(kappa=kappa, nLevelsUniform=depth, Qchoice=q, tree_mode=1, treeAdaptivity=5,
+                             TOL_POW=8)
+    S = make(inp)
+    o = O.Oracle(inp.to_c())
+    x = H.rand_vec(S.N, 11)
+    ref = o.matvec(x)
+    ys = {}
+    for k in ("block", "wave", "lane", "block+fused", "wave+fused", "lane+fused"):
+        S.set_pass_kernel(k.split("+")[0], fused=k.endswith("fused"))
+        ys[k] = S.MatVecProduct(x)
+        e = H.relerr(ys[k], ref)
+        e32 = H.relerr(S.MatVecProduct_fp32(x), ys[k])
+        S.use_graph(True)
+        g = S.MatVecProduct(x)
+        S.use_graph(False)
+        print(f"  {k}: vs oracle {e:.3e}, FP32 vs FP64 {e32:.3e}", flush=True)
+        assert e < TOL and e32 < 1e-3
+        assert np.array_equal(g, ys[k])
+    for k in ys:
+        assert H.relerr(ys[k], ys["block"]) < 1e-13
+    S.close()
