@@ -271,8 +271,8 @@ def seeded_angle(seed):
 
 C3_WORKLOAD = ("C3, 64 angles: uniform depth 7 (16384 leaves x 36 Chebyshev nodes = 589824 unknowns), kappa 160, "
                "8 seeded Gaussian bumps, NCA tol 1e-10; 64 plane waves at 2 pi j/64 solved by FP32-inner batched "
-               "GMRES inside FP64 iterative refinement to FP64-verified relative residual {tol:g}, against plain FP64 "
-               "GMRES(50) on the same GPU")
+               "GMRES(24) inside an FP64 flexible-GMRES(30) refinement to FP64-verified relative residual {tol:g}, "
+               "against plain FP64 GMRES(50) on the same GPU")
 
 
 def _relres(S, x, b):
@@ -282,11 +282,15 @@ def _relres(S, x, b):
 def bench_c3(args):
     """Config C3 with the 64 incident angles of the batched configuration: one
     step is one batch of 64 systems (plane waves at 2 pi j / 64, x0 = 0) solved
-    by batched GMRES with the FP32 block operator on FP32 MFMA inside FP64
-    refinement; every system must reach the FP64 true relative residual tol.
-    The baseline, timed in the same run on the same GPU, is plain FP64 GMRES(50)
-    on single systems of the batch. Right-hand sides and solutions stay in HBM
-    for the timed region."""
+    by batched GMRES with the FP32 block operator on FP32 MFMA inside an FP64
+    flexible GMRES outer iteration (each outer step: one FP32 GMRES(restart)
+    solve, one FP64 block matvec, the corrections re-weighted by the outer
+    least-squares problem); every system must reach the FP64 true relative
+    residual tol. The baseline, timed in the same run on the same GPU, is plain
+    FP64 GMRES(50) on single systems of the batch. Right-hand sides and
+    solutions stay in HBM for the timed region. Beside the line: the same batch
+    by plain FP64 refinement (x += d) around the same inner solves, and by FP64
+    batched GMRES."""
     inp = c3_inputs(0)
     t0 = time.time()
     S = lsg.DFMM(inp)
@@ -331,8 +335,9 @@ def bench_c3(args):
 
     def batch(mode):
         Xd.zero()
-        kw = dict(mode="fp32ir", inner_tol=1e-5, max_outer=40, max_restarts=1) if mode == "fp32ir" else \
-            dict(mode="fp64", max_restarts=40)
+        kw = {"fp32fgmres": dict(mode="fp32fgmres", inner_tol=1e-5, max_outer=args.outer, max_restarts=1),
+              "fp32ir": dict(mode="fp32ir", inner_tol=1e-5, max_outer=60, max_restarts=1),
+              "fp64": dict(mode="fp64", max_restarts=40)}[mode]
         _, st, rel = S.gmres_batch(None, device_ptrs=(Bd.ptr.value, Xd.ptr.value), nrhs=64, ld=N,
                                    restart_len=args.restart, tol=args.tol, **kw)
         if st["status"] != 0 or st["converged"] != 64 or not rel.max() <= args.tol:
@@ -340,13 +345,13 @@ def bench_c3(args):
         return st, rel
 
     for _ in range(args.warmup):
-        st, _ = batch("fp32ir")
+        st, _ = batch("fp32fgmres")
         log(f"warm-up batch: {st}")
     S._check(S._lib.lsg_sync(S._ctx))
     t0 = time.perf_counter()
     steps = []
     for _ in range(args.steps):
-        steps.append(batch("fp32ir"))
+        steps.append(batch("fp32fgmres"))
     S._check(S._lib.lsg_sync(S._ctx))
     elapsed = time.perf_counter() - t0
     value = 64 * args.steps / elapsed
@@ -355,6 +360,14 @@ def bench_c3(args):
     X32 = Xd.download()
     recomputed = [_relres(S, X32[q], Bh[q]) for q in range(64)]
     log(f"recomputed FP64 residuals: max {max(recomputed):.3e}")
+
+    # the same inner solves in plain FP64 refinement (x += d), one timed batch
+    batch("fp32ir")
+    S._check(S._lib.lsg_sync(S._ctx))
+    t0 = time.perf_counter()
+    st_ir, rel_ir = batch("fp32ir")
+    S._check(S._lib.lsg_sync(S._ctx))
+    t_ir = time.perf_counter() - t0
 
     # the same batch with the FP64 block operator (no refinement), one timed batch
     batch("fp64")
@@ -386,14 +399,18 @@ def bench_c3(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "c64 inner (FP32 operators, FP32 MFMA) + c128 refinement",
            "data": "synthetic",
            "config": {"workload": C3_WORKLOAD.format(tol=args.tol), "N": N, "systems_per_step": 64,
-                      "restart_len": args.restart, "inner_tol": 1e-5,
-                      "inner_arnoldi_steps_per_batch": st32["iterations"], "refinements": st32["refinements"],
+                      "restart_len": args.restart, "outer_krylov_dim": args.outer, "inner_tol": 1e-5,
+                      "inner_arnoldi_steps_per_batch": st32["iterations"], "outer_steps": st32["refinements"],
                       "max_rel_residual": float(rel32.max()), "recomputed_max_rel_residual": max(recomputed),
                       "inputs": "B, X resident in HBM (device pointers)"},
            "speedup_vs_plain_fp64": value / plain_sps,
            "plain_fp64": {"solves_per_s": plain_sps, "systems": list(plain_q), "restart_len": 50,
                           "arnoldi_steps": [st["iterations"] for st in plain_stats],
                           "max_rel_residual": max(st["rel_residual"] for st in plain_stats)},
+           "fp32_inner_plain_refinement": {"solves_per_s": 64 / t_ir, "speedup_vs_plain_fp64": 64 / t_ir / plain_sps,
+                                           "inner_arnoldi_steps": st_ir["iterations"],
+                                           "refinements": st_ir["refinements"],
+                                           "max_rel_residual": float(rel_ir.max())},
            "batched_fp64": {"solves_per_s": 64 / t64, "speedup_vs_plain_fp64": 64 / t64 / plain_sps,
                             "arnoldi_steps": st64["iterations"], "max_rel_residual": float(rel64.max()),
                             "max_rel_diff_fp32_inner_vs_fp64": max(float(np.linalg.norm(X32[q] - X64[q]) /
@@ -621,7 +638,8 @@ def main():
                     help="timed steps (default: 2 batches of 64 angles for c3, 10 solves for c2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones (default 1; c2: 2)")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--restart", type=int, default=24, help="restart length of the batched solves")
+    ap.add_argument("--restart", type=int, default=24, help="restart length of the batched solves (FP32 inner)")
+    ap.add_argument("--outer", type=int, default=30, help="outer Krylov dimension of the FP64 flexible GMRES")
     ap.add_argument("--nets", default="1,2,4,8,4,1",
                     help="learned workload: dilation sets of the networks to train, ';'-separated")
     ap.add_argument("--n-eval", type=int, default=256, help="learned workload: held-out systems")

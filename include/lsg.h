@@ -116,20 +116,22 @@ typedef struct lsg_stats {
   double rel_residual;    /* true ||b - A x|| / ||b|| in FP64 */
   double est_residual;    /* Givens residual estimate at exit */
   double seconds;         /* wall time */
-  int refinements;        /* FP64 refinement steps (mode LSG_SOLVE_FP32_IR), else 0 */
+  int refinements;        /* FP64 refinement steps (LSG_SOLVE_FP32_IR) or outer steps (FP32_FGMRES), else 0 */
 } lsg_stats;
 
 /* Solver selection for lsg_gmres_ex. */
 #define LSG_SOLVE_FP64 0     /* GMRES with the FP64 operator (lsg_gmres) */
 #define LSG_SOLVE_FP32_IR 1  /* FP64 iterative refinement around GMRES with the FP32 operator */
+#define LSG_SOLVE_FP32_FGMRES 2 /* lsg_gmres_batch only: FP64 flexible GMRES(max_outer) whose preconditioner is
+                                   one FP32 GMRES(restart_len) solve to inner_tol per outer step */
 
 typedef struct lsg_solve_opts {
-  int mode;               /* LSG_SOLVE_FP64 or LSG_SOLVE_FP32_IR */
+  int mode;               /* LSG_SOLVE_FP64, LSG_SOLVE_FP32_IR or (batched) LSG_SOLVE_FP32_FGMRES */
   int restart_len;        /* Krylov dimension per cycle, 1..500 */
   int max_restarts;       /* cycle cap (per inner solve in FP32_IR mode) */
   double tol;             /* target true FP64 relative residual (> 0 for FP32_IR) */
-  double inner_tol;       /* FP32_IR: floor of the inner relative residual (e.g. 1e-6) */
-  int max_outer;          /* FP32_IR: refinement cap */
+  double inner_tol;       /* FP32_IR: floor of the inner relative residual (e.g. 1e-6); FP32_FGMRES: inner tolerance */
+  int max_outer;          /* FP32_IR: refinement cap; FP32_FGMRES: outer Krylov dimension per cycle (<= 500) */
   int device_ptrs;        /* 1: b and x are device addresses of N complex128 */
 } lsg_solve_opts;
 
@@ -263,7 +265,11 @@ int lsg_gmres_noalloc(lsg_ctx* ctx, const void* b_dev, void* x_dev, int restart_
  * stopping point; each cycle opens with the true FP64 residual of every
  * system. opts->mode LSG_SOLVE_FP32_IR refines the batch in FP64 around
  * batched GMRES with the FP32 block operator (FP32 MFMA), each system to
- * max(inner_tol, tol ||b|| / (2 ||r||)). rel_residuals (host, nrhs, may be
+ * max(inner_tol, tol ||b|| / (2 ||r||)). LSG_SOLVE_FP32_FGMRES runs FP64
+ * flexible GMRES(max_outer) on the batch (at most 20 cycles), each outer step
+ * preconditioned by one FP32 batched GMRES(restart_len) solve to inner_tol:
+ * the corrections are re-weighted by the outer least-squares problem instead
+ * of added. rel_residuals (host, nrhs, may be
  * NULL) receives the final true relative residuals; stats->rel_residual is
  * their maximum and stats->converged the number of systems at tol. */
 int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* opts,

@@ -71,6 +71,7 @@ class lsg_stats(C.Structure):
 
 LSG_SOLVE_FP64 = 0
 LSG_SOLVE_FP32_IR = 1
+LSG_SOLVE_FP32_FGMRES = 2
 
 
 class lsg_solve_opts(C.Structure):
@@ -529,8 +530,12 @@ class DFMM:
                     nrhs=None, mode="fp64", inner_tol=1e-5, max_outer=20):
         """Batched GMRES on the columns of B (N x nrhs, nrhs <= 64); returns
         (X, stats, rel_residuals). With device_ptrs = (b_dev, x_dev) the
-        column-major blocks (leading dimension ld) stay on the device and X is None."""
-        o = lsg_solve_opts(mode={"fp64": LSG_SOLVE_FP64, "fp32ir": LSG_SOLVE_FP32_IR}[mode], restart_len=restart_len,
+        column-major blocks (leading dimension ld) stay on the device and X is None.
+        mode "fp64", "fp32ir" (FP64 refinement around FP32 GMRES) or
+        "fp32fgmres" (FP64 flexible GMRES(max_outer), each outer step
+        preconditioned by one FP32 GMRES(restart_len) solve to inner_tol)."""
+        o = lsg_solve_opts(mode={"fp64": LSG_SOLVE_FP64, "fp32ir": LSG_SOLVE_FP32_IR,
+                                 "fp32fgmres": LSG_SOLVE_FP32_FGMRES}[mode], restart_len=restart_len,
                            max_restarts=max_restarts, tol=tol, inner_tol=inner_tol, max_outer=max_outer,
                            device_ptrs=1 if device_ptrs else 0)
         st = lsg_stats()

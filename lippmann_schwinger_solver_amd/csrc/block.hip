@@ -566,11 +566,13 @@ __global__ void bscale_kernel(const double* __restrict__ w, BGivens st, TV* __re
 }
 
 // Arnoldi step j for every active system: h in [0, j], ||w||^2 at j+1 (from the
-// block dot); ||w - V h|| by Pythagoras; Givens rotation of H column j; the
-// relative estimate |g_{j+1}| / ||b_q|| goes to host_est[j*64 + q]. A system
-// whose estimate reaches 0.5 tol, or whose new vector vanishes, is frozen.
+// block dot); ||w - V h|| by Pythagoras, or from hn2 (||w - V h||^2 per
+// system, when the caller orthogonalised twice); Givens rotation of H column
+// j; the relative estimate |g_{j+1}| / ||b_q|| goes to host_est[j*64 + q]. A
+// system whose estimate reaches 0.5 tol, or whose new vector vanishes, is
+// frozen.
 __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, int j, double* host_est,
-                             int* host_active) {
+                             int* host_active, const double* __restrict__ hn2) {
   const int q = threadIdx.x;
   if (q >= kNB) return;
   if (!st.active[q]) {
@@ -587,7 +589,7 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
     Hc[i * kNB + q] = v;
   }
   const double wn2 = h[(j + 1) * kNB + q].x;
-  const double hn = sqrt(fmax(wn2 - s2, 0.0));
+  const double hn = sqrt(fmax(hn2 ? hn2[q] : wn2 - s2, 0.0));
   for (int i = 0; i < j; ++i) {
     const double c = st.cs[i * kNB + q];
     const double2 s = st.sn[i * kNB + q];
@@ -624,6 +626,28 @@ __global__ void btail_kernel(const double2* __restrict__ h, BGivens st, int m, i
   if (stop) st.active[q] = 0;
   host_est[j * kNB + q] = est;
   host_active[q] = stop ? 0 : 1;
+}
+
+// Second pass of a twice-orthogonalised step: h2 holds the projections of the
+// once-projected vector w' and ||w'||^2 at nc; h1 += h2 gives the step's
+// Hessenberg column, hn2[q] = ||w'||^2 - sum |h2|^2 its subdiagonal squared
+// (no cancellation left: w' is already nearly orthogonal to V).
+__global__ void bcgs2_combine_kernel(double2* __restrict__ h1, const double2* __restrict__ h2, int nc,
+                                     double* __restrict__ hn2) {
+  const int q = threadIdx.x;
+  if (q >= kNB) return;
+  double s2 = 0.0;
+  for (int i = 0; i < nc; ++i) {
+    const double2 a = h1[i * kNB + q], b = h2[i * kNB + q];
+    s2 += b.x * b.x + b.y * b.y;
+    h1[i * kNB + q] = make_double2(a.x + b.x, a.y + b.y);
+  }
+  hn2[q] = h2[nc * kNB + q].x - s2;
+}
+
+__global__ void bfill_kernel(double* __restrict__ a, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 // v[n, q] = (w[n, q] - sum_i V_i[n, q] h[i, q]) / hn_q, zero for systems with hn_q = 0
@@ -865,6 +889,7 @@ std::string BlockOperator::setup(DeviceOperator& o) {
 
 void BlockOperator::release() {
   work.release();
+  outer.release();
   void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part};
   qk2_cols = nullptr;
   bstate32 = nullptr;
@@ -1052,6 +1077,11 @@ std::string BlockWork::reserve(int N_, int m_) {
   return "";
 }
 
+std::string BlockWork::flex() {
+  if (!Z) HIP_OK(hipMalloc((void**)&Z, sizeof(double) * size_t(N) * kLD * size_t(m)));
+  return "";
+}
+
 std::string BlockWork::basis(bool f32) {
   const size_t n = size_t(N) * kLD * size_t(m + 1);
   if (f32 && !V32) HIP_OK(hipMalloc((void**)&V32, sizeof(float) * n));
@@ -1061,7 +1091,7 @@ std::string BlockWork::basis(bool f32) {
 }
 
 void BlockWork::release() {
-  void* ptrs[] = {V, V32, w32, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  void* ptrs[] = {V, V32, w32, Z, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (host_rel) (void)hipHostFree(host_rel);
@@ -1075,7 +1105,8 @@ void BlockWork::release() {
 // the solution update accumulate in FP64 either way.
 template <class TV>
 static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
-                                 double tol, double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
+                                 double tol, double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero,
+                                 bool x_zero) {
   BatchResult res;
   auto t0 = std::chrono::steady_clock::now();
   DeviceOperator& o = *bo.op;
@@ -1177,8 +1208,12 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   const unsigned gu = blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 64);   // kUpdRows rows per (thread, system)
   int total_it = 0;
   for (int cyc = 0; cyc <= max_restarts; ++cyc) {
-    amv(X, w);
-    hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, ntot);
+    if (cyc == 0 && x_zero) {
+      hipMemcpyAsync(w, B, sizeof(double) * size_t(ntot), hipMemcpyDeviceToDevice, s);
+    } else {
+      amv(X, w);
+      hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, ntot);
+    }
     dot(0, w, true);
     hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, nrhs, host_rel, host_active);
     if (hipStreamSynchronize(s) != hipSuccess) {
@@ -1213,7 +1248,8 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
           if (w32) {
             bo.matvec32((const float*)(V + size_t(jj) * vs), w32);
             dot(jj + 1, (const float*)w32, true);
-            hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
+            hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active,
+                               (const double*)nullptr);
             hipLaunchKernelGGL((bupdate_kernel<TV, float>), dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h,
                                (const float*)w32, st.hn, V + size_t(jj + 1) * vs, (long long)N);
             continue;
@@ -1221,7 +1257,8 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
         }
         amv((const TV*)(V + size_t(jj) * vs), w);
         dot(jj + 1, (const double*)w, true);
-        hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active);
+        hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active,
+                           (const double*)nullptr);
         hipLaunchKernelGGL((bupdate_kernel<TV, double>), dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h,
                            (const double*)w, st.hn, V + size_t(jj + 1) * vs, (long long)N);
       }
@@ -1253,12 +1290,14 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
 }
 
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
-                        double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero) {
+                        double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero, bool x_zero) {
   // the FP32 inner solves keep their Krylov basis in FP32 (half the
   // Gram-Schmidt traffic); the FP64 solver keeps it in FP64
   if (fp32)
-    return block_gmres_t<float>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, true, tol_per, allow_zero);
-  return block_gmres_t<double>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, false, tol_per, allow_zero);
+    return block_gmres_t<float>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, true, tol_per, allow_zero,
+                                x_zero);
+  return block_gmres_t<double>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, false, tol_per, allow_zero,
+                               x_zero);
 }
 
 BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
@@ -1362,7 +1401,7 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
     hipMemcpyAsync(mask, maskh.data(), sizeof(int) * kNB, hipMemcpyHostToDevice, s);
     hipLaunchKernelGGL(bmask_kernel, dim3(gv), dim3(kT), 0, s, R, mask, vs);
     hipMemsetAsync(D, 0, sizeof(double) * size_t(vs), s);
-    BatchResult g = block_gmres(bo, R, D, nrhs, m, max_restarts, 0.0, nullptr, true, tol_in.data(), true);
+    BatchResult g = block_gmres(bo, R, D, nrhs, m, max_restarts, 0.0, nullptr, true, tol_in.data(), true, true);
     res.iterations += g.iterations;
     res.restarts += g.restarts;
     if (g.status != 0 && g.status != LSG_ERR_NOCONV) {
@@ -1378,6 +1417,175 @@ BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nr
     res.error = "batched refinement: not every system reached tol";
   }
   cleanup();
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+BatchResult block_fgmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
+                         double tol, double inner_tol, int K, int max_cycles, double* rel_residuals) {
+  BatchResult res;
+  auto t0 = std::chrono::steady_clock::now();
+  DeviceOperator& o = *bo.op;
+  hipStream_t s = o.stream;
+  const int N = o.N;
+  if (nrhs < 1 || nrhs > kNB || !(tol > 0) || !(inner_tol > 0) || K < 1 || K > 500 || m < 1 || m > 500 ||
+      max_restarts < 1 || max_cycles < 1) {
+    res.status = LSG_ERR_ARG;
+    res.error = "batched flexible GMRES: 1 <= nrhs <= 64, tol > 0, inner_tol > 0, 1 <= K, m <= 500";
+    return res;
+  }
+  const long long vs = (long long)N * kLD;
+  const int nrb = (N + kBRows - 1) / kBRows;
+  BlockWork& W = bo.outer;
+  if (W.m != K || W.N != N) {
+    W.release();
+    std::string e = W.reserve(N, K);
+    if (!e.empty()) {
+      W.release();
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched flexible GMRES workspace allocation failed: " + e;
+      return res;
+    }
+  }
+  {
+    std::string e = W.basis(false);
+    if (e.empty()) e = W.flex();
+    if (!e.empty()) {
+      W.release();
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched flexible GMRES basis allocation failed: " + e;
+      return res;
+    }
+  }
+  double *V = W.V, *Z = W.Z, *w = W.w;
+  double2 *partial = W.partial, *h = W.h, *y = W.y;
+  // per system: ones (the unnormalised first projection) and ||w'||^2 - |h2|^2
+  double* small = nullptr;
+  if (hipMalloc((void**)&small, sizeof(double) * 2 * kNB) != hipSuccess) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched flexible GMRES: allocation failed";
+    return res;
+  }
+  struct Release {
+    hipStream_t s;
+    void* p;
+    ~Release() {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(p);
+    }
+  } release_small{s, small};
+  double *ones = small, *hn2 = small + kNB;
+  hipLaunchKernelGGL(bfill_kernel, dim3(1), dim3(kNB), 0, s, ones, kNB, 1.0);
+  BGivens st{W.H, W.cs, W.sn, W.g, W.hn, W.bnorm, W.active, W.kq, W.tolq};
+  double *host_rel = W.host_rel, *host_est = W.host_est;
+  int* host_active = W.host_active;
+  auto dot = [&](int nc, const double* vec) {
+    const int ncx = nc + 1;
+    hipLaunchKernelGGL((bdot_kernel<double, double>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs,
+                       nc, ncx, vec, N, partial);
+    block_reduce(partial, nrb, ncx, partial + size_t(nrb) * (K + 2) * kNB, h, s);
+  };
+  dot(0, B);
+  std::vector<double2> bn2(kNB);
+  hipMemcpyAsync(bn2.data(), h, sizeof(double2) * kNB, hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    res.status = LSG_ERR_DEVICE;
+    res.error = "batched flexible GMRES: norm of b failed";
+    return res;
+  }
+  std::vector<double> bnorm(kNB, 1.0), tolv(kNB, tol), tol_in(kNB, inner_tol);
+  for (int q = 0; q < nrhs; ++q) {
+    bnorm[q] = std::sqrt(std::max(bn2[q].x, 0.0));
+    if (!(bnorm[q] > 0) || !std::isfinite(bnorm[q])) {
+      (void)hipStreamSynchronize(s);
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched flexible GMRES: a right-hand side is zero or holds NaN/Inf";
+      return res;
+    }
+  }
+  hipMemcpyAsync(st.bnorm, bnorm.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(st.tolq, tolv.data(), sizeof(double) * kNB, hipMemcpyHostToDevice, s);
+  const unsigned gv = blocks_for(vs, kT);
+  const unsigned gq = blocks_for((long long)N * kNB, kT);
+  const unsigned gu = blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 64);
+  for (int cyc = 0; cyc <= max_cycles; ++cyc) {
+    bo.matvec(X, w);   // true FP64 residual
+    hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, vs);
+    dot(0, w);
+    hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, K, nrhs, host_rel, host_active);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      res.status = LSG_ERR_DEVICE;
+      res.error = "batched flexible GMRES: device error";
+      break;
+    }
+    int n_act = 0, n_conv = 0;
+    double worst = 0.0;
+    bool finite = true;
+    for (int q = 0; q < nrhs; ++q) {
+      n_act += host_active[q];
+      n_conv += host_rel[q] <= tol ? 1 : 0;
+      worst = std::max(worst, host_rel[q]);
+      finite = finite && std::isfinite(host_rel[q]);
+      if (rel_residuals) rel_residuals[q] = host_rel[q];
+    }
+    res.converged = n_conv;
+    res.max_rel_residual = worst;
+    if (!finite) {
+      res.status = LSG_ERR_NUMERIC;
+      res.error = "batched flexible GMRES: residual is NaN/Inf";
+      break;
+    }
+    if (n_act == 0 || cyc == max_cycles) break;
+    hipLaunchKernelGGL(bscale_kernel<double>, dim3(gv), dim3(kT), 0, s, w, st, V, vs);
+    int steps = 0;
+    for (int k = 0; k < K; ++k) {
+      // z_k ~ A^-1 v_k by the FP32 batched GMRES (columns of frozen systems are
+      // zero: converged at once)
+      double* zk = Z + size_t(k) * vs;
+      hipMemsetAsync(zk, 0, sizeof(double) * size_t(vs), s);
+      BatchResult g = block_gmres(bo, V + size_t(k) * vs, zk, nrhs, m, max_restarts, 0.0, nullptr, true, tol_in.data(),
+                                  true, true);
+      res.iterations += g.iterations;
+      if (g.status != 0 && g.status != LSG_ERR_NOCONV) {
+        res.status = g.status;
+        res.error = "inner FP32 batched solve: " + g.error;
+        break;
+      }
+      bo.matvec(zk, w);
+      // classical Gram-Schmidt twice: A z_k lies almost in span(V) (z_k solves
+      // A z = v_k to inner_tol), so the new direction is a small remainder
+      // whose norm cannot come from ||w||^2 - |h|^2 (that difference cancels)
+      dot(k + 1, w);
+      hipMemcpyAsync(y, h, sizeof(double2) * size_t(k + 1) * kNB, hipMemcpyDeviceToDevice, s);
+      hipLaunchKernelGGL((bupdate_kernel<double, double>), dim3(gu), dim3(kT), 0, s, V, vs, k + 1, y,
+                         (const double*)w, ones, w, (long long)N);
+      dot(k + 1, w);
+      hipLaunchKernelGGL(bcgs2_combine_kernel, dim3(1), dim3(kNB), 0, s, y, h, k + 1, hn2);
+      hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, y, st, K, k, host_est, host_active,
+                         (const double*)hn2);
+      hipLaunchKernelGGL((bupdate_kernel<double, double>), dim3(gu), dim3(kT), 0, s, V, vs, k + 1, h,
+                         (const double*)w, st.hn, V + size_t(k + 1) * vs, (long long)N);
+      ++res.refinements;
+      steps = k + 1;
+      if (hipStreamSynchronize(s) != hipSuccess) {
+        res.status = LSG_ERR_DEVICE;
+        res.error = "batched flexible GMRES: device error";
+        break;
+      }
+      int still = 0;
+      for (int q = 0; q < nrhs; ++q) still += host_active[q];
+      if (still == 0) break;
+    }
+    if (res.status != 0) break;
+    hipLaunchKernelGGL(btrisolve_kernel, dim3(1), dim3(kNB), 0, s, st, K, y);
+    hipLaunchKernelGGL(baxpy_kernel<double>, dim3(gq), dim3(kT), 0, s, Z, vs, steps, y, X, (long long)N);
+    res.restarts = cyc + 1;
+  }
+  if (res.status == 0 && res.converged < nrhs) {
+    res.status = LSG_ERR_NOCONV;
+    res.error = "batched flexible GMRES: not every system reached tol";
+  }
+  (void)hipStreamSynchronize(s);
   res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return res;
 }

@@ -27,6 +27,7 @@ struct BlockWork {
   double *V = nullptr, *w = nullptr;
   float* V32 = nullptr;   // FP32 basis of the FP32 inner solves
   float* w32 = nullptr;   // FP32 product A v_j of the FP32 inner solves (when the leaf kernel writes every output row)
+  double* Z = nullptr;    // flexible GMRES: the m preconditioned directions (FP64), allocated by flex()
   double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *bnorm = nullptr, *tolq = nullptr;
   int *active = nullptr, *kq = nullptr;
@@ -34,6 +35,7 @@ struct BlockWork {
   int* host_active = nullptr;                        // pinned
   std::string reserve(int N, int m);   // everything but the basis
   std::string basis(bool f32);         // the FP64 or FP32 basis, on first use
+  std::string flex();                  // Z, on first use
   void release();
 };
 
@@ -54,6 +56,7 @@ struct BlockOperator {
   std::vector<int> pass_nseg;      // column segments per destination (1: not split)
   void* split_part = nullptr;      // partial products of split passes
   BlockWork work;
+  BlockWork outer;    // the FP64 outer iteration of block_fgmres
   std::string setup(DeviceOperator& o);
   std::string enable_fp32();  // FP32 operators (DeviceOperator::enable_fp32) and pass state
   void release();
@@ -87,14 +90,25 @@ struct BatchResult {
 // fp32: the block matvecs use FP32 operators and FP32 MFMA; tol_per (host, nrhs):
 // per-system tolerances instead of tol; allow_zero: zero right-hand sides are
 // accepted (their systems are converged with x unchanged).
+// x_zero: X is zero on entry (the first residual is B, no matvec).
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
                         double* rel_residuals /* host, nrhs */, bool fp32 = false, const double* tol_per = nullptr,
-                        bool allow_zero = false);
+                        bool allow_zero = false, bool x_zero = false);
 // FP64 iterative refinement of the batch around FP32 batched GMRES:
 // R = B - A X (FP64 block matvec); systems above tol solve A D = R with the FP32
 // block operator to max(inner_tol, tol ||b_q|| / (2 ||r_q||)); X += D.
 BatchResult block_gmres_ir(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
                            double tol, double inner_tol, int max_outer, double* rel_residuals);
+// FP64 flexible GMRES of the batch, outer Krylov dimension K per cycle (at most
+// max_cycles cycles), preconditioned by the FP32 batched GMRES: outer step k
+// solves A z_k = v_k with the FP32 block operator (m steps per cycle,
+// max_restarts cycles, to inner_tol), w = A z_k in FP64 is orthogonalised
+// against the FP64 outer basis (classical Gram-Schmidt, Givens per system), and
+// X += Z y closes the cycle. Unlike refinement, every correction is re-weighted
+// by the outer least-squares problem. Each cycle opens with the true FP64
+// residual of every system, which decides convergence.
+BatchResult block_fgmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
+                         double tol, double inner_tol, int K, int max_cycles, double* rel_residuals);
 
 // column-major complex (x + 2*ld*j) <-> row-split block, nrhs <= nb columns
 void to_split(const double2* x, long long ld, int nrhs, int N, int nb, double* out, hipStream_t s);

@@ -737,7 +737,8 @@ int lsg_gmres_ex(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_opts*
   if (!ctx) return LSG_ERR_ARG;
   if (!o || !b || !x || o->restart_len < 1 || o->restart_len > 500 || o->max_restarts < 1)
     return ctx->set(LSG_ERR_ARG, "bad gmres arguments (restart_len in [1,500], max_restarts >= 1)");
-  if (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR) return ctx->set(LSG_ERR_ARG, "unknown solver mode");
+  if (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR)
+    return ctx->set(LSG_ERR_ARG, "unknown solver mode (FP32_FGMRES is batched only: lsg_gmres_batch)");
   // tol <= 0 is the reference's fixed-cycle mode; NaN or Inf is an error
   if (!std::isfinite(o->tol)) return ctx->set(LSG_ERR_ARG, "tol must be finite");
   if (o->mode == LSG_SOLVE_FP32_IR && (!(o->tol > 0) || !(o->inner_tol > 0) || o->max_outer < 1))
@@ -811,16 +812,21 @@ int lsg_gmres_noalloc(lsg_ctx* ctx, const void* b_dev, void* x_dev, int restart_
   return LSG_OK;
 }
 
+// outer cycles of the batched flexible GMRES (each opens with the true residual)
+constexpr int kFgmresCycles = 20;
+
 int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t ld, const lsg_solve_opts* o,
                     double* rel_residuals, lsg_stats* stats) {
   if (!ctx) return LSG_ERR_ARG;
   const int64_t N = ctx->hb.plan.N;
   if (!o || !b || !x || nrhs < 1 || nrhs > kBlockRhs || ld < N || !(o->tol > 0) || o->restart_len < 1 ||
-      o->restart_len > 500 || o->max_restarts < 1 || (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR) ||
-      (o->mode == LSG_SOLVE_FP32_IR && (!(o->inner_tol > 0) || o->max_outer < 1)))
+      o->restart_len > 500 || o->max_restarts < 1 ||
+      (o->mode != LSG_SOLVE_FP64 && o->mode != LSG_SOLVE_FP32_IR && o->mode != LSG_SOLVE_FP32_FGMRES) ||
+      (o->mode != LSG_SOLVE_FP64 && (!(o->inner_tol > 0) || o->max_outer < 1)) ||
+      (o->mode == LSG_SOLVE_FP32_FGMRES && o->max_outer > 500))
     return ctx->set(LSG_ERR_ARG,
                     "bad batched gmres arguments (1 <= nrhs <= 64, ld >= N, tol > 0, restart_len in [1,500]; "
-                    "FP32_IR: inner_tol > 0, max_outer >= 1)");
+                    "FP32_IR, FP32_FGMRES: inner_tol > 0, max_outer >= 1 (<= 500 for FP32_FGMRES))");
   std::string e = ctx->block_ready();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   hipStream_t s = ctx->dev.stream;
@@ -844,8 +850,11 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
   BatchResult r = o->mode == LSG_SOLVE_FP64
                       ? block_gmres(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
                                     rel.data())
-                      : block_gmres_ir(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
-                                       o->inner_tol, o->max_outer, rel.data());
+                  : o->mode == LSG_SOLVE_FP32_IR
+                      ? block_gmres_ir(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
+                                       o->inner_tol, o->max_outer, rel.data())
+                      : block_fgmres(ctx->blk, ctx->bx, ctx->by, nrhs, o->restart_len, o->max_restarts, o->tol,
+                                     o->inner_tol, o->max_outer, kFgmresCycles, rel.data());
   if (o->device_ptrs) {
     from_split(ctx->by, int(N), kBlockRhs, nrhs, reinterpret_cast<double2*>(x), ld, s);
   } else {

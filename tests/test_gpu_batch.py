@@ -98,3 +98,33 @@ def test_fp32_block_matvec_and_batched_refinement():
     for j in (0, 7, 15):
         assert H.relerr(S.MatVecProduct(X[:, j]), B[:, j]) <= 1.5e-8
         assert H.relerr(X[:, j], X64[:, j]) < 1e-6
+
+
+@pytest.mark.parametrize("m,K", [(24, 30), (8, 40)])
+def test_batched_flexible_gmres_fp32_inner(m, K):
+    """FP64 flexible GMRES(K) of the batch, each outer step preconditioned by
+    one FP32 batched GMRES(m) solve: every system reaches 1e-8 in FP64
+    (recomputed through the single-vector matvec and checked through the
+    oracle's operator for two of them), with the same solutions as the FP64
+    batch to the conditioning times the tolerance; refinement with the same
+    inner solves is run beside it for the step counts."""
+    inp = H.inputs(depth=5, kappa=40.0, q=6, seed=1)
+    S = make(inp)
+    o = O.Oracle(inp.to_c())
+    angles = 2 * np.pi * np.arange(16) / 16
+    B = S.rhs_angles(angles)
+    X, st, rel = S.gmres_batch(B, restart_len=m, max_restarts=1, tol=1e-8, mode="fp32fgmres", inner_tol=1e-5,
+                               max_outer=K)
+    Xi, si, _ = S.gmres_batch(B, restart_len=m, max_restarts=1, tol=1e-8, mode="fp32ir", inner_tol=1e-5,
+                              max_outer=80)
+    X64, s64, _ = S.gmres_batch(B, restart_len=50, max_restarts=20, tol=1e-8)
+    print(f"m={m} K={K}: flexible {st}; refinement {si['iterations']} inner steps, {si['refinements']} refinements; "
+          f"FP64 {s64['iterations']} steps", flush=True)
+    assert st["status"] == 0 and st["converged"] == 16 and rel.max() <= 1e-8 and st["refinements"] >= 1
+    for j in range(16):
+        assert H.relerr(S.MatVecProduct(X[:, j]), B[:, j]) <= 1.05e-8
+        assert H.relerr(X[:, j], X64[:, j]) < 1e-6
+    for j in (0, 9):
+        assert H.relerr(o.matvec(X[:, j]), B[:, j]) <= 1.05e-8
+    S.close()
+    o.close()
