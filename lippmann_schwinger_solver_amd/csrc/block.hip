@@ -485,6 +485,71 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
   }
 }
 
+// The block dot for an FP32 basis and product: lane l of a wavefront covers the
+// systems 2 (l % 32) and 2 (l % 32) + 1 with one 8-byte load per value and
+// half-wave h = l / 32 takes every other row, so one load instruction moves two
+// rows' 512 bytes (the FP32 rows are 512 bytes: 64 real parts, 64 imaginary
+// parts); the two halves are summed by a cross-lane exchange, the four
+// wavefronts through LDS. Same partial layout as bdot_kernel.
+__global__ __launch_bounds__(kT) void bdot2_kernel(const float* __restrict__ V, long long vs, int nc, int ncx,
+                                                   const float* __restrict__ w, int N, double2* __restrict__ partial) {
+  __shared__ double2 red[kT / 64][kBCols][kNB];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, h = lane >> 5, p = lane & 31;
+  const int g0 = blockIdx.y * kBCols;
+  const long long n0 = (long long)blockIdx.x * kBRows;
+  double2 acc[kBCols][2];
+#pragma unroll
+  for (int u = 0; u < kBCols; ++u) acc[u][0] = acc[u][1] = make_double2(0.0, 0.0);
+  for (int k = 0; k < kBRows / 8; ++k) {
+    const long long n = n0 + 8 * k + 2 * wv + h;
+    if (n >= N) break;
+    const float2 wr = *reinterpret_cast<const float2*>(w + n * kLD + 2 * p);
+    const float2 wi = *reinterpret_cast<const float2*>(w + n * kLD + kNB + 2 * p);
+#pragma unroll
+    for (int u = 0; u < kBCols; ++u) {
+      const int col = g0 + u;
+      float2 vr = make_float2(0.f, 0.f), vi = make_float2(0.f, 0.f);
+      if (col < nc) {
+        vr = *reinterpret_cast<const float2*>(V + col * vs + n * kLD + 2 * p);
+        vi = *reinterpret_cast<const float2*>(V + col * vs + n * kLD + kNB + 2 * p);
+      } else if (col == nc) {
+        vr = wr;
+        vi = wi;
+      }
+      acc[u][0].x = fma(double(vr.x), double(wr.x), fma(double(vi.x), double(wi.x), acc[u][0].x));
+      acc[u][0].y = fma(double(vr.x), double(wi.x), fma(-double(vi.x), double(wr.x), acc[u][0].y));
+      acc[u][1].x = fma(double(vr.y), double(wr.y), fma(double(vi.y), double(wi.y), acc[u][1].x));
+      acc[u][1].y = fma(double(vr.y), double(wi.y), fma(-double(vi.y), double(wr.y), acc[u][1].y));
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kBCols; ++u)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      acc[u][e].x += __shfl_xor(acc[u][e].x, 32);
+      acc[u][e].y += __shfl_xor(acc[u][e].y, 32);
+    }
+  if (h == 0) {
+#pragma unroll
+    for (int u = 0; u < kBCols; ++u) {
+      red[wv][u][2 * p] = acc[u][0];
+      red[wv][u][2 * p + 1] = acc[u][1];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < kBCols * kNB; e += kT) {
+    const int u = e / kNB, qq = e % kNB;
+    if (g0 + u < ncx) {
+      double2 sm = red[0][u][qq];
+      for (int k = 1; k < kT / 64; ++k) {
+        sm.x += red[k][u][qq].x;
+        sm.y += red[k][u][qq].y;
+      }
+      partial[((long long)blockIdx.x * ncx + g0 + u) * kNB + qq] = sm;
+    }
+  }
+}
+
 // First stage of the row-block sums: workgroup (col, sg) adds the row blocks
 // rb = sg, sg + kRS, ... into part2[(sg * ncx + col) * 64 + q].
 __global__ __launch_bounds__(kT) void breduce1_kernel(const double2* __restrict__ partial, int nrb, int ncx,
@@ -689,6 +754,52 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, l
       const long long o = base + r * kLD;
       vout[o] = TV((double(w[o]) - ar[r]) * inv);
       vout[o + kNB] = TV((double(w[o + kNB]) - ai[r]) * inv);
+    }
+  }
+}
+
+// The update for an FP32 basis and product: lane l of a half-wavefront covers
+// the systems 2 (l % 32) and 2 (l % 32) + 1 with 8-byte loads (as bdot2_kernel),
+// each thread kUpdRows consecutive rows of its two systems.
+__global__ __launch_bounds__(kT) void bupdate2_kernel(const float* __restrict__ V, long long vs, int nc,
+                                                      const double2* __restrict__ h, const float* __restrict__ w,
+                                                      const double* __restrict__ hn, float* __restrict__ vout,
+                                                      long long n_rows) {
+  const int p = threadIdx.x & 31;
+  const long long n0 = ((long long)blockIdx.x * (kT / 32) + (threadIdx.x >> 5)) * kUpdRows;
+  if (n0 >= n_rows) return;
+  const int nr = int(min((long long)kUpdRows, n_rows - n0));
+  double ar[kUpdRows][2], ai[kUpdRows][2];
+#pragma unroll
+  for (int r = 0; r < kUpdRows; ++r) ar[r][0] = ar[r][1] = ai[r][0] = ai[r][1] = 0.0;
+  const long long base = n0 * kLD + 2 * p;
+#pragma unroll 2
+  for (int i = 0; i < nc; ++i) {
+    const double2 h0 = h[i * kNB + 2 * p], h1 = h[i * kNB + 2 * p + 1];
+    const float* Vi = V + i * vs + base;
+#pragma unroll
+    for (int r = 0; r < kUpdRows; ++r) {
+      if (r < nr) {
+        const float2 vr = *reinterpret_cast<const float2*>(Vi + r * kLD);
+        const float2 vi = *reinterpret_cast<const float2*>(Vi + r * kLD + kNB);
+        ar[r][0] = fma(double(vr.x), h0.x, fma(-double(vi.x), h0.y, ar[r][0]));
+        ai[r][0] = fma(double(vr.x), h0.y, fma(double(vi.x), h0.x, ai[r][0]));
+        ar[r][1] = fma(double(vr.y), h1.x, fma(-double(vi.y), h1.y, ar[r][1]));
+        ai[r][1] = fma(double(vr.y), h1.y, fma(double(vi.y), h1.x, ai[r][1]));
+      }
+    }
+  }
+  const double d0 = hn[2 * p], d1 = hn[2 * p + 1];
+  const double inv0 = d0 > 0.0 ? 1.0 / d0 : 0.0, inv1 = d1 > 0.0 ? 1.0 / d1 : 0.0;
+#pragma unroll
+  for (int r = 0; r < kUpdRows; ++r) {
+    if (r < nr) {
+      const long long o = base + r * kLD;
+      const float2 wr = *reinterpret_cast<const float2*>(w + o), wi = *reinterpret_cast<const float2*>(w + o + kNB);
+      *reinterpret_cast<float2*>(vout + o) =
+          make_float2(float((double(wr.x) - ar[r][0]) * inv0), float((double(wr.y) - ar[r][1]) * inv1));
+      *reinterpret_cast<float2*>(vout + o + kNB) =
+          make_float2(float((double(wi.x) - ai[r][0]) * inv0), float((double(wi.y) - ai[r][1]) * inv1));
     }
   }
 }
@@ -1153,8 +1264,12 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   auto dot = [&](int nc, const auto* vec, bool norm) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(vec)>> TW;
     const int ncx = nc + (norm ? 1 : 0);
-    hipLaunchKernelGGL((bdot_kernel<TV, TW>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx,
-                       vec, N, partial);
+    if constexpr (std::is_same<TV, float>::value && std::is_same<TW, float>::value)
+      hipLaunchKernelGGL(bdot2_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
+                         N, partial);
+    else
+      hipLaunchKernelGGL((bdot_kernel<TV, TW>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc,
+                         ncx, vec, N, partial);
     block_reduce(partial, nrb, ncx, partial + size_t(nrb) * (m + 2) * kNB, h, s);
   };
   // ||b_q||
@@ -1250,8 +1365,9 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
             dot(jj + 1, (const float*)w32, true);
             hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active,
                                (const double*)nullptr);
-            hipLaunchKernelGGL((bupdate_kernel<TV, float>), dim3(gu), dim3(kT), 0, s, V, vs, jj + 1, h,
-                               (const float*)w32, st.hn, V + size_t(jj + 1) * vs, (long long)N);
+            hipLaunchKernelGGL(bupdate2_kernel, dim3(blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 32)), dim3(kT),
+                               0, s, (const float*)V, vs, jj + 1, h, (const float*)w32, st.hn,
+                               (float*)(V + size_t(jj + 1) * vs), (long long)N);
             continue;
           }
         }
