@@ -213,7 +213,8 @@ int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len,
 /* Mixed-precision variant of lsg_matvec (no reference counterpart): the
  * operators and the accumulation inside every pass are FP32 (interleaved
  * re/im), x and y stay FP64; the FP32 copy of the operators is made on the
- * device on first use. */
+ * device on first use. nrhs >= 2 runs the FP32 block operator (64 right-hand
+ * sides per pass on FP32 MFMA). */
 int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
 /* lsg_time_matvec for the FP32 operator. */
 int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms);
@@ -294,6 +295,12 @@ const char* lsg_tuning(const lsg_ctx* ctx);
  * workgroup per group of subtrees (where the operator has such groups). For
  * tests and measurements of each kernel; lsg_autotune replaces the choice. */
 int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel);
+
+/* Right-hand-side tiles of 16 per wavefront in the FP32 block passes (no
+ * reference counterpart): 1 (four wavefronts per workgroup) or 2 (two, each
+ * operator value feeding two MFMAs). Enabling the FP32 block operator picks
+ * the faster by timing; this replaces that choice (tests, measurements). */
+int lsg_set_block_tiles(lsg_ctx* ctx, int tiles);
 
 /* on = 1: capture the FP64 matvec (both streams) into a hipGraph, replayed by
  * every later FP64 matvec of the context (one launch plus a pointer update);

@@ -111,6 +111,7 @@ SYMBOLS = [
     ("lsg_autotune", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     ("lsg_tuning", C.c_char_p, [_P]),
     ("lsg_set_pass_kernel", C.c_int, [_P, C.c_int]),
+    ("lsg_set_block_tiles", C.c_int, [_P, C.c_int]),
     ("lsg_use_graph", C.c_int, [_P, C.c_int]),
     ("lsg_gmres_ragged", C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(lsg_solve_opts),
                                    C.POINTER(lsg_stats)]),
@@ -460,6 +461,16 @@ class DFMM:
                                          Xt.shape[0], self.N))
         return Yt.T
 
+    def matvec_block_fp32(self, X):
+        """A32 X for X of shape (N, nrhs), nrhs >= 2: the FP32 block operator."""
+        Xt = np.ascontiguousarray(np.asarray(X, dtype=np.complex128).T)
+        if Xt.shape[0] < 2:
+            raise ValueError("the FP32 block operator needs at least two right-hand sides")
+        Yt = np.empty_like(Xt)
+        self._check(self._lib.lsg_matvec_fp32(self._ctx, _dptr(Xt.view(np.float64)), _dptr(Yt.view(np.float64)),
+                                              Xt.shape[0], self.N))
+        return Yt.T
+
     def rhs(self):
         out = np.empty(self.N, dtype=np.complex128)
         self._check(self._lib.lsg_rhs(self._ctx, _dptr(out.view(np.float64))))
@@ -663,6 +674,11 @@ class DFMM:
         k = {"block": 0, "wave": 1, "lane": 2}[kernel] + (3 if fused else 0)
         self._check(self._lib.lsg_set_pass_kernel(self._ctx, k))
 
+    def set_block_tiles(self, tiles):
+        """Right-hand-side tiles of 16 per wavefront in the FP32 block passes
+        (1 or 2), replacing the choice made by timing."""
+        self._check(self._lib.lsg_set_block_tiles(self._ctx, tiles))
+
     def use_graph(self, on=True):
         """Replay the FP64 matvec from a captured hipGraph (on) or launch kernel by kernel (off)."""
         self._check(self._lib.lsg_use_graph(self._ctx, 1 if on else 0))
@@ -775,6 +791,11 @@ class RaggedBatch:
         n = C.c_int()
         self._check(self._lib.lsg_ragged_autotune(self._r, reps, C.byref(n)))
         return n.value
+
+    def set_block_tiles(self, tiles):
+        """Right-hand-side tiles of 16 per wavefront in the FP32 block passes
+        (1 or 2), replacing the choice made by timing."""
+        self._check(self._lib.lsg_set_block_tiles(self._ctx, tiles))
 
     def use_graph(self, on=True):
         self._check(self._lib.lsg_ragged_use_graph(self._r, 1 if on else 0))

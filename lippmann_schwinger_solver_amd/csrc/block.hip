@@ -72,6 +72,13 @@ __device__ __forceinline__ int xcd_order(int b, int nb) {
 #endif
 constexpr int kKC = LSG_BLOCK_KC;
 
+// Right-hand-side tiles of 16 per wavefront in the block passes, NT: 1 (four
+// wavefronts per workgroup) or 2 (two wavefronts, each operator value read
+// from LDS feeds two MFMAs; chosen by timing for the FP32 operator, where it
+// pays at C3 but not at C2). Threads per workgroup:
+template <int NT>
+constexpr int block_threads() { return 64 * (kNB / 16 / NT); }
+
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 // gfx950 LDS-direct loads: lane l of the wavefront copies 4 (16) bytes from g
@@ -116,13 +123,14 @@ struct SegDesc {
 // into registers, while it runs the MFMAs of chunk i; one wait and one
 // barrier per chunk. Columns past C_k read zeros (operator: the zero row of
 // the pass state; source: row zrow).
-template <class R, int MT, int MTU>
-__device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], const Segments<R>& sg, int Rw,
+template <class R, int MT, int MTU, int NT>
+__device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT][NT], const Segments<R>& sg, int Rw,
                                               const R (&rm0)[MT], const R (&rm1)[MT], const R* __restrict__ X,
                                               const R* __restrict__ bst, int nx, int zrow, R* As0, R* As1,
                                               int* Sr, const SegDesc* sd) {
   constexpr int KS = kKC / 2;                        // MFMA k-steps per chunk
   constexpr int PB = sizeof(R) == 8 ? 16 : 4;        // bytes per operator copy (one complex FP64, or one float)
+  constexpr int kBT = block_threads<NT>();
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int i16 = lane & 15, k4 = lane >> 4, kb = k4 & 1, kc = k4 >> 1;
   const int sel = (i16 & 1) == kb ? 0 : 1;
@@ -156,10 +164,10 @@ __device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], 
     const char* src = p.A + (size_t)c0 * per_col * PB;
     const char* z = reinterpret_cast<const char*>(zero);
     const int valid = min(kKC, Ck - c0) * per_col;
-    for (int q = 0; q * kT < n_cp; ++q) {
-      const int e = q * kT + t;
+    for (int q = 0; q * kBT < n_cp; ++q) {
+      const int e = q * kBT + t;
       const char* g = e < valid ? src + (size_t)e * PB : z + (e & (kLD * int(sizeof(R)) / PB - 1)) * PB;
-      char* wb = reinterpret_cast<char*>(As) + (size_t)(q * kT + 64 * w) * PB;
+      char* wb = reinterpret_cast<char*>(As) + (size_t)(q * kBT + 64 * w) * PB;
       if (e < n_cp) {
         if (PB == 16) lds_copy16(g, wb);
         else lds_copy4(g, wb);
@@ -174,7 +182,7 @@ __device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], 
     if (lane < kKC) lds_copy4(Sk + min(c0 + lane, Ck - 1), Sb);
   };
   // this lane's source values of chunk i (indices in Sb)
-  auto load_src = [=](int i, const int* Sb, R (&bv)[KS]) __attribute__((always_inline)) {
+  auto load_src = [=](int i, const int* Sb, R (&bv)[KS][NT]) __attribute__((always_inline)) {
     const Pick p = seg_of(i);
     const int c0 = p.c0, Ck = p.C;
     int sv[KS];
@@ -183,13 +191,15 @@ __device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], 
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const int r = c0 + 2 * ks + kc < Ck ? sv[ks] : zrow;
-      bv[ks] = (r < nx ? X : bst)[(size_t)r * kLD + kb * kNB + 16 * w + i16];
+      const R* row = (r < nx ? X : bst) + (size_t)r * kLD + kb * kNB + 16 * NT * w + i16;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) bv[ks][c] = row[16 * c];
     }
   };
   // index buffers: Sc holds chunk i + 1's indices, Sf receives chunk i + 2's
   int* Sc = Sr + kKC;
   int* Sf = Sr;
-  R bcur[KS], bnext[KS];
+  R bcur[KS][NT], bnext[KS][NT];
   copy_idx(0, Sf);
   copy_idx(1, Sc);
   copy_ops(0, As0);
@@ -210,15 +220,20 @@ __device__ __forceinline__ void block_product(typename Mfma<R>::acc (&acc)[MT], 
     for (int ks = 0; ks < KS; ++ks) {
       const int cl = 2 * ks + kc;
 #pragma unroll
-      for (int mt = 0; mt < MTU; ++mt)
-        acc[mt] = Mfma<R>::op(As[cl * 2 * Rw + roff[mt]] * (seg0 ? rm0[mt] : rm1[mt]), bcur[ks], acc[mt]);
+      for (int mt = 0; mt < MTU; ++mt) {
+        const R a = As[cl * 2 * Rw + roff[mt]] * (seg0 ? rm0[mt] : rm1[mt]);
+#pragma unroll
+        for (int c = 0; c < NT; ++c) acc[mt][c] = Mfma<R>::op(a, bcur[ks][c], acc[mt][c]);
+      }
     }
     // the MFMAs stay ahead of the wait for the next chunk's loads
     __builtin_amdgcn_sched_barrier(0);
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) bcur[ks] = bnext[ks];
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int c = 0; c < NT; ++c) bcur[ks][c] = bnext[ks][c];
     int* tmp = Sc;
     Sc = Sf;
     Sf = tmp;
@@ -240,8 +255,8 @@ __device__ __forceinline__ void lane_rmul(R (&rmul)[MT], int Rw, int lane, const
 
 // block_product over the row tiles that hold rows of the destinations (a
 // workgroup-uniform choice); the LDS stage buffers are sized for MT tiles.
-template <class R, int MT>
-__device__ __forceinline__ void block_product_rows(typename Mfma<R>::acc (&acc)[MT], const Segments<R>& sg, int Rw,
+template <class R, int MT, int NT>
+__device__ __forceinline__ void block_product_rows(typename Mfma<R>::acc (&acc)[MT][NT], const Segments<R>& sg, int Rw,
                                                    const R (&rm0)[MT], const R (&rm1)[MT], const R* __restrict__ X,
                                                    const R* __restrict__ bst, int nx, int zrow) {
   __shared__ R As0[2 * 8 * MT * kKC], As1[2 * 8 * MT * kKC];
@@ -256,13 +271,13 @@ __device__ __forceinline__ void block_product_rows(typename Mfma<R>::acc (&acc)[
   __syncthreads();
   const int mtu = (2 * Rw + 15) >> 4;
   if (mtu >= MT) {
-    block_product<R, MT, MT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+    block_product<R, MT, MT, NT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
   } else if (MT > 1 && mtu == MT - 1) {
-    block_product<R, MT, (MT > 1 ? MT - 1 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+    block_product<R, MT, (MT > 1 ? MT - 1 : 1), NT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
   } else if (MT > 2 && mtu == MT - 2) {
-    block_product<R, MT, (MT > 2 ? MT - 2 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+    block_product<R, MT, (MT > 2 ? MT - 2 : 1), NT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
   } else {
-    block_product<R, MT, (MT > 3 ? MT - 3 : 1)>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
+    block_product<R, MT, (MT > 3 ? MT - 3 : 1), NT>(acc, sg, Rw, rm0, rm1, X, bst, nx, zrow, As0, As1, Sr, sd);
   }
 }
 
@@ -272,8 +287,8 @@ __device__ __forceinline__ void block_product_rows(typename Mfma<R>::acc (&acc)[
 // bsplit_reduce_kernel). R is the pass precision (operators, pass state,
 // accumulation); Y is FP64. MT row tiles of 16 reals: 2 * (rows of every
 // destination of the pass) <= 16 MT.
-template <bool ACC, class R, int MT>
-__global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* __restrict__ ops,
+template <bool ACC, class R, int MT, int NT>
+__global__ __launch_bounds__(block_threads<NT>()) void bpass_kernel(const typename Mfma<R>::c2* __restrict__ ops,
                                                    const int* __restrict__ src, R* bst, const R* __restrict__ X,
                                                    double* Y, int N, const int* __restrict__ rows,
                                                    const int* __restrict__ ncol, const int* __restrict__ dst,
@@ -295,41 +310,50 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* _
   sg.C0 = C - cbeg;
   sg.C1 = sg.C2 = 0;
   sg.n0 = sg.n01 = sg.nch = (sg.C0 + kKC - 1) / kKC;
-  Acc acc[MT];
+  Acc acc[MT][NT];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) acc[mt][c] = Acc{0, 0, 0, 0};
   R rm[MT];
   lane_rmul<R, MT>(rm, Rw, lane, nullptr);
-  block_product_rows<R, MT>(acc, sg, Rw, rm, rm, X, bst, nx, zrow);
+  block_product_rows<R, MT, NT>(acc, sg, Rw, rm, rm, X, bst, nx, zrow);
   const int i16 = lane & 15, k4 = lane >> 4;
-  const int q = 16 * w + i16;
   if (kseg > 0) {
     R* P = part + (size_t)item * (MT * 16) * kNB;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int c = 0; c < NT; ++c) {
+      const int q = 16 * (NT * w + c) + i16;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P[(size_t)((mt << 4) + Mfma<R>::row(k4, v)) * kNB + q] = acc[mt][v];
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) P[(size_t)((mt << 4) + Mfma<R>::row(k4, v)) * kNB + q] = acc[mt][c][v];
+    }
     return;
   }
   const int dd = dst[d];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
+  for (int c = 0; c < NT; ++c) {
+    const int q = 16 * (NT * w + c) + i16;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int ri = (mt << 4) + Mfma<R>::row(k4, v);
-      const int r = ri >> 1, a = ri & 1;
-      if (r < Rw) {
-        const int row = dd + r;
-        if (row >= N && row < 2 * N) {
-          double* p = Y + (size_t)(row - N) * kLD + a * kNB + q;
-          double val = double(acc[mt][v]);
-          if (ACC) val += *p;
-          *p = val;
-        } else {
-          R* p = bst + (size_t)row * kLD + a * kNB + q;
-          R val = acc[mt][v];
-          if (ACC) val += *p;
-          *p = val;
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ri = (mt << 4) + Mfma<R>::row(k4, v);
+        const int r = ri >> 1, a = ri & 1;
+        if (r < Rw) {
+          const int row = dd + r;
+          if (row >= N && row < 2 * N) {
+            double* p = Y + (size_t)(row - N) * kLD + a * kNB + q;
+            double val = double(acc[mt][c][v]);
+            if (ACC) val += *p;
+            *p = val;
+          } else {
+            R* p = bst + (size_t)row * kLD + a * kNB + q;
+            R val = acc[mt][c][v];
+            if (ACC) val += *p;
+            *p = val;
+          }
         }
       }
     }
@@ -345,8 +369,8 @@ __global__ __launch_bounds__(kT) void bpass_kernel(const typename Mfma<R>::c2* _
 // -1 for none; rank rows at N + leaf rank) accumulated on top, and the rows
 // stored once: Y = (find_phi ? X : 0) + qk2 .* (T x) + leaf passes (in FP64,
 // or rounded to FP32 for the FP32 inner solves' Krylov product).
-template <class R, class XT, class YT, int MT>
-__global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
+template <class R, class XT, class YT, int MT, int NT>
+__global__ __launch_bounds__(block_threads<NT>()) void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
                                                    const int* __restrict__ near_src, int rank,
                                                    const typename Mfma<R>::c2* __restrict__ ops,
                                                    const int* __restrict__ src, const int* __restrict__ ncol,
@@ -373,26 +397,31 @@ __global__ __launch_bounds__(kT) void bleaf_kernel(const typename Mfma<R>::c2* _
   sg.n0 = (sg.C0 + kKC - 1) / kKC;
   sg.n01 = sg.n0 + (sg.C1 + kKC - 1) / kKC;
   sg.nch = sg.n01 + (sg.C2 + kKC - 1) / kKC;
-  Acc acc[MT];
+  Acc acc[MT][NT];
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) acc[mt] = Acc{0, 0, 0, 0};
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) acc[mt][c] = Acc{0, 0, 0, 0};
   R rm0[MT], rm1[MT];
   lane_rmul<R, MT>(rm0, rank, lane, find_phi ? qk2 + (size_t)leaf * rank : nullptr);
   lane_rmul<R, MT>(rm1, rank, lane, nullptr);
-  block_product_rows<R, MT>(acc, sg, rank, rm0, rm1, X, bst, nx, zrow);
+  block_product_rows<R, MT, NT>(acc, sg, rank, rm0, rm1, X, bst, nx, zrow);
   const int i16 = lane & 15, k4 = lane >> 4;
-  const int q = 16 * w + i16;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
+  for (int c = 0; c < NT; ++c) {
+    const int q = 16 * (NT * w + c) + i16;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int ri = (mt << 4) + Mfma<R>::row(k4, v);
-      const int r = ri >> 1, a = ri & 1;
-      if (r >= rank) continue;
-      const size_t o = ((size_t)leaf * rank + r) * kLD + a * kNB + q;
-      double val = double(acc[mt][v]);
-      if (find_phi) val += double(Xin[o]);
-      Y[o] = YT(val);
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ri = (mt << 4) + Mfma<R>::row(k4, v);
+        const int r = ri >> 1, a = ri & 1;
+        if (r >= rank) continue;
+        const size_t o = ((size_t)leaf * rank + r) * kLD + a * kNB + q;
+        double val = double(acc[mt][c][v]);
+        if (find_phi) val += double(Xin[o]);
+        Y[o] = YT(val);
+      }
     }
   }
 }
@@ -1014,22 +1043,33 @@ void BlockOperator::release() {
   op = nullptr;
 }
 
-template <bool ACC, class R, int MT>
-static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename Mfma<R>::c2* ops, const int* src,
+template <bool ACC, class R, int MT, int NT>
+static void launch_bpass_nt(const BlockOperator& bo, int ndest, const typename Mfma<R>::c2* ops, const int* src,
                             R* bst, const R* X, int nx, double* Y, const int* rows, const int* ncol, const int* dst,
                             const long long* op_off, const long long* src_off, int nseg) {
   hipStream_t s = bo.op->stream;
   const int zrow = int(bo.op->n_state);  // the pass state's all-zero row
   if (nseg <= 1) {
-    hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N, rows,
-                       ncol, dst, op_off, src_off, nx, zrow, 0, 1, (R*)nullptr);
+    hipLaunchKernelGGL((bpass_kernel<ACC, R, MT, NT>), dim3(ndest), dim3(block_threads<NT>()), 0, s, ops, src, bst, X,
+                       Y, bo.op->N, rows, ncol, dst, op_off, src_off, nx, zrow, 0, 1, (R*)nullptr);
     return;
   }
   R* part = reinterpret_cast<R*>(bo.split_part);
-  hipLaunchKernelGGL((bpass_kernel<ACC, R, MT>), dim3(ndest * nseg), dim3(kT), 0, s, ops, src, bst, X, Y, bo.op->N,
-                     rows, ncol, dst, op_off, src_off, nx, zrow, kSplitCols, nseg, part);
+  hipLaunchKernelGGL((bpass_kernel<ACC, R, MT, NT>), dim3(ndest * nseg), dim3(block_threads<NT>()), 0, s, ops, src, bst,
+                     X, Y, bo.op->N, rows, ncol, dst, op_off, src_off, nx, zrow, kSplitCols, nseg, part);
   hipLaunchKernelGGL((bsplit_reduce_kernel<ACC, R>), dim3(ndest), dim3(kT), 0, s, part, nseg, MT * 16, rows, dst, bst,
                      Y, bo.op->N);
+}
+
+// right-hand-side tiles per wavefront: the operator's choice for FP32 (bo.nt32), 1 for FP64
+template <bool ACC, class R, int MT>
+static void launch_bpass_mt(const BlockOperator& bo, int ndest, const typename Mfma<R>::c2* ops, const int* src,
+                            R* bst, const R* X, int nx, double* Y, const int* rows, const int* ncol, const int* dst,
+                            const long long* op_off, const long long* src_off, int nseg) {
+  if (std::is_same<R, float>::value && bo.nt32 == 2)
+    launch_bpass_nt<ACC, R, MT, 2>(bo, ndest, ops, src, bst, X, nx, Y, rows, ncol, dst, op_off, src_off, nseg);
+  else
+    launch_bpass_nt<ACC, R, MT, 1>(bo, ndest, ops, src, bst, X, nx, Y, rows, ncol, dst, op_off, src_off, nseg);
 }
 
 // row tiles from the largest destination of the pass
@@ -1109,12 +1149,18 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
     const C2* ops = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.ops
                                                                                 : (const void*)o.ops32);
     const int mt = std::max(1, (2 * o.rank + 15) / 16);
+    const int nt = std::is_same<R, float>::value ? bo.nt32 : 1;
     const dim3 grid(o.n_leaves);
 #define LSG_LEAF_CASE(M)                                                                                        \
   case M:                                                                                                       \
-    hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, M>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src, \
-                       o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y, \
-                       int(o.n_state));                                                                         \
+    if (nt == 2)                                                                                                \
+      hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, M, 2>), grid, dim3(block_threads<2>()), 0, o.stream, T,        \
+                         bo.near_src, o.rank, ops, o.src, o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, \
+                         o.qk2, o.find_phi ? 1 : 0, Y, int(o.n_state));                                        \
+    else                                                                                                        \
+      hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, M, 1>), grid, dim3(block_threads<1>()), 0, o.stream, T,        \
+                         bo.near_src, o.rank, ops, o.src, o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, \
+                         o.qk2, o.find_phi ? 1 : 0, Y, int(o.n_state));                                        \
     break;
     switch (mt) {
       LSG_LEAF_CASE(1)
@@ -1125,9 +1171,9 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
       LSG_LEAF_CASE(6)
       LSG_LEAF_CASE(7)
       default:
-        hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, 8>), grid, dim3(kT), 0, o.stream, T, bo.near_src, o.rank, ops, o.src,
-                           o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X, o.qk2, o.find_phi ? 1 : 0, Y,
-                           int(o.n_state));
+        hipLaunchKernelGGL((bleaf_kernel<R, XT, YT, 8, 1>), grid, dim3(block_threads<1>()), 0, o.stream, T,
+                           bo.near_src, o.rank, ops, o.src, o.d_ncol, o.d_op, o.d_srcoff, bo.leaf_dest, Xr, bst, nx, X,
+                           o.qk2, o.find_phi ? 1 : 0, Y, int(o.n_state));
     }
 #undef LSG_LEAF_CASE
   };
@@ -1161,6 +1207,40 @@ std::string BlockOperator::enable_fp32() {
   if (bstate32) return "";
   HIP_OK(hipMalloc((void**)&bstate32, sizeof(float) * size_t(op->n_state + 1) * kLD));
   HIP_OK(hipMemset(bstate32, 0, sizeof(float) * size_t(op->n_state + 1) * kLD));
+  // one or two right-hand-side tiles per wavefront in the FP32 passes: the
+  // faster on this operator (a zero block in, best of three after a warm-up)
+  const size_t nb = size_t(op->N) * kLD;
+  float* x = nullptr;
+  double* y = nullptr;
+  HIP_OK(hipMalloc((void**)&x, sizeof(float) * nb));
+  if (hipMalloc((void**)&y, sizeof(double) * nb) != hipSuccess) {
+    (void)hipFree(x);
+    return "FP32 block operator: allocation failed";
+  }
+  hipStream_t s = op->stream;
+  (void)hipMemsetAsync(x, 0, sizeof(float) * nb, s);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  float best[2] = {1e30f, 1e30f};
+  for (int nt = 1; nt <= 2; ++nt) {
+    nt32 = nt;
+    matvec32(x, y);
+    for (int r = 0; r < 3; ++r) {
+      (void)hipEventRecord(a, s);
+      matvec32(x, y);
+      (void)hipEventRecord(b, s);
+      (void)hipEventSynchronize(b);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a, b);
+      best[nt - 1] = std::min(best[nt - 1], ms);
+    }
+  }
+  nt32 = best[1] < best[0] ? 2 : 1;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(x);
+  (void)hipFree(y);
   return "";
 }
 

@@ -52,6 +52,9 @@ struct BlockOperator {
   // operator: Y[:, q] = X[:, q] + qk2_cols[:, q] .* (V X)[:, q]
   double* qk2_cols = nullptr;
   float* bstate32 = nullptr;  // FP32 pass state for matvec32
+  // right-hand-side tiles of 16 per wavefront in the FP32 block passes (1 or
+  // 2), chosen by timing when the FP32 operator is enabled
+  int nt32 = 1;
   std::vector<int> pass_max_rows;  // largest destination of each pass (selects the row-tile count)
   std::vector<int> pass_nseg;      // column segments per destination (1: not split)
   void* split_part = nullptr;      // partial products of split passes

@@ -718,6 +718,28 @@ int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t 
   std::string e = ctx->dev.enable_fp32();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   const size_t nb = sizeof(double2) * size_t(N);
+  if (nrhs >= 2) {
+    // block path: 64 right-hand sides per FP32 block matvec (FP32 MFMA)
+    for (int r = 0; r < nrhs; ++r)
+      if (!all_finite(x + 2 * ld * r, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
+    e = ctx->block_ready();
+    if (e.empty()) e = ctx->blk.enable_fp32();
+    if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+    hipStream_t s = ctx->dev.stream;
+    for (int r0 = 0; r0 < nrhs; r0 += kBlockRhs) {
+      const int nr = std::min(kBlockRhs, nrhs - r0);
+      for (int r = 0; r < nr; ++r)
+        hipMemcpyAsync(ctx->bcm + size_t(r) * N, x + 2 * ld * (r0 + r), nb, hipMemcpyHostToDevice, s);
+      to_split(ctx->bcm, N, nr, int(N), kBlockRhs, ctx->bx, s);
+      ctx->blk.matvec32(static_cast<const double*>(ctx->bx), ctx->by);
+      from_split(ctx->by, int(N), kBlockRhs, nr, ctx->bcm, N, s);
+      for (int r = 0; r < nr; ++r)
+        hipMemcpyAsync(y + 2 * ld * (r0 + r), ctx->bcm + size_t(r) * N, nb, hipMemcpyDeviceToHost, s);
+      hipError_t he = hipStreamSynchronize(s);
+      if (he != hipSuccess) return ctx->set(LSG_ERR_DEVICE, std::string("block matvec: ") + hipGetErrorString(he));
+    }
+    return LSG_OK;
+  }
   for (int r = 0; r < nrhs; ++r) {
     const double* xr = x + 2 * ld * r;
     if (!all_finite(xr, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
@@ -893,6 +915,15 @@ int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed) {
 }
 
 const char* lsg_tuning(const lsg_ctx* ctx) { return ctx ? ctx->tuning.c_str() : ""; }
+
+int lsg_set_block_tiles(lsg_ctx* ctx, int tiles) {
+  if (!ctx || (tiles != 1 && tiles != 2)) return LSG_ERR_ARG;
+  std::string e = ctx->block_ready();
+  if (e.empty()) e = ctx->blk.enable_fp32();
+  if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  ctx->blk.nt32 = tiles;
+  return LSG_OK;
+}
 
 int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel) {
   if (!ctx || kernel < 0 || kernel > 5) return LSG_ERR_ARG;

@@ -128,3 +128,22 @@ def test_batched_flexible_gmres_fp32_inner(m, K):
         assert H.relerr(o.matvec(X[:, j]), B[:, j]) <= 1.05e-8
     S.close()
     o.close()
+
+
+@pytest.mark.parametrize("depth,kappa", [(4, 30.0), (5, 40.0)])
+def test_fp32_block_matvec_tiles_per_wave(depth, kappa):
+    """FP32 block passes with one and with two right-hand-side tiles per
+    wavefront: the same products to FP32 rounding, both within FP32 accuracy of
+    the FP64 block matvec, on all 64 columns."""
+    S = make(H.inputs(depth=depth, kappa=kappa, q=6, seed=2))
+    X = np.stack([H.rand_vec(S.N, 40 + j) for j in range(64)], axis=1)
+    Y64 = S.matvec_block(X)
+    ys = {}
+    for nt in (1, 2):
+        S.set_block_tiles(nt)
+        ys[nt] = S.matvec_block_fp32(X)
+        e = max(H.relerr(ys[nt][:, j], Y64[:, j]) for j in range(64))
+        print(f"tiles {nt}: FP32 vs FP64 block {e:.2e}", flush=True)
+        assert e < 1e-3
+    assert max(H.relerr(ys[2][:, j], ys[1][:, j]) for j in range(64)) < 1e-5
+    S.close()
