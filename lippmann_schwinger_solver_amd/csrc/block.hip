@@ -514,14 +514,17 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
   }
 }
 
-// The block dot for an FP32 basis and product: lane l of a wavefront covers the
-// systems 2 (l % 32) and 2 (l % 32) + 1 with one 8-byte load per value and
-// half-wave h = l / 32 takes every other row, so one load instruction moves two
-// rows' 512 bytes (the FP32 rows are 512 bytes: 64 real parts, 64 imaginary
-// parts); the two halves are summed by a cross-lane exchange, the four
-// wavefronts through LDS. Same partial layout as bdot_kernel.
-__global__ __launch_bounds__(kT) void bdot2_kernel(const float* __restrict__ V, long long vs, int nc, int ncx,
+// The block dot for a low-precision basis (TV: FP32 or FP16) and an FP32
+// product: lane l of a wavefront covers the systems 2 (l % 32) and 2 (l % 32) + 1
+// with one load of a pair per value (8 bytes for FP32, 4 for FP16) and
+// half-wave h = l / 32 takes every other row (a row holds 64 real parts, then
+// 64 imaginary parts); the two halves are summed by a cross-lane exchange, the
+// four wavefronts through LDS. Same partial layout as bdot_kernel; products
+// and sums in FP64.
+template <class TV>
+__global__ __launch_bounds__(kT) void bdot2_kernel(const TV* __restrict__ V, long long vs, int nc, int ncx,
                                                    const float* __restrict__ w, int N, double2* __restrict__ partial) {
+  typedef TV v2 __attribute__((ext_vector_type(2)));
   __shared__ double2 red[kT / 64][kBCols][kNB];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, h = lane >> 5, p = lane & 31;
   const int g0 = blockIdx.y * kBCols;
@@ -539,8 +542,10 @@ __global__ __launch_bounds__(kT) void bdot2_kernel(const float* __restrict__ V, 
       const int col = g0 + u;
       float2 vr = make_float2(0.f, 0.f), vi = make_float2(0.f, 0.f);
       if (col < nc) {
-        vr = *reinterpret_cast<const float2*>(V + col * vs + n * kLD + 2 * p);
-        vi = *reinterpret_cast<const float2*>(V + col * vs + n * kLD + kNB + 2 * p);
+        const v2 a = *reinterpret_cast<const v2*>(V + col * vs + n * kLD + 2 * p);
+        const v2 b = *reinterpret_cast<const v2*>(V + col * vs + n * kLD + kNB + 2 * p);
+        vr = make_float2(float(a.x), float(a.y));
+        vi = make_float2(float(b.x), float(b.y));
       } else if (col == nc) {
         vr = wr;
         vi = wi;
@@ -787,13 +792,17 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, l
   }
 }
 
-// The update for an FP32 basis and product: lane l of a half-wavefront covers
-// the systems 2 (l % 32) and 2 (l % 32) + 1 with 8-byte loads (as bdot2_kernel),
-// each thread kUpdRows consecutive rows of its two systems.
-__global__ __launch_bounds__(kT) void bupdate2_kernel(const float* __restrict__ V, long long vs, int nc,
+// The update for a low-precision basis (TV: FP32 or FP16) and an FP32 product:
+// lane l of a half-wavefront covers the systems 2 (l % 32) and 2 (l % 32) + 1
+// with one load of a pair per value (as bdot2_kernel), each thread kUpdRows
+// consecutive rows of its two systems; sums in FP64, the new basis vector
+// rounded to TV.
+template <class TV>
+__global__ __launch_bounds__(kT) void bupdate2_kernel(const TV* __restrict__ V, long long vs, int nc,
                                                       const double2* __restrict__ h, const float* __restrict__ w,
-                                                      const double* __restrict__ hn, float* __restrict__ vout,
+                                                      const double* __restrict__ hn, TV* __restrict__ vout,
                                                       long long n_rows) {
+  typedef TV v2 __attribute__((ext_vector_type(2)));
   const int p = threadIdx.x & 31;
   const long long n0 = ((long long)blockIdx.x * (kT / 32) + (threadIdx.x >> 5)) * kUpdRows;
   if (n0 >= n_rows) return;
@@ -805,12 +814,12 @@ __global__ __launch_bounds__(kT) void bupdate2_kernel(const float* __restrict__ 
 #pragma unroll 2
   for (int i = 0; i < nc; ++i) {
     const double2 h0 = h[i * kNB + 2 * p], h1 = h[i * kNB + 2 * p + 1];
-    const float* Vi = V + i * vs + base;
+    const TV* Vi = V + i * vs + base;
 #pragma unroll
     for (int r = 0; r < kUpdRows; ++r) {
       if (r < nr) {
-        const float2 vr = *reinterpret_cast<const float2*>(Vi + r * kLD);
-        const float2 vi = *reinterpret_cast<const float2*>(Vi + r * kLD + kNB);
+        const v2 vr = *reinterpret_cast<const v2*>(Vi + r * kLD);
+        const v2 vi = *reinterpret_cast<const v2*>(Vi + r * kLD + kNB);
         ar[r][0] = fma(double(vr.x), h0.x, fma(-double(vi.x), h0.y, ar[r][0]));
         ai[r][0] = fma(double(vr.x), h0.y, fma(double(vi.x), h0.x, ai[r][0]));
         ar[r][1] = fma(double(vr.y), h1.x, fma(-double(vi.y), h1.y, ar[r][1]));
@@ -825,10 +834,9 @@ __global__ __launch_bounds__(kT) void bupdate2_kernel(const float* __restrict__ 
     if (r < nr) {
       const long long o = base + r * kLD;
       const float2 wr = *reinterpret_cast<const float2*>(w + o), wi = *reinterpret_cast<const float2*>(w + o + kNB);
-      *reinterpret_cast<float2*>(vout + o) =
-          make_float2(float((double(wr.x) - ar[r][0]) * inv0), float((double(wr.y) - ar[r][1]) * inv1));
-      *reinterpret_cast<float2*>(vout + o + kNB) =
-          make_float2(float((double(wi.x) - ai[r][0]) * inv0), float((double(wi.y) - ai[r][1]) * inv1));
+      *reinterpret_cast<v2*>(vout + o) = v2{TV((double(wr.x) - ar[r][0]) * inv0), TV((double(wr.y) - ar[r][1]) * inv1)};
+      *reinterpret_cast<v2*>(vout + o + kNB) =
+          v2{TV((double(wi.x) - ai[r][0]) * inv0), TV((double(wi.y) - ai[r][1]) * inv1)};
     }
   }
 }
@@ -1114,15 +1122,16 @@ static void launch_bpass(BlockOperator& bo, int pidx, const R* X, int nx, double
                               o.d_op + b, o.d_srcoff + b, nseg);
 }
 
-// FP32 copy of the input block into the first N rows of the FP32 pass state
-__global__ void bround_kernel(const double* __restrict__ X, float* __restrict__ bst32, long long n_total) {
+// FP32 copy of the input block (FP64 or FP16) into the first N rows of the FP32 pass state
+template <class XT>
+__global__ void bround_kernel(const XT* __restrict__ X, float* __restrict__ bst32, long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_total) bst32[i] = float(X[i]);
 }
 
-// R: pass precision; XT: type of the input block. An FP64 input to the FP32
-// passes is rounded into the pass state's x rows first; an input already in
-// the pass precision is read in place.
+// R: pass precision; XT: type of the input block. An FP64 or FP16 input to
+// the FP32 passes is converted into the pass state's x rows first; an input
+// already in the pass precision is read in place.
 template <class R, class XT, class YT = double>
 static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
   // an FP32 product is stored by the leaf kernel alone (can_store_f32): the
@@ -1139,8 +1148,8 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
     Xr = reinterpret_cast<const R*>(X);
     nx = o.N;
   } else {
-    hipLaunchKernelGGL(bround_kernel, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream,
-                       reinterpret_cast<const double*>(X), bo.bstate32, (long long)o.N * kLD);
+    hipLaunchKernelGGL(bround_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
+                       bo.bstate32, (long long)o.N * kLD);
     Xr = bst;
     nx = 0;
   }
@@ -1200,6 +1209,8 @@ void BlockOperator::matvec(const double* X, double* Y) { block_matvec_impl<doubl
 void BlockOperator::matvec32(const double* X, double* Y) { block_matvec_impl<float, double>(*this, X, Y); }
 void BlockOperator::matvec32(const float* X, double* Y) { block_matvec_impl<float, float>(*this, X, Y); }
 void BlockOperator::matvec32(const float* X, float* Y) { block_matvec_impl<float, float, float>(*this, X, Y); }
+void BlockOperator::matvec32(const _Float16* X, double* Y) { block_matvec_impl<float, _Float16>(*this, X, Y); }
+void BlockOperator::matvec32(const _Float16* X, float* Y) { block_matvec_impl<float, _Float16, float>(*this, X, Y); }
 
 std::string BlockOperator::enable_fp32() {
   std::string e = op->enable_fp32();
@@ -1275,14 +1286,14 @@ std::string BlockWork::flex() {
 
 std::string BlockWork::basis(bool f32) {
   const size_t n = size_t(N) * kLD * size_t(m + 1);
-  if (f32 && !V32) HIP_OK(hipMalloc((void**)&V32, sizeof(float) * n));
+  if (f32 && !V16) HIP_OK(hipMalloc((void**)&V16, sizeof(_Float16) * n));
   if (f32 && !w32) HIP_OK(hipMalloc((void**)&w32, sizeof(float) * size_t(N) * kLD));
   if (!f32 && !V) HIP_OK(hipMalloc((void**)&V, sizeof(double) * n));
   return "";
 }
 
 void BlockWork::release() {
-  void* ptrs[] = {V, V32, w32, Z, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  void* ptrs[] = {V, V16, w32, Z, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (host_rel) (void)hipHostFree(host_rel);
@@ -1291,13 +1302,17 @@ void BlockWork::release() {
   *this = BlockWork{};
 }
 
-// TV: storage of the Krylov basis (double; float for the FP32 inner solves,
-// whose accuracy target is far above FP32 rounding). Dots, norms, Givens and
-// the solution update accumulate in FP64 either way.
+// TV: storage of the Krylov basis (double; _Float16 for the FP32 inner solves,
+// whose accuracy target is far above the basis rounding: the basis vectors have
+// unit norm per system, so their entries sit in FP16's normal range for any N
+// this library builds, and FP16 storage leaves the C3 inner and outer step
+// counts unchanged against FP32 storage). Dots, norms, Givens and the solution
+// update accumulate in FP64 either way.
 template <class TV>
 static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts,
                                  double tol, double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero,
                                  bool x_zero) {
+  constexpr bool lowv = !std::is_same<TV, double>::value;   // the FP16 basis of the FP32 inner solves
   BatchResult res;
   auto t0 = std::chrono::steady_clock::now();
   DeviceOperator& o = *bo.op;
@@ -1323,7 +1338,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     }
   }
   {
-    std::string e = W.basis(std::is_same<TV, float>::value);
+    std::string e = W.basis(lowv);
     if (!e.empty()) {
       W.release();
       res.status = LSG_ERR_DEVICE;
@@ -1331,11 +1346,11 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
       return res;
     }
   }
-  TV* V = reinterpret_cast<TV*>(std::is_same<TV, float>::value ? (void*)W.V32 : (void*)W.V);
+  TV* V = reinterpret_cast<TV*>(lowv ? (void*)W.V16 : (void*)W.V);
   double* w = W.w;
   // the FP32 inner solves keep the Arnoldi product A v_j in FP32 too (the
   // FP32 operator computed it in FP32) where the leaf kernel stores it
-  float* w32 = std::is_same<TV, float>::value && fp32 && bo.can_store_f32() ? W.w32 : nullptr;
+  float* w32 = lowv && fp32 && bo.can_store_f32() ? W.w32 : nullptr;
   double2 *partial = W.partial, *h = W.h, *y = W.y;
   BGivens st{W.H, W.cs, W.sn, W.g, W.hn, W.bnorm, W.active, W.kq, W.tolq};
   double *host_rel = W.host_rel, *host_est = W.host_est;
@@ -1344,8 +1359,8 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   auto dot = [&](int nc, const auto* vec, bool norm) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(vec)>> TW;
     const int ncx = nc + (norm ? 1 : 0);
-    if constexpr (std::is_same<TV, float>::value && std::is_same<TW, float>::value)
-      hipLaunchKernelGGL(bdot2_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
+    if constexpr (lowv && std::is_same<TW, float>::value)
+      hipLaunchKernelGGL(bdot2_kernel<TV>, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
                          N, partial);
     else
       hipLaunchKernelGGL((bdot_kernel<TV, TW>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc,
@@ -1388,7 +1403,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   }
   auto amv = [&](const auto* xin, double* yout) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(xin)>> XT;
-    if constexpr (std::is_same<XT, float>::value) {
+    if constexpr (!std::is_same<XT, double>::value) {
       bo.matvec32(xin, yout);
     } else {
       if (fp32) bo.matvec32(xin, yout);
@@ -1439,15 +1454,15 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
       for (int jj = j; jj < j + batch; ++jj) {
-        if constexpr (std::is_same<TV, float>::value) {
+        if constexpr (lowv) {
           if (w32) {
-            bo.matvec32((const float*)(V + size_t(jj) * vs), w32);
+            bo.matvec32((const TV*)(V + size_t(jj) * vs), w32);
             dot(jj + 1, (const float*)w32, true);
             hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active,
                                (const double*)nullptr);
-            hipLaunchKernelGGL(bupdate2_kernel, dim3(blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 32)), dim3(kT),
-                               0, s, (const float*)V, vs, jj + 1, h, (const float*)w32, st.hn,
-                               (float*)(V + size_t(jj + 1) * vs), (long long)N);
+            hipLaunchKernelGGL(bupdate2_kernel<TV>, dim3(blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 32)),
+                               dim3(kT), 0, s, (const TV*)V, vs, jj + 1, h, (const float*)w32, st.hn,
+                               V + size_t(jj + 1) * vs, (long long)N);
             continue;
           }
         }
@@ -1487,11 +1502,11 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
 
 BatchResult block_gmres(BlockOperator& bo, const double* B, double* X, int nrhs, int m, int max_restarts, double tol,
                         double* rel_residuals, bool fp32, const double* tol_per, bool allow_zero, bool x_zero) {
-  // the FP32 inner solves keep their Krylov basis in FP32 (half the
-  // Gram-Schmidt traffic); the FP64 solver keeps it in FP64
+  // the FP32 inner solves keep their Krylov basis in FP16 (a quarter of the
+  // FP64 Gram-Schmidt traffic); the FP64 solver keeps it in FP64
   if (fp32)
-    return block_gmres_t<float>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, true, tol_per, allow_zero,
-                                x_zero);
+    return block_gmres_t<_Float16>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, true, tol_per, allow_zero,
+                                   x_zero);
   return block_gmres_t<double>(bo, B, X, nrhs, m, max_restarts, tol, rel_residuals, false, tol_per, allow_zero,
                                x_zero);
 }

@@ -25,7 +25,7 @@ constexpr int kBlockRhs = 64;  // right-hand sides per block (4 MFMA column tile
 struct BlockWork {
   int N = 0, m = -1;
   double *V = nullptr, *w = nullptr;
-  float* V32 = nullptr;   // FP32 basis of the FP32 inner solves
+  _Float16* V16 = nullptr;  // FP16 basis of the FP32 inner solves
   float* w32 = nullptr;   // FP32 product A v_j of the FP32 inner solves (when the leaf kernel writes every output row)
   double* Z = nullptr;    // flexible GMRES: the m preconditioned directions (FP64), allocated by flex()
   double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
@@ -34,7 +34,7 @@ struct BlockWork {
   double *host_rel = nullptr, *host_est = nullptr;  // pinned
   int* host_active = nullptr;                        // pinned
   std::string reserve(int N, int m);   // everything but the basis
-  std::string basis(bool f32);         // the FP64 or FP32 basis, on first use
+  std::string basis(bool f16);         // the FP64 or FP16 basis, on first use
   std::string flex();                  // Z, on first use
   void release();
 };
@@ -72,6 +72,9 @@ struct BlockOperator {
   // same with the product stored in FP32; only where the leaf kernel writes
   // every output row (can_store_f32)
   void matvec32(const float* X, float* Y);
+  // FP16 input (the inner solves' basis), converted into the FP32 pass state
+  void matvec32(const _Float16* X, double* Y);
+  void matvec32(const _Float16* X, float* Y);
   bool can_store_f32() const { return op && leaf_fused && !qk2_cols; }
 };
 
