@@ -142,6 +142,7 @@ class DeviceBlock:
 
 MFMA_F64_PEAK_TFLOPS = 78.6
 MFMA_F32_PEAK_TFLOPS = 157.3
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA (no sparsity)
 
 
 def batched_leg(S, info, args, st_plain, plain_sps, restart=50, restart32=30):
@@ -270,9 +271,10 @@ def seeded_angle(seed):
 
 
 C3_WORKLOAD = ("C3, 64 angles: uniform depth 7 (16384 leaves x 36 Chebyshev nodes = 589824 unknowns), kappa 160, "
-               "8 seeded Gaussian bumps, NCA tol 1e-10; 64 plane waves at 2 pi j/64 solved by FP32-inner batched "
-               "GMRES(24) inside an FP64 flexible-GMRES(30) refinement to FP64-verified relative residual {tol:g}, "
-               "against plain FP64 GMRES(50) on the same GPU")
+               "8 seeded Gaussian bumps, NCA tol 1e-10; 64 plane waves at 2 pi j/64 solved by batched GMRES(24) "
+               "inner solves (FP32 pass state, split-BF16 operators on BF16 MFMA, FP16 basis) inside an FP64 "
+               "flexible-GMRES(30) outer iteration to FP64-verified relative residual {tol:g}, against plain FP64 "
+               "GMRES(50) on the same GPU")
 
 
 def _relres(S, x, b):
@@ -369,6 +371,15 @@ def bench_c3(args):
     S._check(S._lib.lsg_sync(S._ctx))
     t_ir = time.perf_counter() - t0
 
+    # the line's solver with FP32 MFMA operators in the inner solves, one timed batch
+    S.set_inner_arith("fp32")
+    S._check(S._lib.lsg_sync(S._ctx))
+    t0 = time.perf_counter()
+    st_f32, rel_f32 = batch("fp32fgmres")
+    S._check(S._lib.lsg_sync(S._ctx))
+    t_f32 = time.perf_counter() - t0
+    S.set_inner_arith("bf16x3")
+
     # the same batch with the FP64 block operator (no refinement), one timed batch
     batch("fp64")
     S._check(S._lib.lsg_sync(S._ctx))
@@ -392,14 +403,18 @@ def bench_c3(args):
         log(f"  pass {p[0]:10s} {p[1] * 1e3:9.1f} us  {p[2] / 1e6:9.2f} MB  {p[2] / (p[1] * 1e-3) / 1e9:8.1f} GB/s")
     mv32 = S.time_matvec_fp32(reps=args.profile_reps)
     blk64 = S.time_matvec_block(reps=5)
+    S.set_inner_arith("fp32")
     blk32 = S.time_matvec_block(reps=5, fp32=True)
+    S.set_inner_arith("bf16x3")
+    blkx3 = S.time_matvec_block(reps=5, fp32=True)
     flops = 8.0 * 64 * (info["E_far"] + info["E_near"])
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "c64 inner (FP32 operators, FP32 MFMA) + c128 refinement",
+           "scaling": "weak", "vs_baseline": None, "dtype": "c64 inner (FP32 pass state, split-BF16 operators on BF16 MFMA, FP16 basis) + c128 outer",
            "data": "synthetic",
            "config": {"workload": C3_WORKLOAD.format(tol=args.tol), "N": N, "systems_per_step": 64,
                       "restart_len": args.restart, "outer_krylov_dim": args.outer, "inner_tol": 1e-5,
+                      "inner_arith": "bf16x3 (a = ah + al, products ah bh + ah bl + al bh, FP32 sums)",
                       "inner_arnoldi_steps_per_batch": st32["iterations"], "outer_steps": st32["refinements"],
                       "max_rel_residual": float(rel32.max()), "recomputed_max_rel_residual": max(recomputed),
                       "inputs": "B, X resident in HBM (device pointers)"},
@@ -411,6 +426,9 @@ def bench_c3(args):
                                            "inner_arnoldi_steps": st_ir["iterations"],
                                            "refinements": st_ir["refinements"],
                                            "max_rel_residual": float(rel_ir.max())},
+           "fp32_mfma_inner": {"solves_per_s": 64 / t_f32, "speedup_vs_plain_fp64": 64 / t_f32 / plain_sps,
+                               "inner_arnoldi_steps": st_f32["iterations"], "outer_steps": st_f32["refinements"],
+                               "max_rel_residual": float(rel_f32.max())},
            "batched_fp64": {"solves_per_s": 64 / t64, "speedup_vs_plain_fp64": 64 / t64 / plain_sps,
                             "arnoldi_steps": st64["iterations"], "max_rel_residual": float(rel64.max()),
                             "max_rel_diff_fp32_inner_vs_fp64": max(float(np.linalg.norm(X32[q] - X64[q]) /
@@ -425,7 +443,10 @@ def bench_c3(args):
                             "fp64_tflops": flops / (blk64 * 1e-3) / 1e12, "fp64_mfma_peak": MFMA_F64_PEAK_TFLOPS,
                             "fp64_frac": flops / (blk64 * 1e-3) / 1e12 / MFMA_F64_PEAK_TFLOPS,
                             "fp32_tflops": flops / (blk32 * 1e-3) / 1e12, "fp32_mfma_peak": MFMA_F32_PEAK_TFLOPS,
-                            "fp32_frac": flops / (blk32 * 1e-3) / 1e12 / MFMA_F32_PEAK_TFLOPS},
+                            "fp32_frac": flops / (blk32 * 1e-3) / 1e12 / MFMA_F32_PEAK_TFLOPS,
+                            "bf16x3_ms": blkx3, "bf16x3_tflops": flops / (blkx3 * 1e-3) / 1e12,
+                            "bf16x3_mfma_peak": MFMA_BF16_PEAK_TFLOPS / 3,
+                            "bf16x3_frac": 3 * flops / (blkx3 * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS},
            "cpu_baseline": None}
     S.close()
     if not args.no_cpu_baseline:

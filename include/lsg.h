@@ -296,11 +296,24 @@ const char* lsg_tuning(const lsg_ctx* ctx);
  * tests and measurements of each kernel; lsg_autotune replaces the choice. */
 int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel);
 
-/* Right-hand-side tiles of 16 per wavefront in the FP32 block passes (no
- * reference counterpart): 1 (four wavefronts per workgroup) or 2 (two, each
- * operator value feeding two MFMAs). Enabling the FP32 block operator picks
- * the faster by timing; this replaces that choice (tests, measurements). */
+/* Right-hand-side tiles of 16 per wavefront in the block passes of the inner
+ * solves' arithmetic (lsg_set_inner_arith; no reference counterpart): 1 (four
+ * wavefronts per workgroup) or 2 (two, each operator value feeding two
+ * MFMAs). Enabling that block operator picks the faster by timing; this
+ * replaces that choice (tests, measurements). */
 int lsg_set_block_tiles(lsg_ctx* ctx, int tiles);
+
+/* Arithmetic of the FP32-inner batched solves (LSG_SOLVE_FP32_IR and
+ * LSG_SOLVE_FP32_FGMRES in lsg_gmres_batch), of lsg_matvec_fp32 with nrhs >= 2
+ * and of lsg_time_matvec_block_fp32 (no reference counterpart). Both keep the
+ * pass state in FP32 and the inner Krylov basis in FP16:
+ *   LSG_ARITH_BF16X3 (default): operators and source values split as
+ *     a = ah + al (bf16 pair), products ah bh + ah bl + al bh on BF16 MFMA
+ *     with FP32 sums, 16 significant bits;
+ *   LSG_ARITH_FP32: FP32 operators on FP32 MFMA, 24 significant bits. */
+#define LSG_ARITH_FP32 0
+#define LSG_ARITH_BF16X3 1
+int lsg_set_inner_arith(lsg_ctx* ctx, int arith);
 
 /* on = 1: capture the FP64 matvec (both streams) into a hipGraph, replayed by
  * every later FP64 matvec of the context (one launch plus a pointer update);

@@ -72,6 +72,8 @@ class lsg_stats(C.Structure):
 LSG_SOLVE_FP64 = 0
 LSG_SOLVE_FP32_IR = 1
 LSG_SOLVE_FP32_FGMRES = 2
+LSG_ARITH_FP32 = 0
+LSG_ARITH_BF16X3 = 1
 
 
 class lsg_solve_opts(C.Structure):
@@ -112,6 +114,7 @@ SYMBOLS = [
     ("lsg_tuning", C.c_char_p, [_P]),
     ("lsg_set_pass_kernel", C.c_int, [_P, C.c_int]),
     ("lsg_set_block_tiles", C.c_int, [_P, C.c_int]),
+    ("lsg_set_inner_arith", C.c_int, [_P, C.c_int]),
     ("lsg_use_graph", C.c_int, [_P, C.c_int]),
     ("lsg_gmres_ragged", C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(_P), C.POINTER(_P), C.POINTER(lsg_solve_opts),
                                    C.POINTER(lsg_stats)]),
@@ -675,9 +678,18 @@ class DFMM:
         self._check(self._lib.lsg_set_pass_kernel(self._ctx, k))
 
     def set_block_tiles(self, tiles):
-        """Right-hand-side tiles of 16 per wavefront in the FP32 block passes
-        (1 or 2), replacing the choice made by timing."""
+        """Right-hand-side tiles of 16 per wavefront in the block passes of the
+        inner solves' arithmetic (1 or 2), replacing the choice made by timing."""
         self._check(self._lib.lsg_set_block_tiles(self._ctx, tiles))
+
+    def set_inner_arith(self, arith):
+        """Arithmetic of the FP32-inner batched solves and of matvec_block_fp32:
+        "bf16x3" (default; operators and sources split into bf16 pairs, BF16
+        MFMA, FP32 sums, 16 significant bits) or "fp32" (FP32 MFMA)."""
+        modes = {"fp32": LSG_ARITH_FP32, "bf16x3": LSG_ARITH_BF16X3}
+        if arith not in modes:
+            raise ValueError(f"inner arithmetic must be one of {sorted(modes)}")
+        self._check(self._lib.lsg_set_inner_arith(self._ctx, modes[arith]))
 
     def use_graph(self, on=True):
         """Replay the FP64 matvec from a captured hipGraph (on) or launch kernel by kernel (off)."""
@@ -791,11 +803,6 @@ class RaggedBatch:
         n = C.c_int()
         self._check(self._lib.lsg_ragged_autotune(self._r, reps, C.byref(n)))
         return n.value
-
-    def set_block_tiles(self, tiles):
-        """Right-hand-side tiles of 16 per wavefront in the FP32 block passes
-        (1 or 2), replacing the choice made by timing."""
-        self._check(self._lib.lsg_set_block_tiles(self._ctx, tiles))
 
     def use_graph(self, on=True):
         self._check(self._lib.lsg_ragged_use_graph(self._r, 1 if on else 0))

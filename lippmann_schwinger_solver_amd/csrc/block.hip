@@ -426,6 +426,369 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_kernel(const typena
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 block product (the inner solves' operator arithmetic): every FP32
+// value a is carried as a = ah + al, ah = bf16(a), al = bf16(a - ah) (16
+// significant bits), and a product as ah bh + ah bl + al bh on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (relative error about 2^-16;
+// the dropped al bl is below that). One MFMA takes K = 32 real columns, 16
+// complex ones: a chunk of the operator.
+//
+// Real-ification with separate real and imaginary row tiles: for 16 complex
+// destination rows (row tile p) and complex columns c,
+//   Re y = sum_c [Re A, Im A] . [Re x, -Im x],  Im y = sum_c [Re A, Im A] . [Im x, Re x],
+// so both tiles take the operator as stored (a word = bf16 Re in the low half,
+// bf16 Im in the high half) and differ only in the source fragment. Lane l
+// feeds A(row 16 p + l % 16, columns 4 (l / 16) .. + 3 of the chunk: eight bf16
+// = four words) and B(the same four columns, right-hand side l % 16 of its
+// tile); the result lane l holds rows 16 p + 4 (l / 16) + v, v < 4.
+//
+// Operator layout (built by to_x3_kernel): per destination, chunks of 16
+// columns; a chunk is Rw rows of 128 bytes, the 16 columns' high words in
+// bytes 0..63 and their low words in 64..127, columns past C zero. In LDS the
+// eight 16-byte slots of row r are stored XOR-swizzled by r & 7, so the 16
+// rows a wavefront reads per slot fall in distinct banks.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// two FP32 values rounded to bf16 (nearest even), packed: a in the low half
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f2{a, b}, b2));
+}
+// high and low words of the pair (a, b)
+__device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsigned& lo) {
+  hi = pk_bf16(a, b);
+  lo = pk_bf16(a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u));
+}
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__global__ void to_x3_kernel(const double2* __restrict__ ops, const long long* __restrict__ op_off,
+                             const int* __restrict__ rows, const int* __restrict__ ncol,
+                             const long long* __restrict__ op3_off, unsigned* __restrict__ out) {
+  const int d = blockIdx.x;
+  const int Rw = rows[d], C = ncol[d], nk = (C + 15) / 16;
+  const double2* A = ops + op_off[d];
+  unsigned* O = out + op3_off[d] * 32;
+  for (int e = threadIdx.x; e < nk * Rw * 16; e += blockDim.x) {
+    const int c16 = e & 15, rr = e >> 4;  // rr = chunk * Rw + row
+    const int k = rr / Rw, r = rr - k * Rw, c = 16 * k + c16;
+    const double2 a = c < C ? A[(size_t)c * Rw + r] : make_double2(0.0, 0.0);
+    unsigned hi, lo;
+    split_pair(float(a.x), float(a.y), hi, lo);
+    O[(size_t)rr * 32 + c16] = hi;
+    O[(size_t)rr * 32 + 16 + c16] = lo;
+  }
+}
+// the near-field tables as one block (rank rows, 9 rank columns)
+__global__ void to_x3_table_kernel(const double2* __restrict__ T, int Rw, int C, unsigned* __restrict__ out) {
+  const int nk = (C + 15) / 16;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)nk * Rw * 16;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int c16 = int(e & 15);
+    const long long rr = e >> 4;
+    const int k = int(rr / Rw), r = int(rr - (long long)k * Rw), c = 16 * k + c16;
+    const double2 a = c < C ? T[(size_t)c * Rw + r] : make_double2(0.0, 0.0);
+    unsigned hi, lo;
+    split_pair(float(a.x), float(a.y), hi, lo);
+    out[rr * 32 + c16] = hi;
+    out[rr * 32 + 16 + c16] = lo;
+  }
+}
+
+// Operand stream of one workgroup in the split-BF16 layout: up to three
+// segments (operator block A_k, column indices S_k, C_k columns).
+struct SegX3 {
+  const char *A0, *A1, *A2;
+  const int *S0, *S1, *S2;
+  int C0, C1, C2;
+  int n0, n01, nch;  // chunk counts: segment 0, segments 0-1, all
+};
+
+// Block product of one workgroup on 64 right-hand sides in split BF16:
+// are/aim[p][c] (wave w: right-hand sides 16 (NT w + c) .. + 15) += the real
+// and imaginary rows of row tile p over all segments. With scale0, the rows are
+// multiplied by scale0[row] once segment 0 is done (the leaf kernel's qk2 on
+// the near field). Pipeline as in block_product: chunk i + 1's operator rows
+// into LDS by LDS-direct loads (double buffered), chunk i + 2's indices into a
+// second pair of buffers, chunk i + 1's source values into registers, while the
+// MFMAs of chunk i run; one wait and one barrier per chunk.
+template <int MTP, int NT>
+__device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
+                                                 int Rw, const float* __restrict__ X, const float* __restrict__ bst,
+                                                 int nx, const float (&scale0)[MTP][4], bool scaled, char* As0,
+                                                 char* As1, int* Sr, const SegDesc* sd) {
+  constexpr int kBT = block_threads<NT>();
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i16 = lane & 15, g = lane >> 4;
+  const int n_cp = 8 * Rw;  // 16-byte slots per chunk
+  struct Pick {
+    const char* A;
+    const int* S;
+    int C, c0;
+  };
+  auto seg_of = [=](int i) __attribute__((always_inline)) {
+    const SegDesc& d = sd[(i >= sg.n0 ? 1 : 0) + (i >= sg.n01 ? 1 : 0)];
+    Pick p;
+    p.A = reinterpret_cast<const char*>(d.A);
+    p.S = d.S;
+    p.C = d.C;
+    p.c0 = (i - d.first) * 16;
+    return p;
+  };
+  auto copy_ops = [=](int i, char* As) __attribute__((always_inline)) {
+    const Pick p = seg_of(i);
+    const char* src = p.A + (size_t)(p.c0 >> 4) * Rw * 128;
+    for (int q = 0; q * kBT < n_cp; ++q) {
+      const int e = q * kBT + t;
+      const int r = e >> 3, sl = (e & 7) ^ (r & 7);
+      char* wb = As + (size_t)(q * kBT + 64 * w) * 16;
+      if (e < n_cp) lds_copy16(src + (size_t)(8 * r + sl) * 16, wb);
+    }
+  };
+  auto copy_idx = [=](int i, int* Sb) __attribute__((always_inline)) {
+    if (w != 0 || i >= sg.nch) return;
+    const Pick p = seg_of(i);
+    if (lane < 16) lds_copy4(p.S + min(p.c0 + lane, p.C - 1), Sb);
+  };
+  // this lane's source values of chunk i: columns 4 g .. 4 g + 3, right-hand
+  // sides of its NT tiles (real and imaginary parts)
+  auto load_src = [=](int i, const int* Sb, float (&xv)[NT][4][2]) __attribute__((always_inline)) {
+    const int4 sv = *reinterpret_cast<const int4*>(Sb + 4 * g);
+    const int s4[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float* row = (s4[j] < nx ? X : bst) + (size_t)s4[j] * kLD + 16 * NT * w + i16;
+#pragma unroll
+      for (int c = 0; c < NT; ++c) {
+        xv[c][j][0] = row[16 * c];
+        xv[c][j][1] = row[kNB + 16 * c];
+      }
+    }
+  };
+  int* Sc = Sr + 16;
+  int* Sf = Sr;
+  float xcur[NT][4][2], xnext[NT][4][2];
+  copy_idx(0, Sf);
+  copy_idx(1, Sc);
+  copy_ops(0, As0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __syncthreads();
+  load_src(0, Sf, xcur);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  for (int i = 0; i < sg.nch; ++i) {
+    const char* As = (i & 1) ? As1 : As0;
+    if (i + 1 < sg.nch) copy_ops(i + 1, (i & 1) ? As0 : As1);
+    copy_idx(i + 2, Sf);
+    if (i + 1 < sg.nch) load_src(i + 1, Sc, xnext);
+    // source fragments: [Re x, -Im x] for the real rows, [Im x, Re x] for the imaginary rows
+    u32x4 b1h[NT], b1l[NT], b2h[NT], b2l[NT];
+#pragma unroll
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        unsigned hi, lo;
+        split_pair(xcur[c][j][0], -xcur[c][j][1], hi, lo);
+        b1h[c][j] = hi;
+        b1l[c][j] = lo;
+        // [Im x, Re x] = the halves of [Re x, -Im x] swapped, the new low half negated
+        b2h[c][j] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
+        b2l[c][j] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+      }
+#pragma unroll
+    for (int p = 0; p < MTP; ++p) {
+      const int r = min(16 * p + i16, Rw - 1);  // rows past Rw compute unused results
+      const char* rowp = As + r * 128;
+      const int sw = r & 7;
+      const bf16x8 ah = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + ((g ^ sw) << 4)));
+      const bf16x8 al = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + (((4 + g) ^ sw) << 4)));
+#pragma unroll
+      for (int c = 0; c < NT; ++c) {
+        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
+        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
+        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1l[c]), are[p][c], 0, 0, 0);
+        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2l[c]), aim[p][c], 0, 0, 0);
+        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
+        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
+      }
+    }
+    if (scaled && i == sg.n0 - 1) {
+#pragma unroll
+      for (int p = 0; p < MTP; ++p)
+#pragma unroll
+        for (int c = 0; c < NT; ++c)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            are[p][c][v] *= scale0[p][v];
+            aim[p][c][v] *= scale0[p][v];
+          }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NT; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xcur[c][j][0] = xnext[c][j][0];
+        xcur[c][j][1] = xnext[c][j][1];
+      }
+    int* tmp = Sc;
+    Sc = Sf;
+    Sf = tmp;
+  }
+}
+
+// LDS of one split-BF16 workgroup: two chunk buffers of up to 64 rows, the
+// index buffers and the segment descriptors
+template <int MTP, int NT>
+__device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
+                                                     int Rw, const float* __restrict__ X,
+                                                     const float* __restrict__ bst, int nx,
+                                                     const float (&scale0)[MTP][4], bool scaled) {
+  __shared__ __attribute__((aligned(16))) char As0[16 * MTP * 128], As1[16 * MTP * 128];
+  __shared__ __attribute__((aligned(16))) int Sr[32];
+  __shared__ SegDesc sd[3];
+  if (sg.nch <= 0) return;
+  if (threadIdx.x == 0) {
+    sd[0] = SegDesc{sg.A0, sg.S0, sg.C0, 0};
+    sd[1] = SegDesc{sg.A1, sg.S1, sg.C1, sg.n0};
+    sd[2] = SegDesc{sg.A2, sg.S2, sg.C2, sg.n01};
+  }
+  __syncthreads();
+  block_product_x3<MTP, NT>(are, aim, sg, Rw, X, bst, nx, scale0, scaled, As0, As1, Sr, sd);
+}
+
+// One pass on 64 right-hand sides in split BF16 (as bpass_kernel<ACC, float,
+// ...>): one workgroup per destination or per column segment of one; MTP row
+// tiles of 16 complex rows (rows of every destination <= 16 MTP); op3_off in
+// 128-byte rows of the split-BF16 operator.
+template <bool ACC, int MTP, int NT>
+__global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
+    const char* __restrict__ ops3, const int* __restrict__ src, float* bst, const float* __restrict__ X, double* Y,
+    int N, const int* __restrict__ rows, const int* __restrict__ ncol, const int* __restrict__ dst,
+    const long long* __restrict__ op3_off, const long long* __restrict__ src_off, int nx, int kseg, int nseg,
+    int rt, float* __restrict__ part) {
+  const int item = xcd_order(blockIdx.x, gridDim.x);
+  const int d = kseg > 0 ? item / nseg : item;
+  const int sgi = kseg > 0 ? item % nseg : 0;
+  const int Rw = rows[d];
+  const int Cfull = ncol[d];
+  const int cbeg = kseg > 0 ? min(Cfull, sgi * kseg) : 0;
+  const int C = kseg > 0 ? min(Cfull, cbeg + kseg) : Cfull;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  SegX3 sg;
+  sg.A0 = sg.A1 = sg.A2 = ops3 + (op3_off[d] + (long long)(cbeg >> 4) * Rw) * 128;
+  sg.S0 = sg.S1 = sg.S2 = src + src_off[d] + cbeg;
+  sg.C0 = C - cbeg;
+  sg.C1 = sg.C2 = 0;
+  sg.n0 = sg.n01 = sg.nch = (sg.C0 + 15) / 16;
+  f32x4 are[MTP][NT], aim[MTP][NT];
+#pragma unroll
+  for (int p = 0; p < MTP; ++p)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) are[p][c] = aim[p][c] = f32x4{0, 0, 0, 0};
+  const float none[MTP][4] = {};
+  block_product_x3_lds<MTP, NT>(are, aim, sg, Rw, X, bst, nx, none, false);
+  const int i16 = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int c = 0; c < NT; ++c) {
+    const int q = 16 * (NT * w + c) + i16;
+#pragma unroll
+    for (int p = 0; p < MTP; ++p)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * p + 4 * g + v;
+        if (r >= Rw) continue;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const float val = a ? aim[p][c][v] : are[p][c][v];
+          if (kseg > 0) {
+            part[((size_t)item * rt + 2 * r + a) * kNB + q] = val;
+            continue;
+          }
+          const int row = dst[d] + r;
+          if (row >= N && row < 2 * N) {
+            double* yp = Y + (size_t)(row - N) * kLD + a * kNB + q;
+            double y = double(val);
+            if (ACC) y += *yp;
+            *yp = y;
+          } else {
+            float* sp = bst + (size_t)row * kLD + a * kNB + q;
+            float y = val;
+            if (ACC) y += *sp;
+            *sp = y;
+          }
+        }
+      }
+  }
+}
+
+// The leaf kernel in split BF16 (as bleaf_kernel<float, XT, YT, ...>): one
+// workgroup per leaf, near field (scaled by qk2 when find_phi), leaf M2L and
+// leaf L2L, x term, rows stored once.
+template <class XT, class YT, int MTP, int NT>
+__global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
+    const char* __restrict__ T3, const int* __restrict__ near_src, int rank, const char* __restrict__ ops3,
+    const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ op3_off,
+    const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const float* __restrict__ X,
+    const float* __restrict__ bst, int nx, const XT* __restrict__ Xin, const double* __restrict__ qk2, int find_phi,
+    YT* __restrict__ Y) {
+  const int leaf = xcd_order(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i16 = lane & 15, g = lane >> 4;
+  SegX3 sg;
+  sg.A0 = T3;
+  sg.S0 = near_src + (size_t)leaf * 9 * rank;
+  sg.C0 = 9 * rank;
+  const int d1 = leaf_dest[2 * leaf], d2 = leaf_dest[2 * leaf + 1];
+  sg.A1 = d1 >= 0 ? ops3 + op3_off[d1] * 128 : T3;
+  sg.S1 = d1 >= 0 ? src + src_off[d1] : sg.S0;
+  sg.C1 = d1 >= 0 ? ncol[d1] : 0;
+  sg.A2 = d2 >= 0 ? ops3 + op3_off[d2] * 128 : T3;
+  sg.S2 = d2 >= 0 ? src + src_off[d2] : sg.S0;
+  sg.C2 = d2 >= 0 ? ncol[d2] : 0;
+  sg.n0 = (sg.C0 + 15) / 16;
+  sg.n01 = sg.n0 + (sg.C1 + 15) / 16;
+  sg.nch = sg.n01 + (sg.C2 + 15) / 16;
+  f32x4 are[MTP][NT], aim[MTP][NT];
+#pragma unroll
+  for (int p = 0; p < MTP; ++p)
+#pragma unroll
+    for (int c = 0; c < NT; ++c) are[p][c] = aim[p][c] = f32x4{0, 0, 0, 0};
+  float sc[MTP][4];
+#pragma unroll
+  for (int p = 0; p < MTP; ++p)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * p + 4 * g + v;
+      sc[p][v] = (find_phi && r < rank) ? float(qk2[(size_t)leaf * rank + r]) : 1.0f;
+    }
+  block_product_x3_lds<MTP, NT>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0);
+#pragma unroll
+  for (int c = 0; c < NT; ++c) {
+    const int q = 16 * (NT * w + c) + i16;
+#pragma unroll
+    for (int p = 0; p < MTP; ++p)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * p + 4 * g + v;
+        if (r >= rank) continue;
+        const size_t o = ((size_t)leaf * rank + r) * kLD + q;
+        double vr = double(are[p][c][v]), vi = double(aim[p][c][v]);
+        if (find_phi) {
+          vr += double(Xin[o]);
+          vi += double(Xin[o + kNB]);
+        }
+        Y[o] = YT(vr);
+        Y[o + kNB] = YT(vi);
+      }
+  }
+}
+
 // Sum of the nseg partial products of destination blockIdx.x (tiles of rt real
 // rows x 64), stored or accumulated into its rows (Y or the pass state).
 template <bool ACC, class R>
@@ -1038,7 +1401,11 @@ std::string BlockOperator::setup(DeviceOperator& o) {
 void BlockOperator::release() {
   work.release();
   outer.release();
-  void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part};
+  void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part, ops3, T3, d_op3};
+  ops3 = nullptr;
+  T3 = nullptr;
+  d_op3 = nullptr;
+  x3_active = false;
   qk2_cols = nullptr;
   bstate32 = nullptr;
   split_part = nullptr;
@@ -1122,6 +1489,72 @@ static void launch_bpass(BlockOperator& bo, int pidx, const R* X, int nx, double
                               o.d_op + b, o.d_srcoff + b, nseg);
 }
 
+// split-BF16 pass: MTP row tiles of 16 complex rows from the largest destination
+template <bool ACC, int NT>
+static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows, int nseg, const float* X, int nx,
+                               double* Y) {
+  DeviceOperator& o = *bo.op;
+  hipStream_t s = o.stream;
+  const int64_t b = p.dest_base;
+  const int mtp = (max_rows + 15) / 16;
+  const int rt = 16 * std::max(1, (2 * max_rows + 15) / 16);  // partial rows (the FP32 layout)
+  const int kseg = nseg > 1 ? kSplitCols : 0;
+  const unsigned grid = unsigned(p.ndest) * unsigned(nseg > 1 ? nseg : 1);
+  float* part = reinterpret_cast<float*>(bo.split_part);
+#define LSG_X3_CASE(M)                                                                                         \
+  case M:                                                                                                      \
+    hipLaunchKernelGGL((bpass_x3_kernel<ACC, M, NT>), dim3(grid), dim3(block_threads<NT>()), 0, s, bo.ops3, o.src, \
+                       bo.bstate32, X, Y, o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, bo.d_op3 + b,            \
+                       o.d_srcoff + b, nx, kseg, nseg > 1 ? nseg : 1, rt, part);                                \
+    break;
+  switch (mtp) {
+    LSG_X3_CASE(1)
+    LSG_X3_CASE(2)
+    LSG_X3_CASE(3)
+    default:
+      LSG_X3_CASE(4)
+  }
+#undef LSG_X3_CASE
+  if (nseg > 1)
+    hipLaunchKernelGGL((bsplit_reduce_kernel<ACC, float>), dim3(p.ndest), dim3(kT), 0, s, part, nseg, rt,
+                       o.d_rows + b, o.d_dst + b, bo.bstate32, Y, o.N);
+}
+
+static void launch_bpass_x3(BlockOperator& bo, int pidx, const float* X, int nx, double* Y) {
+  const DevPass& p = bo.op->passes[size_t(pidx)];
+  const int mr = bo.pass_max_rows[size_t(pidx)];
+  const int nseg = bo.pass_nseg[size_t(pidx)];
+  if (p.kind == 0) {
+    if (bo.nt3 == 2) launch_bpass_x3_nt<false, 2>(bo, p, mr, nseg, X, nx, Y);
+    else launch_bpass_x3_nt<false, 1>(bo, p, mr, nseg, X, nx, Y);
+  } else {
+    if (bo.nt3 == 2) launch_bpass_x3_nt<true, 2>(bo, p, mr, nseg, X, nx, Y);
+    else launch_bpass_x3_nt<true, 1>(bo, p, mr, nseg, X, nx, Y);
+  }
+}
+
+// the split-BF16 leaf kernel
+template <class XT, class YT, int NT>
+static void launch_bleaf_x3_nt(BlockOperator& bo, const float* Xr, int nx, const XT* X, YT* Y) {
+  DeviceOperator& o = *bo.op;
+  const int mtp = (o.rank + 15) / 16;
+  const dim3 grid(o.n_leaves);
+#define LSG_X3L_CASE(M)                                                                                        \
+  case M:                                                                                                      \
+    hipLaunchKernelGGL((bleaf_x3_kernel<XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, o.stream, bo.T3,      \
+                       bo.near_src, o.rank, bo.ops3, o.src, o.d_ncol, bo.d_op3, o.d_srcoff, bo.leaf_dest, Xr,     \
+                       bo.bstate32, nx, X, o.qk2, o.find_phi ? 1 : 0, Y);                                      \
+    break;
+  switch (mtp) {
+    LSG_X3L_CASE(1)
+    LSG_X3L_CASE(2)
+    LSG_X3L_CASE(3)
+    default:
+      LSG_X3L_CASE(4)
+  }
+#undef LSG_X3L_CASE
+}
+
 // FP32 copy of the input block (FP64 or FP16) into the first N rows of the FP32 pass state
 template <class XT>
 __global__ void bround_kernel(const XT* __restrict__ X, float* __restrict__ bst32, long long n_total) {
@@ -1187,6 +1620,32 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
 #undef LSG_LEAF_CASE
   };
   if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
+  if constexpr (std::is_same<R, float>::value) {
+    if (bo.x3_active) {
+      // split-BF16 arithmetic (the inner solves): same passes and leaf kernel
+      for (int p : o.up) launch_bpass_x3(bo, p, Xr, nx, Yd);
+      if (o.T && bo.leaf_fused) {
+        for (int p : o.down) launch_bpass_x3(bo, p, Xr, nx, Yd);
+        if (bo.nt3 == 2) launch_bleaf_x3_nt<XT, YT, 2>(bo, Xr, nx, X, Y);
+        else launch_bleaf_x3_nt<XT, YT, 1>(bo, Xr, nx, X, Y);
+      } else if constexpr (y64) {
+        if (o.T) {
+          if (bo.nt3 == 2) launch_bleaf_x3_nt<XT, YT, 2>(bo, Xr, nx, X, Y);
+          else launch_bleaf_x3_nt<XT, YT, 1>(bo, Xr, nx, X, Y);
+        }
+        for (int p : o.near) launch_bpass_x3(bo, p, Xr, nx, Yd);
+        for (int p : o.down) launch_bpass_x3(bo, p, Xr, nx, Yd);
+        for (int p : o.leaf_m2l) launch_bpass_x3(bo, p, Xr, nx, Yd);
+        for (int p : o.leaf_l2l) launch_bpass_x3(bo, p, Xr, nx, Yd);
+      }
+      if constexpr (y64) {
+        if (bo.qk2_cols)
+          hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
+                             Yd, bo.qk2_cols, (long long)o.N * kLD);
+      }
+      return;
+    }
+  }
   for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Yd);
   if (o.T && bo.leaf_fused) {
     for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Yd);
@@ -1252,6 +1711,75 @@ std::string BlockOperator::enable_fp32() {
   (void)hipEventDestroy(b);
   (void)hipFree(x);
   (void)hipFree(y);
+  return "";
+}
+
+std::string BlockOperator::enable_x3() {
+  std::string e = enable_fp32();
+  if (!e.empty()) return e;
+  if (ops3) return "";
+  DeviceOperator& o = *op;
+  const int64_t nd = o.passes.empty() ? 0 : o.passes.back().dest_base + o.passes.back().ndest;
+  std::vector<int> rows_h(static_cast<size_t>(nd)), ncol_h(static_cast<size_t>(nd));
+  if (nd) {
+    HIP_OK(hipMemcpy(rows_h.data(), o.d_rows, rows_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ncol_h.data(), o.d_ncol, ncol_h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  std::vector<long long> off(static_cast<size_t>(nd));
+  long long total = 0;
+  for (int64_t d = 0; d < nd; ++d) {
+    off[size_t(d)] = total;
+    total += (long long)rows_h[size_t(d)] * ((ncol_h[size_t(d)] + 15) / 16);
+  }
+  HIP_OK(hipMalloc((void**)&ops3, size_t(std::max(total, 1LL)) * 128));
+  HIP_OK(hipMalloc((void**)&d_op3, sizeof(long long) * size_t(std::max<int64_t>(nd, 1))));
+  if (nd) {
+    HIP_OK(hipMemcpy(d_op3, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_x3_kernel, dim3(unsigned(nd)), dim3(kT), 0, o.stream, o.ops, o.d_op, o.d_rows, o.d_ncol,
+                       d_op3, reinterpret_cast<unsigned*>(ops3));
+  }
+  if (o.T) {
+    const long long trows = (long long)o.rank * ((9 * o.rank + 15) / 16);
+    HIP_OK(hipMalloc((void**)&T3, size_t(trows) * 128));
+    hipLaunchKernelGGL(to_x3_table_kernel, dim3(blocks_for(trows * 16, kT)), dim3(kT), 0, o.stream, o.T, o.rank,
+                       9 * o.rank, reinterpret_cast<unsigned*>(T3));
+  }
+  HIP_OK(hipStreamSynchronize(o.stream));
+  // one or two right-hand-side tiles per wavefront: the faster on this operator
+  const size_t nb = size_t(o.N) * kLD;
+  float* x = nullptr;
+  double* y = nullptr;
+  HIP_OK(hipMalloc((void**)&x, sizeof(float) * nb));
+  if (hipMalloc((void**)&y, sizeof(double) * nb) != hipSuccess) {
+    (void)hipFree(x);
+    return "split-BF16 block operator: allocation failed";
+  }
+  hipStream_t s = o.stream;
+  (void)hipMemsetAsync(x, 0, sizeof(float) * nb, s);
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  float best[2] = {1e30f, 1e30f};
+  X3Scope x3_scope(*this, true);
+  for (int nt = 1; nt <= 2; ++nt) {
+    nt3 = nt;
+    matvec32(x, y);
+    for (int r = 0; r < 3; ++r) {
+      (void)hipEventRecord(a, s);
+      matvec32(x, y);
+      (void)hipEventRecord(b, s);
+      (void)hipEventSynchronize(b);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, a, b);
+      best[nt - 1] = std::min(best[nt - 1], ms);
+    }
+  }
+  nt3 = best[1] < best[0] ? 2 : 1;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  HIP_OK(hipGetLastError());
   return "";
 }
 
@@ -1392,14 +1920,17 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
       return res;
     }
   }
+  // the FP32 block matvecs of this solve run the split-BF16 kernels when asked
+  X3Scope x3_scope(bo, bo.x3_active);
   if (fp32) {
-    std::string e = bo.enable_fp32();
+    std::string e = bo.enable_inner();
     if (!e.empty()) {
       cleanup();
       res.status = LSG_ERR_DEVICE;
       res.error = e;
       return res;
     }
+    bo.x3_active = bo.inner_x3;
   }
   auto amv = [&](const auto* xin, double* yout) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(xin)>> XT;

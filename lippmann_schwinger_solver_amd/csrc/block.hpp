@@ -55,6 +55,15 @@ struct BlockOperator {
   // right-hand-side tiles of 16 per wavefront in the FP32 block passes (1 or
   // 2), chosen by timing when the FP32 operator is enabled
   int nt32 = 1;
+  // split-BF16 operator of the inner solves (a = ah + al in bf16, products
+  // ah bh + ah bl + al bh on BF16 MFMA with FP32 sums; 16 significant bits),
+  // built by enable_x3: per destination, chunks of 16 columns of 128-byte rows
+  char* ops3 = nullptr;
+  char* T3 = nullptr;            // the near-field tables in the same layout
+  long long* d_op3 = nullptr;    // per destination: its first 128-byte row in ops3
+  int nt3 = 1;                   // right-hand-side tiles per wavefront (chosen by timing)
+  bool inner_x3 = true;          // the FP32 inner solves use the split-BF16 kernels
+  bool x3_active = false;        // set while they run: FP32 block matvecs go to those kernels
   std::vector<int> pass_max_rows;  // largest destination of each pass (selects the row-tile count)
   std::vector<int> pass_nseg;      // column segments per destination (1: not split)
   void* split_part = nullptr;      // partial products of split passes
@@ -62,6 +71,9 @@ struct BlockOperator {
   BlockWork outer;    // the FP64 outer iteration of block_fgmres
   std::string setup(DeviceOperator& o);
   std::string enable_fp32();  // FP32 operators (DeviceOperator::enable_fp32) and pass state
+  std::string enable_x3();    // the split-BF16 operator (after enable_fp32)
+  // the inner solves' arithmetic: enable_x3 or enable_fp32 (inner_x3)
+  std::string enable_inner() { return inner_x3 ? enable_x3() : enable_fp32(); }
   void release();
   // Y = A X for one block (row-split, nb columns), on op->stream
   void matvec(const double* X, double* Y);
@@ -76,6 +88,14 @@ struct BlockOperator {
   void matvec32(const _Float16* X, double* Y);
   void matvec32(const _Float16* X, float* Y);
   bool can_store_f32() const { return op && leaf_fused && !qk2_cols; }
+};
+
+// x3_active set to `on` for the lifetime of the object, then restored
+struct X3Scope {
+  BlockOperator& bo;
+  bool prev;
+  X3Scope(BlockOperator& b, bool on) : bo(b), prev(b.x3_active) { bo.x3_active = on; }
+  ~X3Scope() { bo.x3_active = prev; }
 };
 
 // Batched restarted GMRES: nb independent systems A x_q = b_q advanced in
@@ -93,7 +113,9 @@ struct BatchResult {
 };
 // B, X: N x nb row-split device blocks (X holds the initial guesses); only the
 // first nrhs columns are solved, the others must be zero.
-// fp32: the block matvecs use FP32 operators and FP32 MFMA; tol_per (host, nrhs):
+// fp32: the block matvecs use the FP32 pass state with split-BF16 MFMA
+// operators (bo.inner_x3) or FP32 operators and FP32 MFMA, and the Krylov basis
+// is stored in FP16; tol_per (host, nrhs):
 // per-system tolerances instead of tol; allow_zero: zero right-hand sides are
 // accepted (their systems are converged with x unchanged).
 // x_zero: X is zero on entry (the first residual is B, no matvec).

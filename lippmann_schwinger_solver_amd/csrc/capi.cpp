@@ -673,8 +673,9 @@ int lsg_gmres_learned(lsg_ctx* ctx, const double* b, double* x, const lsg_solve_
 static int time_block(lsg_ctx* ctx, int reps, double* ms, bool fp32) {
   if (!ctx || reps < 1 || !ms) return LSG_ERR_ARG;
   std::string e = ctx->block_ready();
-  if (e.empty() && fp32) e = ctx->blk.enable_fp32();
+  if (e.empty() && fp32) e = ctx->blk.enable_inner();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+  X3Scope x3_scope(ctx->blk, fp32 && ctx->blk.inner_x3);
   auto mv = [&]() {
     if (fp32) ctx->blk.matvec32(ctx->bx, ctx->by);
     else ctx->blk.matvec(ctx->bx, ctx->by);
@@ -719,12 +720,14 @@ int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t 
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
   const size_t nb = sizeof(double2) * size_t(N);
   if (nrhs >= 2) {
-    // block path: 64 right-hand sides per FP32 block matvec (FP32 MFMA)
+    // block path: 64 right-hand sides per block matvec in the inner solves'
+    // arithmetic (split-BF16 or FP32 MFMA, FP32 pass state)
     for (int r = 0; r < nrhs; ++r)
       if (!all_finite(x + 2 * ld * r, 2 * N)) return ctx->set(LSG_ERR_NUMERIC, "input vector holds NaN/Inf");
     e = ctx->block_ready();
-    if (e.empty()) e = ctx->blk.enable_fp32();
+    if (e.empty()) e = ctx->blk.enable_inner();
     if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
+    X3Scope x3_scope(ctx->blk, ctx->blk.inner_x3);
     hipStream_t s = ctx->dev.stream;
     for (int r0 = 0; r0 < nrhs; r0 += kBlockRhs) {
       const int nr = std::min(kBlockRhs, nrhs - r0);
@@ -919,9 +922,16 @@ const char* lsg_tuning(const lsg_ctx* ctx) { return ctx ? ctx->tuning.c_str() : 
 int lsg_set_block_tiles(lsg_ctx* ctx, int tiles) {
   if (!ctx || (tiles != 1 && tiles != 2)) return LSG_ERR_ARG;
   std::string e = ctx->block_ready();
-  if (e.empty()) e = ctx->blk.enable_fp32();
+  if (e.empty()) e = ctx->blk.enable_inner();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-  ctx->blk.nt32 = tiles;
+  if (ctx->blk.inner_x3) ctx->blk.nt3 = tiles;
+  else ctx->blk.nt32 = tiles;
+  return LSG_OK;
+}
+
+int lsg_set_inner_arith(lsg_ctx* ctx, int arith) {
+  if (!ctx || (arith != LSG_ARITH_FP32 && arith != LSG_ARITH_BF16X3)) return LSG_ERR_ARG;
+  ctx->blk.inner_x3 = arith == LSG_ARITH_BF16X3;
   return LSG_OK;
 }
 

@@ -100,16 +100,19 @@ def test_fp32_block_matvec_and_batched_refinement():
         assert H.relerr(X[:, j], X64[:, j]) < 1e-6
 
 
-@pytest.mark.parametrize("m,K", [(24, 30), (8, 40)])
-def test_batched_flexible_gmres_fp32_inner(m, K):
+@pytest.mark.parametrize("m,K,arith", [(24, 30, "bf16x3"), (8, 40, "bf16x3"), (24, 30, "fp32")])
+def test_batched_flexible_gmres_fp32_inner(m, K, arith):
     """FP64 flexible GMRES(K) of the batch, each outer step preconditioned by
-    one FP32 batched GMRES(m) solve: every system reaches 1e-8 in FP64
+    one FP32-inner batched GMRES(m) solve (split-BF16 or FP32 MFMA operators,
+    FP32 pass state, FP16 basis): every system reaches 1e-8 in FP64
     (recomputed through the single-vector matvec and checked through the
     oracle's operator for two of them), with the same solutions as the FP64
     batch to the conditioning times the tolerance; refinement with the same
     inner solves is run beside it for the step counts."""
     inp = H.inputs(depth=5, kappa=40.0, q=6, seed=1)
+    print(f"case m={m} K={K} arith={arith}", flush=True)
     S = make(inp)
+    S.set_inner_arith(arith)
     o = O.Oracle(inp.to_c())
     angles = 2 * np.pi * np.arange(16) / 16
     B = S.rhs_angles(angles)
@@ -136,6 +139,7 @@ def test_fp32_block_matvec_tiles_per_wave(depth, kappa):
     wavefront: the same products to FP32 rounding, both within FP32 accuracy of
     the FP64 block matvec, on all 64 columns."""
     S = make(H.inputs(depth=depth, kappa=kappa, q=6, seed=2))
+    S.set_inner_arith("fp32")
     X = np.stack([H.rand_vec(S.N, 40 + j) for j in range(64)], axis=1)
     Y64 = S.matvec_block(X)
     ys = {}
@@ -146,4 +150,34 @@ def test_fp32_block_matvec_tiles_per_wave(depth, kappa):
         print(f"tiles {nt}: FP32 vs FP64 block {e:.2e}", flush=True)
         assert e < 1e-3
     assert max(H.relerr(ys[2][:, j], ys[1][:, j]) for j in range(64)) < 1e-5
+    S.close()
+
+
+@pytest.mark.parametrize("depth,kappa,find_phi", [(4, 30.0, 1), (5, 40.0, 1), (4, 30.0, 0)])
+def test_split_bf16_block_matvec(depth, kappa, find_phi):
+    """Split-BF16 block operator (each FP32 value as a bf16 pair, three BF16
+    MFMA products, FP32 sums) against the FP64 block matvec on all 64 columns,
+    with one and with two right-hand-side tiles per wavefront (the same sums in
+    the same order: equal results). The operator amplifies arithmetic error
+    about 80-fold (the FP32 block operator, unit roundoff 6e-8, lands near 5e-6
+    here); the split product carries about 2^-16 per product, which puts it
+    near 2e-4: the bound is 1e-3, and it must be less accurate than FP32 (the
+    two arithmetics really differ)."""
+    S = make(H.inputs(depth=depth, kappa=kappa, q=6, seed=2, find_phi=find_phi))
+    print(f"case depth={depth} kappa={kappa} find_phi={find_phi}", flush=True)
+    X = np.stack([H.rand_vec(S.N, 40 + j) for j in range(64)], axis=1)
+    Y64 = S.matvec_block(X)
+    S.set_inner_arith("fp32")
+    Y32 = S.matvec_block_fp32(X)
+    S.set_inner_arith("bf16x3")
+    ys = {}
+    for nt in (1, 2):
+        S.set_block_tiles(nt)
+        ys[nt] = S.matvec_block_fp32(X)
+        e = max(H.relerr(ys[nt][:, j], Y64[:, j]) for j in range(64))
+        e32 = max(H.relerr(Y32[:, j], Y64[:, j]) for j in range(64))
+        print(f"tiles {nt}: split-BF16 vs FP64 block {e:.2e} (FP32 {e32:.2e})", flush=True)
+        assert e < 1e-3
+        assert e > e32
+    assert max(H.relerr(ys[2][:, j], ys[1][:, j]) for j in range(64)) < 1e-6
     S.close()
