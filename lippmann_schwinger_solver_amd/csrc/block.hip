@@ -515,9 +515,9 @@ struct SegX3 {
 // into LDS by LDS-direct loads (double buffered), chunk i + 2's indices into a
 // second pair of buffers, chunk i + 1's source values into registers, while the
 // MFMAs of chunk i run; one wait and one barrier per chunk.
-template <int MTP, int NT>
+template <int MTP, int NT, class XS>
 __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
-                                                 int Rw, const float* __restrict__ X, const float* __restrict__ bst,
+                                                 int Rw, const XS* __restrict__ X, const float* __restrict__ bst,
                                                  int nx, const float (&scale0)[MTP][4], bool scaled, char* As0,
                                                  char* As1, int* Sr, const SegDesc* sd) {
   constexpr int kBT = block_threads<NT>();
@@ -560,11 +560,19 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
     const int s4[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const float* row = (s4[j] < nx ? X : bst) + (size_t)s4[j] * kLD + 16 * NT * w + i16;
+      const size_t o = (size_t)s4[j] * kLD + 16 * NT * w + i16;
+      if (s4[j] < nx) {
 #pragma unroll
-      for (int c = 0; c < NT; ++c) {
-        xv[c][j][0] = row[16 * c];
-        xv[c][j][1] = row[kNB + 16 * c];
+        for (int c = 0; c < NT; ++c) {
+          xv[c][j][0] = float(X[o + 16 * c]);
+          xv[c][j][1] = float(X[o + kNB + 16 * c]);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+          xv[c][j][0] = bst[o + 16 * c];
+          xv[c][j][1] = bst[o + kNB + 16 * c];
+        }
       }
     }
   };
@@ -644,9 +652,9 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
 
 // LDS of one split-BF16 workgroup: two chunk buffers of up to 64 rows, the
 // index buffers and the segment descriptors
-template <int MTP, int NT>
+template <int MTP, int NT, class XS>
 __device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
-                                                     int Rw, const float* __restrict__ X,
+                                                     int Rw, const XS* __restrict__ X,
                                                      const float* __restrict__ bst, int nx,
                                                      const float (&scale0)[MTP][4], bool scaled) {
   __shared__ __attribute__((aligned(16))) char As0[16 * MTP * 128], As1[16 * MTP * 128];
@@ -659,16 +667,17 @@ __device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x
     sd[2] = SegDesc{sg.A2, sg.S2, sg.C2, sg.n01};
   }
   __syncthreads();
-  block_product_x3<MTP, NT>(are, aim, sg, Rw, X, bst, nx, scale0, scaled, As0, As1, Sr, sd);
+  block_product_x3<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, scale0, scaled, As0, As1, Sr, sd);
 }
 
 // One pass on 64 right-hand sides in split BF16 (as bpass_kernel<ACC, float,
 // ...>): one workgroup per destination or per column segment of one; MTP row
 // tiles of 16 complex rows (rows of every destination <= 16 MTP); op3_off in
-// 128-byte rows of the split-BF16 operator.
-template <bool ACC, int MTP, int NT>
+// 128-byte rows of the split-BF16 operator; XS: type of the input block's
+// rows (the first nx source rows), FP32 or the inner solves' FP16 basis.
+template <bool ACC, int MTP, int NT, class XS>
 __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
-    const char* __restrict__ ops3, const int* __restrict__ src, float* bst, const float* __restrict__ X, double* Y,
+    const char* __restrict__ ops3, const int* __restrict__ src, float* bst, const XS* __restrict__ X, double* Y,
     int N, const int* __restrict__ rows, const int* __restrict__ ncol, const int* __restrict__ dst,
     const long long* __restrict__ op3_off, const long long* __restrict__ src_off, int nx, int kseg, int nseg,
     int rt, float* __restrict__ part) {
@@ -692,7 +701,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
 #pragma unroll
     for (int c = 0; c < NT; ++c) are[p][c] = aim[p][c] = f32x4{0, 0, 0, 0};
   const float none[MTP][4] = {};
-  block_product_x3_lds<MTP, NT>(are, aim, sg, Rw, X, bst, nx, none, false);
+  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, none, false);
   const int i16 = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
@@ -730,11 +739,11 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
 // The leaf kernel in split BF16 (as bleaf_kernel<float, XT, YT, ...>): one
 // workgroup per leaf, near field (scaled by qk2 when find_phi), leaf M2L and
 // leaf L2L, x term, rows stored once.
-template <class XT, class YT, int MTP, int NT>
+template <class XS, class XT, class YT, int MTP, int NT>
 __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
     const char* __restrict__ T3, const int* __restrict__ near_src, int rank, const char* __restrict__ ops3,
     const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ op3_off,
-    const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const float* __restrict__ X,
+    const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const XS* __restrict__ X,
     const float* __restrict__ bst, int nx, const XT* __restrict__ Xin, const double* __restrict__ qk2, int find_phi,
     YT* __restrict__ Y) {
   const int leaf = xcd_order(blockIdx.x, gridDim.x);
@@ -767,7 +776,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
       const int r = 16 * p + 4 * g + v;
       sc[p][v] = (find_phi && r < rank) ? float(qk2[(size_t)leaf * rank + r]) : 1.0f;
     }
-  block_product_x3_lds<MTP, NT>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0);
+  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0);
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
     const int q = 16 * (NT * w + c) + i16;
@@ -829,6 +838,7 @@ __global__ void bdiag_kernel(const XT* __restrict__ X, double* __restrict__ Y, c
 constexpr int kBRows = 256;   // rows per workgroup in the block dot
 constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
 constexpr int kRS = 16;       // first-stage segments of the row-block sums
+constexpr int kUpdRows = 8;   // rows per thread (and system group) in the basis updates
 
 // partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
 // conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
@@ -877,62 +887,69 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
   }
 }
 
-// The block dot for a low-precision basis (TV: FP32 or FP16) and an FP32
-// product: lane l of a wavefront covers the systems 2 (l % 32) and 2 (l % 32) + 1
-// with one load of a pair per value (8 bytes for FP32, 4 for FP16) and
-// half-wave h = l / 32 takes every other row (a row holds 64 real parts, then
-// 64 imaginary parts); the two halves are summed by a cross-lane exchange, the
-// four wavefronts through LDS. Same partial layout as bdot_kernel; products
-// and sums in FP64.
-template <class TV>
-__global__ __launch_bounds__(kT) void bdot2_kernel(const TV* __restrict__ V, long long vs, int nc, int ncx,
-                                                   const float* __restrict__ w, int N, double2* __restrict__ partial) {
-  typedef TV v2 __attribute__((ext_vector_type(2)));
+// The block dot for the inner solves' FP16 basis and FP32 product: lane l of a
+// wavefront covers the systems 4 (l % 16) .. + 3 with one 8-byte load per
+// basis value (16 for the product) and the quarter-wave l / 16 takes one of
+// four consecutive rows, so one load instruction moves four rows' 512 bytes of
+// the basis (its rows are 256 bytes: 64 real parts, then 64 imaginary parts).
+// Each thread sums its 16 rows in FP32 (the basis is FP16: these sums are far
+// below its rounding), the quarter-waves, wavefronts and row blocks in FP64.
+// Same partial layout as bdot_kernel.
+__global__ __launch_bounds__(kT) void bdot_h_kernel(const _Float16* __restrict__ V, long long vs, int nc, int ncx,
+                                                    const float* __restrict__ w, int N, double2* __restrict__ partial) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
   __shared__ double2 red[kT / 64][kBCols][kNB];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, h = lane >> 5, p = lane & 31;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, rs = lane >> 4, sg = lane & 15;
   const int g0 = blockIdx.y * kBCols;
   const long long n0 = (long long)blockIdx.x * kBRows;
-  double2 acc[kBCols][2];
+  float2 acc[kBCols][4];
 #pragma unroll
-  for (int u = 0; u < kBCols; ++u) acc[u][0] = acc[u][1] = make_double2(0.0, 0.0);
-  for (int k = 0; k < kBRows / 8; ++k) {
-    const long long n = n0 + 8 * k + 2 * wv + h;
+  for (int u = 0; u < kBCols; ++u)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[u][e] = make_float2(0.f, 0.f);
+  for (int k = 0; k < kBRows / 16; ++k) {
+    const long long n = n0 + 16 * k + 4 * wv + rs;
     if (n >= N) break;
-    const float2 wr = *reinterpret_cast<const float2*>(w + n * kLD + 2 * p);
-    const float2 wi = *reinterpret_cast<const float2*>(w + n * kLD + kNB + 2 * p);
+    const float4 wr = *reinterpret_cast<const float4*>(w + n * kLD + 4 * sg);
+    const float4 wi = *reinterpret_cast<const float4*>(w + n * kLD + kNB + 4 * sg);
+    const float wre[4] = {wr.x, wr.y, wr.z, wr.w}, wim[4] = {wi.x, wi.y, wi.z, wi.w};
 #pragma unroll
     for (int u = 0; u < kBCols; ++u) {
       const int col = g0 + u;
-      float2 vr = make_float2(0.f, 0.f), vi = make_float2(0.f, 0.f);
+      float vre[4] = {0.f, 0.f, 0.f, 0.f}, vim[4] = {0.f, 0.f, 0.f, 0.f};
       if (col < nc) {
-        const v2 a = *reinterpret_cast<const v2*>(V + col * vs + n * kLD + 2 * p);
-        const v2 b = *reinterpret_cast<const v2*>(V + col * vs + n * kLD + kNB + 2 * p);
-        vr = make_float2(float(a.x), float(a.y));
-        vi = make_float2(float(b.x), float(b.y));
+        const h4 a = *reinterpret_cast<const h4*>(V + col * vs + n * kLD + 4 * sg);
+        const h4 b = *reinterpret_cast<const h4*>(V + col * vs + n * kLD + kNB + 4 * sg);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          vre[e] = float(a[e]);
+          vim[e] = float(b[e]);
+        }
       } else if (col == nc) {
-        vr = wr;
-        vi = wi;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          vre[e] = wre[e];
+          vim[e] = wim[e];
+        }
       }
-      acc[u][0].x = fma(double(vr.x), double(wr.x), fma(double(vi.x), double(wi.x), acc[u][0].x));
-      acc[u][0].y = fma(double(vr.x), double(wi.x), fma(-double(vi.x), double(wr.x), acc[u][0].y));
-      acc[u][1].x = fma(double(vr.y), double(wr.y), fma(double(vi.y), double(wi.y), acc[u][1].x));
-      acc[u][1].y = fma(double(vr.y), double(wi.y), fma(-double(vi.y), double(wr.y), acc[u][1].y));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        acc[u][e].x = fmaf(vre[e], wre[e], fmaf(vim[e], wim[e], acc[u][e].x));
+        acc[u][e].y = fmaf(vre[e], wim[e], fmaf(-vim[e], wre[e], acc[u][e].y));
+      }
     }
   }
 #pragma unroll
   for (int u = 0; u < kBCols; ++u)
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      acc[u][e].x += __shfl_xor(acc[u][e].x, 32);
-      acc[u][e].y += __shfl_xor(acc[u][e].y, 32);
+    for (int e = 0; e < 4; ++e) {
+      double x = double(acc[u][e].x), y = double(acc[u][e].y);
+      x += __shfl_xor(x, 16);
+      y += __shfl_xor(y, 16);
+      x += __shfl_xor(x, 32);
+      y += __shfl_xor(y, 32);
+      if (rs == 0) red[wv][u][4 * sg + e] = make_double2(x, y);
     }
-  if (h == 0) {
-#pragma unroll
-    for (int u = 0; u < kBCols; ++u) {
-      red[wv][u][2 * p] = acc[u][0];
-      red[wv][u][2 * p + 1] = acc[u][1];
-    }
-  }
   __syncthreads();
   for (int e = t; e < kBCols * kNB; e += kT) {
     const int u = e / kNB, qq = e % kNB;
@@ -943,6 +960,74 @@ __global__ __launch_bounds__(kT) void bdot2_kernel(const TV* __restrict__ V, lon
         sm.y += red[k][u][qq].y;
       }
       partial[((long long)blockIdx.x * ncx + g0 + u) * kNB + qq] = sm;
+    }
+  }
+}
+
+// The update for the FP16 basis and FP32 product: v_out = (w - sum_i V_i h_i) /
+// hn; lane l covers the systems 4 (l % 16) .. + 3 (8-byte basis loads) of rows
+// n0 + 4 r + l / 16, r < kUpdRows, so one load instruction moves four rows.
+// The projection sums run in FP32 (the result is rounded to FP16); xout32
+// receives the FP32 value of every stored entry (the next matvec's input).
+__global__ __launch_bounds__(kT) void bupdate_h_kernel(const _Float16* __restrict__ V, long long vs, int nc,
+                                                       const double2* __restrict__ h, const float* __restrict__ w,
+                                                       const double* __restrict__ hn, _Float16* __restrict__ vout,
+                                                       float* __restrict__ xout32, long long n_rows) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, sg = lane & 15, rs = lane >> 4;
+  const long long n0 = ((long long)blockIdx.x * (kT / 64) + (threadIdx.x >> 6)) * (4 * kUpdRows);
+  float2 acc[kUpdRows][4];
+#pragma unroll
+  for (int r = 0; r < kUpdRows; ++r)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[r][e] = make_float2(0.f, 0.f);
+  for (int i = 0; i < nc; ++i) {
+    float hr[4], hi[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double2 hh = h[i * kNB + 4 * sg + e];
+      hr[e] = float(hh.x);
+      hi[e] = float(hh.y);
+    }
+    const _Float16* Vi = V + i * vs + 4 * sg;
+#pragma unroll
+    for (int r = 0; r < kUpdRows; ++r) {
+      const long long n = n0 + 4 * r + rs;
+      if (n < n_rows) {
+        const h4 a = *reinterpret_cast<const h4*>(Vi + n * kLD);
+        const h4 b = *reinterpret_cast<const h4*>(Vi + n * kLD + kNB);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float vr = float(a[e]), vi = float(b[e]);
+          acc[r][e].x = fmaf(vr, hr[e], fmaf(-vi, hi[e], acc[r][e].x));
+          acc[r][e].y = fmaf(vr, hi[e], fmaf(vi, hr[e], acc[r][e].y));
+        }
+      }
+    }
+  }
+  float inv[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double d = hn[4 * sg + e];
+    inv[e] = d > 0.0 ? float(1.0 / d) : 0.0f;
+  }
+#pragma unroll
+  for (int r = 0; r < kUpdRows; ++r) {
+    const long long n = n0 + 4 * r + rs;
+    if (n < n_rows) {
+      const long long o = n * kLD + 4 * sg;
+      const float4 wr = *reinterpret_cast<const float4*>(w + o), wi = *reinterpret_cast<const float4*>(w + o + kNB);
+      const float wre[4] = {wr.x, wr.y, wr.z, wr.w}, wim[4] = {wi.x, wi.y, wi.z, wi.w};
+      h4 a, b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        a[e] = _Float16((wre[e] - acc[r][e].x) * inv[e]);
+        b[e] = _Float16((wim[e] - acc[r][e].y) * inv[e]);
+      }
+      *reinterpret_cast<h4*>(vout + o) = a;
+      *reinterpret_cast<h4*>(vout + o + kNB) = b;
+      *reinterpret_cast<float4*>(xout32 + o) = make_float4(float(a[0]), float(a[1]), float(a[2]), float(a[3]));
+      *reinterpret_cast<float4*>(xout32 + o + kNB) = make_float4(float(b[0]), float(b[1]), float(b[2]), float(b[3]));
     }
   }
 }
@@ -1019,12 +1104,16 @@ __global__ void bcycle_init_kernel(const double2* __restrict__ h, BGivens st, in
 }
 
 // V0[n, q] = w[n, q] / beta_q for active q, 0 otherwise (beta in st.hn)
+// (v32: when given, the FP32 value of each stored entry as well)
 template <class TV>
-__global__ void bscale_kernel(const double* __restrict__ w, BGivens st, TV* __restrict__ v, long long n_total) {
+__global__ void bscale_kernel(const double* __restrict__ w, BGivens st, TV* __restrict__ v, long long n_total,
+                              float* __restrict__ v32) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_total) return;
   const int q = int(i % kNB);
-  v[i] = TV(st.active[q] ? w[i] / st.hn[q] : 0.0);
+  const TV x = TV(st.active[q] ? w[i] / st.hn[q] : 0.0);
+  v[i] = x;
+  if (v32) v32[i] = float(x);
 }
 
 // Arnoldi step j for every active system: h in [0, j], ||w||^2 at j+1 (from the
@@ -1116,7 +1205,6 @@ __global__ void bfill_kernel(double* __restrict__ a, int n, double v) {
 // (frozen before this step) — hn is the step's norm, set by btail. Thread
 // (q, row group) owns kUpdRows consecutive rows of system q, so each
 // coefficient h[i, q] is loaded once per kUpdRows rows.
-constexpr int kUpdRows = 8;
 template <class TV, class TW>
 __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, long long vs, int nc,
                                                      const double2* __restrict__ h, const TW* __restrict__ w,
@@ -1151,55 +1239,6 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, l
       const long long o = base + r * kLD;
       vout[o] = TV((double(w[o]) - ar[r]) * inv);
       vout[o + kNB] = TV((double(w[o + kNB]) - ai[r]) * inv);
-    }
-  }
-}
-
-// The update for a low-precision basis (TV: FP32 or FP16) and an FP32 product:
-// lane l of a half-wavefront covers the systems 2 (l % 32) and 2 (l % 32) + 1
-// with one load of a pair per value (as bdot2_kernel), each thread kUpdRows
-// consecutive rows of its two systems; sums in FP64, the new basis vector
-// rounded to TV.
-template <class TV>
-__global__ __launch_bounds__(kT) void bupdate2_kernel(const TV* __restrict__ V, long long vs, int nc,
-                                                      const double2* __restrict__ h, const float* __restrict__ w,
-                                                      const double* __restrict__ hn, TV* __restrict__ vout,
-                                                      long long n_rows) {
-  typedef TV v2 __attribute__((ext_vector_type(2)));
-  const int p = threadIdx.x & 31;
-  const long long n0 = ((long long)blockIdx.x * (kT / 32) + (threadIdx.x >> 5)) * kUpdRows;
-  if (n0 >= n_rows) return;
-  const int nr = int(min((long long)kUpdRows, n_rows - n0));
-  double ar[kUpdRows][2], ai[kUpdRows][2];
-#pragma unroll
-  for (int r = 0; r < kUpdRows; ++r) ar[r][0] = ar[r][1] = ai[r][0] = ai[r][1] = 0.0;
-  const long long base = n0 * kLD + 2 * p;
-#pragma unroll 2
-  for (int i = 0; i < nc; ++i) {
-    const double2 h0 = h[i * kNB + 2 * p], h1 = h[i * kNB + 2 * p + 1];
-    const TV* Vi = V + i * vs + base;
-#pragma unroll
-    for (int r = 0; r < kUpdRows; ++r) {
-      if (r < nr) {
-        const v2 vr = *reinterpret_cast<const v2*>(Vi + r * kLD);
-        const v2 vi = *reinterpret_cast<const v2*>(Vi + r * kLD + kNB);
-        ar[r][0] = fma(double(vr.x), h0.x, fma(-double(vi.x), h0.y, ar[r][0]));
-        ai[r][0] = fma(double(vr.x), h0.y, fma(double(vi.x), h0.x, ai[r][0]));
-        ar[r][1] = fma(double(vr.y), h1.x, fma(-double(vi.y), h1.y, ar[r][1]));
-        ai[r][1] = fma(double(vr.y), h1.y, fma(double(vi.y), h1.x, ai[r][1]));
-      }
-    }
-  }
-  const double d0 = hn[2 * p], d1 = hn[2 * p + 1];
-  const double inv0 = d0 > 0.0 ? 1.0 / d0 : 0.0, inv1 = d1 > 0.0 ? 1.0 / d1 : 0.0;
-#pragma unroll
-  for (int r = 0; r < kUpdRows; ++r) {
-    if (r < nr) {
-      const long long o = base + r * kLD;
-      const float2 wr = *reinterpret_cast<const float2*>(w + o), wi = *reinterpret_cast<const float2*>(w + o + kNB);
-      *reinterpret_cast<v2*>(vout + o) = v2{TV((double(wr.x) - ar[r][0]) * inv0), TV((double(wr.y) - ar[r][1]) * inv1)};
-      *reinterpret_cast<v2*>(vout + o + kNB) =
-          v2{TV((double(wi.x) - ai[r][0]) * inv0), TV((double(wi.y) - ai[r][1]) * inv1)};
     }
   }
 }
@@ -1490,8 +1529,8 @@ static void launch_bpass(BlockOperator& bo, int pidx, const R* X, int nx, double
 }
 
 // split-BF16 pass: MTP row tiles of 16 complex rows from the largest destination
-template <bool ACC, int NT>
-static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows, int nseg, const float* X, int nx,
+template <bool ACC, int NT, class XS>
+static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows, int nseg, const XS* X, int nx,
                                double* Y) {
   DeviceOperator& o = *bo.op;
   hipStream_t s = o.stream;
@@ -1503,7 +1542,7 @@ static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows
   float* part = reinterpret_cast<float*>(bo.split_part);
 #define LSG_X3_CASE(M)                                                                                         \
   case M:                                                                                                      \
-    hipLaunchKernelGGL((bpass_x3_kernel<ACC, M, NT>), dim3(grid), dim3(block_threads<NT>()), 0, s, bo.ops3, o.src, \
+    hipLaunchKernelGGL((bpass_x3_kernel<ACC, M, NT, XS>), dim3(grid), dim3(block_threads<NT>()), 0, s, bo.ops3, o.src, \
                        bo.bstate32, X, Y, o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, bo.d_op3 + b,            \
                        o.d_srcoff + b, nx, kseg, nseg > 1 ? nseg : 1, rt, part);                                \
     break;
@@ -1520,7 +1559,8 @@ static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows
                        o.d_rows + b, o.d_dst + b, bo.bstate32, Y, o.N);
 }
 
-static void launch_bpass_x3(BlockOperator& bo, int pidx, const float* X, int nx, double* Y) {
+template <class XS>
+static void launch_bpass_x3(BlockOperator& bo, int pidx, const XS* X, int nx, double* Y) {
   const DevPass& p = bo.op->passes[size_t(pidx)];
   const int mr = bo.pass_max_rows[size_t(pidx)];
   const int nseg = bo.pass_nseg[size_t(pidx)];
@@ -1534,14 +1574,14 @@ static void launch_bpass_x3(BlockOperator& bo, int pidx, const float* X, int nx,
 }
 
 // the split-BF16 leaf kernel
-template <class XT, class YT, int NT>
-static void launch_bleaf_x3_nt(BlockOperator& bo, const float* Xr, int nx, const XT* X, YT* Y) {
+template <class XS, class XT, class YT, int NT>
+static void launch_bleaf_x3_nt(BlockOperator& bo, const XS* Xr, int nx, const XT* X, YT* Y) {
   DeviceOperator& o = *bo.op;
   const int mtp = (o.rank + 15) / 16;
   const dim3 grid(o.n_leaves);
 #define LSG_X3L_CASE(M)                                                                                        \
   case M:                                                                                                      \
-    hipLaunchKernelGGL((bleaf_x3_kernel<XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, o.stream, bo.T3,      \
+    hipLaunchKernelGGL((bleaf_x3_kernel<XS, XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, o.stream, bo.T3,      \
                        bo.near_src, o.rank, bo.ops3, o.src, o.d_ncol, bo.d_op3, o.d_srcoff, bo.leaf_dest, Xr,     \
                        bo.bstate32, nx, X, o.qk2, o.find_phi ? 1 : 0, Y);                                      \
     break;
@@ -1562,6 +1602,37 @@ __global__ void bround_kernel(const XT* __restrict__ X, float* __restrict__ bst3
   if (i < n_total) bst32[i] = float(X[i]);
 }
 
+// The block matvec in split BF16 (the inner solves' arithmetic): the same
+// passes and leaf kernel as the FP32 path; the input block's rows are read in
+// place (XS = XT, FP32 or FP16) or from its FP32 copy in the pass state (nx = 0).
+template <class XS, class XT, class YT>
+static void block_matvec_x3(BlockOperator& bo, const XS* Xs, int nx, const XT* X, YT* Y) {
+  constexpr bool y64 = std::is_same<YT, double>::value;
+  double* Yd = y64 ? reinterpret_cast<double*>(Y) : nullptr;
+  DeviceOperator& o = *bo.op;
+  if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
+  auto leaf = [&]() {
+    if (bo.nt3 == 2) launch_bleaf_x3_nt<XS, XT, YT, 2>(bo, Xs, nx, X, Y);
+    else launch_bleaf_x3_nt<XS, XT, YT, 1>(bo, Xs, nx, X, Y);
+  };
+  for (int p : o.up) launch_bpass_x3(bo, p, Xs, nx, Yd);
+  if (o.T && bo.leaf_fused) {
+    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    leaf();
+  } else if constexpr (y64) {
+    if (o.T) leaf();
+    for (int p : o.near) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.leaf_m2l) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.leaf_l2l) launch_bpass_x3(bo, p, Xs, nx, Yd);
+  }
+  if constexpr (y64) {
+    if (bo.qk2_cols)
+      hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Yd,
+                         bo.qk2_cols, (long long)o.N * kLD);
+  }
+}
+
 // R: pass precision; XT: type of the input block. An FP64 or FP16 input to
 // the FP32 passes is converted into the pass state's x rows first; an input
 // already in the pass precision is read in place.
@@ -1575,6 +1646,18 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
   typedef typename Mfma<R>::c2 C2;
   const C2* T = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
   R* bst = reinterpret_cast<R*>(std::is_same<R, double>::value ? (void*)bo.bstate : (void*)bo.bstate32);
+  if constexpr (std::is_same<R, float>::value) {
+    if (bo.x3_active) {
+      if constexpr (std::is_same<XT, double>::value) {
+        hipLaunchKernelGGL(bround_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
+                           bo.bstate32, (long long)o.N * kLD);
+        block_matvec_x3<float, XT, YT>(bo, bo.bstate32, 0, X, Y);
+      } else {
+        block_matvec_x3<XT, XT, YT>(bo, X, o.N, X, Y);
+      }
+      return;
+    }
+  }
   const R* Xr;
   int nx;
   if (std::is_same<R, XT>::value) {
@@ -1620,32 +1703,6 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
 #undef LSG_LEAF_CASE
   };
   if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
-  if constexpr (std::is_same<R, float>::value) {
-    if (bo.x3_active) {
-      // split-BF16 arithmetic (the inner solves): same passes and leaf kernel
-      for (int p : o.up) launch_bpass_x3(bo, p, Xr, nx, Yd);
-      if (o.T && bo.leaf_fused) {
-        for (int p : o.down) launch_bpass_x3(bo, p, Xr, nx, Yd);
-        if (bo.nt3 == 2) launch_bleaf_x3_nt<XT, YT, 2>(bo, Xr, nx, X, Y);
-        else launch_bleaf_x3_nt<XT, YT, 1>(bo, Xr, nx, X, Y);
-      } else if constexpr (y64) {
-        if (o.T) {
-          if (bo.nt3 == 2) launch_bleaf_x3_nt<XT, YT, 2>(bo, Xr, nx, X, Y);
-          else launch_bleaf_x3_nt<XT, YT, 1>(bo, Xr, nx, X, Y);
-        }
-        for (int p : o.near) launch_bpass_x3(bo, p, Xr, nx, Yd);
-        for (int p : o.down) launch_bpass_x3(bo, p, Xr, nx, Yd);
-        for (int p : o.leaf_m2l) launch_bpass_x3(bo, p, Xr, nx, Yd);
-        for (int p : o.leaf_l2l) launch_bpass_x3(bo, p, Xr, nx, Yd);
-      }
-      if constexpr (y64) {
-        if (bo.qk2_cols)
-          hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
-                             Yd, bo.qk2_cols, (long long)o.N * kLD);
-      }
-      return;
-    }
-  }
   for (int p : o.up) launch_bpass<R>(bo, p, Xr, nx, Yd);
   if (o.T && bo.leaf_fused) {
     for (int p : o.down) launch_bpass<R>(bo, p, Xr, nx, Yd);
@@ -1816,12 +1873,13 @@ std::string BlockWork::basis(bool f32) {
   const size_t n = size_t(N) * kLD * size_t(m + 1);
   if (f32 && !V16) HIP_OK(hipMalloc((void**)&V16, sizeof(_Float16) * n));
   if (f32 && !w32) HIP_OK(hipMalloc((void**)&w32, sizeof(float) * size_t(N) * kLD));
+  if (f32 && !x32) HIP_OK(hipMalloc((void**)&x32, sizeof(float) * size_t(N) * kLD));
   if (!f32 && !V) HIP_OK(hipMalloc((void**)&V, sizeof(double) * n));
   return "";
 }
 
 void BlockWork::release() {
-  void* ptrs[] = {V, V16, w32, Z, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
+  void* ptrs[] = {V, V16, w32, x32, Z, w, partial, h, y, H, cs, sn, g, hn, bnorm, active, kq, tolq};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (host_rel) (void)hipHostFree(host_rel);
@@ -1888,7 +1946,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(vec)>> TW;
     const int ncx = nc + (norm ? 1 : 0);
     if constexpr (lowv && std::is_same<TW, float>::value)
-      hipLaunchKernelGGL(bdot2_kernel<TV>, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
+      hipLaunchKernelGGL(bdot_h_kernel, dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc, ncx, vec,
                          N, partial);
     else
       hipLaunchKernelGGL((bdot_kernel<TV, TW>), dim3(nrb, (ncx + kBCols - 1) / kBCols), dim3(kT), 0, s, V, vs, nc,
@@ -1980,20 +2038,21 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
       break;
     }
     if (n_act == 0 || cyc == max_restarts) break;
-    hipLaunchKernelGGL(bscale_kernel<TV>, dim3(gv), dim3(kT), 0, s, w, st, V, ntot);
+    hipLaunchKernelGGL(bscale_kernel<TV>, dim3(gv), dim3(kT), 0, s, w, st, V, ntot, w32 ? W.x32 : (float*)nullptr);
     int steps = 0;
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
       for (int jj = j; jj < j + batch; ++jj) {
         if constexpr (lowv) {
           if (w32) {
-            bo.matvec32((const TV*)(V + size_t(jj) * vs), w32);
+            // the FP32 copy of v_jj (written with it) is the matvec's input
+            bo.matvec32((const float*)W.x32, w32);
             dot(jj + 1, (const float*)w32, true);
             hipLaunchKernelGGL(btail_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, jj, host_est, host_active,
                                (const double*)nullptr);
-            hipLaunchKernelGGL(bupdate2_kernel<TV>, dim3(blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 32)),
+            hipLaunchKernelGGL(bupdate_h_kernel, dim3(blocks_for((N + 4 * kUpdRows - 1) / (4 * kUpdRows), kT / 64)),
                                dim3(kT), 0, s, (const TV*)V, vs, jj + 1, h, (const float*)w32, st.hn,
-                               V + size_t(jj + 1) * vs, (long long)N);
+                               V + size_t(jj + 1) * vs, W.x32, (long long)N);
             continue;
           }
         }
@@ -2278,7 +2337,7 @@ BatchResult block_fgmres(BlockOperator& bo, const double* B, double* X, int nrhs
       break;
     }
     if (n_act == 0 || cyc == max_cycles) break;
-    hipLaunchKernelGGL(bscale_kernel<double>, dim3(gv), dim3(kT), 0, s, w, st, V, vs);
+    hipLaunchKernelGGL(bscale_kernel<double>, dim3(gv), dim3(kT), 0, s, w, st, V, vs, (float*)nullptr);
     int steps = 0;
     for (int k = 0; k < K; ++k) {
       // z_k ~ A^-1 v_k by the FP32 batched GMRES (columns of frozen systems are

@@ -27,6 +27,7 @@ struct BlockWork {
   double *V = nullptr, *w = nullptr;
   _Float16* V16 = nullptr;  // FP16 basis of the FP32 inner solves
   float* w32 = nullptr;   // FP32 product A v_j of the FP32 inner solves (when the leaf kernel writes every output row)
+  float* x32 = nullptr;   // FP32 copy of the current FP16 basis vector: the next block matvec's input
   double* Z = nullptr;    // flexible GMRES: the m preconditioned directions (FP64), allocated by flex()
   double2 *partial = nullptr, *h = nullptr, *y = nullptr, *H = nullptr, *sn = nullptr, *g = nullptr;
   double *cs = nullptr, *hn = nullptr, *bnorm = nullptr, *tolq = nullptr;
