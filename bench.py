@@ -235,13 +235,17 @@ def pmc_probe(S):
 
 def pmc_probe_block(S):
     """3 block matvecs (64 right-hand sides) on FP64 MFMA, then 3 on FP32 MFMA,
-    each after one warm-up, for counter runs; the algorithmic flops of one are
-    printed for the ratio."""
+    then 3 in split BF16, each after one warm-up, for counter runs; the
+    algorithmic flops of one are printed for the ratio."""
     ms = S.time_matvec_block(reps=3)
+    S.set_inner_arith("fp32")
     ms32 = S.time_matvec_block(reps=3, fp32=True)
+    S.set_inner_arith("bf16x3")
+    msx3 = S.time_matvec_block(reps=3, fp32=True)
     info = S.info()
     flops = 8.0 * 64 * (info["E_far"] + info["E_near"])
-    log(f"pmc block probe: block matvec {ms:.3f} ms FP64, {ms32:.3f} ms FP32, {flops:.4e} algorithmic flops each")
+    log(f"pmc block probe: block matvec {ms:.3f} ms FP64, {ms32:.3f} ms FP32, {msx3:.3f} ms split BF16, "
+        f"{flops:.4e} algorithmic flops each")
 
 
 def setup_record(wall_s, info, inp):
@@ -359,6 +363,7 @@ def bench_c3(args):
     value = 64 * args.steps / elapsed
     st32, rel32 = steps[-1]
     log(f"timed: {args.steps} batches of 64 in {elapsed:.3f} s ({value:.2f} solves/s); last {st32}")
+    log("block operator choices:\n" + "\n".join(l for l in S.tuning().splitlines() if "block matvec" in l))
     X32 = Xd.download()
     recomputed = [_relres(S, X32[q], Bh[q]) for q in range(64)]
     log(f"recomputed FP64 residuals: max {max(recomputed):.3e}")

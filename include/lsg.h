@@ -299,8 +299,10 @@ int lsg_set_pass_kernel(lsg_ctx* ctx, int kernel);
 /* Right-hand-side tiles of 16 per wavefront in the block passes of the inner
  * solves' arithmetic (lsg_set_inner_arith; no reference counterpart): 1 (four
  * wavefronts per workgroup) or 2 (two, each operator value feeding two
- * MFMAs). Enabling that block operator picks the faster by timing; this
- * replaces that choice (tests, measurements). */
+ * MFMAs); in split BF16 on a uniform tree also 3 or 4 (1 or 2 tiles, the near
+ * field computed on the second stream beside the upward and downward passes,
+ * the leaf passes added after them). Enabling that block operator picks the
+ * fastest by timing; this replaces that choice (tests, measurements). */
 int lsg_set_block_tiles(lsg_ctx* ctx, int tiles);
 
 /* Arithmetic of the FP32-inner batched solves (LSG_SOLVE_FP32_IR and

@@ -668,6 +668,10 @@ class DFMM:
         self._check(self._lib.lsg_autotune(self._ctx, reps, C.byref(n)))
         return n.value, self._lib.lsg_tuning(self._ctx).decode()
 
+    def tuning(self):
+        """The autotuner's last report and the block operator's timed choices."""
+        return self._lib.lsg_tuning(self._ctx).decode()
+
     def set_pass_kernel(self, kernel, fused=False):
         """Run every pass with one kernel: "block" (one workgroup per
         destination), "wave" (one wavefront per destination) or "lane" (one
@@ -679,7 +683,9 @@ class DFMM:
 
     def set_block_tiles(self, tiles):
         """Right-hand-side tiles of 16 per wavefront in the block passes of the
-        inner solves' arithmetic (1 or 2), replacing the choice made by timing."""
+        inner solves' arithmetic (1 or 2; in split BF16 also 3 or 4: 1 or 2
+        tiles with the near field beside the upward and downward passes),
+        replacing the choice made by timing."""
         self._check(self._lib.lsg_set_block_tiles(self._ctx, tiles))
 
     def set_inner_arith(self, arith):

@@ -507,6 +507,15 @@ struct SegX3 {
   int n0, n01, nch;  // chunk counts: segment 0, segments 0-1, all
 };
 
+// Sub-chunks of 16 columns (one MFMA k-step) per LDS chunk of the split-BF16
+// product: more columns per chunk put more loads in flight per barrier.
+#ifndef LSG_X3_SUB
+#define LSG_X3_SUB 2
+#endif
+constexpr int kX3Sub = LSG_X3_SUB;
+constexpr int kX3Cols = 16 * kX3Sub;  // columns per chunk
+__host__ __device__ constexpr int x3_chunks(int C) { return (C + kX3Cols - 1) / kX3Cols; }
+
 // Block product of one workgroup on 64 right-hand sides in split BF16:
 // are/aim[p][c] (wave w: right-hand sides 16 (NT w + c) .. + 15) += the real
 // and imaginary rows of row tile p over all segments. With scale0, the rows are
@@ -514,20 +523,22 @@ struct SegX3 {
 // the near field). Pipeline as in block_product: chunk i + 1's operator rows
 // into LDS by LDS-direct loads (double buffered), chunk i + 2's indices into a
 // second pair of buffers, chunk i + 1's source values into registers, while the
-// MFMAs of chunk i run; one wait and one barrier per chunk.
+// MFMAs of chunk i run; one wait and one barrier per chunk. A chunk holds up
+// to kX3Sub sub-chunks of 16 columns, stored in LDS as consecutive blocks of
+// Rw rows (row rr of the chunk: sub-chunk rr / Rw).
 template <int MTP, int NT, class XS>
 __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
                                                  int Rw, const XS* __restrict__ X, const float* __restrict__ bst,
                                                  int nx, const float (&scale0)[MTP][4], bool scaled, char* As0,
                                                  char* As1, int* Sr, const SegDesc* sd) {
   constexpr int kBT = block_threads<NT>();
+  constexpr int KS = kX3Sub;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int i16 = lane & 15, g = lane >> 4;
-  const int n_cp = 8 * Rw;  // 16-byte slots per chunk
   struct Pick {
     const char* A;
     const int* S;
-    int C, c0;
+    int C, c0, nsub;
   };
   auto seg_of = [=](int i) __attribute__((always_inline)) {
     const SegDesc& d = sd[(i >= sg.n0 ? 1 : 0) + (i >= sg.n01 ? 1 : 0)];
@@ -535,13 +546,15 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
     p.A = reinterpret_cast<const char*>(d.A);
     p.S = d.S;
     p.C = d.C;
-    p.c0 = (i - d.first) * 16;
+    p.c0 = (i - d.first) * kX3Cols;
+    p.nsub = min(KS, (d.C - p.c0 + 15) >> 4);
     return p;
   };
   auto copy_ops = [=](int i, char* As) __attribute__((always_inline)) {
     const Pick p = seg_of(i);
     const char* src = p.A + (size_t)(p.c0 >> 4) * Rw * 128;
-    for (int q = 0; q * kBT < n_cp; ++q) {
+    const int n_cp = 8 * Rw * p.nsub;  // 16-byte slots of the chunk's sub-chunks
+    for (int q = 0; q * kBT < 8 * Rw * KS; ++q) {
       const int e = q * kBT + t;
       const int r = e >> 3, sl = (e & 7) ^ (r & 7);
       char* wb = As + (size_t)(q * kBT + 64 * w) * 16;
@@ -551,114 +564,138 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
   auto copy_idx = [=](int i, int* Sb) __attribute__((always_inline)) {
     if (w != 0 || i >= sg.nch) return;
     const Pick p = seg_of(i);
-    if (lane < 16) lds_copy4(p.S + min(p.c0 + lane, p.C - 1), Sb);
+    if (lane < kX3Cols) lds_copy4(p.S + min(p.c0 + lane, p.C - 1), Sb);
   };
-  // this lane's source values of chunk i: columns 4 g .. 4 g + 3, right-hand
-  // sides of its NT tiles (real and imaginary parts)
-  auto load_src = [=](int i, const int* Sb, float (&xv)[NT][4][2]) __attribute__((always_inline)) {
-    const int4 sv = *reinterpret_cast<const int4*>(Sb + 4 * g);
-    const int s4[4] = {sv.x, sv.y, sv.z, sv.w};
+  // this lane's source values of chunk i: columns 16 s + 4 g .. + 3 of each
+  // sub-chunk s, right-hand sides of its NT tiles (real and imaginary parts)
+  auto load_src = [=](int i, const int* Sb, float (&xv)[NT][4 * KS][2]) __attribute__((always_inline)) {
+    const Pick p = seg_of(i);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const size_t o = (size_t)s4[j] * kLD + 16 * NT * w + i16;
-      if (s4[j] < nx) {
+    for (int sb = 0; sb < KS; ++sb) {
+      if (sb >= p.nsub) break;
+      const int4 sv = *reinterpret_cast<const int4*>(Sb + 16 * sb + 4 * g);
+      const int s4[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
-        for (int c = 0; c < NT; ++c) {
-          xv[c][j][0] = float(X[o + 16 * c]);
-          xv[c][j][1] = float(X[o + kNB + 16 * c]);
-        }
-      } else {
+      for (int j = 0; j < 4; ++j) {
+        const size_t o = (size_t)s4[j] * kLD + 16 * NT * w + i16;
+        if (s4[j] < nx) {
 #pragma unroll
-        for (int c = 0; c < NT; ++c) {
-          xv[c][j][0] = bst[o + 16 * c];
-          xv[c][j][1] = bst[o + kNB + 16 * c];
+          for (int c = 0; c < NT; ++c) {
+            xv[c][4 * sb + j][0] = float(X[o + 16 * c]);
+            xv[c][4 * sb + j][1] = float(X[o + kNB + 16 * c]);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < NT; ++c) {
+            xv[c][4 * sb + j][0] = bst[o + 16 * c];
+            xv[c][4 * sb + j][1] = bst[o + kNB + 16 * c];
+          }
         }
       }
     }
   };
-  int* Sc = Sr + 16;
-  int* Sf = Sr;
-  float xcur[NT][4][2], xnext[NT][4][2];
-  copy_idx(0, Sf);
-  copy_idx(1, Sc);
+  // chunk c's indices live in index buffer c & 1, its operator rows in As0/As1 by the same parity
+  typedef float XBuf[NT][4 * KS][2];
+  XBuf xa, xb;
+  copy_idx(0, Sr);
+  copy_idx(1, Sr + kX3Cols);
   copy_ops(0, As0);
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   __syncthreads();
-  load_src(0, Sf, xcur);
+  load_src(0, Sr, xa);
   __builtin_amdgcn_s_waitcnt(0x0F70);
   __syncthreads();
-  for (int i = 0; i < sg.nch; ++i) {
+  // chunk i: its source values in xc; chunk i + 1's are loaded into xn
+  auto step = [&](int i, XBuf& xc, XBuf& xn) __attribute__((always_inline)) {
     const char* As = (i & 1) ? As1 : As0;
+    const int nsub = seg_of(i).nsub;
     if (i + 1 < sg.nch) copy_ops(i + 1, (i & 1) ? As0 : As1);
-    copy_idx(i + 2, Sf);
-    if (i + 1 < sg.nch) load_src(i + 1, Sc, xnext);
-    // source fragments: [Re x, -Im x] for the real rows, [Im x, Re x] for the imaginary rows
-    u32x4 b1h[NT], b1l[NT], b2h[NT], b2l[NT];
+    copy_idx(i + 2, Sr + kX3Cols * (i & 1));
+    if (i + 1 < sg.nch) load_src(i + 1, Sr + kX3Cols * ((i + 1) & 1), xn);
 #pragma unroll
-    for (int c = 0; c < NT; ++c)
+    for (int sb = 0; sb < KS; ++sb) {
+      if (sb >= nsub) break;
+      // source fragments: [Re x, -Im x] for the real rows, [Im x, Re x] for the imaginary rows
+      u32x4 b1h[NT], b1l[NT], b2h[NT], b2l[NT];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        unsigned hi, lo;
-        split_pair(xcur[c][j][0], -xcur[c][j][1], hi, lo);
-        b1h[c][j] = hi;
-        b1l[c][j] = lo;
-        // [Im x, Re x] = the halves of [Re x, -Im x] swapped, the new low half negated
-        b2h[c][j] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
-        b2l[c][j] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+      for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          unsigned hi, lo;
+          split_pair(xc[c][4 * sb + j][0], -xc[c][4 * sb + j][1], hi, lo);
+          b1h[c][j] = hi;
+          b1l[c][j] = lo;
+          // [Im x, Re x] = the halves of [Re x, -Im x] swapped, the new low half negated
+          b2h[c][j] = __builtin_amdgcn_alignbit(hi, hi, 16) ^ 0x8000u;
+          b2l[c][j] = __builtin_amdgcn_alignbit(lo, lo, 16) ^ 0x8000u;
+        }
+#pragma unroll
+      for (int p = 0; p < MTP; ++p) {
+        const int r = sb * Rw + min(16 * p + i16, Rw - 1);  // rows past Rw compute unused results
+        const char* rowp = As + r * 128;
+        const int sw = r & 7;
+        const bf16x8 ah = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + ((g ^ sw) << 4)));
+        const bf16x8 al = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + (((4 + g) ^ sw) << 4)));
+#pragma unroll
+        for (int c = 0; c < NT; ++c) {
+          are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
+          aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
+          are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1l[c]), are[p][c], 0, 0, 0);
+          aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2l[c]), aim[p][c], 0, 0, 0);
+          are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
+          aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
+        }
       }
-#pragma unroll
-    for (int p = 0; p < MTP; ++p) {
-      const int r = min(16 * p + i16, Rw - 1);  // rows past Rw compute unused results
-      const char* rowp = As + r * 128;
-      const int sw = r & 7;
-      const bf16x8 ah = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + ((g ^ sw) << 4)));
-      const bf16x8 al = as_bf16x8(*reinterpret_cast<const u32x4*>(rowp + (((4 + g) ^ sw) << 4)));
-#pragma unroll
-      for (int c = 0; c < NT; ++c) {
-        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
-        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
-        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b1l[c]), are[p][c], 0, 0, 0);
-        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2l[c]), aim[p][c], 0, 0, 0);
-        are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
-        aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
-      }
-    }
-    if (scaled && i == sg.n0 - 1) {
-#pragma unroll
-      for (int p = 0; p < MTP; ++p)
-#pragma unroll
-        for (int c = 0; c < NT; ++c)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            are[p][c][v] *= scale0[p][v];
-            aim[p][c][v] *= scale0[p][v];
-          }
     }
     __builtin_amdgcn_sched_barrier(0);
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
+  };
+  // chunks [i0, i1) two at a time (the buffers swap roles without copies);
+  // on return xa holds chunk i1's source values
+  auto run = [&](int i0, int i1) __attribute__((always_inline)) {
+    int i = i0;
+    for (; i + 1 < i1; i += 2) {
+      step(i, xa, xb);
+      step(i + 1, xb, xa);
+    }
+    if (i < i1) {
+      step(i, xa, xb);
 #pragma unroll
-    for (int c = 0; c < NT; ++c)
+      for (int c = 0; c < NT; ++c)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        xcur[c][j][0] = xnext[c][j][0];
-        xcur[c][j][1] = xnext[c][j][1];
-      }
-    int* tmp = Sc;
-    Sc = Sf;
-    Sf = tmp;
+        for (int k = 0; k < 4 * KS; ++k) {
+          xa[c][k][0] = xb[c][k][0];
+          xa[c][k][1] = xb[c][k][1];
+        }
+    }
+  };
+  // segment 0, its row scaling (outside the loops: the accumulators stay in
+  // the MFMA registers), then the other segments
+  run(0, sg.n0);
+  if (scaled) {
+#pragma unroll
+    for (int p = 0; p < MTP; ++p)
+#pragma unroll
+      for (int c = 0; c < NT; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          are[p][c][v] *= scale0[p][v];
+          aim[p][c][v] *= scale0[p][v];
+        }
   }
+  run(sg.n0, sg.nch);
 }
 
-// LDS of one split-BF16 workgroup: two chunk buffers of up to 64 rows, the
-// index buffers and the segment descriptors
+// LDS of one split-BF16 workgroup: two chunk buffers of up to 64 rows per
+// sub-chunk, the index buffers and the segment descriptors
 template <int MTP, int NT, class XS>
 __device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
                                                      int Rw, const XS* __restrict__ X,
                                                      const float* __restrict__ bst, int nx,
                                                      const float (&scale0)[MTP][4], bool scaled) {
-  __shared__ __attribute__((aligned(16))) char As0[16 * MTP * 128], As1[16 * MTP * 128];
-  __shared__ __attribute__((aligned(16))) int Sr[32];
+  __shared__ __attribute__((aligned(16))) char As0[kX3Sub * 16 * MTP * 128], As1[kX3Sub * 16 * MTP * 128];
+  __shared__ __attribute__((aligned(16))) int Sr[2 * kX3Cols];
   __shared__ SegDesc sd[3];
   if (sg.nch <= 0) return;
   if (threadIdx.x == 0) {
@@ -694,7 +731,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
   sg.S0 = sg.S1 = sg.S2 = src + src_off[d] + cbeg;
   sg.C0 = C - cbeg;
   sg.C1 = sg.C2 = 0;
-  sg.n0 = sg.n01 = sg.nch = (sg.C0 + 15) / 16;
+  sg.n0 = sg.n01 = sg.nch = x3_chunks(sg.C0);
   f32x4 are[MTP][NT], aim[MTP][NT];
 #pragma unroll
   for (int p = 0; p < MTP; ++p)
@@ -738,31 +775,34 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
 
 // The leaf kernel in split BF16 (as bleaf_kernel<float, XT, YT, ...>): one
 // workgroup per leaf, near field (scaled by qk2 when find_phi), leaf M2L and
-// leaf L2L, x term, rows stored once.
+// leaf L2L, x term, rows stored once (part 0). Part 1 is the near field and x
+// term alone (stored), part 2 the leaf passes alone (added to the stored rows),
+// so that the near field can run beside the upward and downward passes.
 template <class XS, class XT, class YT, int MTP, int NT>
 __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
     const char* __restrict__ T3, const int* __restrict__ near_src, int rank, const char* __restrict__ ops3,
     const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ op3_off,
     const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const XS* __restrict__ X,
     const float* __restrict__ bst, int nx, const XT* __restrict__ Xin, const double* __restrict__ qk2, int find_phi,
-    YT* __restrict__ Y) {
+    YT* __restrict__ Y, int part) {
   const int leaf = xcd_order(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i16 = lane & 15, g = lane >> 4;
   SegX3 sg;
   sg.A0 = T3;
   sg.S0 = near_src + (size_t)leaf * 9 * rank;
-  sg.C0 = 9 * rank;
-  const int d1 = leaf_dest[2 * leaf], d2 = leaf_dest[2 * leaf + 1];
+  sg.C0 = part == 2 ? 0 : 9 * rank;
+  const int d1 = part == 1 ? -1 : leaf_dest[2 * leaf], d2 = part == 1 ? -1 : leaf_dest[2 * leaf + 1];
   sg.A1 = d1 >= 0 ? ops3 + op3_off[d1] * 128 : T3;
   sg.S1 = d1 >= 0 ? src + src_off[d1] : sg.S0;
   sg.C1 = d1 >= 0 ? ncol[d1] : 0;
   sg.A2 = d2 >= 0 ? ops3 + op3_off[d2] * 128 : T3;
   sg.S2 = d2 >= 0 ? src + src_off[d2] : sg.S0;
   sg.C2 = d2 >= 0 ? ncol[d2] : 0;
-  sg.n0 = (sg.C0 + 15) / 16;
-  sg.n01 = sg.n0 + (sg.C1 + 15) / 16;
-  sg.nch = sg.n01 + (sg.C2 + 15) / 16;
+  if (part == 2 && sg.C1 + sg.C2 == 0) return;  // nothing to add
+  sg.n0 = x3_chunks(sg.C0);
+  sg.n01 = sg.n0 + x3_chunks(sg.C1);
+  sg.nch = sg.n01 + x3_chunks(sg.C2);
   f32x4 are[MTP][NT], aim[MTP][NT];
 #pragma unroll
   for (int p = 0; p < MTP; ++p)
@@ -776,7 +816,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
       const int r = 16 * p + 4 * g + v;
       sc[p][v] = (find_phi && r < rank) ? float(qk2[(size_t)leaf * rank + r]) : 1.0f;
     }
-  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0);
+  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0 && part != 2);
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
     const int q = 16 * (NT * w + c) + i16;
@@ -788,7 +828,10 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
         if (r >= rank) continue;
         const size_t o = ((size_t)leaf * rank + r) * kLD + q;
         double vr = double(are[p][c][v]), vi = double(aim[p][c][v]);
-        if (find_phi) {
+        if (part == 2) {
+          vr += double(Y[o]);
+          vi += double(Y[o + kNB]);
+        } else if (find_phi) {
           vr += double(Xin[o]);
           vi += double(Xin[o + kNB]);
         }
@@ -1438,6 +1481,7 @@ std::string BlockOperator::setup(DeviceOperator& o) {
 }
 
 void BlockOperator::release() {
+  notes.clear();
   work.release();
   outer.release();
   void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part, ops3, T3, d_op3};
@@ -1575,15 +1619,16 @@ static void launch_bpass_x3(BlockOperator& bo, int pidx, const XS* X, int nx, do
 
 // the split-BF16 leaf kernel
 template <class XS, class XT, class YT, int NT>
-static void launch_bleaf_x3_nt(BlockOperator& bo, const XS* Xr, int nx, const XT* X, YT* Y) {
+static void launch_bleaf_x3_nt(BlockOperator& bo, const XS* Xr, int nx, const XT* X, YT* Y, int part,
+                               hipStream_t s) {
   DeviceOperator& o = *bo.op;
   const int mtp = (o.rank + 15) / 16;
   const dim3 grid(o.n_leaves);
 #define LSG_X3L_CASE(M)                                                                                        \
   case M:                                                                                                      \
-    hipLaunchKernelGGL((bleaf_x3_kernel<XS, XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, o.stream, bo.T3,      \
+    hipLaunchKernelGGL((bleaf_x3_kernel<XS, XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, s, bo.T3,     \
                        bo.near_src, o.rank, bo.ops3, o.src, o.d_ncol, bo.d_op3, o.d_srcoff, bo.leaf_dest, Xr,     \
-                       bo.bstate32, nx, X, o.qk2, o.find_phi ? 1 : 0, Y);                                      \
+                       bo.bstate32, nx, X, o.qk2, o.find_phi ? 1 : 0, Y, part);                                \
     break;
   switch (mtp) {
     LSG_X3L_CASE(1)
@@ -1611,16 +1656,29 @@ static void block_matvec_x3(BlockOperator& bo, const XS* Xs, int nx, const XT* X
   double* Yd = y64 ? reinterpret_cast<double*>(Y) : nullptr;
   DeviceOperator& o = *bo.op;
   if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
-  auto leaf = [&]() {
-    if (bo.nt3 == 2) launch_bleaf_x3_nt<XS, XT, YT, 2>(bo, Xs, nx, X, Y);
-    else launch_bleaf_x3_nt<XS, XT, YT, 1>(bo, Xs, nx, X, Y);
+  auto leaf = [&](int part, hipStream_t s) {
+    if (bo.nt3 == 2) launch_bleaf_x3_nt<XS, XT, YT, 2>(bo, Xs, nx, X, Y, part, s);
+    else launch_bleaf_x3_nt<XS, XT, YT, 1>(bo, Xs, nx, X, Y, part, s);
   };
+  if (o.T && bo.leaf_fused && bo.x3_split) {
+    // the near field on the second stream beside the upward and downward
+    // passes, the leaf passes added after both
+    (void)hipEventRecord(o.ev_fork, o.stream);
+    (void)hipStreamWaitEvent(o.aux, o.ev_fork, 0);
+    leaf(1, o.aux);
+    for (int p : o.up) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    (void)hipEventRecord(o.ev_join, o.aux);
+    (void)hipStreamWaitEvent(o.stream, o.ev_join, 0);
+    leaf(2, o.stream);
+    return;
+  }
   for (int p : o.up) launch_bpass_x3(bo, p, Xs, nx, Yd);
   if (o.T && bo.leaf_fused) {
     for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
-    leaf();
+    leaf(0, o.stream);
   } else if constexpr (y64) {
-    if (o.T) leaf();
+    if (o.T) leaf(0, o.stream);
     for (int p : o.near) launch_bpass_x3(bo, p, Xs, nx, Yd);
     for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
     for (int p : o.leaf_m2l) launch_bpass_x3(bo, p, Xs, nx, Yd);
@@ -1764,6 +1822,8 @@ std::string BlockOperator::enable_fp32() {
     }
   }
   nt32 = best[1] < best[0] ? 2 : 1;
+  notes.push_back("FP32 block matvec: tiles per wavefront " + std::to_string(nt32) + " (1: " +
+                  std::to_string(best[0]) + " ms, 2: " + std::to_string(best[1]) + " ms)");
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   (void)hipFree(x);
@@ -1816,10 +1876,13 @@ std::string BlockOperator::enable_x3() {
   hipEvent_t a, b;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  float best[2] = {1e30f, 1e30f};
+  // and the near field beside the passes (x3_split) or after them
+  float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
   X3Scope x3_scope(*this, true);
-  for (int nt = 1; nt <= 2; ++nt) {
-    nt3 = nt;
+  for (int v = 0; v < 4; ++v) {
+    nt3 = 1 + (v & 1);
+    x3_split = v >= 2;
+    if (x3_split && !(o.T && leaf_fused)) continue;
     matvec32(x, y);
     for (int r = 0; r < 3; ++r) {
       (void)hipEventRecord(a, s);
@@ -1828,10 +1891,20 @@ std::string BlockOperator::enable_x3() {
       (void)hipEventSynchronize(b);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, a, b);
-      best[nt - 1] = std::min(best[nt - 1], ms);
+      best[v] = std::min(best[v], ms);
     }
   }
-  nt3 = best[1] < best[0] ? 2 : 1;
+  const int vb = int(std::min_element(best, best + 4) - best);
+  nt3 = 1 + (vb & 1);
+  x3_split = vb >= 2;
+  {
+    std::string n = "split-BF16 block matvec: tiles per wavefront " + std::to_string(nt3) +
+                    (x3_split ? ", near field beside the passes" : ", near field after the passes") + " (";
+    const char* lab[4] = {"1/after", "2/after", "1/beside", "2/beside"};
+    for (int v = 0; v < 4; ++v)
+      if (best[v] < 1e29f) n += std::string(v ? ", " : "") + lab[v] + ": " + std::to_string(best[v]) + " ms";
+    notes.push_back(n + ")");
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   (void)hipFree(x);

@@ -63,6 +63,8 @@ struct BlockOperator {
   char* T3 = nullptr;            // the near-field tables in the same layout
   long long* d_op3 = nullptr;    // per destination: its first 128-byte row in ops3
   int nt3 = 1;                   // right-hand-side tiles per wavefront (chosen by timing)
+  bool x3_split = false;         // near field on the second stream beside the passes (chosen by timing)
+  std::vector<std::string> notes;  // timed choices (tiles per wavefront, near field beside the passes)
   bool inner_x3 = true;          // the FP32 inner solves use the split-BF16 kernels
   bool x3_active = false;        // set while they run: FP32 block matvecs go to those kernels
   std::vector<int> pass_max_rows;  // largest destination of each pass (selects the row-tile count)

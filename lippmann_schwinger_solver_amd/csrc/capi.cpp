@@ -149,7 +149,7 @@ struct lsg_ctx {
   FieldInterp field;          // re-interpolation tables, set up on first use
   LearnedPrecond prec;
   std::vector<double> qk2_user_host;  // column 0 of lsg_set_contrast
-  std::string tuning;                 // last autotuner report
+  mutable std::string tuning;         // last autotuner report and the block operator's choices
   std::string err;
   double2* dx = nullptr;
   double2* dy = nullptr;
@@ -912,20 +912,30 @@ int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed) {
   std::string e = ctx->dev.autotune(reps, n_changed);
   if (e.empty() && had_graph) e = ctx->dev.capture_graph();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-  ctx->tuning.clear();
-  for (const std::string& t : ctx->dev.tuning) ctx->tuning += t + "\n";
   return LSG_OK;
 }
 
-const char* lsg_tuning(const lsg_ctx* ctx) { return ctx ? ctx->tuning.c_str() : ""; }
+const char* lsg_tuning(const lsg_ctx* ctx) {
+  if (!ctx) return "";
+  ctx->tuning.clear();
+  for (const std::string& t : ctx->dev.tuning) ctx->tuning += t + "\n";
+  for (const std::string& t : ctx->blk.notes) ctx->tuning += t + "\n";
+  return ctx->tuning.c_str();
+}
 
 int lsg_set_block_tiles(lsg_ctx* ctx, int tiles) {
-  if (!ctx || (tiles != 1 && tiles != 2)) return LSG_ERR_ARG;
+  if (!ctx || tiles < 1 || tiles > 4) return LSG_ERR_ARG;
   std::string e = ctx->block_ready();
   if (e.empty()) e = ctx->blk.enable_inner();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-  if (ctx->blk.inner_x3) ctx->blk.nt3 = tiles;
-  else ctx->blk.nt32 = tiles;
+  if (tiles > 2 && !(ctx->blk.inner_x3 && ctx->dev.T && ctx->blk.leaf_fused))
+    return ctx->set(LSG_ERR_ARG, "the near field runs beside the passes only in split BF16 on a uniform tree");
+  if (ctx->blk.inner_x3) {
+    ctx->blk.nt3 = 2 - (tiles & 1);
+    ctx->blk.x3_split = tiles > 2;
+  } else {
+    ctx->blk.nt32 = tiles;
+  }
   return LSG_OK;
 }
 
