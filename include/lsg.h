@@ -309,12 +309,12 @@ int lsg_set_block_tiles(lsg_ctx* ctx, int tiles);
  * LSG_SOLVE_FP32_FGMRES in lsg_gmres_batch), of lsg_matvec_fp32 with nrhs >= 2
  * and of lsg_time_matvec_block_fp32 (no reference counterpart). Both keep the
  * pass state in FP32 and the inner Krylov basis in FP16:
- *   LSG_ARITH_BF16X3 (default): operators and source values split as
- *     a = ah + al (bf16 pair), products ah bh + ah bl + al bh on BF16 MFMA
- *     with FP32 sums, 16 significant bits;
+ *   LSG_ARITH_SPLIT_BF16 (default): operators and source values split as
+ *     a = ah + al (bf16 pair, 16 significant bits), products (ah + al)(bh + bl)
+ *     as four BF16 MFMA terms with FP32 sums;
  *   LSG_ARITH_FP32: FP32 operators on FP32 MFMA, 24 significant bits. */
 #define LSG_ARITH_FP32 0
-#define LSG_ARITH_BF16X3 1
+#define LSG_ARITH_SPLIT_BF16 1
 int lsg_set_inner_arith(lsg_ctx* ctx, int arith);
 
 /* on = 1: capture the FP64 matvec (both streams) into a hipGraph, replayed by

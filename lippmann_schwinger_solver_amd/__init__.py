@@ -73,7 +73,7 @@ LSG_SOLVE_FP64 = 0
 LSG_SOLVE_FP32_IR = 1
 LSG_SOLVE_FP32_FGMRES = 2
 LSG_ARITH_FP32 = 0
-LSG_ARITH_BF16X3 = 1
+LSG_ARITH_SPLIT_BF16 = 1
 
 
 class lsg_solve_opts(C.Structure):
@@ -690,9 +690,10 @@ class DFMM:
 
     def set_inner_arith(self, arith):
         """Arithmetic of the FP32-inner batched solves and of matvec_block_fp32:
-        "bf16x3" (default; operators and sources split into bf16 pairs, BF16
-        MFMA, FP32 sums, 16 significant bits) or "fp32" (FP32 MFMA)."""
-        modes = {"fp32": LSG_ARITH_FP32, "bf16x3": LSG_ARITH_BF16X3}
+        "split_bf16" (default; operators and sources split into bf16 pairs, 16
+        significant bits, their products as four BF16 MFMA terms with FP32
+        sums) or "fp32" (FP32 MFMA)."""
+        modes = {"fp32": LSG_ARITH_FP32, "split_bf16": LSG_ARITH_SPLIT_BF16}
         if arith not in modes:
             raise ValueError(f"inner arithmetic must be one of {sorted(modes)}")
         self._check(self._lib.lsg_set_inner_arith(self._ctx, modes[arith]))

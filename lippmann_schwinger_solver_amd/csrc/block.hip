@@ -429,10 +429,10 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_kernel(const typena
 // ---------------------------------------------------------------------------
 // Split-BF16 block product (the inner solves' operator arithmetic): every FP32
 // value a is carried as a = ah + al, ah = bf16(a), al = bf16(a - ah) (16
-// significant bits), and a product as ah bh + ah bl + al bh on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (relative error about 2^-16;
-// the dropped al bl is below that). One MFMA takes K = 32 real columns, 16
-// complex ones: a chunk of the operator.
+// significant bits, the pair within 2^-17 |a| of a), and a product as
+// ah bh + ah bl + al bh + al bl on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation: the exact product of the two pairs, summed in FP32. One MFMA
+// takes K = 32 real columns, 16 complex ones: a sub-chunk of the operator.
 //
 // Real-ification with separate real and imaginary row tiles: for 16 complex
 // destination rows (row tile p) and complex columns c,
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_kernel(const typena
 // = four words) and B(the same four columns, right-hand side l % 16 of its
 // tile); the result lane l holds rows 16 p + 4 (l / 16) + v, v < 4.
 //
-// Operator layout (built by to_x3_kernel): per destination, chunks of 16
+// Operator layout (built by to_sb_kernel): per destination, chunks of 16
 // columns; a chunk is Rw rows of 128 bytes, the 16 columns' high words in
 // bytes 0..63 and their low words in 64..127, columns past C zero. In LDS the
 // eight 16-byte slots of row r are stored XOR-swizzled by r & 7, so the 16
@@ -465,13 +465,13 @@ __device__ __forceinline__ void split_pair(float a, float b, unsigned& hi, unsig
 }
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
-__global__ void to_x3_kernel(const double2* __restrict__ ops, const long long* __restrict__ op_off,
+__global__ void to_sb_kernel(const double2* __restrict__ ops, const long long* __restrict__ op_off,
                              const int* __restrict__ rows, const int* __restrict__ ncol,
-                             const long long* __restrict__ op3_off, unsigned* __restrict__ out) {
+                             const long long* __restrict__ opsb_off, unsigned* __restrict__ out) {
   const int d = blockIdx.x;
   const int Rw = rows[d], C = ncol[d], nk = (C + 15) / 16;
   const double2* A = ops + op_off[d];
-  unsigned* O = out + op3_off[d] * 32;
+  unsigned* O = out + opsb_off[d] * 32;
   for (int e = threadIdx.x; e < nk * Rw * 16; e += blockDim.x) {
     const int c16 = e & 15, rr = e >> 4;  // rr = chunk * Rw + row
     const int k = rr / Rw, r = rr - k * Rw, c = 16 * k + c16;
@@ -483,7 +483,7 @@ __global__ void to_x3_kernel(const double2* __restrict__ ops, const long long* _
   }
 }
 // the near-field tables as one block (rank rows, 9 rank columns)
-__global__ void to_x3_table_kernel(const double2* __restrict__ T, int Rw, int C, unsigned* __restrict__ out) {
+__global__ void to_sb_table_kernel(const double2* __restrict__ T, int Rw, int C, unsigned* __restrict__ out) {
   const int nk = (C + 15) / 16;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)nk * Rw * 16;
        e += (long long)gridDim.x * blockDim.x) {
@@ -500,7 +500,7 @@ __global__ void to_x3_table_kernel(const double2* __restrict__ T, int Rw, int C,
 
 // Operand stream of one workgroup in the split-BF16 layout: up to three
 // segments (operator block A_k, column indices S_k, C_k columns).
-struct SegX3 {
+struct SegSb {
   const char *A0, *A1, *A2;
   const int *S0, *S1, *S2;
   int C0, C1, C2;
@@ -509,12 +509,18 @@ struct SegX3 {
 
 // Sub-chunks of 16 columns (one MFMA k-step) per LDS chunk of the split-BF16
 // product: more columns per chunk put more loads in flight per barrier.
-#ifndef LSG_X3_SUB
-#define LSG_X3_SUB 2
+#ifndef LSG_SB_SUB
+#define LSG_SB_SUB 2
 #endif
-constexpr int kX3Sub = LSG_X3_SUB;
-constexpr int kX3Cols = 16 * kX3Sub;  // columns per chunk
-__host__ __device__ constexpr int x3_chunks(int C) { return (C + kX3Cols - 1) / kX3Cols; }
+constexpr int kSbSub = LSG_SB_SUB;
+// products per term: 4 (the exact product of the pairs) or 3 (al bl dropped:
+// 2^-16 more error per product, 24 more inner steps of 456 on C3 x 64 angles)
+#ifndef LSG_SB_TERMS
+#define LSG_SB_TERMS 4
+#endif
+constexpr int kSbTerms = LSG_SB_TERMS;
+constexpr int kSbCols = 16 * kSbSub;  // columns per chunk
+__host__ __device__ constexpr int sb_chunks(int C) { return (C + kSbCols - 1) / kSbCols; }
 
 // Block product of one workgroup on 64 right-hand sides in split BF16:
 // are/aim[p][c] (wave w: right-hand sides 16 (NT w + c) .. + 15) += the real
@@ -524,15 +530,15 @@ __host__ __device__ constexpr int x3_chunks(int C) { return (C + kX3Cols - 1) / 
 // into LDS by LDS-direct loads (double buffered), chunk i + 2's indices into a
 // second pair of buffers, chunk i + 1's source values into registers, while the
 // MFMAs of chunk i run; one wait and one barrier per chunk. A chunk holds up
-// to kX3Sub sub-chunks of 16 columns, stored in LDS as consecutive blocks of
+// to kSbSub sub-chunks of 16 columns, stored in LDS as consecutive blocks of
 // Rw rows (row rr of the chunk: sub-chunk rr / Rw).
 template <int MTP, int NT, class XS>
-__device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
+__device__ __forceinline__ void block_product_sb(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegSb& sg,
                                                  int Rw, const XS* __restrict__ X, const float* __restrict__ bst,
                                                  int nx, const float (&scale0)[MTP][4], bool scaled, char* As0,
                                                  char* As1, int* Sr, const SegDesc* sd) {
   constexpr int kBT = block_threads<NT>();
-  constexpr int KS = kX3Sub;
+  constexpr int KS = kSbSub;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int i16 = lane & 15, g = lane >> 4;
   struct Pick {
@@ -546,7 +552,7 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
     p.A = reinterpret_cast<const char*>(d.A);
     p.S = d.S;
     p.C = d.C;
-    p.c0 = (i - d.first) * kX3Cols;
+    p.c0 = (i - d.first) * kSbCols;
     p.nsub = min(KS, (d.C - p.c0 + 15) >> 4);
     return p;
   };
@@ -564,7 +570,7 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
   auto copy_idx = [=](int i, int* Sb) __attribute__((always_inline)) {
     if (w != 0 || i >= sg.nch) return;
     const Pick p = seg_of(i);
-    if (lane < kX3Cols) lds_copy4(p.S + min(p.c0 + lane, p.C - 1), Sb);
+    if (lane < kSbCols) lds_copy4(p.S + min(p.c0 + lane, p.C - 1), Sb);
   };
   // this lane's source values of chunk i: columns 16 s + 4 g .. + 3 of each
   // sub-chunk s, right-hand sides of its NT tiles (real and imaginary parts)
@@ -598,7 +604,7 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
   typedef float XBuf[NT][4 * KS][2];
   XBuf xa, xb;
   copy_idx(0, Sr);
-  copy_idx(1, Sr + kX3Cols);
+  copy_idx(1, Sr + kSbCols);
   copy_ops(0, As0);
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   __syncthreads();
@@ -610,8 +616,8 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
     const char* As = (i & 1) ? As1 : As0;
     const int nsub = seg_of(i).nsub;
     if (i + 1 < sg.nch) copy_ops(i + 1, (i & 1) ? As0 : As1);
-    copy_idx(i + 2, Sr + kX3Cols * (i & 1));
-    if (i + 1 < sg.nch) load_src(i + 1, Sr + kX3Cols * ((i + 1) & 1), xn);
+    copy_idx(i + 2, Sr + kSbCols * (i & 1));
+    if (i + 1 < sg.nch) load_src(i + 1, Sr + kSbCols * ((i + 1) & 1), xn);
 #pragma unroll
     for (int sb = 0; sb < KS; ++sb) {
       if (sb >= nsub) break;
@@ -644,6 +650,10 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
           aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, as_bf16x8(b2l[c]), aim[p][c], 0, 0, 0);
           are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b1h[c]), are[p][c], 0, 0, 0);
           aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b2h[c]), aim[p][c], 0, 0, 0);
+          if (kSbTerms == 4) {
+            are[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b1l[c]), are[p][c], 0, 0, 0);
+            aim[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, as_bf16x8(b2l[c]), aim[p][c], 0, 0, 0);
+          }
         }
       }
     }
@@ -690,12 +700,12 @@ __device__ __forceinline__ void block_product_x3(f32x4 (&are)[MTP][NT], f32x4 (&
 // LDS of one split-BF16 workgroup: two chunk buffers of up to 64 rows per
 // sub-chunk, the index buffers and the segment descriptors
 template <int MTP, int NT, class XS>
-__device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegX3& sg,
+__device__ __forceinline__ void block_product_sb_lds(f32x4 (&are)[MTP][NT], f32x4 (&aim)[MTP][NT], const SegSb& sg,
                                                      int Rw, const XS* __restrict__ X,
                                                      const float* __restrict__ bst, int nx,
                                                      const float (&scale0)[MTP][4], bool scaled) {
-  __shared__ __attribute__((aligned(16))) char As0[kX3Sub * 16 * MTP * 128], As1[kX3Sub * 16 * MTP * 128];
-  __shared__ __attribute__((aligned(16))) int Sr[2 * kX3Cols];
+  __shared__ __attribute__((aligned(16))) char As0[kSbSub * 16 * MTP * 128], As1[kSbSub * 16 * MTP * 128];
+  __shared__ __attribute__((aligned(16))) int Sr[2 * kSbCols];
   __shared__ SegDesc sd[3];
   if (sg.nch <= 0) return;
   if (threadIdx.x == 0) {
@@ -704,19 +714,19 @@ __device__ __forceinline__ void block_product_x3_lds(f32x4 (&are)[MTP][NT], f32x
     sd[2] = SegDesc{sg.A2, sg.S2, sg.C2, sg.n01};
   }
   __syncthreads();
-  block_product_x3<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, scale0, scaled, As0, As1, Sr, sd);
+  block_product_sb<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, scale0, scaled, As0, As1, Sr, sd);
 }
 
 // One pass on 64 right-hand sides in split BF16 (as bpass_kernel<ACC, float,
 // ...>): one workgroup per destination or per column segment of one; MTP row
-// tiles of 16 complex rows (rows of every destination <= 16 MTP); op3_off in
+// tiles of 16 complex rows (rows of every destination <= 16 MTP); opsb_off in
 // 128-byte rows of the split-BF16 operator; XS: type of the input block's
 // rows (the first nx source rows), FP32 or the inner solves' FP16 basis.
 template <bool ACC, int MTP, int NT, class XS>
-__global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
-    const char* __restrict__ ops3, const int* __restrict__ src, float* bst, const XS* __restrict__ X, double* Y,
+__global__ __launch_bounds__(block_threads<NT>()) void bpass_sb_kernel(
+    const char* __restrict__ ops_sb, const int* __restrict__ src, float* bst, const XS* __restrict__ X, double* Y,
     int N, const int* __restrict__ rows, const int* __restrict__ ncol, const int* __restrict__ dst,
-    const long long* __restrict__ op3_off, const long long* __restrict__ src_off, int nx, int kseg, int nseg,
+    const long long* __restrict__ opsb_off, const long long* __restrict__ src_off, int nx, int kseg, int nseg,
     int rt, float* __restrict__ part) {
   const int item = xcd_order(blockIdx.x, gridDim.x);
   const int d = kseg > 0 ? item / nseg : item;
@@ -726,19 +736,19 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
   const int cbeg = kseg > 0 ? min(Cfull, sgi * kseg) : 0;
   const int C = kseg > 0 ? min(Cfull, cbeg + kseg) : Cfull;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  SegX3 sg;
-  sg.A0 = sg.A1 = sg.A2 = ops3 + (op3_off[d] + (long long)(cbeg >> 4) * Rw) * 128;
+  SegSb sg;
+  sg.A0 = sg.A1 = sg.A2 = ops_sb + (opsb_off[d] + (long long)(cbeg >> 4) * Rw) * 128;
   sg.S0 = sg.S1 = sg.S2 = src + src_off[d] + cbeg;
   sg.C0 = C - cbeg;
   sg.C1 = sg.C2 = 0;
-  sg.n0 = sg.n01 = sg.nch = x3_chunks(sg.C0);
+  sg.n0 = sg.n01 = sg.nch = sb_chunks(sg.C0);
   f32x4 are[MTP][NT], aim[MTP][NT];
 #pragma unroll
   for (int p = 0; p < MTP; ++p)
 #pragma unroll
     for (int c = 0; c < NT; ++c) are[p][c] = aim[p][c] = f32x4{0, 0, 0, 0};
   const float none[MTP][4] = {};
-  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, none, false);
+  block_product_sb_lds<MTP, NT, XS>(are, aim, sg, Rw, X, bst, nx, none, false);
   const int i16 = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
@@ -779,30 +789,30 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_x3_kernel(
 // term alone (stored), part 2 the leaf passes alone (added to the stored rows),
 // so that the near field can run beside the upward and downward passes.
 template <class XS, class XT, class YT, int MTP, int NT>
-__global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
-    const char* __restrict__ T3, const int* __restrict__ near_src, int rank, const char* __restrict__ ops3,
-    const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ op3_off,
+__global__ __launch_bounds__(block_threads<NT>()) void bleaf_sb_kernel(
+    const char* __restrict__ T_sb, const int* __restrict__ near_src, int rank, const char* __restrict__ ops_sb,
+    const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ opsb_off,
     const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const XS* __restrict__ X,
     const float* __restrict__ bst, int nx, const XT* __restrict__ Xin, const double* __restrict__ qk2, int find_phi,
     YT* __restrict__ Y, int part) {
   const int leaf = xcd_order(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i16 = lane & 15, g = lane >> 4;
-  SegX3 sg;
-  sg.A0 = T3;
+  SegSb sg;
+  sg.A0 = T_sb;
   sg.S0 = near_src + (size_t)leaf * 9 * rank;
   sg.C0 = part == 2 ? 0 : 9 * rank;
   const int d1 = part == 1 ? -1 : leaf_dest[2 * leaf], d2 = part == 1 ? -1 : leaf_dest[2 * leaf + 1];
-  sg.A1 = d1 >= 0 ? ops3 + op3_off[d1] * 128 : T3;
+  sg.A1 = d1 >= 0 ? ops_sb + opsb_off[d1] * 128 : T_sb;
   sg.S1 = d1 >= 0 ? src + src_off[d1] : sg.S0;
   sg.C1 = d1 >= 0 ? ncol[d1] : 0;
-  sg.A2 = d2 >= 0 ? ops3 + op3_off[d2] * 128 : T3;
+  sg.A2 = d2 >= 0 ? ops_sb + opsb_off[d2] * 128 : T_sb;
   sg.S2 = d2 >= 0 ? src + src_off[d2] : sg.S0;
   sg.C2 = d2 >= 0 ? ncol[d2] : 0;
   if (part == 2 && sg.C1 + sg.C2 == 0) return;  // nothing to add
-  sg.n0 = x3_chunks(sg.C0);
-  sg.n01 = sg.n0 + x3_chunks(sg.C1);
-  sg.nch = sg.n01 + x3_chunks(sg.C2);
+  sg.n0 = sb_chunks(sg.C0);
+  sg.n01 = sg.n0 + sb_chunks(sg.C1);
+  sg.nch = sg.n01 + sb_chunks(sg.C2);
   f32x4 are[MTP][NT], aim[MTP][NT];
 #pragma unroll
   for (int p = 0; p < MTP; ++p)
@@ -816,7 +826,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bleaf_x3_kernel(
       const int r = 16 * p + 4 * g + v;
       sc[p][v] = (find_phi && r < rank) ? float(qk2[(size_t)leaf * rank + r]) : 1.0f;
     }
-  block_product_x3_lds<MTP, NT, XS>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0 && part != 2);
+  block_product_sb_lds<MTP, NT, XS>(are, aim, sg, rank, X, bst, nx, sc, find_phi != 0 && part != 2);
 #pragma unroll
   for (int c = 0; c < NT; ++c) {
     const int q = 16 * (NT * w + c) + i16;
@@ -1024,6 +1034,7 @@ __global__ __launch_bounds__(kT) void bupdate_h_kernel(const _Float16* __restric
   for (int r = 0; r < kUpdRows; ++r)
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc[r][e] = make_float2(0.f, 0.f);
+#pragma unroll 2
   for (int i = 0; i < nc; ++i) {
     float hr[4], hi[4];
 #pragma unroll
@@ -1484,11 +1495,11 @@ void BlockOperator::release() {
   notes.clear();
   work.release();
   outer.release();
-  void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part, ops3, T3, d_op3};
-  ops3 = nullptr;
-  T3 = nullptr;
-  d_op3 = nullptr;
-  x3_active = false;
+  void* ptrs[] = {bstate, leaf_dest, near_src, qk2_cols, bstate32, split_part, ops_sb, T_sb, d_opsb};
+  ops_sb = nullptr;
+  T_sb = nullptr;
+  d_opsb = nullptr;
+  sb_active = false;
   qk2_cols = nullptr;
   bstate32 = nullptr;
   split_part = nullptr;
@@ -1574,7 +1585,7 @@ static void launch_bpass(BlockOperator& bo, int pidx, const R* X, int nx, double
 
 // split-BF16 pass: MTP row tiles of 16 complex rows from the largest destination
 template <bool ACC, int NT, class XS>
-static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows, int nseg, const XS* X, int nx,
+static void launch_bpass_sb_nt(BlockOperator& bo, const DevPass& p, int max_rows, int nseg, const XS* X, int nx,
                                double* Y) {
   DeviceOperator& o = *bo.op;
   hipStream_t s = o.stream;
@@ -1584,60 +1595,60 @@ static void launch_bpass_x3_nt(BlockOperator& bo, const DevPass& p, int max_rows
   const int kseg = nseg > 1 ? kSplitCols : 0;
   const unsigned grid = unsigned(p.ndest) * unsigned(nseg > 1 ? nseg : 1);
   float* part = reinterpret_cast<float*>(bo.split_part);
-#define LSG_X3_CASE(M)                                                                                         \
+#define LSG_SB_CASE(M)                                                                                         \
   case M:                                                                                                      \
-    hipLaunchKernelGGL((bpass_x3_kernel<ACC, M, NT, XS>), dim3(grid), dim3(block_threads<NT>()), 0, s, bo.ops3, o.src, \
-                       bo.bstate32, X, Y, o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, bo.d_op3 + b,            \
+    hipLaunchKernelGGL((bpass_sb_kernel<ACC, M, NT, XS>), dim3(grid), dim3(block_threads<NT>()), 0, s, bo.ops_sb, o.src, \
+                       bo.bstate32, X, Y, o.N, o.d_rows + b, o.d_ncol + b, o.d_dst + b, bo.d_opsb + b,            \
                        o.d_srcoff + b, nx, kseg, nseg > 1 ? nseg : 1, rt, part);                                \
     break;
   switch (mtp) {
-    LSG_X3_CASE(1)
-    LSG_X3_CASE(2)
-    LSG_X3_CASE(3)
+    LSG_SB_CASE(1)
+    LSG_SB_CASE(2)
+    LSG_SB_CASE(3)
     default:
-      LSG_X3_CASE(4)
+      LSG_SB_CASE(4)
   }
-#undef LSG_X3_CASE
+#undef LSG_SB_CASE
   if (nseg > 1)
     hipLaunchKernelGGL((bsplit_reduce_kernel<ACC, float>), dim3(p.ndest), dim3(kT), 0, s, part, nseg, rt,
                        o.d_rows + b, o.d_dst + b, bo.bstate32, Y, o.N);
 }
 
 template <class XS>
-static void launch_bpass_x3(BlockOperator& bo, int pidx, const XS* X, int nx, double* Y) {
+static void launch_bpass_sb(BlockOperator& bo, int pidx, const XS* X, int nx, double* Y) {
   const DevPass& p = bo.op->passes[size_t(pidx)];
   const int mr = bo.pass_max_rows[size_t(pidx)];
   const int nseg = bo.pass_nseg[size_t(pidx)];
   if (p.kind == 0) {
-    if (bo.nt3 == 2) launch_bpass_x3_nt<false, 2>(bo, p, mr, nseg, X, nx, Y);
-    else launch_bpass_x3_nt<false, 1>(bo, p, mr, nseg, X, nx, Y);
+    if (bo.nt_sb == 2) launch_bpass_sb_nt<false, 2>(bo, p, mr, nseg, X, nx, Y);
+    else launch_bpass_sb_nt<false, 1>(bo, p, mr, nseg, X, nx, Y);
   } else {
-    if (bo.nt3 == 2) launch_bpass_x3_nt<true, 2>(bo, p, mr, nseg, X, nx, Y);
-    else launch_bpass_x3_nt<true, 1>(bo, p, mr, nseg, X, nx, Y);
+    if (bo.nt_sb == 2) launch_bpass_sb_nt<true, 2>(bo, p, mr, nseg, X, nx, Y);
+    else launch_bpass_sb_nt<true, 1>(bo, p, mr, nseg, X, nx, Y);
   }
 }
 
 // the split-BF16 leaf kernel
 template <class XS, class XT, class YT, int NT>
-static void launch_bleaf_x3_nt(BlockOperator& bo, const XS* Xr, int nx, const XT* X, YT* Y, int part,
+static void launch_bleaf_sb_nt(BlockOperator& bo, const XS* Xr, int nx, const XT* X, YT* Y, int part,
                                hipStream_t s) {
   DeviceOperator& o = *bo.op;
   const int mtp = (o.rank + 15) / 16;
   const dim3 grid(o.n_leaves);
-#define LSG_X3L_CASE(M)                                                                                        \
+#define LSG_SBL_CASE(M)                                                                                        \
   case M:                                                                                                      \
-    hipLaunchKernelGGL((bleaf_x3_kernel<XS, XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, s, bo.T3,     \
-                       bo.near_src, o.rank, bo.ops3, o.src, o.d_ncol, bo.d_op3, o.d_srcoff, bo.leaf_dest, Xr,     \
+    hipLaunchKernelGGL((bleaf_sb_kernel<XS, XT, YT, M, NT>), grid, dim3(block_threads<NT>()), 0, s, bo.T_sb,     \
+                       bo.near_src, o.rank, bo.ops_sb, o.src, o.d_ncol, bo.d_opsb, o.d_srcoff, bo.leaf_dest, Xr,     \
                        bo.bstate32, nx, X, o.qk2, o.find_phi ? 1 : 0, Y, part);                                \
     break;
   switch (mtp) {
-    LSG_X3L_CASE(1)
-    LSG_X3L_CASE(2)
-    LSG_X3L_CASE(3)
+    LSG_SBL_CASE(1)
+    LSG_SBL_CASE(2)
+    LSG_SBL_CASE(3)
     default:
-      LSG_X3L_CASE(4)
+      LSG_SBL_CASE(4)
   }
-#undef LSG_X3L_CASE
+#undef LSG_SBL_CASE
 }
 
 // FP32 copy of the input block (FP64 or FP16) into the first N rows of the FP32 pass state
@@ -1651,38 +1662,38 @@ __global__ void bround_kernel(const XT* __restrict__ X, float* __restrict__ bst3
 // passes and leaf kernel as the FP32 path; the input block's rows are read in
 // place (XS = XT, FP32 or FP16) or from its FP32 copy in the pass state (nx = 0).
 template <class XS, class XT, class YT>
-static void block_matvec_x3(BlockOperator& bo, const XS* Xs, int nx, const XT* X, YT* Y) {
+static void block_matvec_sb(BlockOperator& bo, const XS* Xs, int nx, const XT* X, YT* Y) {
   constexpr bool y64 = std::is_same<YT, double>::value;
   double* Yd = y64 ? reinterpret_cast<double*>(Y) : nullptr;
   DeviceOperator& o = *bo.op;
   if (!y64 && !(o.T && bo.can_store_f32())) return;  // not reached: callers check can_store_f32
   auto leaf = [&](int part, hipStream_t s) {
-    if (bo.nt3 == 2) launch_bleaf_x3_nt<XS, XT, YT, 2>(bo, Xs, nx, X, Y, part, s);
-    else launch_bleaf_x3_nt<XS, XT, YT, 1>(bo, Xs, nx, X, Y, part, s);
+    if (bo.nt_sb == 2) launch_bleaf_sb_nt<XS, XT, YT, 2>(bo, Xs, nx, X, Y, part, s);
+    else launch_bleaf_sb_nt<XS, XT, YT, 1>(bo, Xs, nx, X, Y, part, s);
   };
-  if (o.T && bo.leaf_fused && bo.x3_split) {
+  if (o.T && bo.leaf_fused && bo.sb_split) {
     // the near field on the second stream beside the upward and downward
     // passes, the leaf passes added after both
     (void)hipEventRecord(o.ev_fork, o.stream);
     (void)hipStreamWaitEvent(o.aux, o.ev_fork, 0);
     leaf(1, o.aux);
-    for (int p : o.up) launch_bpass_x3(bo, p, Xs, nx, Yd);
-    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.up) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
     (void)hipEventRecord(o.ev_join, o.aux);
     (void)hipStreamWaitEvent(o.stream, o.ev_join, 0);
     leaf(2, o.stream);
     return;
   }
-  for (int p : o.up) launch_bpass_x3(bo, p, Xs, nx, Yd);
+  for (int p : o.up) launch_bpass_sb(bo, p, Xs, nx, Yd);
   if (o.T && bo.leaf_fused) {
-    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
     leaf(0, o.stream);
   } else if constexpr (y64) {
     if (o.T) leaf(0, o.stream);
-    for (int p : o.near) launch_bpass_x3(bo, p, Xs, nx, Yd);
-    for (int p : o.down) launch_bpass_x3(bo, p, Xs, nx, Yd);
-    for (int p : o.leaf_m2l) launch_bpass_x3(bo, p, Xs, nx, Yd);
-    for (int p : o.leaf_l2l) launch_bpass_x3(bo, p, Xs, nx, Yd);
+    for (int p : o.near) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    for (int p : o.leaf_m2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    for (int p : o.leaf_l2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
   }
   if constexpr (y64) {
     if (bo.qk2_cols)
@@ -1705,13 +1716,13 @@ static void block_matvec_impl(BlockOperator& bo, const XT* X, YT* Y) {
   const C2* T = reinterpret_cast<const C2*>(std::is_same<R, double>::value ? (const void*)o.T : (const void*)o.T32);
   R* bst = reinterpret_cast<R*>(std::is_same<R, double>::value ? (void*)bo.bstate : (void*)bo.bstate32);
   if constexpr (std::is_same<R, float>::value) {
-    if (bo.x3_active) {
+    if (bo.sb_active) {
       if constexpr (std::is_same<XT, double>::value) {
         hipLaunchKernelGGL(bround_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X,
                            bo.bstate32, (long long)o.N * kLD);
-        block_matvec_x3<float, XT, YT>(bo, bo.bstate32, 0, X, Y);
+        block_matvec_sb<float, XT, YT>(bo, bo.bstate32, 0, X, Y);
       } else {
-        block_matvec_x3<XT, XT, YT>(bo, X, o.N, X, Y);
+        block_matvec_sb<XT, XT, YT>(bo, X, o.N, X, Y);
       }
       return;
     }
@@ -1831,10 +1842,10 @@ std::string BlockOperator::enable_fp32() {
   return "";
 }
 
-std::string BlockOperator::enable_x3() {
+std::string BlockOperator::enable_sb() {
   std::string e = enable_fp32();
   if (!e.empty()) return e;
-  if (ops3) return "";
+  if (ops_sb) return "";
   DeviceOperator& o = *op;
   const int64_t nd = o.passes.empty() ? 0 : o.passes.back().dest_base + o.passes.back().ndest;
   std::vector<int> rows_h(static_cast<size_t>(nd)), ncol_h(static_cast<size_t>(nd));
@@ -1848,18 +1859,18 @@ std::string BlockOperator::enable_x3() {
     off[size_t(d)] = total;
     total += (long long)rows_h[size_t(d)] * ((ncol_h[size_t(d)] + 15) / 16);
   }
-  HIP_OK(hipMalloc((void**)&ops3, size_t(std::max(total, 1LL)) * 128));
-  HIP_OK(hipMalloc((void**)&d_op3, sizeof(long long) * size_t(std::max<int64_t>(nd, 1))));
+  HIP_OK(hipMalloc((void**)&ops_sb, size_t(std::max(total, 1LL)) * 128));
+  HIP_OK(hipMalloc((void**)&d_opsb, sizeof(long long) * size_t(std::max<int64_t>(nd, 1))));
   if (nd) {
-    HIP_OK(hipMemcpy(d_op3, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(to_x3_kernel, dim3(unsigned(nd)), dim3(kT), 0, o.stream, o.ops, o.d_op, o.d_rows, o.d_ncol,
-                       d_op3, reinterpret_cast<unsigned*>(ops3));
+    HIP_OK(hipMemcpy(d_opsb, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_sb_kernel, dim3(unsigned(nd)), dim3(kT), 0, o.stream, o.ops, o.d_op, o.d_rows, o.d_ncol,
+                       d_opsb, reinterpret_cast<unsigned*>(ops_sb));
   }
   if (o.T) {
     const long long trows = (long long)o.rank * ((9 * o.rank + 15) / 16);
-    HIP_OK(hipMalloc((void**)&T3, size_t(trows) * 128));
-    hipLaunchKernelGGL(to_x3_table_kernel, dim3(blocks_for(trows * 16, kT)), dim3(kT), 0, o.stream, o.T, o.rank,
-                       9 * o.rank, reinterpret_cast<unsigned*>(T3));
+    HIP_OK(hipMalloc((void**)&T_sb, size_t(trows) * 128));
+    hipLaunchKernelGGL(to_sb_table_kernel, dim3(blocks_for(trows * 16, kT)), dim3(kT), 0, o.stream, o.T, o.rank,
+                       9 * o.rank, reinterpret_cast<unsigned*>(T_sb));
   }
   HIP_OK(hipStreamSynchronize(o.stream));
   // one or two right-hand-side tiles per wavefront: the faster on this operator
@@ -1876,13 +1887,13 @@ std::string BlockOperator::enable_x3() {
   hipEvent_t a, b;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  // and the near field beside the passes (x3_split) or after them
+  // and the near field beside the passes (sb_split) or after them
   float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-  X3Scope x3_scope(*this, true);
+  SbScope sb_scope(*this, true);
   for (int v = 0; v < 4; ++v) {
-    nt3 = 1 + (v & 1);
-    x3_split = v >= 2;
-    if (x3_split && !(o.T && leaf_fused)) continue;
+    nt_sb = 1 + (v & 1);
+    sb_split = v >= 2;
+    if (sb_split && !(o.T && leaf_fused)) continue;
     matvec32(x, y);
     for (int r = 0; r < 3; ++r) {
       (void)hipEventRecord(a, s);
@@ -1895,11 +1906,11 @@ std::string BlockOperator::enable_x3() {
     }
   }
   const int vb = int(std::min_element(best, best + 4) - best);
-  nt3 = 1 + (vb & 1);
-  x3_split = vb >= 2;
+  nt_sb = 1 + (vb & 1);
+  sb_split = vb >= 2;
   {
-    std::string n = "split-BF16 block matvec: tiles per wavefront " + std::to_string(nt3) +
-                    (x3_split ? ", near field beside the passes" : ", near field after the passes") + " (";
+    std::string n = "split-BF16 block matvec: tiles per wavefront " + std::to_string(nt_sb) +
+                    (sb_split ? ", near field beside the passes" : ", near field after the passes") + " (";
     const char* lab[4] = {"1/after", "2/after", "1/beside", "2/beside"};
     for (int v = 0; v < 4; ++v)
       if (best[v] < 1e29f) n += std::string(v ? ", " : "") + lab[v] + ": " + std::to_string(best[v]) + " ms";
@@ -2052,7 +2063,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     }
   }
   // the FP32 block matvecs of this solve run the split-BF16 kernels when asked
-  X3Scope x3_scope(bo, bo.x3_active);
+  SbScope sb_scope(bo, bo.sb_active);
   if (fp32) {
     std::string e = bo.enable_inner();
     if (!e.empty()) {
@@ -2061,7 +2072,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
       res.error = e;
       return res;
     }
-    bo.x3_active = bo.inner_x3;
+    bo.sb_active = bo.inner_sb;
   }
   auto amv = [&](const auto* xin, double* yout) {
     typedef std::remove_cv_t<std::remove_pointer_t<decltype(xin)>> XT;

@@ -57,16 +57,16 @@ struct BlockOperator {
   // 2), chosen by timing when the FP32 operator is enabled
   int nt32 = 1;
   // split-BF16 operator of the inner solves (a = ah + al in bf16, products
-  // ah bh + ah bl + al bh on BF16 MFMA with FP32 sums; 16 significant bits),
-  // built by enable_x3: per destination, chunks of 16 columns of 128-byte rows
-  char* ops3 = nullptr;
-  char* T3 = nullptr;            // the near-field tables in the same layout
-  long long* d_op3 = nullptr;    // per destination: its first 128-byte row in ops3
-  int nt3 = 1;                   // right-hand-side tiles per wavefront (chosen by timing)
-  bool x3_split = false;         // near field on the second stream beside the passes (chosen by timing)
+  // (ah + al)(bh + bl) as four BF16 MFMA terms with FP32 sums; 16 significant bits),
+  // built by enable_sb: per destination, chunks of 16 columns of 128-byte rows
+  char* ops_sb = nullptr;
+  char* T_sb = nullptr;            // the near-field tables in the same layout
+  long long* d_opsb = nullptr;    // per destination: its first 128-byte row in ops_sb
+  int nt_sb = 1;                   // right-hand-side tiles per wavefront (chosen by timing)
+  bool sb_split = false;         // near field on the second stream beside the passes (chosen by timing)
   std::vector<std::string> notes;  // timed choices (tiles per wavefront, near field beside the passes)
-  bool inner_x3 = true;          // the FP32 inner solves use the split-BF16 kernels
-  bool x3_active = false;        // set while they run: FP32 block matvecs go to those kernels
+  bool inner_sb = true;          // the FP32 inner solves use the split-BF16 kernels
+  bool sb_active = false;        // set while they run: FP32 block matvecs go to those kernels
   std::vector<int> pass_max_rows;  // largest destination of each pass (selects the row-tile count)
   std::vector<int> pass_nseg;      // column segments per destination (1: not split)
   void* split_part = nullptr;      // partial products of split passes
@@ -74,9 +74,9 @@ struct BlockOperator {
   BlockWork outer;    // the FP64 outer iteration of block_fgmres
   std::string setup(DeviceOperator& o);
   std::string enable_fp32();  // FP32 operators (DeviceOperator::enable_fp32) and pass state
-  std::string enable_x3();    // the split-BF16 operator (after enable_fp32)
-  // the inner solves' arithmetic: enable_x3 or enable_fp32 (inner_x3)
-  std::string enable_inner() { return inner_x3 ? enable_x3() : enable_fp32(); }
+  std::string enable_sb();    // the split-BF16 operator (after enable_fp32)
+  // the inner solves' arithmetic: enable_sb or enable_fp32 (inner_sb)
+  std::string enable_inner() { return inner_sb ? enable_sb() : enable_fp32(); }
   void release();
   // Y = A X for one block (row-split, nb columns), on op->stream
   void matvec(const double* X, double* Y);
@@ -93,12 +93,12 @@ struct BlockOperator {
   bool can_store_f32() const { return op && leaf_fused && !qk2_cols; }
 };
 
-// x3_active set to `on` for the lifetime of the object, then restored
-struct X3Scope {
+// sb_active set to `on` for the lifetime of the object, then restored
+struct SbScope {
   BlockOperator& bo;
   bool prev;
-  X3Scope(BlockOperator& b, bool on) : bo(b), prev(b.x3_active) { bo.x3_active = on; }
-  ~X3Scope() { bo.x3_active = prev; }
+  SbScope(BlockOperator& b, bool on) : bo(b), prev(b.sb_active) { bo.sb_active = on; }
+  ~SbScope() { bo.sb_active = prev; }
 };
 
 // Batched restarted GMRES: nb independent systems A x_q = b_q advanced in
@@ -117,7 +117,7 @@ struct BatchResult {
 // B, X: N x nb row-split device blocks (X holds the initial guesses); only the
 // first nrhs columns are solved, the others must be zero.
 // fp32: the block matvecs use the FP32 pass state with split-BF16 MFMA
-// operators (bo.inner_x3) or FP32 operators and FP32 MFMA, and the Krylov basis
+// operators (bo.inner_sb) or FP32 operators and FP32 MFMA, and the Krylov basis
 // is stored in FP16; tol_per (host, nrhs):
 // per-system tolerances instead of tol; allow_zero: zero right-hand sides are
 // accepted (their systems are converged with x unchanged).

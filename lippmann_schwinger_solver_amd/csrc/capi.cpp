@@ -675,7 +675,7 @@ static int time_block(lsg_ctx* ctx, int reps, double* ms, bool fp32) {
   std::string e = ctx->block_ready();
   if (e.empty() && fp32) e = ctx->blk.enable_inner();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-  X3Scope x3_scope(ctx->blk, fp32 && ctx->blk.inner_x3);
+  SbScope sb_scope(ctx->blk, fp32 && ctx->blk.inner_sb);
   auto mv = [&]() {
     if (fp32) ctx->blk.matvec32(ctx->bx, ctx->by);
     else ctx->blk.matvec(ctx->bx, ctx->by);
@@ -727,7 +727,7 @@ int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t 
     e = ctx->block_ready();
     if (e.empty()) e = ctx->blk.enable_inner();
     if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-    X3Scope x3_scope(ctx->blk, ctx->blk.inner_x3);
+    SbScope sb_scope(ctx->blk, ctx->blk.inner_sb);
     hipStream_t s = ctx->dev.stream;
     for (int r0 = 0; r0 < nrhs; r0 += kBlockRhs) {
       const int nr = std::min(kBlockRhs, nrhs - r0);
@@ -928,11 +928,11 @@ int lsg_set_block_tiles(lsg_ctx* ctx, int tiles) {
   std::string e = ctx->block_ready();
   if (e.empty()) e = ctx->blk.enable_inner();
   if (!e.empty()) return ctx->set(LSG_ERR_DEVICE, e);
-  if (tiles > 2 && !(ctx->blk.inner_x3 && ctx->dev.T && ctx->blk.leaf_fused))
+  if (tiles > 2 && !(ctx->blk.inner_sb && ctx->dev.T && ctx->blk.leaf_fused))
     return ctx->set(LSG_ERR_ARG, "the near field runs beside the passes only in split BF16 on a uniform tree");
-  if (ctx->blk.inner_x3) {
-    ctx->blk.nt3 = 2 - (tiles & 1);
-    ctx->blk.x3_split = tiles > 2;
+  if (ctx->blk.inner_sb) {
+    ctx->blk.nt_sb = 2 - (tiles & 1);
+    ctx->blk.sb_split = tiles > 2;
   } else {
     ctx->blk.nt32 = tiles;
   }
@@ -940,8 +940,8 @@ int lsg_set_block_tiles(lsg_ctx* ctx, int tiles) {
 }
 
 int lsg_set_inner_arith(lsg_ctx* ctx, int arith) {
-  if (!ctx || (arith != LSG_ARITH_FP32 && arith != LSG_ARITH_BF16X3)) return LSG_ERR_ARG;
-  ctx->blk.inner_x3 = arith == LSG_ARITH_BF16X3;
+  if (!ctx || (arith != LSG_ARITH_FP32 && arith != LSG_ARITH_SPLIT_BF16)) return LSG_ERR_ARG;
+  ctx->blk.inner_sb = arith == LSG_ARITH_SPLIT_BF16;
   return LSG_OK;
 }
 

@@ -100,7 +100,7 @@ def test_fp32_block_matvec_and_batched_refinement():
         assert H.relerr(X[:, j], X64[:, j]) < 1e-6
 
 
-@pytest.mark.parametrize("m,K,arith", [(24, 30, "bf16x3"), (8, 40, "bf16x3"), (24, 30, "fp32")])
+@pytest.mark.parametrize("m,K,arith", [(24, 30, "split_bf16"), (8, 40, "split_bf16"), (24, 30, "fp32")])
 def test_batched_flexible_gmres_fp32_inner(m, K, arith):
     """FP64 flexible GMRES(K) of the batch, each outer step preconditioned by
     one FP32-inner batched GMRES(m) solve (split-BF16 or FP32 MFMA operators,
@@ -155,14 +155,15 @@ def test_fp32_block_matvec_tiles_per_wave(depth, kappa):
 
 @pytest.mark.parametrize("depth,kappa,find_phi", [(4, 30.0, 1), (5, 40.0, 1), (4, 30.0, 0)])
 def test_split_bf16_block_matvec(depth, kappa, find_phi):
-    """Split-BF16 block operator (each FP32 value as a bf16 pair, three BF16
-    MFMA products, FP32 sums) against the FP64 block matvec on all 64 columns,
+    """Split-BF16 block operator (each FP32 value as a bf16 pair, the pairs'
+    products as four BF16 MFMA terms, FP32 sums) against the FP64 block matvec
+    on all 64 columns,
     with one and with two right-hand-side tiles per wavefront (the same sums in
     the same order: equal results), and with the near field beside the passes
     (one more FP32 rounding: within 1e-5). The operator amplifies arithmetic error
     about 80-fold (the FP32 block operator, unit roundoff 6e-8, lands near 5e-6
-    here); the split product carries about 2^-16 per product, which puts it
-    near 2e-4: the bound is 1e-3, and it must be less accurate than FP32 (the
+    here); the split values carry about 2^-17 each, which puts it near 1e-4:
+    the bound is 1e-3, and it must be less accurate than FP32 (the
     two arithmetics really differ)."""
     S = make(H.inputs(depth=depth, kappa=kappa, q=6, seed=2, find_phi=find_phi))
     print(f"case depth={depth} kappa={kappa} find_phi={find_phi}", flush=True)
@@ -170,7 +171,7 @@ def test_split_bf16_block_matvec(depth, kappa, find_phi):
     Y64 = S.matvec_block(X)
     S.set_inner_arith("fp32")
     Y32 = S.matvec_block_fp32(X)
-    S.set_inner_arith("bf16x3")
+    S.set_inner_arith("split_bf16")
     ys = {}
     for nt in (1, 2, 3, 4):
         S.set_block_tiles(nt)
