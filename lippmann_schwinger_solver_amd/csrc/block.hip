@@ -15,8 +15,14 @@ constexpr int kT = 256;
 constexpr int kNB = kBlockRhs;      // 64 right-hand sides = 4 column tiles of 16
 constexpr int kLD = 2 * kNB;        // doubles per block row
 constexpr int kBMaxRows = 64;       // complex rows per destination (8 row tiles of 16 reals)
-constexpr int kSplitCols = 256;     // columns per wavefront in split passes
-constexpr int kSplitMaxDest = 2048; // passes with fewer destinations (and longer rows) are split
+#ifndef LSG_SPLIT_COLS
+#define LSG_SPLIT_COLS 256
+#endif
+#ifndef LSG_SPLIT_MAX_DEST
+#define LSG_SPLIT_MAX_DEST 2048
+#endif
+constexpr int kSplitCols = LSG_SPLIT_COLS;        // columns per workgroup in split passes
+constexpr int kSplitMaxDest = LSG_SPLIT_MAX_DEST; // passes with fewer destinations (and longer rows) are split
 
 #define HIP_OK(expr)                                                              \
   do {                                                                            \
@@ -1034,7 +1040,6 @@ __global__ __launch_bounds__(kT) void bupdate_h_kernel(const _Float16* __restric
   for (int r = 0; r < kUpdRows; ++r)
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc[r][e] = make_float2(0.f, 0.f);
-#pragma unroll 2
   for (int i = 0; i < nc; ++i) {
     float hr[4], hi[4];
 #pragma unroll
