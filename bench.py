@@ -150,10 +150,10 @@ def batched_leg(S, info, args, st_plain, plain_sps, restart=50, restart32=30):
     batched GMRES (one FP64 MFMA block matvec per Arnoldi step); the right-hand
     sides are resident in HBM before timing; every system's FP64 true residual
     must reach tol. `restart` is the FP64 batched restart length, `restart32`
-    the FP32 inner solves' (one cycle per refinement): the Gram-Schmidt of 64
-    systems streams 64 basis vectors per column, so the FP32-inner batch
-    prefers shorter cycles (restart sweeps are recorded with the workload
-    results under profiles/)."""
+    the inner solves' (FP32 pass state, split-BF16 operators, FP16 basis; one
+    cycle per refinement): the Gram-Schmidt of 64 systems streams 64 basis
+    vectors per column, so the low-precision batch prefers shorter cycles
+    (restart sweeps are recorded with the workload results under profiles/)."""
     angles = 2 * np.pi * np.arange(64) / 64
     Bh = np.ascontiguousarray(S.rhs_angles(angles).T)      # 64 x N: column q at q*N
     Bd, Xd = DeviceBlock(Bh), DeviceBlock(np.zeros_like(Bh))
@@ -172,7 +172,7 @@ def batched_leg(S, info, args, st_plain, plain_sps, restart=50, restart32=30):
         if st["status"] != 0 or st["converged"] != 64 or not rel.max() <= args.tol:
             raise RuntimeError(f"batched solve did not verify: {st}")
     X = Xd.download().T
-    # the same batch with the FP32 block operator inside FP64 refinement
+    # the same batch with the low-precision inner solves inside FP64 refinement
     Xd.zero()
     S.gmres_batch(None, device_ptrs=ptrs, nrhs=64, ld=S.N, restart_len=restart32, max_restarts=1, tol=args.tol,
                   mode="fp32ir", inner_tol=1e-5, max_outer=40)   # warm-up

@@ -375,8 +375,15 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_kernel(const typena
 // -1 for none; rank rows at N + leaf rank) accumulated on top, and the rows
 // stored once: Y = (find_phi ? X : 0) + qk2 .* (T x) + leaf passes (in FP64,
 // or rounded to FP32 for the FP32 inner solves' Krylov product).
+// Waves per SIMD asked of the FP64/FP32 leaf kernel with one right-hand-side
+// tile per wavefront: 4 keeps it in 120 registers without spills (three waves
+// without the request; C3 FP64 block matvec 5.68 ms against 5.99 ms).
+#ifndef LSG_LEAF_WAVES
+#define LSG_LEAF_WAVES 4
+#endif
 template <class R, class XT, class YT, int MT, int NT>
-__global__ __launch_bounds__(block_threads<NT>()) void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
+__global__ __attribute__((amdgpu_waves_per_eu(NT == 1 ? LSG_LEAF_WAVES : 1))) __launch_bounds__(block_threads<NT>())
+void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
                                                    const int* __restrict__ near_src, int rank,
                                                    const typename Mfma<R>::c2* __restrict__ ops,
                                                    const int* __restrict__ src, const int* __restrict__ ncol,
@@ -728,8 +735,16 @@ __device__ __forceinline__ void block_product_sb_lds(f32x4 (&are)[MTP][NT], f32x
 // tiles of 16 complex rows (rows of every destination <= 16 MTP); opsb_off in
 // 128-byte rows of the split-BF16 operator; XS: type of the input block's
 // rows (the first nx source rows), FP32 or the inner solves' FP16 basis.
+// Waves per SIMD asked of the split-BF16 kernels with one right-hand-side tile
+// per wavefront: the compiler then keeps the accumulators in VGPRs and fits the
+// leaf kernel in 128 registers without spills, four waves per SIMD (two
+// without the request; C3 block matvec 1.75 ms against 2.20 ms).
+#ifndef LSG_SB_WAVES
+#define LSG_SB_WAVES 4
+#endif
+#define LSG_SB_ATTR __attribute__((amdgpu_waves_per_eu(NT == 1 ? LSG_SB_WAVES : 1)))
 template <bool ACC, int MTP, int NT, class XS>
-__global__ __launch_bounds__(block_threads<NT>()) void bpass_sb_kernel(
+__global__ LSG_SB_ATTR __launch_bounds__(block_threads<NT>()) void bpass_sb_kernel(
     const char* __restrict__ ops_sb, const int* __restrict__ src, float* bst, const XS* __restrict__ X, double* Y,
     int N, const int* __restrict__ rows, const int* __restrict__ ncol, const int* __restrict__ dst,
     const long long* __restrict__ opsb_off, const long long* __restrict__ src_off, int nx, int kseg, int nseg,
@@ -795,7 +810,7 @@ __global__ __launch_bounds__(block_threads<NT>()) void bpass_sb_kernel(
 // term alone (stored), part 2 the leaf passes alone (added to the stored rows),
 // so that the near field can run beside the upward and downward passes.
 template <class XS, class XT, class YT, int MTP, int NT>
-__global__ __launch_bounds__(block_threads<NT>()) void bleaf_sb_kernel(
+__global__ LSG_SB_ATTR __launch_bounds__(block_threads<NT>()) void bleaf_sb_kernel(
     const char* __restrict__ T_sb, const int* __restrict__ near_src, int rank, const char* __restrict__ ops_sb,
     const int* __restrict__ src, const int* __restrict__ ncol, const long long* __restrict__ opsb_off,
     const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const XS* __restrict__ X,
@@ -954,7 +969,11 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
 // Each thread sums its 16 rows in FP32 (the basis is FP16: these sums are far
 // below its rounding), the quarter-waves, wavefronts and row blocks in FP64.
 // Same partial layout as bdot_kernel.
-__global__ __launch_bounds__(kT) void bdot_h_kernel(const _Float16* __restrict__ V, long long vs, int nc, int ncx,
+// (a request of 4 waves spills 24 registers and made the C3 batch 10% slower)
+#ifndef LSG_BDOT_WAVES
+#define LSG_BDOT_WAVES 1
+#endif
+__global__ __attribute__((amdgpu_waves_per_eu(LSG_BDOT_WAVES))) __launch_bounds__(kT) void bdot_h_kernel(const _Float16* __restrict__ V, long long vs, int nc, int ncx,
                                                     const float* __restrict__ w, int N, double2* __restrict__ partial) {
   typedef _Float16 h4 __attribute__((ext_vector_type(4)));
   __shared__ double2 red[kT / 64][kBCols][kNB];
@@ -966,6 +985,12 @@ __global__ __launch_bounds__(kT) void bdot_h_kernel(const _Float16* __restrict__
   for (int u = 0; u < kBCols; ++u)
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc[u][e] = make_float2(0.f, 0.f);
+#ifndef LSG_BDOT_UNROLL
+#define LSG_BDOT_UNROLL 1
+#endif
+#if LSG_BDOT_UNROLL == 2
+#pragma unroll 2
+#endif
   for (int k = 0; k < kBRows / 16; ++k) {
     const long long n = n0 + 16 * k + 4 * wv + rs;
     if (n >= N) break;
