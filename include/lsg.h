@@ -282,7 +282,10 @@ int lsg_gmres_batch(lsg_ctx* ctx, const double* b, double* x, int nrhs, int64_t 
  * uniform tree the FP64 MFMA kernel and the FMA kernel with 1 or 2 leaves per
  * thread, on this operator with HIP events (best of `reps`); keep the
  * fastest. n_changed (may be NULL) receives the number of changed choices;
- * lsg_tuning() returns one line per decision. */
+ * lsg_tuning() returns one line per decision, followed by the block
+ * operators' timed choices once they are enabled (tiles per wavefront, the
+ * near field beside or after the passes); the string stays valid until the
+ * next lsg_tuning() call on ctx. */
 int lsg_autotune(lsg_ctx* ctx, int reps, int* n_changed);
 const char* lsg_tuning(const lsg_ctx* ctx);
 
