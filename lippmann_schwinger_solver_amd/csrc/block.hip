@@ -1712,19 +1712,20 @@ static void block_matvec_sb(BlockOperator& bo, const XS* Xs, int nx, const XT* X
     (void)hipEventRecord(o.ev_join, o.aux);
     (void)hipStreamWaitEvent(o.stream, o.ev_join, 0);
     leaf(2, o.stream);
-    return;
+  } else {
+    for (int p : o.up) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    if (o.T && bo.leaf_fused) {
+      for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
+      leaf(0, o.stream);
+    } else if constexpr (y64) {
+      if (o.T) leaf(0, o.stream);
+      for (int p : o.near) launch_bpass_sb(bo, p, Xs, nx, Yd);
+      for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
+      for (int p : o.leaf_m2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
+      for (int p : o.leaf_l2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
+    }
   }
-  for (int p : o.up) launch_bpass_sb(bo, p, Xs, nx, Yd);
-  if (o.T && bo.leaf_fused) {
-    for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
-    leaf(0, o.stream);
-  } else if constexpr (y64) {
-    if (o.T) leaf(0, o.stream);
-    for (int p : o.near) launch_bpass_sb(bo, p, Xs, nx, Yd);
-    for (int p : o.down) launch_bpass_sb(bo, p, Xs, nx, Yd);
-    for (int p : o.leaf_m2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
-    for (int p : o.leaf_l2l) launch_bpass_sb(bo, p, Xs, nx, Yd);
-  }
+  // a contrast per right-hand side (volume-potential context): Y = X + qk2 .* Y
   if constexpr (y64) {
     if (bo.qk2_cols)
       hipLaunchKernelGGL(bdiag_kernel<XT>, dim3(blocks_for((long long)o.N * kLD, kT)), dim3(kT), 0, o.stream, X, Yd,

@@ -160,3 +160,32 @@ def test_uniform_only_features_rejected(readme):
     S, _ = readme
     with pytest.raises(L.LsgError):
         S.interpolate(np.zeros(S.N, complex), np.zeros(1), np.zeros(1))
+
+
+@pytest.mark.parametrize("arith", ["split_bf16", "fp32"])
+def test_low_precision_batch_on_adaptive_tree(readme, arith):
+    """The inner solves on an adaptive tree, where the leaf kernel does not hold
+    the leaf passes (near field as a pass of stored operators, FP64 product
+    rows, the FP16 basis read by the passes): 8 plane waves by FP32-inner
+    flexible GMRES, every system to 1e-8 in FP64, two of them checked through
+    the oracle's operator; the block matvec in that arithmetic against FP64.
+    This operator amplifies arithmetic error more than the uniform ones (its
+    FP32 single-vector matvec sits near 1e-5 here, about 170 FP32 roundings):
+    split BF16, with 2^-17 per value, lands near 1e-3, FP32 MFMA near 1e-5."""
+    S, o = readme
+    print(f"case arith={arith}", flush=True)
+    S.set_inner_arith(arith)
+    X = np.stack([H.rand_vec(S.N, 30 + j) for j in range(3)], axis=1)
+    e = max(H.relerr(a, b) for a, b in zip(S.matvec_block_fp32(X).T, S.matvec_block(X).T))
+    bound = 5e-3 if arith == "split_bf16" else 1e-4
+    angles = 2 * np.pi * np.arange(8) / 8
+    B = S.rhs_angles(angles)
+    Xs, st, rel = S.gmres_batch(B, restart_len=20, max_restarts=1, tol=1e-8, mode="fp32fgmres", inner_tol=1e-5,
+                                max_outer=30)
+    print(f"{arith}: block vs FP64 {e:.2e}; batch {st}", flush=True)
+    S.set_inner_arith("split_bf16")
+    assert e < bound
+    assert st["status"] == 0 and st["converged"] == 8 and rel.max() <= 1e-8
+    for j in (0, 5):
+        assert H.relerr(S.MatVecProduct(Xs[:, j]), B[:, j]) <= 1.05e-8
+        assert H.relerr(o.matvec(Xs[:, j]), B[:, j]) <= 1.05e-8

@@ -187,3 +187,44 @@ def test_split_bf16_block_matvec(depth, kappa, find_phi):
     for nt in (3, 4):
         assert max(H.relerr(ys[nt][:, j], ys[1][:, j]) for j in range(64)) < 1e-5
     S.close()
+
+
+def test_low_precision_batch_with_contrast_per_column():
+    """A volume-potential context with a different contrast per right-hand side
+    (A_q = I + diag(qk2[:, q]) V): the low-precision block operator applies the
+    contrasts after its product (FP64 product rows and a diagonal pass) in every
+    layout (FP32 MFMA; split BF16 with one or two tiles per wavefront, the near
+    field after or beside the passes), checked column by column against the
+    FP64 block matvec; the batch then needs no more inner steps than with FP32
+    MFMA plus a margin, and every system reaches 1e-8 in FP64 through its own
+    operator. (Flexible GMRES converges even around a wrong inner operator, only
+    slowly: the step count and the matvec check are what catch one.)"""
+    S = make(H.inputs(depth=4, kappa=20.0, q=6, seed=0, find_phi=0))
+    x, y = S.gridPoints[:, 0], S.gridPoints[:, 1]
+    qk2 = np.stack([(20.0 ** 2) * (0.5 + 0.1 * j) * np.exp(-((x - 0.05 * j) ** 2 + y ** 2) / 0.02)
+                    for j in range(6)], axis=1)
+    S.set_contrast(qk2)
+    Xr = np.stack([H.rand_vec(S.N, 50 + j) for j in range(6)], axis=1)
+    Y64 = S.matvec_block(Xr)
+    S.set_inner_arith("fp32")
+    e32 = max(H.relerr(a, b) for a, b in zip(S.matvec_block_fp32(Xr).T, Y64.T))
+    B = np.stack([-qk2[:, j] * np.exp(1j * 20.0 * (x * np.cos(j) + y * np.sin(j))) for j in range(6)], axis=1)
+    _, s32, _ = S.gmres_batch(B, restart_len=20, max_restarts=1, tol=1e-8, mode="fp32fgmres", inner_tol=1e-5,
+                              max_outer=30)
+    S.set_inner_arith("split_bf16")
+    esb = {}
+    for t in (1, 2, 3, 4):
+        S.set_block_tiles(t)
+        esb[t] = max(H.relerr(a, b) for a, b in zip(S.matvec_block_fp32(Xr).T, Y64.T))
+    X, st, rel = S.gmres_batch(B, restart_len=20, max_restarts=1, tol=1e-8, mode="fp32fgmres", inner_tol=1e-5,
+                               max_outer=30)
+    print(f"per-column contrast: block FP32 {e32:.2e}, split BF16 {esb}; batch {st}; FP32 MFMA batch "
+          f"{s32['iterations']} inner steps", flush=True)
+    assert e32 < 1e-5
+    assert all(e < 1e-3 for e in esb.values())
+    assert st["status"] == 0 and st["converged"] == 6 and rel.max() <= 1e-8
+    assert st["iterations"] <= s32["iterations"] + 40
+    for j in (0, 5):
+        S.set_contrast(qk2[:, j])
+        assert H.relerr(S.MatVecProduct(X[:, j]), B[:, j]) <= 1.05e-8
+    S.close()
