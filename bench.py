@@ -684,8 +684,8 @@ def main():
                          "readme: the reference's README run (adaptive tree) beside its published times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: every GPU box of this build has one GPU, and the solver is "
-              "single-device (how it would shard is a design note, not implemented); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the solver is single-device (sharding over several GPUs is "
+              "designed but not implemented); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "setup":
         return bench_setup(args)
