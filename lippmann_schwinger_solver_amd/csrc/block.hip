@@ -1369,6 +1369,38 @@ __global__ __launch_bounds__(kT) void baxpy_kernel(const TV* __restrict__ V, lon
   X[base + kNB] += ai;
 }
 
+// X += V y for the FP16 basis: lane l covers the systems 4 (l % 16) .. + 3 of
+// one row (8-byte basis loads, as bupdate_h_kernel), the quarter-wave l / 16
+// one of four consecutive rows; sums in FP64 (X is the FP64 solution).
+__global__ __launch_bounds__(kT) void baxpy_h_kernel(const _Float16* __restrict__ V, long long vs, int nc,
+                                                     const double2* __restrict__ y, double* __restrict__ X,
+                                                     long long n_rows) {
+  typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, sg = lane & 15, rs = lane >> 4;
+  const long long n = ((long long)blockIdx.x * (kT / 64) + (threadIdx.x >> 6)) * 4 + rs;
+  if (n >= n_rows) return;
+  double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long base = n * kLD + 4 * sg;
+  for (int i = 0; i < nc; ++i) {
+    const h4 a = *reinterpret_cast<const h4*>(V + i * vs + base);
+    const h4 b = *reinterpret_cast<const h4*>(V + i * vs + base + kNB);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double2 yy = y[i * kNB + 4 * sg + e];
+      const double vr = double(a[e]), vi = double(b[e]);
+      ar[e] = fma(vr, yy.x, fma(-vi, yy.y, ar[e]));
+      ai[e] = fma(vr, yy.y, fma(vi, yy.x, ai[e]));
+    }
+  }
+  double4* xr = reinterpret_cast<double4*>(X + base);
+  double4* xi = reinterpret_cast<double4*>(X + base + kNB);
+  double4 u = *xr, v = *xi;
+  u.x += ar[0]; u.y += ar[1]; u.z += ar[2]; u.w += ar[3];
+  v.x += ai[0]; v.y += ai[1]; v.z += ai[2]; v.w += ai[3];
+  *xr = u;
+  *xi = v;
+}
+
 // zero the columns q of a block whose mask[q] is 0
 __global__ void bmask_kernel(double* __restrict__ V, const int* __restrict__ mask, long long n_total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2192,7 +2224,11 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
     if (res.status != 0) break;
     total_it += steps;
     hipLaunchKernelGGL(btrisolve_kernel, dim3(1), dim3(kNB), 0, s, st, m, y);
-    hipLaunchKernelGGL(baxpy_kernel<TV>, dim3(gq), dim3(kT), 0, s, V, vs, steps, y, X, (long long)N);
+    if constexpr (lowv)
+      hipLaunchKernelGGL(baxpy_h_kernel, dim3(blocks_for((N + 3) / 4, kT / 64)), dim3(kT), 0, s, (const _Float16*)V,
+                         vs, steps, y, X, (long long)N);
+    else
+      hipLaunchKernelGGL(baxpy_kernel<TV>, dim3(gq), dim3(kT), 0, s, V, vs, steps, y, X, (long long)N);
     res.restarts = cyc + 1;
   }
   res.iterations = total_it;
