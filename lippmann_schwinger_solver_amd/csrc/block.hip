@@ -72,6 +72,22 @@ __device__ __forceinline__ int xcd_order(int b, int nb) {
   return (i / kXcdGroup) * span + x * kXcdGroup + i % kXcdGroup;
 }
 
+// The same with the run length G as an argument (G <= 0: dispatch order).
+__device__ __forceinline__ int xcd_order_g(int b, int nb, int G) {
+  if (G <= 0) return b;
+  const int span = 8 * G;
+  const int full = nb - nb % span;
+  if (b >= full) return b;
+  const int x = b & 7, i = b >> 3;
+  return (i / G) * span + x * G + i % G;
+}
+// Leaves per XCD run in the leaf kernels: a leaf gathers the x rows of its 9
+// colleagues, so leaves that share colleagues should meet in one XCD's L2.
+#ifndef LSG_XCD_GROUP_LEAF
+#define LSG_XCD_GROUP_LEAF 0
+#endif
+constexpr int kXcdGroupLeaf = LSG_XCD_GROUP_LEAF;
+
 // complex operator columns per LDS chunk
 #ifndef LSG_BLOCK_KC
 #define LSG_BLOCK_KC 16
@@ -394,7 +410,7 @@ void bleaf_kernel(const typename Mfma<R>::c2* __restrict__ T,
                                                    const double* __restrict__ qk2, int find_phi, YT* __restrict__ Y,
                                                    int zrow) {
   typedef typename Mfma<R>::acc Acc;
-  const int leaf = xcd_order(blockIdx.x, gridDim.x);
+  const int leaf = xcd_order_g(blockIdx.x, gridDim.x, kXcdGroupLeaf);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   Segments<R> sg;
   sg.A0 = T;
@@ -816,7 +832,7 @@ __global__ LSG_SB_ATTR __launch_bounds__(block_threads<NT>()) void bleaf_sb_kern
     const long long* __restrict__ src_off, const int* __restrict__ leaf_dest, const XS* __restrict__ X,
     const float* __restrict__ bst, int nx, const XT* __restrict__ Xin, const double* __restrict__ qk2, int find_phi,
     YT* __restrict__ Y, int part) {
-  const int leaf = xcd_order(blockIdx.x, gridDim.x);
+  const int leaf = xcd_order_g(blockIdx.x, gridDim.x, kXcdGroupLeaf);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i16 = lane & 15, g = lane >> 4;
   SegSb sg;
