@@ -213,8 +213,9 @@ int lsg_gmres(lsg_ctx* ctx, const double* b, double* x, int restart_len,
 /* Mixed-precision variant of lsg_matvec (no reference counterpart): the
  * operators and the accumulation inside every pass are FP32 (interleaved
  * re/im), x and y stay FP64; the FP32 copy of the operators is made on the
- * device on first use. nrhs >= 2 runs the FP32 block operator (64 right-hand
- * sides per pass on FP32 MFMA). */
+ * device on first use. nrhs >= 2 runs the low-precision block operator of the
+ * inner solves (64 right-hand sides per pass, FP32 pass state; split-BF16 or
+ * FP32 MFMA as lsg_set_inner_arith selects). */
 int lsg_matvec_fp32(lsg_ctx* ctx, const double* x, double* y, int nrhs, int64_t ld);
 /* lsg_time_matvec for the FP32 operator. */
 int lsg_time_matvec_fp32(lsg_ctx* ctx, int reps, double* ms);
