@@ -930,6 +930,20 @@ constexpr int kBCols = 8;     // Krylov columns per workgroup in the block dot
 constexpr int kRS = 16;       // first-stage segments of the row-block sums
 constexpr int kUpdRows = 8;   // rows per thread (and system group) in the basis updates
 
+// Basis loads of the Gram-Schmidt kernels: non-temporal by default (the
+// streamed basis then leaves the caches to the product, which the dot reads
+// once per group of columns; C3 batch 44.6 against 44.1 solves/s with the FP16
+// kernels), or plain (LSG_GS_NT 0).
+#ifndef LSG_GS_NT
+#define LSG_GS_NT 1
+#endif
+template <class T>
+__device__ __forceinline__ T gs_load(const T* p) {
+  if (LSG_GS_NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+
 // partial[((rb * ncx) + col) * 64 + q] = sum over the workgroup's rows of
 // conj(V_col[n, q]) w[n, q] (col < nc) or |w[n, q]|^2 (col == nc < ncx).
 template <class TV, class TW>
@@ -951,8 +965,8 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
       const int col = g0 + u;
       double vr = 0.0, vi = 0.0;
       if (col < nc) {
-        vr = double(V[col * vs + n * kLD + q]);
-        vi = double(V[col * vs + n * kLD + kNB + q]);
+        vr = double(gs_load(V + col * vs + n * kLD + q));
+        vi = double(gs_load(V + col * vs + n * kLD + kNB + q));
       } else if (col == nc) {
         vr = wr;
         vi = wi;
@@ -975,19 +989,6 @@ __global__ __launch_bounds__(kT) void bdot_kernel(const TV* __restrict__ V, long
       partial[((long long)blockIdx.x * ncx + g0 + u) * kNB + qq] = s;
     }
   }
-}
-
-// Basis loads of the FP16 Gram-Schmidt kernels: non-temporal by default (the
-// streamed basis then leaves the caches to the FP32 product, which the dot
-// reads once per group of columns; C3 batch 44.6 against 44.1 solves/s), or
-// plain (LSG_GS_NT 0).
-#ifndef LSG_GS_NT
-#define LSG_GS_NT 1
-#endif
-template <class T>
-__device__ __forceinline__ T gs_load(const T* p) {
-  if (LSG_GS_NT) return __builtin_nontemporal_load(p);
-  return *p;
 }
 
 // The block dot for the inner solves' FP16 basis and FP32 product: lane l of a
@@ -1338,7 +1339,7 @@ __global__ __launch_bounds__(kT) void bupdate_kernel(const TV* __restrict__ V, l
 #pragma unroll
     for (int r = 0; r < kUpdRows; ++r) {
       if (r < nr) {
-        const double vr = double(Vi[r * kLD]), vi = double(Vi[r * kLD + kNB]);
+        const double vr = double(gs_load(Vi + r * kLD)), vi = double(gs_load(Vi + r * kLD + kNB));
         ar[r] = fma(vr, hh.x, fma(-vi, hh.y, ar[r]));
         ai[r] = fma(vr, hh.y, fma(vi, hh.x, ai[r]));
       }
@@ -2183,13 +2184,16 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
   const unsigned gu = blocks_for((N + kUpdRows - 1) / kUpdRows, kT / 64);   // kUpdRows rows per (thread, system)
   int total_it = 0;
   for (int cyc = 0; cyc <= max_restarts; ++cyc) {
+    // the cycle's first residual: b itself when x = 0 (its squared norms are
+    // still in h from the check of b above), else b - A x
+    const double* r0 = w;
     if (cyc == 0 && x_zero) {
-      hipMemcpyAsync(w, B, sizeof(double) * size_t(ntot), hipMemcpyDeviceToDevice, s);
+      r0 = B;
     } else {
       amv(X, w);
       hipLaunchKernelGGL(bsub_kernel, dim3(gv), dim3(kT), 0, s, B, w, ntot);
+      dot(0, w, true);
     }
-    dot(0, w, true);
     hipLaunchKernelGGL(bcycle_init_kernel, dim3(1), dim3(kNB), 0, s, h, st, m, nrhs, host_rel, host_active);
     if (hipStreamSynchronize(s) != hipSuccess) {
       res.status = LSG_ERR_DEVICE;
@@ -2214,7 +2218,7 @@ static BatchResult block_gmres_t(BlockOperator& bo, const double* B, double* X, 
       break;
     }
     if (n_act == 0 || cyc == max_restarts) break;
-    hipLaunchKernelGGL(bscale_kernel<TV>, dim3(gv), dim3(kT), 0, s, w, st, V, ntot, w32 ? W.x32 : (float*)nullptr);
+    hipLaunchKernelGGL(bscale_kernel<TV>, dim3(gv), dim3(kT), 0, s, r0, st, V, ntot, w32 ? W.x32 : (float*)nullptr);
     int steps = 0;
     for (int j = 0; j < m;) {
       const int batch = std::min(m - j, 4);
