@@ -147,6 +147,7 @@ def test_trained_network_gain():
     assert np.mean(ev["initial_residual_learned"]) < 1.0     # better than the zero guess
 
 
+@pytest.mark.timeout(420)   # about 140 s on one MI355X: 4096 training solves, 3 x 256 evaluation solves
 def test_c5_learned_preconditioner_at_config():
     """C5 at its configured sizes: the 6-layer network (32 channels, 3x3,
     BF16, 128 x 128 grid) trained on 4096 solves the solver generates (depth 6,
